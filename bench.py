@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the trajectory-query hot path on one MI355X.
+
+Headline (value): range queries/s of the plain drop-in GT-quadtree path on
+the 1M-trajectory workload (88.2M points, 1024x1024 Morton grid, 256-point
+leaves, 10k squares with log-uniform side in [1e-4, 1e-1]).  One step = one
+batch of the 10k rectangles through Grid::rangeQueryBatchGPU's pair form:
+host leaf selection, task upload, the refine kernel, per-query count readback;
+the point store is resident in HBM before timing starts and the result pairs
+stay there.
+
+Also reported on the same line: the FSG and STIG paths on the same batch, the
+plain EDR kernel's cell-update rate, the refine kernel's roofline (algorithmic
+bytes 12 per candidate point + 8 per result pair, over its HIP-event time on
+its own stream), and the oracle's CPU restatement of Grid::rangeQueryBatch timed
+on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "range queries/s + EDR cell-updates/s, batched vs plain; p99 latency"
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc, committed under profiles/."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get(name)
+    except (ValueError, OSError):
+        return None
+
+
+def time_range(index, rects, steps, warmup, label):
+    for _ in range(warmup):
+        index.range_pairs(rects)
+    kernel_ms, wall = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        res = index.range_pairs(rects)
+        wall.append(time.perf_counter() - s)
+        kernel_ms.append(res.kernel_ms)
+    total = time.perf_counter() - t0
+    qps = steps * len(rects) / total
+    log(f"{label}: {qps:,.0f} queries/s, step {1e3 * total / steps:.2f} ms, kernel {np.mean(kernel_ms):.3f} ms, "
+        f"select {res.select_ms:.2f} ms, {res.pairs_total:,} pairs from {res.candidates:,} candidates")
+    return res, qps, total, np.array(kernel_ms), np.array(wall)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU split of the "
+              "reference is out of scope for this build", file=sys.stderr)
+        return 2
+
+    import gat_amd as G
+    from gat_amd import workloads as W
+
+    G.init(0)
+    dev = G.device_name()
+    log(f"device {dev}")
+    w = W.CFG2
+    t = time.perf_counter()
+    db = w.db()
+    rects = w.rects()
+    log(f"workload {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points, {len(rects):,} rectangles "
+        f"({time.perf_counter() - t:.1f} s)")
+    t = time.perf_counter()
+    grid = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    log(f"GT-quadtree built and resident ({time.perf_counter() - t:.1f} s): {grid.info().tolist()}")
+
+    res, qps, total, kms, wall = time_range(grid, rects, args.steps, args.warmup, "GT-quadtree plain")
+    alg_bytes = 12 * res.candidates + 8 * res.pairs_total
+    k_mean = float(kms.mean())
+    achieved = alg_bytes / (k_mean * 1e-3) / 1e9
+    traffic = load_traffic("range_refine_kernel<true>:cfg2")
+
+    t = time.perf_counter()
+    fsg = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db)
+    stig = G.StigIndex(w.leaf).build(db)
+    log(f"FSG and STIG built ({time.perf_counter() - t:.1f} s)")
+    fres, fqps, _, fk, _ = time_range(fsg, rects, max(3, args.steps // 2), 1, "FSG plain")
+    sres, sqps, _, sk, _ = time_range(stig, rects, max(3, args.steps // 2), 1, "STIG plain")
+    del fsg, stig
+
+    # Plain EDR kernel: one 128-point query against every trajectory of the
+    # 20k-trajectory workload, both query kinds (independent walk, noisy copy).
+    w1 = W.CFG1
+    db1 = w1.db()
+    g1 = G.GridIndex(W.UNIT_BOX, w1.grid_cell, w1.leaf).build(db1)
+    queries = G.synth_queries(w1.seed, np.array([128, 128], np.int32), db1, copy_mode=2)
+    L = db1.lengths.astype(np.int64)
+    cells = int(L.sum() * 128)
+    for _ in range(2):
+        g1.edr_all(queries.xy(1), w1.eps)
+    reps = 10
+    t = time.perf_counter()
+    for r in range(reps):
+        g1.edr_all(queries.xy(1 + r % 2), w1.eps)
+    edr_s = (time.perf_counter() - t) / reps
+    edr_cells_ps = cells / edr_s
+    log(f"EDR plain: {edr_cells_ps:.3e} cell-updates/s ({cells:,} cells per query, {1e3 * edr_s:.2f} ms)")
+    del g1
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        t = time.perf_counter()
+        og = O.Grid(W.UNIT_BOX, w.grid_cell, w.leaf).add(db)
+        log(f"oracle GT-quadtree built ({time.perf_counter() - t:.1f} s)")
+        t = time.perf_counter()
+        counts, digests = og.range(rects, "digest", nthreads=1)
+        cpu_s = time.perf_counter() - t
+        # The GPU answer of the timed batches, checked against the oracle.
+        ok = bool((res.counts() == counts).all() and (res.digests() == digests).all())
+        t = time.perf_counter()
+        og.range(rects, "digest", nthreads=16)
+        cpu16_s = time.perf_counter() - t
+        cpu = {"value": round(len(rects) / cpu_s, 1), "unit": "queries/s", "cores": 1, "kind": "port",
+               "sample": f"all {len(rects)} rectangles of the timed batch, Grid::rangeQueryBatch restated "
+                         f"(oracle/oracle.cpp), 1 thread; 16 threads as Grid::rangeQueryBatchMultiThread: "
+                         f"{len(rects) / cpu16_s:.1f} queries/s",
+               "parity": ok}
+        log(f"CPU baseline: {cpu['value']} queries/s (1 thread), {len(rects) / cpu16_s:.1f} (16 threads); "
+            f"GPU result identical: {ok}")
+        del og
+
+    line = {
+        "metric": METRIC,
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random walks)",
+        "config": {"workload": "cfg2 plain drop-in: 1M trajectories, 88.2M points, 1024x1024 GT-quadtree, "
+                               "256-point leaves, 10k squares log-uniform side [1e-4,1e-1]",
+                   "index": "GT-quadtree", "rectangles": int(len(rects)), "points": int(db.npoints)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "range_refine_kernel<true>", "kernel_ms": round(k_mean, 4),
+                     "algorithmic_bytes": int(alg_bytes)},
+        "cpu_baseline": cpu,
+        "paths": {
+            "gt_quadtree_plain_qps": round(qps, 1),
+            "fsg_plain_qps": round(fqps, 1),
+            "stig_plain_qps": round(sqps, 1),
+            "edr_plain_cells_per_s": float(f"{edr_cells_ps:.4e}"),
+            "range_p50_ms": round(1e3 * float(np.percentile(wall, 50)), 3),
+            "range_p99_ms": round(1e3 * float(np.percentile(wall, 99)), 3),
+        },
+        "device": dev,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

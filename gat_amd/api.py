@@ -1,0 +1,248 @@
+"""numpy-facing wrappers over the native libraries.
+
+Arrays cross the boundary as plain pointers; every wrapper checks dtype and
+contiguity first.  Trajectory ids are 1-based (trajectory t is
+``offsets[t-1]:offsets[t]`` of ``lon``/``lat``); a point id is the 0-based
+position of the point within its trajectory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _req(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def default_threads() -> int:
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def init(device: int = 0) -> None:
+    """Selects the GPU; raises if no device or the native library is missing."""
+    N.check_host(N.host().gath_init(device))
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    N.check_hip(N.hip().gat_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def eps_threshold(eps: float) -> float:
+    return float(N.hip().gat_eps_threshold(eps))
+
+
+# ---------------------------------------------------------------- workloads
+@dataclass
+class TrajSet:
+    offsets: np.ndarray  # int64, ntraj + 1
+    lon: np.ndarray      # float32
+    lat: np.ndarray      # float32
+
+    @property
+    def ntraj(self) -> int:
+        return len(self.offsets) - 1
+
+    @property
+    def npoints(self) -> int:
+        return int(self.offsets[-1])
+
+    @property
+    def lengths(self) -> np.ndarray:
+        return np.diff(self.offsets).astype(np.int32)
+
+    def xy(self, t: int) -> np.ndarray:
+        """Interleaved (lon, lat) float32 points of trajectory t (1-based)."""
+        b, e = self.offsets[t - 1], self.offsets[t]
+        out = np.empty((e - b, 2), np.float32)
+        out[:, 0] = self.lon[b:e]
+        out[:, 1] = self.lat[b:e]
+        return out
+
+    def interleaved(self) -> np.ndarray:
+        out = np.empty((self.npoints, 2), np.float32)
+        out[:, 0] = self.lon
+        out[:, 1] = self.lat
+        return out
+
+    @staticmethod
+    def from_lengths(lengths: np.ndarray) -> "TrajSet":
+        off = np.zeros(len(lengths) + 1, np.int64)
+        np.cumsum(lengths, out=off[1:])
+        return TrajSet(off, np.empty(int(off[-1]), np.float32), np.empty(int(off[-1]), np.float32))
+
+
+def synth_lengths(seed: int, n: int, kind: str, a: float, b: float, lo: int, hi: int) -> np.ndarray:
+    out = np.empty(n, np.int32)
+    h = N.host()
+    if kind == "uniform":
+        h.gath_synth_lengths_uniform(seed, n, int(a), int(b), _ptr(out))
+    elif kind == "lognormal":
+        h.gath_synth_lengths_lognormal(seed, n, float(a), float(b), lo, hi, _ptr(out))
+    else:
+        raise ValueError(kind)
+    return out
+
+
+def synth_walks(seed: int, lengths: np.ndarray, step_sigma: float = 1e-4, threads: int | None = None) -> TrajSet:
+    ts = TrajSet.from_lengths(_req(lengths, np.int32))
+    N.host().gath_synth_walks(seed, ts.ntraj, _ptr(ts.offsets), step_sigma, _ptr(ts.lon), _ptr(ts.lat),
+                              threads or default_threads())
+    return ts
+
+
+def synth_rects(seed: int, n: int, side_lo: float, side_hi: float, log_uniform: bool = True,
+                hot_frac: float = 0.0, nhot: int = 0, hot_sigma: float = 0.0) -> np.ndarray:
+    out = np.empty((n, 4), np.float32)
+    N.host().gath_synth_rects(seed, n, side_lo, side_hi, int(log_uniform), hot_frac, nhot, hot_sigma, _ptr(out))
+    return out
+
+
+def synth_queries(seed: int, lengths: np.ndarray, db: TrajSet | None, copy_mode: int,
+                  noise_sigma: float = 2e-4, step_sigma: float = 1e-4) -> TrajSet:
+    q = TrajSet.from_lengths(_req(lengths, np.int32))
+    qlen = q.lengths
+    if db is None:
+        db = TrajSet(np.zeros(1, np.int64), np.zeros(0, np.float32), np.zeros(0, np.float32))
+    N.host().gath_synth_queries(seed, q.ntraj, _ptr(qlen), _ptr(q.offsets), _ptr(db.offsets), db.ntraj,
+                                _ptr(db.lon), _ptr(db.lat), copy_mode, noise_sigma, step_sigma,
+                                _ptr(q.lon), _ptr(q.lat))
+    return q
+
+
+# ---------------------------------------------------------------- indexes
+class RangeResult:
+    """A compacted-pair result that stays in HBM until the index's next query."""
+
+    def __init__(self, index: "_Index", nq: int, stats: np.ndarray, ms: np.ndarray):
+        self._index = index
+        self.nq = nq
+        self.pairs_total = int(stats[0])
+        self.candidates = int(stats[1])
+        self.tasks = int(stats[2])
+        self.select_ms, self.upload_ms, self.kernel_ms = (float(x) for x in ms)
+
+    def counts(self) -> np.ndarray:
+        out = np.empty(self.nq, np.int64)
+        N.check_host(N.host().gath_index_last_counts(self._index.handle, _ptr(out)))
+        return out
+
+    def pairs(self) -> np.ndarray:
+        """(pairs_total, 2) int32 (tid, pid), query by query."""
+        out = np.empty((max(self.pairs_total, 1), 2), np.int32)
+        N.check_host(N.host().gath_index_last_fetch(self._index.handle, _ptr(out), self.pairs_total))
+        return out[: self.pairs_total]
+
+    def per_query(self) -> list[np.ndarray]:
+        c = self.counts()
+        p = self.pairs()
+        cut = np.concatenate([[0], np.cumsum(c)])
+        return [p[cut[i]:cut[i + 1]] for i in range(self.nq)]
+
+    def digests(self) -> np.ndarray:
+        out = np.empty(self.nq, np.uint64)
+        N.check_host(N.host().gath_index_last_digests(self._index.handle, _ptr(out)))
+        return out
+
+
+class _Index:
+    handle = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            N.host().gath_index_free(h)
+            self.handle = None
+
+    def build(self, db: TrajSet, upload: bool = True) -> "_Index":
+        N.check_host(N.host().gath_index_build(self.handle, _ptr(_req(db.offsets, np.int64)), db.ntraj,
+                                               _ptr(_req(db.lon, np.float32)), _ptr(_req(db.lat, np.float32)),
+                                               int(upload)))
+        self.ntraj = db.ntraj
+        return self
+
+    def set_threads(self, n: int) -> None:
+        N.check_host(N.host().gath_index_set_threads(self.handle, n))
+
+    def info(self) -> np.ndarray:
+        out = np.zeros(8, np.int64)
+        N.check_host(N.host().gath_index_info(self.handle, _ptr(out)))
+        return out
+
+    def cell_of(self, lon: float, lat: float) -> int:
+        return int(N.host().gath_index_cell_of(self.handle, lon, lat))
+
+    def select(self, boxes: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """Host candidate selection: (tasks (n, 3) uint32 [start, count, qid], candidates per query)."""
+        b = _req(boxes, np.float32).reshape(-1, 4)
+        cand = np.empty(len(b), np.uint64)
+        n = N.host().gath_index_select(self.handle, _ptr(b), len(b), None, 0, _ptr(cand))
+        if n < 0:
+            raise N.GatError(2, "selection failed")
+        tasks = np.empty((max(n, 1), 3), np.uint32)
+        n2 = N.host().gath_index_select(self.handle, _ptr(b), len(b), _ptr(tasks), n, _ptr(cand))
+        assert n2 == n
+        return tasks[:n], cand
+
+    def range_bitmap(self, boxes: np.ndarray) -> np.ndarray:
+        b = _req(boxes, np.float32).reshape(-1, 4)
+        out = np.empty((len(b), self.ntraj + 1), np.uint8)
+        N.check_host(N.host().gath_index_range_bitmap(self.handle, _ptr(b), len(b), _ptr(out)))
+        return out
+
+    def range_pairs(self, boxes: np.ndarray) -> RangeResult:
+        b = _req(boxes, np.float32).reshape(-1, 4)
+        stats = np.zeros(3, np.int64)
+        ms = np.zeros(3, np.float64)
+        N.check_host(N.host().gath_index_range_pairs(self.handle, _ptr(b), len(b), _ptr(stats), _ptr(ms)))
+        return RangeResult(self, len(b), stats, ms)
+
+
+class GridIndex(_Index):
+    """GT-quadtree (Grid.cpp): Morton-ordered cells, quadtree leaves < max_points_in_node."""
+
+    def __init__(self, range_box=(0.0, 0.0, 1.0, 1.0), cell_size: float = 0.01, max_points_in_node: int = 256):
+        r = np.asarray(range_box, np.float32)
+        self.handle = N.host().gath_grid_new(_ptr(r), cell_size, max_points_in_node)
+
+    def edr_all(self, q_xy: np.ndarray, eps: float) -> np.ndarray:
+        q = _req(q_xy, np.float32).reshape(-1, 2)
+        lon, lat = np.ascontiguousarray(q[:, 0]), np.ascontiguousarray(q[:, 1])
+        out = np.empty(self.ntraj, np.int32)
+        N.check_host(N.host().gath_grid_edr_all(self.handle, _ptr(lon), _ptr(lat), len(q), eps, _ptr(out)))
+        return out
+
+    def similarity(self, queries: TrajSet, k: int, eps: float) -> tuple[np.ndarray, np.ndarray]:
+        ids = np.empty((queries.ntraj, k), np.int32)
+        dists = np.empty((queries.ntraj, k), np.int32)
+        N.check_host(N.host().gath_grid_similarity(self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj,
+                                                   _ptr(_req(queries.lon, np.float32)),
+                                                   _ptr(_req(queries.lat, np.float32)), k, eps, _ptr(ids),
+                                                   _ptr(dists)))
+        return ids, dists
+
+
+class FsgIndex(_Index):
+    """FSG (FSG.cpp): flat row-major grid."""
+
+    def __init__(self, range_box=(0.0, 0.0, 1.0, 1.0), cell_size: float = 0.01):
+        r = np.asarray(range_box, np.float32)
+        self.handle = N.host().gath_fsg_new(_ptr(r), cell_size)
+
+
+class StigIndex(_Index):
+    """STIG (STIG.cpp): kd-split leaves of at most block_size points."""
+
+    def __init__(self, block_size: int = 256):
+        self.handle = N.host().gath_stig_new(block_size)

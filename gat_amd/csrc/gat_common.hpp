@@ -1,0 +1,49 @@
+// Internal helpers shared by the translation units of libgat_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "gat_hip.h"
+
+namespace gat {
+
+// Records the calling thread's last error message (read by gat_last_error).
+void set_error(const char* fmt, ...);
+
+inline int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  set_error("%s failed: %s (%d) at %s:%d", what, hipGetErrorString(e), int(e), file, line);
+  return e == hipErrorOutOfMemory ? GAT_ERR_NOMEM : GAT_ERR_HIP;
+}
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+// Number of compute units of the current device (cached per device).
+int cu_count();
+
+}  // namespace gat
+
+#define GAT_HIP_CHECK(call)                                              \
+  do {                                                                   \
+    hipError_t gat_e_ = (call);                                          \
+    if (gat_e_ != hipSuccess) return gat::hip_fail(gat_e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define GAT_REQUIRE(cond, ...)          \
+  do {                                  \
+    if (!(cond)) {                      \
+      gat::set_error(__VA_ARGS__);      \
+      return GAT_ERR_ARG;               \
+    }                                   \
+  } while (0)
+
+// Launch-error check after a kernel launch.
+#define GAT_LAUNCH_CHECK(name)                                                    \
+  do {                                                                            \
+    hipError_t gat_e_ = hipGetLastError();                                        \
+    if (gat_e_ != hipSuccess) return gat::hip_fail(gat_e_, name, __FILE__, __LINE__); \
+  } while (0)

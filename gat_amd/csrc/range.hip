@@ -1,0 +1,232 @@
+// Range refine kernels for gfx950: the per-point containment test behind
+// Grid::rangeQueryBatchGPU, STIG::rangeQueryGPU and FSG::rangeQueryBatchGPU
+// (reference kernels cudaRangeQueryTest, kernel.cu:1004-1074, and
+// cudaRangeQuerySTIG, kernel.cu:1165-1209).
+//
+// Layout: one wave64 refines one task (a run of consecutive store points and
+// a rectangle).  Lanes read the run as float2 (lon, lat) pairs, 64 consecutive
+// points per load instruction, four loads in flight per lane, and test them
+// against the rectangle held in scalar registers.  Hits are compacted with a
+// wave ballot and a per-lane prefix (mbcnt); one atomic per 256 points reserves
+// the wave's slice of the query's output region, so every result pair is
+// written exactly once and no pass re-reads the points.  The (tid, pid) pair
+// of a hit is gathered from the parallel int2 array only for hits.
+#include "gat_common.hpp"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;             // 4 waves per workgroup
+constexpr int kUnroll = 4;              // float2 loads in flight per lane
+constexpr int kSpan = kWave * kUnroll;  // points per wave iteration
+
+struct Task {
+  uint32_t start, count, qid;
+};
+static_assert(sizeof(Task) == sizeof(gat_range_task), "task layout");
+
+// MBB::pInBox (MBB.cpp:21-26): inclusive on all four edges.
+__device__ __forceinline__ bool in_box(const float4& b, float x, float y) {
+  return x <= b.z && x >= b.x && y <= b.w && y >= b.y;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <bool kPairs>
+__global__ __launch_bounds__(kBlock) void range_refine_kernel(
+    const float2* __restrict__ xy, const int2* __restrict__ tp, uint64_t npoints, const Task* __restrict__ tasks,
+    uint64_t ntasks, const float4* __restrict__ boxes, uint32_t nqueries, const uint64_t* __restrict__ qbase,
+    uint32_t* __restrict__ qcount, int2* __restrict__ pairs, uint64_t capacity, uint8_t* __restrict__ bitmap,
+    uint64_t traj_stride) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = uniform((blockIdx.x * kBlock + threadIdx.x) / kWave);
+  const uint64_t nwaves = uint64_t(gridDim.x) * (kBlock / kWave);
+  for (uint64_t t = wave; t < ntasks; t += nwaves) {
+    const Task task = tasks[t];
+    const uint32_t qid = uniform(task.qid);
+    const uint32_t count = uniform(task.count);
+    const uint64_t start = uniform(task.start);
+    // Malformed tasks are skipped rather than read or written out of bounds.
+    if (qid >= nqueries || start + count > npoints) continue;
+    const float4 box = boxes[qid];
+    const float2* p = xy + start;
+    for (uint32_t base = 0; base < count; base += kSpan) {
+      float2 v[kUnroll];
+#pragma unroll
+      for (int k = 0; k < kUnroll; ++k) {
+        uint32_t i = base + k * kWave + lane;
+        v[k] = i < count ? p[i] : make_float2(__builtin_nanf(""), 0.0f);
+      }
+      bool hit[kUnroll];
+      uint64_t mask[kUnroll];
+      uint32_t total = 0;
+#pragma unroll
+      for (int k = 0; k < kUnroll; ++k) {
+        hit[k] = in_box(box, v[k].x, v[k].y);  // NaN fails every comparison
+        mask[k] = __ballot(hit[k]);
+        total += __popcll(mask[k]);
+      }
+      if (total == 0) continue;
+      if constexpr (kPairs) {
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(qcount + qid, total);
+        slot = uniform(slot);
+        const uint64_t first = qbase[qid] + slot;
+        uint32_t before = 0;
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+          const uint64_t at = first + before + lanes_below(mask[k]);
+          if (hit[k] && at < capacity) pairs[at] = tp[start + base + k * kWave + lane];
+          before += __popcll(mask[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k)
+          if (hit[k]) {
+            const uint32_t tid = uint32_t(tp[start + base + k * kWave + lane].x);
+            if (tid < traj_stride) bitmap[uint64_t(qid) * traj_stride + tid] = 1;
+          }
+      }
+    }
+  }
+}
+
+int launch_refine(bool pairs_mode, const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
+                  int64_t ntasks, const float* boxes, int64_t nqueries, const uint64_t* qbase, uint32_t* qcount,
+                  int32_t* pairs, int64_t capacity, uint8_t* bitmap, int64_t traj_stride, void* stream) {
+  GAT_REQUIRE(ntasks >= 0 && npoints >= 0 && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX),
+              "range refine: bad sizes");
+  if (ntasks == 0) return GAT_OK;
+  GAT_REQUIRE(xy && tp && tasks && boxes, "range refine: null input pointer");
+  // Enough waves to keep every SIMD busy, capped so the grid-stride loop keeps
+  // its scalar task fetches amortised.
+  const int64_t waves_needed = ntasks;
+  const int64_t blocks_cap = int64_t(gat::cu_count()) * 16;
+  int64_t blocks = (waves_needed + (kBlock / kWave) - 1) / (kBlock / kWave);
+  if (blocks > blocks_cap) blocks = blocks_cap;
+  hipStream_t s = gat::as_stream(stream);
+  if (pairs_mode) {
+    GAT_REQUIRE(qbase && qcount && pairs, "range refine: null output pointer");
+    GAT_REQUIRE(capacity >= 0, "range refine: negative capacity");
+    range_refine_kernel<true><<<dim3(uint32_t(blocks)), dim3(kBlock), 0, s>>>(
+        reinterpret_cast<const float2*>(xy), reinterpret_cast<const int2*>(tp), uint64_t(npoints),
+        reinterpret_cast<const Task*>(tasks), uint64_t(ntasks), reinterpret_cast<const float4*>(boxes),
+        uint32_t(nqueries), qbase, qcount, reinterpret_cast<int2*>(pairs), uint64_t(capacity), nullptr, 0);
+  } else {
+    GAT_REQUIRE(bitmap && traj_stride > 0, "range refine: null bitmap");
+    range_refine_kernel<false><<<dim3(uint32_t(blocks)), dim3(kBlock), 0, s>>>(
+        reinterpret_cast<const float2*>(xy), reinterpret_cast<const int2*>(tp), uint64_t(npoints),
+        reinterpret_cast<const Task*>(tasks), uint64_t(ntasks), reinterpret_cast<const float4*>(boxes),
+        uint32_t(nqueries), nullptr, nullptr, nullptr, 0, bitmap, uint64_t(traj_stride));
+  }
+  GAT_LAUNCH_CHECK("range_refine_kernel");
+  return GAT_OK;
+}
+
+// Shared body of the two reference-signature handlers.
+int bitmap_handler(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks, int ntasks,
+                   const float* boxes, uint8_t* result_host, int maxTrajNum, int maxQueryNum, void* stream) {
+  GAT_REQUIRE(result_host, "range handler: null result buffer");
+  GAT_REQUIRE(maxTrajNum > 0 && maxQueryNum >= 0 && ntasks >= 0, "range handler: bad sizes");
+  const size_t bytes = size_t(maxTrajNum) * size_t(maxQueryNum);
+  if (bytes == 0) return GAT_OK;
+  hipStream_t s = gat::as_stream(stream);
+  uint8_t* dev = nullptr;
+  GAT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dev), bytes));
+  int rc = GAT_OK;
+  hipError_t e = hipMemsetAsync(dev, 0, bytes, s);
+  if (e != hipSuccess) rc = gat::hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__);
+  if (rc == GAT_OK)
+    rc = launch_refine(false, xy, tp, npoints, tasks, ntasks, boxes, maxQueryNum, nullptr, nullptr, nullptr, 0, dev,
+                       maxTrajNum, stream);
+  if (rc == GAT_OK) {
+    e = hipMemcpyAsync(result_host, dev, bytes, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) rc = gat::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+  }
+  hipError_t es = hipStreamSynchronize(s);
+  hipError_t ef = hipFree(dev);
+  if (rc == GAT_OK && es != hipSuccess) rc = gat::hip_fail(es, "hipStreamSynchronize", __FILE__, __LINE__);
+  if (rc == GAT_OK && ef != hipSuccess) rc = gat::hip_fail(ef, "hipFree", __FILE__, __LINE__);
+  return rc;
+}
+
+// Order-independent digest of a (tid, pid) pair: a 64-bit mix (the splitmix64
+// finaliser) of tid in the high and pid in the low half.  A query's digest is
+// the wrapping sum over its pairs.
+__device__ __forceinline__ uint64_t pair_mix(int2 p) {
+  uint64_t z = (uint64_t(uint32_t(p.x)) << 32) | uint32_t(p.y);
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void pairs_digest_kernel(const int2* __restrict__ pairs,
+                                                             const uint64_t* __restrict__ qbase,
+                                                             const uint32_t* __restrict__ qcount, uint32_t nq,
+                                                             uint64_t* __restrict__ out) {
+  __shared__ uint64_t part[kBlock / kWave];
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    const uint64_t b = qbase[q];
+    const uint32_t n = qcount[q];
+    uint64_t acc = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) acc += pair_mix(pairs[b + i]);
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint64_t total = 0;
+      for (int w = 0; w < kBlock / kWave; ++w) total += part[w];
+      out[q] = total;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gat_pairs_digest(const int32_t* pairs, const uint64_t* qbase, const uint32_t* qcount, int64_t nq,
+                     uint64_t* digest, void* stream) {
+  GAT_REQUIRE(nq >= 0 && nq <= int64_t(UINT32_MAX), "pairs digest: bad query count");
+  if (nq == 0) return GAT_OK;
+  GAT_REQUIRE(pairs && qbase && qcount && digest, "pairs digest: null pointer");
+  const int64_t blocks = nq < int64_t(gat::cu_count()) * 8 ? nq : int64_t(gat::cu_count()) * 8;
+  pairs_digest_kernel<<<dim3(uint32_t(blocks)), dim3(kBlock), 0, gat::as_stream(stream)>>>(
+      reinterpret_cast<const int2*>(pairs), qbase, qcount, uint32_t(nq), digest);
+  GAT_LAUNCH_CHECK("pairs_digest_kernel");
+  return GAT_OK;
+}
+
+int gat_range_refine_bitmap(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
+                            int64_t ntasks, const float* boxes, int64_t nqueries, uint8_t* bitmap, int64_t traj_stride,
+                            void* stream) {
+  return launch_refine(false, xy, tp, npoints, tasks, ntasks, boxes, nqueries, nullptr, nullptr, nullptr, 0, bitmap,
+                       traj_stride, stream);
+}
+
+int gat_range_refine_pairs(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
+                           int64_t ntasks, const float* boxes, int64_t nqueries, const uint64_t* qbase,
+                           uint32_t* qcount, int32_t* pairs, int64_t capacity, void* stream) {
+  return launch_refine(true, xy, tp, npoints, tasks, ntasks, boxes, nqueries, qbase, qcount, pairs, capacity, nullptr,
+                       0, stream);
+}
+
+int gatRangeQueryTestHandler(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
+                             int stateTableLength, const float* boxes, uint8_t* result_host, int maxTrajNum,
+                             int maxQueryNum, void* stream) {
+  return bitmap_handler(xy, tp, npoints, tasks, stateTableLength, boxes, result_host, maxTrajNum, maxQueryNum, stream);
+}
+
+int gatRangeQuerySTIGHandler(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
+                             int stateTableLength, const float* boxes, uint8_t* result_host, int maxTrajNum,
+                             int maxQueryNum, void* stream) {
+  return bitmap_handler(xy, tp, npoints, tasks, stateTableLength, boxes, result_host, maxTrajNum, maxQueryNum, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// Grid — the GT-quadtree index of the reference (Grid.h:34-100): a square grid
+// of 2^k x 2^k cells numbered in Morton order, a cell-ordered point store, and a
+// quadtree over Morton ranges whose leaves hold fewer than maxPointsInNode
+// points (or are single cells).  Range queries select the leaves whose box
+// overlaps the rectangle on the host and refine their points on the GPU;
+// similarity queries run the EDR kernels against a trajectory-ordered copy of
+// the points.  The indexes and stores are built once and stay resident in HBM.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "DeviceArena.h"
+#include "MBB.h"
+#include "RangeEngine.h"
+#include "Trajectory.h"
+
+struct QuadtreeNode {  // Grid.h:19-26, flattened: children are indices into Grid::nodes
+  int NodeID = 0;      // the node's Morton block at its level
+  int level = 0;
+  int64_t numPoints = 0;
+  int64_t pointStart = 0;  // first store index of the node's cells
+  bool isLeaf = true;
+  int child[4] = {-1, -1, -1, -1};  // UL, UR, DL, DR
+  MBB mbb;
+};
+
+// Top-k answer of one similarity query: ids and distances in ascending
+// (distance, trajectory id) order.
+struct TopK {
+  std::vector<int32_t> ids, dist;
+};
+
+class Grid {
+ public:
+  Grid() = default;
+  // Grid.cpp:97-124.  VITURAL_CELL_PARAM only drove the reference's frequency
+  // vectors (Grid.cpp:152) and is kept for signature parity.
+  Grid(const MBB& mbb, float val_cell_size, int VITURAL_CELL_PARAM, int maxPointsInNode = MAXPOINTINNODE_DEFAULT);
+  Grid(const Grid&) = delete;
+  Grid& operator=(const Grid&) = delete;
+
+  // Grid.cpp:210-316: builds the cell-ordered store and the quadtree and makes
+  // both resident on the device (with upload=false only the host structures
+  // are built — used to test the host logic without a GPU).
+  int addDatasetToGrid(Trajectory* db, int traNum);
+  int addDatasetToGrid(const TrajectoryView& db, bool upload = true);
+
+  int WhichCellPointIn(SamplePoint p) const { return int(cellOf(p.lon, p.lat)); }
+  uint32_t cellOf(float lon, float lat) const;
+  MBB cellMBB(uint32_t cellId) const;
+
+  // Grid.cpp:450-542: the trajectory-set answer of every rectangle, computed on
+  // the GPU (ResultTable[i] is resized to trajNum + 1).  resultSetSize, when
+  // non-null, receives each query's number of trajectories.
+  int rangeQueryBatchGPU(const MBB* bounds, int rangeNum, CPURangeQueryResult* ResultTable, int* resultSetSize,
+                         int device_idx = 0);
+  // The same selection refined into compacted (tid, pid) pairs left in HBM.
+  int rangeQueryBatchGPUPairs(const MBB* bounds, int rangeNum, RangeBatch& out);
+  // Grid.cpp:428-446: the leaves one rectangle selects, in traversal order.
+  void findMatchNodeInQuadTree(const MBB& bound, std::vector<int>* leaves) const;
+  // Host candidate selection for a batch (tasks + per-query candidate counts).
+  void selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
+                        std::vector<uint64_t>& candidates) const;
+
+  // Similarity (Grid.cpp:1321-1755, plain form): one EDR launch per query
+  // over every trajectory, then an exact host top-k by (distance, id).
+  // topKSimilarityTraj receives queryTrajNum * kValue ids.
+  int SimilarityQueryBatchOnGPU(Trajectory* qTra, int queryTrajNum, int* topKSimilarityTraj, int kValue);
+  int similarityPlain(const TrajectoryView& queries, int kValue, std::vector<TopK>& out);
+  // EDR distances of one query against every trajectory (plain kernel).
+  int edrAll(const float* qlon, const float* qlat, int qlen, int32_t* dist);
+
+  // Run-time EDR threshold (the reference's compile-time EPSILON, ConstDefine.h:28).
+  double epsilon = 0.005;
+  int hostThreads = 0;  // 0: hardware concurrency (capped at 16)
+
+  // Geometry (Grid.h:69-80).
+  MBB range;
+  float cell_size = 0;
+  int cellNum_axis = 0;
+  int cellnum = 0;
+  int totalLevel = 0;
+  int maxPointsInNode = MAXPOINTINNODE_DEFAULT;
+  int64_t totalPointNum = 0;
+  int trajNum = 0;
+  int nodeNum = 0;
+  std::vector<QuadtreeNode> nodes;   // nodes[0] is the root
+  std::vector<int64_t> cellStart;    // cellnum + 1 prefix of per-cell point counts
+
+  RangeEngine& engine() { return engine_; }
+  double lastSelectMs = 0;
+  float lastKernelMs = 0;
+
+ private:
+  int buildNode(int level, int id);
+  int threads() const;
+  int uploadTrajectories(const TrajectoryView& db);
+  int ensureQueryStore(const TrajectoryView& queries);
+
+  RangeEngine engine_;
+  // Trajectory-ordered copy of the points for EDR (the reference reassembles
+  // trajectories from the cell-ordered store per query, Grid.cpp:1500-1508).
+  DeviceBuffer trajXY_, trajOff_, trajLen_, allTasks_, dist_;
+  DeviceBuffer queryXY_, queryOff_, queryLen_;
+  std::vector<int32_t> trajLenHost_;
+};

@@ -1,0 +1,100 @@
+// The device side of every range index: a resident point store in HBM (the
+// index's points in its own order, as (lon, lat) floats and (tid, pid) ints),
+// pinned staging for the per-batch task lists, and the refine launches.
+//
+// It replaces the reference's per-index arenas and leaf caches: the reference
+// copies each candidate leaf into a bump-allocated device arena on first use
+// (Grid.cpp:715-727, FSG.cpp:268-273) or on every use (STIG.cpp:293) and builds
+// a RangeQueryStateTable per (query, leaf chunk) on the host (Grid.cpp:729-747);
+// here the whole store is uploaded once when the index is built and a batch
+// only uploads its task list.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "DeviceArena.h"
+#include "MBB.h"
+
+// Candidate runs longer than this are split across tasks so one wave never
+// walks an unbounded run (the reference splits at MAXPOINTINNODE,
+// Grid.cpp:729-747).
+constexpr uint32_t kTaskChunk = 1024;
+
+// Builds one query's task list: appends candidate runs in order, merging a run
+// that starts where the previous one of the same query ended, and splitting at
+// kTaskChunk points.
+class TaskList {
+ public:
+  void addRun(uint64_t start, uint64_t count, uint32_t qid);
+  void flush();
+  std::vector<gat_range_task> tasks;
+  std::vector<uint64_t> candidates;  // per query handled by this list (indexed by qid - firstQuery)
+  uint32_t firstQuery = 0;
+
+ private:
+  bool open_ = false;
+  uint64_t runStart_ = 0, runCount_ = 0;
+  uint32_t runQid_ = 0;
+};
+
+// Result of one batch in compacted-pair form; the pairs stay in HBM.
+struct RangeBatch {
+  int queryNum = 0;
+  std::vector<uint64_t> base;   // per query: first pair slot in `pairs`
+  std::vector<uint32_t> count;  // per query: number of result pairs
+  int64_t candidates = 0;       // candidate points scanned by the batch
+  int64_t taskNum = 0;
+  int64_t pairTotal = 0;
+  double selectMs = 0;          // host candidate selection
+  double uploadMs = 0;          // task-list upload
+  float kernelMs = 0;           // refine kernel (device events)
+  DeviceBuffer pairs;           // int32 (tid, pid) pairs
+  DeviceBuffer qbase, qcount;   // device copies of base / count
+  // Copies the pairs to the host, grouped by query (query q's pairs at
+  // out[2*base[q] ...], count[q] of them).
+  int fetch(int32_t* out, int64_t capacityPairs) const;
+  // Per-query order-independent digest (wrapping sum of a 64-bit mix of each
+  // (tid, pid) pair), computed on the device.
+  int digests(uint64_t* out) const;
+};
+
+class RangeEngine {
+ public:
+  RangeEngine() = default;
+  RangeEngine(const RangeEngine&) = delete;
+  RangeEngine& operator=(const RangeEngine&) = delete;
+
+  // Uploads the index-ordered store.
+  int upload(const float* xy, const int32_t* tp, int64_t n);
+  int64_t pointNum() const { return n_; }
+  const float* xyDev() const { return xy_.as<float>(); }
+  const int32_t* tpDev() const { return tp_.as<int32_t>(); }
+  void* stream() { return stream_.get(); }
+  int ensureStream() { return stream_.create(); }
+
+  // Candidate selection callback: appends query q's runs to `list`.
+  using Selector = std::function<void(int q, TaskList& list)>;
+
+  // Selects candidates for every query on `threads` host threads, then refines
+  // them into compacted pairs.
+  int runPairs(const MBB* bounds, int nq, const Selector& select, int threads, RangeBatch& out);
+  // Selects candidates, then fills the trajectory bitmap through the
+  // reference-signature handler (synchronous, host result).
+  int runBitmap(const MBB* bounds, int nq, const Selector& select, int threads, int maxTrajNum,
+                uint8_t* resultHost, bool stigHandler);
+  // Candidate selection only (host), exposed for tests of the host logic.
+  static void select(int nq, const Selector& select, int threads, std::vector<gat_range_task>& tasks,
+                     std::vector<uint64_t>& candidates);
+
+ private:
+  int stage(const MBB* bounds, int nq, const std::vector<gat_range_task>& tasks, const uint64_t* qbase);
+
+  DeviceBuffer xy_, tp_;
+  int64_t n_ = 0;
+  Stream stream_;
+  PinnedBuffer hostStage_;
+  DeviceBuffer devTasks_, devBoxes_;
+  EventTimer timer_;
+};

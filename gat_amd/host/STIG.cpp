@@ -1,0 +1,169 @@
+#include "STIG.h"
+
+#include <algorithm>
+#include <thread>
+
+int STIG::threads() const {
+  if (hostThreads > 0) return hostThreads;
+  unsigned h = std::thread::hardware_concurrency();
+  return int(std::max(1u, std::min(h ? h : 1u, 16u)));
+}
+
+int STIG::initial(int blk, int totalDim, Trajectory* db, int trajNum) {
+  if (totalDim != 2) {
+    gath_set_error("STIG: only two dimensions are supported");
+    return GAT_ERR_ARG;
+  }
+  TrajectoryDB flat = TrajectoryDB::fromArray(db, trajNum);
+  return initial(blk, TrajectoryView::of(flat), true);
+}
+
+// The tree's shape depends only on the node sizes, so it is laid out first and
+// the partitioning of disjoint subtrees then runs in parallel.
+int STIG::skeleton(int depth, int64_t lo, int64_t hi, bool leaf) {
+  Node n;
+  n.leaf = leaf;
+  n.dim = depth % 2;
+  n.lo = lo;
+  n.hi = hi;
+  const int me = int(nodes.size());
+  nodes.push_back(n);
+  if (!leaf) {
+    const int64_t mid = (lo + hi) / 2;
+    const int l = skeleton(depth + 1, lo, mid - 1, mid - lo <= blockSize);
+    const int r = skeleton(depth + 1, mid, hi, hi - mid + 1 <= blockSize);
+    nodes[me].left = l;
+    nodes[me].right = r;
+  }
+  return me;
+}
+
+void STIG::partition(int ni, std::vector<P>& pts, int spawnDepth) {
+  Node& n = nodes[ni];
+  if (n.leaf) {
+    float xl = 999, xh = -999, yl = 999, yh = -999;  // STIG.cpp:77-90
+    for (int64_t i = n.lo; i <= n.hi; ++i) {
+      xl = std::min(xl, pts[i].x);
+      xh = std::max(xh, pts[i].x);
+      yl = std::min(yl, pts[i].y);
+      yh = std::max(yh, pts[i].y);
+    }
+    n.box = MBB(xl, yl, xh, yh);
+    return;
+  }
+  const int64_t mid = (n.lo + n.hi) / 2;
+  const int dim = n.dim;
+  std::nth_element(pts.begin() + n.lo, pts.begin() + mid, pts.begin() + n.hi + 1,
+                   [dim](const P& a, const P& b) { return dim == 0 ? a.x < b.x : a.y < b.y; });
+  n.median = dim == 0 ? pts[mid].x : pts[mid].y;
+  const int l = n.left, r = n.right;
+  if (spawnDepth > 0) {
+    std::thread th([&, l] { partition(l, pts, spawnDepth - 1); });
+    partition(r, pts, spawnDepth - 1);
+    th.join();
+  } else {
+    partition(l, pts, 0);
+    partition(r, pts, 0);
+  }
+}
+
+int STIG::initial(int blk, const TrajectoryView& db, bool upload) {
+  if (blk < 2) {
+    gath_set_error("STIG: block size must be at least 2");
+    return GAT_ERR_ARG;
+  }
+  blockSize = blk;
+  maxTid = db.trajNum;
+  totalPointsNum = db.pointNum();
+  if (totalPointsNum >= int64_t(UINT32_MAX)) {
+    gath_set_error("STIG: %lld points exceed the 32-bit store index", (long long)totalPointsNum);
+    return GAT_ERR_ARG;
+  }
+  std::vector<P> pts(static_cast<size_t>(totalPointsNum));
+  for (int t = 1; t <= db.trajNum; ++t)
+    for (int64_t i = db.offsets[t - 1]; i < db.offsets[t]; ++i)
+      pts[i] = P{db.lon[i], db.lat[i], t, int32_t(i - db.offsets[t - 1])};
+  nodes.clear();
+  if (totalPointsNum == 0) return GAT_OK;
+  skeleton(0, 0, totalPointsNum - 1, totalPointsNum <= blockSize);
+  int spawn = 0;
+  for (int t = threads(); t > 1; t >>= 1) ++spawn;
+  partition(0, pts, spawn);
+  if (!upload) return GAT_OK;
+  std::vector<float> xy(size_t(totalPointsNum) * 2);
+  std::vector<int32_t> tp(size_t(totalPointsNum) * 2);
+  for (int64_t i = 0; i < totalPointsNum; ++i) {
+    xy[2 * i] = pts[i].x;
+    xy[2 * i + 1] = pts[i].y;
+    tp[2 * i] = pts[i].tid;
+    tp[2 * i + 1] = pts[i].pid;
+  }
+  return engine_.upload(xy.data(), tp.data(), totalPointsNum);
+}
+
+int STIG::searchTree(const MBB& q, std::vector<STIGBlock>* out) const {
+  if (nodes.empty()) return 0;
+  int stack[128];
+  int top = 0;
+  stack[top++] = 0;
+  while (top > 0) {
+    const Node& n = nodes[stack[--top]];
+    if (n.leaf) {
+      if (n.lo <= n.hi && n.box.xmin <= q.xmax && n.box.xmax >= q.xmin && n.box.ymin <= q.ymax &&
+          n.box.ymax >= q.ymin)
+        out->push_back(STIGBlock{n.lo, n.hi});
+      continue;
+    }
+    const float qmin = n.dim == 0 ? q.xmin : q.ymin;
+    const float qmax = n.dim == 0 ? q.xmax : q.ymax;
+    // Push right first so the left subtree is visited first (store order).
+    if (qmax >= n.median) stack[top++] = n.right;
+    if (qmin <= n.median) stack[top++] = n.left;
+  }
+  return 0;
+}
+
+void STIG::addLeaves(const MBB& q, uint32_t qid, TaskList& list) const {
+  std::vector<STIGBlock> blocks;
+  searchTree(q, &blocks);
+  for (const STIGBlock& b : blocks) list.addRun(uint64_t(b.startIdx), uint64_t(b.endIdx - b.startIdx + 1), qid);
+}
+
+void STIG::selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
+                            std::vector<uint64_t>& candidates) const {
+  RangeEngine::select(
+      rangeNum, [&](int q, TaskList& list) { addLeaves(bounds[q], uint32_t(q), list); }, threads(), tasks,
+      candidates);
+}
+
+int STIG::rangeQueryGPU(const MBB* bounds, int rangeNum, CPURangeQueryResult* ResultTable, int* resultSetSize,
+                        int device_idx) {
+  if (device_idx != 0 || rangeNum < 0 || (rangeNum > 0 && (!bounds || !ResultTable))) {
+    gath_set_error("STIG::rangeQueryGPU: bad arguments");
+    return GAT_ERR_ARG;
+  }
+  const int stride = maxTid + 1;
+  std::vector<uint8_t> bitmap(size_t(rangeNum) * size_t(stride));
+  auto selector = [&](int q, TaskList& list) { addLeaves(bounds[q], uint32_t(q), list); };
+  if (int rc = engine_.runBitmap(bounds, rangeNum, selector, threads(), stride, bitmap.data(), true)) return rc;
+  for (int q = 0; q < rangeNum; ++q) {
+    ResultTable[q].assign(size_t(stride), false);
+    int n = 0;
+    for (int t = 0; t < stride; ++t)
+      if (bitmap[size_t(q) * stride + t]) {
+        ResultTable[q][t] = true;
+        ++n;
+      }
+    if (resultSetSize) resultSetSize[q] = n;
+  }
+  return GAT_OK;
+}
+
+int STIG::rangeQueryGPUPairs(const MBB* bounds, int rangeNum, RangeBatch& out) {
+  if (rangeNum < 0 || (rangeNum > 0 && !bounds)) {
+    gath_set_error("STIG::rangeQueryGPUPairs: bad arguments");
+    return GAT_ERR_ARG;
+  }
+  auto selector = [&](int q, TaskList& list) { addLeaves(bounds[q], uint32_t(q), list); };
+  return engine_.runPairs(bounds, rangeNum, selector, threads(), out);
+}

@@ -1,0 +1,73 @@
+// STIG — the kd-tree index of the reference (STIG.h:9-66): all points split
+// alternately on x and y at the median down to leaves of at most blockSize
+// points, each leaf a contiguous range of the index-ordered store with the
+// bounding box of its points.
+//
+// The split is a correct kd-split (the reference's std::sort range leaves the
+// last element out, STIG.cpp:101/135, and its left leaf repeats the median
+// point, STIG.cpp:107-118): the node's points are partitioned around the
+// element at (lo+hi)/2, left [lo, mid-1] holding keys <= the median and right
+// [mid, hi] keys >= it.  The search visits the left child when the rectangle
+// reaches down to the median and the right child when it reaches up to it
+// (the reference bounds those half-planes by a hard-coded [0, 180],
+// STIG.cpp:206-230), and keeps a leaf when its box overlaps the rectangle
+// inclusively (the reference's BoxIntersect drops touching leaves,
+// STIG.cpp:243), so the answer is exactly the inclusive containment set.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "MBB.h"
+#include "RangeEngine.h"
+#include "Trajectory.h"
+
+struct STIGBlock {  // STIG.h:9-12
+  int64_t startIdx;
+  int64_t endIdx;  // inclusive
+};
+
+class STIG {
+ public:
+  STIG() = default;
+  STIG(const STIG&) = delete;
+  STIG& operator=(const STIG&) = delete;
+
+  // STIG.cpp:35-66 (totalDim must be 2).
+  int initial(int blockSize, int totalDim, Trajectory* db, int trajNum);
+  int initial(int blockSize, const TrajectoryView& db, bool upload = true);
+  // STIG.cpp:179-252: the leaves a rectangle selects, in store order.
+  int searchTree(const MBB& queryMBB, std::vector<STIGBlock>* allCandBlocks) const;
+  // STIG.cpp:254-338: trajectory-set answers on the GPU.
+  int rangeQueryGPU(const MBB* bounds, int rangeNum, CPURangeQueryResult* ResultTable, int* resultSetSize,
+                    int device_idx = 0);
+  int rangeQueryGPUPairs(const MBB* bounds, int rangeNum, RangeBatch& out);
+  void selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
+                        std::vector<uint64_t>& candidates) const;
+
+  struct Node {
+    bool leaf = true;
+    int dim = 0;
+    float median = 0;
+    int left = -1, right = -1;
+    int64_t lo = 0, hi = -1;  // leaf range (inclusive)
+    MBB box;
+  };
+  std::vector<Node> nodes;  // nodes[0] is the root
+  int blockSize = 0;
+  int64_t totalPointsNum = 0;
+  int maxTid = 0;
+  int hostThreads = 0;
+  RangeEngine& engine() { return engine_; }
+
+ private:
+  struct P {
+    float x, y;
+    int32_t tid, pid;
+  };
+  int skeleton(int depth, int64_t lo, int64_t hi, bool leaf);
+  void partition(int node, std::vector<P>& pts, int spawnDepth);
+  void addLeaves(const MBB& q, uint32_t qid, TaskList& list) const;
+  int threads() const;
+  RangeEngine engine_;
+};

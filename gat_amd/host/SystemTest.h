@@ -1,0 +1,49 @@
+// SystemTest — the reference's query driver (SystemTest.h:13-35): the range
+// tests of the three indexes and the similarity test, run on the GPU path and
+// timed.  The reference replicates one rectangle / one query trajectory N times
+// (SystemTest.cpp:31-33, 167-189); the overloads taking arrays run distinct
+// inputs through the same classes.  Times are wall-clock milliseconds
+// (std::chrono::steady_clock; the reference's MyTimer reports seconds under an
+// "ms" label, ConstDefine.h:144-147).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "FSG.h"
+#include "Grid.h"
+#include "STIG.h"
+#include "Trajectory.h"
+
+class SystemTest {
+ public:
+  SystemTest() = default;
+  // The reference also takes a MortonGrid, which is outside this path.
+  SystemTest(Trajectory* tradb, Grid* g, STIG* stig, FSG* fsg) : tradb(tradb), g(g), stig(stig), fsg(fsg) {}
+
+  int rangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum);
+  int rangeQueryTest(const MBB* bounds, int rangeQueryNum);
+  int STIGrangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum);
+  int STIGrangeQueryTest(const MBB* bounds, int rangeQueryNum);
+  int FSGrangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum);
+  int FSGrangeQueryTest(const MBB* bounds, int rangeQueryNum);
+  int similarityQueryTest(Trajectory t, int similarityScale, int similarityKValue);
+
+  struct Result {
+    double wallMs = 0;      // whole call: host candidate selection, upload, kernel, count readback
+    double selectMs = 0;    // host candidate selection
+    float kernelMs = 0;     // refine kernel (device events)
+    int64_t candidates = 0;
+    int64_t pairs = 0;
+  };
+  Result last;
+  bool verbose = true;
+
+  Trajectory* tradb = nullptr;
+  Grid* g = nullptr;
+  STIG* stig = nullptr;
+  FSG* fsg = nullptr;
+
+ private:
+  int report(const char* what, int n, const RangeBatch& b, double ms);
+};

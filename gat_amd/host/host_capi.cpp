@@ -1,0 +1,259 @@
+// The flat C ABI of libgat_host.so (include/gat_host.h) over the C++ classes.
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "FSG.h"
+#include "Grid.h"
+#include "STIG.h"
+#include "Synth.h"
+#include "gat_host.h"
+
+namespace {
+struct Index {
+  enum Kind { kGrid, kFsg, kStig } kind;
+  std::unique_ptr<Grid> grid;
+  std::unique_ptr<FSG> fsg;
+  std::unique_ptr<STIG> stig;
+  RangeBatch last;
+  int trajNum() const { return kind == kGrid ? grid->trajNum : kind == kFsg ? fsg->trajNum : stig->maxTid; }
+};
+
+Index* as_index(void* p) { return static_cast<Index*>(p); }
+const MBB* as_boxes(const float* b) { return reinterpret_cast<const MBB*>(b); }
+
+int bad(const char* what) {
+  gath_set_error("%s", what);
+  return GAT_ERR_ARG;
+}
+}  // namespace
+
+extern "C" {
+
+int gath_init(int device) {
+  GATH_TRY(gat_init(device), "gat_init");
+  return GAT_OK;
+}
+
+int gath_box_intersect(const float* a, const float* b) {
+  return BoxIntersect(MBB(a[0], a[1], a[2], a[3]), MBB(b[0], b[1], b[2], b[3]));
+}
+
+int gath_pinbox(const float* box, float x, float y) { return MBB(box[0], box[1], box[2], box[3]).pInBox(x, y); }
+
+uint32_t gath_morton(uint32_t x, uint32_t y) { return getIdxFromXY(x, y); }
+
+void gath_synth_lengths_uniform(uint64_t seed, int64_t n, int lo, int hi, int32_t* out) {
+  synth_lengths_uniform(seed, n, lo, hi, out);
+}
+void gath_synth_lengths_lognormal(uint64_t seed, int64_t n, double median, double sigma, int lo, int hi,
+                                  int32_t* out) {
+  synth_lengths_lognormal(seed, n, median, sigma, lo, hi, out);
+}
+void gath_synth_walks(uint64_t seed, int64_t ntraj, const int64_t* offsets, double step_sigma, float* lon,
+                      float* lat, int threads) {
+  synth_walks(seed, ntraj, offsets, step_sigma, lon, lat, threads);
+}
+void gath_synth_rects(uint64_t seed, int64_t n, double side_lo, double side_hi, int log_uniform, double hot_frac,
+                      int nhot, double hot_sigma, float* out4) {
+  synth_rects(seed, n, side_lo, side_hi, log_uniform != 0, hot_frac, nhot, hot_sigma, out4);
+}
+void gath_synth_queries(uint64_t seed, int nq, const int32_t* qlen, const int64_t* q_offsets,
+                        const int64_t* db_offsets, int db_ntraj, const float* db_lon, const float* db_lat,
+                        int copy_mode, double noise_sigma, double step_sigma, float* qlon, float* qlat) {
+  TrajectoryView db{db_offsets, db_ntraj, db_lon, db_lat};
+  synth_queries(seed, nq, qlen, q_offsets, db, copy_mode, noise_sigma, step_sigma, qlon, qlat);
+}
+
+void* gath_grid_new(const float* r, float cell_size, int max_points_in_node) {
+  auto* ix = new (std::nothrow) Index{Index::kGrid, nullptr, nullptr, nullptr, {}};
+  if (!ix) return nullptr;
+  ix->grid.reset(new Grid(MBB(r[0], r[1], r[2], r[3]), cell_size, 2, max_points_in_node));
+  return ix;
+}
+
+void* gath_fsg_new(const float* r, float cell_size) {
+  auto* ix = new (std::nothrow) Index{Index::kFsg, nullptr, nullptr, nullptr, {}};
+  if (!ix) return nullptr;
+  ix->fsg.reset(new FSG(MBB(r[0], r[1], r[2], r[3]), cell_size));
+  return ix;
+}
+
+void* gath_stig_new(int block_size) {
+  auto* ix = new (std::nothrow) Index{Index::kStig, nullptr, nullptr, nullptr, {}};
+  if (!ix) return nullptr;
+  ix->stig.reset(new STIG());
+  ix->stig->blockSize = block_size;
+  return ix;
+}
+
+void gath_index_free(void* p) { delete as_index(p); }
+
+int gath_index_build(void* p, const int64_t* offsets, int ntraj, const float* lon, const float* lat, int upload) {
+  Index* ix = as_index(p);
+  if (!ix || !offsets || ntraj < 0) return bad("gath_index_build: bad arguments");
+  TrajectoryView db{offsets, ntraj, lon, lat};
+  switch (ix->kind) {
+    case Index::kGrid: return ix->grid->addDatasetToGrid(db, upload != 0);
+    case Index::kFsg: return ix->fsg->addDatasetToGrid(db, upload != 0);
+    default: return ix->stig->initial(ix->stig->blockSize, db, upload != 0);
+  }
+}
+
+int gath_index_set_threads(void* p, int threads) {
+  Index* ix = as_index(p);
+  if (!ix) return bad("gath_index_set_threads: null index");
+  if (ix->grid) ix->grid->hostThreads = threads;
+  if (ix->fsg) ix->fsg->hostThreads = threads;
+  if (ix->stig) ix->stig->hostThreads = threads;
+  return GAT_OK;
+}
+
+int gath_index_info(void* p, int64_t* out) {
+  Index* ix = as_index(p);
+  if (!ix || !out) return bad("gath_index_info: bad arguments");
+  for (int i = 0; i < 8; ++i) out[i] = 0;
+  if (ix->grid) {
+    Grid& g = *ix->grid;
+    int64_t leaves = 0;
+    for (auto& n : g.nodes) leaves += n.isLeaf;
+    out[0] = g.totalPointNum; out[1] = g.trajNum; out[2] = g.cellnum; out[3] = g.cellNum_axis;
+    out[4] = int64_t(g.nodes.size()); out[5] = leaves; out[6] = g.totalLevel;
+  } else if (ix->fsg) {
+    FSG& f = *ix->fsg;
+    out[0] = f.totalPointNum; out[1] = f.trajNum; out[2] = f.cellnum; out[3] = f.cell_num_x; out[6] = f.cell_num_y;
+  } else {
+    STIG& s = *ix->stig;
+    int64_t leaves = 0;
+    for (auto& n : s.nodes) leaves += n.leaf;
+    out[0] = s.totalPointsNum; out[1] = s.maxTid; out[2] = leaves; out[4] = int64_t(s.nodes.size()); out[5] = leaves;
+  }
+  return GAT_OK;
+}
+
+int64_t gath_index_cell_of(void* p, float lon, float lat) {
+  Index* ix = as_index(p);
+  if (!ix) return -1;
+  if (ix->grid) {
+    uint32_t c = ix->grid->cellOf(lon, lat);
+    return c == UINT32_MAX ? -1 : int64_t(c);
+  }
+  if (ix->fsg) return ix->fsg->cellOf(lon, lat);
+  return -1;
+}
+
+int64_t gath_index_select(void* p, const float* boxes, int nq, uint32_t* tasks_out, int64_t cap,
+                          uint64_t* candidates_out) {
+  Index* ix = as_index(p);
+  if (!ix || nq < 0 || (nq > 0 && !boxes)) return -1;
+  std::vector<gat_range_task> tasks;
+  std::vector<uint64_t> cand;
+  if (ix->grid) ix->grid->selectCandidates(as_boxes(boxes), nq, tasks, cand);
+  else if (ix->fsg) ix->fsg->selectCandidates(as_boxes(boxes), nq, tasks, cand);
+  else ix->stig->selectCandidates(as_boxes(boxes), nq, tasks, cand);
+  const int64_t n = int64_t(tasks.size());
+  if (tasks_out && n <= cap)
+    for (int64_t i = 0; i < n; ++i) {
+      tasks_out[3 * i] = tasks[i].start;
+      tasks_out[3 * i + 1] = tasks[i].count;
+      tasks_out[3 * i + 2] = tasks[i].qid;
+    }
+  if (candidates_out)
+    for (int q = 0; q < nq; ++q) candidates_out[q] = cand[q];
+  return n;
+}
+
+int gath_index_range_bitmap(void* p, const float* boxes, int nq, uint8_t* out) {
+  Index* ix = as_index(p);
+  if (!ix || nq < 0 || (nq > 0 && (!boxes || !out))) return bad("gath_index_range_bitmap: bad arguments");
+  std::vector<CPURangeQueryResult> res(static_cast<size_t>(nq));
+  int rc;
+  if (ix->grid) rc = ix->grid->rangeQueryBatchGPU(as_boxes(boxes), nq, res.data(), nullptr);
+  else if (ix->fsg) rc = ix->fsg->rangeQueryBatchGPU(as_boxes(boxes), nq, res.data(), nullptr);
+  else rc = ix->stig->rangeQueryGPU(as_boxes(boxes), nq, res.data(), nullptr);
+  if (rc) return rc;
+  const size_t stride = size_t(ix->trajNum()) + 1;
+  for (int q = 0; q < nq; ++q)
+    for (size_t t = 0; t < stride; ++t) out[size_t(q) * stride + t] = res[q][t] ? 1 : 0;
+  return GAT_OK;
+}
+
+int gath_index_range_pairs(void* p, const float* boxes, int nq, int64_t* stats, double* ms) {
+  Index* ix = as_index(p);
+  if (!ix || nq < 0 || (nq > 0 && !boxes)) return bad("gath_index_range_pairs: bad arguments");
+  int rc;
+  if (ix->grid) rc = ix->grid->rangeQueryBatchGPUPairs(as_boxes(boxes), nq, ix->last);
+  else if (ix->fsg) rc = ix->fsg->rangeQueryBatchGPUPairs(as_boxes(boxes), nq, ix->last);
+  else rc = ix->stig->rangeQueryGPUPairs(as_boxes(boxes), nq, ix->last);
+  if (rc) return rc;
+  if (stats) {
+    stats[0] = ix->last.pairTotal;
+    stats[1] = ix->last.candidates;
+    stats[2] = ix->last.taskNum;
+  }
+  if (ms) {
+    ms[0] = ix->last.selectMs;
+    ms[1] = ix->last.uploadMs;
+    ms[2] = ix->last.kernelMs;
+  }
+  return GAT_OK;
+}
+
+int gath_index_last_counts(void* p, int64_t* counts) {
+  Index* ix = as_index(p);
+  if (!ix || !counts) return bad("gath_index_last_counts: bad arguments");
+  for (int q = 0; q < ix->last.queryNum; ++q) counts[q] = ix->last.count[q];
+  return GAT_OK;
+}
+
+int gath_index_last_fetch(void* p, int32_t* pairs, int64_t cap) {
+  Index* ix = as_index(p);
+  if (!ix || (cap > 0 && !pairs)) return bad("gath_index_last_fetch: bad arguments");
+  const RangeBatch& b = ix->last;
+  if (b.pairTotal > cap) {
+    gath_set_error("gath_index_last_fetch: %lld pairs, room for %lld", (long long)b.pairTotal, (long long)cap);
+    return GAT_ERR_CAPACITY;
+  }
+  int64_t at = 0;
+  for (int q = 0; q < b.queryNum; ++q) {
+    if (b.count[q]) {
+      GATH_TRY(gat_memcpy_d2h(pairs + 2 * at, b.pairs.as<int32_t>() + 2 * b.base[q], size_t(b.count[q]) * 8, nullptr),
+               "pair readback");
+    }
+    at += b.count[q];
+  }
+  GATH_TRY(gat_device_sync(), "pair readback");
+  return GAT_OK;
+}
+
+int gath_index_last_digests(void* p, uint64_t* digests) {
+  Index* ix = as_index(p);
+  if (!ix || !digests) return bad("gath_index_last_digests: bad arguments");
+  return ix->last.digests(digests);
+}
+
+int gath_grid_edr_all(void* p, const float* qlon, const float* qlat, int qlen, double eps, int32_t* out) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || !qlon || !qlat || !out) return bad("gath_grid_edr_all: bad arguments");
+  ix->grid->epsilon = eps;
+  return ix->grid->edrAll(qlon, qlat, qlen, out);
+}
+
+int gath_grid_similarity(void* p, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, int k,
+                         double eps, int32_t* ids, int32_t* dists) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || nq < 0 || k < 0 || !q_offsets) return bad("gath_grid_similarity: bad arguments");
+  ix->grid->epsilon = eps;
+  TrajectoryView q{q_offsets, nq, qlon, qlat};
+  std::vector<TopK> res;
+  if (int rc = ix->grid->similarityPlain(q, k, res)) return rc;
+  for (int i = 0; i < nq; ++i)
+    for (int j = 0; j < k; ++j) {
+      const bool have = j < int(res[i].ids.size());
+      ids[int64_t(i) * k + j] = have ? res[i].ids[j] : 0;
+      dists[int64_t(i) * k + j] = have ? res[i].dist[j] : -1;
+    }
+  return GAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+/*
+ * gat_hip.h — the C ABI of libgat_hip.so, the MI355X (gfx950) device library
+ * behind the trajectory-query path of azh18/GAT.
+ *
+ * This is the drop-in boundary: the reference's host classes call the CUDA
+ * handlers declared in cudaKernel.h (cudaKernel.h:14-31) and the CUDA runtime
+ * directly; the host classes of this project call only the functions below.
+ * Signatures carry plain pointers, sizes and an opaque stream handle — no HIP
+ * or torch types.  Every function returns GAT_OK (0) or a non-zero status and
+ * never aborts or throws; gat_last_error() returns the calling thread's last
+ * error message.  Device pointers are marked `_dev`; host pointers `_host`.
+ *
+ * Point stores are structure-of-arrays in HBM: `xy_dev` holds (lon, lat) float
+ * pairs and `tp_dev` holds (trajectory id, point id) int32 pairs, both in the
+ * index's own point order.  Trajectory ids are 1-based (Grid.cpp:215-221);
+ * a point id is the point's 0-based position within its trajectory.
+ */
+#ifndef GAT_HIP_H
+#define GAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#pragma GCC visibility push(default)
+
+enum {
+  GAT_OK = 0,
+  GAT_ERR_HIP = 1,      /* a HIP runtime call failed */
+  GAT_ERR_ARG = 2,      /* invalid argument (null pointer, bad size or length) */
+  GAT_ERR_NOMEM = 3,    /* device or pinned allocation failed */
+  GAT_ERR_CAPACITY = 4, /* an output or scratch buffer is too small */
+  GAT_ERR_STATE = 5     /* library not initialised or object misused */
+};
+
+/* Longest trajectory the EDR kernels accept (the reference caps at 1024,
+ * ConstDefine.h:24; the workloads here reach 2048). */
+#define GAT_MAX_TRAJ_LEN 2048
+
+/* ------------------------------------------------------------------ runtime */
+/* Replaces the reference's direct cudaSetDevice/cudaStreamCreate/cudaMalloc/
+ * cudaMemcpyAsync/cudaFree calls (Grid.cpp:470-476, 497, 717; STIG.cpp:271-293;
+ * FSG.cpp:165-170, 269, 339; SystemTest.cpp:59-78) and the CUDA_CALL abort
+ * macro (cudaKernel.h:10). */
+const char* gat_last_error(void);
+int gat_init(int device);
+int gat_device_name(char* buf, int len);
+int gat_device_props(int64_t* out); /* [cu_count, lds_per_block, total_mem, clock_khz, warp_size] */
+int gat_mem_info(size_t* free_bytes, size_t* total_bytes);
+int gat_malloc(void** ptr_dev, size_t bytes);
+int gat_free(void* ptr_dev);
+int gat_host_alloc(void** ptr_host, size_t bytes); /* pinned host memory */
+int gat_host_free(void* ptr_host);
+int gat_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes, void* stream);
+int gat_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes, void* stream);
+int gat_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
+int gat_memset(void* dst_dev, int value, size_t bytes, void* stream);
+int gat_stream_create(void** stream);
+int gat_stream_destroy(void* stream);
+int gat_stream_sync(void* stream);
+int gat_device_sync(void);
+int gat_event_create(void** event);
+int gat_event_destroy(void* event);
+int gat_event_record(void* event, void* stream);
+int gat_event_sync(void* event);
+int gat_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ------------------------------------------------------------------ range */
+/* One unit of refine work: `count` consecutive store points starting at
+ * `start`, tested against rectangle `qid`.  Replaces RangeQueryStateTable
+ * (ConstDefine.h:92-101): the rectangle lives once per query in `boxes_dev`
+ * (xmin, ymin, xmax, ymax floats, the MBB member order of MBB.h:5) instead of
+ * being copied into every entry, and the point run is an offset into the
+ * resident store instead of a raw device pointer. */
+typedef struct gat_range_task {
+  uint32_t start;
+  uint32_t count;
+  uint32_t qid;
+} gat_range_task;
+
+/* Drop-in for cudaRangeQueryTestHandler (cudaKernel.h:21-22, kernel.cu:1282-1312)
+ * and cudaRangeQuerySTIGHandler (cudaKernel.h:28-29, kernel.cu:1211-1242): refine
+ * every task and set result_host[qid * maxTrajNum + tid] = 1 for each point
+ * inside its rectangle (inclusive containment, MBB.cpp:21-26; the reference's GPU
+ * kernels test strictly, kernel.cu:1055).  result_host (maxQueryNum * maxTrajNum
+ * bytes) is fully overwritten with 0/1.  Synchronous on `stream`, as the
+ * reference (which synchronises the device, kernel.cu:1300-1306). */
+int gatRangeQueryTestHandler(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
+                             const gat_range_task* tasks_dev, int stateTableLength, const float* boxes_dev,
+                             uint8_t* result_host, int maxTrajNum, int maxQueryNum, void* stream);
+int gatRangeQuerySTIGHandler(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
+                             const gat_range_task* tasks_dev, int stateTableLength, const float* boxes_dev,
+                             uint8_t* result_host, int maxTrajNum, int maxQueryNum, void* stream);
+
+/* The same refine, with the trajectory bitmap left in device memory
+ * (bitmap_dev: rows of `traj_stride` bytes, one row per query, zeroed by the
+ * caller).  Asynchronous on `stream`.
+ * In every range entry point, tasks whose qid >= nqueries (maxQueryNum) or whose
+ * run leaves the store's `npoints` points are skipped, and trajectory ids at or
+ * beyond the bitmap stride are ignored: malformed input cannot make the kernel
+ * read or write out of bounds. */
+int gat_range_refine_bitmap(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
+                            const gat_range_task* tasks_dev, int64_t ntasks, const float* boxes_dev,
+                            int64_t nqueries, uint8_t* bitmap_dev, int64_t traj_stride, void* stream);
+
+/* The same refine, emitting compacted (tid, pid) int32 pairs.  Query q's pairs
+ * are written contiguously from pairs_dev[2 * qbase_dev[q]]; qcount_dev[q] (zeroed
+ * by the caller) ends as their number.  The caller sizes each query's region by
+ * its candidate count (an exact upper bound).  The order of pairs within a query
+ * is unspecified.  Pairs that would land at or beyond `pairs_capacity` (in
+ * pairs) are dropped while qcount_dev still counts them, so an undersized
+ * region shows as qcount > region size.  Asynchronous on `stream`. */
+int gat_range_refine_pairs(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
+                           const gat_range_task* tasks_dev, int64_t ntasks, const float* boxes_dev,
+                           int64_t nqueries, const uint64_t* qbase_dev, uint32_t* qcount_dev,
+                           int32_t* pairs_dev, int64_t pairs_capacity, void* stream);
+
+/* Per-query order-independent digest of a compacted-pair result: digest_dev[q]
+ * = wrapping sum over query q's pairs of a 64-bit mix (the splitmix64
+ * finaliser) of (uint64(tid) << 32 | pid).  Lets two results be compared as
+ * sets without copying them.  Asynchronous on `stream`.  (No reference
+ * counterpart: the reference checks results by dumping them to text files,
+ * SystemTest.cpp:79-89.) */
+int gat_pairs_digest(const int32_t* pairs_dev, const uint64_t* qbase_dev, const uint32_t* qcount_dev, int64_t nq,
+                     uint64_t* digest_dev, void* stream);
+
+/* ------------------------------------------------------------------ EDR */
+/* One (query, candidate) pair.  Replaces TaskInfoTableForSimilarity
+ * (ConstDefine.h:108-111). */
+typedef struct gat_edr_task {
+  int32_t qid;
+  int32_t cand;
+} gat_edr_task;
+
+/* Drop-in for EDRDistance_Batch_Handler (cudaKernel.h:24, kernel.cu:503-508):
+ * result_dev[t] = the reference's EDR (Grid.cpp:1222-1318) between query
+ * tasks_dev[t].qid (query_xy_dev from 2*query_offset_dev[qid], query_length_dev[qid]
+ * points) and its candidate (cand_xy_dev from 2*cand_offset_dev[t],
+ * cand_length_dev[t] points).  The SPoint** candidate pointers of the reference
+ * become point offsets into one candidate store.  Two points match when
+ * |dx| < eps and |dy| < eps with the float differences compared as doubles
+ * (Grid.cpp:1281).  Lengths must lie in [1, GAT_MAX_TRAJ_LEN].  Asynchronous on
+ * `stream`, as the reference's handler. */
+int gatEDRDistanceBatchHandler(int queryTaskNum, const gat_edr_task* tasks_dev, const float* query_xy_dev,
+                               const int32_t* query_offset_dev, const int32_t* query_length_dev,
+                               const float* cand_xy_dev, const int64_t* cand_offset_dev,
+                               const int32_t* cand_length_dev, double eps, int32_t* result_dev, void* stream);
+
+/* The float threshold u with (double)|d| < eps  <=>  |d| < u for every float d:
+ * the smallest float >= eps. */
+float gat_eps_threshold(double eps);
+
+#pragma GCC visibility pop
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAT_HIP_H */

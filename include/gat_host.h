@@ -1,0 +1,89 @@
+/*
+ * gat_host.h — C ABI of libgat_host.so, the C++ host mirror of the reference's
+ * index and query classes (Grid, FSG, STIG, SystemTest; the headers in gat_amd/host).
+ *
+ * C++ callers use those classes directly, as the reference's main.cpp uses
+ * its own (main.cpp:130-164).  This flat ABI exists for callers that cannot
+ * link C++ — the Python test harness and bench.py bind it with ctypes.  It has
+ * no counterpart in the reference.  Every function returning int returns 0 on
+ * success and a gat_hip.h status otherwise; gath_last_error() describes the
+ * calling thread's last failure.
+ */
+#ifndef GAT_HOST_H
+#define GAT_HOST_H
+
+#include <stdint.h>
+
+#include "gat_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#pragma GCC visibility push(default)
+
+const char* gath_last_error(void);
+int gath_init(int device);
+
+/* Host geometry: BoxIntersect (MBB.cpp:28-73), MBB::pInBox (MBB.cpp:21-26),
+ * getIdxFromXY (Grid.cpp:25-54). */
+int gath_box_intersect(const float* a, const float* b);
+int gath_pinbox(const float* box, float x, float y);
+uint32_t gath_morton(uint32_t x, uint32_t y);
+
+/* Synthetic workloads (gat_amd/host/Synth.h). */
+void gath_synth_lengths_uniform(uint64_t seed, int64_t n, int lo, int hi, int32_t* out);
+void gath_synth_lengths_lognormal(uint64_t seed, int64_t n, double median, double sigma, int lo, int hi,
+                                  int32_t* out);
+void gath_synth_walks(uint64_t seed, int64_t ntraj, const int64_t* offsets, double step_sigma, float* lon,
+                      float* lat, int threads);
+void gath_synth_rects(uint64_t seed, int64_t n, double side_lo, double side_hi, int log_uniform, double hot_frac,
+                      int nhot, double hot_sigma, float* out4);
+void gath_synth_queries(uint64_t seed, int nq, const int32_t* qlen, const int64_t* q_offsets,
+                        const int64_t* db_offsets, int db_ntraj, const float* db_lon, const float* db_lat,
+                        int copy_mode, double noise_sigma, double step_sigma, float* qlon, float* qlat);
+
+/* Indexes.  range = (xmin, ymin, xmax, ymax). */
+void* gath_grid_new(const float* range, float cell_size, int max_points_in_node);
+void* gath_fsg_new(const float* range, float cell_size);
+void* gath_stig_new(int block_size);
+void gath_index_free(void* index);
+/* Builds the index over trajectories 1..ntraj (offsets: ntraj+1 entries); with
+ * upload = 0 only the host structures are built (no GPU needed). */
+int gath_index_build(void* index, const int64_t* offsets, int ntraj, const float* lon, const float* lat,
+                     int upload);
+/* out[0..7]: points, trajectories, cells (or leaves for STIG), cells per axis
+ * (Grid) / columns (FSG), nodes, leaves, total levels (Grid) / rows (FSG), 0. */
+int gath_index_info(void* index, int64_t* out);
+int gath_index_set_threads(void* index, int threads);
+/* Host candidate selection only: returns the number of tasks; when it is not
+ * above `cap`, tasks (start, count, qid triples) are written to tasks_out.
+ * candidates_out[q] (nq entries) receives each query's candidate points. */
+int64_t gath_index_select(void* index, const float* boxes, int nq, uint32_t* tasks_out, int64_t cap,
+                          uint64_t* candidates_out);
+/* Cell of a point (Grid: Morton id; FSG: row-major id; -1 outside). */
+int64_t gath_index_cell_of(void* index, float lon, float lat);
+/* The reference-signature path: trajectory bitmap (nq * (ntraj + 1) bytes). */
+int gath_index_range_bitmap(void* index, const float* boxes, int nq, uint8_t* out);
+/* Compacted-pair path.  stats[0..2] = pairs, candidates, tasks; ms[0..2] =
+ * host selection, task upload, refine kernel.  The result stays on the device
+ * until the next call. */
+int gath_index_range_pairs(void* index, const float* boxes, int nq, int64_t* stats, double* ms);
+int gath_index_last_counts(void* index, int64_t* counts);
+/* Pairs of the last result, query by query (query q's pairs follow those of
+ * queries 0..q-1); cap in pairs. */
+int gath_index_last_fetch(void* index, int32_t* pairs, int64_t cap);
+int gath_index_last_digests(void* index, uint64_t* digests);
+
+/* Similarity on a Grid. */
+int gath_grid_edr_all(void* grid, const float* qlon, const float* qlat, int qlen, double eps, int32_t* out);
+int gath_grid_similarity(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, int k,
+                         double eps, int32_t* ids, int32_t* dists);
+
+#pragma GCC visibility pop
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAT_HOST_H */

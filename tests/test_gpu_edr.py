@@ -1,0 +1,116 @@
+"""GPU parity of the EDR path against the oracle's restatement of
+Grid::SimilarityExecuter (Grid.cpp:1222-1318).  Bar: bit-exact int32
+distances; top-k ids identical under (distance, id) order."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gat_amd import _native as N, api, workloads as W
+
+from tests.devbuf import DevBuf
+
+pytestmark = pytest.mark.gpu
+EPS = 5e-4
+
+
+@pytest.fixture(scope="module")
+def cfg1_grid(gpu):
+    db = W.CFG1.db()
+    g = api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db)
+    qlen = np.array([128, 128], np.int32)
+    queries = api.synth_queries(W.CFG1.seed, qlen, db, copy_mode=2)  # query 0 independent, query 1 a noisy copy
+    return db, g, queries
+
+
+def test_cfg1_plain_edr_all_candidates(cfg1_grid):
+    db, g, queries = cfg1_grid
+    for qi in range(queries.ntraj):
+        q = queries.xy(qi + 1)
+        got = g.edr_all(q, EPS)
+        want = O.edr_batch(q, db, EPS, nthreads=8)
+        assert (got == want).all(), np.flatnonzero(got != want)[:10]
+    # The noisy copy finds its source: some distances fall well below the
+    # all-mismatch value, so the match test is exercised.
+    F = O.ftable(300)
+    L = db.lengths
+    allmiss = F[np.minimum(L, 128), np.maximum(L, 128)]
+    assert (got < allmiss).sum() >= 1
+
+
+def test_cfg1_topk(cfg1_grid):
+    db, g, queries = cfg1_grid
+    ids, dists = g.similarity(queries, W.CFG1.k, EPS)
+    for qi in range(queries.ntraj):
+        d = O.edr_batch(queries.xy(qi + 1), db, EPS, nthreads=8)
+        oi, od = O.topk(d, np.arange(1, db.ntraj + 1, dtype=np.int32), W.CFG1.k)
+        assert (ids[qi] == oi).all() and (dists[qi] == od).all()
+
+
+def _pair_case(rng, n, m, kind):
+    base = rng.random(2) * 0.8 + 0.1
+    q = base + np.cumsum(rng.normal(0, 1e-4, (n, 2)), 0)
+    if kind == "copy":
+        idx = np.minimum(np.arange(m) * n // m, n - 1)
+        c = q[idx] + rng.normal(0, 2e-4, (m, 2))
+    elif kind == "same":
+        c = q[np.arange(m) % n]
+    else:
+        c = rng.random(2) * 0.8 + 0.1 + np.cumsum(rng.normal(0, 1e-4, (m, 2)), 0)
+    return q.astype(np.float32), c.astype(np.float32)
+
+
+def test_handler_lengths_and_orientation(gpu):
+    """Lengths at the wave / block / LDS boundaries, equal lengths (the query is
+    the rows), query longer than candidate, and heavy matching."""
+    rng = np.random.default_rng(17)
+    lens = [1, 2, 3, 63, 64, 65, 127, 128, 255, 256, 257, 511, 1023, 1024, 1025, 2047, 2048]
+    cases = []
+    for i in range(90):
+        n = int(rng.choice(lens))
+        m = n if i % 4 == 0 else int(rng.choice(lens))
+        cases.append(_pair_case(rng, n, m, ["copy", "same", "indep"][i % 3]))
+    qoff, coff, qxy, cxy = [], [], [], []
+    qa = ca = 0
+    for q, c in cases:
+        qoff.append(qa)
+        coff.append(ca)
+        qxy.append(q)
+        cxy.append(c)
+        qa += len(q)
+        ca += len(c)
+    tasks = np.array([[i, i + 1] for i in range(len(cases))], np.int32)
+    dq = DevBuf.of(np.concatenate(qxy))
+    dc = DevBuf.of(np.concatenate(cxy))
+    dqo = DevBuf.of(np.array(qoff, np.int32))
+    dql = DevBuf.of(np.array([len(q) for q, _ in cases], np.int32))
+    dco = DevBuf.of(np.array(coff, np.int64))
+    dcl = DevBuf.of(np.array([len(c) for _, c in cases], np.int32))
+    dt = DevBuf.of(tasks)
+    out = DevBuf(4 * len(cases))
+    N.check_hip(N.hip().gatEDRDistanceBatchHandler(len(cases), dt.ptr, dq.ptr, dqo.ptr, dql.ptr, dc.ptr, dco.ptr,
+                                                   dcl.ptr, EPS, out.ptr, None))
+    got = out.get(np.int32, len(cases))
+    want = np.array([O.edr(q, c, EPS) for q, c in cases], np.int32)
+    assert (got == want).all(), [(i, len(cases[i][0]), len(cases[i][1])) for i in np.flatnonzero(got != want)]
+    # Another eps: the threshold is rounded exactly for a non-float eps too.
+    N.check_hip(N.hip().gatEDRDistanceBatchHandler(len(cases), dt.ptr, dq.ptr, dqo.ptr, dql.ptr, dc.ptr, dco.ptr,
+                                                   dcl.ptr, 3e-4, out.ptr, None))
+    got = out.get(np.int32, len(cases))
+    want = np.array([O.edr(q, c, 3e-4) for q, c in cases], np.int32)
+    assert (got == want).all()
+
+
+def test_handler_eps_boundary(gpu):
+    below = np.float32(4.9999997e-4)
+    at = np.float32(5e-4)
+    q = np.array([[0, 0], [0, 0]], np.float32)
+    c = np.array([[below, 0], [at, 0]], np.float32)
+    dq = DevBuf.of(q)
+    dc = DevBuf.of(c)
+    dt = DevBuf.of(np.array([[0, 1], [1, 2]], np.int32))
+    dqo, dql = DevBuf.of(np.array([0, 1], np.int32)), DevBuf.of(np.array([1, 1], np.int32))
+    dco, dcl = DevBuf.of(np.array([0, 1], np.int64)), DevBuf.of(np.array([1, 1], np.int32))
+    out = DevBuf(8)
+    N.check_hip(N.hip().gatEDRDistanceBatchHandler(2, dt.ptr, dq.ptr, dqo.ptr, dql.ptr, dc.ptr, dco.ptr, dcl.ptr, EPS,
+                                                   out.ptr, None))
+    assert out.get(np.int32, 2).tolist() == [0, 1]

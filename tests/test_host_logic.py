@@ -1,0 +1,209 @@
+"""CPU tests of the product's host logic (libgat_host.so without a GPU):
+geometry, cell numbering, index construction and candidate selection checked
+against the oracle; the ABI surface of both libraries; the workload generator."""
+import itertools
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gat_amd import _native as N, api, workloads as W
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared(header: Path) -> set[str]:
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(gath?_[a-z0-9_]+|gat[A-Z][A-Za-z0-9_]*)\s*\(", text))
+    return {n for n in names if not n.startswith("gat_range_task") and not n.startswith("gat_edr_task")}
+
+
+def test_libraries_export_every_declared_symbol():
+    hip_decl = _declared(ROOT / "include" / "gat_hip.h")
+    host_decl = _declared(ROOT / "include" / "gat_host.h")
+    assert "gatRangeQueryTestHandler" in hip_decl and "gatEDRDistanceBatchHandler" in hip_decl
+    hip_syms = N.exported_symbols(N.HIP_LIB)
+    host_syms = N.exported_symbols(N.HOST_LIB)
+    assert hip_decl <= hip_syms, hip_decl - hip_syms
+    assert host_decl <= host_syms, host_decl - host_syms
+    # Both libraries load and bind every declared function without a GPU.
+    N.hip()
+    N.host()
+    assert set(N.HIP_FUNCS) == hip_decl
+    assert set(N.HOST_FUNCS) == host_decl
+
+
+def test_eps_threshold_is_smallest_float_at_or_above():
+    for eps in [5e-4, 0.005, 1e-3, 0.1, 0.5, 1.0, 3e-7]:
+        u = np.float32(api.eps_threshold(eps))
+        assert float(u) >= eps
+        assert float(np.nextafter(u, np.float32(0))) < eps
+
+
+def test_geometry_matches_oracle():
+    h = N.host()
+    rng = np.random.default_rng(9)
+    v = rng.integers(0, 5, size=(120, 4)).astype(np.float32) * np.float32(0.25)
+    boxes = np.stack([np.minimum(v[:, 0], v[:, 2]), np.minimum(v[:, 1], v[:, 3]),
+                      np.maximum(v[:, 0], v[:, 2]), np.maximum(v[:, 1], v[:, 3])], 1).astype(np.float32)
+    boxes = np.concatenate([boxes, rng.random((60, 4)).astype(np.float32)])  # includes inverted boxes
+    for a, b in itertools.product(boxes, boxes[:40]):
+        assert h.gath_box_intersect(api._ptr(a), api._ptr(b)) == O.box_intersect(a, b)
+    pts = rng.integers(0, 5, size=(100, 2)).astype(np.float32) * np.float32(0.25)
+    for box in boxes[:60]:
+        for x, y in pts:
+            assert h.gath_pinbox(api._ptr(box), float(x), float(y)) == O.pinbox(box, x, y)
+
+
+def test_morton_matches_oracle():
+    h = N.host()
+    for x in range(0, 300, 7):
+        for y in range(0, 300, 11):
+            assert h.gath_morton(x, y) == O.morton(x, y)
+    rng = np.random.default_rng(4)
+    for x, y in rng.integers(0, 32768, size=(2000, 2)):
+        assert h.gath_morton(int(x), int(y)) == O.morton(int(x), int(y))
+
+
+def _edge_points(cell, n, rng):
+    """Points on and next to cell edges, where float assignment is delicate."""
+    k = rng.integers(0, int(round(1 / cell)) + 1, size=(n, 2)).astype(np.float64)
+    base = (k * cell).astype(np.float32)
+    jitter = rng.integers(-2, 3, size=(n, 2))
+    out = base.copy()
+    for i in range(n):
+        for d in range(2):
+            for _ in range(abs(int(jitter[i, d]))):
+                out[i, d] = np.nextafter(out[i, d], np.float32(np.inf if jitter[i, d] > 0 else -np.inf))
+    return np.clip(out, 0, 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("cell", [0.01, 0.001, 1 / 16, 0.00025])
+def test_cell_assignment_matches_oracle(cell):
+    rng = np.random.default_rng(int(1 / cell))
+    g = api.GridIndex(W.UNIT_BOX, cell, 256)
+    og = O.Grid(W.UNIT_BOX, cell, 256)
+    f = api.FsgIndex(W.UNIT_BOX, cell)
+    of = O.FSG(W.UNIT_BOX, cell)
+    pts = np.concatenate([_edge_points(cell, 3000, rng), rng.random((3000, 2)).astype(np.float32)])
+    for x, y in pts:
+        assert g.cell_of(x, y) == og.cell_of(x, y)
+        assert f.cell_of(x, y) == of.cell_of(x, y)
+
+
+def test_grid_geometry_matches_oracle():
+    for cell in [0.01, 0.001, 0.00025, 0.3]:
+        db = api.synth_walks(1, np.full(10, 5, np.int32))
+        g = api.GridIndex(W.UNIT_BOX, cell, 256).build(db, upload=False).info()
+        o = O.Grid(W.UNIT_BOX, cell, 256).add(db).info()
+        assert g[3] == o["axis"] and g[2] == o["cellnum"] and g[6] == o["total_level"]
+
+
+def _merged_runs(leaves):
+    runs = []
+    for s, c in leaves:
+        if c == 0:
+            continue
+        if runs and runs[-1][0] + runs[-1][1] == s:
+            runs[-1][1] += int(c)
+        else:
+            runs.append([int(s), int(c)])
+    return runs
+
+
+def _task_runs(tasks, q):
+    runs = []
+    for s, c, _ in tasks[tasks[:, 2] == q]:
+        assert c <= 1024
+        if runs and runs[-1][0] + runs[-1][1] == s:
+            runs[-1][1] += int(c)
+        else:
+            runs.append([int(s), int(c)])
+    return runs
+
+
+@pytest.mark.parametrize("cell,leaf", [(0.01, 256), (1 / 16, 8), (0.001, 32)])
+def test_grid_selection_matches_oracle_traversal(cell, leaf):
+    db = W.CFG1.db(2000)
+    og = O.Grid(W.UNIT_BOX, cell, leaf).add(db)
+    pg = api.GridIndex(W.UNIT_BOX, cell, leaf).build(db, upload=False)
+    info = pg.info()
+    oi = og.info()
+    assert (info[4], info[5], info[0]) == (oi["nodes"], oi["leaves"], oi["points"])
+    rects = np.concatenate([W.CFG1.rects(), api.synth_rects(9, 200, 1e-4, 0.5, True)])
+    # Rectangles on cell edges (BoxIntersect drops leaves that only touch).
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, int(1 / cell), size=(100, 2)).astype(np.float64)
+    span = rng.integers(1, 4, size=(100, 2))
+    edge = np.stack([k[:, 0] * cell, 1 - (k[:, 1] + span[:, 1]) * cell, (k[:, 0] + span[:, 0]) * cell,
+                     1 - k[:, 1] * cell], 1).astype(np.float32)
+    rects = np.concatenate([rects, edge]).astype(np.float32)
+    tasks, cand = pg.select(rects)
+    for q in range(len(rects)):
+        want = _merged_runs(og.leaves(rects[q]))
+        assert _task_runs(tasks, q) == want, q
+        assert cand[q] == sum(c for _, c in want)
+
+
+def test_grid_store_order_matches_oracle():
+    """Selection runs index the product store; the oracle's store has the
+    reference's order (cells in Morton order, sub-trajectories in id order), so
+    identical runs mean identical candidate points."""
+    db = W.CFG1.db(500)
+    og = O.Grid(W.UNIT_BOX, 0.01, 64).add(db)
+    x, y, t, p = og.store()
+    # Every cell's block in the oracle store is contiguous and in (tid, pid) order.
+    cells = np.array([og.cell_of(a, b) for a, b in zip(x, y)])
+    assert (np.diff(cells) >= 0).all()
+    same = np.diff(cells) == 0
+    key = t.astype(np.int64) << 32 | p
+    assert (np.diff(key)[same] > 0).all()
+
+
+def test_fsg_and_stig_selection_cover_brute_force():
+    db = W.CFG1.db(1500)
+    rects = np.concatenate([W.CFG1.rects(), api.synth_rects(3, 100, 1e-4, 0.3, True)]).astype(np.float32)
+    brute = O.brute_range(db, rects, "digest")[0]
+    for ix in (api.FsgIndex(W.UNIT_BOX, 0.01).build(db, upload=False),
+               api.StigIndex(64).build(db, upload=False)):
+        tasks, cand = ix.select(rects)
+        assert (cand >= brute).all()
+        assert (tasks[:, 1] <= 1024).all()
+
+
+def test_stig_leaf_count():
+    db = W.CFG1.db(700)
+    s = api.StigIndex(100).build(db, upload=False).info()
+    assert s[0] == db.npoints
+    assert s[5] >= db.npoints // 100
+
+
+def test_generator_is_deterministic_and_shaped():
+    a = W.CFG1.db(300)
+    b = W.CFG1.db(300, threads=3)
+    assert (a.offsets == b.offsets).all() and (a.lon == b.lon).all() and (a.lat == b.lat).all()
+    L = a.lengths
+    assert L.min() >= 16 and L.max() <= 256
+    assert (a.lon >= 0).all() and (a.lon <= 1).all() and (a.lat >= 0).all() and (a.lat <= 1).all()
+    steps = np.hypot(np.diff(a.lon), np.diff(a.lat))
+    inside = np.ones(len(steps), bool)
+    inside[a.offsets[1:-1] - 1] = False
+    assert 0.5e-4 < np.median(steps[inside]) < 3e-4
+    L2 = W.CFG2.lengths(100_000)
+    assert 8 <= L2.min() and L2.max() <= 2048 and 60 <= np.median(L2) <= 68
+    r = W.CFG2.rects(5000)
+    side = r[:, 2] - r[:, 0]
+    assert 0.9e-4 < side.min() and side.max() < 0.11
+    assert abs(np.median(np.log10(side)) - (-2.5)) < 0.1
+
+
+def test_host_rejects_points_outside_the_grid():
+    db = api.synth_walks(1, np.array([3, 3], np.int32))
+    db.lon[4] = 2.0
+    with pytest.raises(N.GatError):
+        api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db, upload=False)
+    with pytest.raises(N.GatError):
+        api.FsgIndex(W.UNIT_BOX, 0.01).build(db, upload=False)

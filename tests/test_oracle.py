@@ -1,0 +1,267 @@
+"""Pins the oracle (CPU restatement of the reference) before anything is checked
+against it: against the reference's own compiled MBB.cpp, against closed forms
+and hand-derived known answers of the EDR recurrence, against independent
+restatements, and against brute force on small inputs.  The reference ships no
+tests or golden data (its only check is the CHECK_CORRECT dump,
+ConstDefine.h:8, SystemTest.cpp:79-89)."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gat_amd import api, workloads as W
+
+EPS = 5e-4
+
+
+# ------------------------------------------------------------- geometry
+def _edge_boxes(rng, n):
+    """Boxes on a coarse lattice so that edges coincide, touch and nest often."""
+    v = rng.integers(0, 5, size=(n, 4)).astype(np.float32) * np.float32(0.25)
+    b = np.stack([np.minimum(v[:, 0], v[:, 2]), np.minimum(v[:, 1], v[:, 3]),
+                  np.maximum(v[:, 0], v[:, 2]), np.maximum(v[:, 1], v[:, 3])], 1)
+    return b.astype(np.float32)
+
+
+def test_pinbox_inclusive_edges():
+    box = np.array([0.25, 0.5, 0.75, 1.0], np.float32)
+    for x, y in [(0.25, 0.5), (0.75, 1.0), (0.25, 1.0), (0.75, 0.5), (0.5, 0.75)]:
+        assert O.pinbox(box, x, y) == 1
+    for x, y in [(np.nextafter(np.float32(0.25), np.float32(0)), 0.75), (0.5, np.nextafter(np.float32(1.0), np.float32(2)))]:
+        assert O.pinbox(box, x, y) == 0
+
+
+def test_geometry_matches_reference_mbb():
+    ref = O.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (the reference sources are absent here)")
+    rng = np.random.default_rng(7)
+    boxes = _edge_boxes(rng, 400)
+    c = O._p
+    for a, b in itertools.product(boxes[:60], boxes[:60]):
+        assert O.box_intersect(a, b) == ref.ref_box_intersect(c(a), c(b)), (a, b)
+    pts = rng.integers(0, 5, size=(200, 2)).astype(np.float32) * np.float32(0.25)
+    for box in boxes[:50]:
+        for x, y in pts:
+            assert O.pinbox(box, x, y) == ref.ref_pinbox(c(box), float(x), float(y))
+
+
+def test_box_intersect_closed_form():
+    # Touching boxes are disjoint; containment codes depend on argument order.
+    a = np.array([0, 0, 1, 1], np.float32)
+    assert O.box_intersect(a, np.array([1, 0, 2, 1], np.float32)) == 0   # touch on x
+    assert O.box_intersect(a, np.array([0, 1, 1, 2], np.float32)) == 0   # touch on y
+    assert O.box_intersect(a, np.array([0.5, 0.5, 2, 2], np.float32)) == 1
+    inner = np.array([0.25, 0.25, 0.75, 0.75], np.float32)
+    assert O.box_intersect(a, inner) == 2
+    assert O.box_intersect(inner, a) == 3
+
+
+def test_morton_known_answers():
+    assert O.morton(0, 0) == 0
+    assert O.morton(1, 0) == 1
+    assert O.morton(0, 1) == 2
+    assert O.morton(1, 1) == 3
+    assert O.morton(3, 5) == 39
+    # Closed form: x's bits in even positions, y's in odd positions.
+    rng = np.random.default_rng(1)
+    for x, y in rng.integers(0, 4096, size=(500, 2)):
+        want = 0
+        for b in range(13):
+            want |= ((int(x) >> b) & 1) << (2 * b)
+            want |= ((int(y) >> b) & 1) << (2 * b + 1)
+        assert O.morton(int(x), int(y)) == want
+
+
+# ------------------------------------------------------------- EDR
+def pts(*coords):
+    return np.array(coords, np.float32).reshape(-1, 2)
+
+
+A, B, C, D = (0.1, 0.1), (0.3, 0.3), (0.5, 0.5), (0.7, 0.7)
+
+
+def test_edr_known_answers():
+    assert O.edr(pts(A), pts(A), EPS) == 0
+    assert O.edr(pts(A), pts(B), EPS) == 1
+    assert O.edr(pts(A), pts(A, B), EPS) == 1           # D[1][2] = L = 1
+    assert O.edr(pts(A), pts(B, C), EPS) == 3           # M == L < U returns U = 3 (textbook: 2)
+    assert O.edr_textbook(pts(A), pts(B, C), EPS) == 2
+    assert O.edr(pts(A, B), pts(A, B), EPS) == 0
+    assert O.edr(pts(), pts(A, B, C), EPS) == 3         # D[0][m] = m
+
+
+def allmiss(n, m):
+    q = np.stack([np.linspace(0.0, 0.1, n), np.zeros(n)], 1).astype(np.float32)
+    c = np.stack([np.linspace(0.5, 0.6, m), np.ones(m) * 0.5], 1).astype(np.float32)
+    return q, c
+
+
+def test_edr_all_mismatch_closed_form_one_row():
+    # With one row and no match: D[1][c] = c when c is odd, c + 1 when c is even
+    # (by induction on c: L ties M exactly when D[1][c-1] = c - 1).
+    for m in range(1, 40):
+        q, c = allmiss(1, m)
+        assert O.edr(q, c, EPS) == m + (1 - m % 2)
+
+
+def test_edr_all_mismatch_table_values():
+    want = {(1, k): v for k, v in zip(range(1, 10), [1, 3, 3, 5, 5, 7, 7, 9, 9])}
+    want.update({(2, k): v for k, v in zip(range(2, 11), [2, 3, 6, 6, 6, 7, 10, 10, 10])})
+    want.update({(3, k): v for k, v in zip(range(3, 12), [3, 7, 7, 7, 7, 11, 11, 11, 11])})
+    want.update({(4, k): v for k, v in zip(range(4, 13), [4, 5, 6, 7, 12, 12, 12, 12, 12])})
+    want.update({(16, k): k for k in range(16, 29)})
+    want[(20, 32)] = 52
+    F = O.ftable(40)
+    for (n, m), v in want.items():
+        q, c = allmiss(n, m)
+        assert O.edr(q, c, EPS) == v, (n, m)
+        assert F[n, m] == v, (n, m)
+
+
+def test_edr_eps_boundary():
+    below = np.float32(4.9999997e-4)
+    at = np.float32(5e-4)  # 5.0000002e-4 > 5e-4 (double)
+    assert float(at) > 5e-4 and float(below) < 5e-4
+    assert O.edr(pts((0, 0)), pts((below, 0)), EPS) == 0
+    assert O.edr(pts((0, 0)), pts((at, 0)), EPS) == 1
+    assert O.edr(pts((0, 0)), pts((0, -below)), EPS) == 0
+    assert O.edr(pts((0, at)), pts((0, 0)), EPS) == 1
+    # The difference is rounded to float before the comparison: 0.25 + below
+    # rounds to a float whose distance from 0.25 is not `below`.
+    x = np.float32(0.25) + below
+    d = np.float32(x - np.float32(0.25))
+    assert O.edr(pts((0.25, 0.25)), pts((x, 0.25)), EPS) == (0 if float(d) < 5e-4 else 1)
+
+
+def _levenshtein(a, b):
+    prev = list(range(len(b) + 1))
+    for i, x in enumerate(a, 1):
+        cur = [i]
+        for j, y in enumerate(b, 1):
+            cur.append(min(prev[j - 1] + (x != y), prev[j] + 1, cur[j - 1] + 1))
+        prev = cur
+    return prev[-1]
+
+
+def test_edr_textbook_is_levenshtein_on_symbols():
+    # Points far apart act as symbols; textbook EDR is then Levenshtein distance.
+    rng = np.random.default_rng(3)
+    sym = np.array([[0.1 * k, 0.05 * k] for k in range(6)], np.float32)
+    for _ in range(200):
+        a = rng.integers(0, 6, rng.integers(1, 9))
+        b = rng.integers(0, 6, rng.integers(1, 9))
+        assert O.edr_textbook(sym[a], sym[b], EPS) == _levenshtein(list(a), list(b))
+
+
+def _walk_pair(rng, n, m, noisy):
+    q = (rng.random(2) * 0.5 + 0.25) + np.cumsum(rng.normal(0, 1e-4, (n, 2)), 0)
+    if noisy:
+        idx = np.minimum(np.arange(m) * n // m, n - 1)
+        c = q[idx] + rng.normal(0, 2e-4, (m, 2))
+    else:
+        c = (rng.random(2) * 0.5 + 0.25) + np.cumsum(rng.normal(0, 1e-4, (m, 2)), 0)
+    return q.astype(np.float32), c.astype(np.float32)
+
+
+def test_edr_restatements_agree_and_bound():
+    rng = np.random.default_rng(11)
+    orient_diff = 0
+    for i in range(300):
+        n, m = rng.integers(1, 70, 2)
+        q, c = _walk_pair(rng, int(n), int(m), noisy=i % 3 != 0)
+        d = O.edr(q, c, EPS)
+        assert d == O.edr_antidiag(q, c, EPS)
+        assert d >= O.edr_textbook(q, c, EPS) >= abs(int(n) - int(m))
+        assert d <= int(n) + int(m)
+        if n == m and O.edr(c, q, EPS) != d:
+            orient_diff += 1
+    # Zero-match pairs take the F-table value.
+    F = O.ftable(80)
+    for _ in range(50):
+        n, m = (int(x) for x in rng.integers(1, 80, 2))
+        q, c = allmiss(n, m)
+        assert O.edr(q, c, EPS) == F[min(n, m), max(n, m)]
+
+
+def test_edr_orientation_rule():
+    # Rows are the shorter trajectory, the query on equal lengths
+    # (Grid.cpp:1234-1247); the rule is asymmetric, so the choice is visible.
+    rng = np.random.default_rng(5)
+    found = False
+    for _ in range(2000):
+        n = int(rng.integers(2, 8))
+        q, c = _walk_pair(rng, n, n, noisy=True)
+        if O.edr(q, c, EPS) != O.edr(c, q, EPS):
+            found = True
+            break
+    assert found
+    # Unequal lengths: the shorter is the rows whichever argument it is.
+    for _ in range(100):
+        q, c = _walk_pair(rng, 5, 9, noisy=True)
+        assert O.edr(q, c, EPS) == O.edr(c, q, EPS)
+
+
+def test_topk_order():
+    rng = np.random.default_rng(2)
+    d = rng.integers(0, 20, 500).astype(np.int32)
+    ids = np.arange(1, 501, dtype=np.int32)
+    oi, od = O.topk(d, ids, 37)
+    order = np.lexsort((ids, d))[:37]
+    assert (oi == ids[order]).all() and (od == d[order]).all()
+
+
+# ------------------------------------------------------------- range
+def _small_db(seed, ntraj=300):
+    lengths = api.synth_lengths(seed, ntraj, "uniform", 1, 60, 1, 60)
+    return api.synth_walks(seed, lengths, step_sigma=2e-2)
+
+
+def _sets(res):
+    return [set(map(tuple, r.tolist())) for r in res]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_indexes_against_brute_force(seed):
+    db = _small_db(seed)
+    rng = np.random.default_rng(seed)
+    rects = api.synth_rects(seed, 60, 0.01, 0.5, True)
+    # Rectangles whose edges sit exactly on cell edges and on data points.
+    cell = np.float32(1 / 16)
+    lat = rng.integers(0, 16, size=(30, 4)).astype(np.float32) * cell
+    rects = np.concatenate([rects, np.stack([np.minimum(lat[:, 0], lat[:, 2]), np.minimum(lat[:, 1], lat[:, 3]),
+                                             np.maximum(lat[:, 0], lat[:, 2]) + cell,
+                                             np.maximum(lat[:, 1], lat[:, 3]) + cell], 1)])
+    k = rng.integers(0, db.npoints, 20)
+    rects = np.concatenate([rects, np.stack([db.lon[k], db.lat[k], db.lon[k] + 0.05, db.lat[k] + 0.05], 1)])
+    rects = rects.astype(np.float32)
+    brute = _sets(O.brute_range(db, rects))
+    grid = _sets(O.Grid(W.UNIT_BOX, float(cell), 8).add(db).range(rects))
+    fsg = _sets(O.FSG(W.UNIT_BOX, float(cell)).add(db).range(rects))
+    stig = _sets(O.STIG().add(db, 8).range(rects))
+    for q in range(len(rects)):
+        assert fsg[q] == brute[q]
+        assert stig[q] == brute[q]
+        assert grid[q] <= brute[q]
+        # Any point the quadtree misses lies on the rectangle's boundary.
+        for tid, pid in brute[q] - grid[q]:
+            i = db.offsets[tid - 1] + pid
+            x, y = db.lon[i], db.lat[i]
+            r = rects[q]
+            assert x in (r[0], r[2]) or y in (r[1], r[3])
+
+
+def test_oracle_range_modes_agree():
+    db = _small_db(4)
+    rects = api.synth_rects(4, 40, 0.02, 0.4, True)
+    g = O.Grid(W.UNIT_BOX, 1 / 16, 8).add(db)
+    pairs = g.range(rects)
+    bitmap = g.range(rects, "bitmap")
+    counts, dig = g.range(rects, "digest")
+    for q, p in enumerate(pairs):
+        assert counts[q] == len(p)
+        assert set(np.flatnonzero(bitmap[q])) == set(p[:, 0].tolist())
+        assert int(dig[q]) == sum(O.pair_digest(int(t), int(i)) for t, i in p) % (1 << 64)
+    # Multi-threaded slicing (Grid.cpp:407-426) gives the same answer.
+    assert all((a == b).all() for a, b in zip(g.range(rects, nthreads=5), pairs))
