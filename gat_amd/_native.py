@@ -61,7 +61,7 @@ HIP_FUNCS = {
     "gatRangeQuerySTIGHandler": (INT, [P, P, I64, P, INT, P, P, INT, INT, P]),
     "gat_range_refine_bitmap": (INT, [P, P, I64, P, I64, P, I64, P, I64, P]),
     "gat_range_refine_pairs": (INT, [P, P, I64, P, I64, P, I64, P, P, P, I64, P]),
-    "gat_pairs_digest": (INT, [P, P, P, I64, P, P]),
+    "gat_pairs_digest": (INT, [P, P, P, I64, I64, P, P]),
     "gatEDRDistanceBatchHandler": (INT, [INT, P, P, P, P, P, P, P, F64, P, P]),
 }
 

@@ -168,13 +168,13 @@ __device__ __forceinline__ uint64_t pair_mix(int2 p) {
 __global__ __launch_bounds__(kBlock) void pairs_digest_kernel(const int2* __restrict__ pairs,
                                                              const uint64_t* __restrict__ qbase,
                                                              const uint32_t* __restrict__ qcount, uint32_t nq,
-                                                             uint64_t* __restrict__ out) {
+                                                             uint64_t capacity, uint64_t* __restrict__ out) {
   __shared__ uint64_t part[kBlock / kWave];
   for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
     const uint64_t b = qbase[q];
-    const uint32_t n = qcount[q];
+    const uint64_t n = b < capacity ? (qcount[q] < capacity - b ? qcount[q] : capacity - b) : 0;
     uint64_t acc = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) acc += pair_mix(pairs[b + i]);
+    for (uint64_t i = threadIdx.x; i < n; i += kBlock) acc += pair_mix(pairs[b + i]);
     for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = acc;
     __syncthreads();
@@ -192,13 +192,13 @@ __global__ __launch_bounds__(kBlock) void pairs_digest_kernel(const int2* __rest
 extern "C" {
 
 int gat_pairs_digest(const int32_t* pairs, const uint64_t* qbase, const uint32_t* qcount, int64_t nq,
-                     uint64_t* digest, void* stream) {
+                     int64_t pairs_capacity, uint64_t* digest, void* stream) {
   GAT_REQUIRE(nq >= 0 && nq <= int64_t(UINT32_MAX), "pairs digest: bad query count");
   if (nq == 0) return GAT_OK;
-  GAT_REQUIRE(pairs && qbase && qcount && digest, "pairs digest: null pointer");
+  GAT_REQUIRE(pairs && qbase && qcount && digest && pairs_capacity >= 0, "pairs digest: bad argument");
   const int64_t blocks = nq < int64_t(gat::cu_count()) * 8 ? nq : int64_t(gat::cu_count()) * 8;
   pairs_digest_kernel<<<dim3(uint32_t(blocks)), dim3(kBlock), 0, gat::as_stream(stream)>>>(
-      reinterpret_cast<const int2*>(pairs), qbase, qcount, uint32_t(nq), digest);
+      reinterpret_cast<const int2*>(pairs), qbase, qcount, uint32_t(nq), uint64_t(pairs_capacity), digest);
   GAT_LAUNCH_CHECK("pairs_digest_kernel");
   return GAT_OK;
 }

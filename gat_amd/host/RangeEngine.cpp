@@ -187,7 +187,7 @@ int RangeBatch::digests(uint64_t* out) const {
   DeviceBuffer d;
   if (int rc = d.reserve(size_t(queryNum) * 8)) return rc;
   GATH_TRY(gat_pairs_digest(pairs.as<int32_t>(), qbase.as<uint64_t>(), qcount.as<uint32_t>(), queryNum,
-                            d.as<uint64_t>(), nullptr),
+                            int64_t(pairs.size() / 8), d.as<uint64_t>(), nullptr),
            "pair digest");
   GATH_TRY(gat_memcpy_d2h(out, d.get(), size_t(queryNum) * 8, nullptr), "digest readback");
   GATH_TRY(gat_device_sync(), "digest readback");

@@ -121,11 +121,12 @@ int gat_range_refine_pairs(const float* xy_dev, const int32_t* tp_dev, int64_t n
 /* Per-query order-independent digest of a compacted-pair result: digest_dev[q]
  * = wrapping sum over query q's pairs of a 64-bit mix (the splitmix64
  * finaliser) of (uint64(tid) << 32 | pid).  Lets two results be compared as
- * sets without copying them.  Asynchronous on `stream`.  (No reference
+ * sets without copying them.  Reads stop at `pairs_capacity` pairs.
+ * Asynchronous on `stream`.  (No reference
  * counterpart: the reference checks results by dumping them to text files,
  * SystemTest.cpp:79-89.) */
 int gat_pairs_digest(const int32_t* pairs_dev, const uint64_t* qbase_dev, const uint32_t* qcount_dev, int64_t nq,
-                     uint64_t* digest_dev, void* stream);
+                     int64_t pairs_capacity, uint64_t* digest_dev, void* stream);
 
 /* ------------------------------------------------------------------ EDR */
 /* One (query, candidate) pair.  Replaces TaskInfoTableForSimilarity
