@@ -73,6 +73,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["all", "range", "copy"], default="all",
+                    help="range: just the timed GT-quadtree batches (for profiling); copy: just the stream-copy "
+                         "calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU split of the "
@@ -83,8 +86,13 @@ def main() -> int:
     from gat_amd import workloads as W
 
     G.init(0)
-    dev = G.device_name()
+    dev = G.device_name().strip()
     log(f"device {dev}")
+    if args.only == "copy":
+        gbs, ms = G.stream_copy_bandwidth(1 << 32, max(args.steps, 1))
+        log(f"stream copy: {gbs:.1f} GB/s ({ms:.3f} ms per 4 GiB copy)")
+        print(json.dumps({"stream_copy_gbs": round(gbs, 1), "kernel_ms": round(ms, 4), "bytes_per_launch": 2 << 32}))
+        return 0
     w = W.CFG2
     t = time.perf_counter()
     db = w.db()
@@ -100,6 +108,13 @@ def main() -> int:
     k_mean = float(kms.mean())
     achieved = alg_bytes / (k_mean * 1e-3) / 1e9
     traffic = load_traffic("range_refine_kernel<true>:cfg2")
+    if args.only == "range":
+        print(json.dumps({"value": round(qps, 1), "kernel_ms": round(k_mean, 4), "achieved_gbs": round(achieved, 1),
+                          "algorithmic_bytes": int(alg_bytes), "candidates": res.candidates,
+                          "pairs": res.pairs_total}))
+        return 0
+    copy_gbs, _ = G.stream_copy_bandwidth(1 << 32, 10)
+    log(f"stream copy reference: {copy_gbs:.1f} GB/s")
 
     t = time.perf_counter()
     fsg = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db)
@@ -179,6 +194,7 @@ def main() -> int:
             "edr_plain_cells_per_s": float(f"{edr_cells_ps:.4e}"),
             "range_p50_ms": round(1e3 * float(np.percentile(wall, 50)), 3),
             "range_p99_ms": round(1e3 * float(np.percentile(wall, 99)), 3),
+            "stream_copy_gbs": round(copy_gbs, 1),
         },
         "device": dev,
     }

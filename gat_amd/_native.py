@@ -32,6 +32,14 @@ INT = C.c_int
 SZ = C.c_size_t
 PP = C.POINTER(C.c_void_p)
 
+
+
+class PointStore(C.Structure):
+    """gat_point_store of include/gat_hip.h."""
+    _fields_ = [("xy_dev", C.c_void_p), ("id_dev", C.c_void_p), ("pid_dev", C.c_void_p), ("npoints", C.c_int64),
+                ("pid_bits", C.c_int32)]
+
+
 # name: (restype, [argtypes])
 HIP_FUNCS = {
     "gat_last_error": (C.c_char_p, []),
@@ -57,10 +65,11 @@ HIP_FUNCS = {
     "gat_event_sync": (INT, [P]),
     "gat_event_elapsed_ms": (INT, [C.POINTER(F32), P, P]),
     "gat_eps_threshold": (F32, [F64]),
-    "gatRangeQueryTestHandler": (INT, [P, P, I64, P, INT, P, P, INT, INT, P]),
-    "gatRangeQuerySTIGHandler": (INT, [P, P, I64, P, INT, P, P, INT, INT, P]),
-    "gat_range_refine_bitmap": (INT, [P, P, I64, P, I64, P, I64, P, I64, P]),
-    "gat_range_refine_pairs": (INT, [P, P, I64, P, I64, P, I64, P, P, P, I64, P]),
+    "gat_stream_copy": (INT, [P, P, SZ, P]),
+    "gatRangeQueryTestHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
+    "gatRangeQuerySTIGHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
+    "gat_range_refine_bitmap": (INT, [P, P, I64, P, I64, P, I64, P]),
+    "gat_range_refine_pairs": (INT, [P, P, I64, P, I64, P, P, P, I64, P]),
     "gat_pairs_digest": (INT, [P, P, P, I64, I64, P, P]),
     "gatEDRDistanceBatchHandler": (INT, [INT, P, P, P, P, P, P, P, F64, P, P]),
 }

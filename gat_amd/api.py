@@ -43,6 +43,38 @@ def eps_threshold(eps: float) -> float:
     return float(N.hip().gat_eps_threshold(eps))
 
 
+def stream_copy_bandwidth(nbytes: int = 1 << 32, reps: int = 10) -> tuple[float, float]:
+    """HBM bandwidth of the plain streaming-copy kernel: (GB/s counting read +
+    write bytes, mean kernel ms), timed with HIP events on its own stream."""
+    h = N.hip()
+    src, dst, s, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    N.check_hip(h.gat_malloc(C.byref(src), nbytes))
+    N.check_hip(h.gat_malloc(C.byref(dst), nbytes))
+    try:
+        N.check_hip(h.gat_stream_create(C.byref(s)))
+        N.check_hip(h.gat_event_create(C.byref(e0)))
+        N.check_hip(h.gat_event_create(C.byref(e1)))
+        N.check_hip(h.gat_memset(src, 1, nbytes, s))
+        N.check_hip(h.gat_stream_copy(dst, src, nbytes, s))
+        N.check_hip(h.gat_event_record(e0, s))
+        for _ in range(reps):
+            N.check_hip(h.gat_stream_copy(dst, src, nbytes, s))
+        N.check_hip(h.gat_event_record(e1, s))
+        N.check_hip(h.gat_event_sync(e1))
+        ms = C.c_float()
+        N.check_hip(h.gat_event_elapsed_ms(C.byref(ms), e0, e1))
+        per = ms.value / reps
+        return 2 * nbytes / (per * 1e-3) / 1e9, per
+    finally:
+        for e in (e0, e1):
+            if e.value:
+                h.gat_event_destroy(e)
+        if s.value:
+            h.gat_stream_destroy(s)
+        h.gat_free(src)
+        h.gat_free(dst)
+
+
 # ---------------------------------------------------------------- workloads
 @dataclass
 class TrajSet:

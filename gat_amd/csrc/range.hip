@@ -3,27 +3,39 @@
 // (reference kernels cudaRangeQueryTest, kernel.cu:1004-1074, and
 // cudaRangeQuerySTIG, kernel.cu:1165-1209).
 //
-// Layout: one wave64 refines one task (a run of consecutive store points and
-// a rectangle).  Lanes read the run as float2 (lon, lat) pairs, 64 consecutive
-// points per load instruction, four loads in flight per lane, and test them
-// against the rectangle held in scalar registers.  Hits are compacted with a
-// wave ballot and a per-lane prefix (mbcnt); one atomic per 256 points reserves
-// the wave's slice of the query's output region, so every result pair is
-// written exactly once and no pass re-reads the points.  The (tid, pid) pair
-// of a hit is gathered from the parallel int2 array only for hits.
+// Layout: one wave64 refines one task (a run of consecutive store points and a
+// rectangle).  Lanes read the run as float2 (lon, lat) pairs, 64 consecutive
+// points per load instruction, up to sixteen loads (1024 points) in flight per
+// lane, and test them against the rectangle held in scalar registers.  Hits are
+// compacted with wave ballots and per-lane prefixes (mbcnt).  The ids of the
+// hit points are gathered before the wave's single atomic on the query's
+// counter returns, so that latency overlaps the gathers; every result pair is
+// then written exactly once and no pass re-reads the points.
+//
+// Ids are stored compactly: (tid << pid_bits) | pid in 32 bits when the
+// trajectory and point ids fit, otherwise a 32-bit tid array and a 16-bit pid
+// array — 4 or 6 bytes per hit instead of the 8 of an (int, int) pair.
 #include "gat_common.hpp"
 
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kBlock = 256;             // 4 waves per workgroup
-constexpr int kUnroll = 4;              // float2 loads in flight per lane
-constexpr int kSpan = kWave * kUnroll;  // points per wave iteration
+constexpr int kBlock = 256;               // 4 waves per workgroup
+constexpr int kSlots = 16;                // float2 loads in flight per lane
+constexpr int kSpan = kWave * kSlots;     // points per wave iteration
 
 struct Task {
   uint32_t start, count, qid;
 };
 static_assert(sizeof(Task) == sizeof(gat_range_task), "task layout");
+
+struct Store {
+  const float2* xy;
+  const uint32_t* id;
+  const uint16_t* pid;
+  uint64_t npoints;
+  uint32_t pid_bits;
+};
 
 // MBB::pInBox (MBB.cpp:21-26): inclusive on all four edges.
 __device__ __forceinline__ bool in_box(const float4& b, float x, float y) {
@@ -36,12 +48,21 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <bool kPairs>
+template <bool kPacked>
+__device__ __forceinline__ int2 point_ids(const Store& s, uint64_t i) {
+  if constexpr (kPacked) {
+    const uint32_t v = s.id[i];
+    return make_int2(int(v >> s.pid_bits), int(v & ((1u << s.pid_bits) - 1u)));
+  } else {
+    return make_int2(int(s.id[i]), int(s.pid[i]));
+  }
+}
+
+template <bool kPairs, bool kPacked>
 __global__ __launch_bounds__(kBlock) void range_refine_kernel(
-    const float2* __restrict__ xy, const int2* __restrict__ tp, uint64_t npoints, const Task* __restrict__ tasks,
-    uint64_t ntasks, const float4* __restrict__ boxes, uint32_t nqueries, const uint64_t* __restrict__ qbase,
-    uint32_t* __restrict__ qcount, int2* __restrict__ pairs, uint64_t capacity, uint8_t* __restrict__ bitmap,
-    uint64_t traj_stride) {
+    Store store, const Task* __restrict__ tasks, uint64_t ntasks, const float4* __restrict__ boxes,
+    uint32_t nqueries, const uint64_t* __restrict__ qbase, uint32_t* __restrict__ qcount, int2* __restrict__ pairs,
+    uint64_t capacity, uint8_t* __restrict__ bitmap, uint64_t traj_stride) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t wave = uniform((blockIdx.x * kBlock + threadIdx.x) / kWave);
   const uint64_t nwaves = uint64_t(gridDim.x) * (kBlock / kWave);
@@ -51,43 +72,46 @@ __global__ __launch_bounds__(kBlock) void range_refine_kernel(
     const uint32_t count = uniform(task.count);
     const uint64_t start = uniform(task.start);
     // Malformed tasks are skipped rather than read or written out of bounds.
-    if (qid >= nqueries || start + count > npoints) continue;
+    if (qid >= nqueries || start + count > store.npoints) continue;
     const float4 box = boxes[qid];
-    const float2* p = xy + start;
+    const float2* p = store.xy + start;
     for (uint32_t base = 0; base < count; base += kSpan) {
-      float2 v[kUnroll];
+      const uint32_t left = count - base;
+      float2 v[kSlots];
 #pragma unroll
-      for (int k = 0; k < kUnroll; ++k) {
-        uint32_t i = base + k * kWave + lane;
-        v[k] = i < count ? p[i] : make_float2(__builtin_nanf(""), 0.0f);
+      for (int k = 0; k < kSlots; ++k) {
+        const uint32_t i = k * kWave + lane;
+        v[k] = i < left ? p[base + i] : make_float2(__builtin_nanf(""), 0.0f);
       }
-      bool hit[kUnroll];
-      uint64_t mask[kUnroll];
+      uint64_t mask[kSlots];
       uint32_t total = 0;
 #pragma unroll
-      for (int k = 0; k < kUnroll; ++k) {
-        hit[k] = in_box(box, v[k].x, v[k].y);  // NaN fails every comparison
-        mask[k] = __ballot(hit[k]);
+      for (int k = 0; k < kSlots; ++k) {
+        mask[k] = __ballot(in_box(box, v[k].x, v[k].y));  // NaN fails every comparison
         total += __popcll(mask[k]);
       }
       if (total == 0) continue;
       if constexpr (kPairs) {
+        int2 ids[kSlots];
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k)
+          if ((mask[k] >> lane) & 1) ids[k] = point_ids<kPacked>(store, start + base + k * kWave + lane);
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(qcount + qid, total);
         slot = uniform(slot);
         const uint64_t first = qbase[qid] + slot;
         uint32_t before = 0;
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
+        for (int k = 0; k < kSlots; ++k) {
           const uint64_t at = first + before + lanes_below(mask[k]);
-          if (hit[k] && at < capacity) pairs[at] = tp[start + base + k * kWave + lane];
+          if (((mask[k] >> lane) & 1) && at < capacity) pairs[at] = ids[k];
           before += __popcll(mask[k]);
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k)
-          if (hit[k]) {
-            const uint32_t tid = uint32_t(tp[start + base + k * kWave + lane].x);
+        for (int k = 0; k < kSlots; ++k)
+          if ((mask[k] >> lane) & 1) {
+            const uint32_t tid = uint32_t(point_ids<kPacked>(store, start + base + k * kWave + lane).x);
             if (tid < traj_stride) bitmap[uint64_t(qid) * traj_stride + tid] = 1;
           }
       }
@@ -95,41 +119,59 @@ __global__ __launch_bounds__(kBlock) void range_refine_kernel(
   }
 }
 
-int launch_refine(bool pairs_mode, const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
-                  int64_t ntasks, const float* boxes, int64_t nqueries, const uint64_t* qbase, uint32_t* qcount,
-                  int32_t* pairs, int64_t capacity, uint8_t* bitmap, int64_t traj_stride, void* stream) {
-  GAT_REQUIRE(ntasks >= 0 && npoints >= 0 && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX),
-              "range refine: bad sizes");
+int check_store(const gat_point_store* s) {
+  GAT_REQUIRE(s, "range refine: null store");
+  GAT_REQUIRE(s->npoints >= 0, "range refine: negative store size");
+  GAT_REQUIRE(s->pid_bits >= 0 && s->pid_bits < 32, "range refine: bad pid_bits");
+  GAT_REQUIRE(s->npoints == 0 || (s->xy_dev && s->id_dev && (s->pid_bits > 0 || s->pid_dev)),
+              "range refine: null store array");
+  return GAT_OK;
+}
+
+int launch_refine(bool pairs_mode, const gat_point_store* sp, const gat_range_task* tasks, int64_t ntasks,
+                  const float* boxes, int64_t nqueries, const uint64_t* qbase, uint32_t* qcount, int32_t* pairs,
+                  int64_t capacity, uint8_t* bitmap, int64_t traj_stride, void* stream) {
+  if (int rc = check_store(sp)) return rc;
+  GAT_REQUIRE(ntasks >= 0 && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX), "range refine: bad sizes");
   if (ntasks == 0) return GAT_OK;
-  GAT_REQUIRE(xy && tp && tasks && boxes, "range refine: null input pointer");
-  // Enough waves to keep every SIMD busy, capped so the grid-stride loop keeps
-  // its scalar task fetches amortised.
-  const int64_t waves_needed = ntasks;
+  GAT_REQUIRE(tasks && boxes, "range refine: null input pointer");
+  const Store st{reinterpret_cast<const float2*>(sp->xy_dev), sp->id_dev, sp->pid_dev, uint64_t(sp->npoints),
+                 uint32_t(sp->pid_bits)};
+  const bool packed = sp->pid_bits > 0;
+  // Enough waves to keep every SIMD busy; beyond that the grid-stride loop
+  // hands each wave further tasks.
   const int64_t blocks_cap = int64_t(gat::cu_count()) * 16;
-  int64_t blocks = (waves_needed + (kBlock / kWave) - 1) / (kBlock / kWave);
+  int64_t blocks = (ntasks + (kBlock / kWave) - 1) / (kBlock / kWave);
   if (blocks > blocks_cap) blocks = blocks_cap;
   hipStream_t s = gat::as_stream(stream);
+  const dim3 grid{uint32_t(blocks)}, block{uint32_t(kBlock)};
+  const auto* T = reinterpret_cast<const Task*>(tasks);
+  const auto* B = reinterpret_cast<const float4*>(boxes);
   if (pairs_mode) {
-    GAT_REQUIRE(qbase && qcount && pairs, "range refine: null output pointer");
-    GAT_REQUIRE(capacity >= 0, "range refine: negative capacity");
-    range_refine_kernel<true><<<dim3(uint32_t(blocks)), dim3(kBlock), 0, s>>>(
-        reinterpret_cast<const float2*>(xy), reinterpret_cast<const int2*>(tp), uint64_t(npoints),
-        reinterpret_cast<const Task*>(tasks), uint64_t(ntasks), reinterpret_cast<const float4*>(boxes),
-        uint32_t(nqueries), qbase, qcount, reinterpret_cast<int2*>(pairs), uint64_t(capacity), nullptr, 0);
+    GAT_REQUIRE(qbase && qcount && pairs && capacity >= 0, "range refine: bad output arguments");
+    auto* P = reinterpret_cast<int2*>(pairs);
+    if (packed)
+      range_refine_kernel<true, true><<<grid, block, 0, s>>>(st, T, uint64_t(ntasks), B, uint32_t(nqueries), qbase,
+                                                               qcount, P, uint64_t(capacity), nullptr, 0);
+    else
+      range_refine_kernel<true, false><<<grid, block, 0, s>>>(st, T, uint64_t(ntasks), B, uint32_t(nqueries), qbase,
+                                                                qcount, P, uint64_t(capacity), nullptr, 0);
   } else {
     GAT_REQUIRE(bitmap && traj_stride > 0, "range refine: null bitmap");
-    range_refine_kernel<false><<<dim3(uint32_t(blocks)), dim3(kBlock), 0, s>>>(
-        reinterpret_cast<const float2*>(xy), reinterpret_cast<const int2*>(tp), uint64_t(npoints),
-        reinterpret_cast<const Task*>(tasks), uint64_t(ntasks), reinterpret_cast<const float4*>(boxes),
-        uint32_t(nqueries), nullptr, nullptr, nullptr, 0, bitmap, uint64_t(traj_stride));
+    if (packed)
+      range_refine_kernel<false, true><<<grid, block, 0, s>>>(st, T, uint64_t(ntasks), B, uint32_t(nqueries), nullptr,
+                                                                nullptr, nullptr, 0, bitmap, uint64_t(traj_stride));
+    else
+      range_refine_kernel<false, false><<<grid, block, 0, s>>>(st, T, uint64_t(ntasks), B, uint32_t(nqueries), nullptr,
+                                                                 nullptr, nullptr, 0, bitmap, uint64_t(traj_stride));
   }
   GAT_LAUNCH_CHECK("range_refine_kernel");
   return GAT_OK;
 }
 
 // Shared body of the two reference-signature handlers.
-int bitmap_handler(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks, int ntasks,
-                   const float* boxes, uint8_t* result_host, int maxTrajNum, int maxQueryNum, void* stream) {
+int bitmap_handler(const gat_point_store* store, const gat_range_task* tasks, int ntasks, const float* boxes,
+                   uint8_t* result_host, int maxTrajNum, int maxQueryNum, void* stream) {
   GAT_REQUIRE(result_host, "range handler: null result buffer");
   GAT_REQUIRE(maxTrajNum > 0 && maxQueryNum >= 0 && ntasks >= 0, "range handler: bad sizes");
   const size_t bytes = size_t(maxTrajNum) * size_t(maxQueryNum);
@@ -141,8 +183,8 @@ int bitmap_handler(const float* xy, const int32_t* tp, int64_t npoints, const ga
   hipError_t e = hipMemsetAsync(dev, 0, bytes, s);
   if (e != hipSuccess) rc = gat::hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__);
   if (rc == GAT_OK)
-    rc = launch_refine(false, xy, tp, npoints, tasks, ntasks, boxes, maxQueryNum, nullptr, nullptr, nullptr, 0, dev,
-                       maxTrajNum, stream);
+    rc = launch_refine(false, store, tasks, ntasks, boxes, maxQueryNum, nullptr, nullptr, nullptr, 0, dev, maxTrajNum,
+                       stream);
   if (rc == GAT_OK) {
     e = hipMemcpyAsync(result_host, dev, bytes, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) rc = gat::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
@@ -191,6 +233,31 @@ __global__ __launch_bounds__(kBlock) void pairs_digest_kernel(const int2* __rest
 
 extern "C" {
 
+int gat_range_refine_bitmap(const gat_point_store* store, const gat_range_task* tasks, int64_t ntasks,
+                            const float* boxes, int64_t nqueries, uint8_t* bitmap, int64_t traj_stride, void* stream) {
+  return launch_refine(false, store, tasks, ntasks, boxes, nqueries, nullptr, nullptr, nullptr, 0, bitmap, traj_stride,
+                       stream);
+}
+
+int gat_range_refine_pairs(const gat_point_store* store, const gat_range_task* tasks, int64_t ntasks,
+                           const float* boxes, int64_t nqueries, const uint64_t* qbase, uint32_t* qcount,
+                           int32_t* pairs, int64_t capacity, void* stream) {
+  return launch_refine(true, store, tasks, ntasks, boxes, nqueries, qbase, qcount, pairs, capacity, nullptr, 0,
+                       stream);
+}
+
+int gatRangeQueryTestHandler(const gat_point_store* store, const gat_range_task* tasks, int stateTableLength,
+                             const float* boxes, uint8_t* result_host, int maxTrajNum, int maxQueryNum,
+                             void* stream) {
+  return bitmap_handler(store, tasks, stateTableLength, boxes, result_host, maxTrajNum, maxQueryNum, stream);
+}
+
+int gatRangeQuerySTIGHandler(const gat_point_store* store, const gat_range_task* tasks, int stateTableLength,
+                             const float* boxes, uint8_t* result_host, int maxTrajNum, int maxQueryNum,
+                             void* stream) {
+  return bitmap_handler(store, tasks, stateTableLength, boxes, result_host, maxTrajNum, maxQueryNum, stream);
+}
+
 int gat_pairs_digest(const int32_t* pairs, const uint64_t* qbase, const uint32_t* qcount, int64_t nq,
                      int64_t pairs_capacity, uint64_t* digest, void* stream) {
   GAT_REQUIRE(nq >= 0 && nq <= int64_t(UINT32_MAX), "pairs digest: bad query count");
@@ -201,32 +268,6 @@ int gat_pairs_digest(const int32_t* pairs, const uint64_t* qbase, const uint32_t
       reinterpret_cast<const int2*>(pairs), qbase, qcount, uint32_t(nq), uint64_t(pairs_capacity), digest);
   GAT_LAUNCH_CHECK("pairs_digest_kernel");
   return GAT_OK;
-}
-
-int gat_range_refine_bitmap(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
-                            int64_t ntasks, const float* boxes, int64_t nqueries, uint8_t* bitmap, int64_t traj_stride,
-                            void* stream) {
-  return launch_refine(false, xy, tp, npoints, tasks, ntasks, boxes, nqueries, nullptr, nullptr, nullptr, 0, bitmap,
-                       traj_stride, stream);
-}
-
-int gat_range_refine_pairs(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
-                           int64_t ntasks, const float* boxes, int64_t nqueries, const uint64_t* qbase,
-                           uint32_t* qcount, int32_t* pairs, int64_t capacity, void* stream) {
-  return launch_refine(true, xy, tp, npoints, tasks, ntasks, boxes, nqueries, qbase, qcount, pairs, capacity, nullptr,
-                       0, stream);
-}
-
-int gatRangeQueryTestHandler(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
-                             int stateTableLength, const float* boxes, uint8_t* result_host, int maxTrajNum,
-                             int maxQueryNum, void* stream) {
-  return bitmap_handler(xy, tp, npoints, tasks, stateTableLength, boxes, result_host, maxTrajNum, maxQueryNum, stream);
-}
-
-int gatRangeQuerySTIGHandler(const float* xy, const int32_t* tp, int64_t npoints, const gat_range_task* tasks,
-                             int stateTableLength, const float* boxes, uint8_t* result_host, int maxTrajNum,
-                             int maxQueryNum, void* stream) {
-  return bitmap_handler(xy, tp, npoints, tasks, stateTableLength, boxes, result_host, maxTrajNum, maxQueryNum, stream);
 }
 
 }  // extern "C"

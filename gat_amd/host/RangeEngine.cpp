@@ -68,13 +68,49 @@ void RangeEngine::select(int nq, const Selector& selector, int threads, std::vec
   }
 }
 
+namespace {
+int bit_length(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
+}  // namespace
+
 int RangeEngine::upload(const float* xy, const int32_t* tp, int64_t n) {
   if (int rc = stream_.create()) return rc;
   n_ = n;
-  if (int rc = xy_.reserve(size_t(std::max<int64_t>(n, 1)) * 8)) return rc;
-  if (int rc = tp_.reserve(size_t(std::max<int64_t>(n, 1)) * 8)) return rc;
+  uint32_t max_tid = 0, max_pid = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (tp[2 * i] < 0 || tp[2 * i + 1] < 0) {
+      gath_set_error("store upload: negative id at point %lld", (long long)i);
+      return GAT_ERR_ARG;
+    }
+    max_tid = std::max(max_tid, uint32_t(tp[2 * i]));
+    max_pid = std::max(max_pid, uint32_t(tp[2 * i + 1]));
+  }
+  const int pid_bits = std::max(bit_length(max_pid), 1);
+  const bool packed = bit_length(max_tid) + pid_bits <= 32;
+  if (!packed && max_pid > 0xFFFFu) {
+    gath_set_error("store upload: point ids up to %u do not fit 16 bits", max_pid);
+    return GAT_ERR_ARG;
+  }
+  const size_t cnt = size_t(std::max<int64_t>(n, 1));
+  if (int rc = xy_.reserve(cnt * 8)) return rc;
+  if (int rc = id_.reserve(cnt * 4)) return rc;
+  std::vector<uint32_t> id(static_cast<size_t>(n));
+  std::vector<uint16_t> pid;
+  if (packed) {
+    pid_.release();
+    for (int64_t i = 0; i < n; ++i) id[i] = (uint32_t(tp[2 * i]) << pid_bits) | uint32_t(tp[2 * i + 1]);
+  } else {
+    if (int rc = pid_.reserve(cnt * 2)) return rc;
+    pid.resize(size_t(n));
+    for (int64_t i = 0; i < n; ++i) {
+      id[i] = uint32_t(tp[2 * i]);
+      pid[i] = uint16_t(tp[2 * i + 1]);
+    }
+  }
   GATH_TRY(gat_memcpy_h2d(xy_.get(), xy, size_t(n) * 8, stream_.get()), "store upload");
-  GATH_TRY(gat_memcpy_h2d(tp_.get(), tp, size_t(n) * 8, stream_.get()), "store upload");
+  GATH_TRY(gat_memcpy_h2d(id_.get(), id.data(), size_t(n) * 4, stream_.get()), "store upload");
+  if (!packed) GATH_TRY(gat_memcpy_h2d(pid_.get(), pid.data(), size_t(n) * 2, stream_.get()), "store upload");
+  store_ = gat_point_store{xy_.as<float>(), id_.as<uint32_t>(), packed ? nullptr : pid_.as<uint16_t>(), n,
+                           packed ? pid_bits : 0};
   return stream_.sync();
 }
 
@@ -130,9 +166,9 @@ int RangeEngine::runPairs(const MBB* bounds, int nq, const Selector& selector, i
   double t2 = now_ms();
   out.uploadMs = t2 - t1;
   if (int rc = timer_.start(stream_.get())) return rc;
-  GATH_TRY(gat_range_refine_pairs(xy_.as<float>(), tp_.as<int32_t>(), n_, devTasks_.as<gat_range_task>(),
-                                  int64_t(tasks.size()), devBoxes_.as<float>(), nq, out.qbase.as<uint64_t>(),
-                                  out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), int64_t(total), stream_.get()),
+  GATH_TRY(gat_range_refine_pairs(&store_, devTasks_.as<gat_range_task>(), int64_t(tasks.size()),
+                                  devBoxes_.as<float>(), nq, out.qbase.as<uint64_t>(), out.qcount.as<uint32_t>(),
+                                  out.pairs.as<int32_t>(), int64_t(total), stream_.get()),
            "range refine");
   if (int rc = timer_.stop(stream_.get())) return rc;
   out.count.assign(size_t(nq), 0);
@@ -159,8 +195,8 @@ int RangeEngine::runBitmap(const MBB* bounds, int nq, const Selector& selector, 
   select(nq, selector, threads, tasks, cand);
   if (int rc = stage(bounds, nq, tasks, nullptr)) return rc;
   auto handler = stigHandler ? gatRangeQuerySTIGHandler : gatRangeQueryTestHandler;
-  GATH_TRY(handler(xy_.as<float>(), tp_.as<int32_t>(), n_, devTasks_.as<gat_range_task>(), int(tasks.size()),
-                   devBoxes_.as<float>(), resultHost, maxTrajNum, nq, stream_.get()),
+  GATH_TRY(handler(&store_, devTasks_.as<gat_range_task>(), int(tasks.size()), devBoxes_.as<float>(), resultHost,
+                   maxTrajNum, nq, stream_.get()),
            "range handler");
   return GAT_OK;
 }

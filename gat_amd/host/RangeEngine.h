@@ -66,11 +66,11 @@ class RangeEngine {
   RangeEngine(const RangeEngine&) = delete;
   RangeEngine& operator=(const RangeEngine&) = delete;
 
-  // Uploads the index-ordered store.
+  // Uploads the index-ordered store: xy holds (lon, lat) pairs, tp (tid, pid)
+  // pairs; ids are packed into 32 bits when they fit (gat_point_store).
   int upload(const float* xy, const int32_t* tp, int64_t n);
   int64_t pointNum() const { return n_; }
-  const float* xyDev() const { return xy_.as<float>(); }
-  const int32_t* tpDev() const { return tp_.as<int32_t>(); }
+  const gat_point_store& store() const { return store_; }
   void* stream() { return stream_.get(); }
   int ensureStream() { return stream_.create(); }
 
@@ -91,7 +91,8 @@ class RangeEngine {
  private:
   int stage(const MBB* bounds, int nq, const std::vector<gat_range_task>& tasks, const uint64_t* qbase);
 
-  DeviceBuffer xy_, tp_;
+  DeviceBuffer xy_, id_, pid_;
+  gat_point_store store_{};
   int64_t n_ = 0;
   Stream stream_;
   PinnedBuffer hostStage_;

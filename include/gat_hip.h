@@ -10,10 +10,10 @@
  * never aborts or throws; gat_last_error() returns the calling thread's last
  * error message.  Device pointers are marked `_dev`; host pointers `_host`.
  *
- * Point stores are structure-of-arrays in HBM: `xy_dev` holds (lon, lat) float
- * pairs and `tp_dev` holds (trajectory id, point id) int32 pairs, both in the
- * index's own point order.  Trajectory ids are 1-based (Grid.cpp:215-221);
- * a point id is the point's 0-based position within its trajectory.
+ * Point stores are structure-of-arrays in HBM (gat_point_store): (lon, lat)
+ * float pairs and compact (trajectory id, point id) arrays, in the index's own
+ * point order.  Trajectory ids are 1-based (Grid.cpp:215-221); a point id is
+ * the point's 0-based position within its trajectory.
  */
 #ifndef GAT_HIP_H
 #define GAT_HIP_H
@@ -68,7 +68,26 @@ int gat_event_record(void* event, void* stream);
 int gat_event_sync(void* event);
 int gat_event_elapsed_ms(float* ms, void* start, void* stop);
 
+/* Streaming copy of `bytes` (a multiple of 16) with a plain grid-stride float4
+ * kernel: the stream-copy HBM bandwidth reference, and the known-size kernel
+ * whose rocprofv3 FETCH_SIZE / WRITE_SIZE readings calibrate those of the other
+ * kernels.  No reference counterpart.  Asynchronous on `stream`. */
+int gat_stream_copy(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
+
 /* ------------------------------------------------------------------ range */
+/* A resident point store: the index's points in its own order.  Ids are
+ * compact: with pid_bits > 0, id_dev[i] = (tid << pid_bits) | pid and pid_dev
+ * is unused; with pid_bits == 0, id_dev[i] = tid and pid_dev[i] = pid.
+ * Replaces the reference's AoS SPoint runs (ConstDefine.h:71-75), which carry
+ * x, y and tid for every point and no point id. */
+typedef struct gat_point_store {
+  const float* xy_dev;     /* npoints (lon, lat) float pairs */
+  const uint32_t* id_dev;  /* npoints ids, see above */
+  const uint16_t* pid_dev; /* npoints point ids when pid_bits == 0 */
+  int64_t npoints;
+  int32_t pid_bits;
+} gat_point_store;
+
 /* One unit of refine work: `count` consecutive store points starting at
  * `start`, tested against rectangle `qid`.  Replaces RangeQueryStateTable
  * (ConstDefine.h:92-101): the rectangle lives once per query in `boxes_dev`
@@ -88,12 +107,12 @@ typedef struct gat_range_task {
  * kernels test strictly, kernel.cu:1055).  result_host (maxQueryNum * maxTrajNum
  * bytes) is fully overwritten with 0/1.  Synchronous on `stream`, as the
  * reference (which synchronises the device, kernel.cu:1300-1306). */
-int gatRangeQueryTestHandler(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
-                             const gat_range_task* tasks_dev, int stateTableLength, const float* boxes_dev,
-                             uint8_t* result_host, int maxTrajNum, int maxQueryNum, void* stream);
-int gatRangeQuerySTIGHandler(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
-                             const gat_range_task* tasks_dev, int stateTableLength, const float* boxes_dev,
-                             uint8_t* result_host, int maxTrajNum, int maxQueryNum, void* stream);
+int gatRangeQueryTestHandler(const gat_point_store* store, const gat_range_task* tasks_dev, int stateTableLength,
+                             const float* boxes_dev, uint8_t* result_host, int maxTrajNum, int maxQueryNum,
+                             void* stream);
+int gatRangeQuerySTIGHandler(const gat_point_store* store, const gat_range_task* tasks_dev, int stateTableLength,
+                             const float* boxes_dev, uint8_t* result_host, int maxTrajNum, int maxQueryNum,
+                             void* stream);
 
 /* The same refine, with the trajectory bitmap left in device memory
  * (bitmap_dev: rows of `traj_stride` bytes, one row per query, zeroed by the
@@ -102,9 +121,9 @@ int gatRangeQuerySTIGHandler(const float* xy_dev, const int32_t* tp_dev, int64_t
  * run leaves the store's `npoints` points are skipped, and trajectory ids at or
  * beyond the bitmap stride are ignored: malformed input cannot make the kernel
  * read or write out of bounds. */
-int gat_range_refine_bitmap(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
-                            const gat_range_task* tasks_dev, int64_t ntasks, const float* boxes_dev,
-                            int64_t nqueries, uint8_t* bitmap_dev, int64_t traj_stride, void* stream);
+int gat_range_refine_bitmap(const gat_point_store* store, const gat_range_task* tasks_dev, int64_t ntasks,
+                            const float* boxes_dev, int64_t nqueries, uint8_t* bitmap_dev, int64_t traj_stride,
+                            void* stream);
 
 /* The same refine, emitting compacted (tid, pid) int32 pairs.  Query q's pairs
  * are written contiguously from pairs_dev[2 * qbase_dev[q]]; qcount_dev[q] (zeroed
@@ -113,10 +132,9 @@ int gat_range_refine_bitmap(const float* xy_dev, const int32_t* tp_dev, int64_t 
  * is unspecified.  Pairs that would land at or beyond `pairs_capacity` (in
  * pairs) are dropped while qcount_dev still counts them, so an undersized
  * region shows as qcount > region size.  Asynchronous on `stream`. */
-int gat_range_refine_pairs(const float* xy_dev, const int32_t* tp_dev, int64_t npoints,
-                           const gat_range_task* tasks_dev, int64_t ntasks, const float* boxes_dev,
-                           int64_t nqueries, const uint64_t* qbase_dev, uint32_t* qcount_dev,
-                           int32_t* pairs_dev, int64_t pairs_capacity, void* stream);
+int gat_range_refine_pairs(const gat_point_store* store, const gat_range_task* tasks_dev, int64_t ntasks,
+                           const float* boxes_dev, int64_t nqueries, const uint64_t* qbase_dev,
+                           uint32_t* qcount_dev, int32_t* pairs_dev, int64_t pairs_capacity, void* stream);
 
 /* Per-query order-independent digest of a compacted-pair result: digest_dev[q]
  * = wrapping sum over query q's pairs of a 64-bit mix (the splitmix64

@@ -3,6 +3,8 @@
 Bar: bit-exact.  The GT-quadtree answer must equal the oracle's restatement of
 Grid::rangeQueryBatch (Grid.cpp:341-405) point for point; FSG and STIG must
 equal inclusive brute-force containment (their reference has no CPU path)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -126,32 +128,41 @@ def test_repeated_batches_reuse_buffers(gpu, cfg1):
         assert (res.counts() == c).all() and (res.digests() == d).all()
 
 
-def test_refine_abi_skips_malformed_tasks(gpu):
+@pytest.mark.parametrize("packed", [True, False])
+def test_refine_abi_skips_malformed_tasks(gpu, packed):
     """The C ABI on its own: hand-built tasks, including ones that point past
-    the store or at a missing query, which the kernel must skip."""
+    the store or at a missing query, which the kernel must skip; both id
+    layouts of gat_point_store."""
     h = N.hip()
     xy = np.array([[0.1, 0.1], [0.2, 0.2], [0.3, 0.3], [0.9, 0.9]], np.float32)
-    tp = np.array([[1, 0], [1, 1], [2, 0], [3, 0]], np.int32)
+    tid = np.array([1, 1, 2, 3], np.uint32)
+    pid = np.array([0, 1, 0, 0], np.uint16)
     boxes = np.array([[0, 0, 0.25, 0.25], [0.25, 0.25, 1, 1]], np.float32)
     tasks = np.array([[0, 4, 0], [0, 4, 1], [2, 5, 0], [0, 2, 7]], np.uint32)  # last two malformed
-    dxy, dtp, db_, dt = DevBuf.of(xy), DevBuf.of(tp), DevBuf.of(boxes), DevBuf.of(tasks)
+    dxy, db_, dt = DevBuf.of(xy), DevBuf.of(boxes), DevBuf.of(tasks)
+    if packed:
+        did, dpid = DevBuf.of((tid << 3) | pid.astype(np.uint32)), None
+        store = N.PointStore(dxy.ptr, did.ptr, None, 4, 3)
+    else:
+        did, dpid = DevBuf.of(tid), DevBuf.of(pid)
+        store = N.PointStore(dxy.ptr, did.ptr, dpid.ptr, 4, 0)
+    sp = ctypes.byref(store)
     qbase = DevBuf.of(np.array([0, 4], np.uint64))
     qcount = DevBuf(8).zero()
     pairs = DevBuf(8 * 8).zero()
-    N.check_hip(h.gat_range_refine_pairs(dxy.ptr, dtp.ptr, 4, dt.ptr, 4, db_.ptr, 2, qbase.ptr, qcount.ptr, pairs.ptr,
-                                         8, None))
+    N.check_hip(h.gat_range_refine_pairs(sp, dt.ptr, 4, db_.ptr, 2, qbase.ptr, qcount.ptr, pairs.ptr, 8, None))
     cnt = qcount.get(np.uint32, 2)
     out = pairs.get(np.int32, 16).reshape(8, 2)
     assert cnt.tolist() == [2, 2]
     assert set(map(tuple, out[:2].tolist())) == {(1, 0), (1, 1)}
     assert set(map(tuple, out[4:6].tolist())) == {(2, 0), (3, 0)}
     bitmap = DevBuf(2 * 4).zero()
-    N.check_hip(h.gat_range_refine_bitmap(dxy.ptr, dtp.ptr, 4, dt.ptr, 4, db_.ptr, 2, bitmap.ptr, 4, None))
+    N.check_hip(h.gat_range_refine_bitmap(sp, dt.ptr, 4, db_.ptr, 2, bitmap.ptr, 4, None))
     assert bitmap.get(np.uint8, 8).reshape(2, 4).tolist() == [[0, 1, 0, 0], [0, 0, 1, 1]]
     # The reference-signature handler: synchronous, host result.
     host = np.full(8, 7, np.uint8)
-    N.check_hip(h.gatRangeQueryTestHandler(dxy.ptr, dtp.ptr, 4, dt.ptr, 4, db_.ptr, api._ptr(host), 4, 2, None))
+    N.check_hip(h.gatRangeQueryTestHandler(sp, dt.ptr, 4, db_.ptr, api._ptr(host), 4, 2, None))
     assert host.reshape(2, 4).tolist() == [[0, 1, 0, 0], [0, 0, 1, 1]]
     # Argument errors come back as a status, not a crash.
-    assert h.gat_range_refine_pairs(dxy.ptr, dtp.ptr, 4, None, 4, db_.ptr, 2, qbase.ptr, qcount.ptr, pairs.ptr, 8,
-                                    None) == 2
+    assert h.gat_range_refine_pairs(sp, None, 4, db_.ptr, 2, qbase.ptr, qcount.ptr, pairs.ptr, 8, None) == 2
+    assert h.gat_range_refine_pairs(None, dt.ptr, 4, db_.ptr, 2, qbase.ptr, qcount.ptr, pairs.ptr, 8, None) == 2
