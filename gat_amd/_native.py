@@ -72,6 +72,9 @@ HIP_FUNCS = {
     "gat_range_refine_pairs": (INT, [P, P, I64, P, I64, P, P, P, I64, P]),
     "gat_pairs_digest": (INT, [P, P, P, I64, I64, P, P]),
     "gatEDRDistanceBatchHandler": (INT, [INT, P, P, P, P, P, P, P, F64, P, P]),
+    "gat_edr_batched": (INT, [P, I64, P, I64, P, P, P, P, I32, P, P, P, I32, I32, F64, P, P]),
+    "gat_edr_filter": (INT, [P, I32, P, I32, F64, P, I64, P, P]),
+    "gat_edr_mismatch_table": (INT, [I32, P, P]),
 }
 
 HOST_FUNCS = {
@@ -101,6 +104,9 @@ HOST_FUNCS = {
     "gath_index_last_digests": (INT, [P, P]),
     "gath_grid_edr_all": (INT, [P, P, P, INT, F64, P]),
     "gath_grid_similarity": (INT, [P, P, INT, P, P, INT, F64, P, P]),
+    "gath_grid_edr_batched_all": (INT, [P, P, INT, P, P, F64, P, P, P]),
+    "gath_grid_similarity_batched": (INT, [P, P, INT, P, P, INT, F64, P, P, P, P]),
+    "gath_grid_mismatch_table": (INT, [P, P]),
 }
 
 _hip = None

@@ -265,6 +265,45 @@ class GridIndex(_Index):
         return ids, dists
 
 
+    def edr_batched_all(self, queries: TrajSet, eps: float) -> tuple[np.ndarray, dict]:
+        """Distances of every (query, trajectory) pair, (nq, ntraj) int32, from
+        one persistent batched launch."""
+        dist = np.empty((queries.ntraj, self.ntraj), np.int32)
+        stats = np.zeros(5, np.int64)
+        ms = np.zeros(3, np.float64)
+        N.check_host(N.host().gath_grid_edr_batched_all(
+            self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj, _ptr(_req(queries.lon, np.float32)),
+            _ptr(_req(queries.lat, np.float32)), eps, _ptr(dist), _ptr(stats), _ptr(ms)))
+        return dist, _edr_stats(stats, ms)
+
+    def similarity_batched(self, queries: TrajSet, k: int, eps: float) -> tuple[np.ndarray, np.ndarray, dict]:
+        """Exact top-k (ascending distance, then id) of every query: the candidate
+        filter resolves most pairs through the all-mismatch table, the rest run
+        the batched DP."""
+        ids = np.empty((queries.ntraj, k), np.int32)
+        dists = np.empty((queries.ntraj, k), np.int32)
+        stats = np.zeros(5, np.int64)
+        ms = np.zeros(3, np.float64)
+        N.check_host(N.host().gath_grid_similarity_batched(
+            self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj, _ptr(_req(queries.lon, np.float32)),
+            _ptr(_req(queries.lat, np.float32)), k, eps, _ptr(ids), _ptr(dists), _ptr(stats), _ptr(ms)))
+        return ids, dists, _edr_stats(stats, ms)
+
+    def mismatch_table(self) -> np.ndarray:
+        w = MAX_TRAJ_LEN + 1
+        out = np.empty((w, w), np.int32)
+        N.check_host(N.host().gath_grid_mismatch_table(self.handle, _ptr(out)))
+        return out
+
+
+MAX_TRAJ_LEN = 2048
+
+
+def _edr_stats(stats, ms) -> dict:
+    return {"pairs": int(stats[0]), "cells": int(stats[1]), "jobs": int(stats[2]), "pruned": int(stats[3]),
+            "kernel_ms": float(ms[0]), "filter_ms": float(ms[1]), "host_ms": float(ms[2])}
+
+
 class FsgIndex(_Index):
     """FSG (FSG.cpp): flat row-major grid."""
 

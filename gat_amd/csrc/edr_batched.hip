@@ -1,0 +1,306 @@
+// Batched EDR for gfx950: many (query, candidate) pairs in one persistent
+// launch.  Same recurrence as Grid::SimilarityExecuter (Grid.cpp:1222-1318);
+// no reference counterpart for the batching itself.
+//
+// Wave-systolic layout.  A wave64 runs four independent pairs, one per 16-lane
+// DPP row ("segment").  Inside a segment, lane l owns DP row r = 16*s + l + 1 of
+// strip s and keeps that row's point in registers; at step t it computes column
+// c = t - l + 1, so a strip sweeps its 16 rows across all columns in m + 15
+// steps.  The three neighbours of a cell come from registers:
+//   U = D[r-1][c]   the left lane's value of the previous step (DPP row_shr:1)
+//   L = D[r][c-1]   the lane's own value of the previous step
+//   M = D[r-1][c-1] the left lane's value two steps back, i.e. the U the lane
+//                   received one step earlier
+// and the cell rule of Grid.cpp:1297-1300 reduces exactly to
+//   D = (M == L) ? U : min(M, L, U)
+// (M strictly smallest -> M; L strictly smallest -> L; every tie for the
+// minimum, and M == L < U in particular, -> U).
+// The first lane of a segment takes U from the row above the strip, kept in
+// LDS as 16-bit values and fed 16 steps at a time through a register that
+// shifts one lane per step (row_shl:1); the strip's last lane hands its row to
+// the next strip the same way in reverse.  Column points are read straight from
+// the NaN-padded trajectory store (64 consecutive points per wave access, L1
+// resident across the 16 lanes that reuse them); a NaN point never matches, so
+// lanes before their first column keep D[r][0] = r untouched and lanes past
+// the last column compute values nobody reads.  The cell D[n][m] is captured
+// by the lane that owns row n at the step it reaches column m.
+//
+// Work: jobs of up to four pairs with the same strip count, ordered longest
+// first, handed out by one atomic counter per launch, so the batch's tail is
+// short and no host round trip separates length buckets.
+#include "gat_common.hpp"
+
+namespace {
+
+constexpr int kSeg = 16;   // lanes per segment (one DPP row)
+constexpr int kSegs = 4;   // pairs per wave
+constexpr int kBlk = 16;   // steps per feed block
+constexpr int kPad = 16;   // NaN points before every trajectory of a padded store
+
+struct Job {
+  int32_t first, count;
+};
+static_assert(sizeof(Job) == sizeof(gat_edr_job), "job layout");
+
+__device__ __forceinline__ int row_shr1(int old, int src) {
+  return __builtin_amdgcn_update_dpp(old, src, 0x111, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int row_shl1(int old, int src) {
+  return __builtin_amdgcn_update_dpp(old, src, 0x101, 0xf, 0xf, false);
+}
+
+template <int MAXM>
+__global__ __launch_bounds__(64) void edr_wave_kernel(const int2* __restrict__ pairs, int64_t npairs,
+                                                      const Job* __restrict__ jobs, int32_t njobs,
+                                                      int32_t* __restrict__ counter, int32_t nqueries, int32_t ncands,
+                                                      const float2* __restrict__ qxy, const int64_t* __restrict__ qoff,
+                                                      const int32_t* __restrict__ qlen, const float2* __restrict__ cxy,
+                                                      const int64_t* __restrict__ coff,
+                                                      const int32_t* __restrict__ clen, float u,
+                                                      int32_t* __restrict__ out) {
+  // Row above the current strip, per segment, at index c + kBlk (columns -15..).
+  __shared__ int16_t above[kSegs][MAXM + 3 * kBlk];
+  const int lane = threadIdx.x;
+  const int seg = lane >> 4;
+  const int l = lane & (kSeg - 1);
+  const float nanf_ = __builtin_nanf("");
+  for (;;) {
+    int ji = 0;
+    if (lane == 0) ji = atomicAdd(counter, 1);
+    ji = __builtin_amdgcn_readfirstlane(ji);
+    if (ji >= njobs) break;
+    const Job job = jobs[ji];
+    // Jobs and pairs that point outside their arrays are skipped.
+    const bool live = seg < job.count && job.first >= 0 && int64_t(job.first) + seg < npairs;
+    int n = 0, m = 0;
+    // Dead segments read rows nowhere (n = 0) and columns from the store's
+    // first trajectory, whose leading padding covers the lanes' look-behind.
+    const float2* R = cxy;
+    const float2* C = cxy + kPad;
+    bool valid = false;
+    if (live) {
+      const int2 pr = pairs[job.first + seg];
+      const bool ids_ok = pr.x >= 0 && pr.x < nqueries && pr.y >= 0 && pr.y < ncands;
+      const int nq = ids_ok ? qlen[pr.x] : 0, nc = ids_ok ? clen[pr.y] : 0;
+      // Lengths outside 1..MAXM (and bad ids) are reported as -1 and never indexed.
+      valid = nq >= 1 && nc >= 1 && nq <= MAXM && nc <= MAXM;
+      if (valid) {
+        // Rows are the shorter trajectory; on equal lengths the query
+        // (Grid.cpp:1234-1247).
+        if (nc >= nq) {
+          R = qxy + qoff[pr.x]; n = nq; C = cxy + coff[pr.y]; m = nc;
+        } else {
+          R = cxy + coff[pr.y]; n = nc; C = qxy + qoff[pr.x]; m = nq;
+        }
+      }
+    }
+    // Wave-uniform extents: the largest column count and strip count of the job.
+    const int m0 = __builtin_amdgcn_readlane(m, 0), m1 = __builtin_amdgcn_readlane(m, 16);
+    const int m2 = __builtin_amdgcn_readlane(m, 32), m3 = __builtin_amdgcn_readlane(m, 48);
+    const int mmax = max(max(m0, m1), max(m2, m3));
+    const int n0 = __builtin_amdgcn_readlane(n, 0), n1 = __builtin_amdgcn_readlane(n, 16);
+    const int n2 = __builtin_amdgcn_readlane(n, 32), n3 = __builtin_amdgcn_readlane(n, 48);
+    const int nmax = max(max(n0, n1), max(n2, n3));
+    // Smallest column and row counts among the live segments: capture starts
+    // at the first strip that is some segment's last and the first column
+    // that is some segment's last.
+    int mmin = mmax, nmin = nmax;
+    if (n0 > 0) { mmin = min(mmin, m0); nmin = min(nmin, n0); }
+    if (n1 > 0) { mmin = min(mmin, m1); nmin = min(nmin, n1); }
+    if (n2 > 0) { mmin = min(mmin, m2); nmin = min(nmin, n2); }
+    if (n3 > 0) { mmin = min(mmin, m3); nmin = min(nmin, n3); }
+    const int first_last_strip = (nmin + kSeg - 1) / kSeg - 1;
+    const int strips = (nmax + kSeg - 1) / kSeg;
+    const int my_last = (n + kSeg - 1) / kSeg - 1;
+    const int steps = mmax + kSeg - 1;
+    // Row 0: D[0][c] = c.
+    for (int i = l; i < MAXM + 3 * kBlk; i += kSeg) above[seg][i] = int16_t(i - kBlk);
+    int res = 0;
+    for (int s = 0; s < strips; ++s) {
+      // Orders the previous strip's (or the initial) writes of `above` before
+      // this strip's reads of them by other lanes.
+      __syncthreads();
+      const int r = kSeg * s + l + 1;
+      const float2 rp = r <= n ? R[r - 1] : make_float2(nanf_, nanf_);
+      const float rx = rp.x, ry = rp.y;
+      int cur = r;  // D[r][0]
+      int u1p = r;  // D[r-1][0] + 1
+      int coll = 0;
+      const int tcap = (s == my_last) ? m + l - 1 : -1;  // step at which lane l holds D[r][m]
+      const bool last_strip_somewhere = s >= first_last_strip;
+      const float2* cp = C - l;  // column point of step t is cp[t]
+      float2 col[kBlk];
+#pragma unroll
+      for (int j = 0; j < kBlk; ++j) col[j] = cp[j];
+      for (int t0 = 0; t0 < steps; t0 += kBlk) {
+        // Feed: U + 1 for the segment's first lane at steps t0 .. t0 + 15.
+        int f1 = int(above[seg][kBlk + t0 + 1 + l]) + 1;
+        float2 nxt[kBlk];
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) nxt[j] = cp[t0 + kBlk + j];
+        const bool capture = last_strip_somewhere && t0 + kBlk - 1 >= mmin - 1;
+        if (capture) {
+#pragma unroll
+          for (int j = 0; j < kBlk; ++j) {
+            const bool match = fabsf(rx - col[j].x) < u && fabsf(ry - col[j].y) < u;
+            const int c1 = cur + 1;
+            const int u1 = row_shr1(f1, c1);
+            f1 = row_shl1(f1, f1);
+            const int mm = u1p - (match ? 1 : 0);
+            const int mn = min(min(u1, c1), mm);
+            cur = (mm == c1) ? u1 : mn;
+            u1p = u1;
+            coll = row_shl1(cur, coll);
+            res = (t0 + j == tcap) ? cur : res;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < kBlk; ++j) {
+            const bool match = fabsf(rx - col[j].x) < u && fabsf(ry - col[j].y) < u;
+            const int c1 = cur + 1;
+            const int u1 = row_shr1(f1, c1);
+            f1 = row_shl1(f1, f1);
+            const int mm = u1p - (match ? 1 : 0);
+            const int mn = min(min(u1, c1), mm);
+            cur = (mm == c1) ? u1 : mn;
+            u1p = u1;
+            coll = row_shl1(cur, coll);
+          }
+        }
+        // Lane 15 produced columns t0 - 14 .. t0 + 1 of row 16*s + 16; they are
+        // the row above the next strip.
+        above[seg][kBlk + t0 - (kSeg - 2) + l] = int16_t(coll);
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) col[j] = nxt[j];
+      }
+    }
+    __syncthreads();  // the next job re-initialises `above`
+    if (live && !valid && l == 0) out[job.first + seg] = -1;
+    if (valid && l == ((n - 1) & (kSeg - 1))) out[job.first + seg] = res;
+  }
+}
+
+// Candidate filter: a (query, candidate) pair can contain a matching point
+// pair only if neither box lies a float-difference of u or more beyond the
+// other on either axis.  Float subtraction is monotone, so for points a (query)
+// and b (candidate) with b.x >= cand.xmin and a.x <= query.xmax,
+// fl(b.x - a.x) >= fl(cand.xmin - query.xmax): when the latter is >= u no point
+// pair matches on x, and likewise for the other three sides.  Pairs the filter
+// drops have no match at all, so their distance is the all-mismatch value.
+__device__ __forceinline__ bool may_match(const float4& q, const float4& c, float u) {
+  return !((c.x - q.z) >= u || (q.x - c.z) >= u || (c.y - q.w) >= u || (q.y - c.w) >= u);
+}
+
+__global__ __launch_bounds__(256) void edr_filter_kernel(const float4* __restrict__ qbox, int32_t nq,
+                                                         const float4* __restrict__ cbox, int32_t nc, float u,
+                                                         int2* __restrict__ out, uint64_t capacity,
+                                                         unsigned long long* __restrict__ count) {
+  const int q = blockIdx.y;
+  if (q >= nq) return;
+  const float4 qb = qbox[q];
+  const int lane = threadIdx.x & 63;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c - lane < nc; c += gridDim.x * 256) {
+    const bool keep = c < nc && may_match(qb, cbox[c], u);
+    const uint64_t mask = __ballot(keep);
+    if (mask == 0) continue;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+    base = __builtin_amdgcn_readfirstlane(uint32_t(base)) | (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base >> 32))) << 32);
+    const uint64_t at = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+    if (keep && at < capacity) out[at] = make_int2(q, c);
+  }
+}
+
+// The all-mismatch table: out[r * (maxlen + 1) + c] = D[r][c] of the recurrence
+// with every substitution costing 1, for 0 <= r, c <= maxlen.  A pair of
+// lengths n <= m with no matching point has distance out[n][m].  One
+// workgroup sweeps the anti-diagonals.
+__global__ __launch_bounds__(1024) void edr_mismatch_table_kernel(int32_t maxlen, int32_t* __restrict__ out) {
+  __shared__ int32_t dg[3][GAT_MAX_TRAJ_LEN + 2];
+  const int w = maxlen + 1;
+  int32_t* d2 = dg[0];
+  int32_t* d1 = dg[1];
+  int32_t* d0 = dg[2];
+  for (int d = 0; d <= 2 * maxlen; ++d) {
+    const int rlo = d - maxlen > 0 ? d - maxlen : 0;
+    const int rhi = d < maxlen ? d : maxlen;
+    for (int r = rlo + int(threadIdx.x); r <= rhi; r += 1024) {
+      const int c = d - r;
+      int v;
+      if (r == 0) v = c;
+      else if (c == 0) v = r;
+      else {
+        const int mm = d2[r - 1] + 1, uu = d1[r - 1] + 1, ll = d1[r] + 1;
+        const bool c1 = mm < uu && mm < ll, c2 = ll < uu && ll < mm;  // Grid.cpp:1297-1300
+        v = c1 ? mm : (c2 ? ll : uu);
+      }
+      d0[r] = v;
+      out[int64_t(r) * w + c] = v;
+    }
+    __syncthreads();
+    int32_t* t = d2;
+    d2 = d1;
+    d1 = d0;
+    d0 = t;
+  }
+}
+
+}  // namespace
+
+extern "C" int gat_edr_filter(const float* qbox, int32_t nq, const float* cbox, int32_t nc, double eps,
+                              int32_t* pairs_out, int64_t capacity, uint64_t* count, void* stream) {
+  GAT_REQUIRE(nq >= 0 && nc >= 0 && capacity >= 0 && nq <= 65535, "EDR filter: bad sizes");
+  GAT_REQUIRE(count, "EDR filter: null count");
+  hipStream_t s = gat::as_stream(stream);
+  GAT_HIP_CHECK(hipMemsetAsync(count, 0, sizeof(uint64_t), s));
+  if (nq == 0 || nc == 0) return GAT_OK;
+  GAT_REQUIRE(qbox && cbox && (pairs_out || capacity == 0), "EDR filter: null pointer");
+  GAT_REQUIRE(eps > 0, "EDR filter: eps must be positive");
+  int64_t bx = (int64_t(nc) + 255) / 256;
+  if (bx > 64) bx = 64;
+  edr_filter_kernel<<<dim3(uint32_t(bx), uint32_t(nq)), dim3(256), 0, s>>>(
+      reinterpret_cast<const float4*>(qbox), nq, reinterpret_cast<const float4*>(cbox), nc, gat_eps_threshold(eps),
+      reinterpret_cast<int2*>(pairs_out), uint64_t(capacity), reinterpret_cast<unsigned long long*>(count));
+  GAT_LAUNCH_CHECK("edr_filter_kernel");
+  return GAT_OK;
+}
+
+extern "C" int gat_edr_mismatch_table(int32_t maxlen, int32_t* out, void* stream) {
+  GAT_REQUIRE(maxlen >= 0 && maxlen <= GAT_MAX_TRAJ_LEN, "mismatch table: maxlen outside 0..%d", GAT_MAX_TRAJ_LEN);
+  GAT_REQUIRE(out, "mismatch table: null output");
+  edr_mismatch_table_kernel<<<dim3(1), dim3(1024), 0, gat::as_stream(stream)>>>(maxlen, out);
+  GAT_LAUNCH_CHECK("edr_mismatch_table_kernel");
+  return GAT_OK;
+}
+
+extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_edr_job* jobs, int64_t njobs,
+                               int32_t* counter, const float* qxy, const int64_t* qoff, const int32_t* qlen,
+                               int32_t nqueries, const float* cxy, const int64_t* coff, const int32_t* clen,
+                               int32_t ncands, int32_t maxlen, double eps, int32_t* out, void* stream) {
+  GAT_REQUIRE(npairs >= 0 && njobs >= 0 && njobs <= INT32_MAX && nqueries >= 0 && ncands >= 0,
+              "EDR batched: bad sizes");
+  if (njobs == 0) return GAT_OK;
+  GAT_REQUIRE(pairs && jobs && counter && qxy && qoff && qlen && cxy && coff && clen && out,
+              "EDR batched: null pointer");
+  GAT_REQUIRE(maxlen >= 1 && maxlen <= GAT_MAX_TRAJ_LEN, "EDR batched: maxlen outside 1..%d", GAT_MAX_TRAJ_LEN);
+  GAT_REQUIRE(eps > 0, "EDR batched: eps must be positive");
+  hipStream_t s = gat::as_stream(stream);
+  GAT_HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(int32_t), s));
+  const float u = gat_eps_threshold(eps);
+  const auto* P = reinterpret_cast<const int2*>(pairs);
+  const auto* J = reinterpret_cast<const Job*>(jobs);
+  const auto* Q = reinterpret_cast<const float2*>(qxy);
+  const auto* Cs = reinterpret_cast<const float2*>(cxy);
+  // Persistent grid: as many single-wave workgroups as fit, capped by the work.
+  const int64_t per_cu = maxlen <= 512 ? 20 : 9;
+  int64_t blocks = int64_t(gat::cu_count()) * per_cu;
+  if (blocks > njobs) blocks = njobs;
+  if (maxlen <= 512)
+    edr_wave_kernel<512><<<dim3(uint32_t(blocks)), dim3(64), 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries,
+                                                                       ncands, Q, qoff, qlen, Cs, coff, clen, u, out);
+  else
+    edr_wave_kernel<GAT_MAX_TRAJ_LEN><<<dim3(uint32_t(blocks)), dim3(64), 0, s>>>(
+        P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff, qlen, Cs, coff, clen, u, out);
+  GAT_LAUNCH_CHECK("edr_wave_kernel");
+  return GAT_OK;
+}

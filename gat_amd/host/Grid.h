@@ -31,6 +31,17 @@ struct TopK {
   std::vector<int32_t> ids, dist;
 };
 
+// Counters of one batched EDR call.
+struct EdrBatchStats {
+  int64_t pairs = 0;       // pairs whose DP ran
+  int64_t cells = 0;       // DP cells of those pairs (sum of n * m)
+  int64_t jobs = 0;
+  int64_t pruned = 0;      // pairs resolved by the all-mismatch table
+  float kernelMs = 0;      // batched DP kernel (device events)
+  float filterMs = 0;      // candidate filter kernel (device events)
+  double hostMs = 0;       // host job building and top-k assembly
+};
+
 class Grid {
  public:
   Grid() = default;
@@ -71,6 +82,17 @@ class Grid {
   // EDR distances of one query against every trajectory (plain kernel).
   int edrAll(const float* qlon, const float* qlat, int qlen, int32_t* dist);
 
+  // Batched forms (no reference counterpart).  edrBatchedAll: the distance of
+  // every (query, trajectory) pair, dist[q * trajNum + t - 1], in one
+  // persistent launch.  similarityBatched: exact top-k by (distance, id) for
+  // every query: pairs the candidate filter drops take the all-mismatch table
+  // value, the rest run the batched DP.
+  int edrBatchedAll(const TrajectoryView& queries, std::vector<int32_t>& dist, EdrBatchStats* stats);
+  int similarityBatched(const TrajectoryView& queries, int kValue, std::vector<TopK>& out, EdrBatchStats* stats);
+  // The all-mismatch table F[r][c], (GAT_MAX_TRAJ_LEN + 1)^2 entries, computed
+  // on the device once.
+  const std::vector<int32_t>& mismatchTable();
+
   // Run-time EDR threshold (the reference's compile-time EPSILON, ConstDefine.h:28).
   double epsilon = 0.005;
   int hostThreads = 0;  // 0: hardware concurrency (capped at 16)
@@ -97,11 +119,18 @@ class Grid {
   int threads() const;
   int uploadTrajectories(const TrajectoryView& db);
   int ensureQueryStore(const TrajectoryView& queries);
+  int runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* stats);
 
   RangeEngine engine_;
   // Trajectory-ordered copy of the points for EDR (the reference reassembles
   // trajectories from the cell-ordered store per query, Grid.cpp:1500-1508).
-  DeviceBuffer trajXY_, trajOff_, trajLen_, allTasks_, dist_;
-  DeviceBuffer queryXY_, queryOff_, queryLen_;
-  std::vector<int32_t> trajLenHost_;
+  // Stores are NaN-padded (GAT_EDR_PAD points before every trajectory,
+  // GAT_EDR_TAIL_PAD after the last) for the batched kernel.
+  DeviceBuffer trajXY_, trajOff_, trajLen_, trajBox_, allTasks_, dist_;
+  DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
+  DeviceBuffer pairsDev_, jobsDev_, outDev_, counterDev_, ftableDev_;
+  EventTimer edrTimer_;
+  std::vector<int32_t> trajLenHost_, queryLenHost_;
+  std::vector<int32_t> ftable_;
+  std::vector<std::vector<int32_t>> lengthIds_;  // per length: trajectory ids, ascending
 };

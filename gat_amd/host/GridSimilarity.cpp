@@ -1,61 +1,109 @@
 // Similarity queries of the Grid (Grid.cpp:852-1755): EDR between query
 // trajectories and every indexed trajectory, then top-k.
 #include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
 #include <numeric>
 
 #include "Grid.h"
 
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// A NaN-padded, interleaved copy of a trajectory set: GAT_EDR_PAD NaN points
+// before every trajectory and GAT_EDR_TAIL_PAD after the last.  offsets[i] is
+// the first point of trajectory i + 1; boxes[4i..4i+3] its bounding box.
+struct Padded {
+  std::vector<float> xy;
+  std::vector<int64_t> offsets;
+  std::vector<int32_t> lengths;
+  std::vector<float> boxes;
+};
+
+Padded pad_store(const TrajectoryView& v) {
+  Padded p;
+  const float nan = std::numeric_limits<float>::quiet_NaN();
+  const int64_t total = v.pointNum() + int64_t(GAT_EDR_PAD) * v.trajNum + GAT_EDR_TAIL_PAD + GAT_EDR_PAD;
+  p.xy.assign(size_t(total) * 2, nan);
+  p.offsets.resize(size_t(v.trajNum));
+  p.lengths.resize(size_t(v.trajNum));
+  p.boxes.resize(size_t(v.trajNum) * 4);
+  int64_t at = GAT_EDR_PAD;
+  for (int t = 0; t < v.trajNum; ++t) {
+    const int64_t b = v.offsets[t], e = v.offsets[t + 1];
+    p.offsets[t] = at;
+    p.lengths[t] = int32_t(e - b);
+    float xl = INFINITY, yl = INFINITY, xh = -INFINITY, yh = -INFINITY;
+    for (int64_t i = b; i < e; ++i, ++at) {
+      p.xy[2 * at] = v.lon[i];
+      p.xy[2 * at + 1] = v.lat[i];
+      xl = std::min(xl, v.lon[i]);
+      xh = std::max(xh, v.lon[i]);
+      yl = std::min(yl, v.lat[i]);
+      yh = std::max(yh, v.lat[i]);
+    }
+    p.boxes[4 * t] = xl;
+    p.boxes[4 * t + 1] = yl;
+    p.boxes[4 * t + 2] = xh;
+    p.boxes[4 * t + 3] = yh;
+    at += GAT_EDR_PAD;
+  }
+  return p;
+}
+
+template <class T>
+int upload_vec(DeviceBuffer& d, const std::vector<T>& v, void* s, const char* what) {
+  if (int rc = d.reserve(std::max<size_t>(v.size() * sizeof(T), 16))) return rc;
+  GATH_TRY(gat_memcpy_h2d(d.get(), v.data(), v.size() * sizeof(T), s), what);
+  return GAT_OK;
+}
+
+int strips_of(int n) { return (n + 15) / 16; }
+
+}  // namespace
+
 int Grid::uploadTrajectories(const TrajectoryView& db) {
-  const int64_t N = db.pointNum();
   const int T = db.trajNum;
-  trajLenHost_.resize(size_t(T));
-  std::vector<float> xy(size_t(N) * 2);
-  for (int64_t i = 0; i < N; ++i) {
-    xy[2 * i] = db.lon[i];
-    xy[2 * i + 1] = db.lat[i];
-  }
+  Padded p = pad_store(db);
+  trajLenHost_ = p.lengths;
   std::vector<gat_edr_task> tasks(static_cast<size_t>(T));
-  for (int t = 0; t < T; ++t) {
-    trajLenHost_[t] = int32_t(db.offsets[t + 1] - db.offsets[t]);
-    tasks[t] = gat_edr_task{0, t + 1};
-  }
+  for (int t = 0; t < T; ++t) tasks[t] = gat_edr_task{0, t + 1};
   void* s = engine_.stream();
-  if (int rc = trajXY_.reserve(std::max<size_t>(xy.size() * 4, 8))) return rc;
-  if (int rc = trajOff_.reserve(std::max<size_t>(size_t(T) * 8, 8))) return rc;
-  if (int rc = trajLen_.reserve(std::max<size_t>(size_t(T) * 4, 4))) return rc;
-  if (int rc = allTasks_.reserve(std::max<size_t>(size_t(T) * 8, 8))) return rc;
+  if (int rc = upload_vec(trajXY_, p.xy, s, "trajectory upload")) return rc;
+  if (int rc = upload_vec(trajOff_, p.offsets, s, "trajectory upload")) return rc;
+  if (int rc = upload_vec(trajLen_, p.lengths, s, "trajectory upload")) return rc;
+  if (int rc = upload_vec(trajBox_, p.boxes, s, "trajectory upload")) return rc;
+  if (int rc = upload_vec(allTasks_, tasks, s, "trajectory upload")) return rc;
   if (int rc = dist_.reserve(std::max<size_t>(size_t(T) * 4, 4))) return rc;
-  GATH_TRY(gat_memcpy_h2d(trajXY_.get(), xy.data(), xy.size() * 4, s), "trajectory upload");
-  GATH_TRY(gat_memcpy_h2d(trajOff_.get(), db.offsets, size_t(T) * 8, s), "trajectory upload");
-  GATH_TRY(gat_memcpy_h2d(trajLen_.get(), trajLenHost_.data(), size_t(T) * 4, s), "trajectory upload");
-  GATH_TRY(gat_memcpy_h2d(allTasks_.get(), tasks.data(), size_t(T) * 8, s), "trajectory upload");
   GATH_TRY(gat_stream_sync(s), "trajectory upload");
+  lengthIds_.assign(GAT_MAX_TRAJ_LEN + 1, {});
+  for (int t = 0; t < T; ++t)
+    if (trajLenHost_[t] >= 1 && trajLenHost_[t] <= GAT_MAX_TRAJ_LEN) lengthIds_[trajLenHost_[t]].push_back(t + 1);
   return GAT_OK;
 }
 
 int Grid::ensureQueryStore(const TrajectoryView& q) {
-  const int64_t N = q.pointNum();
-  std::vector<float> xy(size_t(N) * 2);
-  for (int64_t i = 0; i < N; ++i) {
-    xy[2 * i] = q.lon[i];
-    xy[2 * i + 1] = q.lat[i];
-  }
-  std::vector<int32_t> off(static_cast<size_t>(q.trajNum)), len(size_t(q.trajNum));
   for (int i = 0; i < q.trajNum; ++i) {
-    off[i] = int32_t(q.offsets[i]);
-    len[i] = int32_t(q.offsets[i + 1] - q.offsets[i]);
-    if (len[i] < 1 || len[i] > GAT_MAX_TRAJ_LEN) {
-      gath_set_error("similarity: query %d has %d points (supported: 1..%d)", i, len[i], GAT_MAX_TRAJ_LEN);
+    const int64_t len = q.offsets[i + 1] - q.offsets[i];
+    if (len < 1 || len > GAT_MAX_TRAJ_LEN) {
+      gath_set_error("similarity: query %d has %lld points (supported: 1..%d)", i, (long long)len, GAT_MAX_TRAJ_LEN);
       return GAT_ERR_ARG;
     }
   }
+  Padded p = pad_store(q);
+  queryLenHost_ = p.lengths;
+  std::vector<int32_t> off32(p.offsets.begin(), p.offsets.end());
   void* s = engine_.stream();
-  if (int rc = queryXY_.reserve(std::max<size_t>(xy.size() * 4, 8))) return rc;
-  if (int rc = queryOff_.reserve(std::max<size_t>(off.size() * 4, 4))) return rc;
-  if (int rc = queryLen_.reserve(std::max<size_t>(len.size() * 4, 4))) return rc;
-  GATH_TRY(gat_memcpy_h2d(queryXY_.get(), xy.data(), xy.size() * 4, s), "query upload");
-  GATH_TRY(gat_memcpy_h2d(queryOff_.get(), off.data(), off.size() * 4, s), "query upload");
-  GATH_TRY(gat_memcpy_h2d(queryLen_.get(), len.data(), len.size() * 4, s), "query upload");
+  if (int rc = upload_vec(queryXY_, p.xy, s, "query upload")) return rc;
+  if (int rc = upload_vec(queryOff_, off32, s, "query upload")) return rc;
+  if (int rc = upload_vec(queryOff64_, p.offsets, s, "query upload")) return rc;
+  if (int rc = upload_vec(queryLen_, p.lengths, s, "query upload")) return rc;
+  if (int rc = upload_vec(queryBox_, p.boxes, s, "query upload")) return rc;
   GATH_TRY(gat_stream_sync(s), "query upload");
   return GAT_OK;
 }
@@ -132,5 +180,196 @@ int Grid::edrAll(const float* qlon, const float* qlat, int qlen, int32_t* dist) 
            "EDR batch");
   GATH_TRY(gat_memcpy_d2h(dist, dist_.get(), size_t(trajNum) * 4, s), "EDR readback");
   GATH_TRY(gat_stream_sync(s), "EDR readback");
+  return GAT_OK;
+}
+
+// ------------------------------------------------------------------ batched
+
+const std::vector<int32_t>& Grid::mismatchTable() {
+  if (!ftable_.empty()) return ftable_;
+  const size_t w = GAT_MAX_TRAJ_LEN + 1;
+  void* s = engine_.stream();
+  std::vector<int32_t> t(w * w);
+  if (ftableDev_.reserve(w * w * 4) == GAT_OK && gat_edr_mismatch_table(GAT_MAX_TRAJ_LEN, ftableDev_.as<int32_t>(), s) == GAT_OK &&
+      gat_memcpy_d2h(t.data(), ftableDev_.get(), w * w * 4, s) == GAT_OK && gat_stream_sync(s) == GAT_OK)
+    ftable_.swap(t);
+  else
+    gath_fail_hip(GAT_ERR_HIP, "mismatch table");
+  return ftable_;
+}
+
+// Orders the pairs so that consecutive groups of four share a strip count
+// (ceil(min length / 16)) and similar column counts, longest first, and runs
+// the batched kernel; dist receives each pair's distance in the original order.
+int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* st) {
+  const double h0 = now_ms();
+  const int64_t np = int64_t(pairs.size() / 2);
+  dist.assign(size_t(np), -1);
+  if (np == 0) return GAT_OK;
+  // Counting sort by (strips, longer length), descending.
+  constexpr int kM = GAT_MAX_TRAJ_LEN + 1;
+  constexpr int kS = GAT_MAX_TRAJ_LEN / 16 + 1;
+  std::vector<int32_t> key(static_cast<size_t>(np));
+  int64_t cells = 0;
+  for (int64_t i = 0; i < np; ++i) {
+    const int q = pairs[2 * i], c = pairs[2 * i + 1];
+    if (q < 0 || q >= queryNum || c < 0 || c >= trajNum) {
+      gath_set_error("batched EDR: pair %lld out of range", (long long)i);
+      return GAT_ERR_ARG;
+    }
+    const int a = queryLenHost_[q], b = trajLenHost_[c];
+    const int n = std::min(a, b), m = std::max(a, b);
+    cells += int64_t(n) * m;
+    key[i] = (kS - 1 - strips_of(n)) * kM + (kM - 1 - m);
+  }
+  std::vector<int64_t> cnt(size_t(kS) * kM + 1, 0);
+  for (int64_t i = 0; i < np; ++i) ++cnt[key[i] + 1];
+  for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
+  std::vector<int64_t> perm(static_cast<size_t>(np));
+  for (int64_t i = 0; i < np; ++i) perm[cnt[key[i]]++] = i;
+  std::vector<int32_t> sorted(size_t(np) * 2);
+  std::vector<gat_edr_job> jobs;
+  jobs.reserve(size_t(np / 4 + 1));
+  int64_t i = 0;
+  while (i < np) {
+    const int32_t k0 = key[perm[i]] / kM;
+    int32_t c = 0;
+    while (c < 4 && i + c < np && key[perm[i + c]] / kM == k0) ++c;
+    jobs.push_back(gat_edr_job{int32_t(i), c});
+    i += c;
+  }
+  for (int64_t j = 0; j < np; ++j) {
+    sorted[2 * j] = pairs[2 * perm[j]];
+    sorted[2 * j + 1] = pairs[2 * perm[j] + 1];
+  }
+  void* s = engine_.stream();
+  if (int rc = upload_vec(pairsDev_, sorted, s, "pair upload")) return rc;
+  if (int rc = upload_vec(jobsDev_, jobs, s, "job upload")) return rc;
+  if (int rc = outDev_.reserve(size_t(np) * 4)) return rc;
+  if (int rc = counterDev_.reserve(16)) return rc;
+  int maxlen = 1;
+  for (int v : queryLenHost_) maxlen = std::max(maxlen, v);
+  for (int v : trajLenHost_) maxlen = std::max(maxlen, v);
+  const double h1 = now_ms();
+  if (int rc = edrTimer_.start(s)) return rc;
+  GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>(), int64_t(jobs.size()),
+                           counterDev_.as<int32_t>(), queryXY_.as<float>(), queryOff64_.as<int64_t>(),
+                           queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(), trajOff_.as<int64_t>(),
+                           trajLen_.as<int32_t>(), trajNum, std::min(maxlen, GAT_MAX_TRAJ_LEN), epsilon,
+                           outDev_.as<int32_t>(), s),
+           "batched EDR");
+  if (int rc = edrTimer_.stop(s)) return rc;
+  std::vector<int32_t> out(static_cast<size_t>(np));
+  GATH_TRY(gat_memcpy_d2h(out.data(), outDev_.get(), size_t(np) * 4, s), "batched EDR readback");
+  GATH_TRY(gat_stream_sync(s), "batched EDR readback");
+  float ms = 0;
+  if (int rc = edrTimer_.elapsed(&ms)) return rc;
+  for (int64_t j = 0; j < np; ++j) dist[perm[j]] = out[j];
+  if (st) {
+    st->pairs += np;
+    st->cells += cells;
+    st->jobs += int64_t(jobs.size());
+    st->kernelMs += ms;
+    st->hostMs += (h1 - h0);
+  }
+  return GAT_OK;
+}
+
+int Grid::edrBatchedAll(const TrajectoryView& queries, std::vector<int32_t>& dist, EdrBatchStats* st) {
+  if (int rc = ensureQueryStore(queries)) return rc;
+  const int nq = queries.trajNum;
+  std::vector<int32_t> pairs(size_t(nq) * size_t(trajNum) * 2);
+  for (int q = 0; q < nq; ++q)
+    for (int t = 0; t < trajNum; ++t) {
+      pairs[2 * (size_t(q) * trajNum + t)] = q;
+      pairs[2 * (size_t(q) * trajNum + t) + 1] = t;
+    }
+  return runBatched(pairs, nq, dist, st);
+}
+
+int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vector<TopK>& out, EdrBatchStats* st) {
+  if (int rc = ensureQueryStore(queries)) return rc;
+  const int nq = queries.trajNum;
+  if (nq > 65535) {
+    gath_set_error("similarityBatched: at most 65535 queries per call");
+    return GAT_ERR_ARG;
+  }
+  const std::vector<int32_t>& F = mismatchTable();
+  if (F.empty()) return GAT_ERR_HIP;
+  void* s = engine_.stream();
+  // Candidate filter on the device; grow the survivor buffer until it fits.
+  int64_t cap = std::max<int64_t>(int64_t(nq) * 64, 1024);
+  uint64_t count = 0;
+  EventTimer ft;
+  for (;;) {
+    if (int rc = pairsDev_.reserve(size_t(cap) * 8)) return rc;
+    if (int rc = counterDev_.reserve(16)) return rc;
+    if (int rc = ft.start(s)) return rc;
+    GATH_TRY(gat_edr_filter(queryBox_.as<float>(), nq, trajBox_.as<float>(), trajNum, epsilon, pairsDev_.as<int32_t>(),
+                            cap, counterDev_.as<uint64_t>(), s),
+             "candidate filter");
+    if (int rc = ft.stop(s)) return rc;
+    GATH_TRY(gat_memcpy_d2h(&count, counterDev_.get(), 8, s), "filter readback");
+    GATH_TRY(gat_stream_sync(s), "filter readback");
+    if (int64_t(count) <= cap) break;
+    cap = int64_t(count) + int64_t(count) / 4 + 1024;
+  }
+  float fms = 0;
+  if (int rc = ft.elapsed(&fms)) return rc;
+  std::vector<int32_t> pairs(size_t(count) * 2);
+  GATH_TRY(gat_memcpy_d2h(pairs.data(), pairsDev_.get(), size_t(count) * 8, s), "survivor readback");
+  GATH_TRY(gat_stream_sync(s), "survivor readback");
+  std::vector<int32_t> dist;
+  if (int rc = runBatched(pairs, nq, dist, st)) return rc;
+  const double h0 = now_ms();
+  // Per query: the survivors with their exact distances, and every other
+  // trajectory at the all-mismatch value of its length, merged by (distance, id).
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> surv(static_cast<size_t>(nq));  // (dist, id)
+  for (size_t i = 0; i < count; ++i) surv[pairs[2 * i]].push_back({dist[i], pairs[2 * i + 1] + 1});
+  const int w = GAT_MAX_TRAJ_LEN + 1;
+  out.assign(size_t(nq), TopK{});
+  for (int q = 0; q < nq; ++q) {
+    auto& sv = surv[q];
+    std::sort(sv.begin(), sv.end());
+    std::vector<int32_t> excluded;
+    excluded.reserve(sv.size());
+    for (auto& e : sv) excluded.push_back(e.second);
+    std::sort(excluded.begin(), excluded.end());
+    const int a = queryLenHost_[q];
+    // Lengths present in the index, by all-mismatch value.
+    std::vector<std::pair<int32_t, int32_t>> byF;  // (F value, length)
+    for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L)
+      if (!lengthIds_[L].empty()) byF.push_back({F[size_t(std::min(a, L)) * w + std::max(a, L)], L});
+    std::sort(byF.begin(), byF.end());
+    std::vector<std::pair<int32_t, int32_t>> rest;  // (dist, id), ascending
+    const size_t need = size_t(kValue) + sv.size();
+    for (size_t g = 0; g < byF.size() && rest.size() < need;) {
+      size_t h = g;
+      std::vector<int32_t> ids;
+      while (h < byF.size() && byF[h].first == byF[g].first) {
+        for (int32_t id : lengthIds_[byF[h].second])
+          if (!std::binary_search(excluded.begin(), excluded.end(), id)) ids.push_back(id);
+        ++h;
+      }
+      std::sort(ids.begin(), ids.end());
+      for (int32_t id : ids) rest.push_back({byF[g].first, id});
+      g = h;
+    }
+    std::vector<std::pair<int32_t, int32_t>> merged;
+    merged.reserve(size_t(kValue));
+    std::merge(sv.begin(), sv.end(), rest.begin(), rest.end(), std::back_inserter(merged));
+    const size_t kk = std::min(size_t(kValue), merged.size());
+    out[q].dist.resize(kk);
+    out[q].ids.resize(kk);
+    for (size_t i = 0; i < kk; ++i) {
+      out[q].dist[i] = merged[i].first;
+      out[q].ids[i] = merged[i].second;
+    }
+  }
+  if (st) {
+    st->filterMs += fms;
+    st->pruned += int64_t(nq) * trajNum - int64_t(count);
+    st->hostMs += now_ms() - h0;
+  }
   return GAT_OK;
 }

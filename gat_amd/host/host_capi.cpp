@@ -1,4 +1,5 @@
 // The flat C ABI of libgat_host.so (include/gat_host.h) over the C++ classes.
+#include <algorithm>
 #include <memory>
 #include <new>
 #include <vector>
@@ -253,6 +254,65 @@ int gath_grid_similarity(void* p, const int64_t* q_offsets, int nq, const float*
       ids[int64_t(i) * k + j] = have ? res[i].ids[j] : 0;
       dists[int64_t(i) * k + j] = have ? res[i].dist[j] : -1;
     }
+  return GAT_OK;
+}
+
+namespace {
+void put_stats(const EdrBatchStats& st, int64_t* stats, double* ms) {
+  if (stats) {
+    stats[0] = st.pairs;
+    stats[1] = st.cells;
+    stats[2] = st.jobs;
+    stats[3] = st.pruned;
+    stats[4] = 0;
+  }
+  if (ms) {
+    ms[0] = st.kernelMs;
+    ms[1] = st.filterMs;
+    ms[2] = st.hostMs;
+  }
+}
+}  // namespace
+
+int gath_grid_edr_batched_all(void* p, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
+                              double eps, int32_t* dist, int64_t* stats, double* ms) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || nq < 0 || !q_offsets || !dist) return bad("gath_grid_edr_batched_all: bad arguments");
+  ix->grid->epsilon = eps;
+  TrajectoryView q{q_offsets, nq, qlon, qlat};
+  std::vector<int32_t> d;
+  EdrBatchStats st;
+  if (int rc = ix->grid->edrBatchedAll(q, d, &st)) return rc;
+  std::copy(d.begin(), d.end(), dist);
+  put_stats(st, stats, ms);
+  return GAT_OK;
+}
+
+int gath_grid_similarity_batched(void* p, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
+                                 int k, double eps, int32_t* ids, int32_t* dists, int64_t* stats, double* ms) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || nq < 0 || k < 0 || !q_offsets) return bad("gath_grid_similarity_batched: bad arguments");
+  ix->grid->epsilon = eps;
+  TrajectoryView q{q_offsets, nq, qlon, qlat};
+  std::vector<TopK> res;
+  EdrBatchStats st;
+  if (int rc = ix->grid->similarityBatched(q, k, res, &st)) return rc;
+  for (int i = 0; i < nq; ++i)
+    for (int j = 0; j < k; ++j) {
+      const bool have = j < int(res[i].ids.size());
+      ids[int64_t(i) * k + j] = have ? res[i].ids[j] : 0;
+      dists[int64_t(i) * k + j] = have ? res[i].dist[j] : -1;
+    }
+  put_stats(st, stats, ms);
+  return GAT_OK;
+}
+
+int gath_grid_mismatch_table(void* p, int32_t* out) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || !out) return bad("gath_grid_mismatch_table: bad arguments");
+  const std::vector<int32_t>& F = ix->grid->mismatchTable();
+  if (F.empty()) return GAT_ERR_HIP;
+  std::copy(F.begin(), F.end(), out);
   return GAT_OK;
 }
 

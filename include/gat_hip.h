@@ -39,6 +39,10 @@ enum {
 /* Longest trajectory the EDR kernels accept (the reference caps at 1024,
  * ConstDefine.h:24; the workloads here reach 2048). */
 #define GAT_MAX_TRAJ_LEN 2048
+/* NaN points a padded EDR store holds before every trajectory, and after its
+ * last one (GAT_EDR_TAIL_PAD). */
+#define GAT_EDR_PAD 16
+#define GAT_EDR_TAIL_PAD (GAT_MAX_TRAJ_LEN + 64)
 
 /* ------------------------------------------------------------------ runtime */
 /* Replaces the reference's direct cudaSetDevice/cudaStreamCreate/cudaMalloc/
@@ -167,6 +171,48 @@ int gatEDRDistanceBatchHandler(int queryTaskNum, const gat_edr_task* tasks_dev, 
                                const int32_t* query_offset_dev, const int32_t* query_length_dev,
                                const float* cand_xy_dev, const int64_t* cand_offset_dev,
                                const int32_t* cand_length_dev, double eps, int32_t* result_dev, void* stream);
+
+/* A batched-EDR work item: `count` (1..4) consecutive entries of the pair
+ * list starting at `first`, all with the same strip count ceil(min length / 16). */
+typedef struct gat_edr_job {
+  int32_t first;
+  int32_t count;
+} gat_edr_job;
+
+/* Batched EDR (no reference counterpart; the reference launches one block per
+ * pair in rounds of 80 candidates per query, Grid.cpp:1457-1638): out_dev[i] =
+ * EDR (Grid.cpp:1222-1318) of pair i = (query pairs_dev[2i], candidate
+ * pairs_dev[2i+1]), 0-based indices into the query and candidate stores, for
+ * every pair covered by jobs_dev.  One persistent launch hands jobs out
+ * through counter_dev (one int32 of scratch, reset by the call); jobs should be
+ * ordered longest first.  Both stores must be NaN-padded: at least 16 NaN
+ * points before every trajectory and GAT_MAX_TRAJ_LEN + 64 after the last.
+ * maxlen (the longest trajectory of any pair) selects the kernel's LDS size;
+ * a pair with an id outside [0, nqueries) / [0, ncands) or a length outside
+ * 1..maxlen gets -1, and jobs reaching past npairs are skipped.  Asynchronous
+ * on `stream`. */
+int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job* jobs_dev, int64_t njobs,
+                    int32_t* counter_dev, const float* query_xy_dev, const int64_t* query_offset_dev,
+                    const int32_t* query_length_dev, int32_t nqueries, const float* cand_xy_dev,
+                    const int64_t* cand_offset_dev, const int32_t* cand_length_dev, int32_t ncands, int32_t maxlen,
+                    double eps, int32_t* out_dev, void* stream);
+
+/* Candidate filter for exact top-k pruning (no reference counterpart; the
+ * reference orders candidates by an unproven frequency-distance bound,
+ * FVTable.cpp:77-207): writes every (query, candidate) index pair whose
+ * eps-dilated bounding boxes can hold a matching point pair to pairs_out_dev
+ * (up to `capacity` pairs; *count_dev, reset by the call, ends as the full
+ * number).  Boxes are (xmin, ymin, xmax, ymax) floats per trajectory.  The test
+ * is exactly conservative under float rounding, so every dropped pair has no
+ * matching point pair and its distance is gat_edr_mismatch_table's value.
+ * nq <= 65535.  Asynchronous on `stream`. */
+int gat_edr_filter(const float* query_box_dev, int32_t nq, const float* cand_box_dev, int32_t nc, double eps,
+                   int32_t* pairs_out_dev, int64_t capacity, uint64_t* count_dev, void* stream);
+
+/* out_dev[r * (maxlen + 1) + c] = the recurrence of Grid.cpp:1264-1306 with
+ * every substitution costing 1, for 0 <= r, c <= maxlen: the distance of a pair
+ * of lengths r <= c with no matching point.  Asynchronous on `stream`. */
+int gat_edr_mismatch_table(int32_t maxlen, int32_t* out_dev, void* stream);
 
 /* The float threshold u with (double)|d| < eps  <=>  |d| < u for every float d:
  * the smallest float >= eps. */

@@ -79,6 +79,14 @@ int gath_index_last_digests(void* index, uint64_t* digests);
 int gath_grid_edr_all(void* grid, const float* qlon, const float* qlat, int qlen, double eps, int32_t* out);
 int gath_grid_similarity(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, int k,
                          double eps, int32_t* ids, int32_t* dists);
+/* Batched forms.  stats[0..4] = pairs run, DP cells, jobs, pairs resolved by
+ * the all-mismatch table, 0; ms[0..2] = DP kernel, filter kernel, host work. */
+int gath_grid_edr_batched_all(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
+                              double eps, int32_t* dist, int64_t* stats, double* ms);
+int gath_grid_similarity_batched(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
+                                 int k, double eps, int32_t* ids, int32_t* dists, int64_t* stats, double* ms);
+/* The all-mismatch table, (GAT_MAX_TRAJ_LEN + 1)^2 int32. */
+int gath_grid_mismatch_table(void* grid, int32_t* out);
 
 #pragma GCC visibility pop
 

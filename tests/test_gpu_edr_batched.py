@@ -1,0 +1,128 @@
+"""GPU parity of the batched EDR forms: the persistent wave-systolic kernel, the
+all-mismatch table, the candidate filter and the pruned top-k, all against the
+oracle's restatement of Grid::SimilarityExecuter (Grid.cpp:1222-1318) and
+brute-force top-k by (distance, id).  Bar: bit-exact."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gat_amd import _native as N, api, workloads as W
+
+from tests.devbuf import DevBuf
+
+pytestmark = pytest.mark.gpu
+EPS = 5e-4
+THREADS = 16
+
+
+def _brute(db, queries, eps=EPS):
+    return np.stack([O.edr_batch(queries.xy(i + 1), db, eps, nthreads=THREADS) for i in range(queries.ntraj)])
+
+
+def test_mismatch_table_matches_oracle(gpu):
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(W.CFG1.db(20))
+    assert (g.mismatch_table() == O.ftable(api.MAX_TRAJ_LEN)).all()
+
+
+def test_batched_all_cfg1(gpu):
+    db = W.CFG1.db()
+    g = api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db)
+    q = api.synth_queries(W.CFG1.seed, np.array([128, 128], np.int32), db, copy_mode=2)
+    dist, st = g.edr_batched_all(q, EPS)
+    assert st["pairs"] == 2 * db.ntraj
+    assert (dist == _brute(db, q)).all()
+
+
+def _varied_db(seed, lengths):
+    rng = np.random.default_rng(seed)
+    L = np.array(lengths, np.int32)
+    db = api.synth_walks(seed, L, step_sigma=2e-4)
+    # Pull a third of the trajectories onto one spot so many pairs match.
+    for t in range(0, db.ntraj, 3):
+        b, e = db.offsets[t], db.offsets[t + 1]
+        db.lon[b:e] = np.float32(0.5) + (db.lon[b:e] - db.lon[b]) * np.float32(0.2)
+        db.lat[b:e] = np.float32(0.5) + (db.lat[b:e] - db.lat[b]) * np.float32(0.2)
+    return db
+
+
+@pytest.mark.parametrize("maxlen", [512, 2048])
+def test_batched_all_lengths(gpu, maxlen):
+    base = [1, 2, 15, 16, 17, 31, 32, 33, 63, 64, 65, 100, 255, 256, 257, 511, 512]
+    if maxlen > 512:
+        base += [513, 1000, 1023, 1024, 1025, 2047, 2048]
+    rng = np.random.default_rng(maxlen)
+    lengths = base + list(rng.integers(1, maxlen + 1, 60))
+    db = _varied_db(maxlen, lengths)
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    qlen = np.array([1, 16, 17, 64, 129, min(maxlen, 700), maxlen] + list(rng.integers(1, maxlen + 1, 5)), np.int32)
+    q = api.synth_queries(maxlen + 1, qlen, db, copy_mode=2)
+    dist, st = g.edr_batched_all(q, EPS)
+    want = _brute(db, q)
+    bad = np.argwhere(dist != want)
+    assert len(bad) == 0, [(int(a), int(b), int(qlen[a]), int(db.lengths[b]), int(dist[a, b]), int(want[a, b]))
+                           for a, b in bad[:10]]
+    # Another eps, to check the threshold rounding path.
+    dist2, _ = g.edr_batched_all(q, 3e-3)
+    assert (dist2 == _brute(db, q, 3e-3)).all()
+
+
+def test_similarity_batched_topk(gpu):
+    # A 4000-trajectory slice of the 200k-candidate workload (lognormal lengths,
+    # median 128); queries U{16..1024}, every other one a noisy copy.
+    w = W.CFG3
+    db = w.db(4000)
+    g = api.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    qlen = api.synth_lengths(w.seed + 100, 12, "uniform", 16, 1024, 16, 1024)
+    q = api.synth_queries(w.seed, qlen, db, copy_mode=2)
+    ids, dists, st = g.similarity_batched(q, w.k, w.eps)
+    assert st["pruned"] > 0 and st["pairs"] + st["pruned"] == q.ntraj * db.ntraj
+    full = _brute(db, q, w.eps)
+    tid = np.arange(1, db.ntraj + 1, dtype=np.int32)
+    for i in range(q.ntraj):
+        oi, od = O.topk(full[i], tid, w.k)
+        assert (ids[i] == oi).all() and (dists[i] == od).all(), i
+    # The plain per-query path gives the same answer.
+    pi, pd = g.similarity(q, w.k, w.eps)
+    assert (pi == ids).all() and (pd == dists).all()
+
+
+def test_similarity_batched_cfg1(gpu):
+    db = W.CFG1.db()
+    g = api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db)
+    q = api.synth_queries(W.CFG1.seed, np.array([128, 128], np.int32), db, copy_mode=2)
+    ids, dists, _ = g.similarity_batched(q, W.CFG1.k, EPS)
+    full = _brute(db, q)
+    tid = np.arange(1, db.ntraj + 1, dtype=np.int32)
+    for i in range(2):
+        oi, od = O.topk(full[i], tid, W.CFG1.k)
+        assert (ids[i] == oi).all() and (dists[i] == od).all()
+
+
+def test_filter_boundary_is_exact(gpu):
+    """A candidate box exactly u (the float threshold) beyond the query box is
+    dropped, one float closer is kept; every dropped pair's points cannot match."""
+    h = N.hip()
+    u = np.float32(api.eps_threshold(EPS))
+    q = np.array([[0.25, 0.25, 0.5, 0.5]], np.float32)
+    # Find x such that fl(x - 0.5) == u exactly, and one such that it is just below.
+    cands = []
+    for target in (u, np.nextafter(u, np.float32(0))):
+        x = np.float32(0.5) + target
+        while np.float32(x - np.float32(0.5)) > target:
+            x = np.nextafter(x, np.float32(0))
+        while np.float32(x - np.float32(0.5)) < target:
+            x = np.nextafter(x, np.float32(1))
+        cands.append([x, 0.3, x + np.float32(0.1), 0.4])
+    cb = np.array(cands, np.float32)
+    dq, dc = DevBuf.of(q), DevBuf.of(cb)
+    out = DevBuf(8 * 4)
+    cnt = DevBuf(8)
+    N.check_hip(h.gat_edr_filter(dq.ptr, 1, dc.ptr, 2, EPS, out.ptr, 4, cnt.ptr, None))
+    n = int(cnt.get(np.uint64, 1)[0])
+    pairs = out.get(np.int32, 2 * n).reshape(-1, 2)
+    assert n == 1 and pairs.tolist() == [[0, 1]]
+    # The two boxes' nearest points: dropped pair does not match, kept pair does.
+    assert O.edr(np.array([[0.5, 0.35]], np.float32), np.array([[cb[0, 0], 0.35]], np.float32), EPS) == 1
+    assert O.edr(np.array([[0.5, 0.35]], np.float32), np.array([[cb[1, 0], 0.35]], np.float32), EPS) == 0
