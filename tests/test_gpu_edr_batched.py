@@ -102,20 +102,13 @@ def test_similarity_batched_cfg1(gpu):
 
 def test_filter_boundary_is_exact(gpu):
     """A candidate box exactly u (the float threshold) beyond the query box is
-    dropped, one float closer is kept; every dropped pair's points cannot match."""
+    dropped, one float closer is kept; the dropped pair's points cannot match."""
     h = N.hip()
     u = np.float32(api.eps_threshold(EPS))
-    q = np.array([[0.25, 0.25, 0.5, 0.5]], np.float32)
-    # Find x such that fl(x - 0.5) == u exactly, and one such that it is just below.
-    cands = []
-    for target in (u, np.nextafter(u, np.float32(0))):
-        x = np.float32(0.5) + target
-        while np.float32(x - np.float32(0.5)) > target:
-            x = np.nextafter(x, np.float32(0))
-        while np.float32(x - np.float32(0.5)) < target:
-            x = np.nextafter(x, np.float32(1))
-        cands.append([x, 0.3, x + np.float32(0.1), 0.4])
-    cb = np.array(cands, np.float32)
+    below = np.nextafter(u, np.float32(0))
+    # The query box ends at x = 0, so a candidate starting at x has difference x exactly.
+    q = np.array([[-0.25, 0.25, 0.0, 0.5]], np.float32)
+    cb = np.array([[u, 0.3, u + np.float32(0.1), 0.4], [below, 0.3, below + np.float32(0.1), 0.4]], np.float32)
     dq, dc = DevBuf.of(q), DevBuf.of(cb)
     out = DevBuf(8 * 4)
     cnt = DevBuf(8)
@@ -123,6 +116,6 @@ def test_filter_boundary_is_exact(gpu):
     n = int(cnt.get(np.uint64, 1)[0])
     pairs = out.get(np.int32, 2 * n).reshape(-1, 2)
     assert n == 1 and pairs.tolist() == [[0, 1]]
-    # The two boxes' nearest points: dropped pair does not match, kept pair does.
-    assert O.edr(np.array([[0.5, 0.35]], np.float32), np.array([[cb[0, 0], 0.35]], np.float32), EPS) == 1
-    assert O.edr(np.array([[0.5, 0.35]], np.float32), np.array([[cb[1, 0], 0.35]], np.float32), EPS) == 0
+    # The boxes' nearest points: the dropped pair does not match, the kept one does.
+    assert O.edr(np.array([[0.0, 0.35]], np.float32), np.array([[u, 0.35]], np.float32), EPS) == 1
+    assert O.edr(np.array([[0.0, 0.35]], np.float32), np.array([[below, 0.35]], np.float32), EPS) == 0
