@@ -1,19 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the trajectory-query hot path on one MI355X.
 
-Headline (value): range queries/s of the plain drop-in GT-quadtree path on
-the 1M-trajectory workload (88.2M points, 1024x1024 Morton grid, 256-point
-leaves, 10k squares with log-uniform side in [1e-4, 1e-1]).  One step = one
-batch of the 10k rectangles through Grid::rangeQueryBatchGPU's pair form:
-host leaf selection, task upload, the refine kernel, per-query count readback;
-the point store is resident in HBM before timing starts and the result pairs
-stay there.
+Headline (value): range queries/s of the batched GT-quadtree form on the
+1M-trajectory workload (88.2M points, 1024x1024 Morton grid, 256-point leaves,
+10k squares with log-uniform side in [1e-4, 1e-1]).  One step = the 10k
+rectangles through Grid::rangeQueryBatchedGPU: box upload, the captured device
+pipeline (tree selection, scans, leaf grouping, leaf-major refine) replayed as
+one graph, per-query count and region readback.  The point store and tree are
+resident in HBM before timing starts; the result pairs stay there.
 
-Also reported on the same line: the FSG and STIG paths on the same batch, the
-plain EDR kernel's cell-update rate, the refine kernel's roofline (algorithmic
-bytes 12 per candidate point + 8 per result pair, over its HIP-event time on
-its own stream), and the oracle's CPU restatement of Grid::rangeQueryBatch timed
-on this host.
+Beside it on the same line: the plain drop-in on the same batch (host leaf
+selection as Grid::rangeQueryBatchGPU, task upload, the per-task refine
+kernel) and the ratio; STIG and FSG; the plain and batched EDR cell-update
+rates over one pair set and the batched top-k rate; the leaf-major refine
+kernel's roofline (algorithmic bytes 12 per point of a selected leaf + 8 per
+result pair, over its HIP-event time on its own stream); and the oracle's CPU
+restatement of Grid::rangeQueryBatch timed on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -50,21 +52,32 @@ def load_traffic(name: str):
         return None
 
 
-def time_range(index, rects, steps, warmup, label):
+def time_steps(fn, steps, warmup):
     for _ in range(warmup):
-        index.range_pairs(rects)
-    kernel_ms, wall = [], []
+        fn()
+    wall = []
     t0 = time.perf_counter()
     for _ in range(steps):
         s = time.perf_counter()
-        res = index.range_pairs(rects)
+        res = fn()
         wall.append(time.perf_counter() - s)
-        kernel_ms.append(res.kernel_ms)
-    total = time.perf_counter() - t0
+    return res, time.perf_counter() - t0, np.array(wall)
+
+
+def time_range(index, rects, steps, warmup, label):
+    res, total, wall = time_steps(lambda: index.range_pairs(rects), steps, warmup)
     qps = steps * len(rects) / total
-    log(f"{label}: {qps:,.0f} queries/s, step {1e3 * total / steps:.2f} ms, kernel {np.mean(kernel_ms):.3f} ms, "
+    log(f"{label}: {qps:,.0f} queries/s, step {1e3 * total / steps:.2f} ms, kernel {res.kernel_ms:.3f} ms, "
         f"select {res.select_ms:.2f} ms, {res.pairs_total:,} pairs from {res.candidates:,} candidates")
-    return res, qps, total, np.array(kernel_ms), np.array(wall)
+    return res, qps, total, wall
+
+
+def time_batched(index, rects, steps, warmup, label):
+    res, total, wall = time_steps(lambda: index.range_batched(rects, graph=True), steps, warmup)
+    qps = steps * len(rects) / total
+    log(f"{label}: {qps:,.0f} queries/s, step {1e3 * total / steps:.2f} ms, device pipeline {res.kernel_ms:.3f} ms, "
+        f"replayed {res.replayed}, {res.leaf_queries:,} (leaf, query) entries, {res.work_items:,} items")
+    return res, qps, total, wall
 
 
 def main() -> int:
@@ -73,9 +86,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "range", "copy"], default="all",
-                    help="range: just the timed GT-quadtree batches (for profiling); copy: just the stream-copy "
-                         "calibration kernel")
+    ap.add_argument("--only", choices=["all", "range", "batched", "copy"], default="all",
+                    help="range / batched: just the timed GT-quadtree batches of the plain / batched form (for "
+                         "profiling); copy: just the stream-copy calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU split of the "
@@ -103,45 +116,48 @@ def main() -> int:
     grid = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
     log(f"GT-quadtree built and resident ({time.perf_counter() - t:.1f} s): {grid.info().tolist()}")
 
-    res, qps, total, kms, wall = time_range(grid, rects, args.steps, args.warmup, "GT-quadtree plain")
-    alg_bytes = 12 * res.candidates + 8 * res.pairs_total
-    k_mean = float(kms.mean())
-    achieved = alg_bytes / (k_mean * 1e-3) / 1e9
-    traffic = load_traffic("range_refine_kernel<true>:cfg2")
     if args.only == "range":
-        print(json.dumps({"value": round(qps, 1), "kernel_ms": round(k_mean, 4), "achieved_gbs": round(achieved, 1),
-                          "algorithmic_bytes": int(alg_bytes), "candidates": res.candidates,
-                          "pairs": res.pairs_total}))
+        res, qps, total, _ = time_range(grid, rects, args.steps, args.warmup, "GT-quadtree plain")
+        alg = 12 * res.candidates + 8 * res.pairs_total
+        print(json.dumps({"value": round(qps, 1), "kernel_ms": round(res.kernel_ms, 4),
+                          "achieved_gbs": round(alg / (res.kernel_ms * 1e-3) / 1e9, 1), "algorithmic_bytes": alg}))
         return 0
+    if args.only == "batched":
+        res, qps, total, _ = time_batched(grid, rects, args.steps, args.warmup, "GT-quadtree batched")
+        print(json.dumps({"value": round(qps, 1), "pipeline_ms": round(res.kernel_ms, 4)}))
+        return 0
+
+    pres, pqps, ptotal, pwall = time_range(grid, rects, args.steps, args.warmup, "GT-quadtree plain")
+    bres, bqps, btotal, bwall = time_batched(grid, rects, args.steps, args.warmup, "GT-quadtree batched")
+    # Per-kernel events come from step-by-step runs (a graph replay times the whole pipeline).
+    refine_ms = []
+    for _ in range(5):
+        r = grid.range_batched(rects, graph=False)
+        refine_ms.append(r.refine_ms)
+    refine_ms = float(np.median(refine_ms))
+    unique = grid.last_unique_candidates()
+    alg_bytes = 12 * unique + 8 * r.pairs_total
+    achieved = alg_bytes / (refine_ms * 1e-3) / 1e9
+    same = bool((bres.counts() == pres.counts()).all() and (bres.digests() == pres.digests()).all())
+    log(f"leaf-major refine {refine_ms:.3f} ms: {achieved:,.0f} GB/s algorithmic ({unique:,} distinct candidate "
+        f"points, {r.pairs_total:,} pairs); selection {r.select_ms:.3f} ms; batched == plain: {same}")
+    plain_alg = 12 * pres.candidates + 8 * pres.pairs_total
+    plain_achieved = plain_alg / (pres.kernel_ms * 1e-3) / 1e9
+    traffic = load_traffic("leaf_refine_kernel:cfg2")
     copy_gbs, _ = G.stream_copy_bandwidth(1 << 32, 10)
     log(f"stream copy reference: {copy_gbs:.1f} GB/s")
 
     t = time.perf_counter()
-    fsg = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db)
     stig = G.StigIndex(w.leaf).build(db)
+    fsg = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db)
     log(f"FSG and STIG built ({time.perf_counter() - t:.1f} s)")
-    fres, fqps, _, fk, _ = time_range(fsg, rects, max(3, args.steps // 2), 1, "FSG plain")
-    sres, sqps, _, sk, _ = time_range(stig, rects, max(3, args.steps // 2), 1, "STIG plain")
+    few = max(3, args.steps // 2)
+    _, fqps, _, _ = time_range(fsg, rects, few, 1, "FSG plain")
+    _, sqps, _, _ = time_range(stig, rects, few, 1, "STIG plain")
+    _, sbqps, _, _ = time_batched(stig, rects, few, 1, "STIG batched")
     del fsg, stig
 
-    # Plain EDR kernel: one 128-point query against every trajectory of the
-    # 20k-trajectory workload, both query kinds (independent walk, noisy copy).
-    w1 = W.CFG1
-    db1 = w1.db()
-    g1 = G.GridIndex(W.UNIT_BOX, w1.grid_cell, w1.leaf).build(db1)
-    queries = G.synth_queries(w1.seed, np.array([128, 128], np.int32), db1, copy_mode=2)
-    L = db1.lengths.astype(np.int64)
-    cells = int(L.sum() * 128)
-    for _ in range(2):
-        g1.edr_all(queries.xy(1), w1.eps)
-    reps = 10
-    t = time.perf_counter()
-    for r in range(reps):
-        g1.edr_all(queries.xy(1 + r % 2), w1.eps)
-    edr_s = (time.perf_counter() - t) / reps
-    edr_cells_ps = cells / edr_s
-    log(f"EDR plain: {edr_cells_ps:.3e} cell-updates/s ({cells:,} cells per query, {1e3 * edr_s:.2f} ms)")
-    del g1
+    edr = bench_edr(G, W)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -153,7 +169,7 @@ def main() -> int:
         counts, digests = og.range(rects, "digest", nthreads=1)
         cpu_s = time.perf_counter() - t
         # The GPU answer of the timed batches, checked against the oracle.
-        ok = bool((res.counts() == counts).all() and (res.digests() == digests).all())
+        ok = bool((bres.counts() == counts).all() and (bres.digests() == digests).all())
         t = time.perf_counter()
         og.range(rects, "digest", nthreads=16)
         cpu16_s = time.perf_counter() - t
@@ -168,38 +184,94 @@ def main() -> int:
 
     line = {
         "metric": METRIC,
-        "value": round(qps, 1),
+        "value": round(bqps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "ms_per_step": round(1e3 * btotal / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded random walks)",
-        "config": {"workload": "cfg2 plain drop-in: 1M trajectories, 88.2M points, 1024x1024 GT-quadtree, "
-                               "256-point leaves, 10k squares log-uniform side [1e-4,1e-1]",
+        "config": {"workload": "1M trajectories, 88.2M points, 1024x1024 GT-quadtree, 256-point leaves, 10k "
+                               "squares log-uniform side [1e-4,1e-1]; batched form (device selection, leaf-major "
+                               "join, graph replay)",
                    "index": "GT-quadtree", "rectangles": int(len(rects)), "points": int(db.npoints)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "range_refine_kernel<true>", "kernel_ms": round(k_mean, 4),
+                     "kernel": "leaf_refine_kernel", "kernel_ms": round(refine_ms, 4),
                      "algorithmic_bytes": int(alg_bytes)},
         "cpu_baseline": cpu,
         "paths": {
-            "gt_quadtree_plain_qps": round(qps, 1),
-            "fsg_plain_qps": round(fqps, 1),
+            "gt_quadtree_batched_qps": round(bqps, 1),
+            "gt_quadtree_plain_qps": round(pqps, 1),
+            "batched_over_plain": round(bqps / pqps, 3),
+            "batched_equals_plain": same,
+            "plain_refine_kernel_ms": round(pres.kernel_ms, 4),
+            "plain_refine_frac_hbm": round(plain_achieved / HBM_PEAK_GBS, 4),
+            "batched_selection_ms": round(r.select_ms, 4),
+            "batched_pipeline_ms": round(bres.kernel_ms, 4),
+            "stig_batched_qps": round(sbqps, 1),
             "stig_plain_qps": round(sqps, 1),
-            "edr_plain_cells_per_s": float(f"{edr_cells_ps:.4e}"),
-            "range_p50_ms": round(1e3 * float(np.percentile(wall, 50)), 3),
-            "range_p99_ms": round(1e3 * float(np.percentile(wall, 99)), 3),
+            "fsg_plain_qps": round(fqps, 1),
+            "range_batched_p50_ms": round(1e3 * float(np.percentile(bwall, 50)), 3),
+            "range_batched_p99_ms": round(1e3 * float(np.percentile(bwall, 99)), 3),
+            "range_plain_p50_ms": round(1e3 * float(np.percentile(pwall, 50)), 3),
+            "range_plain_p99_ms": round(1e3 * float(np.percentile(pwall, 99)), 3),
             "stream_copy_gbs": round(copy_gbs, 1),
+            **edr,
         },
         "device": dev,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_edr(G, W) -> dict:
+    """EDR cell-update rates over one pair set (4 queries against the 200k
+    candidates of the top-k workload, both query kinds): plain = one launch per
+    query over every candidate, batched = the persistent kernel over the same
+    800k pairs.  Then the exact top-50 of the 1024-query batch."""
+    w = W.CFG3
+    t = time.perf_counter()
+    db = w.db()
+    g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    log(f"EDR workload {w.name}: {db.ntraj:,} candidates, {db.npoints:,} points ({time.perf_counter() - t:.1f} s)")
+    qlen = np.array([128, 512, 1024, 64], np.int32)
+    q = G.synth_queries(w.seed + 7, qlen, db, copy_mode=2)
+    cells = int(db.lengths.astype(np.int64).sum() * int(qlen.sum()))
+    g.edr_all(q.xy(1), w.eps)
+    t = time.perf_counter()
+    plain = np.stack([g.edr_all(q.xy(i + 1), w.eps) for i in range(q.ntraj)])
+    plain_s = time.perf_counter() - t
+    g.edr_batched_all(q, w.eps)
+    t = time.perf_counter()
+    batched, st = g.edr_batched_all(q, w.eps)
+    batched_s = time.perf_counter() - t
+    same = bool((batched.reshape(plain.shape) == plain).all())
+    plain_rate, batched_rate = cells / plain_s, cells / batched_s
+    kernel_rate = cells / (st["kernel_ms"] * 1e-3)
+    log(f"EDR over {q.ntraj * db.ntraj:,} pairs ({cells:.3e} cells): plain {plain_rate:.3e} cells/s, batched "
+        f"{batched_rate:.3e} cells/s wall, {kernel_rate:.3e} in-kernel; equal: {same}")
+    del plain, batched
+    lo, hi = w.extra["qlen_lo"], w.extra["qlen_hi"]
+    qlen = G.synth_lengths(w.seed + 100, w.extra["nq"], "uniform", lo, hi, lo, hi)
+    queries = G.synth_queries(w.seed, qlen, db, copy_mode=2)
+    g.similarity_batched(queries, w.k, w.eps)
+    t = time.perf_counter()
+    ids, dists, st = g.similarity_batched(queries, w.k, w.eps)
+    topk_s = time.perf_counter() - t
+    log(f"top-{w.k} batched: {queries.ntraj / topk_s:,.1f} queries/s ({1e3 * topk_s:.1f} ms for {queries.ntraj}; "
+        f"{st['pairs']:,} pairs evaluated, {st['pruned']:,} resolved by the mismatch table)")
+    return {"edr_plain_cells_per_s": float(f"{plain_rate:.4e}"),
+            "edr_batched_cells_per_s": float(f"{batched_rate:.4e}"),
+            "edr_batched_kernel_cells_per_s": float(f"{kernel_rate:.4e}"),
+            "edr_batched_over_plain": round(batched_rate / plain_rate, 3),
+            "edr_batched_equals_plain": same,
+            "topk_batched_qps": round(queries.ntraj / topk_s, 2),
+            "topk_pairs_evaluated": int(st["pairs"])}
 
 
 if __name__ == "__main__":

@@ -75,6 +75,17 @@ HIP_FUNCS = {
     "gat_edr_batched": (INT, [P, I64, P, I64, P, P, P, P, I32, P, P, P, I32, I32, F64, P, P]),
     "gat_edr_filter": (INT, [P, I32, P, I32, F64, P, I64, P, P]),
     "gat_edr_mismatch_table": (INT, [I32, P, P]),
+    "gat_fill_u32": (INT, [P, U32, I64, P]),
+    "gat_scan_tiles": (I64, [I64]),
+    "gat_scan_u64": (INT, [P, I64, P, P, P, P]),
+    "gat_scan_leaf_lists": (INT, [P, I64, I32, P, P, P, P]),
+    "gat_tree_select": (INT, [P, I64, I32, P, P, I64, I64, P, P, P, I64, P, P, P]),
+    "gat_leaf_scatter": (INT, [P, P, I64, P, I64, P, P, P, I32, P, P, I64, P, I64, P, I64, P]),
+    "gat_leaf_refine": (INT, [P, P, P, I64, P, I64, I64, P, P, I64, P, P]),
+    "gat_graph_begin": (INT, [P]),
+    "gat_graph_end": (INT, [P, P]),
+    "gat_graph_launch": (INT, [P, P]),
+    "gat_graph_destroy": (INT, [P]),
 }
 
 HOST_FUNCS = {
@@ -99,6 +110,10 @@ HOST_FUNCS = {
     "gath_index_cell_of": (I64, [P, F32, F32]),
     "gath_index_range_bitmap": (INT, [P, P, INT, P]),
     "gath_index_range_pairs": (INT, [P, P, INT, P, P]),
+    "gath_index_range_batched": (INT, [P, P, INT, INT, P, P]),
+    "gath_index_tree_nodes": (I64, [P, P, I64, P]),
+    "gath_index_set_batch_params": (INT, [P, INT]),
+    "gath_index_last_unique_candidates": (INT, [P, P]),
     "gath_index_last_counts": (INT, [P, P]),
     "gath_index_last_fetch": (INT, [P, P, I64]),
     "gath_index_last_digests": (INT, [P, P]),

@@ -154,6 +154,12 @@ def synth_queries(seed: int, lengths: np.ndarray, db: TrajSet | None, copy_mode:
 
 
 # ---------------------------------------------------------------- indexes
+# gat_tree_node (include/gat_hip.h).
+TREE_NODE = np.dtype([("box", np.float32, 4), ("first_child", np.int32), ("start", np.uint32), ("count", np.uint32),
+                      ("flags", np.uint32)])
+NODE_INCLUSIVE, NODE_UNTESTED = 1, 2
+
+
 class RangeResult:
     """A compacted-pair result that stays in HBM until the index's next query."""
 
@@ -164,6 +170,10 @@ class RangeResult:
         self.candidates = int(stats[1])
         self.tasks = int(stats[2])
         self.select_ms, self.upload_ms, self.kernel_ms = (float(x) for x in ms)
+        # Batched form: (leaf, query) entries, leaf-major work items, graph replay.
+        self.leaf_queries = int(stats[2]) if len(stats) > 3 else 0
+        self.work_items = int(stats[3]) if len(stats) > 3 else 0
+        self.replayed = bool(stats[4]) if len(stats) > 4 else False
 
     def counts(self) -> np.ndarray:
         out = np.empty(self.nq, np.int64)
@@ -239,6 +249,41 @@ class _Index:
         ms = np.zeros(3, np.float64)
         N.check_host(N.host().gath_index_range_pairs(self.handle, _ptr(b), len(b), _ptr(stats), _ptr(ms)))
         return RangeResult(self, len(b), stats, ms)
+
+    def range_batched(self, boxes: np.ndarray, graph: bool = False) -> RangeResult:
+        """Batched form (Grid and STIG): device selection plus the leaf-major join
+        for the whole batch; graph=True captures the device pipeline and replays
+        it on later calls that fit.  Same result sets as range_pairs."""
+        b = _req(boxes, np.float32).reshape(-1, 4)
+        stats = np.zeros(5, np.int64)
+        ms = np.zeros(4, np.float64)
+        N.check_host(N.host().gath_index_range_batched(self.handle, _ptr(b), len(b), int(graph), _ptr(stats),
+                                                       _ptr(ms)))
+        res = RangeResult(self, len(b), stats, ms[:3])
+        res.refine_ms = float(ms[3])
+        return res
+
+    def last_unique_candidates(self) -> int:
+        """Points of the leaves the last batched run selected at least once."""
+        out = np.zeros(1, np.int64)
+        N.check_host(N.host().gath_index_last_unique_candidates(self.handle, _ptr(out)))
+        return int(out[0])
+
+    def set_batch_params(self, queries_per_item: int) -> "_Index":
+        """Rectangles per leaf-major work item of the batched form (1..64)."""
+        N.check_host(N.host().gath_index_set_batch_params(self.handle, int(queries_per_item)))
+        return self
+
+    def tree_nodes(self) -> tuple[np.ndarray, int]:
+        """The tree the batched form uploads: a structured array of gat_tree_node
+        in breadth-first order, and its depth."""
+        depth = np.zeros(1, np.int32)
+        n = int(N.host().gath_index_tree_nodes(self.handle, None, 0, _ptr(depth)))
+        if n < 0:
+            N.check_host(n)
+        out = np.zeros(max(n, 1), TREE_NODE)
+        N.host().gath_index_tree_nodes(self.handle, out.ctypes.data, n, _ptr(depth))
+        return out[:n], int(depth[0])
 
 
 class GridIndex(_Index):

@@ -1,0 +1,619 @@
+// Batched range form for gfx950: device-side candidate selection over a
+// breadth-first copy of the GT-quadtree (Grid.cpp:58-94, 428-446) or the STIG
+// kd-tree (STIG.cpp:71-168, 179-252), and a leaf-major join.
+//
+// Pipeline for one batch of rectangles (all kernels, no host round trip):
+//   tree_select_kernel   one wave per rectangle walks the tree with a stack in
+//                        LDS, 64 nodes per step; emits (leaf, query) entries
+//                        and the candidate count of every rectangle;
+//   scan kernels         per-query output regions (exclusive sum of candidate
+//                        counts) and per-leaf list / work-item offsets;
+//   leaf_scatter_kernel  groups the entries by leaf;
+//   leaf_refine_kernel   per work item (one leaf, up to qb rectangles) the
+//                        leaf's points and ids are loaded into registers once
+//                        and tested against each rectangle; hits are compacted
+//                        with ballots and written as (tid, pid) pairs.
+// A point of a leaf selected by r rectangles is read from HBM once instead of
+// r times, which is what takes the batch below the per-query scan's traffic.
+#include "gat_common.hpp"
+
+namespace {
+
+constexpr int kWave = 64;
+
+struct Node {
+  float4 box;
+  int32_t first_child;
+  uint32_t start, count, flags;
+};
+static_assert(sizeof(Node) == sizeof(gat_tree_node), "node layout");
+
+struct Store {
+  const float2* xy;
+  const uint32_t* id;
+  const uint16_t* pid;
+  uint64_t npoints;
+  uint32_t pid_bits;
+};
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  return uint64_t(uniform(uint32_t(v))) | (uint64_t(uniform(uint32_t(v >> 32))) << 32);
+}
+
+// BoxIntersect(a1, b1) != 0 (MBB.cpp:28-60): the pair is ordered so the box
+// starting further left (on a tie, reaching further right; on a full tie, b1)
+// comes first, and boxes that only touch do not intersect.
+__device__ __forceinline__ bool box_intersect(const float4& a1, const float4& b1) {
+  const bool a1_first = a1.x < b1.x || (a1.x == b1.x && a1.z > b1.z);
+  const float4 a = a1_first ? a1 : b1;
+  const float4 b = a1_first ? b1 : a1;
+  return !(b.x >= a.z || b.w <= a.y || b.y >= a.w);
+}
+
+__device__ __forceinline__ bool node_selected(const Node& n, const float4& q) {
+  if (n.flags & GAT_NODE_UNTESTED) return true;
+  if (n.flags & GAT_NODE_INCLUSIVE) return n.box.x <= q.z && n.box.z >= q.x && n.box.y <= q.w && n.box.w >= q.y;
+  return box_intersect(q, n.box);
+}
+
+// MBB::pInBox (MBB.cpp:21-26): inclusive on all four edges.
+__device__ __forceinline__ bool in_box(const float4& b, float x, float y) {
+  return x <= b.z && x >= b.x && y <= b.w && y >= b.y;
+}
+
+// ---------------------------------------------------------------- fill
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------- selection
+// The stack holds node indices still to be tested.  Each step pops up to 64
+// entries from the top (one per lane), tests them, records selected leaves and
+// pushes the children of selected inner nodes in lane order.  Depth never
+// decreases from the bottom of the stack to its top, so each depth keeps at
+// most nchild * 64 entries and a tree of depth D needs (D + 1) * nchild * 64
+// slots; the step size is also cut so a push can never pass the end.
+// Selected leaves collect in an LDS buffer that is flushed to the global
+// (leaf, query) list with one atomic per flush, so a step costs one dependent
+// memory access (its node loads).
+constexpr uint32_t kEmitCap = 256;
+
+__global__ __launch_bounds__(kWave) void tree_select_kernel(
+    const Node* __restrict__ nodes, uint32_t nnodes, uint32_t nchild, const float4* __restrict__ boxes,
+    const uint32_t* __restrict__ order, uint32_t q0, uint32_t nq, uint32_t* __restrict__ node_qcount,
+    uint64_t* __restrict__ cand, uint2* __restrict__ lq, uint64_t lq_cap, unsigned long long* __restrict__ lq_total,
+    uint32_t* __restrict__ status) {
+  __shared__ uint32_t stack[GAT_TREE_STACK];
+  __shared__ uint32_t emitted[kEmitCap];
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t i = blockIdx.x; i < nq; i += gridDim.x) {
+    const uint32_t q = order ? uniform(order[i]) : q0 + i;
+    if (q - q0 >= nq) continue;  // an order entry outside the batch (uniform)
+    const float4 box = boxes[q];
+    uint64_t mine = 0;
+    uint32_t top = 1, nemit = 0;
+    auto flush = [&]() {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(lq_total, (unsigned long long)nemit);
+      base = uniform64(base);
+      for (uint32_t e = lane; e < nemit; e += kWave) {
+        const uint64_t at = base + e;
+        if (at < lq_cap) lq[at] = make_uint2(emitted[e], q);
+      }
+      nemit = 0;
+      __syncthreads();  // the buffer is reused only after every lane has read it
+    };
+    if (lane == 0) stack[0] = 0;
+    __syncthreads();
+    while (top > 0) {
+      uint32_t m = top < kWave ? top : kWave;
+      const uint32_t room = (GAT_TREE_STACK - top) / (nchild - 1);
+      if (m > room) m = room;
+      if (m == 0) {
+        if (lane == 0) atomicOr(status, GAT_STATUS_STACK);
+        break;
+      }
+      top -= m;
+      const bool live = lane < m;
+      const uint32_t idx = live ? stack[top + lane] : 0u;
+      __syncthreads();  // every lane has read its entry before pushes reuse the slots
+      bool emit = false, desc = false;
+      uint32_t cnt = 0, first = 0;
+      if (live && idx < nnodes) {
+        const Node n = nodes[idx];
+        if (node_selected(n, box)) {
+          if (n.first_child < 0) {
+            emit = n.count > 0;
+            cnt = n.count;
+          } else if (uint32_t(n.first_child) + nchild <= nnodes) {
+            desc = true;
+            first = uint32_t(n.first_child);
+          }
+        }
+      }
+      const uint64_t em = __ballot(emit);
+      if (emit) {
+        emitted[nemit + lanes_below(em)] = idx;
+        atomicAdd(node_qcount + idx, 1u);
+        mine += cnt;
+      }
+      nemit += uint32_t(__popcll(em));
+      const uint64_t dm = __ballot(desc);
+      if (desc) {
+        const uint32_t at = top + lanes_below(dm) * nchild;
+        for (uint32_t c = 0; c < nchild; ++c) stack[at + c] = first + c;
+      }
+      top += uint32_t(__popcll(dm)) * nchild;
+      __syncthreads();
+      if (nemit > kEmitCap - kWave) flush();
+    }
+    if (nemit > 0) flush();
+    for (int off = kWave / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+    if (lane == 0) cand[q] = mine;
+  }
+}
+
+// ---------------------------------------------------------------- scans
+// Reduce-then-scan over tiles of 4096 elements: per-tile sums, a carried
+// exclusive scan of the tile sums in one workgroup, then a per-tile scan.
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+struct V2 {
+  uint32_t x, y;
+};
+__device__ __forceinline__ uint64_t vadd(uint64_t a, uint64_t b) { return a + b; }
+__device__ __forceinline__ V2 vadd(V2 a, V2 b) { return V2{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ uint64_t vzero(uint64_t) { return 0; }
+__device__ __forceinline__ V2 vzero(V2) { return V2{0, 0}; }
+__device__ __forceinline__ uint64_t vshfl_up(uint64_t v, int d) { return __shfl_up(v, d); }
+__device__ __forceinline__ V2 vshfl_up(V2 v, int d) { return V2{__shfl_up(v.x, d), __shfl_up(v.y, d)}; }
+
+struct LoadU64 {
+  const uint64_t* in;
+  using T = uint64_t;
+  __device__ T operator()(uint64_t i) const { return in[i]; }
+};
+struct LoadLists {
+  const uint32_t* count;
+  uint32_t qb;
+  using T = V2;
+  __device__ T operator()(uint64_t i) const {
+    const uint32_t c = count[i];
+    return V2{c, (c + qb - 1) / qb};
+  }
+};
+
+// Exclusive scan of one value per thread across the workgroup; returns the
+// thread's prefix and sets *sum to the workgroup total.
+template <class T>
+__device__ T block_exclusive(T v, T* sum) {
+  __shared__ T wave_sums[kScanBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  T inc = v;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const T o = vshfl_up(inc, d);
+    if (lane >= d) inc = vadd(inc, o);
+  }
+  if (lane == kWave - 1) wave_sums[w] = inc;
+  __syncthreads();
+  T before = vzero(v), all = vzero(v);
+  for (int k = 0; k < kScanBlock / kWave; ++k) {
+    if (k < w) before = vadd(before, wave_sums[k]);
+    all = vadd(all, wave_sums[k]);
+  }
+  __syncthreads();
+  *sum = all;
+  T excl = vshfl_up(inc, 1);
+  if (lane == 0) excl = vzero(v);
+  return vadd(before, excl);
+}
+
+template <class L>
+__global__ __launch_bounds__(kScanBlock) void scan_tiles_kernel(L load, uint64_t n, typename L::T* __restrict__ tiles) {
+  using T = typename L::T;
+  const uint64_t b = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+  T s = vzero(T{});
+  for (int k = 0; k < kScanItems; ++k)
+    if (b + k < n) s = vadd(s, load(b + k));
+  T all;
+  block_exclusive(s, &all);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = all;
+}
+
+template <class T>
+__global__ __launch_bounds__(kScanBlock) void scan_carry_kernel(T* __restrict__ tiles, uint64_t ntiles,
+                                                                T* __restrict__ total) {
+  T carry = vzero(T{});
+  for (uint64_t base = 0; base < ntiles; base += kScanTile) {
+    const uint64_t b = base + uint64_t(threadIdx.x) * kScanItems;
+    T v[kScanItems];
+    T s = vzero(T{});
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = b + k < ntiles ? tiles[b + k] : vzero(T{});
+      s = vadd(s, v[k]);
+    }
+    T all;
+    T run = vadd(carry, block_exclusive(s, &all));
+    for (int k = 0; k < kScanItems; ++k)
+      if (b + k < ntiles) {
+        tiles[b + k] = run;
+        run = vadd(run, v[k]);
+      }
+    carry = vadd(carry, all);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+template <class L>
+__global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(L load, uint64_t n,
+                                                                const typename L::T* __restrict__ tiles,
+                                                                typename L::T* __restrict__ out) {
+  using T = typename L::T;
+  const uint64_t b = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+  T v[kScanItems];
+  T s = vzero(T{});
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = b + k < n ? load(b + k) : vzero(T{});
+    s = vadd(s, v[k]);
+  }
+  T all;
+  T run = vadd(tiles[blockIdx.x], block_exclusive(s, &all));
+  for (int k = 0; k < kScanItems; ++k)
+    if (b + k < n) {
+      out[b + k] = run;
+      run = vadd(run, v[k]);
+    }
+}
+
+template <class L>
+int launch_scan(L load, uint64_t n, typename L::T* tiles, typename L::T* out, typename L::T* total, hipStream_t s) {
+  using T = typename L::T;
+  const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+  if (ntiles > 0) {
+    scan_tiles_kernel<L><<<dim3{uint32_t(ntiles)}, dim3{kScanBlock}, 0, s>>>(load, n, tiles);
+    GAT_LAUNCH_CHECK("scan_tiles_kernel");
+  }
+  scan_carry_kernel<T><<<dim3{1}, dim3{kScanBlock}, 0, s>>>(tiles, ntiles, total);
+  GAT_LAUNCH_CHECK("scan_carry_kernel");
+  if (ntiles > 0) {
+    scan_apply_kernel<L><<<dim3{uint32_t(ntiles)}, dim3{kScanBlock}, 0, s>>>(load, n, tiles, out);
+    GAT_LAUNCH_CHECK("scan_apply_kernel");
+  }
+  return GAT_OK;
+}
+
+// ---------------------------------------------------------------- grouping
+// Entries are grouped by leaf into self-contained records, so the refine
+// needs no further lookups: a list entry carries the rectangle, its output
+// region and id; a work item carries the leaf's store run and its slice of
+// the leaf's list.
+struct ListEntry {
+  float4 box;
+  uint64_t base;
+  uint32_t q, pad;
+};
+static_assert(sizeof(ListEntry) == 32, "list entry layout");
+struct Item {
+  uint32_t start, count, list, n;
+};
+static_assert(sizeof(Item) == 16, "item layout");
+
+__global__ void leaf_scatter_kernel(const uint2* __restrict__ lq, const unsigned long long* __restrict__ lq_total,
+                                    uint64_t lq_cap, const Node* __restrict__ nodes, uint32_t nnodes,
+                                    const uint2* __restrict__ node_off, const uint32_t* __restrict__ node_qcount,
+                                    uint32_t* __restrict__ node_fill, uint32_t qb, const float4* __restrict__ boxes,
+                                    const uint64_t* __restrict__ qbase, uint32_t nqueries,
+                                    ListEntry* __restrict__ qlist, uint64_t qlist_cap, Item* __restrict__ items,
+                                    uint64_t items_cap) {
+  const uint64_t total = *lq_total;
+  const uint64_t n = total < lq_cap ? total : lq_cap;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint2 e = lq[i];
+    if (e.x >= nnodes || e.y >= nqueries) continue;
+    const uint32_t r = atomicAdd(node_fill + e.x, 1u);
+    const uint2 off = node_off[e.x];
+    const uint64_t at = uint64_t(off.x) + r;
+    if (at < qlist_cap) {
+      ListEntry le;
+      le.box = boxes[e.y];
+      le.base = qbase[e.y];
+      le.q = e.y;
+      le.pad = 0;
+      qlist[at] = le;
+    }
+    if (r % qb == 0) {
+      const uint64_t it = uint64_t(off.y) + r / qb;
+      const uint32_t len = node_qcount[e.x];
+      if (it < items_cap && r < len) {
+        const uint32_t left = len - r;
+        items[it] = Item{nodes[e.x].start, nodes[e.x].count, uint32_t(at), left < qb ? left : qb};
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- refine
+// One wave per work item: lane j loads list entry j of the item (rectangle,
+// output region, id) while the leaf's points and ids load; then per chunk of
+// up to 256 points: pass 1 counts each rectangle's hits, one vector atomic
+// reserves every rectangle's output slots at once, pass 2 recomputes the same
+// ballots and writes the pairs.  The next item's record is fetched while the
+// current one is processed, so an item costs two dependent memory accesses
+// (its entries and points, then the atomics) whatever its number of
+// rectangles.
+constexpr int kRefBlock = 256;
+constexpr int kSlots = 4;
+constexpr int kSpan = kWave * kSlots;
+constexpr int kMaxQb = kWave;
+
+__device__ __forceinline__ float readlane_f(float v, uint32_t j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), int(j)));
+}
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t j) {
+  return uint32_t(__builtin_amdgcn_readlane(int(v), int(j)));
+}
+
+template <bool kPacked>
+__global__ __launch_bounds__(kRefBlock) void leaf_refine_kernel(
+    Store store, const Item* __restrict__ items, const uint32_t* __restrict__ item_total, uint64_t items_cap,
+    const ListEntry* __restrict__ qlist, uint64_t qlist_cap, uint32_t nqueries, uint32_t* __restrict__ qcount,
+    int2* __restrict__ pairs, uint64_t capacity) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = uniform((blockIdx.x * kRefBlock + threadIdx.x) / kWave);
+  const uint64_t nwaves = uint64_t(gridDim.x) * (kRefBlock / kWave);
+  const uint64_t total = *item_total;
+  const uint64_t nitems = total < items_cap ? total : items_cap;
+  const uint32_t pid_mask = (1u << store.pid_bits) - 1u;
+  const float nan = __builtin_nanf("");
+  Item next = wave < nitems ? items[wave] : Item{0, 0, 0, 0};
+  for (uint64_t t = wave; t < nitems; t += nwaves) {
+    const uint32_t start = uniform(next.start), count = uniform(next.count);
+    const uint32_t list = uniform(next.list), nql = uniform(next.n);
+    if (t + nwaves < nitems) next = items[t + nwaves];
+    if (uint64_t(start) + count > store.npoints || nql == 0 || nql > kMaxQb || uint64_t(list) + nql > qlist_cap)
+      continue;
+    // Lane j: rectangle j of the item.  Invalid ones get a NaN box (no hits).
+    uint32_t qj = 0xFFFFFFFFu;
+    float4 bj = make_float4(nan, nan, nan, nan);
+    uint64_t basej = 0;
+    if (lane < nql) {
+      const ListEntry e = qlist[uint64_t(list) + lane];
+      if (e.q < nqueries) {
+        qj = e.q;
+        bj = e.box;
+        basej = e.base;
+      }
+    }
+    for (uint32_t base = 0; base < count; base += kSpan) {
+      const uint32_t left = count - base;
+      float2 v[kSlots];
+      uint32_t id[kSlots];
+      uint32_t pid[kSlots];
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) {
+        const uint32_t i = k * kWave + lane;
+        v[k] = make_float2(nan, 0.0f);
+        id[k] = 0;
+        pid[k] = 0;
+        if (uint32_t(k * kWave) < left && i < left) {
+          const uint64_t at = uint64_t(start) + base + i;
+          v[k] = store.xy[at];
+          id[k] = store.id[at];
+          if constexpr (!kPacked) pid[k] = store.pid[at];
+        }
+      }
+      // Pass 1: hits per rectangle, kept in lane j.
+      uint32_t myhits = 0;
+      for (uint32_t j = 0; j < nql; ++j) {
+        const float4 box = make_float4(readlane_f(bj.x, j), readlane_f(bj.y, j), readlane_f(bj.z, j),
+                                       readlane_f(bj.w, j));
+        uint32_t h = 0;
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k)
+          if (uint32_t(k * kWave) < left) h += uint32_t(__popcll(__ballot(in_box(box, v[k].x, v[k].y))));
+        if (lane == j) myhits = h;
+      }
+      // One atomic per rectangle with hits, all issued together.
+      uint32_t slot = 0;
+      if (myhits > 0) slot = atomicAdd(qcount + qj, myhits);
+      const uint64_t firstj = basej + slot;
+      // Pass 2: the same ballots, now written out.
+      for (uint32_t j = 0; j < nql; ++j) {
+        if (readlane_u(myhits, j) == 0) continue;
+        const float4 box = make_float4(readlane_f(bj.x, j), readlane_f(bj.y, j), readlane_f(bj.z, j),
+                                       readlane_f(bj.w, j));
+        const uint64_t first =
+            uint64_t(readlane_u(uint32_t(firstj), j)) | (uint64_t(readlane_u(uint32_t(firstj >> 32), j)) << 32);
+        uint32_t before = 0;
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) {
+          if (uint32_t(k * kWave) < left) {
+            const uint64_t mask = __ballot(in_box(box, v[k].x, v[k].y));
+            const uint64_t at = first + before + lanes_below(mask);
+            if (((mask >> lane) & 1) && at < capacity) {
+              int2 p;
+              if constexpr (kPacked) {
+                p = make_int2(int(id[k] >> store.pid_bits), int(id[k] & pid_mask));
+              } else {
+                p = make_int2(int(id[k]), int(pid[k]));
+              }
+              pairs[at] = p;
+            }
+            before += uint32_t(__popcll(mask));
+          }
+        }
+      }
+    }
+  }
+}
+
+// Sum of n uint32 values into *out (which must start at zero).
+__global__ void sum_u32_kernel(const uint32_t* __restrict__ in, uint64_t n, unsigned long long* __restrict__ out) {
+  uint64_t s = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    s += in[i];
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & (kWave - 1)) == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+
+int grid_for(uint64_t work, int per_block, int cap_per_cu) {
+  uint64_t blocks = (work + per_block - 1) / per_block;
+  const uint64_t cap = uint64_t(gat::cu_count()) * cap_per_cu;
+  if (blocks > cap) blocks = cap;
+  if (blocks == 0) blocks = 1;
+  return int(blocks);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gat_fill_u32(uint32_t* dst, uint32_t value, int64_t n, void* stream) {
+  GAT_REQUIRE(n >= 0 && (n == 0 || dst), "gat_fill_u32: bad arguments");
+  if (n == 0) return GAT_OK;
+  fill_u32_kernel<<<dim3{uint32_t(grid_for(uint64_t(n), 256, 8))}, dim3{256}, 0, gat::as_stream(stream)>>>(
+      dst, value, uint64_t(n));
+  GAT_LAUNCH_CHECK("fill_u32_kernel");
+  return GAT_OK;
+}
+
+int64_t gat_scan_tiles(int64_t n) { return n <= 0 ? 1 : (n + kScanTile - 1) / kScanTile; }
+
+int gat_scan_u64(const uint64_t* in, int64_t n, uint64_t* out, uint64_t* total, uint64_t* tiles, void* stream) {
+  GAT_REQUIRE(n >= 0 && total && tiles && (n == 0 || (in && out)), "gat_scan_u64: bad arguments");
+  GAT_REQUIRE(uint64_t(n) / kScanTile < UINT32_MAX, "gat_scan_u64: too many elements");
+  return launch_scan(LoadU64{in}, uint64_t(n), tiles, out, total, gat::as_stream(stream));
+}
+
+int gat_scan_leaf_lists(const uint32_t* count, int64_t nnodes, int32_t qb, uint32_t* node_off, uint32_t* totals,
+                        uint32_t* tiles, void* stream) {
+  GAT_REQUIRE(nnodes >= 0 && qb > 0 && totals && tiles && (nnodes == 0 || (count && node_off)),
+              "gat_scan_leaf_lists: bad arguments");
+  GAT_REQUIRE(uint64_t(nnodes) / kScanTile < UINT32_MAX, "gat_scan_leaf_lists: too many nodes");
+  return launch_scan(LoadLists{count, uint32_t(qb)}, uint64_t(nnodes), reinterpret_cast<V2*>(tiles),
+                     reinterpret_cast<V2*>(node_off), reinterpret_cast<V2*>(totals), gat::as_stream(stream));
+}
+
+int gat_tree_select(const gat_tree_node* nodes, int64_t nnodes, int32_t nchild, const float* boxes,
+                    const uint32_t* order, int64_t q0, int64_t nq, uint32_t* node_qcount, uint64_t* cand,
+                    uint32_t* leaf_query, int64_t lq_capacity, uint64_t* lq_total, uint32_t* status, void* stream) {
+  GAT_REQUIRE(nnodes >= 1 && nnodes < int64_t(INT32_MAX) && nodes, "gat_tree_select: bad tree");
+  GAT_REQUIRE(nchild >= 2 && nchild <= 8, "gat_tree_select: nchild must be 2..8");
+  GAT_REQUIRE(q0 >= 0 && nq >= 0 && q0 + nq <= int64_t(UINT32_MAX), "gat_tree_select: bad query range");
+  GAT_REQUIRE(lq_capacity >= 0 && lq_total && status && node_qcount, "gat_tree_select: bad outputs");
+  if (nq == 0) return GAT_OK;
+  GAT_REQUIRE(boxes && cand && (lq_capacity == 0 || leaf_query), "gat_tree_select: null pointer");
+  const int blocks = grid_for(uint64_t(nq), 1, 40);
+  tree_select_kernel<<<dim3{uint32_t(blocks)}, dim3{kWave}, 0, gat::as_stream(stream)>>>(
+      reinterpret_cast<const Node*>(nodes), uint32_t(nnodes), uint32_t(nchild), reinterpret_cast<const float4*>(boxes),
+      order, uint32_t(q0), uint32_t(nq), node_qcount, cand, reinterpret_cast<uint2*>(leaf_query), uint64_t(lq_capacity),
+      reinterpret_cast<unsigned long long*>(lq_total), status);
+  GAT_LAUNCH_CHECK("tree_select_kernel");
+  return GAT_OK;
+}
+
+int gat_leaf_scatter(const uint32_t* leaf_query, const uint64_t* lq_total, int64_t lq_capacity,
+                     const gat_tree_node* nodes, int64_t nnodes, const uint32_t* node_off,
+                     const uint32_t* node_qcount, uint32_t* node_fill, int32_t qb, const float* boxes,
+                     const uint64_t* qbase, int64_t nqueries, void* qlist, int64_t qlist_capacity, uint32_t* items,
+                     int64_t items_capacity, void* stream) {
+  GAT_REQUIRE(lq_total && nodes && node_off && node_qcount && node_fill && qlist && items && boxes && qbase &&
+                  qb > 0 && qb <= kMaxQb && nnodes >= 1 && nnodes < int64_t(INT32_MAX),
+              "gat_leaf_scatter: bad arguments");
+  GAT_REQUIRE(lq_capacity >= 0 && qlist_capacity >= 0 && qlist_capacity <= int64_t(UINT32_MAX) &&
+                  items_capacity >= 0 && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX),
+              "gat_leaf_scatter: bad capacity");
+  if (lq_capacity == 0) return GAT_OK;
+  GAT_REQUIRE(leaf_query, "gat_leaf_scatter: null entries");
+  leaf_scatter_kernel<<<dim3{uint32_t(grid_for(uint64_t(lq_capacity), 256, 8))}, dim3{256}, 0,
+                        gat::as_stream(stream)>>>(
+      reinterpret_cast<const uint2*>(leaf_query), reinterpret_cast<const unsigned long long*>(lq_total),
+      uint64_t(lq_capacity), reinterpret_cast<const Node*>(nodes), uint32_t(nnodes),
+      reinterpret_cast<const uint2*>(node_off), node_qcount, node_fill, uint32_t(qb),
+      reinterpret_cast<const float4*>(boxes), qbase, uint32_t(nqueries), static_cast<ListEntry*>(qlist),
+      uint64_t(qlist_capacity), reinterpret_cast<Item*>(items), uint64_t(items_capacity));
+  GAT_LAUNCH_CHECK("leaf_scatter_kernel");
+  return GAT_OK;
+}
+
+int gat_leaf_refine(const gat_point_store* sp, const uint32_t* items, const uint32_t* item_total,
+                    int64_t items_capacity, const void* qlist, int64_t qlist_capacity, int64_t nqueries,
+                    uint32_t* qcount, int32_t* pairs, int64_t capacity, uint64_t* pair_total, void* stream) {
+  GAT_REQUIRE(sp && sp->npoints >= 0 && sp->pid_bits >= 0 && sp->pid_bits < 32, "gat_leaf_refine: bad store");
+  GAT_REQUIRE(sp->npoints == 0 || (sp->xy_dev && sp->id_dev && (sp->pid_bits > 0 || sp->pid_dev)),
+              "gat_leaf_refine: null store array");
+  GAT_REQUIRE(item_total && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX) && items_capacity >= 0 &&
+                  qlist_capacity >= 0 && capacity >= 0,
+              "gat_leaf_refine: bad sizes");
+  hipStream_t s = gat::as_stream(stream);
+  if (items_capacity > 0 && nqueries > 0) {
+    GAT_REQUIRE(items && qlist && qcount && pairs, "gat_leaf_refine: null pointer");
+    const Store st{reinterpret_cast<const float2*>(sp->xy_dev), sp->id_dev, sp->pid_dev, uint64_t(sp->npoints),
+                   uint32_t(sp->pid_bits)};
+    const int blocks = grid_for(uint64_t(items_capacity), kRefBlock / kWave, 16);
+    const dim3 grid{uint32_t(blocks)}, block{kRefBlock};
+    auto* I = reinterpret_cast<const Item*>(items);
+    auto* L = static_cast<const ListEntry*>(qlist);
+    auto* P = reinterpret_cast<int2*>(pairs);
+    if (sp->pid_bits > 0)
+      leaf_refine_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
+                                                      uint64_t(qlist_capacity), uint32_t(nqueries), qcount, P,
+                                                      uint64_t(capacity));
+    else
+      leaf_refine_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
+                                                       uint64_t(qlist_capacity), uint32_t(nqueries), qcount, P,
+                                                       uint64_t(capacity));
+    GAT_LAUNCH_CHECK("leaf_refine_kernel");
+  }
+  if (pair_total && nqueries > 0) {
+    GAT_REQUIRE(qcount, "gat_leaf_refine: null counts");
+    sum_u32_kernel<<<dim3{uint32_t(grid_for(uint64_t(nqueries), 256, 1))}, dim3{256}, 0, s>>>(
+        qcount, uint64_t(nqueries), reinterpret_cast<unsigned long long*>(pair_total));
+    GAT_LAUNCH_CHECK("sum_u32_kernel");
+  }
+  return GAT_OK;
+}
+
+int gat_graph_begin(void* stream) {
+  GAT_HIP_CHECK(hipStreamBeginCapture(gat::as_stream(stream), hipStreamCaptureModeThreadLocal));
+  return GAT_OK;
+}
+
+int gat_graph_end(void* stream, void** graph_exec) {
+  GAT_REQUIRE(graph_exec, "gat_graph_end: null output");
+  hipGraph_t g = nullptr;
+  GAT_HIP_CHECK(hipStreamEndCapture(gat::as_stream(stream), &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  const hipError_t err_destroy = hipGraphDestroy(g);
+  if (err != hipSuccess) return gat::hip_fail(err, "hipGraphInstantiate", __FILE__, __LINE__);
+  if (err_destroy != hipSuccess) {
+    (void)hipGraphExecDestroy(e);
+    return gat::hip_fail(err_destroy, "hipGraphDestroy", __FILE__, __LINE__);
+  }
+  *graph_exec = e;
+  return GAT_OK;
+}
+
+int gat_graph_launch(void* graph_exec, void* stream) {
+  GAT_REQUIRE(graph_exec, "gat_graph_launch: null graph");
+  GAT_HIP_CHECK(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), gat::as_stream(stream)));
+  return GAT_OK;
+}
+
+int gat_graph_destroy(void* graph_exec) {
+  if (graph_exec) GAT_HIP_CHECK(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
+  return GAT_OK;
+}
+
+}  // extern "C"

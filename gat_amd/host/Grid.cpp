@@ -155,6 +155,9 @@ int Grid::addDatasetToGrid(const TrajectoryView& db, bool upload) {
 
   if (!upload) return GAT_OK;
   if (int rc = engine_.upload(xy.data(), tp.data(), N)) return rc;
+  int depth = 0;
+  const std::vector<gat_tree_node> tn = treeNodes(&depth);
+  if (int rc = tree_.upload(tn, 4, depth, engine_.stream())) return rc;
   return uploadTrajectories(db);
 }
 
@@ -180,6 +183,52 @@ int Grid::buildNode(int level, int id) {
     }
   }
   return me;
+}
+
+std::vector<gat_tree_node> Grid::treeNodes(int* depth) const {
+  std::vector<gat_tree_node> out;
+  *depth = 0;
+  if (nodes.empty()) return out;
+  // Breadth-first order: the four children of a node are appended together,
+  // so they are contiguous and keep the UL, UR, DL, DR order.
+  std::vector<int> order{0};
+  for (size_t h = 0; h < order.size(); ++h) {
+    const QuadtreeNode& n = nodes[size_t(order[h])];
+    if (!n.isLeaf)
+      for (int k = 0; k < 4; ++k) order.push_back(n.child[k]);
+  }
+  std::vector<int32_t> pos(nodes.size(), -1);
+  for (size_t i = 0; i < order.size(); ++i) pos[size_t(order[i])] = int32_t(i);
+  out.resize(order.size());
+  for (size_t i = 0; i < order.size(); ++i) {
+    const QuadtreeNode& n = nodes[size_t(order[i])];
+    gat_tree_node& d = out[i];
+    d.box[0] = n.mbb.xmin;
+    d.box[1] = n.mbb.ymin;
+    d.box[2] = n.mbb.xmax;
+    d.box[3] = n.mbb.ymax;
+    d.first_child = n.isLeaf ? -1 : pos[size_t(n.child[0])];
+    d.start = n.isLeaf ? uint32_t(n.pointStart) : 0u;
+    d.count = n.isLeaf ? uint32_t(n.numPoints) : 0u;
+    d.flags = i == 0 ? GAT_NODE_UNTESTED : 0u;  // the root is taken as it is
+    *depth = std::max(*depth, n.level);
+  }
+  return out;
+}
+
+int Grid::rangeQueryBatchedGPU(const MBB* bounds, int rangeNum, RangeBatch& out, bool graph) {
+  if (rangeNum < 0 || (rangeNum > 0 && !bounds)) {
+    gath_set_error("Grid::rangeQueryBatchedGPU: bad arguments");
+    return GAT_ERR_ARG;
+  }
+  if (tree_.nnodes == 0) {
+    gath_set_error("Grid::rangeQueryBatchedGPU: no device tree (build the index with upload)");
+    return GAT_ERR_STATE;
+  }
+  const int rc = engine_.runTree(tree_, bounds, rangeNum, out, graph);
+  lastSelectMs = out.selectMs;
+  lastKernelMs = out.kernelMs;
+  return rc;
 }
 
 // Grid.cpp:428-446, iteratively: the root is taken as it is; every other node

@@ -68,6 +68,15 @@ class Grid {
                          int device_idx = 0);
   // The same selection refined into compacted (tid, pid) pairs left in HBM.
   int rangeQueryBatchGPUPairs(const MBB* bounds, int rangeNum, RangeBatch& out);
+  // Batched form (no reference counterpart): the rectangles are selected on the
+  // device against a breadth-first copy of the quadtree and refined leaf by
+  // leaf (RangeEngine::runTree).  Same result sets as rangeQueryBatchGPUPairs.
+  int rangeQueryBatchedGPU(const MBB* bounds, int rangeNum, RangeBatch& out, bool graph = false);
+  // Points of the leaves the last batched run selected at least once.
+  int uniqueCandidates(int64_t* points) { return engine_.uniqueCandidates(tree_, points); }
+  // The quadtree as breadth-first gat_tree_nodes (the root untested, every
+  // other node tested with BoxIntersect != 0); *depth = deepest level.
+  std::vector<gat_tree_node> treeNodes(int* depth) const;
   // Grid.cpp:428-446: the leaves one rectangle selects, in traversal order.
   void findMatchNodeInQuadTree(const MBB& bound, std::vector<int>* leaves) const;
   // Host candidate selection for a batch (tasks + per-query candidate counts).
@@ -122,6 +131,7 @@ class Grid {
   int runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* stats);
 
   RangeEngine engine_;
+  DeviceTree tree_;
   // Trajectory-ordered copy of the points for EDR (the reference reassembles
   // trajectories from the cell-ordered store per query, Grid.cpp:1500-1508).
   // Stores are NaN-padded (GAT_EDR_PAD points before every trajectory,

@@ -7,6 +7,7 @@
 #include <numeric>
 
 #include "Grid.h"
+#include "Parallel.h"
 
 namespace {
 
@@ -324,40 +325,64 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
   const double h0 = now_ms();
   // Per query: the survivors with their exact distances, and every other
   // trajectory at the all-mismatch value of its length, merged by (distance, id).
-  std::vector<std::vector<std::pair<int32_t, int32_t>>> surv(static_cast<size_t>(nq));  // (dist, id)
-  for (size_t i = 0; i < count; ++i) surv[pairs[2 * i]].push_back({dist[i], pairs[2 * i + 1] + 1});
+  std::vector<int64_t> start(size_t(nq) + 1, 0);
+  for (size_t i = 0; i < count; ++i) ++start[size_t(pairs[2 * i]) + 1];
+  for (int q = 0; q < nq; ++q) start[q + 1] += start[q];
+  std::vector<std::pair<int32_t, int32_t>> surv(count);  // (dist, id), grouped by query
+  {
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (size_t i = 0; i < count; ++i) surv[size_t(fill[pairs[2 * i]]++)] = {dist[i], pairs[2 * i + 1] + 1};
+  }
+  std::vector<int32_t> lens;  // lengths present in the index
+  for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L)
+    if (!lengthIds_[L].empty()) lens.push_back(L);
   const int w = GAT_MAX_TRAJ_LEN + 1;
   out.assign(size_t(nq), TopK{});
-  for (int q = 0; q < nq; ++q) {
-    auto& sv = surv[q];
-    std::sort(sv.begin(), sv.end());
+  parallel_for(nq, threads(), [&](int q) {
+    auto sb = surv.begin() + start[q], se = surv.begin() + start[q + 1];
+    std::sort(sb, se);
     std::vector<int32_t> excluded;
-    excluded.reserve(sv.size());
-    for (auto& e : sv) excluded.push_back(e.second);
+    excluded.reserve(size_t(se - sb));
+    for (auto it = sb; it != se; ++it) excluded.push_back(it->second);
     std::sort(excluded.begin(), excluded.end());
     const int a = queryLenHost_[q];
-    // Lengths present in the index, by all-mismatch value.
-    std::vector<std::pair<int32_t, int32_t>> byF;  // (F value, length)
-    for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L)
-      if (!lengthIds_[L].empty()) byF.push_back({F[size_t(std::min(a, L)) * w + std::max(a, L)], L});
+    std::vector<std::pair<int32_t, int32_t>> byF;  // (all-mismatch value, length)
+    byF.reserve(lens.size());
+    for (int L : lens) byF.push_back({F[size_t(std::min(a, L)) * w + std::max(a, L)], L});
     std::sort(byF.begin(), byF.end());
+    const size_t need = size_t(kValue);  // the k best non-survivors suffice
     std::vector<std::pair<int32_t, int32_t>> rest;  // (dist, id), ascending
-    const size_t need = size_t(kValue) + sv.size();
+    rest.reserve(need);
+    struct Cursor {
+      const int32_t* p;
+      const int32_t* e;
+    };
+    auto later = [](const Cursor& x, const Cursor& y) { return *x.p > *y.p; };
+    std::vector<Cursor> heap;
     for (size_t g = 0; g < byF.size() && rest.size() < need;) {
+      // One value of F: k-way merge of its lengths' ascending id lists.
+      heap.clear();
       size_t h = g;
-      std::vector<int32_t> ids;
-      while (h < byF.size() && byF[h].first == byF[g].first) {
-        for (int32_t id : lengthIds_[byF[h].second])
-          if (!std::binary_search(excluded.begin(), excluded.end(), id)) ids.push_back(id);
-        ++h;
+      for (; h < byF.size() && byF[h].first == byF[g].first; ++h) {
+        const auto& ids = lengthIds_[byF[h].second];
+        heap.push_back({ids.data(), ids.data() + ids.size()});
       }
-      std::sort(ids.begin(), ids.end());
-      for (int32_t id : ids) rest.push_back({byF[g].first, id});
+      std::make_heap(heap.begin(), heap.end(), later);
+      while (!heap.empty() && rest.size() < need) {
+        std::pop_heap(heap.begin(), heap.end(), later);
+        Cursor& c = heap.back();
+        const int32_t id = *c.p++;
+        if (!std::binary_search(excluded.begin(), excluded.end(), id)) rest.push_back({byF[g].first, id});
+        if (c.p == c.e)
+          heap.pop_back();
+        else
+          std::push_heap(heap.begin(), heap.end(), later);
+      }
       g = h;
     }
     std::vector<std::pair<int32_t, int32_t>> merged;
-    merged.reserve(size_t(kValue));
-    std::merge(sv.begin(), sv.end(), rest.begin(), rest.end(), std::back_inserter(merged));
+    merged.reserve(size_t(se - sb) + rest.size());
+    std::merge(sb, se, rest.begin(), rest.end(), std::back_inserter(merged));
     const size_t kk = std::min(size_t(kValue), merged.size());
     out[q].dist.resize(kk);
     out[q].ids.resize(kk);
@@ -365,7 +390,7 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
       out[q].dist[i] = merged[i].first;
       out[q].ids[i] = merged[i].second;
     }
-  }
+  });
   if (st) {
     st->filterMs += fms;
     st->pruned += int64_t(nq) * trajNum - int64_t(count);

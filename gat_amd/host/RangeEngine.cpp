@@ -139,6 +139,7 @@ int RangeEngine::stage(const MBB* bounds, int nq, const std::vector<gat_range_ta
 int RangeEngine::runPairs(const MBB* bounds, int nq, const Selector& selector, int threads, RangeBatch& out) {
   if (int rc = stream_.create()) return rc;
   out.queryNum = nq;
+  out.hostStale = false;
   double t0 = now_ms();
   std::vector<gat_range_task> tasks;
   std::vector<uint64_t> cand;
@@ -201,7 +202,19 @@ int RangeEngine::runBitmap(const MBB* bounds, int nq, const Selector& selector, 
   return GAT_OK;
 }
 
-int RangeBatch::fetch(int32_t* out, int64_t capacityPairs) const {
+int RangeBatch::syncHost() {
+  if (!hostStale) return GAT_OK;
+  count.resize(size_t(queryNum));
+  base.resize(size_t(queryNum));
+  GATH_TRY(gat_memcpy_d2h(count.data(), qcount.get(), size_t(queryNum) * 4, nullptr), "count readback");
+  GATH_TRY(gat_memcpy_d2h(base.data(), qbase.get(), size_t(queryNum) * 8, nullptr), "region readback");
+  GATH_TRY(gat_device_sync(), "count readback");
+  hostStale = false;
+  return GAT_OK;
+}
+
+int RangeBatch::fetch(int32_t* out, int64_t capacityPairs) {
+  if (int rc = syncHost()) return rc;
   // Query regions are laid out back to back by candidate count; copy each
   // query's filled prefix to its slot in `out`.
   int64_t need = queryNum ? int64_t(base.back()) + int64_t(count.back()) : 0;
@@ -227,5 +240,279 @@ int RangeBatch::digests(uint64_t* out) const {
            "pair digest");
   GATH_TRY(gat_memcpy_d2h(out, d.get(), size_t(queryNum) * 8, nullptr), "digest readback");
   GATH_TRY(gat_device_sync(), "digest readback");
+  return GAT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form.
+
+int DeviceTree::upload(const std::vector<gat_tree_node>& host, int nc, int d, void* stream) {
+  if (host.empty() || nc < 2) {
+    gath_set_error("device tree: empty tree or fewer than two children per node");
+    return GAT_ERR_ARG;
+  }
+  // A traversal keeps at most nchild * 64 stack entries per depth level.
+  if (int64_t(d + 1) * nc * 64 > GAT_TREE_STACK) {
+    gath_set_error("device tree: depth %d with %d children exceeds the traversal stack", d, nc);
+    return GAT_ERR_ARG;
+  }
+  if (int rc = nodes.reserve(host.size() * sizeof(gat_tree_node))) return rc;
+  GATH_TRY(gat_memcpy_h2d(nodes.get(), host.data(), host.size() * sizeof(gat_tree_node), stream), "tree upload");
+  GATH_TRY(gat_stream_sync(stream), "tree upload");
+  nnodes = int64_t(host.size());
+  nchild = nc;
+  depth = d;
+  return GAT_OK;
+}
+
+RangeEngine::~RangeEngine() { dropGraph(); }
+
+void RangeEngine::dropGraph() {
+  if (graph_) gat_graph_destroy(graph_);
+  graph_ = nullptr;
+  graphKey_ = GraphKey{};
+}
+
+// Uploads the rectangles and the order the selection takes them in: largest
+// area first, so the longest traversals start first and the tail is short.
+int RangeEngine::uploadBoxes(const MBB* bounds, int nq) {
+  static_assert(sizeof(MBB) == 16, "MBB must be four packed floats");
+  const size_t box_bytes = align_up(size_t(nq) * 16, 256);
+  const size_t bytes = box_bytes + size_t(nq) * 4;
+  if (int rc = hostStage_.reserve(std::max<size_t>(bytes, 256))) return rc;
+  if (devBoxes_.size() < bytes) dropGraph();  // the captured pipeline reads the old buffer
+  if (int rc = devBoxes_.reserve(std::max<size_t>(bytes, 256))) return rc;
+  char* h = hostStage_.as<char>();
+  std::memcpy(h, bounds, size_t(nq) * 16);
+  // Counting sort on the binary exponent of the area, largest first: an order
+  // within a factor of two is all the scheduling needs, in O(n).
+  auto* order = reinterpret_cast<uint32_t*>(h + box_bytes);
+  std::vector<uint8_t> key(static_cast<size_t>(nq));
+  uint32_t start[257] = {};
+  for (int q = 0; q < nq; ++q) {
+    const float area =
+        std::max(0.0f, bounds[q].xmax - bounds[q].xmin) * std::max(0.0f, bounds[q].ymax - bounds[q].ymin);
+    uint32_t bits;
+    std::memcpy(&bits, &area, 4);
+    key[q] = uint8_t(255 - std::min<uint32_t>(bits >> 23, 255));  // NaN areas sort last
+    ++start[key[q] + 1];
+  }
+  for (int b = 0; b < 256; ++b) start[b + 1] += start[b];
+  for (int q = 0; q < nq; ++q) order[start[key[q]]++] = uint32_t(q);
+  GATH_TRY(gat_memcpy_h2d(devBoxes_.get(), h, bytes, stream_.get()), "box upload");
+  return GAT_OK;
+}
+
+int RangeEngine::enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out) {
+  void* s = stream_.get();
+  const int64_t N = tree.nnodes;
+  uint32_t* ctr = counters_.as<uint32_t>();
+  GATH_TRY(gat_fill_u32(nodeCount_.as<uint32_t>(), 0, N, s), "batch reset");
+  GATH_TRY(gat_fill_u32(nodeFill_.as<uint32_t>(), 0, N, s), "batch reset");
+  GATH_TRY(gat_fill_u32(out.qcount.as<uint32_t>(), 0, nq, s), "batch reset");
+  GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
+  const auto* order =
+      reinterpret_cast<const uint32_t*>(devBoxes_.as<char>() + align_up(size_t(nq) * 16, 256));
+  GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), N, tree.nchild, devBoxes_.as<float>(), order, 0, nq,
+                           nodeCount_.as<uint32_t>(), cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_,
+                           reinterpret_cast<uint64_t*>(ctr), ctr + 4, s),
+           "tree select");
+  GATH_TRY(gat_scan_u64(cand_.as<uint64_t>(), nq, out.qbase.as<uint64_t>(), reinterpret_cast<uint64_t*>(ctr + 2),
+                        tiles_.as<uint64_t>(), s),
+           "region scan");
+  GATH_TRY(gat_scan_leaf_lists(nodeCount_.as<uint32_t>(), N, queriesPerItem, nodeOff_.as<uint32_t>(), ctr + 5,
+                               tiles_.as<uint32_t>(), s),
+           "leaf list scan");
+  return GAT_OK;
+}
+
+int RangeEngine::enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out) {
+  void* s = stream_.get();
+  const int64_t N = tree.nnodes;
+  uint32_t* ctr = counters_.as<uint32_t>();
+  GATH_TRY(gat_leaf_scatter(lq_.as<uint32_t>(), reinterpret_cast<const uint64_t*>(ctr), lqCap_,
+                            tree.nodes.as<gat_tree_node>(), N, nodeOff_.as<uint32_t>(), nodeCount_.as<uint32_t>(),
+                            nodeFill_.as<uint32_t>(), queriesPerItem, devBoxes_.as<float>(),
+                            out.qbase.as<uint64_t>(), nq, qlist_.get(), qlistCap_, items_.as<uint32_t>(), itemsCap_,
+                            s),
+           "leaf scatter");
+  if (timeRefine_) {
+    if (int rc = refineTimer_.start(s)) return rc;
+  }
+  GATH_TRY(gat_leaf_refine(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.get(), qlistCap_, nq,
+                           out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
+                           reinterpret_cast<uint64_t*>(ctr + 8), s),
+           "leaf refine");
+  if (timeRefine_) {
+    if (int rc = refineTimer_.stop(s)) return rc;
+  }
+  return GAT_OK;
+}
+
+int RangeEngine::uniqueCandidates(const DeviceTree& tree, int64_t* points) {
+  const int64_t N = tree.nnodes;
+  if (N <= 0 || nodeCount_.size() < size_t(N) * 4) {
+    gath_set_error("uniqueCandidates: no batched run on this tree");
+    return GAT_ERR_STATE;
+  }
+  std::vector<uint32_t> cnt(static_cast<size_t>(N));
+  std::vector<gat_tree_node> nodes(static_cast<size_t>(N));
+  GATH_TRY(gat_memcpy_d2h(cnt.data(), nodeCount_.get(), size_t(N) * 4, stream_.get()), "leaf count readback");
+  GATH_TRY(gat_memcpy_d2h(nodes.data(), tree.nodes.get(), size_t(N) * sizeof(gat_tree_node), stream_.get()),
+           "tree readback");
+  if (int rc = stream_.sync()) return rc;
+  int64_t u = 0;
+  for (int64_t i = 0; i < N; ++i)
+    if (cnt[size_t(i)] > 0 && nodes[size_t(i)].first_child < 0) u += nodes[size_t(i)].count;
+  *points = u;
+  return GAT_OK;
+}
+
+// Copies the batch counters back; per-query counts and regions stay on the
+// device until RangeBatch::syncHost.
+int RangeEngine::readCounters(int nq, RangeBatch& out) {
+  GATH_TRY(gat_memcpy_d2h(countersHost_.get(), counters_.get(), 64, stream_.get()), "counter readback");
+  out.hostStale = nq > 0;
+  return stream_.sync();
+}
+
+int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, RangeBatch& out, bool graph) {
+  if (int rc = stream_.create()) return rc;
+  if (tree.nnodes <= 0 || nq < 0 || (nq > 0 && !bounds) || queriesPerItem < 1 || queriesPerItem > 64) {
+    gath_set_error("batched range: bad arguments");
+    return GAT_ERR_ARG;
+  }
+  out.queryNum = nq;
+  out.count.assign(size_t(nq), 0);
+  out.base.assign(size_t(nq), 0);
+  out.hostStale = false;
+  out.candidates = out.taskNum = out.pairTotal = out.leafQueries = out.workItems = 0;
+  out.replayed = false;
+  out.selectMs = out.uploadMs = 0;
+  out.kernelMs = out.refineMs = 0;
+  if (nq == 0) return GAT_OK;
+  const double t0 = now_ms();
+  if (int rc = uploadBoxes(bounds, nq)) return rc;
+  const int64_t N = tree.nnodes;
+  const size_t tileBytes = size_t(std::max(2 * gat_scan_tiles(N) * 4, gat_scan_tiles(nq) * 8));
+  if (nodeCount_.size() < size_t(N) * 4 || tiles_.size() < tileBytes || cand_.size() < size_t(nq) * 8 ||
+      out.qbase.size() < size_t(nq) * 8 || out.qcount.size() < size_t(nq) * 4)
+    dropGraph();
+  if (int rc = nodeCount_.reserve(size_t(N) * 4)) return rc;
+  if (int rc = nodeFill_.reserve(size_t(N) * 4)) return rc;
+  if (int rc = nodeOff_.reserve(size_t(N) * 8)) return rc;
+  if (int rc = tiles_.reserve(tileBytes)) return rc;
+  if (int rc = cand_.reserve(size_t(nq) * 8)) return rc;
+  if (int rc = out.qbase.reserve(size_t(nq) * 8)) return rc;
+  if (int rc = out.qcount.reserve(size_t(nq) * 4)) return rc;
+  if (int rc = counters_.reserve(256)) return rc;
+  if (int rc = countersHost_.reserve(256)) return rc;
+  const uint32_t* ch = countersHost_.as<uint32_t>();
+  auto u64 = [&](int w) { return uint64_t(ch[w]) | (uint64_t(ch[w + 1]) << 32); };
+  auto fits = [&] {
+    return u64(0) <= uint64_t(lqCap_) && int64_t(ch[5]) <= qlistCap_ && int64_t(ch[6]) <= itemsCap_ &&
+           u64(2) <= uint64_t(pairsCap_);
+  };
+  auto finish = [&](bool replay) -> int {
+    if (ch[4] & GAT_STATUS_STACK) {
+      gath_set_error("batched range: traversal stack exceeded");
+      return GAT_ERR_STATE;
+    }
+    out.replayed = replay;
+    out.leafQueries = int64_t(u64(0));
+    out.candidates = int64_t(u64(2));
+    out.workItems = int64_t(ch[6]);
+    out.pairTotal = int64_t(u64(8));
+    return GAT_OK;
+  };
+  GraphKey key;
+  auto make_key = [&] {
+    key.ptr[0] = tree.nodes.get();
+    key.ptr[1] = devBoxes_.get();
+    key.ptr[2] = out.pairs.get();
+    key.ptr[3] = out.qbase.get();
+    key.ptr[4] = out.qcount.get();
+    key.ptr[5] = lq_.get();
+    key.nq = nq;
+    key.lqCap = lqCap_;
+    key.qlistCap = qlistCap_;
+    key.itemsCap = itemsCap_;
+    key.pairsCap = pairsCap_;
+    key.qb = queriesPerItem;
+  };
+  if (graph && graph_) {
+    make_key();
+    if (key == graphKey_) {
+      if (int rc = timer_.start(stream_.get())) return rc;
+      GATH_TRY(gat_graph_launch(graph_, stream_.get()), "graph launch");
+      if (int rc = timer_.stop(stream_.get())) return rc;
+      if (int rc = readCounters(nq, out)) return rc;
+      if (int rc = timer_.elapsed(&out.kernelMs)) return rc;
+      if (fits()) {
+        out.uploadMs = now_ms() - t0;
+        return finish(true);
+      }
+      // The batch outgrew the captured capacities: run it step by step below.
+    }
+  }
+  // Selection, then capacities from its totals, then the join.
+  if (lqCap_ < int64_t(nq) * 16) {
+    lqCap_ = std::max<int64_t>(int64_t(nq) * 16, 1 << 16);
+    if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+  }
+  for (int attempt = 0;; ++attempt) {
+    if (int rc = selTimer_.start(stream_.get())) return rc;
+    if (int rc = enqueueSelect(tree, nq, out)) return rc;
+    if (int rc = selTimer_.stop(stream_.get())) return rc;
+    GATH_TRY(gat_memcpy_d2h(countersHost_.get(), counters_.get(), 64, stream_.get()), "counter readback");
+    if (int rc = stream_.sync()) return rc;
+    if (u64(0) <= uint64_t(lqCap_)) break;
+    if (attempt > 0 || u64(0) >= uint64_t(UINT32_MAX)) {
+      gath_set_error("batched range: %llu (leaf, query) entries do not fit", (unsigned long long)u64(0));
+      return GAT_ERR_CAPACITY;
+    }
+    lqCap_ = int64_t(u64(0) + u64(0) / 4 + 1024);
+    if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+  }
+  float selMs = 0;
+  if (int rc = selTimer_.elapsed(&selMs)) return rc;
+  out.selectMs = selMs;
+  if (ch[4] & GAT_STATUS_STACK) return finish(false);
+  // Room with a quarter to spare, so a captured pipeline serves similar batches.
+  auto grow = [](int64_t need, int64_t& cap) {
+    if (need > cap || cap == 0) cap = need + need / 4 + 1024;
+  };
+  grow(int64_t(ch[5]), qlistCap_);
+  grow(int64_t(ch[6]), itemsCap_);
+  grow(int64_t(u64(2)), pairsCap_);
+  if (int rc = qlist_.reserve(size_t(qlistCap_) * 32)) return rc;
+  if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
+  if (int rc = out.pairs.reserve(size_t(pairsCap_) * 8)) return rc;
+  if (int rc = timer_.start(stream_.get())) return rc;
+  timeRefine_ = true;
+  const int rc_join = enqueueJoin(tree, nq, out);
+  timeRefine_ = false;
+  if (rc_join) return rc_join;
+  if (int rc = timer_.stop(stream_.get())) return rc;
+  if (int rc = readCounters(nq, out)) return rc;
+  if (int rc = timer_.elapsed(&out.kernelMs)) return rc;
+  if (int rc = refineTimer_.elapsed(&out.refineMs)) return rc;
+  out.uploadMs = now_ms() - t0;
+  if (int rc = finish(false)) return rc;
+  if (graph) {
+    make_key();
+    if (!(key == graphKey_) || !graph_) {
+      dropGraph();
+      void* g = nullptr;
+      GATH_TRY(gat_graph_begin(stream_.get()), "graph capture");
+      int rc = enqueueSelect(tree, nq, out);
+      if (rc == GAT_OK) rc = enqueueJoin(tree, nq, out);
+      const int rc_end = gat_graph_end(stream_.get(), &g);
+      if (rc != GAT_OK) return rc;
+      GATH_TRY(rc_end, "graph capture");
+      graph_ = g;
+      graphKey_ = key;
+    }
+  }
   return GAT_OK;
 }

@@ -47,17 +47,35 @@ struct RangeBatch {
   int64_t candidates = 0;       // candidate points scanned by the batch
   int64_t taskNum = 0;
   int64_t pairTotal = 0;
-  double selectMs = 0;          // host candidate selection
+  int64_t leafQueries = 0;      // batched form: selected (leaf, query) entries
+  int64_t workItems = 0;        // batched form: leaf-major work items
+  bool replayed = false;        // batched form: ran as a replay of a captured graph
+  double selectMs = 0;          // host candidate selection (batched form: device selection and scans)
   double uploadMs = 0;          // task-list upload
   float kernelMs = 0;           // refine kernel (device events)
+  float refineMs = 0;           // batched form, step-by-step run: the leaf-major refine kernel alone
   DeviceBuffer pairs;           // int32 (tid, pid) pairs
   DeviceBuffer qbase, qcount;   // device copies of base / count
+  bool hostStale = false;       // base / count not yet copied from the device
+  // Copies base / count from the device when they are stale.
+  int syncHost();
   // Copies the pairs to the host, grouped by query (query q's pairs at
   // out[2*base[q] ...], count[q] of them).
-  int fetch(int32_t* out, int64_t capacityPairs) const;
+  int fetch(int32_t* out, int64_t capacityPairs);
   // Per-query order-independent digest (wrapping sum of a 64-bit mix of each
   // (tid, pid) pair), computed on the device.
   int digests(uint64_t* out) const;
+};
+
+// Device copy of an index's tree for the batched form: gat_tree_node in
+// breadth-first order, nchild children per inner node, leaves at depth <= depth.
+struct DeviceTree {
+  DeviceBuffer nodes;
+  int64_t nnodes = 0;
+  int nchild = 0;
+  int depth = 0;
+  // Checks the traversal stack bound and uploads `host`.
+  int upload(const std::vector<gat_tree_node>& host, int nchild, int depth, void* stream);
 };
 
 class RangeEngine {
@@ -84,12 +102,28 @@ class RangeEngine {
   // reference-signature handler (synchronous, host result).
   int runBitmap(const MBB* bounds, int nq, const Selector& select, int threads, int maxTrajNum,
                 uint8_t* resultHost, bool stigHandler);
+  // Batched form: device selection against `tree`, grouping by leaf and the
+  // leaf-major refine, for the whole batch.  With graph=true the device
+  // pipeline is captured once per buffer layout and replayed while the batch
+  // fits the captured capacities.
+  int runTree(const DeviceTree& tree, const MBB* bounds, int nq, RangeBatch& out, bool graph);
+  int queriesPerItem = 32;  // rectangles per leaf-major work item (1..64)
+  // After a batched run: the points of the leaves at least one rectangle
+  // selected (what the leaf-major refine must read at least once).
+  int uniqueCandidates(const DeviceTree& tree, int64_t* points);
+  ~RangeEngine();
+
   // Candidate selection only (host), exposed for tests of the host logic.
   static void select(int nq, const Selector& select, int threads, std::vector<gat_range_task>& tasks,
                      std::vector<uint64_t>& candidates);
 
  private:
   int stage(const MBB* bounds, int nq, const std::vector<gat_range_task>& tasks, const uint64_t* qbase);
+  int uploadBoxes(const MBB* bounds, int nq);
+  int enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out);
+  int enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out);
+  int readCounters(int nq, RangeBatch& out);
+  void dropGraph();
 
   DeviceBuffer xy_, id_, pid_;
   gat_point_store store_{};
@@ -98,4 +132,25 @@ class RangeEngine {
   PinnedBuffer hostStage_;
   DeviceBuffer devTasks_, devBoxes_;
   EventTimer timer_;
+
+  // Batched-form workspace.  counters_ (uint32 words): [0,1] selected entries,
+  // [2,3] candidate total, [4] status, [5] list total, [6] work items,
+  // [8,9] result pairs.
+  DeviceBuffer nodeCount_, nodeFill_, nodeOff_, tiles_, cand_, lq_, qlist_, items_, counters_;
+  PinnedBuffer countersHost_;
+  int64_t lqCap_ = 0, qlistCap_ = 0, itemsCap_ = 0, pairsCap_ = 0;
+  EventTimer selTimer_, refineTimer_;
+  bool timeRefine_ = false;
+  void* graph_ = nullptr;
+  struct GraphKey {
+    const void* ptr[6] = {};  // tree nodes, boxes, pairs, qbase, qcount, lq
+    int64_t nq = -1, lqCap = 0, qlistCap = 0, itemsCap = 0, pairsCap = 0;
+    int qb = 0;
+    bool operator==(const GraphKey& o) const {
+      for (int i = 0; i < 6; ++i)
+        if (ptr[i] != o.ptr[i]) return false;
+      return nq == o.nq && lqCap == o.lqCap && qlistCap == o.qlistCap && itemsCap == o.itemsCap &&
+             pairsCap == o.pairsCap && qb == o.qb;
+    }
+  } graphKey_;
 };

@@ -1,6 +1,7 @@
 #include "STIG.h"
 
 #include <algorithm>
+#include <limits>
 #include <thread>
 
 int STIG::threads() const {
@@ -98,7 +99,77 @@ int STIG::initial(int blk, const TrajectoryView& db, bool upload) {
     tp[2 * i] = pts[i].tid;
     tp[2 * i + 1] = pts[i].pid;
   }
-  return engine_.upload(xy.data(), tp.data(), totalPointsNum);
+  if (int rc = engine_.upload(xy.data(), tp.data(), totalPointsNum)) return rc;
+  int depth = 0;
+  const std::vector<gat_tree_node> tn = treeNodes(&depth);
+  return tree_.upload(tn, 2, depth, engine_.stream());
+}
+
+std::vector<gat_tree_node> STIG::treeNodes(int* depth) const {
+  std::vector<gat_tree_node> out;
+  *depth = 0;
+  if (nodes.empty()) return out;
+  const float inf = std::numeric_limits<float>::infinity();
+  struct Entry {
+    int node, level;
+    gat_tree_node d;
+  };
+  auto leaf_entry = [&](const Node& n, gat_tree_node& d) {
+    d.box[0] = n.box.xmin;
+    d.box[1] = n.box.ymin;
+    d.box[2] = n.box.xmax;
+    d.box[3] = n.box.ymax;
+    d.first_child = -1;
+    d.start = uint32_t(n.lo);
+    d.count = n.hi >= n.lo ? uint32_t(n.hi - n.lo + 1) : 0u;
+    d.flags = GAT_NODE_INCLUSIVE;
+  };
+  // A leaf's box test implies every split test above it (the left part of a
+  // split holds values <= the median, the right part values >= it), so a
+  // leaf is tested against its box alone.
+  std::vector<Entry> q;
+  Entry root{0, 0, {}};
+  if (nodes[0].leaf) {
+    leaf_entry(nodes[0], root.d);
+  } else {
+    root.d = gat_tree_node{{-inf, -inf, inf, inf}, 0, 0, 0, GAT_NODE_UNTESTED};
+  }
+  q.push_back(root);
+  for (size_t h = 0; h < q.size(); ++h) {
+    const Node& n = nodes[size_t(q[h].node)];
+    *depth = std::max(*depth, q[h].level);
+    if (n.leaf) continue;
+    q[h].d.first_child = int32_t(q.size());
+    const int kids[2] = {n.left, n.right};
+    for (int k = 0; k < 2; ++k) {
+      const Node& c = nodes[size_t(kids[k])];
+      Entry e{kids[k], q[h].level + 1, {}};
+      if (c.leaf) {
+        leaf_entry(c, e.d);
+      } else {
+        // searchTree (STIG.cpp:179-252): left when qmin <= median, right when qmax >= median.
+        float b[4] = {-inf, -inf, inf, inf};
+        b[n.dim + (k == 0 ? 2 : 0)] = n.median;
+        e.d = gat_tree_node{{b[0], b[1], b[2], b[3]}, 0, 0, 0, GAT_NODE_INCLUSIVE};
+      }
+      q.push_back(e);
+    }
+  }
+  out.reserve(q.size());
+  for (const Entry& e : q) out.push_back(e.d);
+  return out;
+}
+
+int STIG::rangeQueryBatchedGPU(const MBB* bounds, int rangeNum, RangeBatch& out, bool graph) {
+  if (rangeNum < 0 || (rangeNum > 0 && !bounds)) {
+    gath_set_error("STIG::rangeQueryBatchedGPU: bad arguments");
+    return GAT_ERR_ARG;
+  }
+  if (tree_.nnodes == 0) {
+    gath_set_error("STIG::rangeQueryBatchedGPU: no device tree (build the index with upload)");
+    return GAT_ERR_STATE;
+  }
+  return engine_.runTree(tree_, bounds, rangeNum, out, graph);
 }
 
 int STIG::searchTree(const MBB& q, std::vector<STIGBlock>* out) const {

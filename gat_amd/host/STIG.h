@@ -44,6 +44,14 @@ class STIG {
   int rangeQueryGPUPairs(const MBB* bounds, int rangeNum, RangeBatch& out);
   void selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
                         std::vector<uint64_t>& candidates) const;
+  // Batched form (no reference counterpart), as Grid::rangeQueryBatchedGPU.
+  int rangeQueryBatchedGPU(const MBB* bounds, int rangeNum, RangeBatch& out, bool graph = false);
+  // Points of the leaves the last batched run selected at least once.
+  int uniqueCandidates(int64_t* points) { return engine_.uniqueCandidates(tree_, points); }
+  // The kd-tree as breadth-first gat_tree_nodes: an inner node's children
+  // carry the half-plane of its split (inclusive, as searchTree), a leaf its
+  // points' box (inclusive); *depth = deepest level.
+  std::vector<gat_tree_node> treeNodes(int* depth) const;
 
   struct Node {
     bool leaf = true;
@@ -70,4 +78,5 @@ class STIG {
   void addLeaves(const MBB& q, uint32_t qid, TaskList& list) const;
   int threads() const;
   RangeEngine engine_;
+  DeviceTree tree_;
 };

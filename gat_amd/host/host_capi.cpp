@@ -200,9 +200,61 @@ int gath_index_range_pairs(void* p, const float* boxes, int nq, int64_t* stats, 
   return GAT_OK;
 }
 
+int gath_index_range_batched(void* p, const float* boxes, int nq, int graph, int64_t* stats, double* ms) {
+  Index* ix = as_index(p);
+  if (!ix || nq < 0 || (nq > 0 && !boxes)) return bad("gath_index_range_batched: bad arguments");
+  int rc;
+  if (ix->grid) rc = ix->grid->rangeQueryBatchedGPU(as_boxes(boxes), nq, ix->last, graph != 0);
+  else if (ix->stig) rc = ix->stig->rangeQueryBatchedGPU(as_boxes(boxes), nq, ix->last, graph != 0);
+  else return bad("gath_index_range_batched: the FSG index has no tree");
+  if (rc) return rc;
+  if (stats) {
+    stats[0] = ix->last.pairTotal;
+    stats[1] = ix->last.candidates;
+    stats[2] = ix->last.leafQueries;
+    stats[3] = ix->last.workItems;
+    stats[4] = ix->last.replayed ? 1 : 0;
+  }
+  if (ms) {
+    ms[0] = ix->last.selectMs;
+    ms[1] = ix->last.uploadMs;
+    ms[2] = ix->last.kernelMs;
+    ms[3] = ix->last.refineMs;
+  }
+  return GAT_OK;
+}
+
+int gath_index_last_unique_candidates(void* p, int64_t* points) {
+  Index* ix = as_index(p);
+  if (!ix || !points || ix->fsg) return bad("gath_index_last_unique_candidates: bad arguments");
+  return ix->grid ? ix->grid->uniqueCandidates(points) : ix->stig->uniqueCandidates(points);
+}
+
+int gath_index_set_batch_params(void* p, int queries_per_item) {
+  Index* ix = as_index(p);
+  if (!ix || queries_per_item < 1 || queries_per_item > 64) return bad("gath_index_set_batch_params: bad arguments");
+  RangeEngine& e = ix->grid ? ix->grid->engine() : ix->fsg ? ix->fsg->engine() : ix->stig->engine();
+  e.queriesPerItem = queries_per_item;
+  return GAT_OK;
+}
+
+int64_t gath_index_tree_nodes(void* p, gat_tree_node* out, int64_t cap, int32_t* depth) {
+  Index* ix = as_index(p);
+  if (!ix || !depth || (cap > 0 && !out)) return bad("gath_index_tree_nodes: bad arguments");
+  std::vector<gat_tree_node> t;
+  int d = 0;
+  if (ix->grid) t = ix->grid->treeNodes(&d);
+  else if (ix->stig) t = ix->stig->treeNodes(&d);
+  else return bad("gath_index_tree_nodes: the FSG index has no tree");
+  *depth = d;
+  for (int64_t i = 0; i < std::min<int64_t>(cap, int64_t(t.size())); ++i) out[i] = t[size_t(i)];
+  return int64_t(t.size());
+}
+
 int gath_index_last_counts(void* p, int64_t* counts) {
   Index* ix = as_index(p);
   if (!ix || !counts) return bad("gath_index_last_counts: bad arguments");
+  if (int rc = ix->last.syncHost()) return rc;
   for (int q = 0; q < ix->last.queryNum; ++q) counts[q] = ix->last.count[q];
   return GAT_OK;
 }
@@ -210,6 +262,7 @@ int gath_index_last_counts(void* p, int64_t* counts) {
 int gath_index_last_fetch(void* p, int32_t* pairs, int64_t cap) {
   Index* ix = as_index(p);
   if (!ix || (cap > 0 && !pairs)) return bad("gath_index_last_fetch: bad arguments");
+  if (int rc = ix->last.syncHost()) return rc;
   const RangeBatch& b = ix->last;
   if (b.pairTotal > cap) {
     gath_set_error("gath_index_last_fetch: %lld pairs, room for %lld", (long long)b.pairTotal, (long long)cap);

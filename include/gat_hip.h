@@ -218,6 +218,105 @@ int gat_edr_mismatch_table(int32_t maxlen, int32_t* out_dev, void* stream);
  * the smallest float >= eps. */
 float gat_eps_threshold(double eps);
 
+/* ---------------------------------------------------------------------------
+ * Batched range form: candidate selection on the device and a leaf-major join.
+ *
+ * No counterpart in the reference, which selects leaves on the host per query
+ * (Grid::findMatchNodeInQuadTreeGPU, Grid.cpp:703-761; STIG::searchTree,
+ * STIG.cpp:179-252) and refines one (query, leaf chunk) per block.  Here a batch
+ * of rectangles is selected against a device copy of the tree, the selected
+ * (leaf, query) pairs are grouped by leaf, and each leaf's points are read once
+ * and tested against every rectangle that selected it.  Results are the same
+ * sets as the plain path: the points of each selected leaf inside the
+ * rectangle (MBB::pInBox, MBB.cpp:21-26).
+ * ------------------------------------------------------------------------- */
+
+/* A tree node in breadth-first order, so a node's children are contiguous at
+ * first_child .. first_child + nchild - 1.  A node is visited when its parent
+ * was and its test passes:
+ *   flags & GAT_NODE_UNTESTED  -> always (the GT-quadtree root, Grid.cpp:428-446);
+ *   flags & GAT_NODE_INCLUSIVE -> box overlaps the rectangle, edges included
+ *                                 (STIG's intersectBlock and leaf test, STIG.cpp:340-347);
+ *   otherwise                  -> BoxIntersect(rectangle, box) != 0 (MBB.cpp:28-73).
+ * Leaves (first_child < 0) name the store run [start, start + count). */
+typedef struct gat_tree_node {
+  float box[4]; /* xmin, ymin, xmax, ymax */
+  int32_t first_child;
+  uint32_t start;
+  uint32_t count;
+  uint32_t flags;
+} gat_tree_node;
+
+#define GAT_NODE_INCLUSIVE 1u
+#define GAT_NODE_UNTESTED 2u
+/* Entries of the per-wave traversal stack; a tree of depth D with nchild
+ * children per node needs (D + 1) * nchild * 64 of them. */
+#define GAT_TREE_STACK 4096
+/* Bit set in *status by gat_tree_select when a traversal would exceed the stack. */
+#define GAT_STATUS_STACK 1u
+
+/* dst[0 .. n) = value, by a kernel (safe inside a captured graph). */
+int gat_fill_u32(uint32_t* dst_dev, uint32_t value, int64_t n, void* stream);
+
+/* Number of tile partials the scans below need for n elements. */
+int64_t gat_scan_tiles(int64_t n);
+
+/* Exclusive prefix sum: out[i] = in[0] + ... + in[i-1]; *total = sum of all.
+ * tiles_dev holds gat_scan_tiles(n) entries of scratch.  out may alias in. */
+int gat_scan_u64(const uint64_t* in_dev, int64_t n, uint64_t* out_dev, uint64_t* total_dev, uint64_t* tiles_dev,
+                 void* stream);
+
+/* Per node, the offsets of its query list and of its work items, a work item
+ * being up to qb queries of one leaf: node_off[2i] = sum over j < i of
+ * count[j], node_off[2i+1] = sum over j < i of ceil(count[j] / qb); totals[0..1]
+ * are the two sums.  tiles_dev holds 2 * gat_scan_tiles(n) uint32 entries. */
+int gat_scan_leaf_lists(const uint32_t* count_dev, int64_t nnodes, int32_t qb, uint32_t* node_off_dev,
+                        uint32_t* totals_dev, uint32_t* tiles_dev, void* stream);
+
+/* Selects, for rectangles q0 .. q0+nq-1 (or, when order_dev is not null, for
+ * rectangles order_dev[0 .. nq-1], taken in that order — largest first
+ * shortens the tail), the leaves they visit (one wave per rectangle).  Appends each (leaf, query) to leaf_query (2
+ * uint32 per entry, order unspecified) while *lq_total < lq_capacity, always
+ * advancing *lq_total; adds 1 to node_qcount[leaf] per entry; writes cand[q] =
+ * the points of q's selected leaves.  *lq_total and node_qcount must start at
+ * zero; leaves with no points are skipped. */
+int gat_tree_select(const gat_tree_node* nodes_dev, int64_t nnodes, int32_t nchild, const float* boxes_dev,
+                    const uint32_t* order_dev, int64_t q0, int64_t nq, uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev,
+                    int64_t lq_capacity, uint64_t* lq_total_dev, uint32_t* status_dev, void* stream);
+
+/* Groups the selected (leaf, query) entries by leaf into self-contained
+ * records.  List entry r of leaf L (32 bytes: the rectangle's 4 floats, its
+ * output region base as uint64, its id as uint32, 4 bytes padding) goes to
+ * qlist[node_off[2*L] + r]; every qb-th entry starts a work item (4 uint32:
+ * the leaf's store start and count, the item's first list entry, its number of
+ * entries <= qb) at items[4 * (node_off[2*L+1] + r/qb)].  qbase_dev holds the
+ * per-query region bases (gat_scan_u64 of the candidate counts).  node_fill
+ * (nnodes) must start at zero; entries beyond lq_capacity are ignored; qb is
+ * 1..64. */
+int gat_leaf_scatter(const uint32_t* leaf_query_dev, const uint64_t* lq_total_dev, int64_t lq_capacity,
+                     const gat_tree_node* nodes_dev, int64_t nnodes, const uint32_t* node_off_dev,
+                     const uint32_t* node_qcount_dev, uint32_t* node_fill_dev, int32_t qb, const float* boxes_dev,
+                     const uint64_t* qbase_dev, int64_t nqueries, void* qlist_dev, int64_t qlist_capacity,
+                     uint32_t* items_dev, int64_t items_capacity, void* stream);
+
+/* The leaf-major refine: per work item the leaf's points and ids are loaded
+ * once and tested against each of the item's rectangles; hits go to query q's
+ * region pairs[base ...] as (tid, pid), qcount[q] counting them (qcount must
+ * start at zero).  *item_total_dev (totals[1] of gat_scan_leaf_lists) gives the
+ * number of items.  When pair_total_dev is not null, the sum of qcount is
+ * added to it afterwards. */
+int gat_leaf_refine(const gat_point_store* store, const uint32_t* items_dev, const uint32_t* item_total_dev,
+                    int64_t items_capacity, const void* qlist_dev, int64_t qlist_capacity, int64_t nqueries,
+                    uint32_t* qcount_dev, int32_t* pairs_dev, int64_t pairs_capacity, uint64_t* pair_total_dev,
+                    void* stream);
+
+/* Stream capture into a replayable graph (hipStreamBeginCapture /
+ * hipGraphInstantiate).  Only kernels are recorded by the batched pipelines. */
+int gat_graph_begin(void* stream);
+int gat_graph_end(void* stream, void** graph_exec);
+int gat_graph_launch(void* graph_exec, void* stream);
+int gat_graph_destroy(void* graph_exec);
+
 #pragma GCC visibility pop
 
 #ifdef __cplusplus

@@ -69,6 +69,20 @@ int gath_index_range_bitmap(void* index, const float* boxes, int nq, uint8_t* ou
  * host selection, task upload, refine kernel.  The result stays on the device
  * until the next call. */
 int gath_index_range_pairs(void* index, const float* boxes, int nq, int64_t* stats, double* ms);
+/* Batched form (Grid and STIG): device selection and the leaf-major join, the
+ * whole batch at once; graph != 0 captures the device pipeline and replays it
+ * on later calls that fit.  stats[0..4] = pairs, candidates, (leaf, query)
+ * entries, work items, replayed (0/1); ms[0..3] = device selection, wall time,
+ * join kernels (the whole pipeline when replayed), the refine kernel alone
+ * (0 when replayed).  Results are read with the gath_index_last_* calls. */
+int gath_index_range_batched(void* index, const float* boxes, int nq, int graph, int64_t* stats, double* ms);
+/* After a batched run: points of the leaves at least one rectangle selected. */
+int gath_index_last_unique_candidates(void* index, int64_t* points);
+/* Tunables of the batched form: rectangles per leaf-major work item (1..64). */
+int gath_index_set_batch_params(void* index, int queries_per_item);
+/* The index's tree as the batched form uploads it (breadth-first
+ * gat_tree_node); returns the node count, copies up to cap nodes. */
+int64_t gath_index_tree_nodes(void* index, gat_tree_node* out, int64_t cap, int32_t* depth);
 int gath_index_last_counts(void* index, int64_t* counts);
 /* Pairs of the last result, query by query (query q's pairs follow those of
  * queries 0..q-1); cap in pairs. */

@@ -1,0 +1,115 @@
+"""GPU parity of the batched range form (device selection, leaf-major join,
+captured-graph replay) against the oracle's restatement of
+Grid::rangeQueryBatch (Grid.cpp:341-405) and inclusive brute force for STIG.
+
+Bar: bit-exact result sets per query, and per-query digests equal to the
+plain path's and the oracle's."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gat_amd import api, workloads as W
+
+from tests.test_gpu_range import _check_pairs, _edge_rects, _sets
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg1(gpu):
+    db = W.CFG1.db()
+    return db, W.CFG1.rects()
+
+
+def _check_digests(res, counts, dig):
+    assert (res.counts() == counts).all()
+    assert (res.digests() == dig).all()
+
+
+def test_cfg1_grid_batched(cfg1):
+    """Point-for-point on the 100 squares; per-query counts and digests on
+    rectangles along cell edges, through data points and over everything
+    (about 20M pairs, too many to compare as sets)."""
+    db, rects = cfg1
+    og = O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db)
+    g = api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db)
+    res = g.range_batched(rects)
+    _check_pairs(res, _sets(og.range(rects)), rects)
+    edge = _edge_rects(db, W.CFG1.grid_cell, 60, 21)
+    res = g.range_batched(edge)
+    _check_digests(res, *og.range(edge, "digest"))
+    plain = g.range_pairs(edge)
+    assert plain.candidates == res.candidates  # same leaves selected
+    assert res.leaf_queries > 0 and res.work_items >= 1
+
+
+def test_cfg1_stig_batched(cfg1):
+    db, rects = cfg1
+    s = api.StigIndex(W.CFG1.leaf).build(db)
+    res = s.range_batched(rects)
+    _check_pairs(res, _sets(O.brute_range(db, rects)), rects)
+    edge = _edge_rects(db, W.CFG1.grid_cell, 40, 22)
+    res = s.range_batched(edge)
+    _check_digests(res, *O.brute_range(db, edge, "digest"))
+    assert s.range_pairs(edge).candidates == res.candidates
+
+
+@pytest.mark.parametrize("cell,leaf", [(1 / 1024, 8), (1 / 16, 4096)])
+def test_tree_shapes(gpu, cell, leaf):
+    """A deep tree (tiny leaves on a 1024^2 grid) and a shallow one whose
+    leaves exceed the 1024 points a work item loads at once."""
+    db = W.CFG1.db(3000)
+    rects = np.concatenate([W.CFG1.rects(), _edge_rects(db, cell, 30, 23),
+                            api.synth_rects(23, 50, 1e-4, 0.5, True)]).astype(np.float32)
+    og = O.Grid(W.UNIT_BOX, cell, leaf).add(db)
+    g = api.GridIndex(W.UNIT_BOX, cell, leaf).build(db)
+    _check_digests(g.range_batched(rects), *og.range(rects, "digest"))
+
+
+def test_hot_leaves_split_into_items(gpu):
+    """Hundreds of rectangles over the same leaves: each leaf's list is split
+    into work items of at most qb rectangles, for several qb."""
+    db = W.CFG1.db(4000)
+    rng = np.random.default_rng(3)
+    c = 0.5 + rng.normal(0, 0.01, size=(600, 2))
+    half = 10 ** rng.uniform(-3.5, -1.5, size=(600, 1))
+    rects = np.concatenate([c - half, c + half], 1).astype(np.float32)
+    og = O.Grid(W.UNIT_BOX, 0.01, 256).add(db)
+    counts, dig = og.range(rects, "digest")
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    for qb in (64, 32, 5, 1):
+        res = g.set_batch_params(qb).range_batched(rects)
+        assert (res.counts() == counts).all() and (res.digests() == dig).all(), qb
+        assert res.work_items >= -(-res.leaf_queries // qb)
+    assert res.work_items == res.leaf_queries  # one rectangle per item
+
+
+def test_graph_replay(gpu):
+    """Capture on the first call, replay on the next ones; a larger batch
+    that outgrows the captured buffers runs step by step and is recaptured.
+    Every call is checked against the oracle."""
+    db = W.CFG1.db(5000)
+    og = O.Grid(W.UNIT_BOX, 0.01, 256).add(db)
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    small = api.synth_rects(31, 300, 1e-4, 0.05, True)
+    other = api.synth_rects(32, 300, 1e-4, 0.05, True)
+    large = api.synth_rects(33, 300, 0.05, 0.3, True)
+    seen = []
+    for rects in (small, small, other, small, large, large, small):
+        res = g.range_batched(rects, graph=True)
+        counts, dig = og.range(rects, "digest")
+        assert (res.counts() == counts).all()
+        assert (res.digests() == dig).all()
+        seen.append(res.replayed)
+    assert seen[0] is False and seen[1] is True
+    assert seen[4] is False  # outgrew the captured capacities
+
+
+def test_empty_and_outside(gpu):
+    db = W.CFG1.db(500)
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    res = g.range_batched(np.zeros((0, 4), np.float32))
+    assert res.nq == 0 and res.pairs_total == 0
+    out = np.array([[1.5, 1.5, 2.0, 2.0], [-1, -1, -0.5, -0.5]], np.float32)
+    res = g.range_batched(out)
+    assert (res.counts() == 0).all() and res.candidates == 0

@@ -207,3 +207,86 @@ def test_host_rejects_points_outside_the_grid():
         api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db, upload=False)
     with pytest.raises(N.GatError):
         api.FsgIndex(W.UNIT_BOX, 0.01).build(db, upload=False)
+
+
+def _node_selected(n, box):
+    """The batched form's node test (gat_tree_node flags), in float32."""
+    if n["flags"] & api.NODE_UNTESTED:
+        return True
+    b = n["box"]
+    if n["flags"] & api.NODE_INCLUSIVE:
+        return bool(b[0] <= box[2] and b[2] >= box[0] and b[1] <= box[3] and b[3] >= box[1])
+    return O.box_intersect(box, b) != 0
+
+
+def _wave_select(nodes, box, nchild, stack_cap=4096, wave=64):
+    """tree_select_kernel restated: pop up to 64 entries from the stack top,
+    test them in lane order, push the children of selected inner nodes.
+    Returns the selected leaf runs and the largest stack size reached."""
+    stack, top_max, sel = [0], 1, []
+    while stack:
+        m = min(len(stack), wave, (stack_cap - len(stack)) // (nchild - 1))
+        assert m > 0
+        batch = stack[len(stack) - m:]
+        del stack[len(stack) - m:]
+        for i in batch:
+            n = nodes[i]
+            if not _node_selected(n, box):
+                continue
+            if n["first_child"] < 0:
+                if n["count"] > 0:
+                    sel.append((int(n["start"]), int(n["count"])))
+            else:
+                stack.extend(range(int(n["first_child"]), int(n["first_child"]) + nchild))
+        top_max = max(top_max, len(stack))
+    return sel, top_max
+
+
+def _runs(leaves):
+    runs = []
+    for s, c in sorted(leaves):
+        if runs and runs[-1][0] + runs[-1][1] == s:
+            runs[-1][1] += c
+        else:
+            runs.append([s, c])
+    return runs
+
+
+@pytest.mark.parametrize("cell,leaf", [(0.01, 256), (1 / 16, 8), (1 / 1024, 8)])
+def test_grid_device_tree_selects_oracle_leaves(cell, leaf):
+    """The breadth-first tree the batched form uploads selects exactly the
+    oracle's leaves (Grid.cpp:428-446), and the traversal stays inside the
+    stack bound the upload checks."""
+    db = W.CFG1.db(1500)
+    og = O.Grid(W.UNIT_BOX, cell, leaf).add(db)
+    nodes, depth = api.GridIndex(W.UNIT_BOX, cell, leaf).build(db, upload=False).tree_nodes()
+    assert nodes[0]["flags"] == api.NODE_UNTESTED and (nodes[1:]["flags"] == 0).all()
+    inner = nodes["first_child"] >= 0
+    assert (nodes["first_child"][inner] > np.nonzero(inner)[0]).all()
+    rng = np.random.default_rng(11)
+    k = rng.integers(0, int(1 / cell), size=(20, 2)).astype(np.float64)
+    edge = np.stack([k[:, 0] * cell, 1 - (k[:, 1] + 2) * cell, (k[:, 0] + 2) * cell, 1 - k[:, 1] * cell],
+                    1).astype(np.float32)
+    rects = np.concatenate([W.CFG1.rects()[:20], api.synth_rects(4, 20, 1e-4, 0.5, True), edge,
+                            np.array([[0, 0, 1, 1], [-1, -1, 2, 2]], np.float32)]).astype(np.float32)
+    for r in rects:
+        sel, top = _wave_select(nodes, r, 4)
+        want = [[int(s), int(c)] for s, c in og.leaves(r) if c > 0]
+        assert _runs(sel) == _runs(want)
+        assert top <= (depth + 1) * 4 * 64
+
+
+def test_stig_device_tree_selects_host_leaves():
+    """STIG's breadth-first tree (split half-planes on inner nodes, leaf boxes)
+    selects the same points as the host kd search (STIG.cpp:179-252)."""
+    db = W.CFG1.db(1200)
+    ix = api.StigIndex(32).build(db, upload=False)
+    nodes, depth = ix.tree_nodes()
+    rects = np.concatenate([W.CFG1.rects()[:30], api.synth_rects(6, 30, 1e-4, 0.5, True),
+                            np.array([[0, 0, 1, 1], [0.5, 0.5, 0.5, 0.5]], np.float32)]).astype(np.float32)
+    tasks, _ = ix.select(rects)
+    for q, r in enumerate(rects):
+        sel, top = _wave_select(nodes, r, 2)
+        t = tasks[tasks[:, 2] == q]
+        assert _runs(sel) == _runs([(int(s), int(c)) for s, c, _ in t])
+        assert top <= (depth + 1) * 2 * 64
