@@ -86,9 +86,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "range", "batched", "copy"], default="all",
+    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "copy"], default="all",
                     help="range / batched: just the timed GT-quadtree batches of the plain / batched form (for "
-                         "profiling); copy: just the stream-copy calibration kernel")
+                         "profiling); edr: just the EDR section; copy: just the stream-copy calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU split of the "
@@ -105,6 +105,9 @@ def main() -> int:
         gbs, ms = G.stream_copy_bandwidth(1 << 32, max(args.steps, 1))
         log(f"stream copy: {gbs:.1f} GB/s ({ms:.3f} ms per 4 GiB copy)")
         print(json.dumps({"stream_copy_gbs": round(gbs, 1), "kernel_ms": round(ms, 4), "bytes_per_launch": 2 << 32}))
+        return 0
+    if args.only == "edr":
+        print(json.dumps(bench_edr(G, W)))
         return 0
     w = W.CFG2
     t = time.perf_counter()
@@ -155,6 +158,7 @@ def main() -> int:
     _, fqps, _, _ = time_range(fsg, rects, few, 1, "FSG plain")
     _, sqps, _, _ = time_range(stig, rects, few, 1, "STIG plain")
     _, sbqps, _, _ = time_batched(stig, rects, few, 1, "STIG batched")
+    _, fbqps, _, _ = time_batched(fsg, rects, few, 1, "FSG batched")
     del fsg, stig
 
     edr = bench_edr(G, W)
@@ -181,6 +185,20 @@ def main() -> int:
         log(f"CPU baseline: {cpu['value']} queries/s (1 thread), {len(rects) / cpu16_s:.1f} (16 threads); "
             f"GPU result identical: {ok}")
         del og
+        # EDR on the CPU: Grid::SimilarityExecuter restated, one 128-point query
+        # against the first 20k candidates of the top-k workload.
+        w3 = W.CFG3
+        db3 = w3.db(20_000)
+        q3 = G.synth_queries(w3.seed + 7, np.array([128], np.int32), db3, copy_mode=2)
+        cells3 = int(db3.lengths.astype(np.int64).sum() * 128)
+        rates = {}
+        for th in (1, 16):
+            t = time.perf_counter()
+            O.edr_batch(q3.xy(1), db3, w3.eps, nthreads=th)
+            rates[th] = cells3 / (time.perf_counter() - t)
+        cpu["edr_cells_per_s"] = {"1_thread": float(f"{rates[1]:.4e}"), "16_threads": float(f"{rates[16]:.4e}"),
+                                  "sample": f"one 128-point query x 20k candidates ({cells3:,} cells)"}
+        log(f"CPU EDR: {rates[1]:.3e} cells/s (1 thread), {rates[16]:.3e} (16 threads)")
 
     line = {
         "metric": METRIC,
@@ -215,6 +233,7 @@ def main() -> int:
             "batched_pipeline_ms": round(bres.kernel_ms, 4),
             "stig_batched_qps": round(sbqps, 1),
             "stig_plain_qps": round(sqps, 1),
+            "fsg_batched_qps": round(fbqps, 1),
             "fsg_plain_qps": round(fqps, 1),
             "range_batched_p50_ms": round(1e3 * float(np.percentile(bwall, 50)), 3),
             "range_batched_p99_ms": round(1e3 * float(np.percentile(bwall, 99)), 3),

@@ -154,6 +154,13 @@ def synth_queries(seed: int, lengths: np.ndarray, db: TrajSet | None, copy_mode:
 
 
 # ---------------------------------------------------------------- indexes
+# gath_request / gath_outcome (include/gat_host.h).
+REQUEST = np.dtype([("kind", np.int32), ("box", np.float32, 4), ("qlen", np.int32), ("qoff", np.int64),
+                    ("arrival_ms", np.float64)], align=True)
+OUTCOME = np.dtype([("admit_ms", np.float64), ("done_ms", np.float64), ("batch", np.int32), ("count", np.uint32),
+                    ("digest", np.uint64)], align=True)
+REQ_RANGE, REQ_TOPK = 0, 1
+
 # gat_tree_node (include/gat_hip.h).
 TREE_NODE = np.dtype([("box", np.float32, 4), ("first_child", np.int32), ("start", np.uint32), ("count", np.uint32),
                       ("flags", np.uint32)])
@@ -251,7 +258,7 @@ class _Index:
         return RangeResult(self, len(b), stats, ms)
 
     def range_batched(self, boxes: np.ndarray, graph: bool = False) -> RangeResult:
-        """Batched form (Grid and STIG): device selection plus the leaf-major join
+        """Batched form (Grid, STIG, FSG): device selection plus the leaf-major join
         for the whole batch; graph=True captures the device pipeline and replays
         it on later calls that fit.  Same result sets as range_pairs."""
         b = _req(boxes, np.float32).reshape(-1, 4)
@@ -292,6 +299,13 @@ class GridIndex(_Index):
     def __init__(self, range_box=(0.0, 0.0, 1.0, 1.0), cell_size: float = 0.01, max_points_in_node: int = 256):
         r = np.asarray(range_box, np.float32)
         self.handle = N.host().gath_grid_new(_ptr(r), cell_size, max_points_in_node)
+
+    def set_edr_params(self, split: int = 512, per_cu_short: int = 0, per_cu_long: int = 0) -> "GridIndex":
+        """Batched-EDR length split between the short and long buckets, and
+        workgroups per CU of each bucket's launch (0: kernel default)."""
+        N.check_host(N.host().gath_grid_set_edr_params(self.handle, int(split), int(per_cu_short),
+                                                       int(per_cu_long)))
+        return self
 
     def edr_all(self, q_xy: np.ndarray, eps: float) -> np.ndarray:
         q = _req(q_xy, np.float32).reshape(-1, 2)
@@ -347,6 +361,31 @@ MAX_TRAJ_LEN = 2048
 def _edr_stats(stats, ms) -> dict:
     return {"pairs": int(stats[0]), "cells": int(stats[1]), "jobs": int(stats[2]), "pruned": int(stats[3]),
             "kernel_ms": float(ms[0]), "filter_ms": float(ms[1]), "host_ms": float(ms[2])}
+
+
+def serve(grid: "GridIndex", requests: np.ndarray, qlon: np.ndarray, qlat: np.ndarray, k: int, eps: float,
+          batched: bool = True, range_slots: int = 2, max_range_batch: int = 4096, max_topk_batch: int = 256,
+          range_budget: float = 4e9, topk_budget: float = 1e9):
+    """Replays a request stream (REQUEST records, arrival times ascending)
+    through the serving loop.  Returns (outcomes, topk ids, topk distances,
+    stats dict); top-k rows are meaningful for top-k requests only."""
+    if requests.dtype != REQUEST:
+        raise TypeError("requests must have the REQUEST dtype")
+    r = np.ascontiguousarray(requests)
+    n = len(r)
+    lon = _req(qlon, np.float32)
+    lat = _req(qlat, np.float32)
+    out = np.zeros(max(n, 1), OUTCOME)
+    ids = np.zeros((max(n, 1), k), np.int32)
+    dists = np.zeros((max(n, 1), k), np.int32)
+    cfg = np.array([range_slots, max_range_batch, max_topk_batch, range_budget, topk_budget, k, eps, int(batched)],
+                   np.float64)
+    st = np.zeros(9, np.float64)
+    N.check_host(N.host().gath_serve(grid.handle, r.ctypes.data, n, _ptr(lon), _ptr(lat), _ptr(cfg),
+                                     out.ctypes.data, _ptr(ids), _ptr(dists), _ptr(st)))
+    keys = ["range_batches", "topk_batches", "range_done", "topk_done", "wall_ms", "peak_range_bytes",
+            "peak_topk_bytes", "range_budget", "topk_budget"]
+    return out[:n], ids[:n], dists[:n], dict(zip(keys, st.tolist()))
 
 
 class FsgIndex(_Index):

@@ -30,7 +30,7 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 
 HIP_SOURCES = ["runtime.hip", "util.hip", "range.hip", "range_batched.hip", "edr.hip", "edr_batched.hip"]
 HOST_SOURCES = ["MBB.cpp", "Synth.cpp", "DeviceArena.cpp", "RangeEngine.cpp", "Grid.cpp", "GridSimilarity.cpp", "FSG.cpp",
-                "STIG.cpp", "SystemTest.cpp", "host_capi.cpp"]
+                "STIG.cpp", "SystemTest.cpp", "Server.cpp", "host_capi.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -58,7 +58,7 @@ def build_hip(verbose: bool = True) -> Path:
         objs.append(str(obj))
     out = LIB_DIR / "libgat_hip.so"
     tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp),
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", *objs, "-o", str(tmp),
           f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, out)
     return out
@@ -76,7 +76,7 @@ def build_host(verbose: bool = True) -> Path:
         objs.append(str(obj))
     out = LIB_DIR / "libgat_host.so"
     tmp = out.with_suffix(".so.tmp")
-    _run(["g++", "-shared", "-pthread", *objs, "-o", str(tmp), f"-L{LIB_DIR}", "-lgat_hip",
+    _run(["g++", "-shared", "-pthread", "-Wl,--no-undefined", *objs, "-o", str(tmp), f"-L{LIB_DIR}", "-lgat_hip",
           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, out)
     return out

@@ -113,7 +113,9 @@ __global__ __launch_bounds__(64) void edr_wave_kernel(const int2* __restrict__ p
     const int strips = (nmax + kSeg - 1) / kSeg;
     const int my_last = (n + kSeg - 1) / kSeg - 1;
     const int steps = mmax + kSeg - 1;
-    // Row 0: D[0][c] = c.
+    // Row 0: D[0][c] = c.  (Kept rolled: unrolled, its length grows with MAXM
+    // and so does the register count.)
+#pragma unroll 2
     for (int i = l; i < MAXM + 3 * kBlk; i += kSeg) above[seg][i] = int16_t(i - kBlk);
     int res = 0;
     for (int s = 0; s < strips; ++s) {
@@ -276,7 +278,8 @@ extern "C" int gat_edr_mismatch_table(int32_t maxlen, int32_t* out, void* stream
 extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_edr_job* jobs, int64_t njobs,
                                int32_t* counter, const float* qxy, const int64_t* qoff, const int32_t* qlen,
                                int32_t nqueries, const float* cxy, const int64_t* coff, const int32_t* clen,
-                               int32_t ncands, int32_t maxlen, double eps, int32_t* out, void* stream) {
+                               int32_t ncands, int32_t maxlen, double eps, int32_t per_cu, int32_t* out,
+                               void* stream) {
   GAT_REQUIRE(npairs >= 0 && njobs >= 0 && njobs <= INT32_MAX && nqueries >= 0 && ncands >= 0,
               "EDR batched: bad sizes");
   if (njobs == 0) return GAT_OK;
@@ -284,23 +287,40 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
               "EDR batched: null pointer");
   GAT_REQUIRE(maxlen >= 1 && maxlen <= GAT_MAX_TRAJ_LEN, "EDR batched: maxlen outside 1..%d", GAT_MAX_TRAJ_LEN);
   GAT_REQUIRE(eps > 0, "EDR batched: eps must be positive");
+  GAT_REQUIRE(per_cu >= 0 && per_cu <= 64, "EDR batched: per_cu outside 0..64");
   hipStream_t s = gat::as_stream(stream);
-  GAT_HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(int32_t), s));
+  // The job counter is cleared by a kernel, so the launch can be captured in a graph.
+  if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(counter), 0u, 1, stream)) return rc;
   const float u = gat_eps_threshold(eps);
   const auto* P = reinterpret_cast<const int2*>(pairs);
   const auto* J = reinterpret_cast<const Job*>(jobs);
   const auto* Q = reinterpret_cast<const float2*>(qxy);
   const auto* Cs = reinterpret_cast<const float2*>(cxy);
-  // Persistent grid: as many single-wave workgroups as fit, capped by the work.
-  const int64_t per_cu = maxlen <= 512 ? 20 : 9;
-  int64_t blocks = int64_t(gat::cu_count()) * per_cu;
+  // LDS class: the smallest row buffer that holds maxlen columns.  Persistent
+  // grid: per_cu single-wave workgroups per CU (default by class), capped by
+  // the work.
+  const int cls = maxlen <= 256 ? 256 : maxlen <= 512 ? 512 : maxlen <= 1024 ? 1024 : GAT_MAX_TRAJ_LEN;
+  const int64_t pc = per_cu > 0 ? per_cu : (cls <= 512 ? 20 : cls == 1024 ? 12 : 9);
+  int64_t blocks = int64_t(gat::cu_count()) * pc;
   if (blocks > njobs) blocks = njobs;
-  if (maxlen <= 512)
-    edr_wave_kernel<512><<<dim3(uint32_t(blocks)), dim3(64), 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries,
-                                                                       ncands, Q, qoff, qlen, Cs, coff, clen, u, out);
-  else
-    edr_wave_kernel<GAT_MAX_TRAJ_LEN><<<dim3(uint32_t(blocks)), dim3(64), 0, s>>>(
-        P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff, qlen, Cs, coff, clen, u, out);
+  const dim3 grid{uint32_t(blocks)}, block{64};
+  switch (cls) {
+    case 256:
+      edr_wave_kernel<256><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff,
+                                                  qlen, Cs, coff, clen, u, out);
+      break;
+    case 512:
+      edr_wave_kernel<512><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff,
+                                                  qlen, Cs, coff, clen, u, out);
+      break;
+    case 1024:
+      edr_wave_kernel<1024><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff,
+                                                   qlen, Cs, coff, clen, u, out);
+      break;
+    default:
+      edr_wave_kernel<GAT_MAX_TRAJ_LEN><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries,
+                                                               ncands, Q, qoff, qlen, Cs, coff, clen, u, out);
+  }
   GAT_LAUNCH_CHECK("edr_wave_kernel");
   return GAT_OK;
 }

@@ -85,6 +85,53 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, uint64_t n
 // memory access (its node loads).
 constexpr uint32_t kEmitCap = 256;
 
+// Per-wave buffer of selected leaves in LDS, flushed to the global (leaf,
+// query) list with one atomic per flush.  Every lane calls every member.
+struct LeafEmitter {
+  uint32_t* buf;
+  uint32_t n;
+  uint32_t q;
+  uint32_t lane;
+  uint2* lq;
+  uint64_t cap;
+  unsigned long long* total;
+  uint32_t* node_qcount;
+  uint64_t cand;  // this lane's share of the rectangle's candidate points
+
+  // Records the lanes with emit set (their leaf idx holds cnt points); flushes
+  // when the buffer could not take another full wave.
+  __device__ void push(bool emit, uint32_t idx, uint32_t cnt) {
+    const uint64_t em = __ballot(emit);
+    if (emit) {
+      buf[n + lanes_below(em)] = idx;
+      atomicAdd(node_qcount + idx, 1u);
+      cand += cnt;
+    }
+    n += uint32_t(__popcll(em));
+  }
+  __device__ void maybe_flush() {
+    if (n > kEmitCap - kWave) flush();
+  }
+  __device__ void flush() {
+    if (n == 0) return;
+    __syncthreads();  // the buffer writes of every lane are visible
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
+    base = uniform64(base);
+    for (uint32_t e = lane; e < n; e += kWave) {
+      const uint64_t at = base + e;
+      if (at < cap) lq[at] = make_uint2(buf[e], q);
+    }
+    n = 0;
+    __syncthreads();  // the buffer is reused only after every lane has read it
+  }
+  __device__ uint64_t wave_cand() {
+    uint64_t s = cand;
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    return s;
+  }
+};
+
 __global__ __launch_bounds__(kWave) void tree_select_kernel(
     const Node* __restrict__ nodes, uint32_t nnodes, uint32_t nchild, const float4* __restrict__ boxes,
     const uint32_t* __restrict__ order, uint32_t q0, uint32_t nq, uint32_t* __restrict__ node_qcount,
@@ -97,19 +144,8 @@ __global__ __launch_bounds__(kWave) void tree_select_kernel(
     const uint32_t q = order ? uniform(order[i]) : q0 + i;
     if (q - q0 >= nq) continue;  // an order entry outside the batch (uniform)
     const float4 box = boxes[q];
-    uint64_t mine = 0;
-    uint32_t top = 1, nemit = 0;
-    auto flush = [&]() {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(lq_total, (unsigned long long)nemit);
-      base = uniform64(base);
-      for (uint32_t e = lane; e < nemit; e += kWave) {
-        const uint64_t at = base + e;
-        if (at < lq_cap) lq[at] = make_uint2(emitted[e], q);
-      }
-      nemit = 0;
-      __syncthreads();  // the buffer is reused only after every lane has read it
-    };
+    LeafEmitter out{emitted, 0, q, lane, lq, lq_cap, lq_total, node_qcount, 0};
+    uint32_t top = 1;
     if (lane == 0) stack[0] = 0;
     __syncthreads();
     while (top > 0) {
@@ -138,13 +174,7 @@ __global__ __launch_bounds__(kWave) void tree_select_kernel(
           }
         }
       }
-      const uint64_t em = __ballot(emit);
-      if (emit) {
-        emitted[nemit + lanes_below(em)] = idx;
-        atomicAdd(node_qcount + idx, 1u);
-        mine += cnt;
-      }
-      nemit += uint32_t(__popcll(em));
+      out.push(emit, idx, cnt);
       const uint64_t dm = __ballot(desc);
       if (desc) {
         const uint32_t at = top + lanes_below(dm) * nchild;
@@ -152,11 +182,58 @@ __global__ __launch_bounds__(kWave) void tree_select_kernel(
       }
       top += uint32_t(__popcll(dm)) * nchild;
       __syncthreads();
-      if (nemit > kEmitCap - kWave) flush();
+      out.maybe_flush();
     }
-    if (nemit > 0) flush();
-    for (int off = kWave / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
-    if (lane == 0) cand[q] = mine;
+    out.flush();
+    const uint64_t total = out.wave_cand();
+    if (lane == 0) cand[q] = total;
+  }
+}
+
+// FSG::cellRange (FSG.cpp:183-186): the monotone float formula, clamped to the grid.
+__device__ __forceinline__ int clamp_cell(float v, int hi) {
+  if (!(v > -1.0f)) return 0;  // also NaN
+  if (!(v < float(hi))) return hi - 1;
+  const int c = int(v);
+  return c < 0 ? 0 : (c > hi - 1 ? hi - 1 : c);
+}
+
+// Flat-grid selection (FSG.cpp:204-312): a rectangle visits every cell of its
+// column range g1..g2 and row range g3..g4; cells[] holds one leaf record per
+// cell (row-major), 64 cells per step.
+__global__ __launch_bounds__(kWave) void grid_select_kernel(
+    const Node* __restrict__ cells, int nx, int ny, float xmin, float ymax, float cs,
+    const float4* __restrict__ boxes, const uint32_t* __restrict__ order, uint32_t q0, uint32_t nq,
+    uint32_t* __restrict__ node_qcount, uint64_t* __restrict__ cand, uint2* __restrict__ lq, uint64_t lq_cap,
+    unsigned long long* __restrict__ lq_total) {
+  __shared__ uint32_t emitted[kEmitCap];
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t i = blockIdx.x; i < nq; i += gridDim.x) {
+    const uint32_t q = order ? uniform(order[i]) : q0 + i;
+    if (q - q0 >= nq) continue;
+    const float4 box = boxes[q];
+    LeafEmitter out{emitted, 0, q, lane, lq, lq_cap, lq_total, node_qcount, 0};
+    const int g1 = clamp_cell((box.x - xmin) / cs, nx), g2 = clamp_cell((box.z - xmin) / cs, nx);
+    const int g3 = clamp_cell((ymax - box.w) / cs, ny), g4 = clamp_cell((ymax - box.y) / cs, ny);
+    if (g1 <= g2 && g3 <= g4) {
+      const uint32_t w = uint32_t(g2 - g1 + 1);
+      const uint32_t ncell = w * uint32_t(g4 - g3 + 1);  // <= nx * ny <= 2^30
+      for (uint32_t base = 0; base < ncell; base += kWave) {
+        const uint32_t c = base + lane;
+        bool emit = false;
+        uint32_t idx = 0, cnt = 0;
+        if (c < ncell) {
+          idx = uint32_t(g3 + int(c / w)) * uint32_t(nx) + uint32_t(g1 + int(c % w));
+          cnt = cells[idx].count;
+          emit = cnt > 0;
+        }
+        out.push(emit, idx, cnt);
+        out.maybe_flush();
+      }
+    }
+    out.flush();
+    const uint64_t total = out.wave_cand();
+    if (lane == 0) cand[q] = total;
   }
 }
 
@@ -343,14 +420,13 @@ __global__ void leaf_scatter_kernel(const uint2* __restrict__ lq, const unsigned
 }
 
 // ---------------------------------------------------------------- refine
-// One wave per work item: lane j loads list entry j of the item (rectangle,
-// output region, id) while the leaf's points and ids load; then per chunk of
-// up to 256 points: pass 1 counts each rectangle's hits, one vector atomic
-// reserves every rectangle's output slots at once, pass 2 recomputes the same
-// ballots and writes the pairs.  The next item's record is fetched while the
-// current one is processed, so an item costs two dependent memory accesses
-// (its entries and points, then the atomics) whatever its number of
-// rectangles.
+// One wave per work item, chunk by chunk (up to 256 leaf points): lane j holds
+// list entry j of the item (rectangle, output region, id); pass 1 counts each
+// rectangle's hits, one vector atomic reserves every rectangle's output slots
+// at once, pass 2 recomputes the same ballots and writes the pairs.  The loop
+// is software-pipelined: the next chunk's points and entries are loaded while
+// the current chunk's atomics are in flight, so a wave waits on one memory
+// round trip per chunk rather than two.
 constexpr int kRefBlock = 256;
 constexpr int kSlots = 4;
 constexpr int kSpan = kWave * kSlots;
@@ -363,6 +439,104 @@ __device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t j) {
   return uint32_t(__builtin_amdgcn_readlane(int(v), int(j)));
 }
 
+// One chunk of a work item in registers.
+struct Chunk {
+  float2 v[kSlots];
+  uint32_t id[kSlots];
+  uint32_t pid[kSlots];
+  float4 bj;       // lane j: rectangle j (NaN when absent: no hits)
+  uint64_t basej;  // lane j: its output region
+  uint32_t qj;     // lane j: its id
+  uint32_t left;   // points in the chunk (uniform)
+  uint32_t nql;    // rectangles of the item (uniform)
+  uint32_t myhits, slot;
+};
+
+template <bool kPacked>
+__device__ __forceinline__ void load_chunk(Chunk& c, const Store& store, const ListEntry* __restrict__ qlist,
+                                           uint32_t nqueries, uint32_t start, uint32_t left, uint32_t list,
+                                           uint32_t nql, uint32_t lane) {
+  const float nan = __builtin_nanf("");
+  c.left = left;
+  c.nql = nql;
+  c.qj = 0xFFFFFFFFu;
+  c.bj = make_float4(nan, nan, nan, nan);
+  c.basej = 0;
+  if (lane < nql) {
+    const ListEntry e = qlist[uint64_t(list) + lane];
+    if (e.q < nqueries) {
+      c.qj = e.q;
+      c.bj = e.box;
+      c.basej = e.base;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const uint32_t i = k * kWave + lane;
+    c.v[k] = make_float2(nan, 0.0f);
+    c.id[k] = 0;
+    c.pid[k] = 0;
+    if (uint32_t(k * kWave) < left && i < left) {
+      const uint64_t at = uint64_t(start) + i;
+      c.v[k] = store.xy[at];
+      c.id[k] = store.id[at];
+      if constexpr (!kPacked) c.pid[k] = store.pid[at];
+    }
+  }
+}
+
+__device__ __forceinline__ float4 lane_box(const Chunk& c, uint32_t j) {
+  return make_float4(readlane_f(c.bj.x, j), readlane_f(c.bj.y, j), readlane_f(c.bj.z, j), readlane_f(c.bj.w, j));
+}
+
+// Pass 1 and the reservation: hits per rectangle into lane j, then one atomic
+// per rectangle with hits (its result is consumed by pass 2).
+__device__ __forceinline__ void count_and_reserve(Chunk& c, uint32_t* __restrict__ qcount, uint32_t lane) {
+  uint32_t myhits = 0;
+  for (uint32_t j = 0; j < c.nql; ++j) {
+    const float4 box = lane_box(c, j);
+    uint32_t h = 0;
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k)
+      if (uint32_t(k * kWave) < c.left) h += uint32_t(__popcll(__ballot(in_box(box, c.v[k].x, c.v[k].y))));
+    if (lane == j) myhits = h;
+  }
+  c.myhits = myhits;
+  c.slot = 0;
+  if (myhits > 0) c.slot = atomicAdd(qcount + c.qj, myhits);
+}
+
+template <bool kPacked>
+__device__ __forceinline__ void write_pairs(const Chunk& c, const Store& store, int2* __restrict__ pairs,
+                                            uint64_t capacity, uint32_t lane) {
+  const uint32_t pid_mask = (1u << store.pid_bits) - 1u;
+  const uint64_t firstj = c.basej + c.slot;
+  for (uint32_t j = 0; j < c.nql; ++j) {
+    if (readlane_u(c.myhits, j) == 0) continue;
+    const float4 box = lane_box(c, j);
+    const uint64_t first =
+        uint64_t(readlane_u(uint32_t(firstj), j)) | (uint64_t(readlane_u(uint32_t(firstj >> 32), j)) << 32);
+    uint32_t before = 0;
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) {
+      if (uint32_t(k * kWave) < c.left) {
+        const uint64_t mask = __ballot(in_box(box, c.v[k].x, c.v[k].y));
+        const uint64_t at = first + before + lanes_below(mask);
+        if (((mask >> lane) & 1) && at < capacity) {
+          int2 p;
+          if constexpr (kPacked) {
+            p = make_int2(int(c.id[k] >> store.pid_bits), int(c.id[k] & pid_mask));
+          } else {
+            p = make_int2(int(c.id[k]), int(c.pid[k]));
+          }
+          pairs[at] = p;
+        }
+        before += uint32_t(__popcll(mask));
+      }
+    }
+  }
+}
+
 template <bool kPacked>
 __global__ __launch_bounds__(kRefBlock) void leaf_refine_kernel(
     Store store, const Item* __restrict__ items, const uint32_t* __restrict__ item_total, uint64_t items_cap,
@@ -373,87 +547,50 @@ __global__ __launch_bounds__(kRefBlock) void leaf_refine_kernel(
   const uint64_t nwaves = uint64_t(gridDim.x) * (kRefBlock / kWave);
   const uint64_t total = *item_total;
   const uint64_t nitems = total < items_cap ? total : items_cap;
-  const uint32_t pid_mask = (1u << store.pid_bits) - 1u;
-  const float nan = __builtin_nanf("");
-  Item next = wave < nitems ? items[wave] : Item{0, 0, 0, 0};
-  for (uint64_t t = wave; t < nitems; t += nwaves) {
-    const uint32_t start = uniform(next.start), count = uniform(next.count);
-    const uint32_t list = uniform(next.list), nql = uniform(next.n);
-    if (t + nwaves < nitems) next = items[t + nwaves];
-    if (uint64_t(start) + count > store.npoints || nql == 0 || nql > kMaxQb || uint64_t(list) + nql > qlist_cap)
-      continue;
-    // Lane j: rectangle j of the item.  Invalid ones get a NaN box (no hits).
-    uint32_t qj = 0xFFFFFFFFu;
-    float4 bj = make_float4(nan, nan, nan, nan);
-    uint64_t basej = 0;
-    if (lane < nql) {
-      const ListEntry e = qlist[uint64_t(list) + lane];
-      if (e.q < nqueries) {
-        qj = e.q;
-        bj = e.box;
-        basej = e.base;
+  if (wave >= nitems) return;
+  // Cursor: item t, chunk offset b; an item that fails validation has no points.
+  uint64_t t = wave;
+  uint32_t b = 0;
+  uint32_t start = 0, count = 0, list = 0, nql = 0;
+  auto take = [&](const Item& it) {
+    start = uniform(it.start);
+    count = uniform(it.count);
+    list = uniform(it.list);
+    nql = uniform(it.n);
+    if (uint64_t(start) + count > store.npoints || nql > kMaxQb || uint64_t(list) + nql > qlist_cap) {
+      count = 0;
+      nql = 0;
+    }
+  };
+  take(items[t]);
+  Item next = t + nwaves < nitems ? items[t + nwaves] : Item{0, 0, 0, 0};
+  Chunk cur;
+  {
+    const uint32_t left = count < uint32_t(kSpan) ? count : uint32_t(kSpan);
+    load_chunk<kPacked>(cur, store, qlist, nqueries, start, left, list, nql, lane);
+  }
+  count_and_reserve(cur, qcount, lane);
+  for (;;) {
+    b += kSpan;
+    if (b >= count) {
+      t += nwaves;
+      b = 0;
+      if (t < nitems) {
+        take(next);
+        if (t + nwaves < nitems) next = items[t + nwaves];
       }
     }
-    for (uint32_t base = 0; base < count; base += kSpan) {
-      const uint32_t left = count - base;
-      float2 v[kSlots];
-      uint32_t id[kSlots];
-      uint32_t pid[kSlots];
-#pragma unroll
-      for (int k = 0; k < kSlots; ++k) {
-        const uint32_t i = k * kWave + lane;
-        v[k] = make_float2(nan, 0.0f);
-        id[k] = 0;
-        pid[k] = 0;
-        if (uint32_t(k * kWave) < left && i < left) {
-          const uint64_t at = uint64_t(start) + base + i;
-          v[k] = store.xy[at];
-          id[k] = store.id[at];
-          if constexpr (!kPacked) pid[k] = store.pid[at];
-        }
-      }
-      // Pass 1: hits per rectangle, kept in lane j.
-      uint32_t myhits = 0;
-      for (uint32_t j = 0; j < nql; ++j) {
-        const float4 box = make_float4(readlane_f(bj.x, j), readlane_f(bj.y, j), readlane_f(bj.z, j),
-                                       readlane_f(bj.w, j));
-        uint32_t h = 0;
-#pragma unroll
-        for (int k = 0; k < kSlots; ++k)
-          if (uint32_t(k * kWave) < left) h += uint32_t(__popcll(__ballot(in_box(box, v[k].x, v[k].y))));
-        if (lane == j) myhits = h;
-      }
-      // One atomic per rectangle with hits, all issued together.
-      uint32_t slot = 0;
-      if (myhits > 0) slot = atomicAdd(qcount + qj, myhits);
-      const uint64_t firstj = basej + slot;
-      // Pass 2: the same ballots, now written out.
-      for (uint32_t j = 0; j < nql; ++j) {
-        if (readlane_u(myhits, j) == 0) continue;
-        const float4 box = make_float4(readlane_f(bj.x, j), readlane_f(bj.y, j), readlane_f(bj.z, j),
-                                       readlane_f(bj.w, j));
-        const uint64_t first =
-            uint64_t(readlane_u(uint32_t(firstj), j)) | (uint64_t(readlane_u(uint32_t(firstj >> 32), j)) << 32);
-        uint32_t before = 0;
-#pragma unroll
-        for (int k = 0; k < kSlots; ++k) {
-          if (uint32_t(k * kWave) < left) {
-            const uint64_t mask = __ballot(in_box(box, v[k].x, v[k].y));
-            const uint64_t at = first + before + lanes_below(mask);
-            if (((mask >> lane) & 1) && at < capacity) {
-              int2 p;
-              if constexpr (kPacked) {
-                p = make_int2(int(id[k] >> store.pid_bits), int(id[k] & pid_mask));
-              } else {
-                p = make_int2(int(id[k]), int(pid[k]));
-              }
-              pairs[at] = p;
-            }
-            before += uint32_t(__popcll(mask));
-          }
-        }
-      }
+    const bool more = t < nitems;
+    Chunk nxt;
+    if (more) {
+      const uint32_t rest = count - b;
+      const uint32_t left = rest < uint32_t(kSpan) ? rest : uint32_t(kSpan);
+      load_chunk<kPacked>(nxt, store, qlist, nqueries, start + b, left, list, nql, lane);
     }
+    write_pairs<kPacked>(cur, store, pairs, capacity, lane);
+    if (!more) break;
+    cur = nxt;
+    count_and_reserve(cur, qcount, lane);
   }
 }
 
@@ -519,6 +656,24 @@ int gat_tree_select(const gat_tree_node* nodes, int64_t nnodes, int32_t nchild, 
       order, uint32_t(q0), uint32_t(nq), node_qcount, cand, reinterpret_cast<uint2*>(leaf_query), uint64_t(lq_capacity),
       reinterpret_cast<unsigned long long*>(lq_total), status);
   GAT_LAUNCH_CHECK("tree_select_kernel");
+  return GAT_OK;
+}
+
+int gat_grid_select(const gat_tree_node* cells, int32_t nx, int32_t ny, float xmin, float ymax, float cell_size,
+                    const float* boxes, const uint32_t* order, int64_t q0, int64_t nq, uint32_t* node_qcount,
+                    uint64_t* cand, uint32_t* leaf_query, int64_t lq_capacity, uint64_t* lq_total, void* stream) {
+  GAT_REQUIRE(cells && nx > 0 && ny > 0 && int64_t(nx) * ny <= (int64_t(1) << 30) && cell_size > 0.0f,
+              "gat_grid_select: bad grid");
+  GAT_REQUIRE(q0 >= 0 && nq >= 0 && q0 + nq <= int64_t(UINT32_MAX), "gat_grid_select: bad query range");
+  GAT_REQUIRE(lq_capacity >= 0 && lq_total && node_qcount, "gat_grid_select: bad outputs");
+  if (nq == 0) return GAT_OK;
+  GAT_REQUIRE(boxes && cand && (lq_capacity == 0 || leaf_query), "gat_grid_select: null pointer");
+  const int blocks = grid_for(uint64_t(nq), 1, 40);
+  grid_select_kernel<<<dim3{uint32_t(blocks)}, dim3{kWave}, 0, gat::as_stream(stream)>>>(
+      reinterpret_cast<const Node*>(cells), nx, ny, xmin, ymax, cell_size, reinterpret_cast<const float4*>(boxes),
+      order, uint32_t(q0), uint32_t(nq), node_qcount, cand, reinterpret_cast<uint2*>(leaf_query),
+      uint64_t(lq_capacity), reinterpret_cast<unsigned long long*>(lq_total));
+  GAT_LAUNCH_CHECK("grid_select_kernel");
   return GAT_OK;
 }
 

@@ -74,7 +74,25 @@ int FSG::addDatasetToGrid(const TrajectoryView& db, bool upload) {
       tp[2 * d + 1] = int32_t(i - b);
     }
   }
-  return engine_.upload(xy.data(), tp.data(), N);
+  if (int rc = engine_.upload(xy.data(), tp.data(), N)) return rc;
+  std::vector<gat_tree_node> cells(static_cast<size_t>(cellnum));
+  for (int64_t c = 0; c < cellnum; ++c) {
+    gat_tree_node& d = cells[size_t(c)];
+    d = gat_tree_node{{0.0f, 0.0f, 0.0f, 0.0f}, -1, uint32_t(cellStart[c]), uint32_t(cellStart[c + 1] - cellStart[c]), 0u};
+  }
+  return tree_.uploadGrid(cells, cell_num_x, cell_num_y, range.xmin, range.ymax, cell_size, engine_.stream());
+}
+
+int FSG::rangeQueryBatchedGPU(const MBB* bounds, int rangeNum, RangeBatch& out, bool graph) {
+  if (rangeNum < 0 || (rangeNum > 0 && !bounds)) {
+    gath_set_error("FSG::rangeQueryBatchedGPU: bad arguments");
+    return GAT_ERR_ARG;
+  }
+  if (tree_.nnodes == 0) {
+    gath_set_error("FSG::rangeQueryBatchedGPU: no device grid (build the index with upload)");
+    return GAT_ERR_STATE;
+  }
+  return engine_.runTree(tree_, bounds, rangeNum, out, graph);
 }
 
 bool FSG::cellRange(const MBB& q, int* g1, int* g2, int* g3, int* g4) const {

@@ -32,6 +32,11 @@ class FSG {
   bool cellRange(const MBB& q, int* g1, int* g2, int* g3, int* g4) const;
   void selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
                         std::vector<uint64_t>& candidates) const;
+  // Batched form (no reference counterpart), as Grid::rangeQueryBatchedGPU:
+  // the cells are the leaves, each rectangle enumerating its cell range on the
+  // device.
+  int rangeQueryBatchedGPU(const MBB* bounds, int rangeNum, RangeBatch& out, bool graph = false);
+  int uniqueCandidates(int64_t* points) { return engine_.uniqueCandidates(tree_, points); }
 
   MBB range;
   float cell_size = 0;
@@ -47,4 +52,5 @@ class FSG {
   void addRows(const MBB& q, uint32_t qid, TaskList& list) const;
   int threads() const;
   RangeEngine engine_;
+  DeviceTree tree_;
 };

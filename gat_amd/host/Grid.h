@@ -102,6 +102,12 @@ class Grid {
   // on the device once.
   const std::vector<int32_t>& mismatchTable();
 
+  // Batched-EDR tunables: pairs whose longer trajectory has at most edrSplit
+  // points form the short bucket; workgroups per CU of each bucket's launch
+  // (0: the kernel's default for its LDS class).
+  int edrSplit = 512;
+  int edrPerCuShort = 0, edrPerCuLong = 0;
+
   // Run-time EDR threshold (the reference's compile-time EPSILON, ConstDefine.h:28).
   double epsilon = 0.005;
   int hostThreads = 0;  // 0: hardware concurrency (capped at 16)
@@ -120,6 +126,7 @@ class Grid {
   std::vector<int64_t> cellStart;    // cellnum + 1 prefix of per-cell point counts
 
   RangeEngine& engine() { return engine_; }
+  const DeviceTree& deviceTree() const { return tree_; }
   double lastSelectMs = 0;
   float lastKernelMs = 0;
 

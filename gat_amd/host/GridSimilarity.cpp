@@ -207,11 +207,17 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   const int64_t np = int64_t(pairs.size() / 2);
   dist.assign(size_t(np), -1);
   if (np == 0) return GAT_OK;
-  // Counting sort by (strips, longer length), descending.
+  // Two length buckets: pairs whose longer trajectory is at most edrSplit
+  // points run in a launch with a small LDS row buffer (more waves per CU),
+  // the rest in one sized for the longest.  Inside a bucket, a counting sort
+  // by (strips, longer length), descending, and jobs of up to four pairs with
+  // the same strip count.
   constexpr int kM = GAT_MAX_TRAJ_LEN + 1;
   constexpr int kS = GAT_MAX_TRAJ_LEN / 16 + 1;
+  const int split = std::max(1, std::min(edrSplit, GAT_MAX_TRAJ_LEN));
   std::vector<int32_t> key(static_cast<size_t>(np));
   int64_t cells = 0;
+  int maxlen[2] = {1, 1};
   for (int64_t i = 0; i < np; ++i) {
     const int q = pairs[2 * i], c = pairs[2 * i + 1];
     if (q < 0 || q >= queryNum || c < 0 || c >= trajNum) {
@@ -221,22 +227,26 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     const int a = queryLenHost_[q], b = trajLenHost_[c];
     const int n = std::min(a, b), m = std::max(a, b);
     cells += int64_t(n) * m;
-    key[i] = (kS - 1 - strips_of(n)) * kM + (kM - 1 - m);
+    const int bucket = m <= split ? 0 : 1;
+    maxlen[bucket] = std::max(maxlen[bucket], std::min(m, GAT_MAX_TRAJ_LEN));
+    key[i] = (bucket * kS + (kS - 1 - strips_of(n))) * kM + (kM - 1 - m);
   }
-  std::vector<int64_t> cnt(size_t(kS) * kM + 1, 0);
+  std::vector<int64_t> cnt(size_t(2 * kS) * kM + 1, 0);
   for (int64_t i = 0; i < np; ++i) ++cnt[key[i] + 1];
   for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
   std::vector<int64_t> perm(static_cast<size_t>(np));
   for (int64_t i = 0; i < np; ++i) perm[cnt[key[i]]++] = i;
   std::vector<int32_t> sorted(size_t(np) * 2);
   std::vector<gat_edr_job> jobs;
-  jobs.reserve(size_t(np / 4 + 1));
+  jobs.reserve(size_t(np / 4 + 2));
+  int64_t bucketJobs[2] = {0, 0};
   int64_t i = 0;
   while (i < np) {
-    const int32_t k0 = key[perm[i]] / kM;
+    const int32_t k0 = key[perm[i]] / kM;  // bucket and strip count
     int32_t c = 0;
     while (c < 4 && i + c < np && key[perm[i + c]] / kM == k0) ++c;
     jobs.push_back(gat_edr_job{int32_t(i), c});
+    ++bucketJobs[k0 / kS];
     i += c;
   }
   for (int64_t j = 0; j < np; ++j) {
@@ -248,17 +258,19 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   if (int rc = upload_vec(jobsDev_, jobs, s, "job upload")) return rc;
   if (int rc = outDev_.reserve(size_t(np) * 4)) return rc;
   if (int rc = counterDev_.reserve(16)) return rc;
-  int maxlen = 1;
-  for (int v : queryLenHost_) maxlen = std::max(maxlen, v);
-  for (int v : trajLenHost_) maxlen = std::max(maxlen, v);
   const double h1 = now_ms();
   if (int rc = edrTimer_.start(s)) return rc;
-  GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>(), int64_t(jobs.size()),
-                           counterDev_.as<int32_t>(), queryXY_.as<float>(), queryOff64_.as<int64_t>(),
-                           queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(), trajOff_.as<int64_t>(),
-                           trajLen_.as<int32_t>(), trajNum, std::min(maxlen, GAT_MAX_TRAJ_LEN), epsilon,
-                           outDev_.as<int32_t>(), s),
-           "batched EDR");
+  int64_t j0 = 0;
+  for (int bucket = 0; bucket < 2; ++bucket) {
+    if (bucketJobs[bucket] == 0) continue;
+    GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>() + j0, bucketJobs[bucket],
+                             counterDev_.as<int32_t>(), queryXY_.as<float>(), queryOff64_.as<int64_t>(),
+                             queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(), trajOff_.as<int64_t>(),
+                             trajLen_.as<int32_t>(), trajNum, maxlen[bucket], epsilon,
+                             bucket == 0 ? edrPerCuShort : edrPerCuLong, outDev_.as<int32_t>(), s),
+             "batched EDR");
+    j0 += bucketJobs[bucket];
+  }
   if (int rc = edrTimer_.stop(s)) return rc;
   std::vector<int32_t> out(static_cast<size_t>(np));
   GATH_TRY(gat_memcpy_d2h(out.data(), outDev_.get(), size_t(np) * 4, s), "batched EDR readback");

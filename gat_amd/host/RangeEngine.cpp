@@ -140,6 +140,7 @@ int RangeEngine::runPairs(const MBB* bounds, int nq, const Selector& selector, i
   if (int rc = stream_.create()) return rc;
   out.queryNum = nq;
   out.hostStale = false;
+  out.stream = stream_.get();
   double t0 = now_ms();
   std::vector<gat_range_task> tasks;
   std::vector<uint64_t> cand;
@@ -206,9 +207,9 @@ int RangeBatch::syncHost() {
   if (!hostStale) return GAT_OK;
   count.resize(size_t(queryNum));
   base.resize(size_t(queryNum));
-  GATH_TRY(gat_memcpy_d2h(count.data(), qcount.get(), size_t(queryNum) * 4, nullptr), "count readback");
-  GATH_TRY(gat_memcpy_d2h(base.data(), qbase.get(), size_t(queryNum) * 8, nullptr), "region readback");
-  GATH_TRY(gat_device_sync(), "count readback");
+  GATH_TRY(gat_memcpy_d2h(count.data(), qcount.get(), size_t(queryNum) * 4, stream), "count readback");
+  GATH_TRY(gat_memcpy_d2h(base.data(), qbase.get(), size_t(queryNum) * 8, stream), "region readback");
+  GATH_TRY(gat_stream_sync(stream), "count readback");
   hostStale = false;
   return GAT_OK;
 }
@@ -224,10 +225,10 @@ int RangeBatch::fetch(int32_t* out, int64_t capacityPairs) {
   }
   for (int q = 0; q < queryNum; ++q) {
     if (count[q] == 0) continue;
-    GATH_TRY(gat_memcpy_d2h(out + 2 * base[q], pairs.as<int32_t>() + 2 * base[q], size_t(count[q]) * 8, nullptr),
+    GATH_TRY(gat_memcpy_d2h(out + 2 * base[q], pairs.as<int32_t>() + 2 * base[q], size_t(count[q]) * 8, stream),
              "pair readback");
   }
-  GATH_TRY(gat_device_sync(), "pair readback");
+  GATH_TRY(gat_stream_sync(stream), "pair readback");
   return GAT_OK;
 }
 
@@ -235,11 +236,18 @@ int RangeBatch::digests(uint64_t* out) const {
   if (queryNum == 0) return GAT_OK;
   DeviceBuffer d;
   if (int rc = d.reserve(size_t(queryNum) * 8)) return rc;
+  return digestsInto(d.as<uint64_t>(), out);
+}
+
+int RangeBatch::digestsInto(uint64_t* dev, uint64_t* out) const {
+  if (queryNum == 0) return GAT_OK;
   GATH_TRY(gat_pairs_digest(pairs.as<int32_t>(), qbase.as<uint64_t>(), qcount.as<uint32_t>(), queryNum,
-                            int64_t(pairs.size() / 8), d.as<uint64_t>(), nullptr),
+                            int64_t(pairs.size() / 8), dev, stream),
            "pair digest");
-  GATH_TRY(gat_memcpy_d2h(out, d.get(), size_t(queryNum) * 8, nullptr), "digest readback");
-  GATH_TRY(gat_device_sync(), "digest readback");
+  if (out) {
+    GATH_TRY(gat_memcpy_d2h(out, dev, size_t(queryNum) * 8, stream), "digest readback");
+    GATH_TRY(gat_stream_sync(stream), "digest readback");
+  }
   return GAT_OK;
 }
 
@@ -265,7 +273,62 @@ int DeviceTree::upload(const std::vector<gat_tree_node>& host, int nc, int d, vo
   return GAT_OK;
 }
 
+int DeviceTree::uploadGrid(const std::vector<gat_tree_node>& cells, int gx, int gy, float x0, float y1, float cs,
+                           void* stream) {
+  if (gx <= 0 || gy <= 0 || cells.size() != size_t(gx) * size_t(gy) || !(cs > 0.0f)) {
+    gath_set_error("device grid: %d x %d cells with %zu records", gx, gy, cells.size());
+    return GAT_ERR_ARG;
+  }
+  if (int rc = nodes.reserve(cells.size() * sizeof(gat_tree_node))) return rc;
+  GATH_TRY(gat_memcpy_h2d(nodes.get(), cells.data(), cells.size() * sizeof(gat_tree_node), stream), "grid upload");
+  GATH_TRY(gat_stream_sync(stream), "grid upload");
+  nnodes = int64_t(cells.size());
+  nchild = 0;
+  depth = 0;
+  nx = gx;
+  ny = gy;
+  xmin = x0;
+  ymax = y1;
+  cellSize = cs;
+  return GAT_OK;
+}
+
 RangeEngine::~RangeEngine() { dropGraph(); }
+
+int RangeEngine::attachStore(const RangeEngine& src) {
+  if (int rc = stream_.create()) return rc;
+  store_ = src.store_;
+  n_ = src.n_;
+  return GAT_OK;
+}
+
+int RangeEngine::reserveBatched(const DeviceTree& tree, int maxQueries, int64_t maxCandidates, RangeBatch& out) {
+  if (int rc = stream_.create()) return rc;
+  const int64_t N = tree.nnodes;
+  const int64_t nq = std::max(maxQueries, 1);
+  dropGraph();
+  if (int rc = devBoxes_.reserve(align_up(size_t(nq) * 16, 256) + size_t(nq) * 4)) return rc;
+  if (int rc = hostStage_.reserve(align_up(size_t(nq) * 16, 256) + size_t(nq) * 4)) return rc;
+  if (int rc = nodeCount_.reserve(size_t(N) * 4)) return rc;
+  if (int rc = nodeFill_.reserve(size_t(N) * 4)) return rc;
+  if (int rc = nodeOff_.reserve(size_t(N) * 8)) return rc;
+  if (int rc = tiles_.reserve(size_t(std::max(2 * gat_scan_tiles(N) * 4, gat_scan_tiles(nq) * 8)))) return rc;
+  if (int rc = cand_.reserve(size_t(nq) * 8)) return rc;
+  if (int rc = out.qbase.reserve(size_t(nq) * 8)) return rc;
+  if (int rc = out.qcount.reserve(size_t(nq) * 4)) return rc;
+  if (int rc = counters_.reserve(256)) return rc;
+  if (int rc = countersHost_.reserve(256)) return rc;
+  // Leaves average well over 16 points, so candidates / 16 entries is ample.
+  const int64_t entries = std::max<int64_t>(maxCandidates / 16 + nq * 64, 1 << 16);
+  lqCap_ = std::max(lqCap_, entries);
+  qlistCap_ = std::max(qlistCap_, entries);
+  itemsCap_ = std::max(itemsCap_, entries);
+  pairsCap_ = std::max(pairsCap_, std::max<int64_t>(maxCandidates, 1024));
+  if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+  if (int rc = qlist_.reserve(size_t(qlistCap_) * 32)) return rc;
+  if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
+  return out.pairs.reserve(size_t(pairsCap_) * 8);
+}
 
 void RangeEngine::dropGraph() {
   if (graph_) gat_graph_destroy(graph_);
@@ -313,10 +376,17 @@ int RangeEngine::enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out) 
   GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
   const auto* order =
       reinterpret_cast<const uint32_t*>(devBoxes_.as<char>() + align_up(size_t(nq) * 16, 256));
-  GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), N, tree.nchild, devBoxes_.as<float>(), order, 0, nq,
-                           nodeCount_.as<uint32_t>(), cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_,
-                           reinterpret_cast<uint64_t*>(ctr), ctr + 4, s),
-           "tree select");
+  if (tree.flat()) {
+    GATH_TRY(gat_grid_select(tree.nodes.as<gat_tree_node>(), tree.nx, tree.ny, tree.xmin, tree.ymax, tree.cellSize,
+                             devBoxes_.as<float>(), order, 0, nq, nodeCount_.as<uint32_t>(), cand_.as<uint64_t>(),
+                             lq_.as<uint32_t>(), lqCap_, reinterpret_cast<uint64_t*>(ctr), s),
+             "grid select");
+  } else {
+    GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), N, tree.nchild, devBoxes_.as<float>(), order, 0, nq,
+                             nodeCount_.as<uint32_t>(), cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_,
+                             reinterpret_cast<uint64_t*>(ctr), ctr + 4, s),
+             "tree select");
+  }
   GATH_TRY(gat_scan_u64(cand_.as<uint64_t>(), nq, out.qbase.as<uint64_t>(), reinterpret_cast<uint64_t*>(ctr + 2),
                         tiles_.as<uint64_t>(), s),
            "region scan");
@@ -386,6 +456,7 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   out.count.assign(size_t(nq), 0);
   out.base.assign(size_t(nq), 0);
   out.hostStale = false;
+  out.stream = stream_.get();
   out.candidates = out.taskNum = out.pairTotal = out.leafQueries = out.workItems = 0;
   out.replayed = false;
   out.selectMs = out.uploadMs = 0;

@@ -57,6 +57,7 @@ struct RangeBatch {
   DeviceBuffer pairs;           // int32 (tid, pid) pairs
   DeviceBuffer qbase, qcount;   // device copies of base / count
   bool hostStale = false;       // base / count not yet copied from the device
+  void* stream = nullptr;       // the engine stream that produced the batch (readbacks wait on it only)
   // Copies base / count from the device when they are stale.
   int syncHost();
   // Copies the pairs to the host, grouped by query (query q's pairs at
@@ -65,17 +66,29 @@ struct RangeBatch {
   // Per-query order-independent digest (wrapping sum of a 64-bit mix of each
   // (tid, pid) pair), computed on the device.
   int digests(uint64_t* out) const;
+  // The same into a caller-owned device buffer of queryNum entries (no
+  // allocation), copied to `out` when it is not null.
+  int digestsInto(uint64_t* dev, uint64_t* out) const;
 };
 
 // Device copy of an index's tree for the batched form: gat_tree_node in
 // breadth-first order, nchild children per inner node, leaves at depth <= depth.
+// A flat grid (nx, ny > 0) is the same with no inner nodes: nodes[] holds one
+// leaf record per cell, row-major from (xmin, ymax), and selection enumerates
+// a rectangle's cell range instead of walking a tree.
 struct DeviceTree {
   DeviceBuffer nodes;
   int64_t nnodes = 0;
   int nchild = 0;
   int depth = 0;
+  int nx = 0, ny = 0;  // flat grid only
+  float xmin = 0, ymax = 0, cellSize = 0;
+  bool flat() const { return nx > 0; }
   // Checks the traversal stack bound and uploads `host`.
   int upload(const std::vector<gat_tree_node>& host, int nchild, int depth, void* stream);
+  // Uploads one leaf record per cell of an nx x ny grid.
+  int uploadGrid(const std::vector<gat_tree_node>& cells, int nx, int ny, float xmin, float ymax, float cellSize,
+                 void* stream);
 };
 
 class RangeEngine {
@@ -112,6 +125,14 @@ class RangeEngine {
   // selected (what the leaf-major refine must read at least once).
   int uniqueCandidates(const DeviceTree& tree, int64_t* points);
   ~RangeEngine();
+
+  // Shares `src`'s resident store (which must outlive this engine): a second
+  // engine over the same points with its own stream and batch workspace.
+  int attachStore(const RangeEngine& src);
+  // Sizes the batched-form workspace for batches of up to maxQueries
+  // rectangles and maxCandidates candidate points, so later batches within
+  // those bounds allocate nothing.
+  int reserveBatched(const DeviceTree& tree, int maxQueries, int64_t maxCandidates, RangeBatch& out);
 
   // Candidate selection only (host), exposed for tests of the host logic.
   static void select(int nq, const Selector& select, int threads, std::vector<gat_range_task>& tasks,

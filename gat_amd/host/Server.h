@@ -1,0 +1,60 @@
+// Serving loop over a resident GT-quadtree index (no reference counterpart:
+// the reference answers one pre-built batch per call, SystemTest.cpp:24-249).
+//
+// The index's point store, tree and trajectory store stay in HBM.  Requests —
+// range rectangles and EDR top-k queries — arrive over time; a range request
+// costs the device memory of its candidate/result buffers (estimated from its
+// area and the store's density), a top-k request that of its query points and
+// filter survivors.  Continuous batching: each range slot (its own stream and
+// workspace over the shared store) and the top-k slot take, whenever they are
+// free, every waiting request of their kind that fits their memory budget, run
+// the batch with the batched form, and hand each request its result; work of
+// different slots overlaps on the device.  Requests that arrive meanwhile wait
+// for the next free slot.  The plain mode answers one request at a time with
+// the plain drop-in path, for comparison.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "Grid.h"
+#include "gat_host.h"
+
+struct ServeConfig {
+  int rangeSlots = 2;             // range batches in flight (each its own stream and workspace)
+  int maxRangeBatch = 4096;       // rectangles per range batch
+  int maxTopkBatch = 256;         // queries per top-k batch
+  double rangeBudgetBytes = 4e9;  // candidate/result bytes one range batch may reserve
+  double topkBudgetBytes = 1e9;   // query/survivor bytes one top-k batch may reserve
+  int k = 20;
+  double eps = 5e-4;
+  bool batched = true;  // false: the plain per-request path, one request at a time in arrival order
+};
+
+struct ServeStats {
+  int64_t rangeBatches = 0, topkBatches = 0;
+  int64_t rangeDone = 0, topkDone = 0;
+  double wallMs = 0;
+  double peakRangeBytes = 0, peakTopkBytes = 0;  // largest admitted batch estimate
+  double rangeBudgetBytes = 0, topkBudgetBytes = 0;  // budgets after the free-memory check
+};
+
+class Server {
+ public:
+  Server(Grid& grid, const ServeConfig& cfg) : grid_(grid), cfg_(cfg) {}
+  // Replays `reqs` (arrival times in ms from the start of the run, ascending)
+  // and fills one outcome per request; top-k requests also fill k ids and
+  // distances at topkIds / topkDists + i * k (ascending (distance, id)).
+  // Top-k query points are (qlon, qlat)[qoff, qoff + qlen).
+  int run(const gath_request* reqs, int n, const float* qlon, const float* qlat, gath_outcome* out, int32_t* topkIds,
+          int32_t* topkDists, ServeStats* stats);
+
+  // Device bytes the candidate and result buffers of one request need (the
+  // admission estimate).
+  double rangeBytes(const float* box) const;
+  double topkBytes(int qlen) const;
+
+ private:
+  Grid& grid_;
+  ServeConfig cfg_;
+};

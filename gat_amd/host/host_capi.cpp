@@ -7,6 +7,7 @@
 #include "FSG.h"
 #include "Grid.h"
 #include "STIG.h"
+#include "Server.h"
 #include "Synth.h"
 #include "gat_host.h"
 
@@ -206,7 +207,7 @@ int gath_index_range_batched(void* p, const float* boxes, int nq, int graph, int
   int rc;
   if (ix->grid) rc = ix->grid->rangeQueryBatchedGPU(as_boxes(boxes), nq, ix->last, graph != 0);
   else if (ix->stig) rc = ix->stig->rangeQueryBatchedGPU(as_boxes(boxes), nq, ix->last, graph != 0);
-  else return bad("gath_index_range_batched: the FSG index has no tree");
+  else rc = ix->fsg->rangeQueryBatchedGPU(as_boxes(boxes), nq, ix->last, graph != 0);
   if (rc) return rc;
   if (stats) {
     stats[0] = ix->last.pairTotal;
@@ -226,8 +227,9 @@ int gath_index_range_batched(void* p, const float* boxes, int nq, int graph, int
 
 int gath_index_last_unique_candidates(void* p, int64_t* points) {
   Index* ix = as_index(p);
-  if (!ix || !points || ix->fsg) return bad("gath_index_last_unique_candidates: bad arguments");
-  return ix->grid ? ix->grid->uniqueCandidates(points) : ix->stig->uniqueCandidates(points);
+  if (!ix || !points) return bad("gath_index_last_unique_candidates: bad arguments");
+  return ix->grid ? ix->grid->uniqueCandidates(points)
+                  : ix->fsg ? ix->fsg->uniqueCandidates(points) : ix->stig->uniqueCandidates(points);
 }
 
 int gath_index_set_batch_params(void* p, int queries_per_item) {
@@ -271,12 +273,13 @@ int gath_index_last_fetch(void* p, int32_t* pairs, int64_t cap) {
   int64_t at = 0;
   for (int q = 0; q < b.queryNum; ++q) {
     if (b.count[q]) {
-      GATH_TRY(gat_memcpy_d2h(pairs + 2 * at, b.pairs.as<int32_t>() + 2 * b.base[q], size_t(b.count[q]) * 8, nullptr),
+      GATH_TRY(gat_memcpy_d2h(pairs + 2 * at, b.pairs.as<int32_t>() + 2 * b.base[q], size_t(b.count[q]) * 8,
+                              b.stream),
                "pair readback");
     }
     at += b.count[q];
   }
-  GATH_TRY(gat_device_sync(), "pair readback");
+  GATH_TRY(gat_stream_sync(b.stream), "pair readback");
   return GAT_OK;
 }
 
@@ -366,6 +369,41 @@ int gath_grid_mismatch_table(void* p, int32_t* out) {
   const std::vector<int32_t>& F = ix->grid->mismatchTable();
   if (F.empty()) return GAT_ERR_HIP;
   std::copy(F.begin(), F.end(), out);
+  return GAT_OK;
+}
+
+int gath_grid_set_edr_params(void* p, int split, int per_cu_short, int per_cu_long) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || split < 1 || split > GAT_MAX_TRAJ_LEN || per_cu_short < 0 || per_cu_short > 64 ||
+      per_cu_long < 0 || per_cu_long > 64)
+    return bad("gath_grid_set_edr_params: bad arguments");
+  ix->grid->edrSplit = split;
+  ix->grid->edrPerCuShort = per_cu_short;
+  ix->grid->edrPerCuLong = per_cu_long;
+  return GAT_OK;
+}
+
+int gath_serve(void* p, const gath_request* reqs, int n, const float* qlon, const float* qlat, const double* cfg,
+               gath_outcome* out, int32_t* topk_ids, int32_t* topk_dists, double* stats) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || !cfg) return bad("gath_serve: bad arguments (a Grid index and a config are required)");
+  ServeConfig c;
+  c.rangeSlots = int(cfg[0]);
+  c.maxRangeBatch = int(cfg[1]);
+  c.maxTopkBatch = int(cfg[2]);
+  c.rangeBudgetBytes = cfg[3];
+  c.topkBudgetBytes = cfg[4];
+  c.k = int(cfg[5]);
+  c.eps = cfg[6];
+  c.batched = cfg[7] != 0;
+  ServeStats st;
+  Server server(*ix->grid, c);
+  if (int rc = server.run(reqs, n, qlon, qlat, out, topk_ids, topk_dists, &st)) return rc;
+  if (stats) {
+    const double v[9] = {double(st.rangeBatches), double(st.topkBatches), double(st.rangeDone), double(st.topkDone),
+                         st.wallMs, st.peakRangeBytes, st.peakTopkBytes, st.rangeBudgetBytes, st.topkBudgetBytes};
+    for (int i = 0; i < 9; ++i) stats[i] = v[i];
+  }
   return GAT_OK;
 }
 

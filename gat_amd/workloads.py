@@ -87,3 +87,35 @@ CFG5 = Workload("cfg5", 5, 22_700_000, "lognormal", 64, 0.8, 8, 2048, grid_cell=
                 extra={"rate": 2000.0, "range_frac": 0.8, "qlen_lo": 16, "qlen_hi": 512})
 
 CONFIGS = {w.name: w for w in (CFG1, CFG2, CFG3, CFG4, CFG5)}
+
+
+def serve_requests(w: Workload, db: api.TrajSet, seconds: float, rate: float | None = None, seed: int | None = None):
+    """A Poisson request stream for the serving loop: arrivals at `rate`
+    requests/s for `seconds`, a fraction extra["range_frac"] of them range
+    squares (side log-uniform in [rect_side_lo, rect_side_hi], uniform
+    centres), the rest top-k queries with lengths U{qlen_lo..qlen_hi}, every
+    other one a noisy copy of a database trajectory.  Returns (requests,
+    query lon, query lat)."""
+    rate = w.extra["rate"] if rate is None else rate
+    rng = np.random.default_rng(w.seed * 1000 + 17 if seed is None else seed)
+    n = max(int(rate * seconds), 1)
+    arrival = np.cumsum(rng.exponential(1e3 / rate, n))
+    kind = np.where(rng.random(n) < w.extra["range_frac"], api.REQ_RANGE, api.REQ_TOPK).astype(np.int32)
+    req = np.zeros(n, api.REQUEST)
+    req["kind"] = kind
+    req["arrival_ms"] = arrival
+    nr = int((kind == api.REQ_RANGE).sum())
+    if nr:
+        req["box"][kind == api.REQ_RANGE] = api.synth_rects(int(rng.integers(1 << 30)), nr, w.rect_side_lo,
+                                                            w.rect_side_hi, True)
+    nt = n - nr
+    qlon = np.zeros(0, np.float32)
+    qlat = np.zeros(0, np.float32)
+    if nt:
+        lo, hi = w.extra["qlen_lo"], w.extra["qlen_hi"]
+        qlen = api.synth_lengths(int(rng.integers(1 << 30)), nt, "uniform", lo, hi, lo, hi)
+        q = api.synth_queries(int(rng.integers(1 << 30)), qlen, db, copy_mode=2)
+        qlon, qlat = q.lon, q.lat
+        req["qlen"][kind == api.REQ_TOPK] = qlen
+        req["qoff"][kind == api.REQ_TOPK] = q.offsets[:-1]
+    return req, qlon, qlat

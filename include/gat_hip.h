@@ -187,15 +187,19 @@ typedef struct gat_edr_job {
  * through counter_dev (one int32 of scratch, reset by the call); jobs should be
  * ordered longest first.  Both stores must be NaN-padded: at least 16 NaN
  * points before every trajectory and GAT_MAX_TRAJ_LEN + 64 after the last.
- * maxlen (the longest trajectory of any pair) selects the kernel's LDS size;
- * a pair with an id outside [0, nqueries) / [0, ncands) or a length outside
- * 1..maxlen gets -1, and jobs reaching past npairs are skipped.  Asynchronous
- * on `stream`. */
+ * maxlen (the longest trajectory of any pair) selects the kernel's LDS class
+ * (row buffers of 256, 512, 1024 or 2048 columns); a pair with an id outside
+ * [0, nqueries) / [0, ncands) or a length outside 1..maxlen gets -1, and jobs
+ * reaching past npairs are skipped.  per_cu is the number of single-wave
+ * workgroups launched per compute unit (0: the class default).  Callers split
+ * pairs into length buckets and launch each bucket with its own maxlen, so
+ * short pairs are not run at the occupancy of the longest.  Asynchronous on
+ * `stream`. */
 int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job* jobs_dev, int64_t njobs,
                     int32_t* counter_dev, const float* query_xy_dev, const int64_t* query_offset_dev,
                     const int32_t* query_length_dev, int32_t nqueries, const float* cand_xy_dev,
                     const int64_t* cand_offset_dev, const int32_t* cand_length_dev, int32_t ncands, int32_t maxlen,
-                    double eps, int32_t* out_dev, void* stream);
+                    double eps, int32_t per_cu, int32_t* out_dev, void* stream);
 
 /* Candidate filter for exact top-k pruning (no reference counterpart; the
  * reference orders candidates by an unproven frequency-distance bound,
@@ -283,6 +287,17 @@ int gat_scan_leaf_lists(const uint32_t* count_dev, int64_t nnodes, int32_t qb, u
 int gat_tree_select(const gat_tree_node* nodes_dev, int64_t nnodes, int32_t nchild, const float* boxes_dev,
                     const uint32_t* order_dev, int64_t q0, int64_t nq, uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev,
                     int64_t lq_capacity, uint64_t* lq_total_dev, uint32_t* status_dev, void* stream);
+
+/* The same selection for the flat grid (FSG::rangeQueryBatchGPU, FSG.cpp:183-312):
+ * rectangle q visits every cell of its column range g1..g2 and row range
+ * g3..g4 (the float formula of FSG.cpp:183-186, clamped to the grid), cells
+ * numbered row * nx + col from (xmin, ymax).  cells_dev holds one leaf record
+ * per cell (first_child -1, the cell's store run); empty cells are skipped.
+ * Outputs as gat_tree_select. */
+int gat_grid_select(const gat_tree_node* cells_dev, int32_t nx, int32_t ny, float xmin, float ymax, float cell_size,
+                    const float* boxes_dev, const uint32_t* order_dev, int64_t q0, int64_t nq,
+                    uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, int64_t lq_capacity,
+                    uint64_t* lq_total_dev, void* stream);
 
 /* Groups the selected (leaf, query) entries by leaf into self-contained
  * records.  List entry r of leaf L (32 bytes: the rectangle's 4 floats, its

@@ -69,7 +69,7 @@ int gath_index_range_bitmap(void* index, const float* boxes, int nq, uint8_t* ou
  * host selection, task upload, refine kernel.  The result stays on the device
  * until the next call. */
 int gath_index_range_pairs(void* index, const float* boxes, int nq, int64_t* stats, double* ms);
-/* Batched form (Grid and STIG): device selection and the leaf-major join, the
+/* Batched form (Grid, STIG and FSG): device selection and the leaf-major join, the
  * whole batch at once; graph != 0 captures the device pipeline and replays it
  * on later calls that fit.  stats[0..4] = pairs, candidates, (leaf, query)
  * entries, work items, replayed (0/1); ms[0..3] = device selection, wall time,
@@ -78,6 +78,10 @@ int gath_index_range_pairs(void* index, const float* boxes, int nq, int64_t* sta
 int gath_index_range_batched(void* index, const float* boxes, int nq, int graph, int64_t* stats, double* ms);
 /* After a batched run: points of the leaves at least one rectangle selected. */
 int gath_index_last_unique_candidates(void* index, int64_t* points);
+/* Batched-EDR tunables of a Grid: the length split between the short and long
+ * buckets (1..GAT_MAX_TRAJ_LEN) and workgroups per CU of each bucket's launch
+ * (0..64, 0 = the kernel's default). */
+int gath_grid_set_edr_params(void* grid, int split, int per_cu_short, int per_cu_long);
 /* Tunables of the batched form: rectangles per leaf-major work item (1..64). */
 int gath_index_set_batch_params(void* index, int queries_per_item);
 /* The index's tree as the batched form uploads it (breadth-first
@@ -101,6 +105,40 @@ int gath_grid_similarity_batched(void* grid, const int64_t* q_offsets, int nq, c
                                  int k, double eps, int32_t* ids, int32_t* dists, int64_t* stats, double* ms);
 /* The all-mismatch table, (GAT_MAX_TRAJ_LEN + 1)^2 int32. */
 int gath_grid_mismatch_table(void* grid, int32_t* out);
+
+
+/* ---------------------------------------------------------------- serving
+ * The serving loop over a Grid (no reference counterpart): requests arrive at
+ * their arrival times, range slots and the top-k slot batch every waiting
+ * request that fits their device-memory budget, and each request's admission
+ * and completion times are recorded. */
+enum { GATH_REQ_RANGE = 0, GATH_REQ_TOPK = 1 };
+
+typedef struct gath_request {
+  int32_t kind;      /* GATH_REQ_RANGE or GATH_REQ_TOPK */
+  float box[4];      /* range: xmin, ymin, xmax, ymax */
+  int32_t qlen;      /* top-k: query length (1..GAT_MAX_TRAJ_LEN) */
+  int64_t qoff;      /* top-k: first query point in the request point arrays */
+  double arrival_ms; /* from the start of the run; non-decreasing */
+} gath_request;
+
+typedef struct gath_outcome {
+  double admit_ms; /* the batch carrying the request was launched */
+  double done_ms;  /* the request's result was on the host */
+  int32_t batch;   /* batch number (range and top-k batches are numbered separately) */
+  uint32_t count;  /* range: result pairs */
+  uint64_t digest; /* range: order-independent digest of the pairs (gat_pairs_digest) */
+} gath_outcome;
+
+/* cfg[0..7] = range slots, max rectangles per range batch, max queries per
+ * top-k batch, range batch budget (bytes), top-k batch budget (bytes), k, eps,
+ * batched (0: the plain per-request path).  topk_ids / topk_dists receive k
+ * entries per request (used for top-k requests; ids 0 / distance -1 pad a
+ * short answer).  stats[0..8] = range batches, top-k batches, range done,
+ * top-k done, wall ms, largest range batch estimate, largest top-k batch
+ * estimate, range budget and top-k budget after the free-memory check. */
+int gath_serve(void* grid, const gath_request* reqs, int n, const float* qlon, const float* qlat, const double* cfg,
+               gath_outcome* out, int32_t* topk_ids, int32_t* topk_dists, double* stats);
 
 #pragma GCC visibility pop
 

@@ -54,6 +54,20 @@ def test_cfg1_stig_batched(cfg1):
     assert s.range_pairs(edge).candidates == res.candidates
 
 
+@pytest.mark.parametrize("cell", [0.01, 1 / 16, 1 / 1024])
+def test_fsg_batched(cfg1, cell):
+    """The flat grid's cells as leaves: every cell of the clamped row/column
+    range (FSG.cpp:183-312) against inclusive brute force."""
+    db, rects = cfg1
+    f = api.FsgIndex(W.UNIT_BOX, cell).build(db)
+    res = f.range_batched(rects)
+    _check_pairs(res, _sets(O.brute_range(db, rects)), rects)
+    edge = _edge_rects(db, cell, 40, 24)
+    res = f.range_batched(edge)
+    _check_digests(res, *O.brute_range(db, edge, "digest"))
+    assert f.range_pairs(edge).candidates == res.candidates
+
+
 @pytest.mark.parametrize("cell,leaf", [(1 / 1024, 8), (1 / 16, 4096)])
 def test_tree_shapes(gpu, cell, leaf):
     """A deep tree (tiny leaves on a 1024^2 grid) and a shallow one whose
@@ -113,3 +127,7 @@ def test_empty_and_outside(gpu):
     out = np.array([[1.5, 1.5, 2.0, 2.0], [-1, -1, -0.5, -0.5]], np.float32)
     res = g.range_batched(out)
     assert (res.counts() == 0).all() and res.candidates == 0
+    f = api.FsgIndex(W.UNIT_BOX, 0.01).build(db)
+    assert f.range_batched(np.zeros((0, 4), np.float32)).nq == 0
+    res = f.range_batched(out)  # clamped to the border cells: candidates, no hits
+    assert (res.counts() == 0).all()

@@ -290,3 +290,50 @@ def test_stig_device_tree_selects_host_leaves():
         t = tasks[tasks[:, 2] == q]
         assert _runs(sel) == _runs([(int(s), int(c)) for s, c, _ in t])
         assert top <= (depth + 1) * 2 * 64
+
+
+def test_record_layouts_match_headers(tmp_path):
+    """The numpy records the bindings pass through the C ABI have the field
+    offsets and sizes the headers give their structs (compiled and printed)."""
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "gat_host.h"
+#define F(T, m) printf("%s.%s %zu\n", #T, #m, offsetof(T, m))
+int main(void) {
+  F(gath_request, kind); F(gath_request, box); F(gath_request, qlen); F(gath_request, qoff);
+  F(gath_request, arrival_ms); printf("gath_request %zu\n", sizeof(gath_request));
+  F(gath_outcome, admit_ms); F(gath_outcome, done_ms); F(gath_outcome, batch); F(gath_outcome, count);
+  F(gath_outcome, digest); printf("gath_outcome %zu\n", sizeof(gath_outcome));
+  F(gat_tree_node, box); F(gat_tree_node, first_child); F(gat_tree_node, start); F(gat_tree_node, count);
+  F(gat_tree_node, flags); printf("gat_tree_node %zu\n", sizeof(gat_tree_node));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                               text=True).stdout.split("\n") if line)
+    for name, dt in (("gath_request", api.REQUEST), ("gath_outcome", api.OUTCOME), ("gat_tree_node", api.TREE_NODE)):
+        assert int(got[name]) == dt.itemsize, name
+        for field, (_, off) in dt.fields.items():
+            assert int(got[f"{name}.{field}"]) == off, (name, field)
+
+
+def test_serve_request_stream_shape():
+    db = W.CFG1.db(300)
+    req, qlon, qlat = W.serve_requests(W.CFG5, db, 0.5, rate=2000)
+    assert req.dtype == api.REQUEST and len(req) == 1000
+    assert (np.diff(req["arrival_ms"]) >= 0).all()
+    frac = (req["kind"] == api.REQ_RANGE).mean()
+    assert 0.7 < frac < 0.9
+    t = req[req["kind"] == api.REQ_TOPK]
+    assert ((t["qlen"] >= 16) & (t["qlen"] <= 512)).all()
+    assert (t["qoff"] + t["qlen"] <= len(qlon)).all() and len(qlon) == len(qlat)
+    r = req[req["kind"] == api.REQ_RANGE]["box"]
+    side = r[:, 2] - r[:, 0]
+    assert (side > 0).all() and (side <= 5e-2 * 1.0001).all()
+    again, _, _ = W.serve_requests(W.CFG5, db, 0.5, rate=2000)
+    assert (again == req).all()
