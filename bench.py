@@ -86,7 +86,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "copy"], default="all",
+    ap.add_argument("--serve-ntraj", type=int, default=None,
+                    help="trajectories of the serving index (default: the full 22.7M, about 2B points)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of the served request stream")
+    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "serve", "copy"], default="all",
                     help="range / batched: just the timed GT-quadtree batches of the plain / batched form (for "
                          "profiling); edr: just the EDR section; copy: just the stream-copy calibration kernel")
     args = ap.parse_args()
@@ -109,6 +112,9 @@ def main() -> int:
     if args.only == "edr":
         print(json.dumps(bench_edr(G, W)))
         return 0
+    if args.only == "serve":
+        print(json.dumps(bench_serve(G, W, args.serve_ntraj, args.serve_seconds)))
+        return 0
     w = W.CFG2
     t = time.perf_counter()
     db = w.db()
@@ -130,6 +136,8 @@ def main() -> int:
         print(json.dumps({"value": round(qps, 1), "pipeline_ms": round(res.kernel_ms, 4)}))
         return 0
 
+    tuned_range = grid.tune_batched(rects)
+    log(f"autotuned rectangles per work item: {tuned_range}")
     pres, pqps, ptotal, pwall = time_range(grid, rects, args.steps, args.warmup, "GT-quadtree plain")
     bres, bqps, btotal, bwall = time_batched(grid, rects, args.steps, args.warmup, "GT-quadtree batched")
     # Per-kernel events come from step-by-step runs (a graph replay times the whole pipeline).
@@ -162,6 +170,8 @@ def main() -> int:
     del fsg, stig
 
     edr = bench_edr(G, W)
+    del grid
+    serve = bench_serve(G, W, args.serve_ntraj, args.serve_seconds)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -230,6 +240,7 @@ def main() -> int:
             "plain_refine_kernel_ms": round(pres.kernel_ms, 4),
             "plain_refine_frac_hbm": round(plain_achieved / HBM_PEAK_GBS, 4),
             "batched_selection_ms": round(r.select_ms, 4),
+            "autotuned_queries_per_item": tuned_range["queries_per_item"],
             "batched_pipeline_ms": round(bres.kernel_ms, 4),
             "stig_batched_qps": round(sbqps, 1),
             "stig_plain_qps": round(sqps, 1),
@@ -241,11 +252,55 @@ def main() -> int:
             "range_plain_p99_ms": round(1e3 * float(np.percentile(pwall, 99)), 3),
             "stream_copy_gbs": round(copy_gbs, 1),
             **edr,
+            **serve,
         },
         "device": dev,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_serve(G, W, ntraj, seconds) -> dict:
+    """The serving loop on the resident 2B-point index: Poisson arrivals at
+    2000 requests/s (80% range squares, side log-uniform in [1e-4, 5e-2]; 20%
+    EDR top-20, query lengths U{16..512}) for `seconds`, batched as they come;
+    then the plain per-request path on the first requests of the same stream
+    (it cannot keep up: one top-k query over every trajectory takes seconds)."""
+    w = W.CFG5
+    t = time.perf_counter()
+    db = w.db(ntraj)
+    g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    log(f"serving index {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points resident "
+        f"({time.perf_counter() - t:.1f} s)")
+    req, qlon, qlat = W.serve_requests(w, db, seconds)
+    out, ids, dists, st = G.serve(g, req, qlon, qlat, w.k, w.eps)
+    lat = out["done_ms"] - req["arrival_ms"]
+    rng = req["kind"] == 0
+    res = {"serve_points": int(db.npoints), "serve_requests": int(len(req)),
+           "serve_rate_offered": w.extra["rate"],
+           "serve_throughput_rps": round(len(req) / (max(out["done_ms"].max(), req["arrival_ms"].max()) / 1e3), 1),
+           "serve_range_p50_ms": round(float(np.percentile(lat[rng], 50)), 3),
+           "serve_range_p99_ms": round(float(np.percentile(lat[rng], 99)), 3),
+           "serve_topk_p50_ms": round(float(np.percentile(lat[~rng], 50)), 3),
+           "serve_topk_p99_ms": round(float(np.percentile(lat[~rng], 99)), 3),
+           "serve_range_batches": int(st["range_batches"]), "serve_topk_batches": int(st["topk_batches"])}
+    log(f"served {len(req):,} requests in {st['wall_ms']:.0f} ms: range p50 {res['serve_range_p50_ms']} ms p99 "
+        f"{res['serve_range_p99_ms']} ms; top-{w.k} p50 {res['serve_topk_p50_ms']} ms p99 "
+        f"{res['serve_topk_p99_ms']} ms; {res['serve_range_batches']} range / {res['serve_topk_batches']} top-k "
+        f"batches")
+    sub = req[:40].copy()
+    p_out, _, _, _ = G.serve(g, sub, qlon, qlat, w.k, w.eps, batched=False)
+    p_lat = p_out["done_ms"] - sub["arrival_ms"]
+    p_rng = sub["kind"] == 0
+    res.update({"serve_plain_requests": int(len(sub)),
+                "serve_plain_range_p50_ms": round(float(np.percentile(p_lat[p_rng], 50)), 3),
+                "serve_plain_topk_p50_ms": round(float(np.percentile(p_lat[~p_rng], 50)), 3) if (~p_rng).any()
+                else None,
+                "serve_plain_p99_ms": round(float(np.percentile(p_lat, 99)), 3),
+                "serve_plain_same_answers": bool((p_out["digest"] == out["digest"][:len(sub)]).all())})
+    log(f"plain per-request path on the first {len(sub)} requests: range p50 {res['serve_plain_range_p50_ms']} ms, "
+        f"top-k p50 {res['serve_plain_topk_p50_ms']} ms, p99 {res['serve_plain_p99_ms']} ms")
+    return res
 
 
 def bench_edr(G, W) -> dict:
@@ -258,6 +313,9 @@ def bench_edr(G, W) -> dict:
     db = w.db()
     g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
     log(f"EDR workload {w.name}: {db.ntraj:,} candidates, {db.npoints:,} points ({time.perf_counter() - t:.1f} s)")
+    tune_q = G.synth_queries(w.seed + 8, np.array([256, 900], np.int32), db, copy_mode=2)
+    tuned = g.tune_edr(tune_q, w.eps)
+    log(f"autotuned batched EDR: {tuned}")
     qlen = np.array([128, 512, 1024, 64], np.int32)
     q = G.synth_queries(w.seed + 7, qlen, db, copy_mode=2)
     cells = int(db.lengths.astype(np.int64).sum() * int(qlen.sum()))
@@ -290,7 +348,8 @@ def bench_edr(G, W) -> dict:
             "edr_batched_over_plain": round(batched_rate / plain_rate, 3),
             "edr_batched_equals_plain": same,
             "topk_batched_qps": round(queries.ntraj / topk_s, 2),
-            "topk_pairs_evaluated": int(st["pairs"])}
+            "topk_pairs_evaluated": int(st["pairs"]),
+            "edr_autotuned": tuned}
 
 
 if __name__ == "__main__":

@@ -80,6 +80,7 @@ HIP_FUNCS = {
     "gat_scan_u64": (INT, [P, I64, P, P, P, P]),
     "gat_scan_leaf_lists": (INT, [P, I64, I32, P, P, P, P]),
     "gat_tree_select": (INT, [P, I64, I32, P, P, I64, I64, P, P, P, I64, P, P, P]),
+    "gat_leaf_tasks": (INT, [P, P, I64, P, I64, P, P]),
     "gat_grid_select": (INT, [P, I32, I32, F32, F32, F32, P, P, I64, I64, P, P, P, I64, P, P]),
     "gat_leaf_scatter": (INT, [P, P, I64, P, I64, P, P, P, I32, P, P, I64, P, I64, P, I64, P]),
     "gat_leaf_refine": (INT, [P, P, P, I64, P, I64, I64, P, P, I64, P, P]),
@@ -113,11 +114,13 @@ HOST_FUNCS = {
     "gath_index_range_pairs": (INT, [P, P, INT, P, P]),
     "gath_index_range_batched": (INT, [P, P, INT, INT, P, P]),
     "gath_index_tree_nodes": (I64, [P, P, I64, P]),
-    "gath_index_set_batch_params": (INT, [P, INT]),
+    "gath_index_set_batch_params": (INT, [P, INT, INT]),
     "gath_index_last_unique_candidates": (INT, [P, P]),
     "gath_index_last_counts": (INT, [P, P]),
     "gath_serve": (INT, [P, P, INT, P, P, P, P, P, P, P]),
     "gath_grid_set_edr_params": (INT, [P, INT, INT, INT]),
+    "gath_tune_range": (INT, [P, P, INT, INT, P]),
+    "gath_tune_edr": (INT, [P, P, INT, P, P, F64, INT, P]),
     "gath_index_last_fetch": (INT, [P, P, I64]),
     "gath_index_last_digests": (INT, [P, P]),
     "gath_grid_edr_all": (INT, [P, P, P, INT, F64, P]),

@@ -276,10 +276,24 @@ class _Index:
         N.check_host(N.host().gath_index_last_unique_candidates(self.handle, _ptr(out)))
         return int(out[0])
 
-    def set_batch_params(self, queries_per_item: int) -> "_Index":
-        """Rectangles per leaf-major work item of the batched form (1..64)."""
-        N.check_host(N.host().gath_index_set_batch_params(self.handle, int(queries_per_item)))
+    def set_batch_params(self, queries_per_item: int = 32, direct_max_queries: int = 256) -> "_Index":
+        """Batched form: rectangles per leaf-major work item (1..64), and the
+        largest batch that takes the direct form (per-task refine, nothing
+        sized by the tree touched)."""
+        N.check_host(N.host().gath_index_set_batch_params(self.handle, int(queries_per_item),
+                                                          int(direct_max_queries)))
         return self
+
+    def tune_batched(self, boxes: np.ndarray, reps: int = 3) -> dict:
+        """Autotunes the batched form on `boxes` (device pipeline time, median
+        of reps): rectangles per work item, then leaf-major against direct at
+        this batch size; keeps the fastest."""
+        b = _req(boxes, np.float32).reshape(-1, 4)
+        out = np.zeros(8, np.float64)
+        N.check_host(N.host().gath_tune_range(self.handle, _ptr(b), len(b), int(reps), _ptr(out)))
+        return {"queries_per_item": int(out[0]),
+                "ms": dict(zip([4, 8, 16, 32, 64], [round(float(x), 4) for x in out[1:6]])),
+                "direct_ms": round(float(out[6]), 4), "direct_max_queries": int(out[7])}
 
     def tree_nodes(self) -> tuple[np.ndarray, int]:
         """The tree the batched form uploads: a structured array of gat_tree_node
@@ -306,6 +320,16 @@ class GridIndex(_Index):
         N.check_host(N.host().gath_grid_set_edr_params(self.handle, int(split), int(per_cu_short),
                                                        int(per_cu_long)))
         return self
+
+    def tune_edr(self, queries: TrajSet, eps: float, reps: int = 2) -> dict:
+        """Autotunes batched EDR (bucket split, workgroups per CU of each
+        bucket) on every pair of `queries` x the index's trajectories."""
+        out = np.zeros(5, np.float64)
+        N.check_host(N.host().gath_tune_edr(self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj,
+                                            _ptr(_req(queries.lon, np.float32)), _ptr(_req(queries.lat, np.float32)),
+                                            eps, int(reps), _ptr(out)))
+        return {"split": int(out[0]), "per_cu_short": int(out[1]), "per_cu_long": int(out[2]),
+                "kernel_ms": round(float(out[3]), 4), "evaluations": int(out[4])}
 
     def edr_all(self, q_xy: np.ndarray, eps: float) -> np.ndarray:
         q = _req(q_xy, np.float32).reshape(-1, 2)

@@ -104,7 +104,7 @@ struct LeafEmitter {
     const uint64_t em = __ballot(emit);
     if (emit) {
       buf[n + lanes_below(em)] = idx;
-      atomicAdd(node_qcount + idx, 1u);
+      if (node_qcount) atomicAdd(node_qcount + idx, 1u);
       cand += cnt;
     }
     n += uint32_t(__popcll(em));
@@ -132,8 +132,18 @@ struct LeafEmitter {
   }
 };
 
+// The stack holds the first-child index of inner nodes already selected.
+// Each step pops up to 64 of them (one per lane); a lane loads its node's
+// nchild children (contiguous in breadth-first order), tests each, records the
+// selected leaves and pushes the first-child index of each selected inner
+// child, lane by lane in order.  A step therefore costs one dependent memory
+// access and handles up to 64 * nchild nodes.  Depth never decreases from the
+// bottom of the stack to its top, so each depth keeps at most nchild * 64
+// entries and a tree of depth D needs (D + 1) * nchild * 64 slots; the step
+// size is also cut so a push can never pass the end.
+template <int NCH>
 __global__ __launch_bounds__(kWave) void tree_select_kernel(
-    const Node* __restrict__ nodes, uint32_t nnodes, uint32_t nchild, const float4* __restrict__ boxes,
+    const Node* __restrict__ nodes, uint32_t nnodes, const float4* __restrict__ boxes,
     const uint32_t* __restrict__ order, uint32_t q0, uint32_t nq, uint32_t* __restrict__ node_qcount,
     uint64_t* __restrict__ cand, uint2* __restrict__ lq, uint64_t lq_cap, unsigned long long* __restrict__ lq_total,
     uint32_t* __restrict__ status) {
@@ -145,12 +155,21 @@ __global__ __launch_bounds__(kWave) void tree_select_kernel(
     if (q - q0 >= nq) continue;  // an order entry outside the batch (uniform)
     const float4 box = boxes[q];
     LeafEmitter out{emitted, 0, q, lane, lq, lq_cap, lq_total, node_qcount, 0};
-    uint32_t top = 1;
-    if (lane == 0) stack[0] = 0;
+    uint32_t top = 0;
+    {
+      const Node root = nodes[0];
+      const bool sel = node_selected(root, box);
+      if (root.first_child < 0) {
+        out.push(sel && root.count > 0 && lane == 0, 0u, root.count);
+      } else if (sel && uint32_t(root.first_child) + NCH <= nnodes) {
+        if (lane == 0) stack[0] = uint32_t(root.first_child);
+        top = 1;
+      }
+    }
     __syncthreads();
     while (top > 0) {
       uint32_t m = top < kWave ? top : kWave;
-      const uint32_t room = (GAT_TREE_STACK - top) / (nchild - 1);
+      const uint32_t room = (GAT_TREE_STACK - top) / (NCH - 1);
       if (m > room) m = room;
       if (m == 0) {
         if (lane == 0) atomicOr(status, GAT_STATUS_STACK);
@@ -158,31 +177,43 @@ __global__ __launch_bounds__(kWave) void tree_select_kernel(
       }
       top -= m;
       const bool live = lane < m;
-      const uint32_t idx = live ? stack[top + lane] : 0u;
+      const uint32_t fc = live ? stack[top + lane] : 0u;
       __syncthreads();  // every lane has read its entry before pushes reuse the slots
-      bool emit = false, desc = false;
-      uint32_t cnt = 0, first = 0;
-      if (live && idx < nnodes) {
-        const Node n = nodes[idx];
-        if (node_selected(n, box)) {
-          if (n.first_child < 0) {
-            emit = n.count > 0;
-            cnt = n.count;
-          } else if (uint32_t(n.first_child) + nchild <= nnodes) {
-            desc = true;
-            first = uint32_t(n.first_child);
+      uint32_t grand[NCH];
+      bool inner[NCH];
+      uint32_t below = 0, pushed = 0;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const uint32_t idx = fc + uint32_t(c);
+        bool emit = false;
+        uint32_t cnt = 0;
+        inner[c] = false;
+        grand[c] = 0;
+        if (live && idx < nnodes) {
+          const Node n = nodes[idx];
+          if (node_selected(n, box)) {
+            if (n.first_child < 0) {
+              emit = n.count > 0;
+              cnt = n.count;
+            } else if (uint32_t(n.first_child) + NCH <= nnodes) {
+              inner[c] = true;
+              grand[c] = uint32_t(n.first_child);
+            }
           }
         }
+        out.push(emit, idx, cnt);
+        out.maybe_flush();
+        const uint64_t dm = __ballot(inner[c]);
+        below += lanes_below(dm);
+        pushed += uint32_t(__popcll(dm));
       }
-      out.push(emit, idx, cnt);
-      const uint64_t dm = __ballot(desc);
-      if (desc) {
-        const uint32_t at = top + lanes_below(dm) * nchild;
-        for (uint32_t c = 0; c < nchild; ++c) stack[at + c] = first + c;
-      }
-      top += uint32_t(__popcll(dm)) * nchild;
+      // Lane-major order: a lane's selected inner children follow every lower lane's.
+      uint32_t at = top + below;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        if (inner[c]) stack[at++] = grand[c];
+      top += pushed;
       __syncthreads();
-      out.maybe_flush();
     }
     out.flush();
     const uint64_t total = out.wave_cand();
@@ -419,6 +450,22 @@ __global__ void leaf_scatter_kernel(const uint2* __restrict__ lq, const unsigned
   }
 }
 
+// Direct form for small batches: each selected (leaf, query) entry becomes a
+// refine task of the per-task kernel (range.hip), so nothing sized by the
+// tree is touched.
+__global__ void leaf_tasks_kernel(const uint2* __restrict__ lq, const unsigned long long* __restrict__ lq_total,
+                                  uint64_t lq_cap, const Node* __restrict__ nodes, uint32_t nnodes,
+                                  gat_range_task* __restrict__ tasks) {
+  const uint64_t total = *lq_total;
+  const uint64_t n = total < lq_cap ? total : lq_cap;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint2 e = lq[i];
+    // An entry naming no leaf becomes an empty task.
+    const bool ok = e.x < nnodes;
+    tasks[i] = gat_range_task{ok ? nodes[e.x].start : 0u, ok ? nodes[e.x].count : 0u, e.y};
+  }
+}
+
 // ---------------------------------------------------------------- refine
 // One wave per work item, chunk by chunk (up to 256 leaf points): lane j holds
 // list entry j of the item (rectangle, output region, id); pass 1 counts each
@@ -645,16 +692,24 @@ int gat_tree_select(const gat_tree_node* nodes, int64_t nnodes, int32_t nchild, 
                     const uint32_t* order, int64_t q0, int64_t nq, uint32_t* node_qcount, uint64_t* cand,
                     uint32_t* leaf_query, int64_t lq_capacity, uint64_t* lq_total, uint32_t* status, void* stream) {
   GAT_REQUIRE(nnodes >= 1 && nnodes < int64_t(INT32_MAX) && nodes, "gat_tree_select: bad tree");
-  GAT_REQUIRE(nchild >= 2 && nchild <= 8, "gat_tree_select: nchild must be 2..8");
+  GAT_REQUIRE(nchild == 2 || nchild == 4, "gat_tree_select: nchild must be 2 or 4");
   GAT_REQUIRE(q0 >= 0 && nq >= 0 && q0 + nq <= int64_t(UINT32_MAX), "gat_tree_select: bad query range");
-  GAT_REQUIRE(lq_capacity >= 0 && lq_total && status && node_qcount, "gat_tree_select: bad outputs");
+  GAT_REQUIRE(lq_capacity >= 0 && lq_total && status, "gat_tree_select: bad outputs");
   if (nq == 0) return GAT_OK;
   GAT_REQUIRE(boxes && cand && (lq_capacity == 0 || leaf_query), "gat_tree_select: null pointer");
   const int blocks = grid_for(uint64_t(nq), 1, 40);
-  tree_select_kernel<<<dim3{uint32_t(blocks)}, dim3{kWave}, 0, gat::as_stream(stream)>>>(
-      reinterpret_cast<const Node*>(nodes), uint32_t(nnodes), uint32_t(nchild), reinterpret_cast<const float4*>(boxes),
-      order, uint32_t(q0), uint32_t(nq), node_qcount, cand, reinterpret_cast<uint2*>(leaf_query), uint64_t(lq_capacity),
-      reinterpret_cast<unsigned long long*>(lq_total), status);
+  const dim3 grid{uint32_t(blocks)}, block{kWave};
+  hipStream_t s = gat::as_stream(stream);
+  const auto* N = reinterpret_cast<const Node*>(nodes);
+  const auto* B = reinterpret_cast<const float4*>(boxes);
+  auto* L = reinterpret_cast<uint2*>(leaf_query);
+  auto* T = reinterpret_cast<unsigned long long*>(lq_total);
+  if (nchild == 4)
+    tree_select_kernel<4><<<grid, block, 0, s>>>(N, uint32_t(nnodes), B, order, uint32_t(q0), uint32_t(nq),
+                                                 node_qcount, cand, L, uint64_t(lq_capacity), T, status);
+  else
+    tree_select_kernel<2><<<grid, block, 0, s>>>(N, uint32_t(nnodes), B, order, uint32_t(q0), uint32_t(nq),
+                                                 node_qcount, cand, L, uint64_t(lq_capacity), T, status);
   GAT_LAUNCH_CHECK("tree_select_kernel");
   return GAT_OK;
 }
@@ -665,7 +720,7 @@ int gat_grid_select(const gat_tree_node* cells, int32_t nx, int32_t ny, float xm
   GAT_REQUIRE(cells && nx > 0 && ny > 0 && int64_t(nx) * ny <= (int64_t(1) << 30) && cell_size > 0.0f,
               "gat_grid_select: bad grid");
   GAT_REQUIRE(q0 >= 0 && nq >= 0 && q0 + nq <= int64_t(UINT32_MAX), "gat_grid_select: bad query range");
-  GAT_REQUIRE(lq_capacity >= 0 && lq_total && node_qcount, "gat_grid_select: bad outputs");
+  GAT_REQUIRE(lq_capacity >= 0 && lq_total, "gat_grid_select: bad outputs");
   if (nq == 0) return GAT_OK;
   GAT_REQUIRE(boxes && cand && (lq_capacity == 0 || leaf_query), "gat_grid_select: null pointer");
   const int blocks = grid_for(uint64_t(nq), 1, 40);
@@ -674,6 +729,21 @@ int gat_grid_select(const gat_tree_node* cells, int32_t nx, int32_t ny, float xm
       order, uint32_t(q0), uint32_t(nq), node_qcount, cand, reinterpret_cast<uint2*>(leaf_query),
       uint64_t(lq_capacity), reinterpret_cast<unsigned long long*>(lq_total));
   GAT_LAUNCH_CHECK("grid_select_kernel");
+  return GAT_OK;
+}
+
+int gat_leaf_tasks(const uint32_t* leaf_query, const uint64_t* lq_total, int64_t lq_capacity,
+                   const gat_tree_node* nodes, int64_t nnodes, gat_range_task* tasks, void* stream) {
+  GAT_REQUIRE(lq_total && nodes && tasks && nnodes >= 1 && nnodes < int64_t(INT32_MAX) && lq_capacity >= 0,
+              "gat_leaf_tasks: bad arguments");
+  if (lq_capacity == 0) return GAT_OK;
+  GAT_REQUIRE(leaf_query, "gat_leaf_tasks: null entries");
+  leaf_tasks_kernel<<<dim3{uint32_t(grid_for(uint64_t(lq_capacity), 256, 8))}, dim3{256}, 0,
+                      gat::as_stream(stream)>>>(reinterpret_cast<const uint2*>(leaf_query),
+                                                reinterpret_cast<const unsigned long long*>(lq_total),
+                                                uint64_t(lq_capacity), reinterpret_cast<const Node*>(nodes),
+                                                uint32_t(nnodes), tasks);
+  GAT_LAUNCH_CHECK("leaf_tasks_kernel");
   return GAT_OK;
 }
 

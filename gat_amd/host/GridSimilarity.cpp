@@ -231,11 +231,18 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     maxlen[bucket] = std::max(maxlen[bucket], std::min(m, GAT_MAX_TRAJ_LEN));
     key[i] = (bucket * kS + (kS - 1 - strips_of(n))) * kM + (kM - 1 - m);
   }
-  std::vector<int64_t> cnt(size_t(2 * kS) * kM + 1, 0);
-  for (int64_t i = 0; i < np; ++i) ++cnt[key[i] + 1];
-  for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
   std::vector<int64_t> perm(static_cast<size_t>(np));
-  for (int64_t i = 0; i < np; ++i) perm[cnt[key[i]]++] = i;
+  std::iota(perm.begin(), perm.end(), int64_t(0));
+  if (np < (int64_t(1) << 16)) {
+    // Few pairs (a serving batch): a comparison sort beats clearing and
+    // scanning the 2 x 129 x 2049-entry count table.
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  } else {
+    std::vector<int64_t> cnt(size_t(2 * kS) * kM + 1, 0);
+    for (int64_t i = 0; i < np; ++i) ++cnt[key[i] + 1];
+    for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
+    for (int64_t i = 0; i < np; ++i) perm[cnt[key[i]]++] = i;
+  }
   std::vector<int32_t> sorted(size_t(np) * 2);
   std::vector<gat_edr_job> jobs;
   jobs.reserve(size_t(np / 4 + 2));

@@ -327,7 +327,85 @@ int RangeEngine::reserveBatched(const DeviceTree& tree, int maxQueries, int64_t 
   if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
   if (int rc = qlist_.reserve(size_t(qlistCap_) * 32)) return rc;
   if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
+  if (int rc = directTasks_.reserve(size_t(lqCap_) * sizeof(gat_range_task))) return rc;
   return out.pairs.reserve(size_t(pairsCap_) * 8);
+}
+
+// Direct form: selection without per-node counts, then one refine task per
+// selected (leaf, rectangle).  Boxes are already on the device.
+int RangeEngine::runDirect(const DeviceTree& tree, int nq, RangeBatch& out) {
+  void* s = stream_.get();
+  const int64_t N = tree.nnodes;
+  uint32_t* ctr = counters_.as<uint32_t>();
+  const uint32_t* ch = countersHost_.as<uint32_t>();
+  auto u64 = [&](int w) { return uint64_t(ch[w]) | (uint64_t(ch[w + 1]) << 32); };
+  const auto* order =
+      reinterpret_cast<const uint32_t*>(devBoxes_.as<char>() + align_up(size_t(nq) * 16, 256));
+  for (int attempt = 0;; ++attempt) {
+    if (int rc = selTimer_.start(s)) return rc;
+    GATH_TRY(gat_fill_u32(out.qcount.as<uint32_t>(), 0, nq, s), "batch reset");
+    GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
+    if (tree.flat()) {
+      GATH_TRY(gat_grid_select(tree.nodes.as<gat_tree_node>(), tree.nx, tree.ny, tree.xmin, tree.ymax,
+                               tree.cellSize, devBoxes_.as<float>(), order, 0, nq, nullptr, cand_.as<uint64_t>(),
+                               lq_.as<uint32_t>(), lqCap_, reinterpret_cast<uint64_t*>(ctr), s),
+               "grid select");
+    } else {
+      GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), N, tree.nchild, devBoxes_.as<float>(), order, 0, nq,
+                               nullptr, cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_,
+                               reinterpret_cast<uint64_t*>(ctr), ctr + 4, s),
+               "tree select");
+    }
+    GATH_TRY(gat_scan_u64(cand_.as<uint64_t>(), nq, out.qbase.as<uint64_t>(), reinterpret_cast<uint64_t*>(ctr + 2),
+                          tiles_.as<uint64_t>(), s),
+             "region scan");
+    if (int rc = selTimer_.stop(s)) return rc;
+    GATH_TRY(gat_memcpy_d2h(countersHost_.get(), counters_.get(), 64, s), "counter readback");
+    if (int rc = stream_.sync()) return rc;
+    if (ch[4] & GAT_STATUS_STACK) {
+      gath_set_error("batched range: traversal stack exceeded");
+      return GAT_ERR_STATE;
+    }
+    if (u64(0) <= uint64_t(lqCap_)) break;
+    if (attempt > 0 || u64(0) >= uint64_t(UINT32_MAX)) {
+      gath_set_error("batched range: %llu (leaf, query) entries do not fit", (unsigned long long)u64(0));
+      return GAT_ERR_CAPACITY;
+    }
+    lqCap_ = int64_t(u64(0) + u64(0) / 4 + 1024);
+    if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+  }
+  float selMs = 0;
+  if (int rc = selTimer_.elapsed(&selMs)) return rc;
+  const int64_t ntasks = int64_t(u64(0));
+  const int64_t cand = int64_t(u64(2));
+  if (pairsCap_ < cand || pairsCap_ == 0) pairsCap_ = cand + cand / 4 + 1024;
+  if (int rc = out.pairs.reserve(size_t(pairsCap_) * 8)) return rc;
+  if (int rc = directTasks_.reserve(size_t(std::max<int64_t>(lqCap_, 1)) * sizeof(gat_range_task))) return rc;
+  if (int rc = timer_.start(s)) return rc;
+  GATH_TRY(gat_leaf_tasks(lq_.as<uint32_t>(), reinterpret_cast<const uint64_t*>(ctr), lqCap_,
+                          tree.nodes.as<gat_tree_node>(), N, directTasks_.as<gat_range_task>(), s),
+           "leaf tasks");
+  GATH_TRY(gat_range_refine_pairs(&store_, directTasks_.as<gat_range_task>(), ntasks, devBoxes_.as<float>(), nq,
+                                  out.qbase.as<uint64_t>(), out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(),
+                                  pairsCap_, s),
+           "range refine");
+  if (int rc = timer_.stop(s)) return rc;
+  out.count.resize(size_t(nq));
+  out.base.resize(size_t(nq));
+  GATH_TRY(gat_memcpy_d2h(out.count.data(), out.qcount.get(), size_t(nq) * 4, s), "count readback");
+  GATH_TRY(gat_memcpy_d2h(out.base.data(), out.qbase.get(), size_t(nq) * 8, s), "region readback");
+  if (int rc = stream_.sync()) return rc;
+  if (int rc = timer_.elapsed(&out.kernelMs)) return rc;
+  out.selectMs = selMs;
+  out.refineMs = out.kernelMs;
+  out.hostStale = false;
+  out.leafQueries = ntasks;
+  out.taskNum = ntasks;
+  out.candidates = cand;
+  out.workItems = 0;
+  out.pairTotal = 0;
+  for (int q = 0; q < nq; ++q) out.pairTotal += out.count[size_t(q)];
+  return GAT_OK;
 }
 
 void RangeEngine::dropGraph() {
@@ -465,6 +543,22 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   const double t0 = now_ms();
   if (int rc = uploadBoxes(bounds, nq)) return rc;
   const int64_t N = tree.nnodes;
+  if (nq <= directMaxQueries) {
+    // Direct form: workspace sized by the batch only.
+    if (int rc = tiles_.reserve(size_t(gat_scan_tiles(nq)) * 8)) return rc;
+    if (int rc = cand_.reserve(size_t(nq) * 8)) return rc;
+    if (int rc = out.qbase.reserve(size_t(nq) * 8)) return rc;
+    if (int rc = out.qcount.reserve(size_t(nq) * 4)) return rc;
+    if (int rc = counters_.reserve(256)) return rc;
+    if (int rc = countersHost_.reserve(256)) return rc;
+    if (lqCap_ < int64_t(nq) * 16) {
+      lqCap_ = std::max<int64_t>(int64_t(nq) * 16, 1 << 16);
+      if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+    }
+    const int rc = runDirect(tree, nq, out);
+    out.uploadMs = now_ms() - t0;
+    return rc;
+  }
   const size_t tileBytes = size_t(std::max(2 * gat_scan_tiles(N) * 4, gat_scan_tiles(nq) * 8));
   if (nodeCount_.size() < size_t(N) * 4 || tiles_.size() < tileBytes || cand_.size() < size_t(nq) * 8 ||
       out.qbase.size() < size_t(nq) * 8 || out.qcount.size() < size_t(nq) * 4)

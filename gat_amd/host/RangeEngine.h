@@ -121,6 +121,10 @@ class RangeEngine {
   // fits the captured capacities.
   int runTree(const DeviceTree& tree, const MBB* bounds, int nq, RangeBatch& out, bool graph);
   int queriesPerItem = 32;  // rectangles per leaf-major work item (1..64)
+  // Batches of at most this many rectangles take the direct form: every
+  // selected (leaf, rectangle) becomes a task of the per-task refine kernel,
+  // and nothing sized by the tree is cleared or scanned.
+  int directMaxQueries = 256;
   // After a batched run: the points of the leaves at least one rectangle
   // selected (what the leaf-major refine must read at least once).
   int uniqueCandidates(const DeviceTree& tree, int64_t* points);
@@ -144,6 +148,7 @@ class RangeEngine {
   int enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out);
   int enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out);
   int readCounters(int nq, RangeBatch& out);
+  int runDirect(const DeviceTree& tree, int nq, RangeBatch& out);
   void dropGraph();
 
   DeviceBuffer xy_, id_, pid_;
@@ -157,7 +162,7 @@ class RangeEngine {
   // Batched-form workspace.  counters_ (uint32 words): [0,1] selected entries,
   // [2,3] candidate total, [4] status, [5] list total, [6] work items,
   // [8,9] result pairs.
-  DeviceBuffer nodeCount_, nodeFill_, nodeOff_, tiles_, cand_, lq_, qlist_, items_, counters_;
+  DeviceBuffer nodeCount_, nodeFill_, nodeOff_, tiles_, cand_, lq_, qlist_, items_, counters_, directTasks_;
   PinnedBuffer countersHost_;
   int64_t lqCap_ = 0, qlistCap_ = 0, itemsCap_ = 0, pairsCap_ = 0;
   EventTimer selTimer_, refineTimer_;

@@ -6,6 +6,7 @@
 
 #include "FSG.h"
 #include "Grid.h"
+#include "Autotune.h"
 #include "STIG.h"
 #include "Server.h"
 #include "Synth.h"
@@ -232,11 +233,13 @@ int gath_index_last_unique_candidates(void* p, int64_t* points) {
                   : ix->fsg ? ix->fsg->uniqueCandidates(points) : ix->stig->uniqueCandidates(points);
 }
 
-int gath_index_set_batch_params(void* p, int queries_per_item) {
+int gath_index_set_batch_params(void* p, int queries_per_item, int direct_max_queries) {
   Index* ix = as_index(p);
-  if (!ix || queries_per_item < 1 || queries_per_item > 64) return bad("gath_index_set_batch_params: bad arguments");
+  if (!ix || queries_per_item < 1 || queries_per_item > 64 || direct_max_queries < 0)
+    return bad("gath_index_set_batch_params: bad arguments");
   RangeEngine& e = ix->grid ? ix->grid->engine() : ix->fsg ? ix->fsg->engine() : ix->stig->engine();
   e.queriesPerItem = queries_per_item;
+  e.directMaxQueries = direct_max_queries;
   return GAT_OK;
 }
 
@@ -380,6 +383,37 @@ int gath_grid_set_edr_params(void* p, int split, int per_cu_short, int per_cu_lo
   ix->grid->edrSplit = split;
   ix->grid->edrPerCuShort = per_cu_short;
   ix->grid->edrPerCuLong = per_cu_long;
+  return GAT_OK;
+}
+
+int gath_tune_range(void* p, const float* boxes, int nq, int reps, double* out) {
+  Index* ix = as_index(p);
+  if (!ix || !out) return bad("gath_tune_range: bad arguments");
+  TuneRangeResult r;
+  int rc;
+  if (ix->grid) rc = tuneRangeBatched(*ix->grid, as_boxes(boxes), nq, reps, &r);
+  else if (ix->stig) rc = tuneRangeBatched(*ix->stig, as_boxes(boxes), nq, reps, &r);
+  else rc = tuneRangeBatched(*ix->fsg, as_boxes(boxes), nq, reps, &r);
+  if (rc) return rc;
+  out[0] = r.best;
+  for (size_t i = 0; i < r.ms.size(); ++i) out[1 + i] = r.ms[i];
+  out[6] = r.directMs;
+  out[7] = r.directMaxQueries;
+  return GAT_OK;
+}
+
+int gath_tune_edr(void* p, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, double eps,
+                  int reps, double* out) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || !q_offsets || !qlon || !qlat || nq < 1 || !out) return bad("gath_tune_edr: bad arguments");
+  ix->grid->epsilon = eps;
+  TuneEdrResult r;
+  if (int rc = tuneEdrBatched(*ix->grid, TrajectoryView{q_offsets, nq, qlon, qlat}, reps, &r)) return rc;
+  out[0] = r.split;
+  out[1] = r.perCuShort;
+  out[2] = r.perCuLong;
+  out[3] = r.ms;
+  out[4] = r.evaluations;
   return GAT_OK;
 }
 

@@ -254,7 +254,7 @@ typedef struct gat_tree_node {
 #define GAT_NODE_INCLUSIVE 1u
 #define GAT_NODE_UNTESTED 2u
 /* Entries of the per-wave traversal stack; a tree of depth D with nchild
- * children per node needs (D + 1) * nchild * 64 of them. */
+ * (2 or 4) children per inner node needs (D + 1) * nchild * 64 of them. */
 #define GAT_TREE_STACK 4096
 /* Bit set in *status by gat_tree_select when a traversal would exceed the stack. */
 #define GAT_STATUS_STACK 1u
@@ -281,9 +281,10 @@ int gat_scan_leaf_lists(const uint32_t* count_dev, int64_t nnodes, int32_t qb, u
  * rectangles order_dev[0 .. nq-1], taken in that order — largest first
  * shortens the tail), the leaves they visit (one wave per rectangle).  Appends each (leaf, query) to leaf_query (2
  * uint32 per entry, order unspecified) while *lq_total < lq_capacity, always
- * advancing *lq_total; adds 1 to node_qcount[leaf] per entry; writes cand[q] =
- * the points of q's selected leaves.  *lq_total and node_qcount must start at
- * zero; leaves with no points are skipped. */
+ * advancing *lq_total; adds 1 to node_qcount[leaf] per entry (unless
+ * node_qcount_dev is null); writes cand[q] = the points of q's selected leaves.
+ * *lq_total and node_qcount must start at zero; leaves with no points are
+ * skipped. */
 int gat_tree_select(const gat_tree_node* nodes_dev, int64_t nnodes, int32_t nchild, const float* boxes_dev,
                     const uint32_t* order_dev, int64_t q0, int64_t nq, uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev,
                     int64_t lq_capacity, uint64_t* lq_total_dev, uint32_t* status_dev, void* stream);
@@ -298,6 +299,12 @@ int gat_grid_select(const gat_tree_node* cells_dev, int32_t nx, int32_t ny, floa
                     const float* boxes_dev, const uint32_t* order_dev, int64_t q0, int64_t nq,
                     uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, int64_t lq_capacity,
                     uint64_t* lq_total_dev, void* stream);
+
+/* The direct form for small batches: entry i of the (leaf, query) list becomes
+ * tasks_dev[i] = {leaf start, leaf count, query} for gat_range_refine_pairs, so
+ * no per-node array is touched.  tasks_dev holds lq_capacity tasks. */
+int gat_leaf_tasks(const uint32_t* leaf_query_dev, const uint64_t* lq_total_dev, int64_t lq_capacity,
+                   const gat_tree_node* nodes_dev, int64_t nnodes, gat_range_task* tasks_dev, void* stream);
 
 /* Groups the selected (leaf, query) entries by leaf into self-contained
  * records.  List entry r of leaf L (32 bytes: the rectangle's 4 floats, its

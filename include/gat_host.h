@@ -82,8 +82,10 @@ int gath_index_last_unique_candidates(void* index, int64_t* points);
  * buckets (1..GAT_MAX_TRAJ_LEN) and workgroups per CU of each bucket's launch
  * (0..64, 0 = the kernel's default). */
 int gath_grid_set_edr_params(void* grid, int split, int per_cu_short, int per_cu_long);
-/* Tunables of the batched form: rectangles per leaf-major work item (1..64). */
-int gath_index_set_batch_params(void* index, int queries_per_item);
+/* Tunables of the batched form: rectangles per leaf-major work item (1..64),
+ * and the largest batch that takes the direct form (one refine task per
+ * selected leaf and rectangle, nothing sized by the tree touched). */
+int gath_index_set_batch_params(void* index, int queries_per_item, int direct_max_queries);
 /* The index's tree as the batched form uploads it (breadth-first
  * gat_tree_node); returns the node count, copies up to cap nodes. */
 int64_t gath_index_tree_nodes(void* index, gat_tree_node* out, int64_t cap, int32_t* depth);
@@ -106,6 +108,20 @@ int gath_grid_similarity_batched(void* grid, const int64_t* q_offsets, int nq, c
 /* The all-mismatch table, (GAT_MAX_TRAJ_LEN + 1)^2 int32. */
 int gath_grid_mismatch_table(void* grid, int32_t* out);
 
+
+/* ---------------------------------------------------------------- autotuner
+ * Times candidate launch shapes of the batched forms on the caller's sample and
+ * leaves the fastest set on the index (no reference counterpart).
+ * gath_tune_range: rectangles per leaf-major work item over {4, 8, 16, 32, 64},
+ * then the leaf-major against the direct form at this batch size; out[0] = the
+ * choice, out[1..5] = median device pipeline ms of each, out[6] = the direct
+ * form's ms, out[7] = the resulting direct-form threshold (rectangles).
+ * gath_tune_edr (Grid): bucket split and workgroups per CU of each bucket, by
+ * coordinate descent; out[0..4] = split, per-CU short, per-CU long, best kernel
+ * ms, evaluations. */
+int gath_tune_range(void* index, const float* boxes, int nq, int reps, double* out);
+int gath_tune_edr(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, double eps,
+                  int reps, double* out);
 
 /* ---------------------------------------------------------------- serving
  * The serving loop over a Grid (no reference counterpart): requests arrive at

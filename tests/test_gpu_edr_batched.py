@@ -119,3 +119,18 @@ def test_filter_boundary_is_exact(gpu):
     # The boxes' nearest points: the dropped pair does not match, the kept one does.
     assert O.edr(np.array([[0.0, 0.35]], np.float32), np.array([[u, 0.35]], np.float32), EPS) == 1
     assert O.edr(np.array([[0.0, 0.35]], np.float32), np.array([[below, 0.35]], np.float32), EPS) == 0
+
+
+def test_autotuned_edr_gives_the_same_distances(gpu):
+    db = _varied_db(77, [1, 17, 64, 300, 511, 513, 700, 1024, 1025, 2048] + list(range(20, 1200, 37)))
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    q = api.synth_queries(78, np.array([33, 600, 1500], np.int32), db, copy_mode=2)
+    want = _brute(db, q)
+    tuned = g.tune_edr(q, EPS, reps=1)
+    assert tuned["split"] in (256, 512, 1024, 2048) and tuned["evaluations"] == 17
+    dist, _ = g.edr_batched_all(q, EPS)
+    assert (dist == want).all()
+    for split in (256, 2048):
+        g.set_edr_params(split, 6, 24)
+        dist, _ = g.edr_batched_all(q, EPS)
+        assert (dist == want).all(), split

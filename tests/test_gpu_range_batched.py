@@ -15,6 +15,15 @@ from tests.test_gpu_range import _check_pairs, _edge_rects, _sets
 pytestmark = pytest.mark.gpu
 
 
+# Both forms of the batched pipeline: leaf-major join (direct_max_queries 0)
+# and direct per-task refine (every batch below the threshold).
+FORMS = {"leaf": 0, "direct": 1 << 30}
+
+
+def _form(ix, form):
+    return ix.set_batch_params(32, FORMS[form])
+
+
 @pytest.fixture(scope="module")
 def cfg1(gpu):
     db = W.CFG1.db()
@@ -26,13 +35,14 @@ def _check_digests(res, counts, dig):
     assert (res.digests() == dig).all()
 
 
-def test_cfg1_grid_batched(cfg1):
+@pytest.mark.parametrize("form", FORMS)
+def test_cfg1_grid_batched(cfg1, form):
     """Point-for-point on the 100 squares; per-query counts and digests on
     rectangles along cell edges, through data points and over everything
     (about 20M pairs, too many to compare as sets)."""
     db, rects = cfg1
     og = O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db)
-    g = api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db)
+    g = _form(api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db), form)
     res = g.range_batched(rects)
     _check_pairs(res, _sets(og.range(rects)), rects)
     edge = _edge_rects(db, W.CFG1.grid_cell, 60, 21)
@@ -40,12 +50,13 @@ def test_cfg1_grid_batched(cfg1):
     _check_digests(res, *og.range(edge, "digest"))
     plain = g.range_pairs(edge)
     assert plain.candidates == res.candidates  # same leaves selected
-    assert res.leaf_queries > 0 and res.work_items >= 1
+    assert res.leaf_queries > 0 and (res.work_items >= 1 or form == "direct")
 
 
-def test_cfg1_stig_batched(cfg1):
+@pytest.mark.parametrize("form", FORMS)
+def test_cfg1_stig_batched(cfg1, form):
     db, rects = cfg1
-    s = api.StigIndex(W.CFG1.leaf).build(db)
+    s = _form(api.StigIndex(W.CFG1.leaf).build(db), form)
     res = s.range_batched(rects)
     _check_pairs(res, _sets(O.brute_range(db, rects)), rects)
     edge = _edge_rects(db, W.CFG1.grid_cell, 40, 22)
@@ -54,12 +65,13 @@ def test_cfg1_stig_batched(cfg1):
     assert s.range_pairs(edge).candidates == res.candidates
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("cell", [0.01, 1 / 16, 1 / 1024])
-def test_fsg_batched(cfg1, cell):
+def test_fsg_batched(cfg1, cell, form):
     """The flat grid's cells as leaves: every cell of the clamped row/column
     range (FSG.cpp:183-312) against inclusive brute force."""
     db, rects = cfg1
-    f = api.FsgIndex(W.UNIT_BOX, cell).build(db)
+    f = _form(api.FsgIndex(W.UNIT_BOX, cell).build(db), form)
     res = f.range_batched(rects)
     _check_pairs(res, _sets(O.brute_range(db, rects)), rects)
     edge = _edge_rects(db, cell, 40, 24)
@@ -68,15 +80,16 @@ def test_fsg_batched(cfg1, cell):
     assert f.range_pairs(edge).candidates == res.candidates
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("cell,leaf", [(1 / 1024, 8), (1 / 16, 4096)])
-def test_tree_shapes(gpu, cell, leaf):
+def test_tree_shapes(gpu, cell, leaf, form):
     """A deep tree (tiny leaves on a 1024^2 grid) and a shallow one whose
     leaves exceed the 1024 points a work item loads at once."""
     db = W.CFG1.db(3000)
     rects = np.concatenate([W.CFG1.rects(), _edge_rects(db, cell, 30, 23),
                             api.synth_rects(23, 50, 1e-4, 0.5, True)]).astype(np.float32)
     og = O.Grid(W.UNIT_BOX, cell, leaf).add(db)
-    g = api.GridIndex(W.UNIT_BOX, cell, leaf).build(db)
+    g = _form(api.GridIndex(W.UNIT_BOX, cell, leaf).build(db), form)
     _check_digests(g.range_batched(rects), *og.range(rects, "digest"))
 
 
@@ -92,7 +105,7 @@ def test_hot_leaves_split_into_items(gpu):
     counts, dig = og.range(rects, "digest")
     g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
     for qb in (64, 32, 5, 1):
-        res = g.set_batch_params(qb).range_batched(rects)
+        res = g.set_batch_params(qb, 0).range_batched(rects)
         assert (res.counts() == counts).all() and (res.digests() == dig).all(), qb
         assert res.work_items >= -(-res.leaf_queries // qb)
     assert res.work_items == res.leaf_queries  # one rectangle per item
@@ -104,7 +117,7 @@ def test_graph_replay(gpu):
     Every call is checked against the oracle."""
     db = W.CFG1.db(5000)
     og = O.Grid(W.UNIT_BOX, 0.01, 256).add(db)
-    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db).set_batch_params(32, 0)
     small = api.synth_rects(31, 300, 1e-4, 0.05, True)
     other = api.synth_rects(32, 300, 1e-4, 0.05, True)
     large = api.synth_rects(33, 300, 0.05, 0.3, True)
@@ -119,15 +132,25 @@ def test_graph_replay(gpu):
     assert seen[4] is False  # outgrew the captured capacities
 
 
-def test_empty_and_outside(gpu):
+@pytest.mark.parametrize("form", FORMS)
+def test_empty_and_outside(gpu, form):
     db = W.CFG1.db(500)
-    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    g = _form(api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db), form)
     res = g.range_batched(np.zeros((0, 4), np.float32))
     assert res.nq == 0 and res.pairs_total == 0
     out = np.array([[1.5, 1.5, 2.0, 2.0], [-1, -1, -0.5, -0.5]], np.float32)
     res = g.range_batched(out)
     assert (res.counts() == 0).all() and res.candidates == 0
-    f = api.FsgIndex(W.UNIT_BOX, 0.01).build(db)
+    f = _form(api.FsgIndex(W.UNIT_BOX, 0.01).build(db), form)
     assert f.range_batched(np.zeros((0, 4), np.float32)).nq == 0
     res = f.range_batched(out)  # clamped to the border cells: candidates, no hits
     assert (res.counts() == 0).all()
+
+
+def test_autotuned_batches_give_the_same_answers(cfg1):
+    db, rects = cfg1
+    og = O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db)
+    g = api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db)
+    tuned = g.tune_batched(rects, reps=1)
+    assert tuned["queries_per_item"] in (4, 8, 16, 32, 64)
+    _check_digests(g.range_batched(rects), *og.range(rects, "digest"))

@@ -220,24 +220,37 @@ def _node_selected(n, box):
 
 
 def _wave_select(nodes, box, nchild, stack_cap=4096, wave=64):
-    """tree_select_kernel restated: pop up to 64 entries from the stack top,
-    test them in lane order, push the children of selected inner nodes.
-    Returns the selected leaf runs and the largest stack size reached."""
-    stack, top_max, sel = [0], 1, []
+    """tree_select_kernel restated: the stack holds first-child indices of
+    selected inner nodes; pop up to 64 from the top, test each popped node's
+    children in lane order, record selected leaves, push the first-child index
+    of each selected inner child (lane by lane).  Returns the selected leaf
+    runs and the largest stack size reached."""
+    sel = []
+    root = nodes[0]
+    stack = []
+    if root["first_child"] < 0:
+        if _node_selected(root, box) and root["count"] > 0:
+            sel.append((int(root["start"]), int(root["count"])))
+    elif _node_selected(root, box):
+        stack.append(int(root["first_child"]))
+    top_max = len(stack)
     while stack:
         m = min(len(stack), wave, (stack_cap - len(stack)) // (nchild - 1))
         assert m > 0
         batch = stack[len(stack) - m:]
         del stack[len(stack) - m:]
-        for i in batch:
-            n = nodes[i]
-            if not _node_selected(n, box):
-                continue
-            if n["first_child"] < 0:
-                if n["count"] > 0:
-                    sel.append((int(n["start"]), int(n["count"])))
-            else:
-                stack.extend(range(int(n["first_child"]), int(n["first_child"]) + nchild))
+        pushes = []
+        for fc in batch:
+            for c in range(nchild):
+                n = nodes[fc + c]
+                if not _node_selected(n, box):
+                    continue
+                if n["first_child"] < 0:
+                    if n["count"] > 0:
+                        sel.append((int(n["start"]), int(n["count"])))
+                else:
+                    pushes.append(int(n["first_child"]))
+        stack.extend(pushes)
         top_max = max(top_max, len(stack))
     return sel, top_max
 
