@@ -83,7 +83,7 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, uint64_t n
 // Selected leaves collect in an LDS buffer that is flushed to the global
 // (leaf, query) list with one atomic per flush, so a step costs one dependent
 // memory access (its node loads).
-constexpr uint32_t kEmitCap = 256;
+constexpr uint32_t kEmitCap = 1024;
 
 // Per-wave buffer of selected leaves in LDS, flushed to the global (leaf,
 // query) list with one atomic per flush.  Every lane calls every member.
