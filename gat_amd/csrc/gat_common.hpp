@@ -47,3 +47,18 @@ int cu_count();
     hipError_t gat_e_ = hipGetLastError();                                        \
     if (gat_e_ != hipSuccess) return gat::hip_fail(gat_e_, name, __FILE__, __LINE__); \
   } while (0)
+
+namespace gat {
+
+// Order-independent digest of a (tid, pid) pair: a 64-bit mix (the splitmix64
+// finaliser) of tid in the high and pid in the low half.  A query's digest is
+// the wrapping sum over its pairs.
+__device__ __forceinline__ uint64_t pair_mix(int2 p) {
+  uint64_t z = (uint64_t(uint32_t(p.x)) << 32) | uint32_t(p.y);
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace gat

@@ -196,17 +196,6 @@ int bitmap_handler(const gat_point_store* store, const gat_range_task* tasks, in
   return rc;
 }
 
-// Order-independent digest of a (tid, pid) pair: a 64-bit mix (the splitmix64
-// finaliser) of tid in the high and pid in the low half.  A query's digest is
-// the wrapping sum over its pairs.
-__device__ __forceinline__ uint64_t pair_mix(int2 p) {
-  uint64_t z = (uint64_t(uint32_t(p.x)) << 32) | uint32_t(p.y);
-  z += 0x9e3779b97f4a7c15ULL;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-
 __global__ __launch_bounds__(kBlock) void pairs_digest_kernel(const int2* __restrict__ pairs,
                                                              const uint64_t* __restrict__ qbase,
                                                              const uint32_t* __restrict__ qcount, uint32_t nq,
@@ -216,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void pairs_digest_kernel(const int2* __rest
     const uint64_t b = qbase[q];
     const uint64_t n = b < capacity ? (qcount[q] < capacity - b ? qcount[q] : capacity - b) : 0;
     uint64_t acc = 0;
-    for (uint64_t i = threadIdx.x; i < n; i += kBlock) acc += pair_mix(pairs[b + i]);
+    for (uint64_t i = threadIdx.x; i < n; i += kBlock) acc += gat::pair_mix(pairs[b + i]);
     for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = acc;
     __syncthreads();

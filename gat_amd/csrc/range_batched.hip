@@ -467,13 +467,17 @@ __global__ void leaf_tasks_kernel(const uint2* __restrict__ lq, const unsigned l
 }
 
 // ---------------------------------------------------------------- refine
-// One wave per work item, chunk by chunk (up to 256 leaf points): lane j holds
-// list entry j of the item (rectangle, output region, id); pass 1 counts each
-// rectangle's hits, one vector atomic reserves every rectangle's output slots
-// at once, pass 2 recomputes the same ballots and writes the pairs.  The loop
-// is software-pipelined: the next chunk's points and entries are loaded while
-// the current chunk's atomics are in flight, so a wave waits on one memory
-// round trip per chunk rather than two.
+// One wave per work item, chunk by chunk (up to 256 leaf points, NS = 1..4
+// slots of 64, the code specialised per NS so no per-slot branch runs): lane
+// j holds list entry j of the item (rectangle, output region, id).
+//   pass 1: for each rectangle, the hit masks of the chunk's slots straight
+//           from the compare instructions (four compares ANDed per slot);
+//           lane j keeps rectangle j's masks and hit count;
+//   one vector atomic reserves every rectangle's output slots at once;
+//   pass 2: for each rectangle with hits, its cached masks place every hit
+//           (rank = set bits below the lane) and the pair is written.
+// The work per (rectangle, slot) is a handful of VALU/SALU instructions; the
+// kernel is bound by that instruction stream as much as by memory.
 constexpr int kRefBlock = 256;
 constexpr int kSlots = 4;
 constexpr int kSpan = kWave * kSlots;
@@ -486,100 +490,80 @@ __device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t j) {
   return uint32_t(__builtin_amdgcn_readlane(int(v), int(j)));
 }
 
-// One chunk of a work item in registers.
-struct Chunk {
-  float2 v[kSlots];
-  uint32_t id[kSlots];
-  uint32_t pid[kSlots];
-  float4 bj;       // lane j: rectangle j (NaN when absent: no hits)
-  uint64_t basej;  // lane j: its output region
-  uint32_t qj;     // lane j: its id
-  uint32_t left;   // points in the chunk (uniform)
-  uint32_t nql;    // rectangles of the item (uniform)
-  uint32_t myhits, slot;
-};
+// Lanes whose point (x, y) lies in box, inclusive on every edge
+// (MBB::pInBox, MBB.cpp:21-26); NaN points are in no box.
+__device__ __forceinline__ uint64_t box_mask(const float4& b, float x, float y) {
+  constexpr int kOLE = 5, kOGE = 3;  // ordered <=, >=
+  return __builtin_amdgcn_fcmpf(x, b.z, kOLE) & __builtin_amdgcn_fcmpf(x, b.x, kOGE) &
+         __builtin_amdgcn_fcmpf(y, b.w, kOLE) & __builtin_amdgcn_fcmpf(y, b.y, kOGE);
+}
 
-template <bool kPacked>
-__device__ __forceinline__ void load_chunk(Chunk& c, const Store& store, const ListEntry* __restrict__ qlist,
-                                           uint32_t nqueries, uint32_t start, uint32_t left, uint32_t list,
-                                           uint32_t nql, uint32_t lane) {
+template <int NS, bool kPacked>
+__device__ __forceinline__ void refine_chunk(const Store& store, uint64_t at0, uint32_t left, uint32_t nql,
+                                             const float4& bj, uint32_t qj, uint64_t basej,
+                                             uint32_t* __restrict__ qcount, int2* __restrict__ pairs,
+                                             uint64_t capacity, uint32_t lane) {
   const float nan = __builtin_nanf("");
-  c.left = left;
-  c.nql = nql;
-  c.qj = 0xFFFFFFFFu;
-  c.bj = make_float4(nan, nan, nan, nan);
-  c.basej = 0;
-  if (lane < nql) {
-    const ListEntry e = qlist[uint64_t(list) + lane];
-    if (e.q < nqueries) {
-      c.qj = e.q;
-      c.bj = e.box;
-      c.basej = e.base;
-    }
-  }
+  float2 v[NS];
+  uint32_t id[NS];
+  uint32_t pid[NS];
 #pragma unroll
-  for (int k = 0; k < kSlots; ++k) {
+  for (int k = 0; k < NS; ++k) {
     const uint32_t i = k * kWave + lane;
-    c.v[k] = make_float2(nan, 0.0f);
-    c.id[k] = 0;
-    c.pid[k] = 0;
-    if (uint32_t(k * kWave) < left && i < left) {
-      const uint64_t at = uint64_t(start) + i;
-      c.v[k] = store.xy[at];
-      c.id[k] = store.id[at];
-      if constexpr (!kPacked) c.pid[k] = store.pid[at];
+    v[k] = make_float2(nan, 0.0f);
+    id[k] = 0;
+    pid[k] = 0;
+    if (i < left) {
+      v[k] = store.xy[at0 + i];
+      id[k] = store.id[at0 + i];
+      if constexpr (!kPacked) pid[k] = store.pid[at0 + i];
     }
   }
-}
-
-__device__ __forceinline__ float4 lane_box(const Chunk& c, uint32_t j) {
-  return make_float4(readlane_f(c.bj.x, j), readlane_f(c.bj.y, j), readlane_f(c.bj.z, j), readlane_f(c.bj.w, j));
-}
-
-// Pass 1 and the reservation: hits per rectangle into lane j, then one atomic
-// per rectangle with hits (its result is consumed by pass 2).
-__device__ __forceinline__ void count_and_reserve(Chunk& c, uint32_t* __restrict__ qcount, uint32_t lane) {
-  uint32_t myhits = 0;
-  for (uint32_t j = 0; j < c.nql; ++j) {
-    const float4 box = lane_box(c, j);
-    uint32_t h = 0;
+  // Pass 1: lane j keeps rectangle j's masks (as 32-bit halves) and hits.
+  uint32_t mlo[NS], mhi[NS];
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k)
-      if (uint32_t(k * kWave) < c.left) h += uint32_t(__popcll(__ballot(in_box(box, c.v[k].x, c.v[k].y))));
-    if (lane == j) myhits = h;
+  for (int k = 0; k < NS; ++k) mlo[k] = mhi[k] = 0;
+  uint32_t myhits = 0;
+  for (uint32_t j = 0; j < nql; ++j) {
+    const float4 box = make_float4(readlane_f(bj.x, j), readlane_f(bj.y, j), readlane_f(bj.z, j),
+                                   readlane_f(bj.w, j));
+    uint32_t h = 0;
+    const bool mine = lane == j;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const uint64_t m = box_mask(box, v[k].x, v[k].y);
+      h += uint32_t(__popcll(m));
+      mlo[k] = mine ? uint32_t(m) : mlo[k];
+      mhi[k] = mine ? uint32_t(m >> 32) : mhi[k];
+    }
+    myhits = mine ? h : myhits;
   }
-  c.myhits = myhits;
-  c.slot = 0;
-  if (myhits > 0) c.slot = atomicAdd(qcount + c.qj, myhits);
-}
-
-template <bool kPacked>
-__device__ __forceinline__ void write_pairs(const Chunk& c, const Store& store, int2* __restrict__ pairs,
-                                            uint64_t capacity, uint32_t lane) {
+  // One atomic per rectangle with hits, all issued together.
+  uint32_t slot = 0;
+  if (myhits > 0) slot = atomicAdd(qcount + qj, myhits);
+  const uint64_t firstj = basej + slot;
   const uint32_t pid_mask = (1u << store.pid_bits) - 1u;
-  const uint64_t firstj = c.basej + c.slot;
-  for (uint32_t j = 0; j < c.nql; ++j) {
-    if (readlane_u(c.myhits, j) == 0) continue;
-    const float4 box = lane_box(c, j);
+  // Pass 2: place and write every hit.
+  for (uint32_t j = 0; j < nql; ++j) {
+    if (readlane_u(myhits, j) == 0) continue;
     const uint64_t first =
         uint64_t(readlane_u(uint32_t(firstj), j)) | (uint64_t(readlane_u(uint32_t(firstj >> 32), j)) << 32);
     uint32_t before = 0;
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k) {
-      if (uint32_t(k * kWave) < c.left) {
-        const uint64_t mask = __ballot(in_box(box, c.v[k].x, c.v[k].y));
-        const uint64_t at = first + before + lanes_below(mask);
-        if (((mask >> lane) & 1) && at < capacity) {
-          int2 p;
-          if constexpr (kPacked) {
-            p = make_int2(int(c.id[k] >> store.pid_bits), int(c.id[k] & pid_mask));
-          } else {
-            p = make_int2(int(c.id[k]), int(c.pid[k]));
-          }
-          pairs[at] = p;
+    for (int k = 0; k < NS; ++k) {
+      const uint32_t lo = readlane_u(mlo[k], j), hi = readlane_u(mhi[k], j);
+      const uint64_t m = uint64_t(lo) | (uint64_t(hi) << 32);
+      const uint64_t at = first + before + lanes_below(m);
+      if (((m >> lane) & 1) && at < capacity) {
+        int2 p;
+        if constexpr (kPacked) {
+          p = make_int2(int(id[k] >> store.pid_bits), int(id[k] & pid_mask));
+        } else {
+          p = make_int2(int(id[k]), int(pid[k]));
         }
-        before += uint32_t(__popcll(mask));
+        pairs[at] = p;
       }
+      before += uint32_t(__popcll(m));
     }
   }
 }
@@ -594,50 +578,44 @@ __global__ __launch_bounds__(kRefBlock) void leaf_refine_kernel(
   const uint64_t nwaves = uint64_t(gridDim.x) * (kRefBlock / kWave);
   const uint64_t total = *item_total;
   const uint64_t nitems = total < items_cap ? total : items_cap;
-  if (wave >= nitems) return;
-  // Cursor: item t, chunk offset b; an item that fails validation has no points.
-  uint64_t t = wave;
-  uint32_t b = 0;
-  uint32_t start = 0, count = 0, list = 0, nql = 0;
-  auto take = [&](const Item& it) {
-    start = uniform(it.start);
-    count = uniform(it.count);
-    list = uniform(it.list);
-    nql = uniform(it.n);
-    if (uint64_t(start) + count > store.npoints || nql > kMaxQb || uint64_t(list) + nql > qlist_cap) {
-      count = 0;
-      nql = 0;
-    }
-  };
-  take(items[t]);
-  Item next = t + nwaves < nitems ? items[t + nwaves] : Item{0, 0, 0, 0};
-  Chunk cur;
-  {
-    const uint32_t left = count < uint32_t(kSpan) ? count : uint32_t(kSpan);
-    load_chunk<kPacked>(cur, store, qlist, nqueries, start, left, list, nql, lane);
-  }
-  count_and_reserve(cur, qcount, lane);
-  for (;;) {
-    b += kSpan;
-    if (b >= count) {
-      t += nwaves;
-      b = 0;
-      if (t < nitems) {
-        take(next);
-        if (t + nwaves < nitems) next = items[t + nwaves];
+  const float nan = __builtin_nanf("");
+  Item next = wave < nitems ? items[wave] : Item{0, 0, 0, 0};
+  for (uint64_t t = wave; t < nitems; t += nwaves) {
+    const uint32_t start = uniform(next.start), count = uniform(next.count);
+    const uint32_t list = uniform(next.list), nql = uniform(next.n);
+    if (t + nwaves < nitems) next = items[t + nwaves];
+    if (uint64_t(start) + count > store.npoints || nql == 0 || nql > kMaxQb || uint64_t(list) + nql > qlist_cap)
+      continue;
+    // Lane j: rectangle j of the item.  Invalid ones get a NaN box (no hits).
+    uint32_t qj = 0xFFFFFFFFu;
+    float4 bj = make_float4(nan, nan, nan, nan);
+    uint64_t basej = 0;
+    if (lane < nql) {
+      const ListEntry e = qlist[uint64_t(list) + lane];
+      if (e.q < nqueries) {
+        qj = e.q;
+        bj = e.box;
+        basej = e.base;
       }
     }
-    const bool more = t < nitems;
-    Chunk nxt;
-    if (more) {
-      const uint32_t rest = count - b;
-      const uint32_t left = rest < uint32_t(kSpan) ? rest : uint32_t(kSpan);
-      load_chunk<kPacked>(nxt, store, qlist, nqueries, start + b, left, list, nql, lane);
+    for (uint32_t base = 0; base < count; base += kSpan) {
+      const uint32_t left = count - base;
+      const uint64_t at0 = uint64_t(start) + base;
+      const uint32_t ns = left >= uint32_t(kSpan) ? uint32_t(kSlots) : (left + kWave - 1) / kWave;
+      switch (ns) {
+        case 1:
+          refine_chunk<1, kPacked>(store, at0, left, nql, bj, qj, basej, qcount, pairs, capacity, lane);
+          break;
+        case 2:
+          refine_chunk<2, kPacked>(store, at0, left, nql, bj, qj, basej, qcount, pairs, capacity, lane);
+          break;
+        case 3:
+          refine_chunk<3, kPacked>(store, at0, left, nql, bj, qj, basej, qcount, pairs, capacity, lane);
+          break;
+        default:
+          refine_chunk<4, kPacked>(store, at0, left, nql, bj, qj, basej, qcount, pairs, capacity, lane);
+      }
     }
-    write_pairs<kPacked>(cur, store, pairs, capacity, lane);
-    if (!more) break;
-    cur = nxt;
-    count_and_reserve(cur, qcount, lane);
   }
 }
 
