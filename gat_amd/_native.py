@@ -63,6 +63,7 @@ HIP_FUNCS = {
     "gat_event_destroy": (INT, [P]),
     "gat_event_record": (INT, [P, P]),
     "gat_event_sync": (INT, [P]),
+    "gat_stream_wait_event": (INT, [P, P]),
     "gat_event_elapsed_ms": (INT, [C.POINTER(F32), P, P]),
     "gat_eps_threshold": (F32, [F64]),
     "gat_stream_copy": (INT, [P, P, SZ, P]),
@@ -118,6 +119,9 @@ HOST_FUNCS = {
     "gath_index_last_unique_candidates": (INT, [P, P]),
     "gath_index_last_counts": (INT, [P, P]),
     "gath_serve": (INT, [P, P, INT, P, P, P, P, P, P, P]),
+    "gath_mixed_new": (P, [P, P, P]),
+    "gath_mixed_free": (None, [P]),
+    "gath_mixed_run": (INT, [P, P, INT, P, P, P, P, P]),
     "gath_grid_set_edr_params": (INT, [P, INT, INT, INT]),
     "gath_tune_range": (INT, [P, P, INT, INT, P]),
     "gath_tune_edr": (INT, [P, P, INT, P, P, F64, INT, P]),

@@ -412,6 +412,39 @@ def serve(grid: "GridIndex", requests: np.ndarray, qlon: np.ndarray, qlat: np.nd
     return out[:n], ids[:n], dists[:n], dict(zip(keys, st.tolist()))
 
 
+class MixedRange:
+    """One batch over the three indexes of a data set: each rectangle routed by
+    area (FSG below fsg_max_area, GT-quadtree below grid_max_area, STIG above),
+    in groups of group_size per index; the three device pipelines of a group run
+    as one captured graph.  Results: per-rectangle counts and digests."""
+
+    def __init__(self, grid: "GridIndex", stig: "StigIndex", fsg: "FsgIndex"):
+        self._keep = (grid, stig, fsg)
+        self.handle = N.host().gath_mixed_new(grid.handle, stig.handle, fsg.handle)
+        if not self.handle:
+            N.check_host(2)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            N.host().gath_mixed_free(h)
+            self.handle = None
+
+    def run(self, boxes: np.ndarray, fsg_max_area: float = 1e-6, grid_max_area: float = 1e-3,
+            group_size: int = 2048, graph: bool = True):
+        b = _req(boxes, np.float32).reshape(-1, 4)
+        n = len(b)
+        counts = np.zeros(max(n, 1), np.uint32)
+        digests = np.zeros(max(n, 1), np.uint64)
+        route = np.zeros(max(n, 1), np.int8)
+        st = np.zeros(6, np.float64)
+        cfg = np.array([fsg_max_area, grid_max_area, group_size, int(graph)], np.float64)
+        N.check_host(N.host().gath_mixed_run(self.handle, _ptr(b), n, _ptr(cfg), _ptr(counts), _ptr(digests),
+                                             _ptr(route), _ptr(st)))
+        keys = ["pairs", "candidates", "groups", "replays", "ms", "digest_ms"]
+        return counts[:n], digests[:n], route[:n], dict(zip(keys, st.tolist()))
+
+
 class FsgIndex(_Index):
     """FSG (FSG.cpp): flat row-major grid."""
 

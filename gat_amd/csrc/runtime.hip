@@ -178,6 +178,12 @@ int gat_event_record(void* event, void* stream) {
   return GAT_OK;
 }
 
+int gat_stream_wait_event(void* stream, void* event) {
+  GAT_REQUIRE(event, "gat_stream_wait_event: null event");
+  GAT_HIP_CHECK(hipStreamWaitEvent(gat::as_stream(stream), static_cast<hipEvent_t>(event), 0));
+  return GAT_OK;
+}
+
 int gat_event_sync(void* event) {
   GAT_REQUIRE(event, "gat_event_sync: null event");
   GAT_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(event)));

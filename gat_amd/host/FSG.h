@@ -47,6 +47,7 @@ class FSG {
   int hostThreads = 0;
   std::vector<int64_t> cellStart;  // cellnum + 1
   RangeEngine& engine() { return engine_; }
+  const DeviceTree& deviceTree() const { return tree_; }
 
  private:
   void addRows(const MBB& q, uint32_t qid, TaskList& list) const;

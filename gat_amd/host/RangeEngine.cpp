@@ -524,6 +524,55 @@ int RangeEngine::readCounters(int nq, RangeBatch& out) {
   return stream_.sync();
 }
 
+bool RangeEngine::countersFit() const {
+  const uint32_t* ch = countersHost_.as<uint32_t>();
+  auto u64 = [&](int w) { return uint64_t(ch[w]) | (uint64_t(ch[w + 1]) << 32); };
+  return u64(0) <= uint64_t(lqCap_) && int64_t(ch[5]) <= qlistCap_ && int64_t(ch[6]) <= itemsCap_ &&
+         u64(2) <= uint64_t(pairsCap_);
+}
+
+int RangeEngine::finishCounters(RangeBatch& out, bool replay) {
+  const uint32_t* ch = countersHost_.as<uint32_t>();
+  auto u64 = [&](int w) { return uint64_t(ch[w]) | (uint64_t(ch[w + 1]) << 32); };
+  if (ch[4] & GAT_STATUS_STACK) {
+    gath_set_error("batched range: traversal stack exceeded");
+    return GAT_ERR_STATE;
+  }
+  out.replayed = replay;
+  out.leafQueries = int64_t(u64(0));
+  out.candidates = int64_t(u64(2));
+  out.workItems = int64_t(ch[6]);
+  out.pairTotal = int64_t(u64(8));
+  return GAT_OK;
+}
+
+int RangeEngine::stageBatch(const MBB* bounds, int nq) {
+  const size_t need = align_up(size_t(nq) * 16, 256) + size_t(nq) * 4;
+  if (nq < 1 || !bounds || devBoxes_.size() < need) {
+    gath_set_error("stageBatch: the workspace was not sized for %d rectangles", nq);
+    return GAT_ERR_STATE;
+  }
+  return uploadBoxes(bounds, nq);
+}
+
+int RangeEngine::enqueueBatch(const DeviceTree& tree, int nq, RangeBatch& out) {
+  if (int rc = enqueueSelect(tree, nq, out)) return rc;
+  return enqueueJoin(tree, nq, out);
+}
+
+int RangeEngine::finishBatch(int nq, RangeBatch& out, bool* fits) {
+  out.queryNum = nq;
+  out.count.assign(size_t(nq), 0);
+  out.base.assign(size_t(nq), 0);
+  out.stream = stream_.get();
+  out.taskNum = 0;
+  out.selectMs = out.uploadMs = 0;
+  out.kernelMs = out.refineMs = 0;
+  if (int rc = readCounters(nq, out)) return rc;
+  *fits = countersFit();
+  return *fits ? finishCounters(out, true) : GAT_OK;
+}
+
 int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, RangeBatch& out, bool graph) {
   if (int rc = stream_.create()) return rc;
   if (tree.nnodes <= 0 || nq < 0 || (nq > 0 && !bounds) || queriesPerItem < 1 || queriesPerItem > 64) {
@@ -574,22 +623,8 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   if (int rc = countersHost_.reserve(256)) return rc;
   const uint32_t* ch = countersHost_.as<uint32_t>();
   auto u64 = [&](int w) { return uint64_t(ch[w]) | (uint64_t(ch[w + 1]) << 32); };
-  auto fits = [&] {
-    return u64(0) <= uint64_t(lqCap_) && int64_t(ch[5]) <= qlistCap_ && int64_t(ch[6]) <= itemsCap_ &&
-           u64(2) <= uint64_t(pairsCap_);
-  };
-  auto finish = [&](bool replay) -> int {
-    if (ch[4] & GAT_STATUS_STACK) {
-      gath_set_error("batched range: traversal stack exceeded");
-      return GAT_ERR_STATE;
-    }
-    out.replayed = replay;
-    out.leafQueries = int64_t(u64(0));
-    out.candidates = int64_t(u64(2));
-    out.workItems = int64_t(ch[6]);
-    out.pairTotal = int64_t(u64(8));
-    return GAT_OK;
-  };
+  auto fits = [&] { return countersFit(); };
+  auto finish = [&](bool replay) -> int { return finishCounters(out, replay); };
   GraphKey key;
   auto make_key = [&] {
     key.ptr[0] = tree.nodes.get();

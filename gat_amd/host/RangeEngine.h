@@ -130,6 +130,16 @@ class RangeEngine {
   int uniqueCandidates(const DeviceTree& tree, int64_t* points);
   ~RangeEngine();
 
+  // Pieces of the batched form for a coordinator that captures several
+  // engines into one graph.  After a runTree of the same batch size has sized
+  // the workspace: stageBatch uploads the rectangles, enqueueBatch queues the
+  // whole device pipeline on this engine's stream (no host sync, capturable),
+  // finishBatch reads the counters back and reports in *fits whether the batch
+  // stayed inside the workspace (if not, rerun it with runTree).
+  int stageBatch(const MBB* bounds, int nq);
+  int enqueueBatch(const DeviceTree& tree, int nq, RangeBatch& out);
+  int finishBatch(int nq, RangeBatch& out, bool* fits);
+
   // Shares `src`'s resident store (which must outlive this engine): a second
   // engine over the same points with its own stream and batch workspace.
   int attachStore(const RangeEngine& src);
@@ -148,6 +158,8 @@ class RangeEngine {
   int enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out);
   int enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out);
   int readCounters(int nq, RangeBatch& out);
+  bool countersFit() const;
+  int finishCounters(RangeBatch& out, bool replay);
   int runDirect(const DeviceTree& tree, int nq, RangeBatch& out);
   void dropGraph();
 

@@ -67,6 +67,7 @@ class STIG {
   int maxTid = 0;
   int hostThreads = 0;
   RangeEngine& engine() { return engine_; }
+  const DeviceTree& deviceTree() const { return tree_; }
 
  private:
   struct P {

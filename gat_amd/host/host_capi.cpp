@@ -7,6 +7,7 @@
 #include "FSG.h"
 #include "Grid.h"
 #include "Autotune.h"
+#include "MixedRange.h"
 #include "STIG.h"
 #include "Server.h"
 #include "Synth.h"
@@ -414,6 +415,41 @@ int gath_tune_edr(void* p, const int64_t* q_offsets, int nq, const float* qlon, 
   out[2] = r.perCuLong;
   out[3] = r.ms;
   out[4] = r.evaluations;
+  return GAT_OK;
+}
+
+void* gath_mixed_new(void* grid, void* stig, void* fsg) {
+  Index* g = as_index(grid);
+  Index* s = as_index(stig);
+  Index* f = as_index(fsg);
+  if (!g || !g->grid || !s || !s->stig || !f || !f->fsg) {
+    gath_set_error("gath_mixed_new: needs a Grid, a STIG and an FSG index");
+    return nullptr;
+  }
+  return new (std::nothrow) MixedRange(*g->grid, *s->stig, *f->fsg);
+}
+
+void gath_mixed_free(void* m) { delete static_cast<MixedRange*>(m); }
+
+int gath_mixed_run(void* m, const float* boxes, int nq, const double* cfg, uint32_t* counts, uint64_t* digests,
+                   int8_t* route, double* stats) {
+  auto* mr = static_cast<MixedRange*>(m);
+  if (!mr || !cfg || nq < 0 || (nq > 0 && (!boxes || !counts || !digests || !route)))
+    return bad("gath_mixed_run: bad arguments");
+  MixedConfig c;
+  c.fsgMaxArea = cfg[0];
+  c.gridMaxArea = cfg[1];
+  c.groupSize = int(cfg[2]);
+  c.graph = cfg[3] != 0;
+  MixedResult r;
+  if (int rc = mr->run(as_boxes(boxes), nq, c, &r)) return rc;
+  std::copy(r.count.begin(), r.count.end(), counts);
+  std::copy(r.digest.begin(), r.digest.end(), digests);
+  std::copy(r.route.begin(), r.route.end(), route);
+  if (stats) {
+    const double v[6] = {double(r.pairs), double(r.candidates), double(r.groups), double(r.replays), r.ms, r.digestMs};
+    for (int i = 0; i < 6; ++i) stats[i] = v[i];
+  }
   return GAT_OK;
 }
 

@@ -70,6 +70,9 @@ int gat_event_create(void** event);
 int gat_event_destroy(void* event);
 int gat_event_record(void* event, void* stream);
 int gat_event_sync(void* event);
+/* Later work on `stream` waits for `event` (also inside a graph capture: the
+ * way a captured pipeline forks onto other streams and joins back). */
+int gat_stream_wait_event(void* stream, void* event);
 int gat_event_elapsed_ms(float* ms, void* start, void* stop);
 
 /* Streaming copy of `bytes` (a multiple of 16) with a plain grid-stride float4

@@ -123,6 +123,18 @@ int gath_tune_range(void* index, const float* boxes, int nq, int reps, double* o
 int gath_tune_edr(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, double eps,
                   int reps, double* out);
 
+/* ---------------------------------------------------------------- mixed
+ * One batch of rectangles over the three indexes of a data set: each goes to
+ * FSG (area < cfg[0]), the GT-quadtree (area < cfg[1]) or STIG, in groups of
+ * cfg[2] rectangles per index, the three device pipelines of a group captured
+ * as one graph and replayed (cfg[3] != 0).  Per rectangle: counts, digests
+ * (as gat_pairs_digest), route (0 FSG, 1 GT-quadtree, 2 STIG); stats[0..5] =
+ * pairs, candidates, groups, graph replays, wall ms, of which digest ms. */
+void* gath_mixed_new(void* grid, void* stig, void* fsg);
+void gath_mixed_free(void* mixed);
+int gath_mixed_run(void* mixed, const float* boxes, int nq, const double* cfg, uint32_t* counts, uint64_t* digests,
+                   int8_t* route, double* stats);
+
 /* ---------------------------------------------------------------- serving
  * The serving loop over a Grid (no reference counterpart): requests arrive at
  * their arrival times, range slots and the top-k slot batch every waiting
