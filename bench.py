@@ -89,7 +89,10 @@ def main() -> int:
     ap.add_argument("--serve-ntraj", type=int, default=None,
                     help="trajectories of the serving index (default: the full 22.7M, about 2B points)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of the served request stream")
-    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "serve", "copy"], default="all",
+    ap.add_argument("--mixed-ntraj", type=int, default=None,
+                    help="trajectories of the mixed workload (default: the full 10M, about 0.88B points)")
+    ap.add_argument("--mixed-nq", type=int, default=None, help="rectangles of the mixed workload (default 100k)")
+    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "serve", "mixed", "copy"], default="all",
                     help="range / batched: just the timed GT-quadtree batches of the plain / batched form (for "
                          "profiling); edr: just the EDR section; copy: just the stream-copy calibration kernel")
     args = ap.parse_args()
@@ -111,6 +114,9 @@ def main() -> int:
         return 0
     if args.only == "edr":
         print(json.dumps(bench_edr(G, W)))
+        return 0
+    if args.only == "mixed":
+        print(json.dumps(bench_mixed(G, W, args.mixed_ntraj, args.mixed_nq)))
         return 0
     if args.only == "serve":
         print(json.dumps(bench_serve(G, W, args.serve_ntraj, args.serve_seconds)))
@@ -258,6 +264,58 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_mixed(G, W, ntraj, nq) -> dict:
+    """The mixed batch: 100k squares (side log-uniform in [1e-4, 1e-1], 70% of
+    centres around 16 Gaussian hotspots) over 10M trajectories, each routed by
+    area to FSG, the GT-quadtree or STIG, run in groups whose three pipelines
+    are one captured graph; per-rectangle counts and digests, 1000 of them
+    checked against the oracle's restatement of the index they were routed to.
+    (Run with --only mixed: the full batch writes ~0.5 TB of pairs.)"""
+    w = W.CFG4
+    t = time.perf_counter()
+    db = w.db(ntraj)
+    rects = w.rects(nq)
+    g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    s = G.StigIndex(w.leaf).build(db)
+    f = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db)
+    log(f"mixed workload {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points, {len(rects):,} rectangles, "
+        f"three indexes resident ({time.perf_counter() - t:.1f} s)")
+    m = G.MixedRange(g, s, f)
+    m.run(rects[:6144], group_size=2048)  # warm-up: sizes the workspaces
+    counts, digests, route, st = m.run(rects, group_size=2048)
+    pipeline_s = (st["ms"] - st["digest_ms"]) / 1e3
+    res = {"mixed_points": int(db.npoints), "mixed_rectangles": int(len(rects)),
+           "mixed_pairs": int(st["pairs"]), "mixed_groups": int(st["groups"]), "mixed_replays": int(st["replays"]),
+           "mixed_routed": {k: int((route == i).sum()) for i, k in enumerate(("fsg", "gt_quadtree", "stig"))},
+           "mixed_qps": round(len(rects) / pipeline_s, 1), "mixed_pairs_per_s": float(f"{st['pairs'] / pipeline_s:.4e}"),
+           "mixed_wall_s": round(st["ms"] / 1e3, 3), "mixed_digest_s": round(st["digest_ms"] / 1e3, 3)}
+    log(f"mixed: {res['mixed_qps']:,.0f} rectangles/s ({st['pairs']:,.0f} pairs, {res['mixed_pairs_per_s']:.3e} "
+        f"pairs/s, {st['groups']:.0f} groups, {st['replays']:.0f} graph replays; digests {res['mixed_digest_s']} s)")
+    import oracle as O
+    rng = np.random.default_rng(4)
+    sample = np.sort(rng.choice(len(rects), size=min(1000, len(rects)), replace=False))
+    t = time.perf_counter()
+    ok = True
+    # The GT-quadtree and FSG restatements for their rectangles; STIG's answer
+    # is inclusive brute force (its corrected search is exact).
+    for r in (1, 0, 2):
+        idx = sample[route[sample] == r]
+        if len(idx) == 0:
+            continue
+        if r == 2:
+            c, d = O.brute_range(db, rects[idx], "digest", nthreads=16)
+        else:
+            ref = (O.Grid(W.UNIT_BOX, w.grid_cell, w.leaf) if r == 1 else O.FSG(W.UNIT_BOX, w.fsg_cell)).add(db)
+            c, d = ref.range(rects[idx], "digest", nthreads=16)
+            del ref
+        good = bool((counts[idx] == c).all() and (digests[idx] == d).all())
+        log(f"  route {r}: {len(idx)} sampled rectangles identical to the oracle: {good}")
+        ok = ok and good
+    res["mixed_sample_parity"] = ok
+    log(f"mixed parity on {len(sample)} sampled rectangles: {ok} ({time.perf_counter() - t:.1f} s)")
+    return res
 
 
 def bench_serve(G, W, ntraj, seconds) -> dict:
