@@ -3,11 +3,12 @@
 // no reference counterpart for the batching itself.
 //
 // Wave-systolic layout.  A wave64 runs four independent pairs, one per 16-lane
-// DPP row ("segment").  Inside a segment, lane l owns DP row r = 16*s + l + 1 of
-// strip s and keeps that row's point in registers; at step t it computes column
-// c = t - l + 1, so a strip sweeps its 16 rows across all columns in m + 15
-// steps.  The three neighbours of a cell come from registers:
-//   U = D[r-1][c]   the left lane's value of the previous step (DPP row_shr:1)
+// DPP row ("segment"), or one long pair over all 64 lanes.  Inside a segment of
+// S lanes, lane l owns DP row r = S*s + l + 1 of strip s and keeps that row's
+// point in registers; at step t it computes column c = t - l + 1, so a strip
+// sweeps its S rows across all columns in m + S - 1 steps.  The three
+// neighbours of a cell come from registers:
+//   U = D[r-1][c]   the left lane's value of the previous step (DPP shift)
 //   L = D[r][c-1]   the lane's own value of the previous step
 //   M = D[r-1][c-1] the left lane's value two steps back, i.e. the U the lane
 //                   received one step earlier
@@ -15,25 +16,28 @@
 //   D = (M == L) ? U : min(M, L, U)
 // (M strictly smallest -> M; L strictly smallest -> L; every tie for the
 // minimum, and M == L < U in particular, -> U).
-// The first lane of a segment takes U from the row above the strip, kept in
-// LDS as 16-bit values and fed 16 steps at a time through a register that
-// shifts one lane per step (row_shl:1); the strip's last lane hands its row to
-// the next strip the same way in reverse.  Column points are read straight from
-// the NaN-padded trajectory store (64 consecutive points per wave access, L1
-// resident across the 16 lanes that reuse them); a NaN point never matches, so
-// lanes before their first column keep D[r][0] = r untouched and lanes past
-// the last column compute values nobody reads.  The cell D[n][m] is captured
-// by the lane that owns row n at the step it reaches column m.
+// The first lane of a segment takes U + 1 from the row above the strip, kept
+// in LDS as 16-bit values of D + 1 and read once per step as a broadcast (the
+// value is the `old` operand of the shift, so lane 0 keeps it).  Each lane
+// stores its cell plus one into the same buffer at its own column; the strip's
+// last lane stores every column last, which leaves its row there for the next
+// strip, and each column is read (step c - 1) no later than its first store of
+// the strip (lane 0, same step, after the read).  Column points are read
+// straight from the NaN-padded trajectory store (64 consecutive points per
+// wave access, L1 resident across the lanes that reuse them); a NaN point
+// never matches, so lanes before their first column keep D[r][0] = r untouched
+// and lanes past the last column compute values nobody reads.  The cell
+// D[n][m] is captured by the lane that owns row n at the step it reaches
+// column m.
 //
-// Work: jobs of up to four pairs with the same strip count, ordered longest
-// first, handed out by one atomic counter per launch, so the batch's tail is
-// short and no host round trip separates length buckets.
+// Work: jobs of up to four pairs (one for 64-lane segments) with the same
+// strip count, ordered longest first, handed out by one atomic counter per
+// launch, so the batch's tail is short and no host round trip separates length
+// buckets.
 #include "gat_common.hpp"
 
 namespace {
 
-constexpr int kSeg = 16;   // lanes per segment (one DPP row)
-constexpr int kSegs = 4;   // pairs per wave
 constexpr int kBlk = 16;   // steps per feed block
 constexpr int kPad = 16;   // NaN points before every trajectory of a padded store
 
@@ -42,15 +46,33 @@ struct Job {
 };
 static_assert(sizeof(Job) == sizeof(gat_edr_job), "job layout");
 
-__device__ __forceinline__ int row_shr1(int old, int src) {
-  return __builtin_amdgcn_update_dpp(old, src, 0x111, 0xf, 0xf, false);
-}
-__device__ __forceinline__ int row_shl1(int old, int src) {
-  return __builtin_amdgcn_update_dpp(old, src, 0x101, 0xf, 0xf, false);
+// Stores of different lanes to one LDS column come from different steps'
+// store instructions, and the row left behind is the last lane's only if those
+// instructions keep program order.  Per lane their addresses differ, so the
+// scheduler may reorder them; this barrier lets every other kind of
+// instruction (ALU, global loads, LDS reads) move across it, but not LDS
+// writes.
+__device__ __forceinline__ void keep_store_order() {
+  // 0x1 ALU, 0x2 VALU, 0x4 SALU, 0x10 VMEM, 0x20 VMEM read, 0x100 DS read.
+  __builtin_amdgcn_sched_barrier(0x1 | 0x2 | 0x4 | 0x10 | 0x20 | 0x100);
 }
 
-template <int MAXM>
-__global__ __launch_bounds__(64) void edr_wave_kernel(const int2* __restrict__ pairs, int64_t npairs,
+// Lane l of a segment takes src from lane l - 1; lane 0 keeps old.  Segments
+// of 16 lanes are DPP rows (row_shr:1); a 64-lane segment is the wave
+// (wave_shr:1).
+template <int SEG>
+__device__ __forceinline__ int shift_in(int old, int src) {
+  static_assert(SEG == 16 || SEG == 64, "segments of 16 or 64 lanes");
+  return __builtin_amdgcn_update_dpp(old, src, SEG == 16 ? 0x111 : 0x138, 0xf, 0xf, false);
+}
+
+// SEG lanes per pair: 16 (four pairs per wave) or 64 (one pair per wave, a
+// quarter of the strips, for the longest pairs, whose strips would otherwise
+// run one after another for milliseconds).  Waves per SIMD each class can hold:
+// four 16-lane pairs' rows of 2048 columns (16.8 KB of LDS per wave) cap that
+// class at two; the others are held to 128 VGPRs so four fit.
+template <int MAXM, int SEG>
+__global__ __launch_bounds__(64, (MAXM >= 2048 && SEG == 16) ? 2 : 4) void edr_wave_kernel(const int2* __restrict__ pairs, int64_t npairs,
                                                       const Job* __restrict__ jobs, int32_t njobs,
                                                       int32_t* __restrict__ counter, int32_t nqueries, int32_t ncands,
                                                       const float2* __restrict__ qxy, const int64_t* __restrict__ qoff,
@@ -58,12 +80,20 @@ __global__ __launch_bounds__(64) void edr_wave_kernel(const int2* __restrict__ p
                                                       const int64_t* __restrict__ coff,
                                                       const int32_t* __restrict__ clen, float u,
                                                       int32_t* __restrict__ out) {
-  // Row above the current strip, per segment, at index c + kBlk (columns -15..).
-  __shared__ int16_t above[kSegs][MAXM + 3 * kBlk];
+  constexpr int kSeg = SEG;        // lanes per pair
+  constexpr int kSegs = 64 / SEG;  // pairs per wave
+  // Row above the current strip plus one, D[r0][c] + 1, at [c + kOff][segment]
+  // for columns 2 - SEG .. MAXM + SEG + kBlk - 1 (the lanes' look-behind, the
+  // columns, and the last blocks' overrun).  Segments interleave, so one
+  // segment's consecutive columns are 2 * kSegs bytes apart and each step's
+  // feed is a load of its own.
+  constexpr int kOff = SEG;
+  constexpr int kRows = MAXM + 2 * SEG + kBlk;
+  __shared__ int16_t above[kRows][kSegs];
   const int lane = threadIdx.x;
-  const int seg = lane >> 4;
+  const int seg = lane / kSeg;
   const int l = lane & (kSeg - 1);
-  const float nanf_ = __builtin_nanf("");
+  const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
   for (;;) {
     int ji = 0;
     if (lane == 0) ji = atomicAdd(counter, 1);
@@ -74,7 +104,8 @@ __global__ __launch_bounds__(64) void edr_wave_kernel(const int2* __restrict__ p
     const bool live = seg < job.count && job.first >= 0 && int64_t(job.first) + seg < npairs;
     int n = 0, m = 0;
     // Dead segments read rows nowhere (n = 0) and columns from the store's
-    // first trajectory, whose leading padding covers the lanes' look-behind.
+    // first trajectory, whose leading padding covers the lanes' look-behind
+    // (loads further back than the padding are replaced by NaN, below).
     const float2* R = cxy;
     const float2* C = cxy + kPad;
     bool valid = false;
@@ -113,67 +144,97 @@ __global__ __launch_bounds__(64) void edr_wave_kernel(const int2* __restrict__ p
     const int strips = (nmax + kSeg - 1) / kSeg;
     const int my_last = (n + kSeg - 1) / kSeg - 1;
     const int steps = mmax + kSeg - 1;
-    // Row 0: D[0][c] = c.  (Kept rolled: unrolled, its length grows with MAXM
+    // Row 0: D[0][c] + 1 = c + 1.  (Kept rolled: unrolled, its length grows with MAXM
     // and so does the register count.)
 #pragma unroll 2
-    for (int i = l; i < MAXM + 3 * kBlk; i += kSeg) above[seg][i] = int16_t(i - kBlk);
+    for (int i = l; i < kRows; i += kSeg) above[i][seg] = int16_t(i - kOff + 1);
     int res = 0;
     for (int s = 0; s < strips; ++s) {
       // Orders the previous strip's (or the initial) writes of `above` before
       // this strip's reads of them by other lanes.
       __syncthreads();
       const int r = kSeg * s + l + 1;
-      const float2 rp = r <= n ? R[r - 1] : make_float2(nanf_, nanf_);
+      const float2 rp = r <= n ? R[r - 1] : nan2;
       const float rx = rp.x, ry = rp.y;
-      int cur = r;  // D[r][0]
-      int u1p = r;  // D[r-1][0] + 1
-      int coll = 0;
+      int c1 = r + 1;  // D[r][c - 1] + 1, from D[r][0] = r
+      int u1p = r;     // D[r-1][c - 1] + 1, from D[r-1][0] + 1
       const int tcap = (s == my_last) ? m + l - 1 : -1;  // step at which lane l holds D[r][m]
       const bool last_strip_somewhere = s >= first_last_strip;
       const float2* cp = C - l;  // column point of step t is cp[t]
-      float2 col[kBlk];
+      // Column points of steps tb .. tb + 15.  Lanes look back up to SEG - 1
+      // points before the trajectory.  A lane whose block starts more than
+      // kPad points before it (64-lane segments, first blocks only) has all 16
+      // points before it: it takes NaN, never matching, and loads from the
+      // padding instead.
+      auto load_block = [&](float2(&dst)[kBlk], int tb) __attribute__((always_inline)) {
+        if (SEG <= kPad + 1 || tb - (kSeg - 1) >= -kPad) {
 #pragma unroll
-      for (int j = 0; j < kBlk; ++j) col[j] = cp[j];
-      for (int t0 = 0; t0 < steps; t0 += kBlk) {
-        // Feed: U + 1 for the segment's first lane at steps t0 .. t0 + 15.
-        int f1 = int(above[seg][kBlk + t0 + 1 + l]) + 1;
-        float2 nxt[kBlk];
+          for (int j = 0; j < kBlk; ++j) dst[j] = cp[tb + j];
+        } else {
+          const bool before = tb - l < -kPad;
+          const float2* bp = before ? C - kPad : cp + tb;
 #pragma unroll
-        for (int j = 0; j < kBlk; ++j) nxt[j] = cp[t0 + kBlk + j];
+          for (int j = 0; j < kBlk; ++j) {
+            const float2 v = bp[j];
+            dst[j] = before ? nan2 : v;
+          }
+        }
+      };
+      // One feed block of kBlk steps over the column points in `src` (steps
+      // t0 .. t0 + 15) while the next block's points load into `dst`.  Two
+      // register sets alternate, so no block copies one into the other.
+      auto sweep = [&](const float2(&src)[kBlk], float2(&dst)[kBlk], int t0) __attribute__((always_inline)) {
+        // Feed: U + 1 for the segment's first lane at step t0 + j (column
+        // t0 + j + 1), read for the whole block before its first store: this
+        // strip stores column c first at step c - 1, and no earlier block of
+        // the strip stores past column t0.
+        const int16_t(*feed)[kSegs] = &above[kOff + t0 + 1];
+        int fv[kBlk];
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) fv[j] = feed[j][seg];
+        // Every lane stores its cell plus one at its column, t0 + j - l + 1.
+        // The last lane stores a column last, so after the strip the buffer
+        // holds the strip's last row (columns up to the widest segment's last);
+        // lane 0 stores column c at the step that reads its feed, after the
+        // read.
+        int16_t(*own)[kSegs] = &above[kOff + t0 - l + 1];
+        load_block(dst, t0 + kBlk);
         const bool capture = last_strip_somewhere && t0 + kBlk - 1 >= mmin - 1;
         if (capture) {
 #pragma unroll
           for (int j = 0; j < kBlk; ++j) {
-            const bool match = fabsf(rx - col[j].x) < u && fabsf(ry - col[j].y) < u;
-            const int c1 = cur + 1;
-            const int u1 = row_shr1(f1, c1);
-            f1 = row_shl1(f1, f1);
+            const bool match = fabsf(rx - src[j].x) < u && fabsf(ry - src[j].y) < u;
+            const int u1 = shift_in<SEG>(fv[j], c1);
             const int mm = u1p - (match ? 1 : 0);
             const int mn = min(min(u1, c1), mm);
-            cur = (mm == c1) ? u1 : mn;
+            const int d = (mm == c1) ? u1 : mn;
+            res = (t0 + j == tcap) ? d : res;
+            c1 = d + 1;
             u1p = u1;
-            coll = row_shl1(cur, coll);
-            res = (t0 + j == tcap) ? cur : res;
+            own[j][seg] = int16_t(c1);
+            keep_store_order();
           }
         } else {
 #pragma unroll
           for (int j = 0; j < kBlk; ++j) {
-            const bool match = fabsf(rx - col[j].x) < u && fabsf(ry - col[j].y) < u;
-            const int c1 = cur + 1;
-            const int u1 = row_shr1(f1, c1);
-            f1 = row_shl1(f1, f1);
+            const bool match = fabsf(rx - src[j].x) < u && fabsf(ry - src[j].y) < u;
+            const int u1 = shift_in<SEG>(fv[j], c1);
             const int mm = u1p - (match ? 1 : 0);
             const int mn = min(min(u1, c1), mm);
-            cur = (mm == c1) ? u1 : mn;
+            const int d = (mm == c1) ? u1 : mn;
+            c1 = d + 1;
             u1p = u1;
-            coll = row_shl1(cur, coll);
+            own[j][seg] = int16_t(c1);
+            keep_store_order();
           }
         }
-        // Lane 15 produced columns t0 - 14 .. t0 + 1 of row 16*s + 16; they are
-        // the row above the next strip.
-        above[seg][kBlk + t0 - (kSeg - 2) + l] = int16_t(coll);
-#pragma unroll
-        for (int j = 0; j < kBlk; ++j) col[j] = nxt[j];
+      };
+      float2 colA[kBlk], colB[kBlk];
+      load_block(colA, 0);
+      for (int t0 = 0; t0 < steps; t0 += 2 * kBlk) {
+        sweep(colA, colB, t0);
+        if (t0 + kBlk >= steps) break;
+        sweep(colB, colA, t0 + kBlk);
       }
     }
     __syncthreads();  // the next job re-initialises `above`
@@ -278,10 +339,11 @@ extern "C" int gat_edr_mismatch_table(int32_t maxlen, int32_t* out, void* stream
 extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_edr_job* jobs, int64_t njobs,
                                int32_t* counter, const float* qxy, const int64_t* qoff, const int32_t* qlen,
                                int32_t nqueries, const float* cxy, const int64_t* coff, const int32_t* clen,
-                               int32_t ncands, int32_t maxlen, double eps, int32_t per_cu, int32_t* out,
-                               void* stream) {
+                               int32_t ncands, int32_t maxlen, double eps, int32_t per_cu, int32_t lanes_per_pair,
+                               int32_t* out, void* stream) {
   GAT_REQUIRE(npairs >= 0 && njobs >= 0 && njobs <= INT32_MAX && nqueries >= 0 && ncands >= 0,
               "EDR batched: bad sizes");
+  GAT_REQUIRE(lanes_per_pair == 16 || lanes_per_pair == 64, "EDR batched: lanes_per_pair must be 16 or 64");
   if (njobs == 0) return GAT_OK;
   GAT_REQUIRE(pairs && jobs && counter && qxy && qoff && qlen && cxy && coff && clen && out,
               "EDR batched: null pointer");
@@ -296,31 +358,33 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
   const auto* J = reinterpret_cast<const Job*>(jobs);
   const auto* Q = reinterpret_cast<const float2*>(qxy);
   const auto* Cs = reinterpret_cast<const float2*>(cxy);
-  // LDS class: the smallest row buffer that holds maxlen columns.  Persistent
-  // grid: per_cu single-wave workgroups per CU (default by class), capped by
-  // the work.
-  const int cls = maxlen <= 256 ? 256 : maxlen <= 512 ? 512 : maxlen <= 1024 ? 1024 : GAT_MAX_TRAJ_LEN;
-  const int64_t pc = per_cu > 0 ? per_cu : (cls <= 512 ? 20 : cls == 1024 ? 12 : 9);
+  // LDS class: the smallest row buffer that holds maxlen columns (64-lane
+  // pairs: 1024 or 2048).  Persistent grid: per_cu single-wave workgroups per
+  // CU (default by class), capped by the work.
+  const bool wide = lanes_per_pair == 64;
+  int cls = maxlen <= 256 ? 256 : maxlen <= 512 ? 512 : maxlen <= 1024 ? 1024 : GAT_MAX_TRAJ_LEN;
+  if (wide && cls < 1024) cls = 1024;
+  const int64_t pc = per_cu > 0 ? per_cu : wide ? 16 : (cls <= 512 ? 20 : cls == 1024 ? 16 : 9);
   int64_t blocks = int64_t(gat::cu_count()) * pc;
   if (blocks > njobs) blocks = njobs;
   const dim3 grid{uint32_t(blocks)}, block{64};
-  switch (cls) {
-    case 256:
-      edr_wave_kernel<256><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff,
-                                                  qlen, Cs, coff, clen, u, out);
-      break;
-    case 512:
-      edr_wave_kernel<512><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff,
-                                                  qlen, Cs, coff, clen, u, out);
-      break;
-    case 1024:
-      edr_wave_kernel<1024><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff,
-                                                   qlen, Cs, coff, clen, u, out);
-      break;
-    default:
-      edr_wave_kernel<GAT_MAX_TRAJ_LEN><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries,
-                                                               ncands, Q, qoff, qlen, Cs, coff, clen, u, out);
+#define GAT_EDR_LAUNCH(M, SEG)                                                                                  \
+  edr_wave_kernel<M, SEG><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff, \
+                                                 qlen, Cs, coff, clen, u, out)
+  if (wide) {
+    if (cls == 1024)
+      GAT_EDR_LAUNCH(1024, 64);
+    else
+      GAT_EDR_LAUNCH(GAT_MAX_TRAJ_LEN, 64);
+  } else {
+    switch (cls) {
+      case 256: GAT_EDR_LAUNCH(256, 16); break;
+      case 512: GAT_EDR_LAUNCH(512, 16); break;
+      case 1024: GAT_EDR_LAUNCH(1024, 16); break;
+      default: GAT_EDR_LAUNCH(GAT_MAX_TRAJ_LEN, 16);
+    }
   }
+#undef GAT_EDR_LAUNCH
   GAT_LAUNCH_CHECK("edr_wave_kernel");
   return GAT_OK;
 }

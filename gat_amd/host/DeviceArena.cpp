@@ -69,6 +69,27 @@ int Stream::sync() const {
   return GAT_OK;
 }
 
+SideStream::~SideStream() {
+  if (fork_) gat_event_destroy(fork_);
+  if (join_) gat_event_destroy(join_);
+  if (s_) gat_stream_destroy(s_);
+}
+
+int SideStream::fork(void* owner) {
+  if (!s_) GATH_TRY(gat_stream_create(&s_), "side stream creation");
+  if (!fork_) GATH_TRY(gat_event_create(&fork_), "event creation");
+  if (!join_) GATH_TRY(gat_event_create(&join_), "event creation");
+  GATH_TRY(gat_event_record(fork_, owner), "event record");
+  GATH_TRY(gat_stream_wait_event(s_, fork_), "side stream wait");
+  return GAT_OK;
+}
+
+int SideStream::join(void* owner) {
+  GATH_TRY(gat_event_record(join_, s_), "event record");
+  GATH_TRY(gat_stream_wait_event(owner, join_), "stream wait");
+  return GAT_OK;
+}
+
 EventTimer::~EventTimer() {
   if (a_) gat_event_destroy(a_);
   if (b_) gat_event_destroy(b_);

@@ -85,6 +85,25 @@ class Stream {
 };
 
 // Brackets device work on one stream with a pair of events.
+// A second stream for work that overlaps the owner's stream: fork() makes it
+// wait for what the owner's stream has queued so far, join() makes the owner's
+// stream wait for what was queued on it since.
+class SideStream {
+ public:
+  SideStream() = default;
+  ~SideStream();
+  SideStream(const SideStream&) = delete;
+  SideStream& operator=(const SideStream&) = delete;
+  int fork(void* owner);
+  int join(void* owner);
+  void* stream() const { return s_; }
+
+ private:
+  void* s_ = nullptr;
+  void* fork_ = nullptr;
+  void* join_ = nullptr;
+};
+
 class EventTimer {
  public:
   EventTimer() = default;

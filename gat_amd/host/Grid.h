@@ -103,8 +103,10 @@ class Grid {
   const std::vector<int32_t>& mismatchTable();
 
   // Batched-EDR tunables: pairs whose longer trajectory has at most edrSplit
-  // points form the short bucket; workgroups per CU of each bucket's launch
-  // (0: the kernel's default for its LDS class).
+  // points form the short bucket, those up to max(edrSplit, 1024) the long
+  // one, and longer pairs a third bucket at the kernel's default occupancy;
+  // workgroups per CU of the short and long launches (0: the kernel's default
+  // for its LDS class).
   int edrSplit = 512;
   int edrPerCuShort = 0, edrPerCuLong = 0;
 
@@ -147,6 +149,7 @@ class Grid {
   DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
   DeviceBuffer pairsDev_, jobsDev_, outDev_, counterDev_, ftableDev_;
   EventTimer edrTimer_;
+  SideStream edrSide_;  // the longest EDR bucket runs here, beside the others
   std::vector<int32_t> trajLenHost_, queryLenHost_;
   std::vector<int32_t> ftable_;
   std::vector<std::vector<int32_t>> lengthIds_;  // per length: trajectory ids, ascending

@@ -207,17 +207,21 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   const int64_t np = int64_t(pairs.size() / 2);
   dist.assign(size_t(np), -1);
   if (np == 0) return GAT_OK;
-  // Two length buckets: pairs whose longer trajectory is at most edrSplit
-  // points run in a launch with a small LDS row buffer (more waves per CU),
-  // the rest in one sized for the longest.  Inside a bucket, a counting sort
-  // by (strips, longer length), descending, and jobs of up to four pairs with
-  // the same strip count.
+  // Length buckets: pairs whose longer trajectory is at most edrSplit points
+  // run in a launch with a small LDS row buffer (more waves per CU), those up
+  // to kMid points in one sized for kMid, four pairs per wave, and the rest one
+  // pair per wave (64-lane strips, a quarter as many one after another).
+  // Inside a bucket, a counting sort by (strips, longer length), descending,
+  // and jobs of up to four pairs with the same strip count.
   constexpr int kM = GAT_MAX_TRAJ_LEN + 1;
   constexpr int kS = GAT_MAX_TRAJ_LEN / 16 + 1;
+  constexpr int kBuckets = 3;
+  constexpr int kMid = 1024;
   const int split = std::max(1, std::min(edrSplit, GAT_MAX_TRAJ_LEN));
+  const int mid = std::max(split, kMid);
   std::vector<int32_t> key(static_cast<size_t>(np));
   int64_t cells = 0;
-  int maxlen[2] = {1, 1};
+  int maxlen[kBuckets] = {1, 1, 1};
   for (int64_t i = 0; i < np; ++i) {
     const int q = pairs[2 * i], c = pairs[2 * i + 1];
     if (q < 0 || q >= queryNum || c < 0 || c >= trajNum) {
@@ -227,7 +231,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     const int a = queryLenHost_[q], b = trajLenHost_[c];
     const int n = std::min(a, b), m = std::max(a, b);
     cells += int64_t(n) * m;
-    const int bucket = m <= split ? 0 : 1;
+    const int bucket = m <= split ? 0 : m <= mid ? 1 : 2;
     maxlen[bucket] = std::max(maxlen[bucket], std::min(m, GAT_MAX_TRAJ_LEN));
     key[i] = (bucket * kS + (kS - 1 - strips_of(n))) * kM + (kM - 1 - m);
   }
@@ -238,7 +242,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     // scanning the 2 x 129 x 2049-entry count table.
     std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
   } else {
-    std::vector<int64_t> cnt(size_t(2 * kS) * kM + 1, 0);
+    std::vector<int64_t> cnt(size_t(kBuckets * kS) * kM + 1, 0);
     for (int64_t i = 0; i < np; ++i) ++cnt[key[i] + 1];
     for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
     for (int64_t i = 0; i < np; ++i) perm[cnt[key[i]]++] = i;
@@ -246,12 +250,14 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   std::vector<int32_t> sorted(size_t(np) * 2);
   std::vector<gat_edr_job> jobs;
   jobs.reserve(size_t(np / 4 + 2));
-  int64_t bucketJobs[2] = {0, 0};
+  int64_t bucketJobs[kBuckets] = {0, 0, 0};
   int64_t i = 0;
   while (i < np) {
     const int32_t k0 = key[perm[i]] / kM;  // bucket and strip count
+    // Jobs of the longest bucket run one pair over a whole wave.
+    const int32_t per_job = k0 / kS == 2 ? 1 : 4;
     int32_t c = 0;
-    while (c < 4 && i + c < np && key[perm[i + c]] / kM == k0) ++c;
+    while (c < per_job && i + c < np && key[perm[i + c]] / kM == k0) ++c;
     jobs.push_back(gat_edr_job{int32_t(i), c});
     ++bucketJobs[k0 / kS];
     i += c;
@@ -267,17 +273,31 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   if (int rc = counterDev_.reserve(16)) return rc;
   const double h1 = now_ms();
   if (int rc = edrTimer_.start(s)) return rc;
-  int64_t j0 = 0;
-  for (int bucket = 0; bucket < 2; ++bucket) {
-    if (bucketJobs[bucket] == 0) continue;
-    GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>() + j0, bucketJobs[bucket],
-                             counterDev_.as<int32_t>(), queryXY_.as<float>(), queryOff64_.as<int64_t>(),
-                             queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(), trajOff_.as<int64_t>(),
-                             trajLen_.as<int32_t>(), trajNum, maxlen[bucket], epsilon,
-                             bucket == 0 ? edrPerCuShort : edrPerCuLong, outDev_.as<int32_t>(), s),
+  // Each bucket has its own job counter.  The longest bucket holds few jobs
+  // whose strips run for milliseconds each: launched first on a side stream, it
+  // overlaps the other buckets instead of trailing them on an idle device.
+  int64_t j0[kBuckets] = {0, bucketJobs[0], bucketJobs[0] + bucketJobs[1]};
+  auto launch = [&](int bucket, void* stream) -> int {
+    GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>() + j0[bucket],
+                             bucketJobs[bucket], counterDev_.as<int32_t>() + bucket, queryXY_.as<float>(),
+                             queryOff64_.as<int64_t>(), queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(),
+                             trajOff_.as<int64_t>(), trajLen_.as<int32_t>(), trajNum, maxlen[bucket], epsilon,
+                             bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0, bucket == 2 ? 64 : 16,
+                             outDev_.as<int32_t>(), stream),
              "batched EDR");
-    j0 += bucketJobs[bucket];
+    return GAT_OK;
+  };
+  const bool side = bucketJobs[2] > 0 && bucketJobs[0] + bucketJobs[1] > 0;
+  if (side) {
+    if (int rc = edrSide_.fork(s)) return rc;
+    if (int rc = launch(2, edrSide_.stream())) return rc;
   }
+  for (int bucket = 0; bucket < kBuckets; ++bucket) {
+    if (bucketJobs[bucket] == 0 || (side && bucket == 2)) continue;
+    if (int rc = launch(bucket, s)) return rc;
+  }
+  if (side)
+    if (int rc = edrSide_.join(s)) return rc;
   if (int rc = edrTimer_.stop(s)) return rc;
   std::vector<int32_t> out(static_cast<size_t>(np));
   GATH_TRY(gat_memcpy_d2h(out.data(), outDev_.get(), size_t(np) * 4, s), "batched EDR readback");

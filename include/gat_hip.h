@@ -194,15 +194,18 @@ typedef struct gat_edr_job {
  * (row buffers of 256, 512, 1024 or 2048 columns); a pair with an id outside
  * [0, nqueries) / [0, ncands) or a length outside 1..maxlen gets -1, and jobs
  * reaching past npairs are skipped.  per_cu is the number of single-wave
- * workgroups launched per compute unit (0: the class default).  Callers split
- * pairs into length buckets and launch each bucket with its own maxlen, so
- * short pairs are not run at the occupancy of the longest.  Asynchronous on
- * `stream`. */
+ * workgroups launched per compute unit (0: the class default).
+ * lanes_per_pair is 16 (four pairs per wave, jobs of up to four pairs) or 64
+ * (one pair per wave and a quarter of the sequential strips, for the longest
+ * pairs; a job's entries past its first are not computed, so such jobs hold one
+ * pair).  Callers split pairs into length buckets and launch each bucket with
+ * its own maxlen, so short pairs are not run at the occupancy of the longest.
+ * Asynchronous on `stream`. */
 int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job* jobs_dev, int64_t njobs,
                     int32_t* counter_dev, const float* query_xy_dev, const int64_t* query_offset_dev,
                     const int32_t* query_length_dev, int32_t nqueries, const float* cand_xy_dev,
                     const int64_t* cand_offset_dev, const int32_t* cand_length_dev, int32_t ncands, int32_t maxlen,
-                    double eps, int32_t per_cu, int32_t* out_dev, void* stream);
+                    double eps, int32_t per_cu, int32_t lanes_per_pair, int32_t* out_dev, void* stream);
 
 /* Candidate filter for exact top-k pruning (no reference counterpart; the
  * reference orders candidates by an unproven frequency-distance bound,

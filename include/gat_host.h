@@ -80,7 +80,8 @@ int gath_index_range_batched(void* index, const float* boxes, int nq, int graph,
 int gath_index_last_unique_candidates(void* index, int64_t* points);
 /* Batched-EDR tunables of a Grid: the length split between the short and long
  * buckets (1..GAT_MAX_TRAJ_LEN) and workgroups per CU of each bucket's launch
- * (0..64, 0 = the kernel's default). */
+ * (0..64, 0 = the kernel's default).  Pairs longer than max(split, 1024)
+ * points run in a third launch at the kernel's default. */
 int gath_grid_set_edr_params(void* grid, int split, int per_cu_short, int per_cu_long);
 /* Tunables of the batched form: rectangles per leaf-major work item (1..64),
  * and the largest batch that takes the direct form (one refine task per
