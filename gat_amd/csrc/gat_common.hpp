@@ -50,6 +50,16 @@ int cu_count();
 
 namespace gat {
 
+// A result pair written with a streaming (nontemporal) store: pair outputs
+// are written once and not read again by the kernel that writes them.
+__device__ __forceinline__ void store_pair(int2* dst, int2 v) {
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  v2i w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<v2i*>(dst));
+}
+
 // Order-independent digest of a (tid, pid) pair: a 64-bit mix (the splitmix64
 // finaliser) of tid in the high and pid in the low half.  A query's digest is
 // the wrapping sum over its pairs.

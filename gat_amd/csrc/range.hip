@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void range_refine_kernel(
 #pragma unroll
         for (int k = 0; k < kSlots; ++k) {
           const uint64_t at = first + before + lanes_below(mask[k]);
-          if (((mask[k] >> lane) & 1) && at < capacity) pairs[at] = ids[k];
+          if (((mask[k] >> lane) & 1) && at < capacity) gat::store_pair(pairs + at, ids[k]);
           before += __popcll(mask[k]);
         }
       } else {

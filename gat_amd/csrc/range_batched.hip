@@ -561,7 +561,7 @@ __device__ __forceinline__ void refine_chunk(const Store& store, uint64_t at0, u
         } else {
           p = make_int2(int(id[k]), int(pid[k]));
         }
-        pairs[at] = p;
+        gat::store_pair(pairs + at, p);
       }
       before += uint32_t(__popcll(m));
     }
