@@ -92,13 +92,15 @@ def main() -> int:
     ap.add_argument("--mixed-ntraj", type=int, default=None,
                     help="trajectories of the mixed workload (default: the full 10M, about 0.88B points)")
     ap.add_argument("--mixed-nq", type=int, default=None, help="rectangles of the mixed workload (default 100k)")
-    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "serve", "mixed", "copy"], default="all",
+    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "serve", "mixed", "copy", "hbm"],
+                    default="all",
                     help="range / batched: just the timed GT-quadtree batches of the plain / batched form (for "
-                         "profiling); edr: just the EDR section; copy: just the stream-copy calibration kernel")
+                         "profiling); edr: just the EDR section; copy: just the stream-copy calibration kernel; "
+                         "hbm: write / read / packed-run access-pattern ceilings")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU split of the "
-              "reference is out of scope for this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); runs over several GPUs are "
+              "not supported", file=sys.stderr)
         return 2
 
     import gat_amd as G
@@ -111,6 +113,9 @@ def main() -> int:
         gbs, ms = G.stream_copy_bandwidth(1 << 32, max(args.steps, 1))
         log(f"stream copy: {gbs:.1f} GB/s ({ms:.3f} ms per 4 GiB copy)")
         print(json.dumps({"stream_copy_gbs": round(gbs, 1), "kernel_ms": round(ms, 4), "bytes_per_launch": 2 << 32}))
+        return 0
+    if args.only == "hbm":
+        print(json.dumps(bench_hbm(G)))
         return 0
     if args.only == "edr":
         print(json.dumps(bench_edr(G, W)))
@@ -359,6 +364,24 @@ def bench_serve(G, W, ntraj, seconds) -> dict:
     log(f"plain per-request path on the first {len(sub)} requests: range p50 {res['serve_plain_range_p50_ms']} ms, "
         f"top-k p50 {res['serve_plain_topk_p50_ms']} ms, p99 {res['serve_plain_p99_ms']} ms")
     return res
+
+
+def bench_hbm(G) -> dict:
+    """Ceilings of the HBM access patterns behind the range kernels' result
+    writes, over a 4 GiB buffer: streams, then runs of 8-byte pairs packed back
+    to back (as per-query result regions fill) written from each run's first
+    pair or through line-aligned windows, and 256-byte-aligned runs."""
+    out = {}
+    for kind, name in [(0, "write stream"), (1, "read stream")]:
+        gbs, ms = G.hbm_pattern_bandwidth(kind)
+        log(f"{name}: {gbs:.1f} GB/s ({ms:.3f} ms)")
+        out[name.replace(" ", "_") + "_gbs"] = round(gbs, 1)
+    for run in (28, 32, 110, 255, 256, 257):
+        for kind, name in [(2, "packed"), (3, "line-aligned windows")]:
+            gbs, ms = G.hbm_pattern_bandwidth(kind, run=run)
+            log(f"runs of {run} pairs, {name}: {gbs:.1f} GB/s ({ms:.3f} ms)")
+            out[f"runs{run}_{'aligned' if kind == 3 else 'packed'}_gbs"] = round(gbs, 1)
+    return out
 
 
 def bench_edr(G, W) -> dict:

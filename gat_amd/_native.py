@@ -67,6 +67,7 @@ HIP_FUNCS = {
     "gat_event_elapsed_ms": (INT, [C.POINTER(F32), P, P]),
     "gat_eps_threshold": (F32, [F64]),
     "gat_stream_copy": (INT, [P, P, SZ, P]),
+    "gat_hbm_pattern": (INT, [INT, P, SZ, U32, P]),
     "gatRangeQueryTestHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
     "gatRangeQuerySTIGHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
     "gat_range_refine_bitmap": (INT, [P, P, I64, P, I64, P, I64, P]),

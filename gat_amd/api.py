@@ -43,6 +43,44 @@ def eps_threshold(eps: float) -> float:
     return float(N.hip().gat_eps_threshold(eps))
 
 
+def hbm_pattern_bandwidth(kind: int, nbytes: int = 1 << 32, run: int = 0, reps: int = 10) -> tuple[float, float]:
+    """Bandwidth of one HBM access pattern (gat_hbm_pattern: 0 write stream,
+    1 read stream, 2 packed runs of `run` pairs, 3 the same runs through
+    line-aligned windows): (GB/s of the bytes the pattern covers, mean kernel
+    ms), timed with HIP events on its own stream."""
+    h = N.hip()
+    buf, s, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    N.check_hip(h.gat_malloc(C.byref(buf), nbytes))
+    try:
+        N.check_hip(h.gat_stream_create(C.byref(s)))
+        N.check_hip(h.gat_event_create(C.byref(e0)))
+        N.check_hip(h.gat_event_create(C.byref(e1)))
+        N.check_hip(h.gat_memset(buf, 0, nbytes, s))
+        N.check_hip(h.gat_hbm_pattern(kind, buf, nbytes, run, s))
+        N.check_hip(h.gat_event_record(e0, s))
+        for _ in range(reps):
+            N.check_hip(h.gat_hbm_pattern(kind, buf, nbytes, run, s))
+        N.check_hip(h.gat_event_record(e1, s))
+        N.check_hip(h.gat_event_sync(e1))
+        ms = C.c_float()
+        N.check_hip(h.gat_event_elapsed_ms(C.byref(ms), e0, e1))
+        per = ms.value / reps
+        covered = nbytes
+        if kind >= 2:
+            nruns = 1
+            while nruns * 2 * run * 8 <= nbytes:
+                nruns *= 2
+            covered = nruns * run * 8
+        return covered / (per * 1e-3) / 1e9, per
+    finally:
+        for e in (e0, e1):
+            if e.value:
+                h.gat_event_destroy(e)
+        if s.value:
+            h.gat_stream_destroy(s)
+        h.gat_free(buf)
+
+
 def stream_copy_bandwidth(nbytes: int = 1 << 32, reps: int = 10) -> tuple[float, float]:
     """HBM bandwidth of the plain streaming-copy kernel: (GB/s counting read +
     write bytes, mean kernel ms), timed with HIP events on its own stream."""

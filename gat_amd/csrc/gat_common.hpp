@@ -50,8 +50,8 @@ int cu_count();
 
 namespace gat {
 
-// A result pair written with a streaming (nontemporal) store: pair outputs
-// are written once and not read again by the kernel that writes them.
+// A result pair written with a streaming (nontemporal) store, for kernels
+// whose output runs are long and written once (the per-query scan).
 __device__ __forceinline__ void store_pair(int2* dst, int2 v) {
   typedef int v2i __attribute__((ext_vector_type(2)));
   v2i w;

@@ -561,7 +561,10 @@ __device__ __forceinline__ void refine_chunk(const Store& store, uint64_t at0, u
         } else {
           p = make_int2(int(id[k]), int(pid[k]));
         }
-        gat::store_pair(pairs + at, p);
+        // A cached store: runs of different items that share a line can
+        // meet in the L2 before it is written back (faster here than the
+        // streaming store the per-query scan uses).
+        pairs[at] = p;
       }
       before += uint32_t(__popcll(m));
     }

@@ -81,6 +81,15 @@ int gat_event_elapsed_ms(float* ms, void* start, void* stop);
  * kernels.  No reference counterpart.  Asynchronous on `stream`. */
 int gat_stream_copy(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
 
+/* HBM access-pattern ceilings over a buffer of `bytes` (a positive multiple of
+ * 16): kind 0 writes the buffer with 16-byte streaming stores, 1 reads it; 2
+ * and 3 write runs of `run` (1..4096) 8-byte pairs packed back to back, one
+ * run per wave step in scattered order, as result runs from many waves land
+ * in per-query regions: 2 with 64-lane stores from each run's first pair, 3
+ * through 64-lane windows aligned to 128-byte lines.  No reference
+ * counterpart.  Asynchronous on `stream`. */
+int gat_hbm_pattern(int kind, void* buf_dev, size_t bytes, uint32_t run, void* stream);
+
 /* ------------------------------------------------------------------ range */
 /* A resident point store: the index's points in its own order.  Ids are
  * compact: with pid_bits > 0, id_dev[i] = (tid << pid_bits) | pid and pid_dev
