@@ -133,8 +133,9 @@ def main() -> int:
     log(f"workload {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points, {len(rects):,} rectangles "
         f"({time.perf_counter() - t:.1f} s)")
     t = time.perf_counter()
-    grid = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
-    log(f"GT-quadtree built and resident ({time.perf_counter() - t:.1f} s): {grid.info().tolist()}")
+    grid = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db, device=True)
+    build_device_s = time.perf_counter() - t
+    log(f"GT-quadtree built on the device and resident ({build_device_s:.2f} s): {grid.info().tolist()}")
 
     if args.only == "range":
         res, qps, total, _ = time_range(grid, rects, args.steps, args.warmup, "GT-quadtree plain")
@@ -170,9 +171,16 @@ def main() -> int:
     log(f"stream copy reference: {copy_gbs:.1f} GB/s")
 
     t = time.perf_counter()
+    host_grid = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    build_host_s = time.perf_counter() - t
+    same_store = bool((host_grid.cell_starts() == grid.cell_starts()).all())
+    del host_grid
+    log(f"GT-quadtree host build for comparison: {build_host_s:.2f} s (device {build_device_s:.2f} s); same cell "
+        f"offsets: {same_store}")
+    t = time.perf_counter()
     stig = G.StigIndex(w.leaf).build(db)
-    fsg = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db)
-    log(f"FSG and STIG built ({time.perf_counter() - t:.1f} s)")
+    fsg = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db, device=True)
+    log(f"FSG (device) and STIG (host) built ({time.perf_counter() - t:.1f} s)")
     few = max(3, args.steps // 2)
     _, fqps, _, _ = time_range(fsg, rects, few, 1, "FSG plain")
     _, sqps, _, _ = time_range(stig, rects, few, 1, "STIG plain")
@@ -262,6 +270,9 @@ def main() -> int:
             "range_plain_p50_ms": round(1e3 * float(np.percentile(pwall, 50)), 3),
             "range_plain_p99_ms": round(1e3 * float(np.percentile(pwall, 99)), 3),
             "stream_copy_gbs": round(copy_gbs, 1),
+            "gt_quadtree_build_device_s": round(build_device_s, 3),
+            "gt_quadtree_build_host_s": round(build_host_s, 3),
+            "device_build_equals_host": same_store,
             **edr,
             **serve,
         },
@@ -282,9 +293,9 @@ def bench_mixed(G, W, ntraj, nq) -> dict:
     t = time.perf_counter()
     db = w.db(ntraj)
     rects = w.rects(nq)
-    g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db, device=True)
     s = G.StigIndex(w.leaf).build(db)
-    f = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db)
+    f = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db, device=True)
     log(f"mixed workload {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points, {len(rects):,} rectangles, "
         f"three indexes resident ({time.perf_counter() - t:.1f} s)")
     m = G.MixedRange(g, s, f)
@@ -332,7 +343,7 @@ def bench_serve(G, W, ntraj, seconds) -> dict:
     w = W.CFG5
     t = time.perf_counter()
     db = w.db(ntraj)
-    g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db)
+    g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db, device=True)
     log(f"serving index {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points resident "
         f"({time.perf_counter() - t:.1f} s)")
     req, qlon, qlat = W.serve_requests(w, db, seconds)

@@ -68,6 +68,7 @@ HIP_FUNCS = {
     "gat_eps_threshold": (F32, [F64]),
     "gat_stream_copy": (INT, [P, P, SZ, P]),
     "gat_hbm_pattern": (INT, [INT, P, SZ, U32, P]),
+    "gat_cell_store_build": (INT, [P, P, P, P, I32, I64, I32, P, P, P, P, P, P]),
     "gatRangeQueryTestHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
     "gatRangeQuerySTIGHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
     "gat_range_refine_bitmap": (INT, [P, P, I64, P, I64, P, I64, P]),
@@ -116,6 +117,8 @@ HOST_FUNCS = {
     "gath_index_range_pairs": (INT, [P, P, INT, P, P]),
     "gath_index_range_batched": (INT, [P, P, INT, INT, P, P]),
     "gath_index_tree_nodes": (I64, [P, P, I64, P]),
+    "gath_index_fetch_store": (INT, [P, P, P, P, P]),
+    "gath_index_cell_starts": (I64, [P, P, I64]),
     "gath_index_set_batch_params": (INT, [P, INT, INT]),
     "gath_index_last_unique_candidates": (INT, [P, P]),
     "gath_index_last_counts": (INT, [P, P]),

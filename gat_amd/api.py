@@ -252,10 +252,12 @@ class _Index:
             N.host().gath_index_free(h)
             self.handle = None
 
-    def build(self, db: TrajSet, upload: bool = True) -> "_Index":
+    def build(self, db: TrajSet, upload: bool = True, device: bool = False) -> "_Index":
+        """Builds the index over `db`; upload=False keeps it on the host.
+        device=True builds the cell-ordered store on the GPU (Grid and FSG)."""
         N.check_host(N.host().gath_index_build(self.handle, _ptr(_req(db.offsets, np.int64)), db.ntraj,
                                                _ptr(_req(db.lon, np.float32)), _ptr(_req(db.lat, np.float32)),
-                                               int(upload)))
+                                               (2 if device else 1) if upload else 0))
         self.ntraj = db.ntraj
         return self
 
@@ -332,6 +334,29 @@ class _Index:
         return {"queries_per_item": int(out[0]),
                 "ms": dict(zip([4, 8, 16, 32, 64], [round(float(x), 4) for x in out[1:6]])),
                 "direct_ms": round(float(out[6]), 4), "direct_max_queries": int(out[7])}
+
+    def fetch_store(self) -> tuple[np.ndarray, np.ndarray, np.ndarray | None, int]:
+        """The resident point store: (xy (n, 2) float32, id uint32, pid uint16 or
+        None, pid_bits)."""
+        meta = np.zeros(2, np.int64)
+        N.check_host(N.host().gath_index_fetch_store(self.handle, None, None, None, _ptr(meta)))
+        n, bits = int(meta[0]), int(meta[1])
+        xy = np.zeros((max(n, 1), 2), np.float32)
+        ids = np.zeros(max(n, 1), np.uint32)
+        pid = np.zeros(max(n, 1), np.uint16) if bits == 0 else None
+        if n > 0:
+            N.check_host(N.host().gath_index_fetch_store(self.handle, xy.ctypes.data, ids.ctypes.data,
+                                                         pid.ctypes.data if pid is not None else None, _ptr(meta)))
+        return xy[:n], ids[:n], (pid[:n] if pid is not None else None), bits
+
+    def cell_starts(self) -> np.ndarray:
+        """Grid / FSG: the store offset of every cell (cells + 1 entries)."""
+        n = int(N.host().gath_index_cell_starts(self.handle, None, 0))
+        if n < 0:
+            N.check_host(n)
+        out = np.zeros(max(n, 1), np.int64)
+        N.host().gath_index_cell_starts(self.handle, out.ctypes.data, n)
+        return out[:n]
 
     def tree_nodes(self) -> tuple[np.ndarray, int]:
         """The tree the batched form uploads: a structured array of gat_tree_node

@@ -44,6 +44,17 @@ int FSG::addDatasetToGrid(const TrajectoryView& db, bool upload) {
     gath_set_error("FSG: %lld points exceed the 32-bit store index", (long long)N);
     return GAT_ERR_ARG;
   }
+  if (upload && deviceBuild) {
+    const gat_cell_grid g{0, cell_num_x, cell_num_y, range.xmin, range.ymax, cell_size};
+    int64_t bad = -1;
+    if (int rc = engine_.buildStore(g, db, &cellStart, &bad)) return rc;
+    if (bad >= 0) {
+      gath_set_error("FSG: point %lld (%g, %g) lies outside the grid", (long long)bad, double(db.lon[bad]),
+                     double(db.lat[bad]));
+      return GAT_ERR_ARG;
+    }
+    return uploadCells();
+  }
   // Stable bucketing by cell: cells in id order, points of a cell in
   // trajectory order (FSG.cpp:122-141).
   std::vector<uint32_t> cell(static_cast<size_t>(N));
@@ -75,6 +86,11 @@ int FSG::addDatasetToGrid(const TrajectoryView& db, bool upload) {
     }
   }
   if (int rc = engine_.upload(xy.data(), tp.data(), N)) return rc;
+  return uploadCells();
+}
+
+// The device cell table over the cell offsets, once the point store is resident.
+int FSG::uploadCells() {
   std::vector<gat_tree_node> cells(static_cast<size_t>(cellnum));
   for (int64_t c = 0; c < cellnum; ++c) {
     gat_tree_node& d = cells[size_t(c)];

@@ -45,6 +45,9 @@ class FSG {
   int64_t totalPointNum = 0;
   int trajNum = 0;
   int hostThreads = 0;
+  // With upload: build the cell-ordered store on the device (gat_cell_store_build)
+  // instead of bucketing on the host; the same store either way.
+  bool deviceBuild = false;
   std::vector<int64_t> cellStart;  // cellnum + 1
   RangeEngine& engine() { return engine_; }
   const DeviceTree& deviceTree() const { return tree_; }
@@ -52,6 +55,7 @@ class FSG {
  private:
   void addRows(const MBB& q, uint32_t qid, TaskList& list) const;
   int threads() const;
+  int uploadCells();
   RangeEngine engine_;
   DeviceTree tree_;
 };

@@ -68,6 +68,17 @@ int Grid::addDatasetToGrid(const TrajectoryView& db, bool upload) {
     gath_set_error("Grid: %lld points exceed the 32-bit store index", (long long)N);
     return GAT_ERR_ARG;
   }
+  if (upload && deviceBuild) {
+    const gat_cell_grid g{1, cellNum_axis, cellNum_axis, range.xmin, range.ymax, cell_size};
+    int64_t bad = -1;
+    if (int rc = engine_.buildStore(g, db, &cellStart, &bad)) return rc;
+    if (bad >= 0) {
+      gath_set_error("Grid: point %lld (%g, %g) lies outside the grid", (long long)bad, double(db.lon[bad]),
+                     double(db.lat[bad]));
+      return GAT_ERR_ARG;
+    }
+    return finishBuild(db);
+  }
   // Split the trajectories into contiguous groups of about equal point count;
   // each thread owns one group, so the order within a cell stays the
   // reference's (sub-trajectories in trajectory-id order, Grid.cpp:242-261).
@@ -149,12 +160,24 @@ int Grid::addDatasetToGrid(const TrajectoryView& db, bool upload) {
   cell.clear();
   cell.shrink_to_fit();
 
+  if (!upload) {
+    nodes.clear();
+    nodeNum = 1;
+    buildNode(0, 0);
+    return GAT_OK;
+  }
+  if (int rc = engine_.upload(xy.data(), tp.data(), N)) return rc;
+  xy = std::vector<float>();
+  tp = std::vector<int32_t>();
+  return finishBuild(db);
+}
+
+// The quadtree over the cell offsets, its device copy and the trajectory
+// store, once the point store is resident.
+int Grid::finishBuild(const TrajectoryView& db) {
   nodes.clear();
   nodeNum = 1;
   buildNode(0, 0);
-
-  if (!upload) return GAT_OK;
-  if (int rc = engine_.upload(xy.data(), tp.data(), N)) return rc;
   int depth = 0;
   const std::vector<gat_tree_node> tn = treeNodes(&depth);
   if (int rc = tree_.upload(tn, 4, depth, engine_.stream())) return rc;

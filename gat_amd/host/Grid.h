@@ -113,6 +113,9 @@ class Grid {
   // Run-time EDR threshold (the reference's compile-time EPSILON, ConstDefine.h:28).
   double epsilon = 0.005;
   int hostThreads = 0;  // 0: hardware concurrency (capped at 16)
+  // With upload: build the cell-ordered store on the device (gat_cell_store_build)
+  // instead of bucketing on the host; the same store either way.
+  bool deviceBuild = false;
 
   // Geometry (Grid.h:69-80).
   MBB range;
@@ -134,6 +137,7 @@ class Grid {
 
  private:
   int buildNode(int level, int id);
+  int finishBuild(const TrajectoryView& db);
   int threads() const;
   int uploadTrajectories(const TrajectoryView& db);
   int ensureQueryStore(const TrajectoryView& queries);

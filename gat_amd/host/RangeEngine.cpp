@@ -114,6 +114,66 @@ int RangeEngine::upload(const float* xy, const int32_t* tp, int64_t n) {
   return stream_.sync();
 }
 
+int RangeEngine::buildStore(const gat_cell_grid& grid, const TrajectoryView& db, std::vector<int64_t>* cellStart,
+                            int64_t* badPoint) {
+  if (int rc = stream_.create()) return rc;
+  *badPoint = -1;
+  const int64_t n = db.pointNum();
+  if (db.trajNum < 0 || !db.offsets || db.offsets[0] != 0) {
+    gath_set_error("store build: offsets must start at 0");
+    return GAT_ERR_ARG;
+  }
+  uint32_t max_pid = 0;
+  for (int t = 1; t <= db.trajNum; ++t) {
+    const int64_t len = db.offsets[t] - db.offsets[t - 1];
+    if (len < 0) {
+      gath_set_error("store build: offsets decrease at trajectory %d", t);
+      return GAT_ERR_ARG;
+    }
+    if (len > 0) max_pid = std::max(max_pid, uint32_t(std::min<int64_t>(len - 1, UINT32_MAX)));
+  }
+  const uint32_t max_tid = uint32_t(db.trajNum);
+  const int pid_bits = std::max(bit_length(max_pid), 1);
+  const bool packed = bit_length(max_tid) + pid_bits <= 32;
+  if (!packed && max_pid > 0xFFFFu) {
+    gath_set_error("store build: point ids up to %u do not fit 16 bits", max_pid);
+    return GAT_ERR_ARG;
+  }
+  const size_t cnt = size_t(std::max<int64_t>(n, 1));
+  const uint64_t cellnum = uint64_t(grid.cells_x) * uint64_t(grid.cells_y);
+  void* s = stream_.get();
+  DeviceBuffer lon, lat, off, starts;
+  if (int rc = lon.reserve(cnt * 4)) return rc;
+  if (int rc = lat.reserve(cnt * 4)) return rc;
+  if (int rc = off.reserve((size_t(db.trajNum) + 1) * 8)) return rc;
+  if (int rc = starts.reserve((cellnum + 1) * 8)) return rc;
+  if (int rc = xy_.reserve(cnt * 8)) return rc;
+  if (int rc = id_.reserve(cnt * 4)) return rc;
+  if (packed)
+    pid_.release();
+  else if (int rc = pid_.reserve(cnt * 2))
+    return rc;
+  if (n > 0) {
+    GATH_TRY(gat_memcpy_h2d(lon.get(), db.lon, size_t(n) * 4, s), "trajectory upload");
+    GATH_TRY(gat_memcpy_h2d(lat.get(), db.lat, size_t(n) * 4, s), "trajectory upload");
+  }
+  GATH_TRY(gat_memcpy_h2d(off.get(), db.offsets, (size_t(db.trajNum) + 1) * 8, s), "trajectory upload");
+  GATH_TRY(gat_cell_store_build(&grid, lon.as<float>(), lat.as<float>(), off.as<int64_t>(), db.trajNum, n,
+                                packed ? pid_bits : 0, xy_.as<float>(), id_.as<uint32_t>(),
+                                packed ? nullptr : pid_.as<uint16_t>(), starts.as<int64_t>(), badPoint, s),
+           "store build");
+  n_ = 0;
+  store_ = gat_point_store{};
+  if (*badPoint >= 0) return GAT_OK;
+  cellStart->assign(size_t(cellnum) + 1, 0);
+  GATH_TRY(gat_memcpy_d2h(cellStart->data(), starts.get(), (cellnum + 1) * 8, s), "cell offsets");
+  if (int rc = stream_.sync()) return rc;
+  n_ = n;
+  store_ = gat_point_store{xy_.as<float>(), id_.as<uint32_t>(), packed ? nullptr : pid_.as<uint16_t>(), n,
+                           packed ? pid_bits : 0};
+  return GAT_OK;
+}
+
 // Copies boxes, tasks (and qbase) into one pinned staging block and then to the
 // device in as few transfers as the layout allows.
 int RangeEngine::stage(const MBB* bounds, int nq, const std::vector<gat_range_task>& tasks, const uint64_t* qbase) {

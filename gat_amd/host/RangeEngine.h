@@ -16,6 +16,7 @@
 
 #include "DeviceArena.h"
 #include "MBB.h"
+#include "Trajectory.h"
 
 // Candidate runs longer than this are split across tasks so one wave never
 // walks an unbounded run (the reference splits at MAXPOINTINNODE,
@@ -100,6 +101,12 @@ class RangeEngine {
   // Uploads the index-ordered store: xy holds (lon, lat) pairs, tp (tid, pid)
   // pairs; ids are packed into 32 bits when they fit (gat_point_store).
   int upload(const float* xy, const int32_t* tp, int64_t n);
+  // Builds the store on the device from trajectory-ordered points, ordered by
+  // the cells of `grid` (gat_cell_store_build), with the same id packing as
+  // upload(); *cellStart receives the grid's cellnum + 1 store offsets and
+  // *badPoint the first point off the grid (then no store is kept) or -1.
+  int buildStore(const gat_cell_grid& grid, const TrajectoryView& db, std::vector<int64_t>* cellStart,
+                 int64_t* badPoint);
   int64_t pointNum() const { return n_; }
   const gat_point_store& store() const { return store_; }
   void* stream() { return stream_.get(); }

@@ -96,10 +96,15 @@ void gath_index_free(void* p) { delete as_index(p); }
 int gath_index_build(void* p, const int64_t* offsets, int ntraj, const float* lon, const float* lat, int upload) {
   Index* ix = as_index(p);
   if (!ix || !offsets || ntraj < 0) return bad("gath_index_build: bad arguments");
+  if (upload < 0 || upload > 2) return bad("gath_index_build: upload must be 0, 1 or 2");
   TrajectoryView db{offsets, ntraj, lon, lat};
   switch (ix->kind) {
-    case Index::kGrid: return ix->grid->addDatasetToGrid(db, upload != 0);
-    case Index::kFsg: return ix->fsg->addDatasetToGrid(db, upload != 0);
+    case Index::kGrid:
+      ix->grid->deviceBuild = upload == 2;
+      return ix->grid->addDatasetToGrid(db, upload != 0);
+    case Index::kFsg:
+      ix->fsg->deviceBuild = upload == 2;
+      return ix->fsg->addDatasetToGrid(db, upload != 0);
     default: return ix->stig->initial(ix->stig->blockSize, db, upload != 0);
   }
 }
@@ -255,6 +260,32 @@ int64_t gath_index_tree_nodes(void* p, gat_tree_node* out, int64_t cap, int32_t*
   *depth = d;
   for (int64_t i = 0; i < std::min<int64_t>(cap, int64_t(t.size())); ++i) out[i] = t[size_t(i)];
   return int64_t(t.size());
+}
+
+int gath_index_fetch_store(void* p, float* xy, uint32_t* id, uint16_t* pid, int64_t* meta) {
+  Index* ix = as_index(p);
+  if (!ix || !meta) return bad("gath_index_fetch_store: bad arguments");
+  RangeEngine& e = ix->grid ? ix->grid->engine() : ix->fsg ? ix->fsg->engine() : ix->stig->engine();
+  const gat_point_store& st = e.store();
+  meta[0] = st.npoints;
+  meta[1] = st.pid_bits;
+  if (!xy || st.npoints == 0) return GAT_OK;
+  if (!id || (st.pid_bits == 0 && !pid)) return bad("gath_index_fetch_store: null id array");
+  const size_t n = size_t(st.npoints);
+  GATH_TRY(gat_memcpy_d2h(xy, st.xy_dev, n * 8, e.stream()), "store readback");
+  GATH_TRY(gat_memcpy_d2h(id, st.id_dev, n * 4, e.stream()), "store readback");
+  if (st.pid_bits == 0) GATH_TRY(gat_memcpy_d2h(pid, st.pid_dev, n * 2, e.stream()), "store readback");
+  GATH_TRY(gat_stream_sync(e.stream()), "store readback");
+  return GAT_OK;
+}
+
+int64_t gath_index_cell_starts(void* p, int64_t* out, int64_t cap) {
+  Index* ix = as_index(p);
+  if (!ix || (cap > 0 && !out)) return bad("gath_index_cell_starts: bad arguments");
+  const std::vector<int64_t>* cs = ix->grid ? &ix->grid->cellStart : ix->fsg ? &ix->fsg->cellStart : nullptr;
+  if (!cs) return bad("gath_index_cell_starts: STIG has no cells");
+  for (int64_t i = 0; i < std::min<int64_t>(cap, int64_t(cs->size())); ++i) out[i] = (*cs)[size_t(i)];
+  return int64_t(cs->size());
 }
 
 int gath_index_last_counts(void* p, int64_t* counts) {

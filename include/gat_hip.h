@@ -81,6 +81,37 @@ int gat_event_elapsed_ms(float* ms, void* start, void* stop);
  * kernels.  No reference counterpart.  Asynchronous on `stream`. */
 int gat_stream_copy(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
 
+/* ---------------------------------------------------------- index build */
+
+/* Geometry of a cell grid: cells_x columns and cells_y rows of cell_size,
+ * columns counted right from xmin and rows down from ymax.  morton = 1: the
+ * GT-quadtree's square Morton-numbered grid (Grid.cpp:97-124; cells_x ==
+ * cells_y, a power of two); 0: FSG's row-major grid (FSG.cpp:7-28). */
+typedef struct gat_cell_grid {
+  int32_t morton;
+  int32_t cells_x, cells_y;
+  float xmin, ymax, cell_size;
+} gat_cell_grid;
+
+/* Builds the cell-ordered point store on the device (replaces the host
+ * bucketing of Grid::addDatasetToGrid, Grid.cpp:202-261, and
+ * FSG::addDatasetToGrid, FSG.cpp:95-143): trajectories 1..ntraj, trajectory t
+ * at points [offsets[t-1], offsets[t]) of lon / lat (trajectory order;
+ * offsets_dev nondecreasing from 0 to npoints).  Every point goes to the cell
+ * of the reference's float formula (Grid.cpp:202-208, FSG.cpp:88-93); points
+ * are ordered by cell and, within a cell, keep trajectory and point order.
+ * Outputs, in that order: xy_out (npoints lon, lat pairs), id_out (trajectory
+ * id << pid_bits | point id when pid_bits > 0, else the trajectory id with the
+ * point id in pid_out) and cell_start_out (cells_x * cells_y + 1 store
+ * offsets, the last = npoints).  *bad_point receives the first point that lies
+ * off the grid (nothing else is written then), or -1.  Allocates and frees its
+ * own scratch (about 20 bytes per point); returns after `stream` has drained.
+ * No reference counterpart (the reference builds on the host). */
+int gat_cell_store_build(const gat_cell_grid* grid, const float* lon_dev, const float* lat_dev,
+                         const int64_t* offsets_dev, int32_t ntraj, int64_t npoints, int32_t pid_bits,
+                         float* xy_out_dev, uint32_t* id_out_dev, uint16_t* pid_out_dev, int64_t* cell_start_out_dev,
+                         int64_t* bad_point, void* stream);
+
 /* HBM access-pattern ceilings over a buffer of `bytes` (a positive multiple of
  * 16): kind 0 writes the buffer with 16-byte streaming stores, 1 reads it; 2
  * and 3 write runs of `run` (1..4096) 8-byte pairs packed back to back, one

@@ -49,7 +49,9 @@ void* gath_fsg_new(const float* range, float cell_size);
 void* gath_stig_new(int block_size);
 void gath_index_free(void* index);
 /* Builds the index over trajectories 1..ntraj (offsets: ntraj+1 entries); with
- * upload = 0 only the host structures are built (no GPU needed). */
+ * upload = 0 only the host structures are built (no GPU needed); 1 buckets the
+ * points on the host and uploads the store; 2 uploads the trajectories and
+ * builds the same store on the device (Grid and FSG; STIG builds on the host). */
 int gath_index_build(void* index, const int64_t* offsets, int ntraj, const float* lon, const float* lat,
                      int upload);
 /* out[0..7]: points, trajectories, cells (or leaves for STIG), cells per axis
@@ -90,6 +92,13 @@ int gath_index_set_batch_params(void* index, int queries_per_item, int direct_ma
 /* The index's tree as the batched form uploads it (breadth-first
  * gat_tree_node); returns the node count, copies up to cap nodes. */
 int64_t gath_index_tree_nodes(void* index, gat_tree_node* out, int64_t cap, int32_t* depth);
+/* The resident point store: meta[0] = points, meta[1] = pid_bits (0: point ids
+ * in `pid`); with xy non-null, copies xy (2 floats per point), id and, when
+ * pid_bits is 0, pid to the host. */
+int gath_index_fetch_store(void* index, float* xy, uint32_t* id, uint16_t* pid, int64_t* meta);
+/* Grid / FSG: the cell offsets (cells + 1 entries; returns the count, copies
+ * up to cap). */
+int64_t gath_index_cell_starts(void* index, int64_t* out, int64_t cap);
 int gath_index_last_counts(void* index, int64_t* counts);
 /* Pairs of the last result, query by query (query q's pairs follow those of
  * queries 0..q-1); cap in pairs. */
