@@ -109,6 +109,14 @@ HOST_FUNCS = {
     "gath_stig_new": (P, [INT]),
     "gath_index_free": (None, [P]),
     "gath_index_build": (INT, [P, P, INT, P, P, INT]),
+    "gath_traj_read": (P, [C.c_char_p]),
+    "gath_traj_ingest_csv": (P, [C.c_char_p, C.c_char_p]),
+    "gath_traj_free": (None, [P]),
+    "gath_traj_info": (INT, [P, P, P]),
+    "gath_traj_copy": (INT, [P, P, P, P, P, P]),
+    "gath_traj_write": (INT, [C.c_char_p, P, INT, P, P, P, P, P]),
+    "gath_traj_days_between": (INT, [C.c_char_p, C.c_char_p]),
+    "gath_traj_vehicle_distance": (F32, [F32, F32, F32, F32]),
     "gath_index_info": (INT, [P, P]),
     "gath_index_set_threads": (INT, [P, INT]),
     "gath_index_select": (I64, [P, P, INT, P, I64, P]),

@@ -81,6 +81,54 @@ def hbm_pattern_bandwidth(kind: int, nbytes: int = 1 << 32, run: int = 0, reps: 
         h.gat_free(buf)
 
 
+def _table(handle) -> tuple["TrajSet", np.ndarray, np.ndarray, np.ndarray]:
+    if not handle:
+        raise N.GatError(2, N.host().gath_last_error().decode(errors="replace"))
+    try:
+        sizes = np.zeros(2, np.int64)
+        bounds = np.zeros(4, np.float32)
+        N.check_host(N.host().gath_traj_info(handle, _ptr(sizes), _ptr(bounds)))
+        n, m = int(sizes[0]), int(sizes[1])
+        offsets = np.zeros(n + 1, np.int64)
+        lon, lat = np.zeros(max(m, 1), np.float32), np.zeros(max(m, 1), np.float32)
+        tid, tim = np.zeros(max(m, 1), np.int32), np.zeros(max(m, 1), np.int32)
+        N.check_host(N.host().gath_traj_copy(handle, _ptr(offsets), _ptr(lon), _ptr(lat), _ptr(tid), _ptr(tim)))
+        return TrajSet(offsets, lon[:m], lat[:m]), tid[:m], tim[:m], bounds
+    finally:
+        N.host().gath_traj_free(handle)
+
+
+def read_trajectory_file(path: str) -> tuple["TrajSet", np.ndarray, np.ndarray, np.ndarray]:
+    """The reference's preprocessed trajectory file (PreProcess.cpp:225-315):
+    (trajectories, per-point tid field, per-point time, bounds xmin, ymin,
+    xmax, ymax)."""
+    return _table(N.host().gath_traj_read(str(path).encode()))
+
+
+def ingest_vehicle_csv(path: str, base_date: str) -> tuple["TrajSet", np.ndarray, np.ndarray, np.ndarray]:
+    """Trajectories from raw vehicle records (PreProcess.cpp:48-201), as
+    read_trajectory_file returns them; times in seconds from base_date."""
+    return _table(N.host().gath_traj_ingest_csv(str(path).encode(), base_date.encode()))
+
+
+def write_trajectory_file(path: str, db: "TrajSet", tid: np.ndarray | None = None, time: np.ndarray | None = None,
+                          bounds: np.ndarray | None = None) -> None:
+    """Writes the preprocessed format (PreProcess.cpp:203-223).  Defaults: tid
+    field = the trajectory id, time = 0, bounds = the points' box."""
+    m = db.npoints
+    if tid is None:
+        tid = np.repeat(np.arange(1, db.ntraj + 1, dtype=np.int32), np.diff(db.offsets))
+    if time is None:
+        time = np.zeros(m, np.int32)
+    if bounds is None:
+        bounds = (np.array([db.lon.min(), db.lat.min(), db.lon.max(), db.lat.max()], np.float32) if m else
+                  np.zeros(4, np.float32))
+    N.check_host(N.host().gath_traj_write(str(path).encode(), _ptr(_req(db.offsets, np.int64)), db.ntraj,
+                                          _ptr(_req(db.lon, np.float32)), _ptr(_req(db.lat, np.float32)),
+                                          _ptr(_req(tid, np.int32)), _ptr(_req(time, np.int32)),
+                                          _ptr(_req(bounds, np.float32))))
+
+
 def stream_copy_bandwidth(nbytes: int = 1 << 32, reps: int = 10) -> tuple[float, float]:
     """HBM bandwidth of the plain streaming-copy kernel: (GB/s counting read +
     write bytes, mean kernel ms), timed with HIP events on its own stream."""

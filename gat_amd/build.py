@@ -30,7 +30,8 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 
 HIP_SOURCES = ["runtime.hip", "util.hip", "range.hip", "range_batched.hip", "edr.hip", "edr_batched.hip", "build.hip"]
 HOST_SOURCES = ["MBB.cpp", "Synth.cpp", "DeviceArena.cpp", "RangeEngine.cpp", "Grid.cpp", "GridSimilarity.cpp", "FSG.cpp",
-                "STIG.cpp", "SystemTest.cpp", "Server.cpp", "Autotune.cpp", "MixedRange.cpp", "host_capi.cpp"]
+                "STIG.cpp", "SystemTest.cpp", "Server.cpp", "Autotune.cpp", "MixedRange.cpp", "TrajFile.cpp",
+                "host_capi.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -9,6 +9,7 @@
 #include "Autotune.h"
 #include "MixedRange.h"
 #include "STIG.h"
+#include "TrajFile.h"
 #include "Server.h"
 #include "Synth.h"
 #include "gat_host.h"
@@ -260,6 +261,95 @@ int64_t gath_index_tree_nodes(void* p, gat_tree_node* out, int64_t cap, int32_t*
   *depth = d;
   for (int64_t i = 0; i < std::min<int64_t>(cap, int64_t(t.size())); ++i) out[i] = t[size_t(i)];
   return int64_t(t.size());
+}
+
+// ------------------------------------------------------------ file formats
+
+namespace {
+TrajectoryTable* as_table(void* p) { return static_cast<TrajectoryTable*>(p); }
+}  // namespace
+
+void* gath_traj_read(const char* path) {
+  if (!path) {
+    gath_set_error("gath_traj_read: null path");
+    return nullptr;
+  }
+  auto* t = new (std::nothrow) TrajectoryTable();
+  if (!t) return nullptr;
+  if (readTrajectoryFile(path, t) != GAT_OK) {
+    delete t;
+    return nullptr;
+  }
+  return t;
+}
+
+void* gath_traj_ingest_csv(const char* path, const char* base_date) {
+  if (!path || !base_date) {
+    gath_set_error("gath_traj_ingest_csv: null argument");
+    return nullptr;
+  }
+  auto* t = new (std::nothrow) TrajectoryTable();
+  if (!t) return nullptr;
+  if (ingestVehicleCsv(path, base_date, t) != GAT_OK) {
+    delete t;
+    return nullptr;
+  }
+  return t;
+}
+
+void gath_traj_free(void* t) { delete as_table(t); }
+
+int gath_traj_info(void* p, int64_t* sizes, float* bounds) {
+  TrajectoryTable* t = as_table(p);
+  if (!t || !sizes || !bounds) return bad("gath_traj_info: bad arguments");
+  sizes[0] = t->trajNum();
+  sizes[1] = t->offsets.back();
+  bounds[0] = t->xmin;
+  bounds[1] = t->ymin;
+  bounds[2] = t->xmax;
+  bounds[3] = t->ymax;
+  return GAT_OK;
+}
+
+int gath_traj_copy(void* p, int64_t* offsets, float* lon, float* lat, int32_t* tid, int32_t* time) {
+  TrajectoryTable* t = as_table(p);
+  if (!t || !offsets) return bad("gath_traj_copy: bad arguments");
+  const size_t np = t->lon.size();
+  if (np > 0 && (!lon || !lat || !tid || !time)) return bad("gath_traj_copy: null point array");
+  std::copy(t->offsets.begin(), t->offsets.end(), offsets);
+  std::copy(t->lon.begin(), t->lon.end(), lon);
+  std::copy(t->lat.begin(), t->lat.end(), lat);
+  std::copy(t->tid.begin(), t->tid.end(), tid);
+  std::copy(t->time.begin(), t->time.end(), time);
+  return GAT_OK;
+}
+
+int gath_traj_write(const char* path, const int64_t* offsets, int ntraj, const float* lon, const float* lat,
+                    const int32_t* tid, const int32_t* time, const float* bounds) {
+  if (!path || !offsets || ntraj < 0 || !bounds) return bad("gath_traj_write: bad arguments");
+  TrajectoryTable t;
+  t.offsets.assign(offsets, offsets + ntraj + 1);
+  const int64_t np = t.offsets.back();
+  if (t.offsets[0] != 0 || np < 0) return bad("gath_traj_write: offsets must run from 0");
+  if (np > 0 && (!lon || !lat || !tid || !time)) return bad("gath_traj_write: null point array");
+  t.lon.assign(lon, lon + np);
+  t.lat.assign(lat, lat + np);
+  t.tid.assign(tid, tid + np);
+  t.time.assign(time, time + np);
+  t.xmin = bounds[0];
+  t.ymin = bounds[1];
+  t.xmax = bounds[2];
+  t.ymax = bounds[3];
+  return writeTrajectoryFile(path, t);
+}
+
+int gath_traj_days_between(const char* a, const char* b) {
+  if (!a || !b) return -1;
+  return daysBetweenDates(a, b);
+}
+
+float gath_traj_vehicle_distance(float lat_a, float lon_a, float lat_b, float lon_b) {
+  return vehicleDistance(lat_a, lon_a, lat_b, lon_b);
 }
 
 int gath_index_fetch_store(void* p, float* xy, uint32_t* id, uint16_t* pid, int64_t* meta) {

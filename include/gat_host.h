@@ -43,6 +43,31 @@ void gath_synth_queries(uint64_t seed, int nq, const int32_t* qlen, const int64_
                         const int64_t* db_offsets, int db_ntraj, const float* db_lon, const float* db_lat,
                         int copy_mode, double noise_sigma, double step_sigma, float* qlon, float* qlat);
 
+/* On-disk formats (PreProcess.cpp).  gath_traj_read parses the preprocessed
+ * text file (readTraFromFormatedFile, PreProcess.cpp:225-315: trajectory
+ * count, "xmin,ymin,xmax,ymax", lengths, then "lat,lon,tid,time;" per point,
+ * latitude first); gath_traj_ingest_csv builds trajectories from raw vehicle
+ * records (the PreProcess constructor, PreProcess.cpp:48-201: field 2 vehicle
+ * id, 3 longitude, 4 latitude, 8 "yyyy-mm-dd hh:mm:ss", times in seconds
+ * from base_date; a vehicle's trajectory ends at 1024 points or a gap over
+ * 3600 s, points at 50 m/s or more are dropped, Trajectory.cpp:22-48).  Both
+ * return a table handle (NULL on error, see gath_last_error) holding
+ * trajectories 1..n: gath_traj_info gives sizes[0] = n, sizes[1] = points and
+ * the bounds; gath_traj_copy copies offsets (n + 1), lon, lat, tid and time.
+ * gath_traj_write writes the preprocessed format (writeTraDataToFile,
+ * PreProcess.cpp:203-223).  gath_traj_days_between and
+ * gath_traj_vehicle_distance expose the ingest's date arithmetic
+ * (DateTime.cpp:44-95) and speed estimate (CalcInAxis.h:9-21). */
+void* gath_traj_read(const char* path);
+void* gath_traj_ingest_csv(const char* path, const char* base_date);
+void gath_traj_free(void* table);
+int gath_traj_info(void* table, int64_t* sizes, float* bounds);
+int gath_traj_copy(void* table, int64_t* offsets, float* lon, float* lat, int32_t* tid, int32_t* time);
+int gath_traj_write(const char* path, const int64_t* offsets, int ntraj, const float* lon, const float* lat,
+                    const int32_t* tid, const int32_t* time, const float* bounds);
+int gath_traj_days_between(const char* date_a, const char* date_b);
+float gath_traj_vehicle_distance(float lat_a, float lon_a, float lat_b, float lon_b);
+
 /* Indexes.  range = (xmin, ymin, xmax, ymax). */
 void* gath_grid_new(const float* range, float cell_size, int max_points_in_node);
 void* gath_fsg_new(const float* range, float cell_size);
