@@ -387,11 +387,13 @@ def bench_hbm(G) -> dict:
         gbs, ms = G.hbm_pattern_bandwidth(kind)
         log(f"{name}: {gbs:.1f} GB/s ({ms:.3f} ms)")
         out[name.replace(" ", "_") + "_gbs"] = round(gbs, 1)
-    for run in (28, 32, 110, 255, 256, 257):
-        for kind, name in [(2, "packed"), (3, "line-aligned windows")]:
+    for run in (28, 32, 110, 255, 256, 257, 830):
+        for kind, name, key in [(2, "packed", "packed"), (3, "line-aligned windows", "aligned"),
+                                (4, "packed, cached stores", "packed_cached"),
+                                (5, "line-aligned windows, cached stores", "aligned_cached")]:
             gbs, ms = G.hbm_pattern_bandwidth(kind, run=run)
             log(f"runs of {run} pairs, {name}: {gbs:.1f} GB/s ({ms:.3f} ms)")
-            out[f"runs{run}_{'aligned' if kind == 3 else 'packed'}_gbs"] = round(gbs, 1)
+            out[f"runs{run}_{key}_gbs"] = round(gbs, 1)
     return out
 
 

@@ -87,6 +87,8 @@ HIP_FUNCS = {
     "gat_grid_select": (INT, [P, I32, I32, F32, F32, F32, P, P, I64, I64, P, P, P, I64, P, P]),
     "gat_leaf_scatter": (INT, [P, P, I64, P, I64, P, P, P, I32, P, P, I64, P, I64, P, I64, P]),
     "gat_leaf_refine": (INT, [P, P, P, I64, P, I64, I64, P, P, I64, P, P]),
+    "gat_leaf_refine_runs": (INT, [P, P, P, I64, P, I64, I64, P, P, I64, P, I64, P, P, P, P, P]),
+    "gat_runs_digest": (INT, [P, I64, P, I64, I64, P, P]),
     "gat_graph_begin": (INT, [P]),
     "gat_graph_end": (INT, [P, P]),
     "gat_graph_launch": (INT, [P, P]),
@@ -138,6 +140,8 @@ HOST_FUNCS = {
     "gath_tune_range": (INT, [P, P, INT, INT, P]),
     "gath_tune_edr": (INT, [P, P, INT, P, P, F64, INT, P]),
     "gath_index_last_fetch": (INT, [P, P, I64]),
+    "gath_index_last_runs": (I64, [P, P, I64]),
+    "gath_index_set_result_layout": (INT, [P, INT]),
     "gath_index_last_digests": (INT, [P, P]),
     "gath_grid_edr_all": (INT, [P, P, P, INT, F64, P]),
     "gath_grid_similarity": (INT, [P, P, INT, P, P, INT, F64, P, P]),

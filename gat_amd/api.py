@@ -46,8 +46,9 @@ def eps_threshold(eps: float) -> float:
 def hbm_pattern_bandwidth(kind: int, nbytes: int = 1 << 32, run: int = 0, reps: int = 10) -> tuple[float, float]:
     """Bandwidth of one HBM access pattern (gat_hbm_pattern: 0 write stream,
     1 read stream, 2 packed runs of `run` pairs, 3 the same runs through
-    line-aligned windows): (GB/s of the bytes the pattern covers, mean kernel
-    ms), timed with HIP events on its own stream."""
+    line-aligned windows, 4 and 5 the same two with cached stores): (GB/s of
+    the bytes the pattern covers, mean kernel ms), timed with HIP events on
+    its own stream."""
     h = N.hip()
     buf, s, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
     N.check_hip(h.gat_malloc(C.byref(buf), nbytes))
@@ -251,6 +252,7 @@ REQ_RANGE, REQ_TOPK = 0, 1
 TREE_NODE = np.dtype([("box", np.float32, 4), ("first_child", np.int32), ("start", np.uint32), ("count", np.uint32),
                       ("flags", np.uint32)])
 NODE_INCLUSIVE, NODE_UNTESTED = 1, 2
+RUN = np.dtype([("q", np.uint32), ("count", np.uint32), ("first", np.uint64)])  # gat_range_run
 
 
 class RangeResult:
@@ -278,6 +280,19 @@ class RangeResult:
         out = np.empty((max(self.pairs_total, 1), 2), np.int32)
         N.check_host(N.host().gath_index_last_fetch(self._index.handle, _ptr(out), self.pairs_total))
         return out[: self.pairs_total]
+
+    def runs(self) -> np.ndarray:
+        """Run-indexed batches: the run records (RUN dtype) of the result's
+        compacted pairs; empty for per-query layouts."""
+        n = int(N.host().gath_index_last_runs(self._index.handle, None, 0))
+        if n < 0:
+            N.check_host(n)
+        out = np.zeros(max(n, 1), RUN)
+        if n > 0:
+            got = int(N.host().gath_index_last_runs(self._index.handle, out.ctypes.data, n))
+            if got < 0:
+                N.check_host(got)
+        return out[:n]
 
     def per_query(self) -> list[np.ndarray]:
         c = self.counts()
@@ -363,6 +378,13 @@ class _Index:
         out = np.zeros(1, np.int64)
         N.check_host(N.host().gath_index_last_unique_candidates(self.handle, _ptr(out)))
         return int(out[0])
+
+    def set_result_layout(self, run_indexed: bool = True) -> "_Index":
+        """Leaf-major result layout: pairs by work item with a run index
+        (default), or per-query regions.  Counts, digests and fetched pairs are
+        the same either way."""
+        N.check_host(N.host().gath_index_set_result_layout(self.handle, int(run_indexed)))
+        return self
 
     def set_batch_params(self, queries_per_item: int = 32, direct_max_queries: int = 256) -> "_Index":
         """Batched form: rectangles per leaf-major work item (1..64), and the

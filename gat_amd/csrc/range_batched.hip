@@ -416,6 +416,22 @@ struct Item {
 };
 static_assert(sizeof(Item) == 16, "item layout");
 
+// Output bound of a work item for the run-indexed refine: every rectangle
+// could take every point, rounded up to whole 128-byte lines (16 pairs);
+// items past the total bound nothing.
+struct LoadItemBounds {
+  const Item* items;
+  const uint32_t* item_total;
+  uint64_t cap;
+  using T = uint64_t;
+  __device__ T operator()(uint64_t i) const {
+    const uint64_t total = *item_total;
+    if (i >= (total < cap ? total : cap)) return 0;
+    const Item it = items[i];
+    return (uint64_t(it.n) * it.count + 15) & ~uint64_t(15);
+  }
+};
+
 __global__ void leaf_scatter_kernel(const uint2* __restrict__ lq, const unsigned long long* __restrict__ lq_total,
                                     uint64_t lq_cap, const Node* __restrict__ nodes, uint32_t nnodes,
                                     const uint2* __restrict__ node_off, const uint32_t* __restrict__ node_qcount,
@@ -622,6 +638,344 @@ __global__ __launch_bounds__(kRefBlock) void leaf_refine_kernel(
   }
 }
 
+// ------------------------------------------------------- run-indexed refine
+// The same join with the output laid out by work item instead of by query:
+// work item t owns the region [item_base[t], item_base[t + 1]) of `pairs`
+// (an exclusive scan of its bound, rectangles x leaf points rounded up to a
+// 128-byte line), its hits fill the region from the start, rectangle after
+// rectangle, and list entry e of the item records its run (query, hit count,
+// first pair) in runs[e].  A region is written by one wave and starts on a
+// line, so no line is shared between waves; in per-query regions every run's
+// end lines are.  Nothing is reserved with atomics (one counter for every item
+// would serialise the batch).  An item whose leaf spans several chunks first
+// counts each rectangle's hits over all of them, so it still makes one run per
+// rectangle.
+struct Run {
+  uint32_t q, count;
+  uint64_t first;
+};
+static_assert(sizeof(Run) == sizeof(gat_range_run), "run record layout");
+
+template <int NS, bool kPacked>
+__device__ __forceinline__ void load_chunk(const Store& store, uint64_t at0, uint32_t left, uint32_t lane,
+                                           float2 (&v)[NS], uint32_t (&id)[NS], uint32_t (&pid)[NS]) {
+  const float nan = __builtin_nanf("");
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const uint32_t i = k * kWave + lane;
+    v[k] = make_float2(nan, 0.0f);
+    id[k] = 0;
+    pid[k] = 0;
+    if (i < left) {
+      v[k] = store.xy[at0 + i];
+      id[k] = store.id[at0 + i];
+      if constexpr (!kPacked) pid[k] = store.pid[at0 + i];
+    }
+  }
+}
+
+// Lane j receives rectangle j's hit masks (32-bit halves) and hit count.
+template <int NS>
+__device__ __forceinline__ uint32_t mask_pass(const float2 (&v)[NS], uint32_t nql, const float4& bj, uint32_t lane,
+                                              uint32_t (&mlo)[NS], uint32_t (&mhi)[NS]) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) mlo[k] = mhi[k] = 0;
+  uint32_t myhits = 0;
+  for (uint32_t j = 0; j < nql; ++j) {
+    const float4 box = make_float4(readlane_f(bj.x, j), readlane_f(bj.y, j), readlane_f(bj.z, j),
+                                   readlane_f(bj.w, j));
+    uint32_t h = 0;
+    const bool mine = lane == j;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const uint64_t m = box_mask(box, v[k].x, v[k].y);
+      h += uint32_t(__popcll(m));
+      mlo[k] = mine ? uint32_t(m) : mlo[k];
+      mhi[k] = mine ? uint32_t(m >> 32) : mhi[k];
+    }
+    myhits = mine ? h : myhits;
+  }
+  return myhits;
+}
+
+// Writes every hit; rectangle j's hits of this chunk start at lane j's `first`.
+template <int NS, bool kPacked>
+__device__ __forceinline__ void write_pass(const Store& store, const uint32_t (&id)[NS], const uint32_t (&pid)[NS],
+                                           uint32_t nql, uint32_t myhits, uint64_t first_j,
+                                           const uint32_t (&mlo)[NS], const uint32_t (&mhi)[NS],
+                                           int2* __restrict__ pairs, uint64_t capacity, uint32_t lane) {
+  const uint32_t pid_mask = (1u << store.pid_bits) - 1u;
+  for (uint32_t j = 0; j < nql; ++j) {
+    if (readlane_u(myhits, j) == 0) continue;
+    const uint64_t first =
+        uint64_t(readlane_u(uint32_t(first_j), j)) | (uint64_t(readlane_u(uint32_t(first_j >> 32), j)) << 32);
+    uint32_t before = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const uint64_t m = uint64_t(readlane_u(mlo[k], j)) | (uint64_t(readlane_u(mhi[k], j)) << 32);
+      const uint64_t at = first + before + lanes_below(m);
+      if (((m >> lane) & 1) && at < capacity) {
+        int2 p;
+        if constexpr (kPacked) {
+          p = make_int2(int(id[k] >> store.pid_bits), int(id[k] & pid_mask));
+        } else {
+          p = make_int2(int(id[k]), int(pid[k]));
+        }
+        pairs[at] = p;
+      }
+      before += uint32_t(__popcll(m));
+    }
+  }
+}
+
+// Per-wave staging ring of the run-indexed refine: an item's pairs are
+// appended here in output order and leave in whole blocks of 128 pairs
+// (1 KB, one 16-byte store per lane, line-aligned because the item's region
+// is), so a block costs one store instruction instead of one per rectangle
+// and slot.
+constexpr uint32_t kRing = 512;   // pairs per wave (4 KB)
+constexpr uint32_t kFlush = 128;  // pairs per flush: 2 per lane
+
+// Orders one lane's LDS writes before other lanes' reads of them and the
+// reads before later writes to the same slots (LDS executes a wave's
+// instructions in order; this keeps the compiler from reordering them).
+__device__ __forceinline__ void lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+// write_pass through the ring: the item's pairs fill [base, base + total)
+// back to back, rectangle j's from base + (first_j - base).
+template <int NS, bool kPacked>
+__device__ __forceinline__ void write_pass_staged(const Store& store, const uint32_t (&id)[NS],
+                                                  const uint32_t (&pid)[NS], uint32_t nql, uint32_t myhits,
+                                                  uint64_t base, const uint32_t (&mlo)[NS],
+                                                  const uint32_t (&mhi)[NS], int2* __restrict__ pairs,
+                                                  uint64_t capacity, uint32_t lane, int2* ring) {
+  const uint32_t pid_mask = (1u << store.pid_bits) - 1u;
+  uint32_t fill = 0, flushed = 0;  // pairs appended / stored, relative to base
+  auto flush_block = [&]() __attribute__((always_inline)) {
+    lds_order();
+    const int4 v = reinterpret_cast<const int4*>(ring)[((flushed & (kRing - 1)) >> 1) + lane];
+    const uint64_t at = base + flushed + 2 * lane;
+    if (at + 1 < capacity) *reinterpret_cast<int4*>(pairs + at) = v;
+    flushed += kFlush;
+    lds_order();
+  };
+  for (uint32_t j = 0; j < nql; ++j) {
+    const uint32_t h = readlane_u(myhits, j);
+    if (h == 0) continue;
+    uint32_t before = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const uint64_t m = uint64_t(readlane_u(mlo[k], j)) | (uint64_t(readlane_u(mhi[k], j)) << 32);
+      if ((m >> lane) & 1) {
+        int2 p;
+        if constexpr (kPacked) {
+          p = make_int2(int(id[k] >> store.pid_bits), int(id[k] & pid_mask));
+        } else {
+          p = make_int2(int(id[k]), int(pid[k]));
+        }
+        ring[(fill + before + lanes_below(m)) & (kRing - 1)] = p;
+      }
+      before += uint32_t(__popcll(m));
+    }
+    fill += h;
+    // Unstored pairs stay under kFlush after each rectangle and a rectangle
+    // adds at most 256, so appends never reach a slot still waiting.
+    while (fill - flushed >= kFlush) flush_block();
+  }
+  // The tail: fewer than 128 pairs, two per lane while both exist.
+  lds_order();
+  const uint32_t rem = fill - flushed;
+  const uint32_t i = 2 * lane;
+  const uint64_t at = base + flushed + i;
+  if (i + 1 < rem) {
+    const int4 v = reinterpret_cast<const int4*>(ring)[((flushed & (kRing - 1)) >> 1) + lane];
+    if (at + 1 < capacity) *reinterpret_cast<int4*>(pairs + at) = v;
+  } else if (i < rem) {
+    const int2 v = ring[(flushed + i) & (kRing - 1)];
+    if (at < capacity) pairs[at] = v;
+  }
+  lds_order();
+}
+
+template <int NS, bool kPacked>
+__device__ __forceinline__ uint32_t count_chunk(const Store& store, uint64_t at0, uint32_t left, uint32_t nql,
+                                                const float4& bj, uint32_t lane) {
+  float2 v[NS];
+  uint32_t id[NS], pid[NS], mlo[NS], mhi[NS];
+  load_chunk<NS, kPacked>(store, at0, left, lane, v, id, pid);
+  return mask_pass<NS>(v, nql, bj, lane, mlo, mhi);
+}
+
+template <int NS, bool kPacked>
+__device__ __forceinline__ uint32_t refine_run_chunk(const Store& store, uint64_t at0, uint32_t left, uint32_t nql,
+                                                     const float4& bj, uint64_t first_j, int2* __restrict__ pairs,
+                                                     uint64_t capacity, uint32_t lane) {
+  float2 v[NS];
+  uint32_t id[NS], pid[NS], mlo[NS], mhi[NS];
+  load_chunk<NS, kPacked>(store, at0, left, lane, v, id, pid);
+  const uint32_t myhits = mask_pass<NS>(v, nql, bj, lane, mlo, mhi);
+  write_pass<NS, kPacked>(store, id, pid, nql, myhits, first_j, mlo, mhi, pairs, capacity, lane);
+  return myhits;
+}
+
+// Slots of 64 points a chunk of `left` (>= 1) remaining points spans, at most kSlots.
+__device__ __forceinline__ uint32_t slots_for(uint32_t left) {
+  return left >= uint32_t(kSpan) ? uint32_t(kSlots) : (left + kWave - 1) / kWave;
+}
+
+template <bool kPacked>
+__device__ __forceinline__ uint32_t count_any(const Store& store, uint64_t at0, uint32_t left, uint32_t nql,
+                                              const float4& bj, uint32_t lane) {
+  switch (slots_for(left)) {
+    case 1: return count_chunk<1, kPacked>(store, at0, left, nql, bj, lane);
+    case 2: return count_chunk<2, kPacked>(store, at0, left, nql, bj, lane);
+    case 3: return count_chunk<3, kPacked>(store, at0, left, nql, bj, lane);
+    default: return count_chunk<4, kPacked>(store, at0, left, nql, bj, lane);
+  }
+}
+
+template <bool kPacked>
+__device__ __forceinline__ uint32_t refine_run_any(const Store& store, uint64_t at0, uint32_t left, uint32_t nql,
+                                                   const float4& bj, uint64_t first_j, int2* __restrict__ pairs,
+                                                   uint64_t capacity, uint32_t lane) {
+  switch (slots_for(left)) {
+    case 1: return refine_run_chunk<1, kPacked>(store, at0, left, nql, bj, first_j, pairs, capacity, lane);
+    case 2: return refine_run_chunk<2, kPacked>(store, at0, left, nql, bj, first_j, pairs, capacity, lane);
+    case 3: return refine_run_chunk<3, kPacked>(store, at0, left, nql, bj, first_j, pairs, capacity, lane);
+    default: return refine_run_chunk<4, kPacked>(store, at0, left, nql, bj, first_j, pairs, capacity, lane);
+  }
+}
+
+// Inclusive prefix sum over the wave's lanes.
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t y = uint32_t(__shfl_up(int(x), d, kWave));
+    x += lane >= uint32_t(d) ? y : 0u;
+  }
+  return x;
+}
+
+struct RunOut {
+  uint32_t* qcount;
+  Run* runs;
+  uint64_t runs_cap;
+  const uint64_t* item_base;
+};
+
+// Places the item's runs (lane j: `hits` of rectangle j, query qj, list entry
+// list + j) one after another from the start of its region and records them;
+// false when the item has no hits.  *first_j: where lane j's pairs start.
+__device__ __forceinline__ bool place_item(uint32_t hits, uint32_t qj, uint32_t lane, uint32_t nql, uint32_t list,
+                                           uint64_t base, const RunOut& o, uint64_t* first_j) {
+  const uint32_t incl = wave_prefix(hits, lane);
+  *first_j = base + (incl - hits);
+  if (lane < nql && uint64_t(list) + lane < o.runs_cap) o.runs[uint64_t(list) + lane] = Run{qj, hits, *first_j};
+  if (hits > 0) atomicAdd(o.qcount + qj, hits);
+  return readlane_u(incl, kWave - 1) > 0;
+}
+
+// An item whose leaf fits one chunk: masks once, then the writes.
+template <int NS, bool kPacked>
+__device__ __forceinline__ void single_chunk_item(const Store& store, uint32_t start, uint32_t count, uint32_t nql,
+                                                  uint32_t list, uint64_t base, const float4& bj, uint32_t qj,
+                                                  const RunOut& o, int2* __restrict__ pairs, uint64_t capacity,
+                                                  uint32_t lane, int2* ring) {
+  float2 v[NS];
+  uint32_t id[NS], pid[NS], mlo[NS], mhi[NS];
+  load_chunk<NS, kPacked>(store, start, count, lane, v, id, pid);
+  const uint32_t hits = mask_pass<NS>(v, nql, bj, lane, mlo, mhi);
+  uint64_t first_j = 0;
+  if (!place_item(hits, qj, lane, nql, list, base, o, &first_j)) return;
+  write_pass_staged<NS, kPacked>(store, id, pid, nql, hits, base, mlo, mhi, pairs, capacity, lane, ring);
+}
+
+template <bool kPacked>
+__global__ __launch_bounds__(kRefBlock) void leaf_refine_runs_kernel(
+    Store store, const Item* __restrict__ items, const uint32_t* __restrict__ item_total, uint64_t items_cap,
+    const ListEntry* __restrict__ qlist, uint64_t qlist_cap, uint32_t nqueries, int2* __restrict__ pairs,
+    uint64_t capacity, RunOut o) {
+  __shared__ int4 rings[kRefBlock / kWave][kRing / 2];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  int2* ring = reinterpret_cast<int2*>(rings[threadIdx.x / kWave]);
+  const uint64_t wave = uniform((blockIdx.x * kRefBlock + threadIdx.x) / kWave);
+  const uint64_t nwaves = uint64_t(gridDim.x) * (kRefBlock / kWave);
+  const uint64_t total = *item_total;
+  const uint64_t nitems = total < items_cap ? total : items_cap;
+  const float nan = __builtin_nanf("");
+  Item next = wave < nitems ? items[wave] : Item{0, 0, 0, 0};
+  for (uint64_t t = wave; t < nitems; t += nwaves) {
+    const uint32_t start = uniform(next.start), count = uniform(next.count);
+    const uint32_t list = uniform(next.list), nql = uniform(next.n);
+    if (t + nwaves < nitems) next = items[t + nwaves];
+    if (uint64_t(start) + count > store.npoints || count == 0 || nql == 0 || nql > kMaxQb ||
+        uint64_t(list) + nql > qlist_cap)
+      continue;
+    uint32_t qj = 0xFFFFFFFFu;
+    float4 bj = make_float4(nan, nan, nan, nan);
+    if (lane < nql) {
+      const ListEntry e = qlist[uint64_t(list) + lane];
+      if (e.q < nqueries) {
+        qj = e.q;
+        bj = e.box;
+      }
+    }
+    const uint64_t base = o.item_base[t];
+    if (count <= uint32_t(kSpan)) {
+      switch (slots_for(count)) {
+        case 1:
+          single_chunk_item<1, kPacked>(store, start, count, nql, list, base, bj, qj, o, pairs, capacity, lane,
+                                           ring);
+          break;
+        case 2:
+          single_chunk_item<2, kPacked>(store, start, count, nql, list, base, bj, qj, o, pairs, capacity, lane,
+                                           ring);
+          break;
+        case 3:
+          single_chunk_item<3, kPacked>(store, start, count, nql, list, base, bj, qj, o, pairs, capacity, lane,
+                                           ring);
+          break;
+        default:
+          single_chunk_item<4, kPacked>(store, start, count, nql, list, base, bj, qj, o, pairs, capacity, lane,
+                                           ring);
+      }
+      continue;
+    }
+    // A leaf of several chunks: every rectangle's hits over all of them first,
+    // so each rectangle gets one run; then chunk by chunk into it.
+    uint32_t hits = 0;
+    for (uint32_t c = 0; c < count; c += kSpan)
+      hits += count_any<kPacked>(store, uint64_t(start) + c, count - c, nql, bj, lane);
+    uint64_t first_j = 0;
+    if (!place_item(hits, qj, lane, nql, list, base, o, &first_j)) continue;
+    for (uint32_t c = 0; c < count; c += kSpan)
+      first_j += refine_run_any<kPacked>(store, uint64_t(start) + c, count - c, nql, bj, first_j, pairs, capacity,
+                                         lane);
+  }
+}
+
+// Per-query digest of run-indexed pairs: one wave per run adds the run's
+// wrapping sum of pair mixes to its query's entry (out must start at zero).
+__global__ __launch_bounds__(256) void runs_digest_kernel(const int2* __restrict__ pairs, uint64_t npairs,
+                                                          const Run* __restrict__ runs, uint64_t nruns,
+                                                          uint32_t nqueries, unsigned long long* __restrict__ out) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t nw = uint64_t(gridDim.x) * (blockDim.x / kWave);
+  for (uint64_t r = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave; r < nruns; r += nw) {
+    const Run run = runs[r];
+    if (run.q >= nqueries || run.first > npairs || run.count > npairs - run.first) continue;
+    uint64_t sum = 0;
+    for (uint64_t i = lane; i < run.count; i += kWave) sum += gat::pair_mix(pairs[run.first + i]);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(sum)), d, kWave));
+      const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(sum >> 32)), d, kWave));
+      sum += uint64_t(lo) | (uint64_t(hi) << 32);
+    }
+    if (lane == 0) atomicAdd(out + run.q, (unsigned long long)sum);
+  }
+}
+
 // Sum of n uint32 values into *out (which must start at zero).
 __global__ void sum_u32_kernel(const uint32_t* __restrict__ in, uint64_t n, unsigned long long* __restrict__ out) {
   uint64_t s = 0;
@@ -787,6 +1141,69 @@ int gat_leaf_refine(const gat_point_store* sp, const uint32_t* items, const uint
         qcount, uint64_t(nqueries), reinterpret_cast<unsigned long long*>(pair_total));
     GAT_LAUNCH_CHECK("sum_u32_kernel");
   }
+  return GAT_OK;
+}
+
+int gat_leaf_refine_runs(const gat_point_store* sp, const uint32_t* items, const uint32_t* item_total,
+                         int64_t items_capacity, const void* qlist, int64_t qlist_capacity, int64_t nqueries,
+                         uint32_t* qcount, int32_t* pairs, int64_t capacity, gat_range_run* runs,
+                         int64_t runs_capacity, uint64_t* item_base, uint64_t* tiles, uint64_t* bound_total,
+                         uint64_t* pair_total, void* stream) {
+  GAT_REQUIRE(sp && sp->npoints >= 0 && sp->pid_bits >= 0 && sp->pid_bits < 32, "gat_leaf_refine_runs: bad store");
+  GAT_REQUIRE(sp->npoints == 0 || (sp->xy_dev && sp->id_dev && (sp->pid_bits > 0 || sp->pid_dev)),
+              "gat_leaf_refine_runs: null store array");
+  GAT_REQUIRE(item_total && bound_total && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX) &&
+                  items_capacity >= 0 && qlist_capacity >= 0 && capacity >= 0 && runs_capacity >= 0,
+              "gat_leaf_refine_runs: bad sizes");
+  hipStream_t s = gat::as_stream(stream);
+  if (items_capacity > 0 && nqueries > 0) {
+    GAT_REQUIRE(items && qlist && qcount && pairs && runs && item_base && tiles,
+                "gat_leaf_refine_runs: null pointer");
+    // Regions: the exclusive scan of the items' bounds; *bound_total = their sum.
+    if (int rc = launch_scan(LoadItemBounds{reinterpret_cast<const Item*>(items), item_total,
+                                            uint64_t(items_capacity)},
+                             uint64_t(items_capacity), tiles, item_base, bound_total, s))
+      return rc;
+    const Store st{reinterpret_cast<const float2*>(sp->xy_dev), sp->id_dev, sp->pid_dev, uint64_t(sp->npoints),
+                   uint32_t(sp->pid_bits)};
+    const int blocks = grid_for(uint64_t(items_capacity), kRefBlock / kWave, 16);
+    const dim3 grid{uint32_t(blocks)}, block{kRefBlock};
+    auto* I = reinterpret_cast<const Item*>(items);
+    auto* L = static_cast<const ListEntry*>(qlist);
+    auto* P = reinterpret_cast<int2*>(pairs);
+    const RunOut o{qcount, reinterpret_cast<Run*>(runs), uint64_t(runs_capacity), item_base};
+    if (sp->pid_bits > 0)
+      leaf_refine_runs_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
+                                                           uint64_t(qlist_capacity), uint32_t(nqueries), P,
+                                                           uint64_t(capacity), o);
+    else
+      leaf_refine_runs_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
+                                                            uint64_t(qlist_capacity), uint32_t(nqueries), P,
+                                                            uint64_t(capacity), o);
+    GAT_LAUNCH_CHECK("leaf_refine_runs_kernel");
+  }
+  if (pair_total && nqueries > 0) {
+    GAT_REQUIRE(qcount, "gat_leaf_refine_runs: null counts");
+    sum_u32_kernel<<<dim3{uint32_t(grid_for(uint64_t(nqueries), 256, 1))}, dim3{256}, 0, s>>>(
+        qcount, uint64_t(nqueries), reinterpret_cast<unsigned long long*>(pair_total));
+    GAT_LAUNCH_CHECK("sum_u32_kernel");
+  }
+  return GAT_OK;
+}
+
+int gat_runs_digest(const int32_t* pairs, int64_t npairs, const gat_range_run* runs, int64_t nruns,
+                    int64_t nqueries, uint64_t* out, void* stream) {
+  GAT_REQUIRE(npairs >= 0 && nruns >= 0 && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX),
+              "gat_runs_digest: bad sizes");
+  if (nqueries == 0) return GAT_OK;
+  GAT_REQUIRE(out && (nruns == 0 || (pairs && runs)), "gat_runs_digest: null pointer");
+  if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(out), 0u, 2 * nqueries, stream)) return rc;
+  if (nruns == 0) return GAT_OK;
+  runs_digest_kernel<<<dim3{uint32_t(grid_for(uint64_t(nruns) * kWave, 256, 16))}, dim3{256}, 0,
+                       gat::as_stream(stream)>>>(reinterpret_cast<const int2*>(pairs), uint64_t(npairs),
+                                                 reinterpret_cast<const Run*>(runs), uint64_t(nruns),
+                                                 uint32_t(nqueries), reinterpret_cast<unsigned long long*>(out));
+  GAT_LAUNCH_CHECK("runs_digest_kernel");
   return GAT_OK;
 }
 

@@ -55,6 +55,8 @@ __global__ __launch_bounds__(256) void stream_read_kernel(const v4f* __restrict_
 // runs from many waves leaves in per-query regions.  aligned = false: 64-lane
 // stores starting at the run's first pair; true: 64-lane windows aligned to
 // 16 pairs (128-byte lines), so only the run's two end lines are partial.
+// Streaming (nontemporal) or cached stores.
+template <bool kStreaming>
 __global__ __launch_bounds__(256) void packed_runs_kernel(v2i* __restrict__ dst, uint64_t nruns, uint32_t run,
                                                           bool aligned) {
   const uint32_t lane = threadIdx.x & 63;
@@ -68,7 +70,10 @@ __global__ __launch_bounds__(256) void packed_runs_kernel(v2i* __restrict__ dst,
       const uint64_t i = b + lane;
       if (i >= s && i < e) {
         const v2i p = {int(r), int(lane)};
-        __builtin_nontemporal_store(p, dst + i);
+        if constexpr (kStreaming)
+          __builtin_nontemporal_store(p, dst + i);
+        else
+          dst[i] = p;
       }
     }
   }
@@ -77,7 +82,7 @@ __global__ __launch_bounds__(256) void packed_runs_kernel(v2i* __restrict__ dst,
 }  // namespace
 
 extern "C" int gat_hbm_pattern(int kind, void* buf, size_t bytes, uint32_t run, void* stream) {
-  GAT_REQUIRE(kind >= 0 && kind <= 3, "HBM pattern: kind outside 0..3");
+  GAT_REQUIRE(kind >= 0 && kind <= 5, "HBM pattern: kind outside 0..5");
   GAT_REQUIRE(bytes % 16 == 0 && bytes >= 16, "HBM pattern: size must be a positive multiple of 16 bytes");
   GAT_REQUIRE(buf, "HBM pattern: null buffer");
   hipStream_t s = gat::as_stream(stream);
@@ -95,7 +100,12 @@ extern "C" int gat_hbm_pattern(int kind, void* buf, size_t bytes, uint32_t run, 
     GAT_REQUIRE(run >= 1 && run <= 4096 && uint64_t(run) * 8 <= bytes, "HBM pattern: run outside 1..4096 pairs");
     uint64_t nruns = 1;
     while (nruns * 2 * run * 8 <= bytes) nruns *= 2;
-    packed_runs_kernel<<<dim3(uint32_t(cap)), dim3(256), 0, s>>>(static_cast<v2i*>(buf), nruns, run, kind == 3);
+    if (kind <= 3)
+      packed_runs_kernel<true><<<dim3(uint32_t(cap)), dim3(256), 0, s>>>(static_cast<v2i*>(buf), nruns, run,
+                                                                         kind == 3);
+    else
+      packed_runs_kernel<false><<<dim3(uint32_t(cap)), dim3(256), 0, s>>>(static_cast<v2i*>(buf), nruns, run,
+                                                                          kind == 5);
   }
   GAT_LAUNCH_CHECK("HBM pattern kernel");
   return GAT_OK;

@@ -274,6 +274,65 @@ int RangeBatch::syncHost() {
   return GAT_OK;
 }
 
+// Run-indexed batches: the runs and the used pairs, copied once.
+static int download_runs(const RangeBatch& b, std::vector<gat_range_run>& runs, std::vector<int32_t>& pairs) {
+  runs.resize(size_t(b.runTotal));
+  pairs.resize(size_t(b.pairSpan) * 2);
+  if (b.runTotal > 0)
+    GATH_TRY(gat_memcpy_d2h(runs.data(), b.runs.get(), size_t(b.runTotal) * sizeof(gat_range_run), b.stream),
+             "run readback");
+  if (b.pairSpan > 0)
+    GATH_TRY(gat_memcpy_d2h(pairs.data(), b.pairs.get(), size_t(b.pairSpan) * 8, b.stream), "pair readback");
+  GATH_TRY(gat_stream_sync(b.stream), "pair readback");
+  for (const gat_range_run& r : runs)
+    if (r.count > 0 && (r.q >= uint32_t(b.queryNum) || r.first + r.count > uint64_t(b.pairSpan))) {
+      gath_set_error("run index: a run lies outside the batch");
+      return GAT_ERR_STATE;
+    }
+  return GAT_OK;
+}
+
+int RangeBatch::fetchRuns(gat_range_run* out, int64_t capacity) {
+  if (!runIndexed) return GAT_OK;
+  if (runTotal > capacity) {
+    gath_set_error("fetchRuns: %lld runs, room for %lld", (long long)runTotal, (long long)capacity);
+    return GAT_ERR_CAPACITY;
+  }
+  if (runTotal > 0)
+    GATH_TRY(gat_memcpy_d2h(out, runs.get(), size_t(runTotal) * sizeof(gat_range_run), stream), "run readback");
+  GATH_TRY(gat_stream_sync(stream), "run readback");
+  return GAT_OK;
+}
+
+int RangeBatch::fetchCompact(int32_t* out, int64_t capacityPairs) {
+  if (int rc = syncHost()) return rc;
+  if (pairTotal > capacityPairs) {
+    gath_set_error("fetch: %lld pairs, room for %lld", (long long)pairTotal, (long long)capacityPairs);
+    return GAT_ERR_CAPACITY;
+  }
+  std::vector<int64_t> at(size_t(queryNum) + 1, 0);
+  for (int q = 0; q < queryNum; ++q) at[size_t(q) + 1] = at[size_t(q)] + count[size_t(q)];
+  if (runIndexed) {
+    std::vector<gat_range_run> rs;
+    std::vector<int32_t> ps;
+    if (int rc = download_runs(*this, rs, ps)) return rc;
+    for (const gat_range_run& r : rs) {
+      if (r.count == 0) continue;
+      std::memcpy(out + 2 * at[r.q], ps.data() + 2 * r.first, size_t(r.count) * 8);
+      at[r.q] += r.count;
+    }
+    return GAT_OK;
+  }
+  for (int q = 0; q < queryNum; ++q) {
+    if (count[size_t(q)] == 0) continue;
+    GATH_TRY(gat_memcpy_d2h(out + 2 * at[size_t(q)], pairs.as<int32_t>() + 2 * base[size_t(q)],
+                            size_t(count[size_t(q)]) * 8, stream),
+             "pair readback");
+  }
+  GATH_TRY(gat_stream_sync(stream), "pair readback");
+  return GAT_OK;
+}
+
 int RangeBatch::fetch(int32_t* out, int64_t capacityPairs) {
   if (int rc = syncHost()) return rc;
   // Query regions are laid out back to back by candidate count; copy each
@@ -282,6 +341,18 @@ int RangeBatch::fetch(int32_t* out, int64_t capacityPairs) {
   if (need > capacityPairs) {
     gath_set_error("fetch: %lld pair slots needed, %lld available", (long long)need, (long long)capacityPairs);
     return GAT_ERR_CAPACITY;
+  }
+  if (runIndexed) {
+    std::vector<gat_range_run> rs;
+    std::vector<int32_t> ps;
+    if (int rc = download_runs(*this, rs, ps)) return rc;
+    std::vector<uint64_t> at(base);
+    for (const gat_range_run& r : rs) {
+      if (r.count == 0) continue;
+      std::memcpy(out + 2 * at[r.q], ps.data() + 2 * r.first, size_t(r.count) * 8);
+      at[r.q] += r.count;
+    }
+    return GAT_OK;
   }
   for (int q = 0; q < queryNum; ++q) {
     if (count[q] == 0) continue;
@@ -301,9 +372,15 @@ int RangeBatch::digests(uint64_t* out) const {
 
 int RangeBatch::digestsInto(uint64_t* dev, uint64_t* out) const {
   if (queryNum == 0) return GAT_OK;
-  GATH_TRY(gat_pairs_digest(pairs.as<int32_t>(), qbase.as<uint64_t>(), qcount.as<uint32_t>(), queryNum,
-                            int64_t(pairs.size() / 8), dev, stream),
-           "pair digest");
+  if (runIndexed) {
+    GATH_TRY(gat_runs_digest(pairs.as<int32_t>(), int64_t(pairs.size() / 8), runs.as<gat_range_run>(), runTotal,
+                             queryNum, dev, stream),
+             "run digest");
+  } else {
+    GATH_TRY(gat_pairs_digest(pairs.as<int32_t>(), qbase.as<uint64_t>(), qcount.as<uint32_t>(), queryNum,
+                              int64_t(pairs.size() / 8), dev, stream),
+             "pair digest");
+  }
   if (out) {
     GATH_TRY(gat_memcpy_d2h(out, dev, size_t(queryNum) * 8, stream), "digest readback");
     GATH_TRY(gat_stream_sync(stream), "digest readback");
@@ -388,6 +465,9 @@ int RangeEngine::reserveBatched(const DeviceTree& tree, int maxQueries, int64_t 
   if (int rc = qlist_.reserve(size_t(qlistCap_) * 32)) return rc;
   if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
   if (int rc = directTasks_.reserve(size_t(lqCap_) * sizeof(gat_range_task))) return rc;
+  if (int rc = out.runs.reserve(size_t(qlistCap_) * sizeof(gat_range_run))) return rc;
+  if (int rc = itemBase_.reserve(size_t(itemsCap_) * 8)) return rc;
+  if (int rc = itemTiles_.reserve(size_t(gat_scan_tiles(itemsCap_)) * 8)) return rc;
   return out.pairs.reserve(size_t(pairsCap_) * 8);
 }
 
@@ -459,6 +539,8 @@ int RangeEngine::runDirect(const DeviceTree& tree, int nq, RangeBatch& out) {
   out.selectMs = selMs;
   out.refineMs = out.kernelMs;
   out.hostStale = false;
+  out.runIndexed = false;
+  out.runTotal = 0;
   out.leafQueries = ntasks;
   out.taskNum = ntasks;
   out.candidates = cand;
@@ -547,10 +629,20 @@ int RangeEngine::enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out) {
   if (timeRefine_) {
     if (int rc = refineTimer_.start(s)) return rc;
   }
-  GATH_TRY(gat_leaf_refine(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.get(), qlistCap_, nq,
-                           out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
-                           reinterpret_cast<uint64_t*>(ctr + 8), s),
-           "leaf refine");
+  if (runIndexed) {
+    GATH_TRY(gat_leaf_refine_runs(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.get(), qlistCap_, nq,
+                                  out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
+                                  out.runs.as<gat_range_run>(), int64_t(out.runs.size() / sizeof(gat_range_run)),
+                                  itemBase_.as<uint64_t>(), itemTiles_.as<uint64_t>(),
+                                  reinterpret_cast<uint64_t*>(ctr + 10), reinterpret_cast<uint64_t*>(ctr + 8), s),
+             "leaf refine");
+  } else {
+    GATH_TRY(gat_leaf_refine(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.get(), qlistCap_, nq,
+                             out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
+                             reinterpret_cast<uint64_t*>(ctr + 8), s),
+             "leaf refine");
+  }
+  out.runIndexed = runIndexed;
   if (timeRefine_) {
     if (int rc = refineTimer_.stop(s)) return rc;
   }
@@ -588,7 +680,7 @@ bool RangeEngine::countersFit() const {
   const uint32_t* ch = countersHost_.as<uint32_t>();
   auto u64 = [&](int w) { return uint64_t(ch[w]) | (uint64_t(ch[w + 1]) << 32); };
   return u64(0) <= uint64_t(lqCap_) && int64_t(ch[5]) <= qlistCap_ && int64_t(ch[6]) <= itemsCap_ &&
-         u64(2) <= uint64_t(pairsCap_);
+         u64(2) <= uint64_t(pairsCap_) && u64(10) <= uint64_t(pairsCap_);
 }
 
 int RangeEngine::finishCounters(RangeBatch& out, bool replay) {
@@ -603,6 +695,8 @@ int RangeEngine::finishCounters(RangeBatch& out, bool replay) {
   out.candidates = int64_t(u64(2));
   out.workItems = int64_t(ch[6]);
   out.pairTotal = int64_t(u64(8));
+  out.runTotal = out.runIndexed ? int64_t(ch[5]) : 0;
+  out.pairSpan = out.runIndexed ? int64_t(u64(10)) : 0;
   return GAT_OK;
 }
 
@@ -645,6 +739,8 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   out.hostStale = false;
   out.stream = stream_.get();
   out.candidates = out.taskNum = out.pairTotal = out.leafQueries = out.workItems = 0;
+  out.runTotal = out.pairSpan = 0;
+  out.runIndexed = false;
   out.replayed = false;
   out.selectMs = out.uploadMs = 0;
   out.kernelMs = out.refineMs = 0;
@@ -668,6 +764,7 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
     out.uploadMs = now_ms() - t0;
     return rc;
   }
+  out.runIndexed = runIndexed;
   const size_t tileBytes = size_t(std::max(2 * gat_scan_tiles(N) * 4, gat_scan_tiles(nq) * 8));
   if (nodeCount_.size() < size_t(N) * 4 || tiles_.size() < tileBytes || cand_.size() < size_t(nq) * 8 ||
       out.qbase.size() < size_t(nq) * 8 || out.qcount.size() < size_t(nq) * 4)
@@ -693,6 +790,8 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
     key.ptr[3] = out.qbase.get();
     key.ptr[4] = out.qcount.get();
     key.ptr[5] = lq_.get();
+    key.ptr[6] = out.runs.get();
+    key.runs = runIndexed;
     key.nq = nq;
     key.lqCap = lqCap_;
     key.qlistCap = qlistCap_;
@@ -744,10 +843,14 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   };
   grow(int64_t(ch[5]), qlistCap_);
   grow(int64_t(ch[6]), itemsCap_);
-  grow(int64_t(u64(2)), pairsCap_);
+  // Run-indexed regions round each work item up to a line: 15 pairs each at most.
+  grow(int64_t(u64(2)) + (runIndexed ? 16 * int64_t(ch[6]) : 0), pairsCap_);
   if (int rc = qlist_.reserve(size_t(qlistCap_) * 32)) return rc;
   if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
   if (int rc = out.pairs.reserve(size_t(pairsCap_) * 8)) return rc;
+  if (int rc = out.runs.reserve(size_t(qlistCap_) * sizeof(gat_range_run))) return rc;
+  if (int rc = itemBase_.reserve(size_t(itemsCap_) * 8)) return rc;
+  if (int rc = itemTiles_.reserve(size_t(gat_scan_tiles(itemsCap_)) * 8)) return rc;
   if (int rc = timer_.start(stream_.get())) return rc;
   timeRefine_ = true;
   const int rc_join = enqueueJoin(tree, nq, out);

@@ -57,6 +57,13 @@ struct RangeBatch {
   float refineMs = 0;           // batched form, step-by-step run: the leaf-major refine kernel alone
   DeviceBuffer pairs;           // int32 (tid, pid) pairs
   DeviceBuffer qbase, qcount;   // device copies of base / count
+  // Run-indexed layout (leaf-major form with runIndexed): the pairs are
+  // compacted over the batch in work-item order, query q's set being the union
+  // of its runs (gat_range_run: query, count, first pair); base is unused.
+  bool runIndexed = false;
+  DeviceBuffer runs;
+  int64_t runTotal = 0;   // run records (one per list entry; count 0 when no hits)
+  int64_t pairSpan = 0;   // run-indexed: extent of `pairs` the regions cover
   bool hostStale = false;       // base / count not yet copied from the device
   void* stream = nullptr;       // the engine stream that produced the batch (readbacks wait on it only)
   // Copies base / count from the device when they are stale.
@@ -64,6 +71,11 @@ struct RangeBatch {
   // Copies the pairs to the host, grouped by query (query q's pairs at
   // out[2*base[q] ...], count[q] of them).
   int fetch(int32_t* out, int64_t capacityPairs);
+  // Copies the pairs to the host query after query, back to back (query q's
+  // after those of queries 0..q-1); capacityPairs must hold pairTotal.
+  int fetchCompact(int32_t* out, int64_t capacityPairs);
+  // Run-indexed layout: the run records (runTotal of them).
+  int fetchRuns(gat_range_run* out, int64_t capacity);
   // Per-query order-independent digest (wrapping sum of a 64-bit mix of each
   // (tid, pid) pair), computed on the device.
   int digests(uint64_t* out) const;
@@ -132,6 +144,11 @@ class RangeEngine {
   // selected (leaf, rectangle) becomes a task of the per-task refine kernel,
   // and nothing sized by the tree is cleared or scanned.
   int directMaxQueries = 256;
+  // Leaf-major form: lay the pairs out by work item with a run index
+  // (gat_leaf_refine_runs) instead of in per-query regions.  Each work item's
+  // pairs go to one contiguous chunk, so far fewer lines are shared between
+  // waves; per-query counts, digests and fetches are the same either way.
+  bool runIndexed = true;
   // After a batched run: the points of the leaves at least one rectangle
   // selected (what the leaf-major refine must read at least once).
   int uniqueCandidates(const DeviceTree& tree, int64_t* points);
@@ -180,22 +197,24 @@ class RangeEngine {
 
   // Batched-form workspace.  counters_ (uint32 words): [0,1] selected entries,
   // [2,3] candidate total, [4] status, [5] list total, [6] work items,
-  // [8,9] result pairs.
+  // [8,9] result pairs, [10,11] run-indexed region total.
   DeviceBuffer nodeCount_, nodeFill_, nodeOff_, tiles_, cand_, lq_, qlist_, items_, counters_, directTasks_;
+  DeviceBuffer itemBase_, itemTiles_;  // run-indexed refine: item regions and their scan scratch
   PinnedBuffer countersHost_;
   int64_t lqCap_ = 0, qlistCap_ = 0, itemsCap_ = 0, pairsCap_ = 0;
   EventTimer selTimer_, refineTimer_;
   bool timeRefine_ = false;
   void* graph_ = nullptr;
   struct GraphKey {
-    const void* ptr[6] = {};  // tree nodes, boxes, pairs, qbase, qcount, lq
+    const void* ptr[7] = {};  // tree nodes, boxes, pairs, qbase, qcount, lq, runs
     int64_t nq = -1, lqCap = 0, qlistCap = 0, itemsCap = 0, pairsCap = 0;
     int qb = 0;
+    bool runs = false;
     bool operator==(const GraphKey& o) const {
-      for (int i = 0; i < 6; ++i)
+      for (int i = 0; i < 7; ++i)
         if (ptr[i] != o.ptr[i]) return false;
       return nq == o.nq && lqCap == o.lqCap && qlistCap == o.qlistCap && itemsCap == o.itemsCap &&
-             pairsCap == o.pairsCap && qb == o.qb;
+             pairsCap == o.pairsCap && qb == o.qb && runs == o.runs;
     }
   } graphKey_;
 };

@@ -389,22 +389,24 @@ int gath_index_last_counts(void* p, int64_t* counts) {
 int gath_index_last_fetch(void* p, int32_t* pairs, int64_t cap) {
   Index* ix = as_index(p);
   if (!ix || (cap > 0 && !pairs)) return bad("gath_index_last_fetch: bad arguments");
-  if (int rc = ix->last.syncHost()) return rc;
+  return ix->last.fetchCompact(pairs, cap);
+}
+
+int64_t gath_index_last_runs(void* p, gat_range_run* out, int64_t cap) {
+  Index* ix = as_index(p);
+  if (!ix || (cap > 0 && !out)) return bad("gath_index_last_runs: bad arguments");
   const RangeBatch& b = ix->last;
-  if (b.pairTotal > cap) {
-    gath_set_error("gath_index_last_fetch: %lld pairs, room for %lld", (long long)b.pairTotal, (long long)cap);
-    return GAT_ERR_CAPACITY;
-  }
-  int64_t at = 0;
-  for (int q = 0; q < b.queryNum; ++q) {
-    if (b.count[q]) {
-      GATH_TRY(gat_memcpy_d2h(pairs + 2 * at, b.pairs.as<int32_t>() + 2 * b.base[q], size_t(b.count[q]) * 8,
-                              b.stream),
-               "pair readback");
-    }
-    at += b.count[q];
-  }
-  GATH_TRY(gat_stream_sync(b.stream), "pair readback");
+  if (!b.runIndexed) return 0;
+  if (cap >= b.runTotal && b.runTotal > 0)
+    if (int rc = ix->last.fetchRuns(out, cap)) return rc;
+  return b.runTotal;
+}
+
+int gath_index_set_result_layout(void* p, int run_indexed) {
+  Index* ix = as_index(p);
+  if (!ix) return bad("gath_index_set_result_layout: null index");
+  RangeEngine& e = ix->grid ? ix->grid->engine() : ix->fsg ? ix->fsg->engine() : ix->stig->engine();
+  e.runIndexed = run_indexed != 0;
   return GAT_OK;
 }
 

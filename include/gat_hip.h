@@ -117,8 +117,9 @@ int gat_cell_store_build(const gat_cell_grid* grid, const float* lon_dev, const 
  * and 3 write runs of `run` (1..4096) 8-byte pairs packed back to back, one
  * run per wave step in scattered order, as result runs from many waves land
  * in per-query regions: 2 with 64-lane stores from each run's first pair, 3
- * through 64-lane windows aligned to 128-byte lines.  No reference
- * counterpart.  Asynchronous on `stream`. */
+ * through 64-lane windows aligned to 128-byte lines (streaming stores); 4 and
+ * 5 the same with cached stores.  No reference counterpart.  Asynchronous on
+ * `stream`. */
 int gat_hbm_pattern(int kind, void* buf_dev, size_t bytes, uint32_t run, void* stream);
 
 /* ------------------------------------------------------------------ range */
@@ -366,6 +367,38 @@ int gat_leaf_scatter(const uint32_t* leaf_query_dev, const uint64_t* lq_total_de
                      const uint32_t* node_qcount_dev, uint32_t* node_fill_dev, int32_t qb, const float* boxes_dev,
                      const uint64_t* qbase_dev, int64_t nqueries, void* qlist_dev, int64_t qlist_capacity,
                      uint32_t* items_dev, int64_t items_capacity, void* stream);
+
+/* A run of result pairs of the run-indexed refine: `count` pairs of query q
+ * at pairs[first ...]. */
+typedef struct gat_range_run {
+  uint32_t q;
+  uint32_t count;
+  uint64_t first;
+} gat_range_run;
+
+/* The leaf-major refine with the output laid out by work item (no reference
+ * counterpart): work item t owns the region [item_base[t], item_base[t+1]) of
+ * `pairs`, an exclusive scan of its bound (rectangles x leaf points, rounded
+ * up to 16 pairs) written to item_base_dev (items_capacity entries; tiles_dev
+ * holds gat_scan_tiles(items_capacity) scratch entries; *bound_total_dev
+ * receives the sum, which must not exceed `capacity`).  An item's hits fill its
+ * region from the start, rectangle after rectangle; its list entry e records
+ * the rectangle's run (query, hits, first pair) in runs_dev[e] (count 0 when
+ * none; runs_capacity must cover the list total); qcount[q] counts query q's
+ * pairs (must start at zero).  No atomic is shared between work items.
+ * pair_total as gat_leaf_refine. */
+int gat_leaf_refine_runs(const gat_point_store* store, const uint32_t* items_dev, const uint32_t* item_total_dev,
+                         int64_t items_capacity, const void* qlist_dev, int64_t qlist_capacity, int64_t nqueries,
+                         uint32_t* qcount_dev, int32_t* pairs_dev, int64_t capacity, gat_range_run* runs_dev,
+                         int64_t runs_capacity, uint64_t* item_base_dev, uint64_t* tiles_dev,
+                         uint64_t* bound_total_dev, uint64_t* pair_total_dev, void* stream);
+
+/* Per-query digests of run-indexed pairs (the same value gat_pairs_digest gives
+ * for query-major regions): out_dev[q] = wrapping sum of the pair mix over
+ * every pair of query q's runs (runs of count 0 contribute nothing).  Clears
+ * out_dev first.  Asynchronous. */
+int gat_runs_digest(const int32_t* pairs_dev, int64_t npairs, const gat_range_run* runs_dev, int64_t nruns,
+                    int64_t nqueries, uint64_t* out_dev, void* stream);
 
 /* The leaf-major refine: per work item the leaf's points and ids are loaded
  * once and tested against each of the item's rectangles; hits go to query q's

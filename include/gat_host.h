@@ -128,6 +128,14 @@ int gath_index_last_counts(void* index, int64_t* counts);
 /* Pairs of the last result, query by query (query q's pairs follow those of
  * queries 0..q-1); cap in pairs. */
 int gath_index_last_fetch(void* index, int32_t* pairs, int64_t cap);
+/* Run-indexed batches (leaf-major form, the default): the run records of the
+ * last batch (query, count, first pair into its compacted pairs); returns
+ * their number (0 for other layouts) and copies them when cap holds them all. */
+int64_t gath_index_last_runs(void* index, gat_range_run* out, int64_t cap);
+/* Leaf-major result layout of an index's batched form: 1 (default) pairs by
+ * work item with a run index, 0 per-query regions.  Counts, digests and
+ * gath_index_last_fetch are the same either way. */
+int gath_index_set_result_layout(void* index, int run_indexed);
 int gath_index_last_digests(void* index, uint64_t* digests);
 
 /* Similarity on a Grid. */

@@ -154,3 +154,73 @@ def test_autotuned_batches_give_the_same_answers(cfg1):
     tuned = g.tune_batched(rects, reps=1)
     assert tuned["queries_per_item"] in (4, 8, 16, 32, 64)
     _check_digests(g.range_batched(rects), *og.range(rects, "digest"))
+
+
+def _tiles(runs, total):
+    """The run records cover the batch's pairs once each: runs with hits are
+    disjoint, start on their work item's line-aligned region and hold
+    pairs_total pairs in all."""
+    r = np.sort(runs[runs["count"] > 0], order="first")
+    ends = r["first"].astype(np.int64) + r["count"].astype(np.int64)
+    return int(r["count"].sum()) == total and (len(r) < 2 or (r["first"][1:].astype(np.int64) >= ends[:-1]).all())
+
+
+@pytest.mark.parametrize("kind", ["grid", "stig", "fsg"])
+def test_result_layouts_agree(cfg1, kind):
+    """Run-indexed and per-query layouts of the leaf-major form: the same sets,
+    counts and digests; the run index tiles the compacted pairs and sums to the
+    per-query counts."""
+    db, rects = cfg1
+    edge = _edge_rects(db, W.CFG1.grid_cell, 60, 31)
+    batch = np.concatenate([rects, edge])
+    make = {"grid": lambda: api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf),
+            "stig": lambda: api.StigIndex(W.CFG1.leaf),
+            "fsg": lambda: api.FsgIndex(W.UNIT_BOX, W.CFG1.fsg_cell)}[kind]
+    ix = _form(make().build(db), "leaf")
+    per = ix.set_result_layout(False).range_batched(batch)
+    pc, pd = per.counts(), per.digests()
+    assert len(per.runs()) == 0
+    res = ix.set_result_layout(True).range_batched(batch)
+    assert (res.counts() == pc).all() and (res.digests() == pd).all()
+    runs = res.runs()
+    assert len(runs) > 0 and _tiles(runs, res.pairs_total)
+    assert (np.bincount(runs["q"], weights=runs["count"], minlength=len(batch)).astype(np.int64) == pc).all()
+    want = _sets(O.brute_range(db, rects)) if kind != "grid" else \
+        _sets(O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db).range(rects))
+    got = res.per_query()
+    for q in range(len(rects)):
+        assert set(map(tuple, got[q].tolist())) == want[q], q
+
+
+def test_runs_over_multichunk_leaves(gpu):
+    """Leaves of thousands of points (whole trajectories inside one cell of the
+    deepest level): each (item, rectangle) still makes one run, its hits
+    counted over every chunk first."""
+    lengths = np.full(300, 2048, np.int32)
+    db = api.synth_walks(3, lengths, step_sigma=1e-7)
+    rects = np.concatenate([np.array([[0, 0, 1, 1], [0.25, 0.25, 0.75, 0.75], [0.1, 0.1, 0.6, 0.3]], np.float32),
+                            W.CFG1.rects()[:400]])
+    og = O.Grid(W.UNIT_BOX, 1 / 16, 256).add(db)
+    g = _form(api.GridIndex(W.UNIT_BOX, 1 / 16, 256).build(db), "leaf")
+    nodes, _ = g.tree_nodes()
+    leaves = nodes[nodes["first_child"] < 0]
+    assert leaves["count"].max() > 1024  # several chunks per leaf
+    res = g.set_result_layout(True).range_batched(rects)
+    _check_pairs(res, _sets(og.range(rects)), rects)
+    runs = res.runs()
+    assert _tiles(runs, res.pairs_total)
+    assert len(runs) == res.leaf_queries  # one record per (leaf, rectangle) entry
+    counts, dig = og.range(rects, "digest")
+    assert (res.digests() == dig).all()
+
+
+def test_run_indexed_graph_replay(cfg1):
+    db, rects = cfg1
+    og = O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db)
+    g = _form(api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db), "leaf").set_result_layout(True)
+    want = og.range(rects, "digest")
+    for i in range(4):
+        res = g.range_batched(rects, graph=True)
+        assert res.replayed == (i > 0)
+        assert (res.counts() == want[0]).all() and (res.digests() == want[1]).all()
+        assert _tiles(res.runs(), res.pairs_total)

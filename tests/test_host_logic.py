@@ -321,6 +321,8 @@ int main(void) {
   F(gath_outcome, digest); printf("gath_outcome %zu\n", sizeof(gath_outcome));
   F(gat_tree_node, box); F(gat_tree_node, first_child); F(gat_tree_node, start); F(gat_tree_node, count);
   F(gat_tree_node, flags); printf("gat_tree_node %zu\n", sizeof(gat_tree_node));
+  F(gat_range_run, q); F(gat_range_run, count); F(gat_range_run, first);
+  printf("gat_range_run %zu\n", sizeof(gat_range_run));
   return 0;
 }
 ''')
@@ -329,7 +331,8 @@ int main(void) {
     subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
     got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
                                                                text=True).stdout.split("\n") if line)
-    for name, dt in (("gath_request", api.REQUEST), ("gath_outcome", api.OUTCOME), ("gat_tree_node", api.TREE_NODE)):
+    for name, dt in (("gath_request", api.REQUEST), ("gath_outcome", api.OUTCOME), ("gat_tree_node", api.TREE_NODE),
+                     ("gat_range_run", api.RUN)):
         assert int(got[name]) == dt.itemsize, name
         for field, (_, off) in dt.fields.items():
             assert int(got[f"{name}.{field}"]) == off, (name, field)
