@@ -92,10 +92,10 @@ def main() -> int:
     ap.add_argument("--mixed-ntraj", type=int, default=None,
                     help="trajectories of the mixed workload (default: the full 10M, about 0.88B points)")
     ap.add_argument("--mixed-nq", type=int, default=None, help="rectangles of the mixed workload (default 100k)")
-    ap.add_argument("--only", choices=["all", "range", "batched", "edr", "serve", "mixed", "copy", "hbm"],
+    ap.add_argument("--only", choices=["all", "range", "batched", "mgrid", "edr", "serve", "mixed", "copy", "hbm"],
                     default="all",
                     help="range / batched: just the timed GT-quadtree batches of the plain / batched form (for "
-                         "profiling); edr: just the EDR section; copy: just the stream-copy calibration kernel; "
+                         "profiling); mgrid: the batched form over MortonGrid's implicit tree; edr: just the EDR section; copy: just the stream-copy calibration kernel; "
                          "hbm: write / read / packed-run access-pattern ceilings")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -133,7 +133,8 @@ def main() -> int:
     log(f"workload {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points, {len(rects):,} rectangles "
         f"({time.perf_counter() - t:.1f} s)")
     t = time.perf_counter()
-    grid = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db, device=True)
+    kind = G.MortonGridIndex if args.only == "mgrid" else G.GridIndex
+    grid = kind(W.UNIT_BOX, w.grid_cell, w.leaf).build(db, device=True)
     build_device_s = time.perf_counter() - t
     log(f"GT-quadtree built on the device and resident ({build_device_s:.2f} s): {grid.info().tolist()}")
 
@@ -143,11 +144,14 @@ def main() -> int:
         print(json.dumps({"value": round(qps, 1), "kernel_ms": round(res.kernel_ms, 4),
                           "achieved_gbs": round(alg / (res.kernel_ms * 1e-3) / 1e9, 1), "algorithmic_bytes": alg}))
         return 0
-    if args.only == "batched":
-        res, qps, total, _ = time_batched(grid, rects, args.steps, args.warmup, "GT-quadtree batched")
-        refine_ms = float(np.median([grid.range_batched(rects, graph=False).refine_ms for _ in range(3)]))
+    if args.only in ("batched", "mgrid"):
+        label = "MortonGrid batched" if args.only == "mgrid" else "GT-quadtree batched"
+        res, qps, total, _ = time_batched(grid, rects, args.steps, args.warmup, label)
+        runs = [grid.range_batched(rects, graph=False) for _ in range(3)]
+        refine_ms = float(np.median([x.refine_ms for x in runs]))
+        select_ms = float(np.median([x.select_ms for x in runs]))
         print(json.dumps({"value": round(qps, 1), "pipeline_ms": round(res.kernel_ms, 4),
-                          "refine_kernel_ms": round(refine_ms, 4)}))
+                          "refine_kernel_ms": round(refine_ms, 4), "selection_ms": round(select_ms, 4)}))
         return 0
 
     tuned_range = grid.tune_batched(rects)
@@ -189,6 +193,17 @@ def main() -> int:
     _, sbqps, _, _ = time_batched(stig, rects, few, 1, "STIG batched")
     _, fbqps, _, _ = time_batched(fsg, rects, few, 1, "FSG batched")
     del fsg, stig
+    # MortonGrid: the same quadtree kept implicitly, walked on the device
+    # without node records; its answers must be the GT-quadtree's.
+    mgrid = G.MortonGridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db, device=True)
+    mgrid.set_batch_params(tuned_range["queries_per_item"], 256)
+    _, mqps, _, _ = time_range(mgrid, rects, few, 1, "MortonGrid plain")
+    mres, mbqps, _, _ = time_batched(mgrid, rects, args.steps, args.warmup, "MortonGrid batched")
+    msel = float(np.median([mgrid.range_batched(rects, graph=False).select_ms for _ in range(3)]))
+    msame = bool((mres.counts() == bres.counts()).all() and (mres.digests() == bres.digests()).all())
+    log(f"MortonGrid selection {msel:.3f} ms (GT-quadtree {r.select_ms:.3f} ms); answers equal the GT-quadtree's: "
+        f"{msame}")
+    del mgrid
 
     edr = bench_edr(G, W)
     del grid
@@ -266,6 +281,10 @@ def main() -> int:
             "stig_batched_qps": round(sbqps, 1),
             "stig_plain_qps": round(sqps, 1),
             "fsg_batched_qps": round(fbqps, 1),
+            "mgrid_batched_qps": round(mbqps, 1),
+            "mgrid_plain_qps": round(mqps, 1),
+            "mgrid_selection_ms": round(msel, 4),
+            "mgrid_equals_grid": msame,
             "fsg_plain_qps": round(fqps, 1),
             "range_batched_p50_ms": round(1e3 * float(np.percentile(bwall, 50)), 3),
             "range_batched_p99_ms": round(1e3 * float(np.percentile(bwall, 99)), 3),

@@ -85,6 +85,7 @@ HIP_FUNCS = {
     "gat_tree_select": (INT, [P, I64, I32, P, P, I64, I64, P, P, P, I64, P, P, P]),
     "gat_leaf_tasks": (INT, [P, P, I64, P, I64, P, P]),
     "gat_grid_select": (INT, [P, I32, I32, F32, F32, F32, P, P, I64, I64, P, P, P, I64, P, P]),
+    "gat_morton_select": (INT, [P, I32, I64, F32, F32, F32, P, P, I64, I64, P, P, P, I64, P, P, P]),
     "gat_leaf_scatter": (INT, [P, P, I64, P, I64, P, P, P, I32, P, P, I64, P, I64, P, I64, P]),
     "gat_leaf_refine": (INT, [P, P, P, I64, P, I64, I64, P, P, I64, P, P]),
     "gat_leaf_refine_runs": (INT, [P, P, P, I64, P, I64, I64, P, P, I64, P, I64, P, P, P, P, P]),
@@ -107,6 +108,8 @@ HOST_FUNCS = {
     "gath_synth_rects": (None, [U64, I64, F64, F64, INT, F64, INT, F64, P]),
     "gath_synth_queries": (None, [U64, INT, P, P, P, INT, P, P, INT, F64, F64, P, P]),
     "gath_grid_new": (P, [P, F32, INT]),
+    "gath_morton_grid_new": (P, [P, F32, INT]),
+    "gath_system_range_test": (INT, [P, INT, P, INT, P]),
     "gath_fsg_new": (P, [P, F32]),
     "gath_stig_new": (P, [INT]),
     "gath_index_free": (None, [P]),

@@ -386,6 +386,17 @@ class _Index:
         N.check_host(N.host().gath_index_set_result_layout(self.handle, int(run_indexed)))
         return self
 
+    def system_range_test(self, which: str, boxes: np.ndarray) -> tuple[int, int]:
+        """The reference's SystemTest range entry point `which` ("grid",
+        "stig", "fsg", "mgrid": rangeQueryTest, STIGrangeQueryTest,
+        FSGrangeQueryTest, MortonGridRangeQueryTest) on this index: (candidate
+        points scanned, result pairs)."""
+        kinds = {"grid": 0, "stig": 1, "fsg": 2, "mgrid": 3}
+        b = _req(boxes, np.float32).reshape(-1, 4)
+        out = np.zeros(2, np.int64)
+        N.check_host(N.host().gath_system_range_test(self.handle, kinds[which], _ptr(b), len(b), _ptr(out)))
+        return int(out[0]), int(out[1])
+
     def set_batch_params(self, queries_per_item: int = 32, direct_max_queries: int = 256) -> "_Index":
         """Batched form: rectangles per leaf-major work item (1..64), and the
         largest batch that takes the direct form (per-task refine, nothing
@@ -576,6 +587,16 @@ class MixedRange:
                                              _ptr(route), _ptr(st)))
         keys = ["pairs", "candidates", "groups", "replays", "ms", "digest_ms"]
         return counts[:n], digests[:n], route[:n], dict(zip(keys, st.tolist()))
+
+
+class MortonGridIndex(GridIndex):
+    """MortonGrid (MortonGrid.cpp): the GT-quadtree kept implicitly — leaf
+    flags per (level, id), node boxes from their cells; the batched form walks
+    it on the device without node records.  Same answers as GridIndex."""
+
+    def __init__(self, range_box=(0.0, 0.0, 1.0, 1.0), cell_size: float = 0.01, max_points_in_node: int = 256):
+        r = np.asarray(range_box, np.float32)
+        self.handle = N.host().gath_morton_grid_new(_ptr(r), cell_size, max_points_in_node)
 
 
 class FsgIndex(_Index):

@@ -29,7 +29,7 @@ ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 HIPCC = str(ROCM / "bin" / "hipcc")
 
 HIP_SOURCES = ["runtime.hip", "util.hip", "range.hip", "range_batched.hip", "edr.hip", "edr_batched.hip", "build.hip"]
-HOST_SOURCES = ["MBB.cpp", "Synth.cpp", "DeviceArena.cpp", "RangeEngine.cpp", "Grid.cpp", "GridSimilarity.cpp", "FSG.cpp",
+HOST_SOURCES = ["MBB.cpp", "Synth.cpp", "DeviceArena.cpp", "RangeEngine.cpp", "Grid.cpp", "GridSimilarity.cpp", "MortonGrid.cpp", "FSG.cpp",
                 "STIG.cpp", "SystemTest.cpp", "Server.cpp", "Autotune.cpp", "MixedRange.cpp", "TrajFile.cpp",
                 "host_capi.cpp"]
 

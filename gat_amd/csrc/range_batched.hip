@@ -268,6 +268,136 @@ __global__ __launch_bounds__(kWave) void grid_select_kernel(
   }
 }
 
+// ------------------------------------------------- implicit quadtree selection
+// MortonGrid (MortonGrid.cpp:8-13, 238-268, 355-393): the same quadtree over
+// Morton ranges as the GT-quadtree, kept without node records.  Node (level l,
+// id) covers cells [id * 4^(L-l), (id+1) * 4^(L-l)) and its box runs from its
+// first cell's corner to its last cell's (generateMBBfromNode), computed here
+// from the cells' Morton-decoded columns and rows with the float formula of
+// the cell boxes (MortonGrid.cpp:41-46).  Each cell c has a word pair
+// cells[c] = (code, store start of c), and cells[4^L] = (-, total points).
+// code = (rank << 4) | level for the leaf whose first cell is c (rank: its
+// position among the leaves in Morton order), and a value naming no level
+// where no leaf starts; a node at level l with first cell c is a leaf exactly
+// when code names level l (leaves are aligned Morton blocks, so the leaf
+// holding a node's first cell starts there).  A leaf's points are the store
+// starts of its first cell and of the cell after it, two loads issued
+// together.  Selected leaves are emitted by rank, so the leaf-major kernels
+// read their store runs from the leaf records leaves[rank].
+//
+// Stack entries are (first cell << 4) | level of selected inner nodes; the
+// traversal otherwise follows tree_select_kernel: the root is taken as it is,
+// every other node when BoxIntersect(rectangle, box) != 0, 64 entries a step.
+__device__ __forceinline__ uint32_t morton_compact(uint32_t v) {
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+  v = (v | (v >> 4)) & 0x00FF00FFu;
+  v = (v | (v >> 8)) & 0x0000FFFFu;
+  return v;
+}
+
+struct CellGeometry {
+  float xmin, ymax, cs;
+  // Box of the Morton block [first, first + span): the first cell's xmin and
+  // ymax, the last cell's xmax and ymin (rows count down from ymax).
+  __device__ __forceinline__ float4 block_box(uint32_t first, uint32_t span) const {
+    const uint32_t last = first + span - 1u;
+    const uint32_t c0 = morton_compact(first), r0 = morton_compact(first >> 1);
+    const uint32_t c1 = morton_compact(last), r1 = morton_compact(last >> 1);
+    return make_float4(__fadd_rn(xmin, __fmul_rn(cs, float(c0))), __fsub_rn(ymax, __fmul_rn(cs, float(r1 + 1u))),
+                       __fadd_rn(xmin, __fmul_rn(cs, float(c1 + 1u))), __fsub_rn(ymax, __fmul_rn(cs, float(r0))));
+  }
+};
+
+constexpr uint32_t kMaxImplicitLevel = 14;  // first cell < 2^28 fits a stack entry
+
+__global__ __launch_bounds__(kWave) void morton_select_kernel(
+    const uint2* __restrict__ cells, uint32_t total_level, uint32_t nleaves,
+    CellGeometry geo, const float4* __restrict__ boxes, const uint32_t* __restrict__ order, uint32_t q0, uint32_t nq,
+    uint32_t* __restrict__ node_qcount, uint64_t* __restrict__ cand, uint2* __restrict__ lq, uint64_t lq_cap,
+    unsigned long long* __restrict__ lq_total, uint32_t* __restrict__ status) {
+  __shared__ uint32_t stack[GAT_TREE_STACK];
+  __shared__ uint32_t emitted[kEmitCap];
+  constexpr int NCH = 4;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t L = total_level;
+  for (uint32_t i = blockIdx.x; i < nq; i += gridDim.x) {
+    const uint32_t q = order ? uniform(order[i]) : q0 + i;
+    if (q - q0 >= nq) continue;  // an order entry outside the batch (uniform)
+    const float4 box = boxes[q];
+    LeafEmitter out{emitted, 0, q, lane, lq, lq_cap, lq_total, node_qcount, 0};
+    uint32_t top = 0;
+    {
+      const uint2 w = cells[0];
+      if ((w.x & 15u) == 0u) {  // the root is a leaf: taken untested
+        const uint32_t rank = w.x >> 4;
+        const uint32_t cnt = rank < nleaves ? cells[1u << (2 * L)].y - w.y : 0u;
+        out.push(lane == 0 && cnt > 0, rank, cnt);
+      } else {
+        if (lane == 0) stack[0] = 0u;  // first cell 0, level 0
+        top = 1;
+      }
+    }
+    __syncthreads();
+    while (top > 0) {
+      uint32_t m = top < kWave ? top : kWave;
+      const uint32_t room = (GAT_TREE_STACK - top) / (NCH - 1);
+      if (m > room) m = room;
+      if (m == 0) {
+        if (lane == 0) atomicOr(status, GAT_STATUS_STACK);
+        break;
+      }
+      top -= m;
+      const bool live = lane < m;
+      const uint32_t e = live ? stack[top + lane] : 0u;
+      __syncthreads();  // every lane has read its entry before pushes reuse the slots
+      const uint32_t first = e >> 4, cl = (e & 15u) + 1u;  // children's level
+      const bool deeper = live && cl <= L;
+      const uint32_t shift = deeper ? 2u * (L - cl) : 0u;
+      uint32_t grand[NCH];
+      bool inner[NCH];
+      uint32_t below = 0, pushed = 0;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const uint32_t fk = first + (uint32_t(c) << shift);
+        bool emit = false;
+        uint32_t cnt = 0, rank = 0;
+        inner[c] = false;
+        grand[c] = 0;
+        if (deeper && box_intersect(box, geo.block_box(fk, 1u << shift))) {
+          const uint2 w = cells[fk];
+          const uint32_t end = cells[fk + (1u << shift)].y;
+          if ((w.x & 15u) == cl) {
+            rank = w.x >> 4;
+            if (rank < nleaves) {
+              cnt = end - w.y;
+              emit = cnt > 0;
+            }
+          } else if (cl < L) {
+            inner[c] = true;
+            grand[c] = (fk << 4) | cl;
+          }
+        }
+        out.push(emit, rank, cnt);
+        out.maybe_flush();
+        const uint64_t dm = __ballot(inner[c]);
+        below += lanes_below(dm);
+        pushed += uint32_t(__popcll(dm));
+      }
+      uint32_t at = top + below;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        if (inner[c]) stack[at++] = grand[c];
+      top += pushed;
+      __syncthreads();
+    }
+    out.flush();
+    const uint64_t total = out.wave_cand();
+    if (lane == 0) cand[q] = total;
+  }
+}
+
 // ---------------------------------------------------------------- scans
 // Reduce-then-scan over tiles of 4096 elements: per-tile sums, a carried
 // exclusive scan of the tile sums in one workgroup, then a per-tile scan.
@@ -1050,6 +1180,27 @@ int gat_tree_select(const gat_tree_node* nodes, int64_t nnodes, int32_t nchild, 
     tree_select_kernel<2><<<grid, block, 0, s>>>(N, uint32_t(nnodes), B, order, uint32_t(q0), uint32_t(nq),
                                                  node_qcount, cand, L, uint64_t(lq_capacity), T, status);
   GAT_LAUNCH_CHECK("tree_select_kernel");
+  return GAT_OK;
+}
+
+int gat_morton_select(const uint32_t* cell_words, int32_t total_level, int64_t nleaves, float xmin, float ymax,
+                      float cell_size, const float* boxes, const uint32_t* order, int64_t q0, int64_t nq,
+                      uint32_t* node_qcount, uint64_t* cand, uint32_t* leaf_query, int64_t lq_capacity,
+                      uint64_t* lq_total, uint32_t* status, void* stream) {
+  GAT_REQUIRE(total_level >= 1 && uint32_t(total_level) <= kMaxImplicitLevel && cell_words && nleaves >= 1 &&
+                  nleaves <= (int64_t(1) << (2 * total_level)) && cell_size > 0.0f,
+              "gat_morton_select: bad implicit tree");
+  GAT_REQUIRE(q0 >= 0 && nq >= 0 && q0 + nq <= int64_t(UINT32_MAX), "gat_morton_select: bad query range");
+  GAT_REQUIRE(lq_capacity >= 0 && lq_total && status, "gat_morton_select: bad outputs");
+  if (nq == 0) return GAT_OK;
+  GAT_REQUIRE(boxes && cand && (lq_capacity == 0 || leaf_query), "gat_morton_select: null pointer");
+  const int blocks = grid_for(uint64_t(nq), 1, 40);
+  morton_select_kernel<<<dim3{uint32_t(blocks)}, dim3{kWave}, 0, gat::as_stream(stream)>>>(
+      reinterpret_cast<const uint2*>(cell_words), uint32_t(total_level), uint32_t(nleaves),
+      CellGeometry{xmin, ymax, cell_size}, reinterpret_cast<const float4*>(boxes), order, uint32_t(q0), uint32_t(nq),
+      node_qcount, cand, reinterpret_cast<uint2*>(leaf_query), uint64_t(lq_capacity),
+      reinterpret_cast<unsigned long long*>(lq_total), status);
+  GAT_LAUNCH_CHECK("morton_select_kernel");
   return GAT_OK;
 }
 

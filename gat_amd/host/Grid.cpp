@@ -160,12 +160,7 @@ int Grid::addDatasetToGrid(const TrajectoryView& db, bool upload) {
   cell.clear();
   cell.shrink_to_fit();
 
-  if (!upload) {
-    nodes.clear();
-    nodeNum = 1;
-    buildNode(0, 0);
-    return GAT_OK;
-  }
+  if (!upload) return buildHostTree();
   if (int rc = engine_.upload(xy.data(), tp.data(), N)) return rc;
   xy = std::vector<float>();
   tp = std::vector<int32_t>();
@@ -174,10 +169,21 @@ int Grid::addDatasetToGrid(const TrajectoryView& db, bool upload) {
 
 // The quadtree over the cell offsets, its device copy and the trajectory
 // store, once the point store is resident.
-int Grid::finishBuild(const TrajectoryView& db) {
+int Grid::buildHostTree() {
   nodes.clear();
   nodeNum = 1;
   buildNode(0, 0);
+  return GAT_OK;
+}
+
+int64_t Grid::leafNum() const {
+  int64_t n = 0;
+  for (const QuadtreeNode& q : nodes) n += q.isLeaf;
+  return n;
+}
+
+int Grid::finishBuild(const TrajectoryView& db) {
+  if (int rc = buildHostTree()) return rc;
   int depth = 0;
   const std::vector<gat_tree_node> tn = treeNodes(&depth);
   if (int rc = tree_.upload(tn, 4, depth, engine_.stream())) return rc;
@@ -273,16 +279,17 @@ void Grid::findMatchNodeInQuadTree(const MBB& bound, std::vector<int>* leaves) c
   }
 }
 
+void Grid::addLeafRuns(const MBB& bound, uint32_t q, TaskList& list) const {
+  std::vector<int> leaves;
+  findMatchNodeInQuadTree(bound, &leaves);
+  for (int li : leaves) list.addRun(uint64_t(nodes[li].pointStart), uint64_t(nodes[li].numPoints), q);
+}
+
 void Grid::selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
                             std::vector<uint64_t>& candidates) const {
   RangeEngine::select(
-      rangeNum,
-      [&](int q, TaskList& list) {
-        std::vector<int> leaves;
-        findMatchNodeInQuadTree(bounds[q], &leaves);
-        for (int li : leaves) list.addRun(uint64_t(nodes[li].pointStart), uint64_t(nodes[li].numPoints), uint32_t(q));
-      },
-      threads(), tasks, candidates);
+      rangeNum, [&](int q, TaskList& list) { addLeafRuns(bounds[q], uint32_t(q), list); }, threads(), tasks,
+      candidates);
 }
 
 int Grid::rangeQueryBatchGPU(const MBB* bounds, int rangeNum, CPURangeQueryResult* ResultTable, int* resultSetSize,
@@ -297,11 +304,7 @@ int Grid::rangeQueryBatchGPU(const MBB* bounds, int rangeNum, CPURangeQueryResul
   }
   const int stride = trajNum + 1;
   std::vector<uint8_t> bitmap(size_t(rangeNum) * size_t(stride));
-  auto selector = [&](int q, TaskList& list) {
-    std::vector<int> leaves;
-    findMatchNodeInQuadTree(bounds[q], &leaves);
-    for (int li : leaves) list.addRun(uint64_t(nodes[li].pointStart), uint64_t(nodes[li].numPoints), uint32_t(q));
-  };
+  auto selector = [&](int q, TaskList& list) { addLeafRuns(bounds[q], uint32_t(q), list); };
   if (int rc = engine_.runBitmap(bounds, rangeNum, selector, threads(), stride, bitmap.data(), false)) return rc;
   for (int q = 0; q < rangeNum; ++q) {
     ResultTable[q].assign(size_t(stride), false);
@@ -322,11 +325,7 @@ int Grid::rangeQueryBatchGPUPairs(const MBB* bounds, int rangeNum, RangeBatch& o
     gath_set_error("Grid::rangeQueryBatchGPUPairs: bad arguments");
     return GAT_ERR_ARG;
   }
-  auto selector = [&](int q, TaskList& list) {
-    std::vector<int> leaves;
-    findMatchNodeInQuadTree(bounds[q], &leaves);
-    for (int li : leaves) list.addRun(uint64_t(nodes[li].pointStart), uint64_t(nodes[li].numPoints), uint32_t(q));
-  };
+  auto selector = [&](int q, TaskList& list) { addLeafRuns(bounds[q], uint32_t(q), list); };
   const int rc = engine_.runPairs(bounds, rangeNum, selector, threads(), out);
   lastSelectMs = out.selectMs;
   lastKernelMs = out.kernelMs;

@@ -50,6 +50,7 @@ class Grid {
   Grid(const MBB& mbb, float val_cell_size, int VITURAL_CELL_PARAM, int maxPointsInNode = MAXPOINTINNODE_DEFAULT);
   Grid(const Grid&) = delete;
   Grid& operator=(const Grid&) = delete;
+  virtual ~Grid() = default;
 
   // Grid.cpp:210-316: builds the cell-ordered store and the quadtree and makes
   // both resident on the device (with upload=false only the host structures
@@ -76,7 +77,9 @@ class Grid {
   int uniqueCandidates(int64_t* points) { return engine_.uniqueCandidates(tree_, points); }
   // The quadtree as breadth-first gat_tree_nodes (the root untested, every
   // other node tested with BoxIntersect != 0); *depth = deepest level.
-  std::vector<gat_tree_node> treeNodes(int* depth) const;
+  virtual std::vector<gat_tree_node> treeNodes(int* depth) const;
+  // Number of quadtree leaves.
+  virtual int64_t leafNum() const;
   // Grid.cpp:428-446: the leaves one rectangle selects, in traversal order.
   void findMatchNodeInQuadTree(const MBB& bound, std::vector<int>* leaves) const;
   // Host candidate selection for a batch (tasks + per-query candidate counts).
@@ -135,16 +138,23 @@ class Grid {
   double lastSelectMs = 0;
   float lastKernelMs = 0;
 
- private:
-  int buildNode(int level, int id);
-  int finishBuild(const TrajectoryView& db);
+ protected:
+  // The host tree over cellStart, then (finishBuild) its device form and the
+  // trajectory store.
+  virtual int buildHostTree();
+  virtual int finishBuild(const TrajectoryView& db);
+  // Appends the store runs of the leaves `bound` selects, as query q.
+  virtual void addLeafRuns(const MBB& bound, uint32_t q, TaskList& list) const;
   int threads() const;
   int uploadTrajectories(const TrajectoryView& db);
-  int ensureQueryStore(const TrajectoryView& queries);
-  int runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* stats);
 
   RangeEngine engine_;
   DeviceTree tree_;
+
+ private:
+  int buildNode(int level, int id);
+  int ensureQueryStore(const TrajectoryView& queries);
+  int runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* stats);
   // Trajectory-ordered copy of the points for EDR (the reference reassembles
   // trajectories from the cell-ordered store per query, Grid.cpp:1500-1508).
   // Stores are NaN-padded (GAT_EDR_PAD points before every trajectory,

@@ -201,6 +201,12 @@ int RangeEngine::runPairs(const MBB* bounds, int nq, const Selector& selector, i
   out.queryNum = nq;
   out.hostStale = false;
   out.stream = stream_.get();
+  // Per-query regions: nothing of an earlier batched run's layout applies.
+  out.runIndexed = false;
+  out.runTotal = out.pairSpan = 0;
+  out.replayed = false;
+  out.leafQueries = out.workItems = 0;
+  out.refineMs = 0;
   double t0 = now_ms();
   std::vector<gat_range_task> tasks;
   std::vector<uint64_t> cand;
@@ -430,6 +436,35 @@ int DeviceTree::uploadGrid(const std::vector<gat_tree_node>& cells, int gx, int 
   return GAT_OK;
 }
 
+int DeviceTree::uploadImplicit(const std::vector<gat_tree_node>& leaves, const std::vector<uint32_t>& words,
+                               int level, float x0, float y1, float cs, void* stream) {
+  if (level < 1 || level > 14 || leaves.empty() || words.size() != 2 * ((size_t(1) << (2 * level)) + 1) ||
+      2 * leaves.size() > words.size() || !(cs > 0.0f)) {
+    gath_set_error("implicit quadtree: level %d with %zu leaves and %zu cell words", level, leaves.size(),
+                   words.size());
+    return GAT_ERR_ARG;
+  }
+  if (int64_t(level + 1) * 4 * 64 > GAT_TREE_STACK) {
+    gath_set_error("implicit quadtree: depth %d exceeds the traversal stack", level);
+    return GAT_ERR_ARG;
+  }
+  if (int rc = nodes.reserve(leaves.size() * sizeof(gat_tree_node))) return rc;
+  if (int rc = cellWords.reserve(words.size() * sizeof(uint32_t))) return rc;
+  GATH_TRY(gat_memcpy_h2d(nodes.get(), leaves.data(), leaves.size() * sizeof(gat_tree_node), stream),
+           "implicit tree upload");
+  GATH_TRY(gat_memcpy_h2d(cellWords.get(), words.data(), words.size() * sizeof(uint32_t), stream),
+           "implicit tree upload");
+  GATH_TRY(gat_stream_sync(stream), "implicit tree upload");
+  nnodes = int64_t(leaves.size());
+  nchild = 4;
+  depth = level;
+  implicitLevel = level;
+  xmin = x0;
+  ymax = y1;
+  cellSize = cs;
+  return GAT_OK;
+}
+
 RangeEngine::~RangeEngine() { dropGraph(); }
 
 int RangeEngine::attachStore(const RangeEngine& src) {
@@ -479,23 +514,11 @@ int RangeEngine::runDirect(const DeviceTree& tree, int nq, RangeBatch& out) {
   uint32_t* ctr = counters_.as<uint32_t>();
   const uint32_t* ch = countersHost_.as<uint32_t>();
   auto u64 = [&](int w) { return uint64_t(ch[w]) | (uint64_t(ch[w + 1]) << 32); };
-  const auto* order =
-      reinterpret_cast<const uint32_t*>(devBoxes_.as<char>() + align_up(size_t(nq) * 16, 256));
   for (int attempt = 0;; ++attempt) {
     if (int rc = selTimer_.start(s)) return rc;
     GATH_TRY(gat_fill_u32(out.qcount.as<uint32_t>(), 0, nq, s), "batch reset");
     GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
-    if (tree.flat()) {
-      GATH_TRY(gat_grid_select(tree.nodes.as<gat_tree_node>(), tree.nx, tree.ny, tree.xmin, tree.ymax,
-                               tree.cellSize, devBoxes_.as<float>(), order, 0, nq, nullptr, cand_.as<uint64_t>(),
-                               lq_.as<uint32_t>(), lqCap_, reinterpret_cast<uint64_t*>(ctr), s),
-               "grid select");
-    } else {
-      GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), N, tree.nchild, devBoxes_.as<float>(), order, 0, nq,
-                               nullptr, cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_,
-                               reinterpret_cast<uint64_t*>(ctr), ctr + 4, s),
-               "tree select");
-    }
+    if (int rc = enqueueTreeSelect(tree, nq, nullptr)) return rc;
     GATH_TRY(gat_scan_u64(cand_.as<uint64_t>(), nq, out.qbase.as<uint64_t>(), reinterpret_cast<uint64_t*>(ctr + 2),
                           tiles_.as<uint64_t>(), s),
              "region scan");
@@ -586,6 +609,30 @@ int RangeEngine::uploadBoxes(const MBB* bounds, int nq) {
   return GAT_OK;
 }
 
+int RangeEngine::enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nodeCount) {
+  void* s = stream_.get();
+  uint32_t* ctr = counters_.as<uint32_t>();
+  auto* total = reinterpret_cast<uint64_t*>(ctr);
+  const auto* order =
+      reinterpret_cast<const uint32_t*>(devBoxes_.as<char>() + align_up(size_t(nq) * 16, 256));
+  if (tree.flat()) {
+    GATH_TRY(gat_grid_select(tree.nodes.as<gat_tree_node>(), tree.nx, tree.ny, tree.xmin, tree.ymax, tree.cellSize,
+                             devBoxes_.as<float>(), order, 0, nq, nodeCount, cand_.as<uint64_t>(),
+                             lq_.as<uint32_t>(), lqCap_, total, s),
+             "grid select");
+  } else if (tree.implicit()) {
+    GATH_TRY(gat_morton_select(tree.cellWords.as<uint32_t>(), tree.implicitLevel, tree.nnodes, tree.xmin, tree.ymax,
+                               tree.cellSize, devBoxes_.as<float>(), order, 0, nq, nodeCount, cand_.as<uint64_t>(),
+                               lq_.as<uint32_t>(), lqCap_, total, ctr + 4, s),
+             "implicit tree select");
+  } else {
+    GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), tree.nnodes, tree.nchild, devBoxes_.as<float>(), order,
+                             0, nq, nodeCount, cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_, total, ctr + 4, s),
+             "tree select");
+  }
+  return GAT_OK;
+}
+
 int RangeEngine::enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out) {
   void* s = stream_.get();
   const int64_t N = tree.nnodes;
@@ -594,19 +641,7 @@ int RangeEngine::enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out) 
   GATH_TRY(gat_fill_u32(nodeFill_.as<uint32_t>(), 0, N, s), "batch reset");
   GATH_TRY(gat_fill_u32(out.qcount.as<uint32_t>(), 0, nq, s), "batch reset");
   GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
-  const auto* order =
-      reinterpret_cast<const uint32_t*>(devBoxes_.as<char>() + align_up(size_t(nq) * 16, 256));
-  if (tree.flat()) {
-    GATH_TRY(gat_grid_select(tree.nodes.as<gat_tree_node>(), tree.nx, tree.ny, tree.xmin, tree.ymax, tree.cellSize,
-                             devBoxes_.as<float>(), order, 0, nq, nodeCount_.as<uint32_t>(), cand_.as<uint64_t>(),
-                             lq_.as<uint32_t>(), lqCap_, reinterpret_cast<uint64_t*>(ctr), s),
-             "grid select");
-  } else {
-    GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), N, tree.nchild, devBoxes_.as<float>(), order, 0, nq,
-                             nodeCount_.as<uint32_t>(), cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_,
-                             reinterpret_cast<uint64_t*>(ctr), ctr + 4, s),
-             "tree select");
-  }
+  if (int rc = enqueueTreeSelect(tree, nq, nodeCount_.as<uint32_t>())) return rc;
   GATH_TRY(gat_scan_u64(cand_.as<uint64_t>(), nq, out.qbase.as<uint64_t>(), reinterpret_cast<uint64_t*>(ctr + 2),
                         tiles_.as<uint64_t>(), s),
            "region scan");
@@ -791,6 +826,7 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
     key.ptr[4] = out.qcount.get();
     key.ptr[5] = lq_.get();
     key.ptr[6] = out.runs.get();
+    key.ptr[7] = tree.cellWords.get();
     key.runs = runIndexed;
     key.nq = nq;
     key.lqCap = lqCap_;

@@ -88,7 +88,10 @@ struct RangeBatch {
 // breadth-first order, nchild children per inner node, leaves at depth <= depth.
 // A flat grid (nx, ny > 0) is the same with no inner nodes: nodes[] holds one
 // leaf record per cell, row-major from (xmin, ymax), and selection enumerates
-// a rectangle's cell range instead of walking a tree.
+// a rectangle's cell range instead of walking a tree.  An implicit quadtree
+// (implicitLevel L > 0, gat_morton_select) keeps no inner nodes either:
+// nodes[] holds the leaves in Morton order and cellWords a (leaf code, store
+// start) pair per cell of the 2^L x 2^L Morton grid, plus one past the end.
 struct DeviceTree {
   DeviceBuffer nodes;
   int64_t nnodes = 0;
@@ -96,12 +99,19 @@ struct DeviceTree {
   int depth = 0;
   int nx = 0, ny = 0;  // flat grid only
   float xmin = 0, ymax = 0, cellSize = 0;
+  DeviceBuffer cellWords;  // implicit quadtree only
+  int implicitLevel = 0;
   bool flat() const { return nx > 0; }
+  bool implicit() const { return implicitLevel > 0; }
   // Checks the traversal stack bound and uploads `host`.
   int upload(const std::vector<gat_tree_node>& host, int nchild, int depth, void* stream);
   // Uploads one leaf record per cell of an nx x ny grid.
   int uploadGrid(const std::vector<gat_tree_node>& cells, int nx, int ny, float xmin, float ymax, float cellSize,
                  void* stream);
+  // Uploads an implicit quadtree: the leaf records in Morton order and the
+  // 4^level + 1 (code, store start) cell pairs (gat_morton_select).
+  int uploadImplicit(const std::vector<gat_tree_node>& leaves, const std::vector<uint32_t>& words, int level,
+                     float xmin, float ymax, float cellSize, void* stream);
 };
 
 class RangeEngine {
@@ -180,6 +190,8 @@ class RangeEngine {
   int stage(const MBB* bounds, int nq, const std::vector<gat_range_task>& tasks, const uint64_t* qbase);
   int uploadBoxes(const MBB* bounds, int nq);
   int enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out);
+  // The selection kernel for the tree's kind (nodeCount may be null).
+  int enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nodeCount);
   int enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out);
   int readCounters(int nq, RangeBatch& out);
   bool countersFit() const;
@@ -206,12 +218,12 @@ class RangeEngine {
   bool timeRefine_ = false;
   void* graph_ = nullptr;
   struct GraphKey {
-    const void* ptr[7] = {};  // tree nodes, boxes, pairs, qbase, qcount, lq, runs
+    const void* ptr[8] = {};  // tree nodes, boxes, pairs, qbase, qcount, lq, runs, cell words
     int64_t nq = -1, lqCap = 0, qlistCap = 0, itemsCap = 0, pairsCap = 0;
     int qb = 0;
     bool runs = false;
     bool operator==(const GraphKey& o) const {
-      for (int i = 0; i < 7; ++i)
+      for (int i = 0; i < 8; ++i)
         if (ptr[i] != o.ptr[i]) return false;
       return nq == o.nq && lqCap == o.lqCap && qlistCap == o.qlistCap && itemsCap == o.itemsCap &&
              pairsCap == o.pairsCap && qb == o.qb && runs == o.runs;

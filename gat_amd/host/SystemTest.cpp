@@ -61,6 +61,19 @@ int SystemTest::FSGrangeQueryTest(const MBB* bounds, int rangeQueryNum) {
   return report("FSG", rangeQueryNum, b, now_ms() - t0);
 }
 
+int SystemTest::MortonGridRangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum) {
+  std::vector<MBB> bounds(size_t(std::max(rangeQueryNum, 0)), rangeQueryMBB);
+  return MortonGridRangeQueryTest(bounds.data(), rangeQueryNum);
+}
+
+int SystemTest::MortonGridRangeQueryTest(const MBB* bounds, int rangeQueryNum) {
+  if (!mgrid) return GAT_ERR_STATE;
+  RangeBatch b;
+  const double t0 = now_ms();
+  if (int rc = mgrid->rangeQueryBatchGPUPairs(bounds, rangeQueryNum, b)) return rc;
+  return report("MortonGrid", rangeQueryNum, b, now_ms() - t0);
+}
+
 int SystemTest::similarityQueryTest(Trajectory t, int similarityScale, int similarityKValue) {
   if (!g) return GAT_ERR_STATE;
   std::vector<Trajectory> q(size_t(std::max(similarityScale, 0)), t);

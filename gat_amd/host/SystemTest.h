@@ -12,14 +12,17 @@
 
 #include "FSG.h"
 #include "Grid.h"
+#include "MortonGrid.h"
 #include "STIG.h"
 #include "Trajectory.h"
 
 class SystemTest {
  public:
   SystemTest() = default;
-  // The reference also takes a MortonGrid, which is outside this path.
   SystemTest(Trajectory* tradb, Grid* g, STIG* stig, FSG* fsg) : tradb(tradb), g(g), stig(stig), fsg(fsg) {}
+  // SystemTest.cpp:15-22.
+  SystemTest(Trajectory* tradb, Grid* g, STIG* stig, FSG* fsg, MortonGrid* mgrid)
+      : tradb(tradb), g(g), stig(stig), fsg(fsg), mgrid(mgrid) {}
 
   int rangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum);
   int rangeQueryTest(const MBB* bounds, int rangeQueryNum);
@@ -27,6 +30,9 @@ class SystemTest {
   int STIGrangeQueryTest(const MBB* bounds, int rangeQueryNum);
   int FSGrangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum);
   int FSGrangeQueryTest(const MBB* bounds, int rangeQueryNum);
+  // SystemTest.cpp:386-445 (the reference issues 50 rectangles per call).
+  int MortonGridRangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum);
+  int MortonGridRangeQueryTest(const MBB* bounds, int rangeQueryNum);
   int similarityQueryTest(Trajectory t, int similarityScale, int similarityKValue);
 
   struct Result {
@@ -43,6 +49,7 @@ class SystemTest {
   Grid* g = nullptr;
   STIG* stig = nullptr;
   FSG* fsg = nullptr;
+  MortonGrid* mgrid = nullptr;
 
  private:
   int report(const char* what, int n, const RangeBatch& b, double ms);

@@ -8,9 +8,11 @@
 #include "Grid.h"
 #include "Autotune.h"
 #include "MixedRange.h"
+#include "MortonGrid.h"
 #include "STIG.h"
 #include "TrajFile.h"
 #include "Server.h"
+#include "SystemTest.h"
 #include "Synth.h"
 #include "gat_host.h"
 
@@ -77,6 +79,13 @@ void* gath_grid_new(const float* r, float cell_size, int max_points_in_node) {
   return ix;
 }
 
+void* gath_morton_grid_new(const float* r, float cell_size, int max_points_in_node) {
+  auto* ix = new (std::nothrow) Index{Index::kGrid, nullptr, nullptr, nullptr, {}};
+  if (!ix) return nullptr;
+  ix->grid.reset(new MortonGrid(MBB(r[0], r[1], r[2], r[3]), cell_size, 2, max_points_in_node));
+  return ix;
+}
+
 void* gath_fsg_new(const float* r, float cell_size) {
   auto* ix = new (std::nothrow) Index{Index::kFsg, nullptr, nullptr, nullptr, {}};
   if (!ix) return nullptr;
@@ -125,10 +134,8 @@ int gath_index_info(void* p, int64_t* out) {
   for (int i = 0; i < 8; ++i) out[i] = 0;
   if (ix->grid) {
     Grid& g = *ix->grid;
-    int64_t leaves = 0;
-    for (auto& n : g.nodes) leaves += n.isLeaf;
     out[0] = g.totalPointNum; out[1] = g.trajNum; out[2] = g.cellnum; out[3] = g.cellNum_axis;
-    out[4] = int64_t(g.nodes.size()); out[5] = leaves; out[6] = g.totalLevel;
+    out[4] = int64_t(g.nodes.size()); out[5] = g.leafNum(); out[6] = g.totalLevel;
   } else if (ix->fsg) {
     FSG& f = *ix->fsg;
     out[0] = f.totalPointNum; out[1] = f.trajNum; out[2] = f.cellnum; out[3] = f.cell_num_x; out[6] = f.cell_num_y;
@@ -247,6 +254,42 @@ int gath_index_set_batch_params(void* p, int queries_per_item, int direct_max_qu
   RangeEngine& e = ix->grid ? ix->grid->engine() : ix->fsg ? ix->fsg->engine() : ix->stig->engine();
   e.queriesPerItem = queries_per_item;
   e.directMaxQueries = direct_max_queries;
+  return GAT_OK;
+}
+
+int gath_system_range_test(void* p, int which, const float* boxes, int nq, int64_t* out) {
+  Index* ix = as_index(p);
+  if (!ix || nq < 0 || (nq > 0 && !boxes) || !out) return bad("gath_system_range_test: bad arguments");
+  SystemTest t;
+  t.verbose = false;
+  int rc;
+  const MBB* b = as_boxes(boxes);
+  switch (which) {
+    case 0:
+      if (!ix->grid) return bad("gath_system_range_test: rangeQueryTest needs a Grid");
+      t.g = ix->grid.get();
+      rc = t.rangeQueryTest(b, nq);
+      break;
+    case 1:
+      if (!ix->stig) return bad("gath_system_range_test: STIGrangeQueryTest needs a STIG");
+      t.stig = ix->stig.get();
+      rc = t.STIGrangeQueryTest(b, nq);
+      break;
+    case 2:
+      if (!ix->fsg) return bad("gath_system_range_test: FSGrangeQueryTest needs an FSG");
+      t.fsg = ix->fsg.get();
+      rc = t.FSGrangeQueryTest(b, nq);
+      break;
+    case 3:
+      t.mgrid = dynamic_cast<MortonGrid*>(ix->grid.get());
+      if (!t.mgrid) return bad("gath_system_range_test: MortonGridRangeQueryTest needs a MortonGrid");
+      rc = t.MortonGridRangeQueryTest(b, nq);
+      break;
+    default: return bad("gath_system_range_test: which must be 0..3");
+  }
+  if (rc) return rc;
+  out[0] = t.last.candidates;
+  out[1] = t.last.pairs;
   return GAT_OK;
 }
 

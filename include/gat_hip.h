@@ -347,6 +347,26 @@ int gat_grid_select(const gat_tree_node* cells_dev, int32_t nx, int32_t ny, floa
                     uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, int64_t lq_capacity,
                     uint64_t* lq_total_dev, void* stream);
 
+/* The same selection over an implicit quadtree (MortonGrid, MortonGrid.cpp:8-13,
+ * 238-268, 355-393, which replaces the reference's findMatchNodeInQuadTree
+ * walk over its isLeaf bitset; no reference GPU counterpart): the GT-quadtree
+ * of a 2^L x 2^L Morton grid (L = total_level, 1..14) kept without node
+ * records.  Node (level l, id) covers cells [id * 4^(L-l), (id+1) * 4^(L-l))
+ * and its box is computed from its first and last cells (cell (row i, col j)
+ * spans [xmin + cs*j, xmin + cs*(j+1)] x [ymax - cs*(i+1), ymax - cs*i]).
+ * cell_words_dev holds 4^L + 1 pairs of uint32: for cell c, (code, store
+ * start of c), where code = (rank << 4) | level at the first cell of each leaf
+ * (rank: the leaf's position among the leaves in Morton order) and a value
+ * whose low 4 bits name no level elsewhere (0xFFFFFFFF); the last pair's
+ * second word is the total point count.  The root is taken as it is, every
+ * other node when BoxIntersect(rectangle, box) != 0.  Outputs as
+ * gat_tree_select with leaf = rank (node_qcount has nleaves entries); the
+ * leaf-major kernels then read leaf records in rank order. */
+int gat_morton_select(const uint32_t* cell_words_dev, int32_t total_level, int64_t nleaves, float xmin, float ymax,
+                      float cell_size, const float* boxes_dev, const uint32_t* order_dev, int64_t q0, int64_t nq,
+                      uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, int64_t lq_capacity,
+                      uint64_t* lq_total_dev, uint32_t* status_dev, void* stream);
+
 /* The direct form for small batches: entry i of the (leaf, query) list becomes
  * tasks_dev[i] = {leaf start, leaf count, query} for gat_range_refine_pairs, so
  * no per-node array is touched.  tasks_dev holds lq_capacity tasks. */

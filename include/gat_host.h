@@ -70,6 +70,9 @@ float gath_traj_vehicle_distance(float lat_a, float lon_a, float lat_b, float lo
 
 /* Indexes.  range = (xmin, ymin, xmax, ymax). */
 void* gath_grid_new(const float* range, float cell_size, int max_points_in_node);
+/* MortonGrid (MortonGrid.h:27-86): the GT-quadtree kept implicitly; same
+ * answers as gath_grid_new's index, at most 2^14 cells per axis. */
+void* gath_morton_grid_new(const float* range, float cell_size, int max_points_in_node);
 void* gath_fsg_new(const float* range, float cell_size);
 void* gath_stig_new(int block_size);
 void gath_index_free(void* index);
@@ -114,8 +117,15 @@ int gath_grid_set_edr_params(void* grid, int split, int per_cu_short, int per_cu
  * and the largest batch that takes the direct form (one refine task per
  * selected leaf and rectangle, nothing sized by the tree touched). */
 int gath_index_set_batch_params(void* index, int queries_per_item, int direct_max_queries);
+/* The reference's SystemTest range entry points (SystemTest.cpp:24-102,
+ * 251-384, 386-445) over the rectangles `boxes`: which = 0 rangeQueryTest
+ * (a Grid or MortonGrid index), 1 STIGrangeQueryTest, 2 FSGrangeQueryTest,
+ * 3 MortonGridRangeQueryTest.  out[0] = candidate points scanned, out[1] =
+ * result pairs. */
+int gath_system_range_test(void* index, int which, const float* boxes, int nq, int64_t* out);
 /* The index's tree as the batched form uploads it (breadth-first
- * gat_tree_node); returns the node count, copies up to cap nodes. */
+ * gat_tree_node; a MortonGrid's implicit tree gives its leaves in Morton
+ * order); returns the node count, copies up to cap nodes. */
 int64_t gath_index_tree_nodes(void* index, gat_tree_node* out, int64_t cap, int32_t* depth);
 /* The resident point store: meta[0] = points, meta[1] = pid_bits (0: point ids
  * in `pid`); with xy non-null, copies xy (2 floats per point), id and, when

@@ -36,6 +36,15 @@ _FUNCS = {
     "orc_grid_store": (INT, [P, P, P, P, P]),
     "orc_grid_leaves": (I64, [P, P, P, I64]),
     "orc_grid_range": (I64, [P, P, INT, INT, P, I64, P, P, P, I64]),
+    "orc_mgrid_new": (P, [P, F32, INT]),
+    "orc_mgrid_free": (None, [P]),
+    "orc_mgrid_add": (INT, [P, P, INT, P, P]),
+    "orc_mgrid_bitmap_index": (INT, [INT, INT]),
+    "orc_mgrid_info": (INT, [P, P]),
+    "orc_mgrid_is_leaf": (INT, [P, INT, INT]),
+    "orc_mgrid_node_box": (INT, [P, INT, INT, P]),
+    "orc_mgrid_leaves": (I64, [P, P, P, I64]),
+    "orc_mgrid_range": (I64, [P, P, INT, INT, P, I64, P, P, P, I64]),
     "orc_fsg_new": (P, [P, F32]),
     "orc_fsg_free": (None, [P]),
     "orc_fsg_add": (INT, [P, P, INT, P, P]),
@@ -176,6 +185,47 @@ class Grid(_RangeIndex):
 
     def range(self, boxes, mode="pairs", nthreads=1):
         return self._range(lib().orc_grid_range, boxes, nthreads, mode, self.ntraj)
+
+
+class MortonGrid(_RangeIndex):
+    """Restatement of MortonGrid (implicit GT-quadtree) — MortonGrid.cpp:8-393."""
+    _free = "orc_mgrid_free"
+
+    def __init__(self, range_box, cell_size, max_points_in_node):
+        self.handle = lib().orc_mgrid_new(_p(_f32(range_box)), cell_size, max_points_in_node)
+
+    @staticmethod
+    def bitmap_index(level, nid):
+        return lib().orc_mgrid_bitmap_index(level, nid)
+
+    def add(self, db):
+        lib().orc_mgrid_add(self.handle, _p(db.offsets), db.ntraj, _p(db.lon), _p(db.lat))
+        self.ntraj = db.ntraj
+        return self
+
+    def info(self):
+        out = np.zeros(5, np.int64)
+        lib().orc_mgrid_info(self.handle, _p(out))
+        return dict(zip(["axis", "cellnum", "total_level", "leaves", "points"], out.tolist()))
+
+    def is_leaf(self, level, nid):
+        return bool(lib().orc_mgrid_is_leaf(self.handle, level, nid))
+
+    def node_box(self, level, nid):
+        out = np.zeros(4, np.float32)
+        lib().orc_mgrid_node_box(self.handle, level, nid, _p(out))
+        return out
+
+    def leaves(self, box):
+        """(level, id, store start, points) of the leaves one rectangle
+        selects, in traversal order."""
+        n = lib().orc_mgrid_leaves(self.handle, _p(_f32(box)), None, 0)
+        out = np.zeros((max(n, 1), 4), np.int64)
+        lib().orc_mgrid_leaves(self.handle, _p(_f32(box)), _p(out), n)
+        return out[:n]
+
+    def range(self, boxes, mode="pairs", nthreads=1):
+        return self._range(lib().orc_mgrid_range, boxes, nthreads, mode, self.ntraj)
 
 
 class FSG(_RangeIndex):

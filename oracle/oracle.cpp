@@ -289,6 +289,96 @@ struct GridIndex {
   }
 };
 
+// MortonGrid — MortonGrid.cpp:8-13 (leaf flag index), 23-49 (geometry),
+// 125-131 (cell of a point), 134-214 (store), 238-268 (leaf flags, node
+// boxes), 270-335 (rangeQueryBatch), 355-393 (findMatchNodeInQuadTree).  The
+// tree is only a flag per (level, id); the reference's flag set is a
+// std::bitset of (4^10 - 1) / 3 entries (MortonGrid.h:17), which a grid of
+// 2^10 cells per axis (leaves at level 10) overruns, so here it is sized by
+// the grid.
+struct MortonIndex {
+  Box range{};
+  float cell_size = 0;
+  int axis = 0, cellnum = 0, total_level = 0, max_pts = 0;
+  std::vector<Box> cell_box;  // cellPtr[c].mbb
+  CellStore store;
+  std::vector<bool> is_leaf;
+
+  static int bitmap_index(int level, int id) {  // findIdxInBitmap
+    if (level <= 0) return 0;
+    return (int(std::pow(4, level) + 0.5) - 1) / 3 + id;
+  }
+
+  void init(const Box& r, float cs, int maxp) {
+    range = r;
+    cell_size = cs;
+    max_pts = maxp;
+    int divx = int((r.xmax - r.xmin) / cs) + 1;
+    int divy = int((r.ymax - r.ymin) / cs) + 1;
+    int mv = std::max(divx, divy);
+    axis = mv >> int(std::log2(mv)) << (int(std::log2(mv)) + 1);
+    cellnum = axis * axis;
+    total_level = int(std::log2(cellnum) / std::log2(4));
+    cell_box.assign(cellnum, Box{});
+    for (int i = 0; i <= axis - 1; i++)
+      for (int j = 0; j <= axis - 1; j++)
+        cell_box[morton(j, i)] = Box{range.xmin + cell_size * j, range.ymax - cell_size * (i + 1),
+                                     range.xmin + cell_size * (j + 1), range.ymax - cell_size * i};
+  }
+
+  int cell_of(float lon, float lat) const {
+    int row = int((range.ymax - lat) / cell_size);
+    int col = int((lon - range.xmin) / cell_size);
+    return morton(col, row);
+  }
+
+  void build_tree(int level, int id) {
+    int span = int(std::pow(4, total_level - level));
+    int64_t pts = 0;
+    for (int i = id * span; i <= (id + 1) * span - 1; i++) pts += store.count[i];
+    if (pts < max_pts || level == total_level) {
+      is_leaf[bitmap_index(level, id)] = true;
+      return;
+    }
+    for (int k = 0; k < 4; ++k) build_tree(level + 1, (id << 2) + k);
+  }
+
+  Box node_box(int level, int id) const {  // generateMBBfromNode
+    int span = int(std::pow(4, total_level - level));
+    const Box& first = cell_box[id * span];
+    const Box& last = cell_box[(id + 1) * span - 1];
+    return Box{first.xmin, last.ymin, last.xmax, first.ymax};
+  }
+
+  void add(const int64_t* offs, int ntraj, const float* lon, const float* lat) {
+    store.build(cellnum, offs, ntraj, lon, lat, [this](float a, float b) { return cell_of(a, b); });
+    is_leaf.assign(size_t(bitmap_index(total_level + 1, 0)), false);
+    build_tree(0, 0);
+  }
+
+  void match(int level, int id, const Box& q, std::vector<std::pair<int, int>>& out) const {
+    if (is_leaf[bitmap_index(level, id)]) {
+      out.push_back({level, id});
+      return;
+    }
+    for (int k = 0; k < 4; ++k)
+      if (box_intersect(q, node_box(level + 1, (id << 2) + k))) match(level + 1, (id << 2) + k, q, out);
+  }
+
+  void query(int qi, const Box& q, Sink& sink) const {
+    std::vector<std::pair<int, int>> nodes;
+    match(0, 0, q, nodes);
+    for (const auto& n : nodes) {
+      int span = int(std::pow(4, total_level - n.first) + 0.5);
+      for (int c = n.second * span; c <= (n.second + 1) * span - 1; c++) {
+        int64_t b = store.start[c], e = b + store.count[c];
+        for (int64_t i = b; i < e; ++i)
+          if (pin_box(q, store.x[i], store.y[i])) sink.hit(qi, store.tid[i], store.pid[i]);
+      }
+    }
+  }
+};
+
 // FSG — FSG.cpp:7-28 (geometry), 88-93 (cell of a point), 95-143 (store),
 // 183-261 (candidate cells of a rectangle).
 struct FsgIndex {
@@ -672,6 +762,71 @@ int64_t orc_grid_leaves(void* gp, const float* box, int64_t* out, int64_t cap) {
 int64_t orc_grid_range(void* g, const float* boxes, int nq, int nthreads, uint8_t* bitmap, int64_t stride,
                        int64_t* counts, uint64_t* digests, int32_t* pairs, int64_t cap) {
   return run_range(*static_cast<GridIndex*>(g), boxes, nq, nthreads, bitmap, stride, counts, digests, pairs, cap);
+}
+
+// ---------------------------------------------------------------- MortonGrid
+void* orc_mgrid_new(const float* range, float cell_size, int max_points_in_node) {
+  auto* m = new MortonIndex();
+  m->init(box_from(range), cell_size, max_points_in_node);
+  return m;
+}
+
+void orc_mgrid_free(void* m) { delete static_cast<MortonIndex*>(m); }
+
+int orc_mgrid_add(void* m, const int64_t* offs, int ntraj, const float* lon, const float* lat) {
+  static_cast<MortonIndex*>(m)->add(offs, ntraj, lon, lat);
+  return 0;
+}
+
+int orc_mgrid_bitmap_index(int level, int id) { return MortonIndex::bitmap_index(level, id); }
+
+// out: axis, cellnum, total_level, leaves, points
+int orc_mgrid_info(void* mp, int64_t* out) {
+  auto* m = static_cast<MortonIndex*>(mp);
+  int64_t leaves = 0;
+  for (bool b : m->is_leaf) leaves += b;
+  out[0] = m->axis;
+  out[1] = m->cellnum;
+  out[2] = m->total_level;
+  out[3] = leaves;
+  out[4] = int64_t(m->store.x.size());
+  return 0;
+}
+
+int orc_mgrid_is_leaf(void* mp, int level, int id) {
+  auto* m = static_cast<MortonIndex*>(mp);
+  return m->is_leaf[MortonIndex::bitmap_index(level, id)] ? 1 : 0;
+}
+
+int orc_mgrid_node_box(void* mp, int level, int id, float* out) {
+  const Box b = static_cast<MortonIndex*>(mp)->node_box(level, id);
+  out[0] = b.xmin; out[1] = b.ymin; out[2] = b.xmax; out[3] = b.ymax;
+  return 0;
+}
+
+// Leaves one rectangle selects, in traversal order, as (level, id, store
+// start, point count).
+int64_t orc_mgrid_leaves(void* mp, const float* box, int64_t* out, int64_t cap) {
+  auto* m = static_cast<MortonIndex*>(mp);
+  std::vector<std::pair<int, int>> nodes;
+  m->match(0, 0, box_from(box), nodes);
+  int64_t n = int64_t(nodes.size());
+  for (int64_t i = 0; i < n && i < cap; ++i) {
+    int span = int(std::pow(4, m->total_level - nodes[i].first) + 0.5);
+    int first = nodes[i].second * span;
+    int64_t pts = 0;
+    for (int c = first; c < first + span; ++c) pts += m->store.count[c];
+    out[4 * i] = nodes[i].first;
+    out[4 * i + 1] = nodes[i].second;
+    out[4 * i + 2] = m->store.start[first];
+    out[4 * i + 3] = pts;
+  }
+  return n;
+}
+
+int64_t orc_mgrid_range(void* m, const float* boxes, int nq, int nthreads, uint8_t* bitmap, int64_t stride,
+                        int64_t* counts, uint64_t* digests, int32_t* pairs, int64_t cap) {
+  return run_range(*static_cast<MortonIndex*>(m), boxes, nq, nthreads, bitmap, stride, counts, digests, pairs, cap);
 }
 
 // ---------------------------------------------------------------- FSG

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from gat_amd import _native as N, api, workloads as W
+from gat_amd import GatError, _native as N, api, workloads as W
 
 from tests.devbuf import DevBuf
 
@@ -166,3 +166,26 @@ def test_refine_abi_skips_malformed_tasks(gpu, packed):
     # Argument errors come back as a status, not a crash.
     assert h.gat_range_refine_pairs(sp, None, 4, db_.ptr, 2, qbase.ptr, qcount.ptr, pairs.ptr, 8, None) == 2
     assert h.gat_range_refine_pairs(None, dt.ptr, 4, db_.ptr, 2, qbase.ptr, qcount.ptr, pairs.ptr, 8, None) == 2
+
+
+def test_system_test_entry_points(cfg1):
+    """The reference's SystemTest range entry points (SystemTest.cpp:24-102,
+    251-384, 386-445) run the plain GPU path of each index: as many result
+    pairs as the oracle finds, over the same candidates as the index's own
+    plain call; a MortonGrid answers rangeQueryTest too (it is a Grid)."""
+    db, rects = cfg1
+    want = sum(len(r) for r in O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db).range(rects))
+    brute = sum(len(r) for r in O.brute_range(db, rects))
+    cases = [("grid", api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf), want),
+             ("mgrid", api.MortonGridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf), want),
+             ("stig", api.StigIndex(W.CFG1.leaf), brute),
+             ("fsg", api.FsgIndex(W.UNIT_BOX, W.CFG1.grid_cell), brute)]
+    for which, ix, pairs in cases:
+        ix.build(db)
+        cand, got = ix.system_range_test(which, rects)
+        assert got == pairs, which
+        assert cand == ix.range_pairs(rects).candidates, which
+    mg = cases[1][1]
+    assert mg.system_range_test("grid", rects)[1] == want
+    with pytest.raises(GatError):
+        cases[0][1].system_range_test("mgrid", rects)

@@ -224,3 +224,81 @@ def test_run_indexed_graph_replay(cfg1):
         assert res.replayed == (i > 0)
         assert (res.counts() == want[0]).all() and (res.digests() == want[1]).all()
         assert _tiles(res.runs(), res.pairs_total)
+
+
+# ------------------------------------------------------------- MortonGrid
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("cell,leaf", [(0.01, 256), (1 / 1024, 8), (1 / 16, 4096)])
+def test_mgrid_batched(cfg1, cell, leaf, form):
+    """MortonGrid's implicit tree walked on the device (gat_morton_select):
+    result sets identical to the oracle's MortonGrid restatement, digests on
+    edge rectangles equal to it and to the plain path, the same candidates as
+    the explicit GT-quadtree."""
+    db, rects = cfg1
+    om = O.MortonGrid(W.UNIT_BOX, cell, leaf).add(db)
+    m = _form(api.MortonGridIndex(W.UNIT_BOX, cell, leaf).build(db), form)
+    res = m.range_batched(rects)
+    _check_pairs(res, _sets(om.range(rects)), rects)
+    edge = _edge_rects(db, cell, 40, 25)
+    res = m.range_batched(edge)
+    counts, dig = om.range(edge, "digest")
+    _check_digests(res, counts, dig)
+    plain = m.range_pairs(edge)
+    _check_digests(plain, counts, dig)
+    assert plain.candidates == res.candidates
+    g = _form(api.GridIndex(W.UNIT_BOX, cell, leaf).build(db), form)
+    gres = g.range_batched(edge)
+    assert gres.candidates == res.candidates and gres.leaf_queries == res.leaf_queries
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_mgrid_root_leaf_and_empty(gpu, form):
+    """A tree that is only its root (fewer points than the leaf capacity): the
+    root is taken untested (MortonGrid.cpp:355-360), so a rectangle that misses
+    every point still scans them all; empty batches and rectangles outside the
+    grid give empty answers."""
+    db = W.CFG1.db(300)
+    om = O.MortonGrid(W.UNIT_BOX, 1 / 16, 1 << 20).add(db)
+    assert om.info()["leaves"] == 1
+    m = _form(api.MortonGridIndex(W.UNIT_BOX, 1 / 16, 1 << 20).build(db), form)
+    rects = np.array([[0.1, 0.1, 0.3, 0.3], [2, 2, 3, 3], [0, 0, 1, 1]], np.float32)
+    res = m.range_batched(rects)
+    _check_pairs(res, _sets(om.range(rects)), rects)
+    assert res.candidates == 3 * db.npoints
+    assert m.range_batched(np.zeros((0, 4), np.float32)).pairs_total == 0
+
+
+def test_mgrid_graph_replay_and_device_build(gpu):
+    """The store built on the device feeds the implicit tree too; captured
+    pipelines replay with the implicit select and agree with the oracle."""
+    db = W.CFG1.db(5000)
+    om = O.MortonGrid(W.UNIT_BOX, 1 / 1024, 32).add(db)
+    m = api.MortonGridIndex(W.UNIT_BOX, 1 / 1024, 32).build(db, device=True).set_batch_params(32, 0)
+    small = api.synth_rects(41, 300, 1e-4, 0.05, True)
+    other = api.synth_rects(42, 300, 1e-4, 0.05, True)
+    seen = []
+    for rects in (small, small, other, small):
+        res = m.range_batched(rects, graph=True)
+        _check_digests(res, *om.range(rects, "digest"))
+        seen.append(res.replayed)
+    assert seen[0] is False and seen[1] is True
+
+
+def test_plain_after_batched_keeps_its_layout(cfg1):
+    """The plain path after a run-indexed batched run on the same index (one
+    result record per index): its per-query regions are read as such, so its
+    digests and fetched pairs are the oracle's, and a batched run after it is
+    run-indexed again."""
+    db, rects = cfg1
+    og = O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db)
+    g = _form(api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db), "leaf")
+    edge = _edge_rects(db, W.CFG1.grid_cell, 30, 27)
+    counts, dig = og.range(edge, "digest")
+    for _ in range(2):
+        res = g.range_batched(edge)
+        assert len(res.runs()) > 0
+        _check_digests(res, counts, dig)
+        _check_digests(g.range_pairs(edge), counts, dig)
+    g.range_batched(rects)
+    _check_pairs(g.range_pairs(rects), _sets(og.range(rects)), rects)

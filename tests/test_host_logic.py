@@ -353,3 +353,130 @@ def test_serve_request_stream_shape():
     assert (side > 0).all() and (side <= 5e-2 * 1.0001).all()
     again, _, _ = W.serve_requests(W.CFG5, db, 0.5, rate=2000)
     assert (again == req).all()
+
+
+# ------------------------------------------------------------- MortonGrid
+
+def _edge_and_random_rects(cell, n_edge=60, seed=5):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, int(1 / cell), size=(n_edge, 2)).astype(np.float64)
+    span = rng.integers(1, 4, size=(n_edge, 2))
+    edge = np.stack([k[:, 0] * cell, 1 - (k[:, 1] + span[:, 1]) * cell, (k[:, 0] + span[:, 0]) * cell,
+                     1 - k[:, 1] * cell], 1).astype(np.float32)
+    return np.concatenate([W.CFG1.rects()[:60], api.synth_rects(seed, 60, 1e-4, 0.5, True), edge,
+                           np.array([[0, 0, 1, 1], [-1, -1, 2, 2], [3, 3, 4, 4]], np.float32)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("cell,leaf", [(0.01, 256), (1 / 16, 8), (0.001, 32), (0.25, 100000)])
+def test_mgrid_selection_matches_oracle(cell, leaf):
+    """The product MortonGrid's host selection (MortonGrid.cpp:355-393) gives
+    the oracle's leaves as store runs, and the same leaves as the product Grid."""
+    db = W.CFG1.db(2000)
+    om = O.MortonGrid(W.UNIT_BOX, cell, leaf).add(db)
+    pm = api.MortonGridIndex(W.UNIT_BOX, cell, leaf).build(db, upload=False)
+    pg = api.GridIndex(W.UNIT_BOX, cell, leaf).build(db, upload=False)
+    info, oi = pm.info(), om.info()
+    assert (info[0], info[5], info[6]) == (oi["points"], oi["leaves"], oi["total_level"])
+    assert info[5] == pg.info()[5]
+    rects = _edge_and_random_rects(cell)
+    tasks, cand = pm.select(rects)
+    gtasks, gcand = pg.select(rects)
+    assert (tasks == gtasks).all() and (cand == gcand).all()
+    for q in range(len(rects)):
+        want = _merged_runs([(s, c) for _, _, s, c in om.leaves(rects[q]).tolist()])
+        assert _task_runs(tasks, q) == want, q
+
+
+def _morton_compact(v):
+    v &= 0x55555555
+    v = (v | (v >> 1)) & 0x33333333
+    v = (v | (v >> 2)) & 0x0F0F0F0F
+    v = (v | (v >> 4)) & 0x00FF00FF
+    return (v | (v >> 8)) & 0x0000FFFF
+
+
+def _morton_wave_select(codes, starts, level, xmin, ymax, cs, box, stack_cap=4096, wave=64):
+    """morton_select_kernel restated: the stack holds (first cell, level) of
+    selected inner nodes; children's boxes come from their first and last
+    cells in float32, a child is a leaf when the code at its first cell names
+    its level, and its points are the store starts of its first cell and of
+    the cell after it."""
+    f32 = np.float32
+    xmin, ymax, cs = f32(xmin), f32(ymax), f32(cs)
+
+    def block_box(first, span):
+        last = first + span - 1
+        c0, r0 = _morton_compact(first), _morton_compact(first >> 1)
+        c1, r1 = _morton_compact(last), _morton_compact(last >> 1)
+        return np.array([xmin + cs * f32(c0), ymax - cs * f32(r1 + 1), xmin + cs * f32(c1 + 1),
+                         ymax - cs * f32(r0)], np.float32)
+
+    sel = []
+    stack = []
+    code = int(codes[0])
+    if code & 15 == 0:
+        cnt = int(starts[1 << (2 * level)] - starts[0])
+        if cnt > 0:
+            sel.append((int(starts[0]), cnt))
+    else:
+        stack.append((0, 0))
+    top_max = len(stack)
+    while stack:
+        m = min(len(stack), wave, (stack_cap - len(stack)) // 3)
+        assert m > 0
+        batch = stack[len(stack) - m:]
+        del stack[len(stack) - m:]
+        pushes = []
+        for first, lv in batch:
+            cl = lv + 1
+            span = 1 << (2 * (level - cl))
+            for c in range(4):
+                fk = first + c * span
+                if O.box_intersect(box, block_box(fk, span)) == 0:
+                    continue
+                code = int(codes[fk])
+                cnt = int(starts[fk + span] - starts[fk])
+                if code & 15 == cl:
+                    if cnt > 0:
+                        sel.append((int(starts[fk]), cnt))
+                elif cl < level:
+                    pushes.append((fk, cl))
+        stack.extend(pushes)
+        top_max = max(top_max, len(stack))
+    return sel, top_max
+
+
+@pytest.mark.parametrize("cell,leaf", [(0.01, 256), (1 / 16, 8), (1 / 1024, 8), (0.25, 100000)])
+def test_mgrid_device_tree_selects_oracle_leaves(cell, leaf):
+    """The implicit tree the batched form uploads — leaf records in Morton
+    order and one word per cell, (rank << 4) | level at each leaf's first
+    cell — walked as morton_select_kernel walks it, selects the oracle's
+    leaves, within the stack bound."""
+    db = W.CFG1.db(1500)
+    om = O.MortonGrid(W.UNIT_BOX, cell, leaf).add(db)
+    pm = api.MortonGridIndex(W.UNIT_BOX, cell, leaf).build(db, upload=False)
+    recs, depth = pm.tree_nodes()
+    L = int(pm.info()[6])
+    assert depth == L and len(recs) == om.info()["leaves"]
+    assert (recs["first_child"] == -1).all()
+    # Leaves in Morton order tile the store.
+    assert recs["start"][0] == 0 and (recs["start"][1:] == recs["start"][:-1] + recs["count"][:-1]).all()
+    assert int(recs["count"].sum()) == db.npoints
+    # The cell codes, rebuilt from the leaf boxes (first cell and width), and
+    # the store start of every cell.
+    starts = pm.cell_starts()
+    assert len(starts) == (1 << (2 * L)) + 1 and starts[-1] == db.npoints
+    codes = np.full(1 << (2 * L), 0xFFFFFFFF, np.uint64)
+    for k, r in enumerate(recs):
+        col = int(round(float(r["box"][0]) / cell))
+        row = int(round((1.0 - float(r["box"][3])) / cell))
+        width = int(round(float(r["box"][2] - r["box"][0]) / cell))
+        lv = L - (width.bit_length() - 1)
+        codes[O.morton(col, row)] = (k << 4) | lv
+        assert starts[O.morton(col, row)] == r["start"]
+    rects = _edge_and_random_rects(cell, 20, 11)
+    for r in rects:
+        sel, top = _morton_wave_select(codes, starts, L, 0.0, 1.0, cell, r)
+        want = [(int(s), int(c)) for _, _, s, c in om.leaves(r).tolist() if c > 0]
+        assert _runs(sel) == _runs(want)
+        assert top <= (L + 1) * 4 * 64

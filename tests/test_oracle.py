@@ -265,3 +265,76 @@ def test_oracle_range_modes_agree():
         assert int(dig[q]) == sum(O.pair_digest(int(t), int(i)) for t, i in p) % (1 << 64)
     # Multi-threaded slicing (Grid.cpp:407-426) gives the same answer.
     assert all((a == b).all() for a, b in zip(g.range(rects, nthreads=5), pairs))
+
+
+# ------------------------------------------------------------- MortonGrid
+
+def test_mgrid_bitmap_index_known_answers():
+    """findIdxInBitmap (MortonGrid.cpp:8-13): levels above hold (4^l - 1) / 3
+    nodes.  Level 10 starts at 349,525, the size of the reference's flag set
+    (MortonGrid.h:17), which a 1024 x 1024 grid's single-cell leaves overrun."""
+    want = {(0, 0): 0, (1, 0): 1, (1, 3): 4, (2, 0): 5, (2, 15): 20, (3, 0): 21, (3, 63): 84, (10, 0): 349525}
+    for (level, nid), idx in want.items():
+        assert O.MortonGrid.bitmap_index(level, nid) == idx
+
+
+def test_mgrid_geometry_known_answers():
+    """Range [0, 4]^2 with unit cells: 5 cells needed per axis, rounded to 8
+    (MortonGrid.cpp:29-33), so 3 levels; node boxes by hand from their first
+    and last cells (generateMBBfromNode, MortonGrid.cpp:263-268), rows counted
+    down from the range's top edge y = 4 (the grid extends below y = 0)."""
+    m = O.MortonGrid((0, 0, 4, 4), 1.0, 2)
+    db = api.TrajSet(np.array([0, 1], np.int64), np.array([0.5], np.float32), np.array([3.5], np.float32))
+    m.add(db)
+    info = m.info()
+    assert (info["axis"], info["cellnum"], info["total_level"]) == (8, 64, 3)
+    # Cell (row i, col j) spans [j, j+1] x [4 - (i+1), 4 - i].
+    assert list(m.node_box(1, 0)) == [0, 0, 4, 4]      # UL: cells 0..15 = rows 0..3, cols 0..3
+    assert list(m.node_box(1, 1)) == [4, 0, 8, 4]      # UR: cols 4..7
+    assert list(m.node_box(1, 2)) == [0, -4, 4, 0]     # DL: rows 4..7
+    assert list(m.node_box(2, 5)) == [6, 2, 8, 4]      # cells 20..23: cols 6..7, rows 0..1
+    assert list(m.node_box(3, 20)) == [6, 3, 7, 4]     # cell 20 = (col 6, row 0)
+
+
+def test_mgrid_leaf_rule_known_answer():
+    """Three points in one cell, maxPointsInNode 2 (MortonGrid.cpp:238-261):
+    the cell's ancestors hold 3 >= 2 points and split down to the cell, a
+    single cell is a leaf at the last level, and every empty sibling is a
+    leaf where it appears: 3 + 3 + 4 leaves."""
+    m = O.MortonGrid((0, 0, 4, 4), 1.0, 2)
+    db = api.TrajSet(np.array([0, 3], np.int64), np.array([0.2, 0.5, 0.7], np.float32),
+                     np.array([3.9, 3.5, 3.1], np.float32))
+    m.add(db)
+    assert m.info()["leaves"] == 10
+    assert not m.is_leaf(0, 0) and not m.is_leaf(1, 0) and not m.is_leaf(2, 0)
+    assert m.is_leaf(3, 0) and m.is_leaf(3, 1) and m.is_leaf(2, 1) and m.is_leaf(1, 3)
+    # A rectangle over the point's cell only: the leaf (3, 0) with its 3 points.
+    got = m.leaves((0.1, 3.1, 0.9, 3.9))
+    assert [tuple(r) for r in got.tolist()] == [(3, 0, 0, 3)]
+    # Touching the cell from outside selects nothing there (BoxIntersect is strict).
+    assert all(tuple(r[:2]) != (3, 0) for r in m.leaves((1.0, 3.1, 1.5, 3.9)).tolist())
+
+
+@pytest.mark.parametrize("cell,leaf", [(1 / 16, 8), (0.01, 64), (1 / 1024, 8), (0.25, 100000)])
+def test_mgrid_matches_grid_restatement(cell, leaf):
+    """MortonGrid and Grid are two restatements of two reference files
+    (MortonGrid.cpp, Grid.cpp) with the same geometry, store and leaf rule, so
+    they select the same leaves in the same order and answer alike; the last
+    shape makes the root a leaf (taken untested by both)."""
+    db = _small_db(7)
+    rects = np.concatenate([api.synth_rects(7, 80, 1e-3, 0.5, True),
+                            np.array([[0, 0, 1, 1], [0.5, 0.5, 0.5, 0.5], [-1, -1, -0.5, -0.5]], np.float32)])
+    c = np.float32(cell)
+    rng = np.random.default_rng(7)
+    k = rng.integers(0, int(1 / cell), size=(40, 2)).astype(np.float32)
+    rects = np.concatenate([rects, np.stack([k[:, 0] * c, 1 - (k[:, 1] + 2) * c, (k[:, 0] + 2) * c,
+                                             1 - k[:, 1] * c], 1)]).astype(np.float32)
+    g = O.Grid(W.UNIT_BOX, cell, leaf).add(db)
+    m = O.MortonGrid(W.UNIT_BOX, cell, leaf).add(db)
+    assert m.info()["leaves"] == g.info()["leaves"]
+    for r in rects:
+        assert [tuple(x) for x in m.leaves(r)[:, 2:].tolist()] == [tuple(x) for x in g.leaves(r).tolist()]
+    gc, gd = g.range(rects, "digest")
+    mc, md = m.range(rects, "digest")
+    assert (gc == mc).all() and (gd == md).all()
+    assert _sets(m.range(rects[:30])) == _sets(g.range(rects[:30]))
