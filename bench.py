@@ -184,9 +184,19 @@ def main() -> int:
     log(f"GT-quadtree host build for comparison: {build_host_s:.2f} s (device {build_device_s:.2f} s); same cell "
         f"offsets: {same_store}")
     t = time.perf_counter()
-    stig = G.StigIndex(w.leaf).build(db)
+    stig_host = G.StigIndex(w.leaf).build(db)
+    stig_host_s = time.perf_counter() - t
+    t = time.perf_counter()
+    stig = G.StigIndex(w.leaf).build(db, device=True)
+    stig_device_s = time.perf_counter() - t
     fsg = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db, device=True)
-    log(f"FSG (device) and STIG (host) built ({time.perf_counter() - t:.1f} s)")
+    hres = stig_host.range_batched(rects)
+    hc, hd = hres.counts(), hres.digests()
+    del stig_host
+    dres = stig.range_batched(rects)
+    stig_same = bool((dres.counts() == hc).all() and (dres.digests() == hd).all())
+    log(f"STIG kd-split on the device {stig_device_s:.2f} s (host {stig_host_s:.2f} s), same answers: {stig_same}; "
+        f"FSG built on the device")
     few = max(3, args.steps // 2)
     _, fqps, _, _ = time_range(fsg, rects, few, 1, "FSG plain")
     _, sqps, _, _ = time_range(stig, rects, few, 1, "STIG plain")
@@ -292,6 +302,9 @@ def main() -> int:
             "range_plain_p99_ms": round(1e3 * float(np.percentile(pwall, 99)), 3),
             "stream_copy_gbs": round(copy_gbs, 1),
             "gt_quadtree_build_device_s": round(build_device_s, 3),
+            "stig_build_device_s": round(stig_device_s, 3),
+            "stig_build_host_s": round(stig_host_s, 3),
+            "stig_device_build_same_answers": stig_same,
             "gt_quadtree_build_host_s": round(build_host_s, 3),
             "device_build_equals_host": same_store,
             **edr,

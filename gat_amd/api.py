@@ -317,7 +317,8 @@ class _Index:
 
     def build(self, db: TrajSet, upload: bool = True, device: bool = False) -> "_Index":
         """Builds the index over `db`; upload=False keeps it on the host.
-        device=True builds the cell-ordered store on the GPU (Grid and FSG)."""
+        device=True builds the store on the GPU (Grid, MortonGrid and FSG: the
+        same cell-ordered store; STIG: the kd order, same tree shape)."""
         N.check_host(N.host().gath_index_build(self.handle, _ptr(_req(db.offsets, np.int64)), db.ntraj,
                                                _ptr(_req(db.lon, np.float32)), _ptr(_req(db.lat, np.float32)),
                                                (2 if device else 1) if upload else 0))

@@ -19,6 +19,28 @@
 // The trajectory store is read once per pass and every pass is a streaming
 // kernel, so the build is bound by HBM bandwidth and the sort's passes over
 // 8 bytes of key and value per point.
+//
+// Device construction of STIG's kd-tree store (STIG::initial / createIndex,
+// STIG.cpp:35-168, with the corrected split): the tree's shape depends only
+// on node sizes, so the host lays out the skeleton and the device orders the
+// points, one depth at a time:
+//   kd_key_kernel     the k-th node split at this depth covers positions
+//                     [lo_k, hi_k]; a position inside it gets the key
+//                     (2(k+1), coordinate of its point on the depth's axis as
+//                     sortable bits), a position in the gap after the r-th
+//                     node (2r + 1, 0) — non-decreasing along the positions
+//                     outside the split nodes;
+//   radix sort        stable, 64-bit keys with the point indices: every split
+//                     node's points come out ordered by the axis (so the left
+//                     half is <= the median point <= the right half, what
+//                     nth_element guarantees) and everything else stays put;
+//   kd_median_kernel  the median of node k: the axis coordinate of the point
+//                     now at (lo_k + hi_k) / 2;
+// then kd_box_kernel the leaves' bounding boxes (STIG.cpp:77-90) and
+// gather_kernel the store in the final order.
+#include <algorithm>
+#include <utility>
+
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "gat_common.hpp"
@@ -109,6 +131,79 @@ __global__ __launch_bounds__(kBlock) void cell_start_kernel(const uint32_t* __re
   }
 }
 
+// Unsigned order of the result = float order of f (-0 before +0; no NaNs in
+// a trajectory store).
+__device__ __forceinline__ uint32_t float_key(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ __launch_bounds__(kBlock) void kd_key_kernel(const float* __restrict__ lon, const float* __restrict__ lat,
+                                                        const uint32_t* __restrict__ perm, uint64_t n,
+                                                        const int64_t* __restrict__ lo, const int64_t* __restrict__ hi,
+                                                        uint32_t nsplit, uint32_t dim, uint64_t* __restrict__ keys) {
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    // r = number of split nodes starting at or before i.
+    uint32_t a = 0, b = nsplit;
+    while (a < b) {
+      const uint32_t m = (a + b) >> 1;
+      if (lo[m] <= int64_t(i)) a = m + 1;
+      else b = m;
+    }
+    uint64_t key = uint64_t(2 * a + 1) << 32;
+    if (a > 0 && int64_t(i) <= hi[a - 1]) {
+      const uint32_t j = perm[i];
+      key = (uint64_t(2 * a) << 32) | float_key(dim ? lat[j] : lon[j]);
+    }
+    keys[i] = key;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kd_median_kernel(const float* __restrict__ lon,
+                                                           const float* __restrict__ lat,
+                                                           const uint32_t* __restrict__ perm,
+                                                           const int64_t* __restrict__ lo,
+                                                           const int64_t* __restrict__ hi, uint32_t nsplit,
+                                                           uint32_t dim, float* __restrict__ median) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsplit) return;
+  const uint32_t j = perm[(lo[k] + hi[k]) / 2];
+  median[k] = dim ? lat[j] : lon[j];
+}
+
+// One wave per leaf [lo, hi]: (xmin, ymin, xmax, ymax) of its points, from
+// (999, 999, -999, -999) as STIG.cpp:77-90.
+__global__ __launch_bounds__(kBlock) void kd_box_kernel(const float* __restrict__ lon, const float* __restrict__ lat,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const int64_t* __restrict__ lo, const int64_t* __restrict__ hi,
+                                                        uint64_t nleaves, float4* __restrict__ box) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = uint64_t(gridDim.x) * (kBlock / 64);
+  for (uint64_t w = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64; w < nleaves; w += nw) {
+    float xl = 999.0f, yl = 999.0f, xh = -999.0f, yh = -999.0f;
+    for (int64_t i = lo[w] + lane; i <= hi[w]; i += 64) {
+      const uint32_t j = perm[i];
+      xl = fminf(xl, lon[j]);
+      xh = fmaxf(xh, lon[j]);
+      yl = fminf(yl, lat[j]);
+      yh = fmaxf(yh, lat[j]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      xl = fminf(xl, __shfl_xor(xl, off));
+      xh = fmaxf(xh, __shfl_xor(xh, off));
+      yl = fminf(yl, __shfl_xor(yl, off));
+      yh = fmaxf(yh, __shfl_xor(yh, off));
+    }
+    if (lane == 0) box[w] = make_float4(xl, yl, xh, yh);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void iota_kernel(uint32_t* __restrict__ v, uint64_t n) {
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = uint32_t(i);
+}
+
 struct Scratch {
   void* p = nullptr;
   ~Scratch() {
@@ -194,6 +289,108 @@ extern "C" int gat_cell_store_build(const gat_cell_grid* grid, const float* lon,
                                                                                cell_start_out);
   GAT_LAUNCH_CHECK("cell_start_kernel");
   // The scratch block is freed on return, after the stream has drained.
+  GAT_HIP_CHECK(hipStreamSynchronize(s));
+  return GAT_OK;
+}
+
+extern "C" int gat_kd_store_build(const float* lon, const float* lat, const int64_t* offsets, int32_t ntraj,
+                                  int64_t npoints, int32_t pid_bits, const int64_t* split_lo, const int64_t* split_hi,
+                                  const int64_t* depth_first, int32_t depths, const int64_t* leaf_lo,
+                                  const int64_t* leaf_hi, int64_t nleaves, float* xy_out, uint32_t* id_out,
+                                  uint16_t* pid_out, float* median_out, float* leaf_box_out, void* stream) {
+  GAT_REQUIRE(ntraj >= 0 && npoints >= 1 && npoints < int64_t(UINT32_MAX) && depths >= 0 && nleaves >= 1,
+              "kd store build: bad sizes");
+  GAT_REQUIRE(pid_bits >= 0 && pid_bits < 32, "kd store build: pid_bits outside 0..31");
+  GAT_REQUIRE(lon && lat && offsets && xy_out && id_out && (pid_bits > 0 || pid_out) && leaf_lo && leaf_hi &&
+                  leaf_box_out && depth_first && (depths == 0 || (split_lo && split_hi && median_out)),
+              "kd store build: null argument");
+  const uint64_t n = uint64_t(npoints);
+  const int64_t nsplit = depth_first[depths];
+  GAT_REQUIRE(depth_first[0] == 0 && nsplit >= 0 && nsplit < int64_t(UINT32_MAX / 2), "kd store build: bad depths");
+  // The host's skeleton: split nodes per depth disjoint and ascending, leaves
+  // inside [0, n).  A bad layout is rejected here, before any kernel reads it.
+  for (int32_t d = 0; d < depths; ++d) {
+    GAT_REQUIRE(depth_first[d] <= depth_first[d + 1], "kd store build: depths out of order");
+    for (int64_t k = depth_first[d]; k < depth_first[d + 1]; ++k) {
+      GAT_REQUIRE(split_lo[k] >= 0 && split_lo[k] < split_hi[k] && split_hi[k] < npoints,
+                  "kd store build: split node outside the points");
+      GAT_REQUIRE(k == depth_first[d] || split_lo[k] > split_hi[k - 1], "kd store build: split nodes overlap");
+    }
+  }
+  for (int64_t k = 0; k < nleaves; ++k)
+    GAT_REQUIRE(leaf_lo[k] >= 0 && leaf_lo[k] <= leaf_hi[k] && leaf_hi[k] < npoints,
+                "kd store build: leaf outside the points");
+  hipStream_t s = gat::as_stream(stream);
+  int max_rank_bits = 1;
+  for (int32_t d = 0; d < depths; ++d) {
+    const uint64_t top = 2 * uint64_t(depth_first[d + 1] - depth_first[d]) + 1;
+    while (max_rank_bits < 31 && (uint64_t(1) << max_rank_bits) <= top) ++max_rank_bits;
+  }
+  size_t sort_bytes = 0;
+  GAT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, sort_bytes, static_cast<const uint64_t*>(nullptr),
+                                          static_cast<uint64_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
+                                          static_cast<uint32_t*>(nullptr), size_t(n), 0, 32 + max_rank_bits, s));
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t k8 = al(n * 8), k4 = al(n * 4);
+  const size_t sp = al(size_t(std::max<int64_t>(nsplit, 1)) * 8), lf = al(size_t(nleaves) * 8);
+  const size_t total = 2 * k8 + 3 * k4 + 2 * sp + al(size_t(std::max<int64_t>(nsplit, 1)) * 4) + 2 * lf +
+                       al(size_t(nleaves) * 16) + sort_bytes;
+  Scratch scratch;
+  GAT_HIP_CHECK(hipMalloc(&scratch.p, total));
+  char* at = static_cast<char*>(scratch.p);
+  auto take = [&](size_t b) {
+    char* p = at;
+    at += b;
+    return p;
+  };
+  auto* keys_a = reinterpret_cast<uint64_t*>(take(k8));
+  auto* keys_b = reinterpret_cast<uint64_t*>(take(k8));
+  auto* perm_a = reinterpret_cast<uint32_t*>(take(k4));
+  auto* perm_b = reinterpret_cast<uint32_t*>(take(k4));
+  auto* tid = reinterpret_cast<uint32_t*>(take(k4));
+  auto* dlo = reinterpret_cast<int64_t*>(take(sp));
+  auto* dhi = reinterpret_cast<int64_t*>(take(sp));
+  auto* dmed = reinterpret_cast<float*>(take(al(size_t(std::max<int64_t>(nsplit, 1)) * 4)));
+  auto* llo = reinterpret_cast<int64_t*>(take(lf));
+  auto* lhi = reinterpret_cast<int64_t*>(take(lf));
+  auto* lbox = reinterpret_cast<float4*>(take(al(size_t(nleaves) * 16)));
+  void* sort_tmp = take(sort_bytes);
+  if (nsplit > 0) {
+    GAT_HIP_CHECK(hipMemcpyAsync(dlo, split_lo, size_t(nsplit) * 8, hipMemcpyHostToDevice, s));
+    GAT_HIP_CHECK(hipMemcpyAsync(dhi, split_hi, size_t(nsplit) * 8, hipMemcpyHostToDevice, s));
+  }
+  GAT_HIP_CHECK(hipMemcpyAsync(llo, leaf_lo, size_t(nleaves) * 8, hipMemcpyHostToDevice, s));
+  GAT_HIP_CHECK(hipMemcpyAsync(lhi, leaf_hi, size_t(nleaves) * 8, hipMemcpyHostToDevice, s));
+  const dim3 grid(uint32_t(blocks_for(n, 16))), block(kBlock);
+  iota_kernel<<<grid, block, 0, s>>>(perm_a, n);
+  GAT_LAUNCH_CHECK("iota_kernel");
+  for (int32_t d = 0; d < depths; ++d) {
+    const int64_t first = depth_first[d], cnt = depth_first[d + 1] - first;
+    if (cnt == 0) continue;
+    const uint32_t dim = uint32_t(d & 1);  // even depths split on x, odd on y (STIG.cpp:94-140)
+    kd_key_kernel<<<grid, block, 0, s>>>(lon, lat, perm_a, n, dlo + first, dhi + first, uint32_t(cnt), dim, keys_a);
+    GAT_LAUNCH_CHECK("kd_key_kernel");
+    int bits = 1;
+    while (bits < 31 && (uint64_t(1) << bits) <= uint64_t(2 * cnt + 1)) ++bits;
+    GAT_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_a, keys_b, perm_a, perm_b, size_t(n), 0,
+                                            32 + bits, s));
+    std::swap(perm_a, perm_b);
+    kd_median_kernel<<<dim3(uint32_t((cnt + kBlock - 1) / kBlock)), block, 0, s>>>(
+        lon, lat, perm_a, dlo + first, dhi + first, uint32_t(cnt), dim, dmed + first);
+    GAT_LAUNCH_CHECK("kd_median_kernel");
+  }
+  kd_box_kernel<<<dim3(uint32_t(blocks_for(uint64_t(nleaves) * 64, 16))), block, 0, s>>>(lon, lat, perm_a, llo, lhi,
+                                                                                         uint64_t(nleaves), lbox);
+  GAT_LAUNCH_CHECK("kd_box_kernel");
+  GAT_HIP_CHECK(hipMemsetAsync(tid, 0, n * 4, s));
+  tid_kernel<<<dim3(uint32_t(blocks_for(uint64_t(ntraj) * 64, 16))), block, 0, s>>>(offsets, uint32_t(ntraj), n,
+                                                                                    tid);
+  GAT_LAUNCH_CHECK("tid_kernel");
+  gather_kernel<<<grid, block, 0, s>>>(lon, lat, offsets, tid, perm_a, n, uint32_t(ntraj), uint32_t(pid_bits),
+                                       reinterpret_cast<float2*>(xy_out), id_out, pid_out);
+  GAT_LAUNCH_CHECK("gather_kernel");
+  if (nsplit > 0) GAT_HIP_CHECK(hipMemcpyAsync(median_out, dmed, size_t(nsplit) * 4, hipMemcpyDeviceToHost, s));
+  GAT_HIP_CHECK(hipMemcpyAsync(leaf_box_out, lbox, size_t(nleaves) * 16, hipMemcpyDeviceToHost, s));
   GAT_HIP_CHECK(hipStreamSynchronize(s));
   return GAT_OK;
 }

@@ -114,11 +114,9 @@ int RangeEngine::upload(const float* xy, const int32_t* tp, int64_t n) {
   return stream_.sync();
 }
 
-int RangeEngine::buildStore(const gat_cell_grid& grid, const TrajectoryView& db, std::vector<int64_t>* cellStart,
-                            int64_t* badPoint) {
-  if (int rc = stream_.create()) return rc;
-  *badPoint = -1;
-  const int64_t n = db.pointNum();
+// The id packing of a store over `db` (as upload() chooses it): pid_bits > 0
+// packs (tid, pid) into 32 bits, 0 keeps a separate 16-bit pid array.
+static int store_packing(const TrajectoryView& db, int* pidBitsOut) {
   if (db.trajNum < 0 || !db.offsets || db.offsets[0] != 0) {
     gath_set_error("store build: offsets must start at 0");
     return GAT_ERR_ARG;
@@ -132,24 +130,28 @@ int RangeEngine::buildStore(const gat_cell_grid& grid, const TrajectoryView& db,
     }
     if (len > 0) max_pid = std::max(max_pid, uint32_t(std::min<int64_t>(len - 1, UINT32_MAX)));
   }
-  const uint32_t max_tid = uint32_t(db.trajNum);
   const int pid_bits = std::max(bit_length(max_pid), 1);
-  const bool packed = bit_length(max_tid) + pid_bits <= 32;
+  const bool packed = bit_length(uint32_t(db.trajNum)) + pid_bits <= 32;
   if (!packed && max_pid > 0xFFFFu) {
     gath_set_error("store build: point ids up to %u do not fit 16 bits", max_pid);
     return GAT_ERR_ARG;
   }
+  *pidBitsOut = packed ? pid_bits : 0;
+  return GAT_OK;
+}
+
+// Uploads the trajectories for a device build and sizes the store arrays.
+int RangeEngine::stageBuild(const TrajectoryView& db, int pidBits, DeviceBuffer& lon, DeviceBuffer& lat,
+                            DeviceBuffer& off) {
+  const int64_t n = db.pointNum();
   const size_t cnt = size_t(std::max<int64_t>(n, 1));
-  const uint64_t cellnum = uint64_t(grid.cells_x) * uint64_t(grid.cells_y);
   void* s = stream_.get();
-  DeviceBuffer lon, lat, off, starts;
   if (int rc = lon.reserve(cnt * 4)) return rc;
   if (int rc = lat.reserve(cnt * 4)) return rc;
   if (int rc = off.reserve((size_t(db.trajNum) + 1) * 8)) return rc;
-  if (int rc = starts.reserve((cellnum + 1) * 8)) return rc;
   if (int rc = xy_.reserve(cnt * 8)) return rc;
   if (int rc = id_.reserve(cnt * 4)) return rc;
-  if (packed)
+  if (pidBits > 0)
     pid_.release();
   else if (int rc = pid_.reserve(cnt * 2))
     return rc;
@@ -158,19 +160,61 @@ int RangeEngine::buildStore(const gat_cell_grid& grid, const TrajectoryView& db,
     GATH_TRY(gat_memcpy_h2d(lat.get(), db.lat, size_t(n) * 4, s), "trajectory upload");
   }
   GATH_TRY(gat_memcpy_h2d(off.get(), db.offsets, (size_t(db.trajNum) + 1) * 8, s), "trajectory upload");
-  GATH_TRY(gat_cell_store_build(&grid, lon.as<float>(), lat.as<float>(), off.as<int64_t>(), db.trajNum, n,
-                                packed ? pid_bits : 0, xy_.as<float>(), id_.as<uint32_t>(),
-                                packed ? nullptr : pid_.as<uint16_t>(), starts.as<int64_t>(), badPoint, s),
-           "store build");
   n_ = 0;
   store_ = gat_point_store{};
+  return GAT_OK;
+}
+
+int RangeEngine::buildStore(const gat_cell_grid& grid, const TrajectoryView& db, std::vector<int64_t>* cellStart,
+                            int64_t* badPoint) {
+  if (int rc = stream_.create()) return rc;
+  *badPoint = -1;
+  int pid_bits = 0;
+  if (int rc = store_packing(db, &pid_bits)) return rc;
+  const int64_t n = db.pointNum();
+  const uint64_t cellnum = uint64_t(grid.cells_x) * uint64_t(grid.cells_y);
+  void* s = stream_.get();
+  DeviceBuffer lon, lat, off, starts;
+  if (int rc = starts.reserve((cellnum + 1) * 8)) return rc;
+  if (int rc = stageBuild(db, pid_bits, lon, lat, off)) return rc;
+  GATH_TRY(gat_cell_store_build(&grid, lon.as<float>(), lat.as<float>(), off.as<int64_t>(), db.trajNum, n, pid_bits,
+                                xy_.as<float>(), id_.as<uint32_t>(), pid_bits > 0 ? nullptr : pid_.as<uint16_t>(),
+                                starts.as<int64_t>(), badPoint, s),
+           "store build");
   if (*badPoint >= 0) return GAT_OK;
   cellStart->assign(size_t(cellnum) + 1, 0);
   GATH_TRY(gat_memcpy_d2h(cellStart->data(), starts.get(), (cellnum + 1) * 8, s), "cell offsets");
   if (int rc = stream_.sync()) return rc;
   n_ = n;
-  store_ = gat_point_store{xy_.as<float>(), id_.as<uint32_t>(), packed ? nullptr : pid_.as<uint16_t>(), n,
-                           packed ? pid_bits : 0};
+  store_ = gat_point_store{xy_.as<float>(), id_.as<uint32_t>(), pid_bits > 0 ? nullptr : pid_.as<uint16_t>(), n,
+                           pid_bits};
+  return GAT_OK;
+}
+
+int RangeEngine::buildKdStore(const TrajectoryView& db, const KdSkeleton& k, std::vector<float>* medians,
+                              std::vector<float>* leafBoxes) {
+  if (int rc = stream_.create()) return rc;
+  int pid_bits = 0;
+  if (int rc = store_packing(db, &pid_bits)) return rc;
+  const int64_t n = db.pointNum();
+  if (n < 1 || k.leafLo.empty() || k.depthFirst.empty()) {
+    gath_set_error("kd store build: no points or no leaves");
+    return GAT_ERR_ARG;
+  }
+  DeviceBuffer lon, lat, off;
+  if (int rc = stageBuild(db, pid_bits, lon, lat, off)) return rc;
+  medians->assign(k.splitLo.size(), 0.0f);
+  leafBoxes->assign(k.leafLo.size() * 4, 0.0f);
+  GATH_TRY(gat_kd_store_build(lon.as<float>(), lat.as<float>(), off.as<int64_t>(), db.trajNum, n, pid_bits,
+                              k.splitLo.data(), k.splitHi.data(), k.depthFirst.data(),
+                              int32_t(k.depthFirst.size()) - 1, k.leafLo.data(), k.leafHi.data(),
+                              int64_t(k.leafLo.size()), xy_.as<float>(), id_.as<uint32_t>(),
+                              pid_bits > 0 ? nullptr : pid_.as<uint16_t>(), medians->data(), leafBoxes->data(),
+                              stream_.get()),
+           "kd store build");
+  n_ = n;
+  store_ = gat_point_store{xy_.as<float>(), id_.as<uint32_t>(), pid_bits > 0 ? nullptr : pid_.as<uint16_t>(), n,
+                           pid_bits};
   return GAT_OK;
 }
 

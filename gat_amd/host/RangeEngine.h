@@ -129,6 +129,16 @@ class RangeEngine {
   // *badPoint the first point off the grid (then no store is kept) or -1.
   int buildStore(const gat_cell_grid& grid, const TrajectoryView& db, std::vector<int64_t>* cellStart,
                  int64_t* badPoint);
+  // Builds the store on the device in the order of a kd-tree of the given
+  // shape (gat_kd_store_build), with the same id packing; *medians gets one
+  // value per split node, *leafBoxes four per leaf.
+  struct KdSkeleton {
+    std::vector<int64_t> splitLo, splitHi;  // split nodes, depth by depth, ascending within a depth
+    std::vector<int64_t> depthFirst;        // depths + 1 offsets into splitLo / splitHi
+    std::vector<int64_t> leafLo, leafHi;    // inclusive leaf ranges
+  };
+  int buildKdStore(const TrajectoryView& db, const KdSkeleton& skeleton, std::vector<float>* medians,
+                   std::vector<float>* leafBoxes);
   int64_t pointNum() const { return n_; }
   const gat_point_store& store() const { return store_; }
   void* stream() { return stream_.get(); }
@@ -188,6 +198,7 @@ class RangeEngine {
 
  private:
   int stage(const MBB* bounds, int nq, const std::vector<gat_range_task>& tasks, const uint64_t* qbase);
+  int stageBuild(const TrajectoryView& db, int pidBits, DeviceBuffer& lon, DeviceBuffer& lat, DeviceBuffer& off);
   int uploadBoxes(const MBB* bounds, int nq);
   int enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out);
   // The selection kernel for the tree's kind (nodeCount may be null).

@@ -87,6 +87,7 @@ int STIG::initial(int blk, const TrajectoryView& db, bool upload) {
   nodes.clear();
   if (totalPointsNum == 0) return GAT_OK;
   skeleton(0, 0, totalPointsNum - 1, totalPointsNum <= blockSize);
+  if (upload && deviceBuild) return buildOnDevice(db);
   int spawn = 0;
   for (int t = threads(); t > 1; t >>= 1) ++spawn;
   partition(0, pts, spawn);
@@ -100,6 +101,46 @@ int STIG::initial(int blk, const TrajectoryView& db, bool upload) {
     tp[2 * i + 1] = pts[i].pid;
   }
   if (int rc = engine_.upload(xy.data(), tp.data(), totalPointsNum)) return rc;
+  int depth = 0;
+  const std::vector<gat_tree_node> tn = treeNodes(&depth);
+  return tree_.upload(tn, 2, depth, engine_.stream());
+}
+
+// The skeleton's split nodes depth by depth (each depth ascending in
+// position) and its leaves go to the device, which orders the points; the
+// medians and leaf boxes come back into the nodes.
+int STIG::buildOnDevice(const TrajectoryView& db) {
+  RangeEngine::KdSkeleton k;
+  std::vector<int> splitNode, leafNode;
+  std::vector<int> level{0};
+  k.depthFirst.push_back(0);
+  while (!level.empty()) {
+    std::vector<int> next;
+    std::sort(level.begin(), level.end(), [&](int a, int b) { return nodes[a].lo < nodes[b].lo; });
+    for (int ni : level) {
+      const Node& n = nodes[size_t(ni)];
+      if (n.leaf) {
+        leafNode.push_back(ni);
+        k.leafLo.push_back(n.lo);
+        k.leafHi.push_back(n.hi);
+        continue;
+      }
+      splitNode.push_back(ni);
+      k.splitLo.push_back(n.lo);
+      k.splitHi.push_back(n.hi);
+      next.push_back(n.left);
+      next.push_back(n.right);
+    }
+    if (next.empty()) break;
+    k.depthFirst.push_back(int64_t(k.splitLo.size()));
+    level.swap(next);
+  }
+  if (k.depthFirst.back() != int64_t(k.splitLo.size())) k.depthFirst.push_back(int64_t(k.splitLo.size()));
+  std::vector<float> med, boxes;
+  if (int rc = engine_.buildKdStore(db, k, &med, &boxes)) return rc;
+  for (size_t i = 0; i < splitNode.size(); ++i) nodes[size_t(splitNode[i])].median = med[i];
+  for (size_t i = 0; i < leafNode.size(); ++i)
+    nodes[size_t(leafNode[i])].box = MBB(boxes[4 * i], boxes[4 * i + 1], boxes[4 * i + 2], boxes[4 * i + 3]);
   int depth = 0;
   const std::vector<gat_tree_node> tn = treeNodes(&depth);
   return tree_.upload(tn, 2, depth, engine_.stream());

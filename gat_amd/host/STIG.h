@@ -66,6 +66,11 @@ class STIG {
   int64_t totalPointsNum = 0;
   int maxTid = 0;
   int hostThreads = 0;
+  // With upload: order the points on the device (gat_kd_store_build) instead
+  // of partitioning them on the host.  Same tree shape and medians defined
+  // the same way; points with equal coordinates may sit on the other side of
+  // a median than in the host build, which changes no answer.
+  bool deviceBuild = false;
   RangeEngine& engine() { return engine_; }
   const DeviceTree& deviceTree() const { return tree_; }
 
@@ -75,6 +80,7 @@ class STIG {
     int32_t tid, pid;
   };
   int skeleton(int depth, int64_t lo, int64_t hi, bool leaf);
+  int buildOnDevice(const TrajectoryView& db);
   void partition(int node, std::vector<P>& pts, int spawnDepth);
   void addLeaves(const MBB& q, uint32_t qid, TaskList& list) const;
   int threads() const;

@@ -115,7 +115,9 @@ int gath_index_build(void* p, const int64_t* offsets, int ntraj, const float* lo
     case Index::kFsg:
       ix->fsg->deviceBuild = upload == 2;
       return ix->fsg->addDatasetToGrid(db, upload != 0);
-    default: return ix->stig->initial(ix->stig->blockSize, db, upload != 0);
+    default:
+      ix->stig->deviceBuild = upload == 2;
+      return ix->stig->initial(ix->stig->blockSize, db, upload != 0);
   }
 }
 

@@ -112,6 +112,28 @@ int gat_cell_store_build(const gat_cell_grid* grid, const float* lon_dev, const 
                          float* xy_out_dev, uint32_t* id_out_dev, uint16_t* pid_out_dev, int64_t* cell_start_out_dev,
                          int64_t* bad_point, void* stream);
 
+/* Builds STIG's kd-tree store on the device (STIG::initial / createIndex,
+ * STIG.cpp:35-168, with the corrected split: a node [lo, hi] splits at
+ * mid = (lo + hi) / 2 into [lo, mid - 1] and [mid, hi], the points of the left
+ * part <= the point at mid <= those of the right part on the node's axis, x at
+ * even depths, y at odd).  The host passes the tree's shape (it depends only
+ * on node sizes): the split nodes of depth d are
+ * (split_lo[k], split_hi[k]) for k in [depth_first[d], depth_first[d+1]),
+ * ascending and disjoint, d < depths; and the leaves (leaf_lo, leaf_hi)
+ * (inclusive ranges).  Skeleton arrays, median_out (one float per split node:
+ * the axis coordinate of the point at mid) and leaf_box_out (xmin, ymin, xmax,
+ * ymax per leaf, from (999, 999, -999, -999) as STIG.cpp:77-90) are host
+ * memory; the trajectories and the store (as gat_cell_store_build's) are
+ * device memory.  Each depth is one stable radix sort, so points with equal
+ * coordinates keep their previous order and the build is deterministic.
+ * Allocates its own scratch (about 32 bytes per point); returns after
+ * `stream` has drained.  No reference counterpart. */
+int gat_kd_store_build(const float* lon_dev, const float* lat_dev, const int64_t* offsets_dev, int32_t ntraj,
+                       int64_t npoints, int32_t pid_bits, const int64_t* split_lo, const int64_t* split_hi,
+                       const int64_t* depth_first, int32_t depths, const int64_t* leaf_lo, const int64_t* leaf_hi,
+                       int64_t nleaves, float* xy_out_dev, uint32_t* id_out_dev, uint16_t* pid_out_dev,
+                       float* median_out, float* leaf_box_out, void* stream);
+
 /* HBM access-pattern ceilings over a buffer of `bytes` (a positive multiple of
  * 16): kind 0 writes the buffer with 16-byte streaming stores, 1 reads it; 2
  * and 3 write runs of `run` (1..4096) 8-byte pairs packed back to back, one

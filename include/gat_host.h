@@ -79,7 +79,9 @@ void gath_index_free(void* index);
 /* Builds the index over trajectories 1..ntraj (offsets: ntraj+1 entries); with
  * upload = 0 only the host structures are built (no GPU needed); 1 buckets the
  * points on the host and uploads the store; 2 uploads the trajectories and
- * builds the same store on the device (Grid and FSG; STIG builds on the host). */
+ * builds the store on the device: the same store for Grid, MortonGrid and
+ * FSG; for STIG the same tree shape and medians, with points of equal
+ * coordinates possibly on the other side of a median (same answers). */
 int gath_index_build(void* index, const int64_t* offsets, int ntraj, const float* lon, const float* lat,
                      int upload);
 /* out[0..7]: points, trajectories, cells (or leaves for STIG), cells per axis

@@ -146,3 +146,78 @@ def test_empty_dataset(gpu):
     assert (g.cell_starts() == 0).all()
     res = g.range_batched(np.array([[0, 0, 1, 1]], np.float32))
     assert res.pairs_total == 0
+
+
+# ------------------------------------------------------------- STIG kd-split
+
+def _subtree_ranges(nodes):
+    """Store range [lo, hi) of every node of a breadth-first tree (children
+    follow their parent)."""
+    lo = nodes["start"].astype(np.int64).copy()
+    hi = lo + nodes["count"].astype(np.int64)
+    for i in range(len(nodes) - 1, -1, -1):
+        c = int(nodes["first_child"][i])
+        if c >= 0:
+            lo[i] = min(lo[c], lo[c + 1])
+            hi[i] = max(hi[c], hi[c + 1])
+    return lo, hi
+
+
+@pytest.mark.parametrize("block", [8, 256])
+def test_stig_device_build(gpu, block):
+    """STIG's kd-split on the device (gat_kd_store_build): the host build's
+    tree shape (the split positions depend only on node sizes, STIG.cpp:71-168
+    corrected), every point once, every leaf box the exact box of its points,
+    every split half-plane holding its whole subtree (what the search's
+    inclusive tests rely on, STIG.cpp:179-252), deterministic, and answers
+    equal to inclusive brute force on both query forms."""
+    for name, db in _dbs():
+        host = api.StigIndex(block).build(db)
+        dev = api.StigIndex(block).build(db, device=True)
+        hn, hd = host.tree_nodes()
+        dn, dd = dev.tree_nodes()
+        assert hd == dd and len(hn) == len(dn), name
+        for f in ("first_child", "start", "count", "flags"):
+            assert (hn[f] == dn[f]).all(), (name, f)
+        xy, ids, pid, bits = dev.fetch_store()
+        t, p = _ids(ids, pid, bits)
+        tids = np.repeat(np.arange(1, db.ntraj + 1), np.diff(db.offsets))
+        pids = np.arange(db.npoints) - np.repeat(db.offsets[:-1], np.diff(db.offsets))
+        order = np.lexsort((p, t))
+        assert (t[order] == tids).all() and (p[order] == pids).all(), name
+        # The coordinates travel with their ids.
+        src = db.offsets[t - 1] + p
+        assert (xy[:, 0] == db.lon[src]).all() and (xy[:, 1] == db.lat[src]).all(), name
+        lo, hi = _subtree_ranges(dn)
+        for i, nd in enumerate(dn):
+            x, y = xy[lo[i]:hi[i], 0], xy[lo[i]:hi[i], 1]
+            b = nd["box"]
+            if nd["first_child"] < 0:
+                assert list(b) == [x.min(), y.min(), x.max(), y.max()], (name, i)
+            else:
+                assert (x >= b[0]).all() and (x <= b[2]).all() and (y >= b[1]).all() and (y <= b[3]).all(), (name, i)
+        again = api.StigIndex(block).build(db, device=True).fetch_store()
+        assert (again[0].view(np.uint32) == xy.view(np.uint32)).all() and (again[1] == ids).all(), name
+        rects = np.concatenate([api.synth_rects(5, 60, 1e-3, 0.3, True), _edge_rects(db, 1 / 16, 30, 6)])
+        counts, dig = O.brute_range(db, rects, "digest")
+        # A result reads its index's last batch: check each before the next call.
+        for run in (lambda: dev.range_pairs(rects), lambda: dev.set_batch_params(32, 0).range_batched(rects)):
+            res = run()
+            assert (res.counts() == counts).all() and (res.digests() == dig).all(), name
+
+
+def test_stig_device_build_root_leaf_and_cfg1(gpu):
+    """A dataset no larger than a block is one leaf (no split depth); cfg1's
+    2.7M points give the same answers as the host-built index."""
+    small = W.CFG1.db(3)
+    dev = api.StigIndex(1 << 20).build(small, device=True)
+    nodes, depth = dev.tree_nodes()
+    assert len(nodes) == 1 and depth == 0 and nodes["count"][0] == small.npoints
+    db, rects = W.CFG1.db(), W.CFG1.rects()
+    host = api.StigIndex(W.CFG1.leaf).build(db)
+    dev = api.StigIndex(W.CFG1.leaf).build(db, device=True)
+    a = host.range_batched(rects)
+    ca, da = a.counts(), a.digests()
+    b = dev.range_batched(rects)
+    assert (b.counts() == ca).all() and (b.digests() == da).all()
+    _check_pairs(dev.range_pairs(rects), _sets(O.brute_range(db, rects)), rects)
