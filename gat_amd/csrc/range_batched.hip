@@ -531,16 +531,10 @@ int launch_scan(L load, uint64_t n, typename L::T* tiles, typename L::T* out, ty
 }
 
 // ---------------------------------------------------------------- grouping
-// Entries are grouped by leaf into self-contained records, so the refine
-// needs no further lookups: a list entry carries the rectangle, its output
-// region and id; a work item carries the leaf's store run and its slice of
-// the leaf's list.
-struct ListEntry {
-  float4 box;
-  uint64_t base;
-  uint32_t q, pad;
-};
-static_assert(sizeof(ListEntry) == 32, "list entry layout");
+// Entries are grouped by leaf: a list entry is the rectangle's id (4 bytes;
+// the refine reads the rectangle and its output region from the batch's
+// tables, a few hundred KB that stay in L2), a work item carries the leaf's
+// store run and its slice of the leaf's list.
 struct Item {
   uint32_t start, count, list, n;
 };
@@ -565,9 +559,8 @@ struct LoadItemBounds {
 __global__ void leaf_scatter_kernel(const uint2* __restrict__ lq, const unsigned long long* __restrict__ lq_total,
                                     uint64_t lq_cap, const Node* __restrict__ nodes, uint32_t nnodes,
                                     const uint2* __restrict__ node_off, const uint32_t* __restrict__ node_qcount,
-                                    uint32_t* __restrict__ node_fill, uint32_t qb, const float4* __restrict__ boxes,
-                                    const uint64_t* __restrict__ qbase, uint32_t nqueries,
-                                    ListEntry* __restrict__ qlist, uint64_t qlist_cap, Item* __restrict__ items,
+                                    uint32_t* __restrict__ node_fill, uint32_t qb, uint32_t nqueries,
+                                    uint32_t* __restrict__ qlist, uint64_t qlist_cap, Item* __restrict__ items,
                                     uint64_t items_cap) {
   const uint64_t total = *lq_total;
   const uint64_t n = total < lq_cap ? total : lq_cap;
@@ -577,14 +570,7 @@ __global__ void leaf_scatter_kernel(const uint2* __restrict__ lq, const unsigned
     const uint32_t r = atomicAdd(node_fill + e.x, 1u);
     const uint2 off = node_off[e.x];
     const uint64_t at = uint64_t(off.x) + r;
-    if (at < qlist_cap) {
-      ListEntry le;
-      le.box = boxes[e.y];
-      le.base = qbase[e.y];
-      le.q = e.y;
-      le.pad = 0;
-      qlist[at] = le;
-    }
+    if (at < qlist_cap) qlist[at] = e.y;
     if (r % qb == 0) {
       const uint64_t it = uint64_t(off.y) + r / qb;
       const uint32_t len = node_qcount[e.x];
@@ -720,8 +706,9 @@ __device__ __forceinline__ void refine_chunk(const Store& store, uint64_t at0, u
 template <bool kPacked>
 __global__ __launch_bounds__(kRefBlock) void leaf_refine_kernel(
     Store store, const Item* __restrict__ items, const uint32_t* __restrict__ item_total, uint64_t items_cap,
-    const ListEntry* __restrict__ qlist, uint64_t qlist_cap, uint32_t nqueries, uint32_t* __restrict__ qcount,
-    int2* __restrict__ pairs, uint64_t capacity) {
+    const uint32_t* __restrict__ qlist, uint64_t qlist_cap, const float4* __restrict__ boxes,
+    const uint64_t* __restrict__ qbase, uint32_t nqueries, uint32_t* __restrict__ qcount, int2* __restrict__ pairs,
+    uint64_t capacity) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t wave = uniform((blockIdx.x * kRefBlock + threadIdx.x) / kWave);
   const uint64_t nwaves = uint64_t(gridDim.x) * (kRefBlock / kWave);
@@ -740,11 +727,11 @@ __global__ __launch_bounds__(kRefBlock) void leaf_refine_kernel(
     float4 bj = make_float4(nan, nan, nan, nan);
     uint64_t basej = 0;
     if (lane < nql) {
-      const ListEntry e = qlist[uint64_t(list) + lane];
-      if (e.q < nqueries) {
-        qj = e.q;
-        bj = e.box;
-        basej = e.base;
+      const uint32_t q = qlist[uint64_t(list) + lane];
+      if (q < nqueries) {
+        qj = q;
+        bj = boxes[q];
+        basej = qbase[q];
       }
     }
     for (uint32_t base = 0; base < count; base += kSpan) {
@@ -1027,8 +1014,8 @@ __device__ __forceinline__ void single_chunk_item(const Store& store, uint32_t s
 template <bool kPacked>
 __global__ __launch_bounds__(kRefBlock, 8) void leaf_refine_runs_kernel(
     Store store, const Item* __restrict__ items, const uint32_t* __restrict__ item_total, uint64_t items_cap,
-    const ListEntry* __restrict__ qlist, uint64_t qlist_cap, uint32_t nqueries, int2* __restrict__ pairs,
-    uint64_t capacity, RunOut o) {
+    const uint32_t* __restrict__ qlist, uint64_t qlist_cap, const float4* __restrict__ boxes, uint32_t nqueries,
+    int2* __restrict__ pairs, uint64_t capacity, RunOut o) {
   __shared__ int4 rings[kRefBlock / kWave][kRing / 2];
   const uint32_t lane = threadIdx.x & (kWave - 1);
   int2* ring = reinterpret_cast<int2*>(rings[threadIdx.x / kWave]);
@@ -1048,10 +1035,10 @@ __global__ __launch_bounds__(kRefBlock, 8) void leaf_refine_runs_kernel(
     uint32_t qj = 0xFFFFFFFFu;
     float4 bj = make_float4(nan, nan, nan, nan);
     if (lane < nql) {
-      const ListEntry e = qlist[uint64_t(list) + lane];
-      if (e.q < nqueries) {
-        qj = e.q;
-        bj = e.box;
+      const uint32_t q = qlist[uint64_t(list) + lane];
+      if (q < nqueries) {
+        qj = q;
+        bj = boxes[q];
       }
     }
     const uint64_t base = o.item_base[t];
@@ -1239,10 +1226,9 @@ int gat_leaf_tasks(const uint32_t* leaf_query, const uint64_t* lq_total, int64_t
 
 int gat_leaf_scatter(const uint32_t* leaf_query, const uint64_t* lq_total, int64_t lq_capacity,
                      const gat_tree_node* nodes, int64_t nnodes, const uint32_t* node_off,
-                     const uint32_t* node_qcount, uint32_t* node_fill, int32_t qb, const float* boxes,
-                     const uint64_t* qbase, int64_t nqueries, void* qlist, int64_t qlist_capacity, uint32_t* items,
-                     int64_t items_capacity, void* stream) {
-  GAT_REQUIRE(lq_total && nodes && node_off && node_qcount && node_fill && qlist && items && boxes && qbase &&
+                     const uint32_t* node_qcount, uint32_t* node_fill, int32_t qb, int64_t nqueries, uint32_t* qlist,
+                     int64_t qlist_capacity, uint32_t* items, int64_t items_capacity, void* stream) {
+  GAT_REQUIRE(lq_total && nodes && node_off && node_qcount && node_fill && qlist && items &&
                   qb > 0 && qb <= kMaxQb && nnodes >= 1 && nnodes < int64_t(INT32_MAX),
               "gat_leaf_scatter: bad arguments");
   GAT_REQUIRE(lq_capacity >= 0 && qlist_capacity >= 0 && qlist_capacity <= int64_t(UINT32_MAX) &&
@@ -1254,16 +1240,16 @@ int gat_leaf_scatter(const uint32_t* leaf_query, const uint64_t* lq_total, int64
                         gat::as_stream(stream)>>>(
       reinterpret_cast<const uint2*>(leaf_query), reinterpret_cast<const unsigned long long*>(lq_total),
       uint64_t(lq_capacity), reinterpret_cast<const Node*>(nodes), uint32_t(nnodes),
-      reinterpret_cast<const uint2*>(node_off), node_qcount, node_fill, uint32_t(qb),
-      reinterpret_cast<const float4*>(boxes), qbase, uint32_t(nqueries), static_cast<ListEntry*>(qlist),
+      reinterpret_cast<const uint2*>(node_off), node_qcount, node_fill, uint32_t(qb), uint32_t(nqueries), qlist,
       uint64_t(qlist_capacity), reinterpret_cast<Item*>(items), uint64_t(items_capacity));
   GAT_LAUNCH_CHECK("leaf_scatter_kernel");
   return GAT_OK;
 }
 
 int gat_leaf_refine(const gat_point_store* sp, const uint32_t* items, const uint32_t* item_total,
-                    int64_t items_capacity, const void* qlist, int64_t qlist_capacity, int64_t nqueries,
-                    uint32_t* qcount, int32_t* pairs, int64_t capacity, uint64_t* pair_total, void* stream) {
+                    int64_t items_capacity, const uint32_t* qlist, int64_t qlist_capacity, const float* boxes,
+                    const uint64_t* qbase, int64_t nqueries, uint32_t* qcount, int32_t* pairs, int64_t capacity,
+                    uint64_t* pair_total, void* stream) {
   GAT_REQUIRE(sp && sp->npoints >= 0 && sp->pid_bits >= 0 && sp->pid_bits < 32, "gat_leaf_refine: bad store");
   GAT_REQUIRE(sp->npoints == 0 || (sp->xy_dev && sp->id_dev && (sp->pid_bits > 0 || sp->pid_dev)),
               "gat_leaf_refine: null store array");
@@ -1272,22 +1258,22 @@ int gat_leaf_refine(const gat_point_store* sp, const uint32_t* items, const uint
               "gat_leaf_refine: bad sizes");
   hipStream_t s = gat::as_stream(stream);
   if (items_capacity > 0 && nqueries > 0) {
-    GAT_REQUIRE(items && qlist && qcount && pairs, "gat_leaf_refine: null pointer");
+    GAT_REQUIRE(items && qlist && boxes && qbase && qcount && pairs, "gat_leaf_refine: null pointer");
     const Store st{reinterpret_cast<const float2*>(sp->xy_dev), sp->id_dev, sp->pid_dev, uint64_t(sp->npoints),
                    uint32_t(sp->pid_bits)};
     const int blocks = grid_for(uint64_t(items_capacity), kRefBlock / kWave, 16);
     const dim3 grid{uint32_t(blocks)}, block{kRefBlock};
     auto* I = reinterpret_cast<const Item*>(items);
-    auto* L = static_cast<const ListEntry*>(qlist);
+    auto* B = reinterpret_cast<const float4*>(boxes);
     auto* P = reinterpret_cast<int2*>(pairs);
     if (sp->pid_bits > 0)
-      leaf_refine_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
-                                                      uint64_t(qlist_capacity), uint32_t(nqueries), qcount, P,
-                                                      uint64_t(capacity));
+      leaf_refine_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
+                                                      uint64_t(qlist_capacity), B, qbase, uint32_t(nqueries), qcount,
+                                                      P, uint64_t(capacity));
     else
-      leaf_refine_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
-                                                       uint64_t(qlist_capacity), uint32_t(nqueries), qcount, P,
-                                                       uint64_t(capacity));
+      leaf_refine_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
+                                                       uint64_t(qlist_capacity), B, qbase, uint32_t(nqueries), qcount,
+                                                       P, uint64_t(capacity));
     GAT_LAUNCH_CHECK("leaf_refine_kernel");
   }
   if (pair_total && nqueries > 0) {
@@ -1300,8 +1286,8 @@ int gat_leaf_refine(const gat_point_store* sp, const uint32_t* items, const uint
 }
 
 int gat_leaf_refine_runs(const gat_point_store* sp, const uint32_t* items, const uint32_t* item_total,
-                         int64_t items_capacity, const void* qlist, int64_t qlist_capacity, int64_t nqueries,
-                         uint32_t* qcount, int32_t* pairs, int64_t capacity, gat_range_run* runs,
+                         int64_t items_capacity, const uint32_t* qlist, int64_t qlist_capacity, const float* boxes,
+                         int64_t nqueries, uint32_t* qcount, int32_t* pairs, int64_t capacity, gat_range_run* runs,
                          int64_t runs_capacity, uint64_t* item_base, uint64_t* tiles, uint64_t* bound_total,
                          uint64_t* pair_total, void* stream) {
   GAT_REQUIRE(sp && sp->npoints >= 0 && sp->pid_bits >= 0 && sp->pid_bits < 32, "gat_leaf_refine_runs: bad store");
@@ -1312,7 +1298,7 @@ int gat_leaf_refine_runs(const gat_point_store* sp, const uint32_t* items, const
               "gat_leaf_refine_runs: bad sizes");
   hipStream_t s = gat::as_stream(stream);
   if (items_capacity > 0 && nqueries > 0) {
-    GAT_REQUIRE(items && qlist && qcount && pairs && runs && item_base && tiles,
+    GAT_REQUIRE(items && qlist && boxes && qcount && pairs && runs && item_base && tiles,
                 "gat_leaf_refine_runs: null pointer");
     // Regions: the exclusive scan of the items' bounds; *bound_total = their sum.
     if (int rc = launch_scan(LoadItemBounds{reinterpret_cast<const Item*>(items), item_total,
@@ -1324,16 +1310,16 @@ int gat_leaf_refine_runs(const gat_point_store* sp, const uint32_t* items, const
     const int blocks = grid_for(uint64_t(items_capacity), kRefBlock / kWave, 16);
     const dim3 grid{uint32_t(blocks)}, block{kRefBlock};
     auto* I = reinterpret_cast<const Item*>(items);
-    auto* L = static_cast<const ListEntry*>(qlist);
+    auto* B = reinterpret_cast<const float4*>(boxes);
     auto* P = reinterpret_cast<int2*>(pairs);
     const RunOut o{qcount, reinterpret_cast<Run*>(runs), uint64_t(runs_capacity), item_base};
     if (sp->pid_bits > 0)
-      leaf_refine_runs_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
-                                                           uint64_t(qlist_capacity), uint32_t(nqueries), P,
+      leaf_refine_runs_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
+                                                           uint64_t(qlist_capacity), B, uint32_t(nqueries), P,
                                                            uint64_t(capacity), o);
     else
-      leaf_refine_runs_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), L,
-                                                            uint64_t(qlist_capacity), uint32_t(nqueries), P,
+      leaf_refine_runs_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
+                                                            uint64_t(qlist_capacity), B, uint32_t(nqueries), P,
                                                             uint64_t(capacity), o);
     GAT_LAUNCH_CHECK("leaf_refine_runs_kernel");
   }

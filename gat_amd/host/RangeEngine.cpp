@@ -541,7 +541,7 @@ int RangeEngine::reserveBatched(const DeviceTree& tree, int maxQueries, int64_t 
   itemsCap_ = std::max(itemsCap_, entries);
   pairsCap_ = std::max(pairsCap_, std::max<int64_t>(maxCandidates, 1024));
   if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
-  if (int rc = qlist_.reserve(size_t(qlistCap_) * 32)) return rc;
+  if (int rc = qlist_.reserve(size_t(qlistCap_) * 4)) return rc;
   if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
   if (int rc = directTasks_.reserve(size_t(lqCap_) * sizeof(gat_range_task))) return rc;
   if (int rc = out.runs.reserve(size_t(qlistCap_) * sizeof(gat_range_run))) return rc;
@@ -701,23 +701,22 @@ int RangeEngine::enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out) {
   uint32_t* ctr = counters_.as<uint32_t>();
   GATH_TRY(gat_leaf_scatter(lq_.as<uint32_t>(), reinterpret_cast<const uint64_t*>(ctr), lqCap_,
                             tree.nodes.as<gat_tree_node>(), N, nodeOff_.as<uint32_t>(), nodeCount_.as<uint32_t>(),
-                            nodeFill_.as<uint32_t>(), queriesPerItem, devBoxes_.as<float>(),
-                            out.qbase.as<uint64_t>(), nq, qlist_.get(), qlistCap_, items_.as<uint32_t>(), itemsCap_,
-                            s),
+                            nodeFill_.as<uint32_t>(), queriesPerItem, nq, qlist_.as<uint32_t>(), qlistCap_,
+                            items_.as<uint32_t>(), itemsCap_, s),
            "leaf scatter");
   if (timeRefine_) {
     if (int rc = refineTimer_.start(s)) return rc;
   }
   if (runIndexed) {
-    GATH_TRY(gat_leaf_refine_runs(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.get(), qlistCap_, nq,
-                                  out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
+    GATH_TRY(gat_leaf_refine_runs(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.as<uint32_t>(),
+                                  qlistCap_, devBoxes_.as<float>(), nq, out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
                                   out.runs.as<gat_range_run>(), int64_t(out.runs.size() / sizeof(gat_range_run)),
                                   itemBase_.as<uint64_t>(), itemTiles_.as<uint64_t>(),
                                   reinterpret_cast<uint64_t*>(ctr + 10), reinterpret_cast<uint64_t*>(ctr + 8), s),
              "leaf refine");
   } else {
-    GATH_TRY(gat_leaf_refine(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.get(), qlistCap_, nq,
-                             out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
+    GATH_TRY(gat_leaf_refine(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.as<uint32_t>(), qlistCap_,
+                             devBoxes_.as<float>(), out.qbase.as<uint64_t>(), nq, out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
                              reinterpret_cast<uint64_t*>(ctr + 8), s),
              "leaf refine");
   }
@@ -925,7 +924,7 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   grow(int64_t(ch[6]), itemsCap_);
   // Run-indexed regions round each work item up to a line: 15 pairs each at most.
   grow(int64_t(u64(2)) + (runIndexed ? 16 * int64_t(ch[6]) : 0), pairsCap_);
-  if (int rc = qlist_.reserve(size_t(qlistCap_) * 32)) return rc;
+  if (int rc = qlist_.reserve(size_t(qlistCap_) * 4)) return rc;
   if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
   if (int rc = out.pairs.reserve(size_t(pairsCap_) * 8)) return rc;
   if (int rc = out.runs.reserve(size_t(qlistCap_) * sizeof(gat_range_run))) return rc;

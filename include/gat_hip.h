@@ -395,20 +395,17 @@ int gat_morton_select(const uint32_t* cell_words_dev, int32_t total_level, int64
 int gat_leaf_tasks(const uint32_t* leaf_query_dev, const uint64_t* lq_total_dev, int64_t lq_capacity,
                    const gat_tree_node* nodes_dev, int64_t nnodes, gat_range_task* tasks_dev, void* stream);
 
-/* Groups the selected (leaf, query) entries by leaf into self-contained
- * records.  List entry r of leaf L (32 bytes: the rectangle's 4 floats, its
- * output region base as uint64, its id as uint32, 4 bytes padding) goes to
- * qlist[node_off[2*L] + r]; every qb-th entry starts a work item (4 uint32:
- * the leaf's store start and count, the item's first list entry, its number of
- * entries <= qb) at items[4 * (node_off[2*L+1] + r/qb)].  qbase_dev holds the
- * per-query region bases (gat_scan_u64 of the candidate counts).  node_fill
- * (nnodes) must start at zero; entries beyond lq_capacity are ignored; qb is
- * 1..64. */
+/* Groups the selected (leaf, query) entries by leaf.  List entry r of leaf L
+ * (the rectangle's id, one uint32) goes to qlist[node_off[2*L] + r]; every
+ * qb-th entry starts a work item (4 uint32: the leaf's store start and count,
+ * the item's first list entry, its number of entries <= qb) at
+ * items[4 * (node_off[2*L+1] + r/qb)].  node_fill (nnodes) must start at zero;
+ * entries beyond lq_capacity and ids >= nqueries are ignored; qb is 1..64. */
 int gat_leaf_scatter(const uint32_t* leaf_query_dev, const uint64_t* lq_total_dev, int64_t lq_capacity,
                      const gat_tree_node* nodes_dev, int64_t nnodes, const uint32_t* node_off_dev,
-                     const uint32_t* node_qcount_dev, uint32_t* node_fill_dev, int32_t qb, const float* boxes_dev,
-                     const uint64_t* qbase_dev, int64_t nqueries, void* qlist_dev, int64_t qlist_capacity,
-                     uint32_t* items_dev, int64_t items_capacity, void* stream);
+                     const uint32_t* node_qcount_dev, uint32_t* node_fill_dev, int32_t qb, int64_t nqueries,
+                     uint32_t* qlist_dev, int64_t qlist_capacity, uint32_t* items_dev, int64_t items_capacity,
+                     void* stream);
 
 /* A run of result pairs of the run-indexed refine: `count` pairs of query q
  * at pairs[first ...]. */
@@ -428,10 +425,12 @@ typedef struct gat_range_run {
  * the rectangle's run (query, hits, first pair) in runs_dev[e] (count 0 when
  * none; runs_capacity must cover the list total); qcount[q] counts query q's
  * pairs (must start at zero).  No atomic is shared between work items.
- * pair_total as gat_leaf_refine. */
+ * boxes_dev: the batch's rectangles (4 floats each) that the list's ids
+ * index.  pair_total as gat_leaf_refine. */
 int gat_leaf_refine_runs(const gat_point_store* store, const uint32_t* items_dev, const uint32_t* item_total_dev,
-                         int64_t items_capacity, const void* qlist_dev, int64_t qlist_capacity, int64_t nqueries,
-                         uint32_t* qcount_dev, int32_t* pairs_dev, int64_t capacity, gat_range_run* runs_dev,
+                         int64_t items_capacity, const uint32_t* qlist_dev, int64_t qlist_capacity,
+                         const float* boxes_dev, int64_t nqueries, uint32_t* qcount_dev, int32_t* pairs_dev,
+                         int64_t capacity, gat_range_run* runs_dev,
                          int64_t runs_capacity, uint64_t* item_base_dev, uint64_t* tiles_dev,
                          uint64_t* bound_total_dev, uint64_t* pair_total_dev, void* stream);
 
@@ -446,10 +445,13 @@ int gat_runs_digest(const int32_t* pairs_dev, int64_t npairs, const gat_range_ru
  * once and tested against each of the item's rectangles; hits go to query q's
  * region pairs[base ...] as (tid, pid), qcount[q] counting them (qcount must
  * start at zero).  *item_total_dev (totals[1] of gat_scan_leaf_lists) gives the
- * number of items.  When pair_total_dev is not null, the sum of qcount is
- * added to it afterwards. */
+ * number of items; the list's ids index boxes_dev (4 floats per rectangle)
+ * and qbase_dev (the per-query region bases, gat_scan_u64 of the candidate
+ * counts).  When pair_total_dev is not null, the sum of qcount is added to it
+ * afterwards. */
 int gat_leaf_refine(const gat_point_store* store, const uint32_t* items_dev, const uint32_t* item_total_dev,
-                    int64_t items_capacity, const void* qlist_dev, int64_t qlist_capacity, int64_t nqueries,
+                    int64_t items_capacity, const uint32_t* qlist_dev, int64_t qlist_capacity,
+                    const float* boxes_dev, const uint64_t* qbase_dev, int64_t nqueries,
                     uint32_t* qcount_dev, int32_t* pairs_dev, int64_t pairs_capacity, uint64_t* pair_total_dev,
                     void* stream);
 
