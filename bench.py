@@ -469,7 +469,8 @@ def bench_edr(G, W) -> dict:
     ids, dists, st = g.similarity_batched(queries, w.k, w.eps)
     topk_s = time.perf_counter() - t
     log(f"top-{w.k} batched: {queries.ntraj / topk_s:,.1f} queries/s ({1e3 * topk_s:.1f} ms for {queries.ntraj}; "
-        f"{st['pairs']:,} pairs evaluated, {st['pruned']:,} resolved by the mismatch table)")
+        f"{st['pairs']:,} pairs evaluated, {st['pruned']:,} resolved by the mismatch table; filter kernel "
+        f"{st['filter_ms']:.3f} ms, DP kernel {st['kernel_ms']:.3f} ms, host merge {st['host_ms']:.3f} ms)")
     return {"edr_plain_cells_per_s": float(f"{plain_rate:.4e}"),
             "edr_batched_cells_per_s": float(f"{batched_rate:.4e}"),
             "edr_batched_kernel_cells_per_s": float(f"{kernel_rate:.4e}"),

@@ -167,4 +167,7 @@ class Grid {
   std::vector<int32_t> trajLenHost_, queryLenHost_;
   std::vector<int32_t> ftable_;
   std::vector<std::vector<int32_t>> lengthIds_;  // per length: trajectory ids, ascending
+  // Per query length a: the index's trajectory lengths L ordered by their
+  // all-mismatch value F[min(a, L)][max(a, L)] (then by L), built on first use.
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> lengthOrder_;
 };

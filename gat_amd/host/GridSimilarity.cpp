@@ -83,6 +83,7 @@ int Grid::uploadTrajectories(const TrajectoryView& db) {
   if (int rc = dist_.reserve(std::max<size_t>(size_t(T) * 4, 4))) return rc;
   GATH_TRY(gat_stream_sync(s), "trajectory upload");
   lengthIds_.assign(GAT_MAX_TRAJ_LEN + 1, {});
+  lengthOrder_.assign(GAT_MAX_TRAJ_LEN + 1, {});
   for (int t = 0; t < T; ++t)
     if (trajLenHost_[t] >= 1 && trajLenHost_[t] <= GAT_MAX_TRAJ_LEN) lengthIds_[trajLenHost_[t]].push_back(t + 1);
   return GAT_OK;
@@ -217,8 +218,18 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   constexpr int kS = GAT_MAX_TRAJ_LEN / 16 + 1;
   constexpr int kBuckets = 3;
   constexpr int kMid = 1024;
+  constexpr int64_t kLatencyJobsPerSimd = 4;  // the batched kernel holds 4 waves per SIMD: one round
   const int split = std::max(1, std::min(edrSplit, GAT_MAX_TRAJ_LEN));
-  const int mid = std::max(split, kMid);
+  // A batch that fits one round of the device's waves (a top-k batch's
+  // survivors, a serving batch) ends when its longest pair does, and a pair
+  // runs its strips one after another: four pairs per wave on 16-lane strips
+  // take a 1000 x 1000 pair through 63 strips, one pair per wave on 64-lane
+  // strips through 16.  Such batches send every pair longer than the split to
+  // the one-pair-per-wave bucket.
+  int64_t props[5] = {0, 0, 0, 0, 0};
+  const int64_t cus = gat_device_props(props) == GAT_OK && props[0] > 0 ? props[0] : 256;
+  const bool latencyBound = (np + 3) / 4 < cus * 4 * kLatencyJobsPerSimd;
+  const int mid = latencyBound ? split : std::max(split, kMid);
   std::vector<int32_t> key(static_cast<size_t>(np));
   int64_t cells = 0;
   int maxlen[kBuckets] = {1, 1, 1};
@@ -376,6 +387,28 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
   for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L)
     if (!lengthIds_[L].empty()) lens.push_back(L);
   const int w = GAT_MAX_TRAJ_LEN + 1;
+  // The length order of every query length of the batch not seen before (the
+  // all-mismatch values do not follow the length difference monotonically, so
+  // each query length needs its own sort; it is kept for later batches).
+  {
+    std::vector<int> fresh;
+    std::vector<char> seen(size_t(w), 0);
+    for (int q = 0; q < nq; ++q) {
+      const int a = queryLenHost_[q];
+      if (a >= 0 && a < w && !seen[size_t(a)] && lengthOrder_[size_t(a)].empty()) {
+        seen[size_t(a)] = 1;
+        fresh.push_back(a);
+      }
+    }
+    parallel_for(int(fresh.size()), threads(), [&](int i) {
+      const int a = fresh[size_t(i)];
+      std::vector<std::pair<int32_t, int32_t>> byF;
+      byF.reserve(lens.size());
+      for (int L : lens) byF.push_back({F[size_t(std::min(a, L)) * w + std::max(a, L)], L});
+      std::sort(byF.begin(), byF.end());
+      lengthOrder_[size_t(a)] = std::move(byF);
+    });
+  }
   out.assign(size_t(nq), TopK{});
   parallel_for(nq, threads(), [&](int q) {
     auto sb = surv.begin() + start[q], se = surv.begin() + start[q + 1];
@@ -384,11 +417,7 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
     excluded.reserve(size_t(se - sb));
     for (auto it = sb; it != se; ++it) excluded.push_back(it->second);
     std::sort(excluded.begin(), excluded.end());
-    const int a = queryLenHost_[q];
-    std::vector<std::pair<int32_t, int32_t>> byF;  // (all-mismatch value, length)
-    byF.reserve(lens.size());
-    for (int L : lens) byF.push_back({F[size_t(std::min(a, L)) * w + std::max(a, L)], L});
-    std::sort(byF.begin(), byF.end());
+    const std::vector<std::pair<int32_t, int32_t>>& byF = lengthOrder_[size_t(queryLenHost_[q])];
     const size_t need = size_t(kValue);  // the k best non-survivors suffice
     std::vector<std::pair<int32_t, int32_t>> rest;  // (dist, id), ascending
     rest.reserve(need);
