@@ -215,7 +215,8 @@ def main() -> int:
         f"{msame}")
     del mgrid
 
-    edr = bench_edr(G, W)
+    edr = bench_edr(G, W, cpu_topk=not args.no_cpu_baseline)
+    topk_cpu = edr.pop("topk_cpu")
     del grid
     serve = bench_serve(G, W, args.serve_ntraj, args.serve_seconds)
 
@@ -255,6 +256,15 @@ def main() -> int:
         cpu["edr_cells_per_s"] = {"1_thread": float(f"{rates[1]:.4e}"), "16_threads": float(f"{rates[16]:.4e}"),
                                   "sample": f"one 128-point query x 20k candidates ({cells3:,} cells)"}
         log(f"CPU EDR: {rates[1]:.3e} cells/s (1 thread), {rates[16]:.3e} (16 threads)")
+        cpu["topk"] = topk_cpu
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
+        cpu["host"] = {"model": model, "nproc": os.cpu_count(),
+                       "threads_note": "pools capped at 16 threads, the one-GPU share of the host's cores"}
 
     line = {
         "metric": METRIC,
@@ -431,11 +441,14 @@ def bench_hbm(G) -> dict:
     return out
 
 
-def bench_edr(G, W) -> dict:
+def bench_edr(G, W, cpu_topk: bool = False) -> dict:
     """EDR cell-update rates over one pair set (4 queries against the 200k
     candidates of the top-k workload, both query kinds): plain = one launch per
     query over every candidate, batched = the persistent kernel over the same
-    800k pairs.  Then the exact top-50 of the 1024-query batch."""
+    800k pairs.  Then the exact top-50 of the 1024-query batch; with cpu_topk,
+    two of its queries answered on the CPU by brute force (the oracle's EDR
+    over all 200k candidates on 16 threads, then the top 50 by (distance, id))
+    and compared with the GPU's answers."""
     w = W.CFG3
     t = time.perf_counter()
     db = w.db()
@@ -471,6 +484,27 @@ def bench_edr(G, W) -> dict:
     log(f"top-{w.k} batched: {queries.ntraj / topk_s:,.1f} queries/s ({1e3 * topk_s:.1f} ms for {queries.ntraj}; "
         f"{st['pairs']:,} pairs evaluated, {st['pruned']:,} resolved by the mismatch table; filter kernel "
         f"{st['filter_ms']:.3f} ms, DP kernel {st['kernel_ms']:.3f} ms, host merge {st['host_ms']:.3f} ms)")
+    cpu = None
+    if cpu_topk:
+        import oracle as O
+        order = np.argsort(qlen, kind="stable")
+        sample = [int(order[len(order) // 4]), int(order[len(order) // 2])]  # lengths near the 25th and 50th percentile
+        all_ids = np.arange(1, db.ntraj + 1, dtype=np.int32)
+        ok = True
+        t = time.perf_counter()
+        for qi in sample:
+            d = O.edr_batch(queries.xy(qi + 1), db, w.eps, nthreads=16)
+            oi, od = O.topk(d, all_ids, w.k)
+            ok = ok and bool((oi == ids[qi][:len(oi)]).all() and (od == dists[qi][:len(od)]).all())
+        cpu_s = time.perf_counter() - t
+        cells = int(db.lengths.astype(np.int64).sum()) * int(qlen[sample].sum())
+        cpu = {"value": round(len(sample) / cpu_s, 4), "unit": "queries/s", "cores": 16,
+               "sample": f"queries {sample} (lengths {qlen[sample].tolist()}) of the 1024-query batch, exact "
+                         f"top-{w.k} by brute force over all {db.ntraj:,} candidates ({cells:.3e} cells; the "
+                         f"reference's SimilarityQueryBatchCPUParallel prunes with its frequency queue first)",
+               "equals_gpu": ok}
+        log(f"CPU top-{w.k}: {cpu['value']} queries/s on 16 threads ({cells / cpu_s:.3e} cells/s), GPU answers "
+            f"identical: {ok}")
     return {"edr_plain_cells_per_s": float(f"{plain_rate:.4e}"),
             "edr_batched_cells_per_s": float(f"{batched_rate:.4e}"),
             "edr_batched_kernel_cells_per_s": float(f"{kernel_rate:.4e}"),
@@ -478,7 +512,8 @@ def bench_edr(G, W) -> dict:
             "edr_batched_equals_plain": same,
             "topk_batched_qps": round(queries.ntraj / topk_s, 2),
             "topk_pairs_evaluated": int(st["pairs"]),
-            "edr_autotuned": tuned}
+            "edr_autotuned": tuned,
+            "topk_cpu": cpu}
 
 
 if __name__ == "__main__":
