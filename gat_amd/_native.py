@@ -89,7 +89,9 @@ HIP_FUNCS = {
     "gat_kd_store_build": (INT, [P, P, P, I32, I64, I32, P, P, P, I32, P, P, I64, P, P, P, P, P, P]),
     "gat_leaf_scatter": (INT, [P, P, I64, P, I64, P, P, P, I32, I64, P, I64, P, I64, P]),
     "gat_leaf_refine": (INT, [P, P, P, I64, P, I64, P, P, I64, P, P, I64, P, P]),
-    "gat_leaf_refine_runs": (INT, [P, P, P, I64, P, I64, P, I64, P, P, I64, P, I64, P, P, P, P, P]),
+    "gat_leaf_item_regions": (INT, [P, P, I64, P, P, P, P]),
+    "gat_leaf_refine_runs": (INT, [P, P, P, I64, P, I64, P, I64, P, I64, P, I64, P, P]),
+    "gat_run_counts": (INT, [P, P, P, I64, I64, P, I64, I64, P, P, P]),
     "gat_runs_digest": (INT, [P, I64, P, I64, I64, P, P]),
     "gat_graph_begin": (INT, [P]),
     "gat_graph_end": (INT, [P, P]),

@@ -1334,44 +1334,62 @@ int gat_leaf_refine(const gat_point_store* sp, const uint32_t* items, const uint
   return GAT_OK;
 }
 
+int gat_leaf_item_regions(const uint32_t* items, const uint32_t* item_total, int64_t items_capacity,
+                          uint64_t* item_base, uint64_t* tiles, uint64_t* bound_total, void* stream) {
+  GAT_REQUIRE(item_total && bound_total && items_capacity >= 0, "gat_leaf_item_regions: bad arguments");
+  if (items_capacity == 0) return GAT_OK;
+  GAT_REQUIRE(items && item_base && tiles, "gat_leaf_item_regions: null pointer");
+  // The exclusive scan of the items' bounds; *bound_total = their sum.
+  return launch_scan(LoadItemBounds{reinterpret_cast<const Item*>(items), item_total, uint64_t(items_capacity)},
+                     uint64_t(items_capacity), tiles, item_base, bound_total, gat::as_stream(stream));
+}
+
 int gat_leaf_refine_runs(const gat_point_store* sp, const uint32_t* items, const uint32_t* item_total,
                          int64_t items_capacity, const uint32_t* qlist, int64_t qlist_capacity, const float* boxes,
-                         int64_t nqueries, uint32_t* qcount, int32_t* pairs, int64_t capacity, gat_range_run* runs,
-                         int64_t runs_capacity, uint64_t* item_base, uint64_t* tiles, uint64_t* bound_total,
-                         uint64_t* pair_total, void* stream) {
+                         int64_t nqueries, int32_t* pairs, int64_t capacity, gat_range_run* runs,
+                         int64_t runs_capacity, const uint64_t* item_base, void* stream) {
   GAT_REQUIRE(sp && sp->npoints >= 0 && sp->pid_bits >= 0 && sp->pid_bits < 32, "gat_leaf_refine_runs: bad store");
   GAT_REQUIRE(sp->npoints == 0 || (sp->xy_dev && sp->id_dev && (sp->pid_bits > 0 || sp->pid_dev)),
               "gat_leaf_refine_runs: null store array");
-  GAT_REQUIRE(item_total && bound_total && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX) &&
-                  items_capacity >= 0 && qlist_capacity >= 0 && capacity >= 0 && runs_capacity >= 0,
+  GAT_REQUIRE(item_total && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX) && items_capacity >= 0 &&
+                  qlist_capacity >= 0 && capacity >= 0 && runs_capacity >= 0,
               "gat_leaf_refine_runs: bad sizes");
+  if (items_capacity == 0 || nqueries == 0) return GAT_OK;
+  GAT_REQUIRE(items && qlist && boxes && pairs && runs && item_base, "gat_leaf_refine_runs: null pointer");
   hipStream_t s = gat::as_stream(stream);
-  if (items_capacity > 0 && nqueries > 0) {
-    GAT_REQUIRE(items && qlist && boxes && qcount && pairs && runs && item_base && tiles,
-                "gat_leaf_refine_runs: null pointer");
-    // Regions: the exclusive scan of the items' bounds; *bound_total = their sum.
-    if (int rc = launch_scan(LoadItemBounds{reinterpret_cast<const Item*>(items), item_total,
-                                            uint64_t(items_capacity)},
-                             uint64_t(items_capacity), tiles, item_base, bound_total, s))
-      return rc;
-    const Store st{reinterpret_cast<const float2*>(sp->xy_dev), sp->id_dev, sp->pid_dev, uint64_t(sp->npoints),
-                   uint32_t(sp->pid_bits)};
-    const int blocks = grid_for(uint64_t(items_capacity), kRefBlock / kWave, 16);
-    const dim3 grid{uint32_t(blocks)}, block{kRefBlock};
-    auto* I = reinterpret_cast<const Item*>(items);
-    auto* B = reinterpret_cast<const float4*>(boxes);
-    auto* P = reinterpret_cast<int2*>(pairs);
-    const RunOut o{reinterpret_cast<Run*>(runs), uint64_t(runs_capacity), item_base};
-    if (sp->pid_bits > 0)
-      leaf_refine_runs_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
-                                                           uint64_t(qlist_capacity), B, uint32_t(nqueries), P,
-                                                           uint64_t(capacity), o);
-    else
-      leaf_refine_runs_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
-                                                            uint64_t(qlist_capacity), B, uint32_t(nqueries), P,
-                                                            uint64_t(capacity), o);
-    GAT_LAUNCH_CHECK("leaf_refine_runs_kernel");
+  const Store st{reinterpret_cast<const float2*>(sp->xy_dev), sp->id_dev, sp->pid_dev, uint64_t(sp->npoints),
+                 uint32_t(sp->pid_bits)};
+  const int blocks = grid_for(uint64_t(items_capacity), kRefBlock / kWave, 16);
+  const dim3 grid{uint32_t(blocks)}, block{kRefBlock};
+  auto* I = reinterpret_cast<const Item*>(items);
+  auto* B = reinterpret_cast<const float4*>(boxes);
+  auto* P = reinterpret_cast<int2*>(pairs);
+  const RunOut o{reinterpret_cast<Run*>(runs), uint64_t(runs_capacity), item_base};
+  if (sp->pid_bits > 0)
+    leaf_refine_runs_kernel<true><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
+                                                         uint64_t(qlist_capacity), B, uint32_t(nqueries), P,
+                                                         uint64_t(capacity), o);
+  else
+    leaf_refine_runs_kernel<false><<<grid, block, 0, s>>>(st, I, item_total, uint64_t(items_capacity), qlist,
+                                                          uint64_t(qlist_capacity), B, uint32_t(nqueries), P,
+                                                          uint64_t(capacity), o);
+  GAT_LAUNCH_CHECK("leaf_refine_runs_kernel");
+  return GAT_OK;
+}
+
+int gat_run_counts(const gat_point_store* sp, const uint32_t* items, const uint32_t* item_total,
+                   int64_t items_capacity, int64_t qlist_capacity, const gat_range_run* runs, int64_t runs_capacity,
+                   int64_t nqueries, uint32_t* qcount, uint64_t* pair_total, void* stream) {
+  GAT_REQUIRE(sp && sp->npoints >= 0 && item_total && items_capacity >= 0 && qlist_capacity >= 0 &&
+                  runs_capacity >= 0 && nqueries >= 0 && nqueries <= int64_t(UINT32_MAX),
+              "gat_run_counts: bad arguments");
+  if (nqueries == 0) return GAT_OK;
+  GAT_REQUIRE(qcount, "gat_run_counts: null counts");
+  hipStream_t s = gat::as_stream(stream);
+  if (items_capacity > 0) {
+    GAT_REQUIRE(items && runs, "gat_run_counts: null pointer");
     const uint64_t list_cap = uint64_t(qlist_capacity < runs_capacity ? qlist_capacity : runs_capacity);
+    auto* I = reinterpret_cast<const Item*>(items);
     const auto* R = reinterpret_cast<const Run*>(runs);
     const dim3 cgrid{uint32_t(grid_for(uint64_t(items_capacity), 256, 2))};
     if (uint64_t(nqueries) <= kCountLdsQueries)
@@ -1383,8 +1401,7 @@ int gat_leaf_refine_runs(const gat_point_store* sp, const uint32_t* items, const
                                                          qcount);
     GAT_LAUNCH_CHECK("run_counts_kernel");
   }
-  if (pair_total && nqueries > 0) {
-    GAT_REQUIRE(qcount, "gat_leaf_refine_runs: null counts");
+  if (pair_total) {
     sum_u32_kernel<<<dim3{uint32_t(grid_for(uint64_t(nqueries), 256, 1))}, dim3{256}, 0, s>>>(
         qcount, uint64_t(nqueries), reinterpret_cast<unsigned long long*>(pair_total));
     GAT_LAUNCH_CHECK("sum_u32_kernel");

@@ -704,26 +704,34 @@ int RangeEngine::enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out) {
                             nodeFill_.as<uint32_t>(), queriesPerItem, nq, qlist_.as<uint32_t>(), qlistCap_,
                             items_.as<uint32_t>(), itemsCap_, s),
            "leaf scatter");
-  if (timeRefine_) {
-    if (int rc = refineTimer_.start(s)) return rc;
-  }
   if (runIndexed) {
+    GATH_TRY(gat_leaf_item_regions(items_.as<uint32_t>(), ctr + 6, itemsCap_, itemBase_.as<uint64_t>(),
+                                   itemTiles_.as<uint64_t>(), reinterpret_cast<uint64_t*>(ctr + 10), s),
+             "item regions");
+    if (timeRefine_)
+      if (int rc = refineTimer_.start(s)) return rc;
     GATH_TRY(gat_leaf_refine_runs(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.as<uint32_t>(),
-                                  qlistCap_, devBoxes_.as<float>(), nq, out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
+                                  qlistCap_, devBoxes_.as<float>(), nq, out.pairs.as<int32_t>(), pairsCap_,
                                   out.runs.as<gat_range_run>(), int64_t(out.runs.size() / sizeof(gat_range_run)),
-                                  itemBase_.as<uint64_t>(), itemTiles_.as<uint64_t>(),
-                                  reinterpret_cast<uint64_t*>(ctr + 10), reinterpret_cast<uint64_t*>(ctr + 8), s),
+                                  itemBase_.as<uint64_t>(), s),
              "leaf refine");
+    if (timeRefine_)
+      if (int rc = refineTimer_.stop(s)) return rc;
+    GATH_TRY(gat_run_counts(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlistCap_,
+                            out.runs.as<gat_range_run>(), int64_t(out.runs.size() / sizeof(gat_range_run)), nq,
+                            out.qcount.as<uint32_t>(), reinterpret_cast<uint64_t*>(ctr + 8), s),
+             "run counts");
   } else {
+    if (timeRefine_)
+      if (int rc = refineTimer_.start(s)) return rc;
     GATH_TRY(gat_leaf_refine(&store_, items_.as<uint32_t>(), ctr + 6, itemsCap_, qlist_.as<uint32_t>(), qlistCap_,
-                             devBoxes_.as<float>(), out.qbase.as<uint64_t>(), nq, out.qcount.as<uint32_t>(), out.pairs.as<int32_t>(), pairsCap_,
-                             reinterpret_cast<uint64_t*>(ctr + 8), s),
+                             devBoxes_.as<float>(), out.qbase.as<uint64_t>(), nq, out.qcount.as<uint32_t>(),
+                             out.pairs.as<int32_t>(), pairsCap_, reinterpret_cast<uint64_t*>(ctr + 8), s),
              "leaf refine");
+    if (timeRefine_)
+      if (int rc = refineTimer_.stop(s)) return rc;
   }
   out.runIndexed = runIndexed;
-  if (timeRefine_) {
-    if (int rc = refineTimer_.stop(s)) return rc;
-  }
   return GAT_OK;
 }
 

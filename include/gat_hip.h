@@ -416,23 +416,35 @@ typedef struct gat_range_run {
 } gat_range_run;
 
 /* The leaf-major refine with the output laid out by work item (no reference
- * counterpart): work item t owns the region [item_base[t], item_base[t+1]) of
- * `pairs`, an exclusive scan of its bound (rectangles x leaf points, rounded
- * up to 16 pairs) written to item_base_dev (items_capacity entries; tiles_dev
- * holds gat_scan_tiles(items_capacity) scratch entries; *bound_total_dev
- * receives the sum, which must not exceed `capacity`).  An item's hits fill its
- * region from the start, rectangle after rectangle; its list entry e records
- * the rectangle's run (query, hits, first pair) in runs_dev[e] (count 0 when
- * none; runs_capacity must cover the list total); qcount[q] counts query q's
- * pairs (must start at zero).  No atomic is shared between work items.
- * boxes_dev: the batch's rectangles (4 floats each) that the list's ids
- * index.  pair_total as gat_leaf_refine. */
+ * counterpart), in three calls on one stream:
+ *
+ * gat_leaf_item_regions: work item t's region of `pairs` is
+ * [item_base[t], item_base[t+1]), an exclusive scan of its bound (rectangles x
+ * leaf points, rounded up to 16 pairs) written to item_base_dev
+ * (items_capacity entries; tiles_dev holds gat_scan_tiles(items_capacity)
+ * scratch entries); *bound_total_dev receives the sum, which must not exceed
+ * the pair capacity of the refine.
+ *
+ * gat_leaf_refine_runs: an item's hits fill its region from the start,
+ * rectangle after rectangle; its list entry e records the rectangle's run
+ * (query, hits, first pair) in runs_dev[e] (count 0 when none; runs_capacity
+ * must cover the list total).  boxes_dev: the batch's rectangles (4 floats
+ * each) that the list's ids index.  No atomics.
+ *
+ * gat_run_counts: qcount[q] = the pairs of query q summed from the run
+ * records (qcount must start at zero); when pair_total_dev is not null, the
+ * sum of qcount is added to it.  Items the refine skipped as malformed are
+ * skipped here by the same checks. */
+int gat_leaf_item_regions(const uint32_t* items_dev, const uint32_t* item_total_dev, int64_t items_capacity,
+                          uint64_t* item_base_dev, uint64_t* tiles_dev, uint64_t* bound_total_dev, void* stream);
 int gat_leaf_refine_runs(const gat_point_store* store, const uint32_t* items_dev, const uint32_t* item_total_dev,
                          int64_t items_capacity, const uint32_t* qlist_dev, int64_t qlist_capacity,
-                         const float* boxes_dev, int64_t nqueries, uint32_t* qcount_dev, int32_t* pairs_dev,
-                         int64_t capacity, gat_range_run* runs_dev,
-                         int64_t runs_capacity, uint64_t* item_base_dev, uint64_t* tiles_dev,
-                         uint64_t* bound_total_dev, uint64_t* pair_total_dev, void* stream);
+                         const float* boxes_dev, int64_t nqueries, int32_t* pairs_dev, int64_t capacity,
+                         gat_range_run* runs_dev, int64_t runs_capacity, const uint64_t* item_base_dev, void* stream);
+int gat_run_counts(const gat_point_store* store, const uint32_t* items_dev, const uint32_t* item_total_dev,
+                   int64_t items_capacity, int64_t qlist_capacity, const gat_range_run* runs_dev,
+                   int64_t runs_capacity, int64_t nqueries, uint32_t* qcount_dev, uint64_t* pair_total_dev,
+                   void* stream);
 
 /* Per-query digests of run-indexed pairs (the same value gat_pairs_digest gives
  * for query-major regions): out_dev[q] = wrapping sum of the pair mix over
