@@ -302,3 +302,59 @@ def test_plain_after_batched_keeps_its_layout(cfg1):
         _check_digests(g.range_pairs(edge), counts, dig)
     g.range_batched(rects)
     _check_pairs(g.range_pairs(rects), _sets(og.range(rects)), rects)
+
+
+@pytest.mark.parametrize("nqueries", [3, 20000])
+def test_run_indexed_abi_skips_malformed_items(gpu, nqueries):
+    """The run-indexed join through the C ABI on hand-built work items, three
+    of them malformed (a run past the store, no rectangles, a list slice past
+    the list), which the refine must skip and the count pass must skip too:
+    their run records hold garbage that must not reach the counts.  Both count
+    paths: LDS-aggregated (3 rectangles) and direct (20,000)."""
+    import ctypes
+    from gat_amd import _native as N
+    from tests.devbuf import DevBuf
+    h = N.hip()
+    xy = np.array([[0.1, 0.1], [0.2, 0.2], [0.3, 0.3], [0.9, 0.9], [0.5, 0.5], [0.6, 0.6]], np.float32)
+    tid = np.array([1, 1, 2, 3, 4, 4], np.uint32)
+    pid = np.array([0, 1, 0, 0, 0, 1], np.uint32)
+    dxy, did = DevBuf.of(xy), DevBuf.of((tid << 3) | pid)
+    store = N.PointStore(dxy.ptr, did.ptr, None, 6, 3)
+    sp = ctypes.byref(store)
+    boxes = np.zeros((nqueries, 4), np.float32)
+    boxes[:3] = [[0, 0, 0.25, 0.25], [0.25, 0.25, 1, 1], [0.55, 0.55, 0.7, 0.7]]
+    qlist = np.array([0, 1, 2, 0, 1, 0], np.uint32)
+    items = np.array([[0, 4, 0, 2],    # rectangles 0 and 1 over points 0..3
+                      [4, 2, 2, 1],    # rectangle 2 over points 4..5
+                      [5, 4, 3, 1],    # run past the store
+                      [0, 2, 4, 0],    # no rectangles
+                      [0, 2, 5, 3]],   # list slice past the list
+                     np.uint32)
+    dboxes, dq, ditems = DevBuf.of(boxes), DevBuf.of(qlist), DevBuf.of(items)
+    total = DevBuf.of(np.array([5], np.uint32))
+    base, bound = DevBuf(5 * 8).zero(), DevBuf(8).zero()
+    tiles = DevBuf(8 * int(h.gat_scan_tiles(5))).zero()
+    N.check_hip(h.gat_leaf_item_regions(ditems.ptr, total.ptr, 5, base.ptr, tiles.ptr, bound.ptr, None))
+    # Regions: rectangles x points rounded up to 16 pairs, malformed items too.
+    assert base.get(np.uint64, 5).tolist() == [0, 16, 32, 48, 48] and bound.get(np.uint64, 1)[0] == 64
+    garbage = np.zeros(6, api.RUN)
+    garbage["q"], garbage["count"] = 1, 77
+    runs, pairs = DevBuf.of(garbage), DevBuf(64 * 8).zero()
+    N.check_hip(h.gat_leaf_refine_runs(sp, ditems.ptr, total.ptr, 5, dq.ptr, 6, dboxes.ptr, nqueries, pairs.ptr, 64,
+                                       runs.ptr, 6, base.ptr, None))
+    got = runs.get(api.RUN, 6)
+    assert [(int(r["q"]), int(r["count"]), int(r["first"])) for r in got[:3]] == [(0, 2, 0), (1, 2, 2), (2, 1, 16)]
+    assert (got["count"][3:] == 77).all()  # entries of skipped items are left alone
+    out = pairs.get(np.int32, 128).reshape(64, 2)
+    assert set(map(tuple, out[0:2].tolist())) == {(1, 0), (1, 1)}
+    assert set(map(tuple, out[2:4].tolist())) == {(2, 0), (3, 0)}
+    assert tuple(out[16]) == (4, 1)
+    qcount, ptotal = DevBuf(4 * nqueries).zero(), DevBuf(8).zero()
+    N.check_hip(h.gat_run_counts(sp, ditems.ptr, total.ptr, 5, 6, runs.ptr, 6, nqueries, qcount.ptr, ptotal.ptr, None))
+    counts = qcount.get(np.uint32, nqueries)
+    assert counts[:3].tolist() == [2, 2, 1] and not counts[3:].any()
+    assert ptotal.get(np.uint64, 1)[0] == 5
+    # Argument errors come back as a status, not a crash.
+    assert h.gat_run_counts(sp, ditems.ptr, total.ptr, 5, 6, runs.ptr, 6, nqueries, None, None, None) == 2
+    assert h.gat_leaf_refine_runs(sp, None, total.ptr, 5, dq.ptr, 6, dboxes.ptr, nqueries, pairs.ptr, 64, runs.ptr,
+                                  6, base.ptr, None) == 2
