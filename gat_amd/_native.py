@@ -112,6 +112,7 @@ HOST_FUNCS = {
     "gath_synth_queries": (None, [U64, INT, P, P, P, INT, P, P, INT, F64, F64, P, P]),
     "gath_grid_new": (P, [P, F32, INT]),
     "gath_morton_grid_new": (P, [P, F32, INT]),
+    "gath_grid_set_edr_waves": (INT, [P, INT]),
     "gath_system_range_test": (INT, [P, INT, P, INT, P]),
     "gath_fsg_new": (P, [P, F32]),
     "gath_stig_new": (P, [INT]),
@@ -152,6 +153,7 @@ HOST_FUNCS = {
     "gath_grid_edr_all": (INT, [P, P, P, INT, F64, P]),
     "gath_grid_similarity": (INT, [P, P, INT, P, P, INT, F64, P, P]),
     "gath_grid_edr_batched_all": (INT, [P, P, INT, P, P, F64, P, P, P]),
+    "gath_grid_edr_batched_pairs": (INT, [P, P, INT, P, P, P, I64, F64, P, P, P]),
     "gath_grid_similarity_batched": (INT, [P, P, INT, P, P, INT, F64, P, P, P, P]),
     "gath_grid_mismatch_table": (INT, [P, P]),
 }

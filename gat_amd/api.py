@@ -469,12 +469,18 @@ class GridIndex(_Index):
     def tune_edr(self, queries: TrajSet, eps: float, reps: int = 2) -> dict:
         """Autotunes batched EDR (bucket split, workgroups per CU of each
         bucket) on every pair of `queries` x the index's trajectories."""
-        out = np.zeros(5, np.float64)
+        out = np.zeros(7, np.float64)
         N.check_host(N.host().gath_tune_edr(self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj,
                                             _ptr(_req(queries.lon, np.float32)), _ptr(_req(queries.lat, np.float32)),
                                             eps, int(reps), _ptr(out)))
         return {"split": int(out[0]), "per_cu_short": int(out[1]), "per_cu_long": int(out[2]),
-                "kernel_ms": round(float(out[3]), 4), "evaluations": int(out[4])}
+                "kernel_ms": round(float(out[3]), 4), "evaluations": int(out[4]),
+                "waves_per_long_pair": int(out[5]), "one_round_ms": round(float(out[6]), 4)}
+
+    def set_edr_waves(self, waves_per_pair: int) -> "GridIndex":
+        """Waves per long pair (1, 2, 4, 8) for batches that fit one round."""
+        N.check_host(N.host().gath_grid_set_edr_waves(self.handle, int(waves_per_pair)))
+        return self
 
     def edr_all(self, q_xy: np.ndarray, eps: float) -> np.ndarray:
         q = _req(q_xy, np.float32).reshape(-1, 2)
@@ -503,6 +509,18 @@ class GridIndex(_Index):
             self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj, _ptr(_req(queries.lon, np.float32)),
             _ptr(_req(queries.lat, np.float32)), eps, _ptr(dist), _ptr(stats), _ptr(ms)))
         return dist, _edr_stats(stats, ms)
+
+    def edr_batched_pairs(self, queries: TrajSet, pairs: np.ndarray, eps: float) -> tuple[np.ndarray, dict]:
+        """Distances of given (query index, trajectory index) pairs, both
+        0-based, (n, 2) int32, from the batched kernels."""
+        p = _req(pairs, np.int32).reshape(-1, 2)
+        dist = np.empty(max(len(p), 1), np.int32)
+        stats = np.zeros(5, np.int64)
+        ms = np.zeros(3, np.float64)
+        N.check_host(N.host().gath_grid_edr_batched_pairs(
+            self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj, _ptr(_req(queries.lon, np.float32)),
+            _ptr(_req(queries.lat, np.float32)), _ptr(p), len(p), eps, _ptr(dist), _ptr(stats), _ptr(ms)))
+        return dist[:len(p)], _edr_stats(stats, ms)
 
     def similarity_batched(self, queries: TrajSet, k: int, eps: float) -> tuple[np.ndarray, np.ndarray, dict]:
         """Exact top-k (ascending distance, then id) of every query: the candidate

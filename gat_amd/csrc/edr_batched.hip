@@ -243,6 +243,160 @@ __global__ __launch_bounds__(64, (MAXM >= 2048 && SEG == 16) ? 2 : 4) void edr_w
   }
 }
 
+// One pair over W waves of a workgroup (no reference counterpart): wave w
+// runs the pair's 64-row strips w, w + W, ... in lockstep with the others,
+// one block of 16 steps per iteration and a workgroup barrier between
+// iterations, so a pair's strips overlap instead of running one after another.
+// A strip reads the row above it from the previous strip's wave: that wave's
+// last lane writes its row, D + 1 per column, into the wave's own LDS row
+// buffer at step c + 62 for column c, and the consumer reads column c at step
+// c - 1 of its own strip (the feed of the block holding that step, loaded when
+// the block starts).  So a strip starts kLag >= 5 blocks after the previous
+// one (column 16b + 16, the last a consumer block b reads, is written in the
+// producer's block b + 4), and no sooner than its wave has finished its
+// previous strip — which also keeps a wave's writes to its buffer behind the
+// next wave's reads of the previous contents (those reads start kLag blocks
+// earlier on every column).  Every wave runs the same number of iterations,
+// so the barriers always match.  The cell rule, the row/column orientation
+// and the capture of D[n][m] are edr_wave_kernel's with 64-lane segments.
+constexpr int kLag = 6;
+
+template <int MAXM, int W>
+__global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict__ pairs, int64_t npairs,
+                                                          const Job* __restrict__ jobs, int32_t njobs,
+                                                          int32_t* __restrict__ counter, int32_t nqueries,
+                                                          int32_t ncands, const float2* __restrict__ qxy,
+                                                          const int64_t* __restrict__ qoff,
+                                                          const int32_t* __restrict__ qlen,
+                                                          const float2* __restrict__ cxy,
+                                                          const int64_t* __restrict__ coff,
+                                                          const int32_t* __restrict__ clen, float u,
+                                                          int32_t* __restrict__ out) {
+  // Columns -62 .. 0 sit before column 0 (lane 63's look-behind); a block's
+  // feed reads up to column 16 * nblocks <= m + 78 (the last block's overrun).
+  constexpr int kOff = 64;
+  constexpr int kLen = kOff + MAXM + 96;
+  __shared__ int16_t rows[W][kLen];
+  __shared__ int32_t job_slot;
+  const int wv = threadIdx.x / 64;
+  const int l = threadIdx.x & 63;
+  const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+  for (;;) {
+    if (threadIdx.x == 0) job_slot = atomicAdd(counter, 1);
+    __syncthreads();
+    const int ji = job_slot;
+    __syncthreads();  // every wave has read the slot before the next job overwrites it
+    if (ji >= njobs) break;
+    const Job job = jobs[ji];
+    // One pair per job; everything below is uniform across the workgroup.
+    const bool live = job.count >= 1 && job.first >= 0 && int64_t(job.first) < npairs;
+    int n = 0, m = 0;
+    const float2* R = cxy;
+    const float2* C = cxy + kPad;
+    bool valid = false;
+    if (live) {
+      const int2 pr = pairs[job.first];
+      const bool ids_ok = pr.x >= 0 && pr.x < nqueries && pr.y >= 0 && pr.y < ncands;
+      const int nq = ids_ok ? qlen[pr.x] : 0, nc = ids_ok ? clen[pr.y] : 0;
+      valid = nq >= 1 && nc >= 1 && nq <= MAXM && nc <= MAXM;
+      if (valid) {
+        if (nc >= nq) {
+          R = qxy + qoff[pr.x]; n = nq; C = cxy + coff[pr.y]; m = nc;
+        } else {
+          R = cxy + coff[pr.y]; n = nc; C = qxy + qoff[pr.x]; m = nq;
+        }
+      }
+    }
+    if (!valid) {
+      if (live && threadIdx.x == 0) out[job.first] = -1;
+      continue;
+    }
+    const int strips = (n + 63) / 64;
+    const int steps = m + 63;
+    const int nblocks = (steps + kBlk - 1) / kBlk;
+    // Strip s starts at block (s / W) * P + (s % W) * kLag, P = max(nblocks,
+    // W * kLag): kLag after strip s - 1 and P >= nblocks after the same wave's
+    // previous strip.
+    const int P = max(nblocks, W * kLag);
+    const int kend = ((strips - 1) / W) * P + ((strips - 1) % W) * kLag + nblocks;
+    // Per-lane state of the wave's current strip.
+    int cur = -1;       // strip
+    int c1 = 0, u1p = 0, tcap = -1, res = 0;
+    float rx = 0.0f, ry = 0.0f;
+    const float2* cp = C;
+    float2 colA[kBlk], colB[kBlk];
+    auto load_block = [&](float2(&dst)[kBlk], int tb) __attribute__((always_inline)) {
+      if (tb - 63 >= -kPad) {
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) dst[j] = cp[tb + j];
+      } else {
+        const bool before = tb - l < -kPad;
+        const float2* bp = before ? C - kPad : cp + tb;
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) {
+          const float2 v = bp[j];
+          dst[j] = before ? nan2 : v;
+        }
+      }
+    };
+    // One iteration: the wave's block of its current strip, if any, with the
+    // block's column points in src and the next block's loading into dst.
+    auto step = [&](int k, float2(&src)[kBlk], float2(&dst)[kBlk]) __attribute__((always_inline)) {
+      const int d = k - wv * kLag;
+      if (d < 0) return;
+      const int i = d / P;
+      const int b = d - i * P;
+      const int s = wv + i * W;
+      if (b >= nblocks || s >= strips) return;
+      const int t0 = b * kBlk;
+      if (b == 0) {  // a new strip: row point, D[r][0] + 1, D[r-1][0] + 1
+        cur = s;
+        const int r = 64 * s + l + 1;
+        const float2 rp = r <= n ? R[r - 1] : nan2;
+        rx = rp.x;
+        ry = rp.y;
+        c1 = r + 1;
+        u1p = r;
+        tcap = (s == strips - 1) ? m + l - 1 : -1;
+        cp = C - l;
+        load_block(src, 0);
+      }
+      // Feed: D[r0][t0 + j + 1] + 1 from the previous strip's wave (row 0: c + 1).
+      int fv[kBlk];
+      if (s == 0) {
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) fv[j] = t0 + j + 2;
+      } else {
+        const int16_t* feed = &rows[(s - 1) % W][kOff + t0 + 1];
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) fv[j] = feed[j];
+      }
+      if (t0 + kBlk < steps) load_block(dst, t0 + kBlk);
+      int16_t* own = &rows[wv][kOff + t0 - 63 + 1];
+#pragma unroll
+      for (int j = 0; j < kBlk; ++j) {
+        const bool match = fabsf(rx - src[j].x) < u && fabsf(ry - src[j].y) < u;
+        const int u1 = shift_in<64>(fv[j], c1);
+        const int mm = u1p - (match ? 1 : 0);
+        const int mn = min(min(u1, c1), mm);
+        const int d = (mm == c1) ? u1 : mn;
+        res = (t0 + j == tcap) ? d : res;
+        c1 = d + 1;
+        u1p = u1;
+        if (l == 63) own[j] = int16_t(c1);
+      }
+    };
+    for (int k = 0; k < kend; k += 2) {
+      step(k, colA, colB);
+      __syncthreads();
+      step(k + 1, colB, colA);
+      __syncthreads();
+    }
+    if (cur == strips - 1 && l == ((n - 1) & 63)) out[job.first] = res;
+    __syncthreads();  // the next job's strips reuse the row buffers
+  }
+}
+
 // Candidate filter: a (query, candidate) pair can contain a matching point
 // pair only if neither box lies a float-difference of u or more beyond the
 // other on either axis.  Float subtraction is monotone, so for points a (query)
@@ -343,7 +497,9 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
                                int32_t* out, void* stream) {
   GAT_REQUIRE(npairs >= 0 && njobs >= 0 && njobs <= INT32_MAX && nqueries >= 0 && ncands >= 0,
               "EDR batched: bad sizes");
-  GAT_REQUIRE(lanes_per_pair == 16 || lanes_per_pair == 64, "EDR batched: lanes_per_pair must be 16 or 64");
+  GAT_REQUIRE(lanes_per_pair == 16 || lanes_per_pair == 64 || lanes_per_pair == 128 || lanes_per_pair == 256 ||
+                  lanes_per_pair == 512,
+              "EDR batched: lanes_per_pair must be 16, 64, 128, 256 or 512");
   if (njobs == 0) return GAT_OK;
   GAT_REQUIRE(pairs && jobs && counter && qxy && qoff && qlen && cxy && coff && clen && out,
               "EDR batched: null pointer");
@@ -361,13 +517,32 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
   // LDS class: the smallest row buffer that holds maxlen columns (64-lane
   // pairs: 1024 or 2048).  Persistent grid: per_cu single-wave workgroups per
   // CU (default by class), capped by the work.
-  const bool wide = lanes_per_pair == 64;
+  const bool wide = lanes_per_pair >= 64;
+  const int waves = lanes_per_pair / 64;  // > 1: one pair over several waves (edr_pipe_kernel)
   int cls = maxlen <= 256 ? 256 : maxlen <= 512 ? 512 : maxlen <= 1024 ? 1024 : GAT_MAX_TRAJ_LEN;
   if (wide && cls < 1024) cls = 1024;
-  const int64_t pc = per_cu > 0 ? per_cu : wide ? 16 : (cls <= 512 ? 20 : cls == 1024 ? 16 : 9);
+  const int64_t pc = per_cu > 0 ? per_cu : waves > 1 ? 16 / waves : wide ? 16 : (cls <= 512 ? 20 : cls == 1024 ? 16 : 9);
   int64_t blocks = int64_t(gat::cu_count()) * pc;
   if (blocks > njobs) blocks = njobs;
   const dim3 grid{uint32_t(blocks)}, block{64};
+  if (waves > 1) {
+    const dim3 wblock{uint32_t(64 * waves)};
+#define GAT_EDR_PIPE(M, W)                                                                                       \
+  edr_pipe_kernel<M, W><<<grid, wblock, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff, \
+                                                qlen, Cs, coff, clen, u, out)
+    if (cls == 1024) {
+      if (waves == 2) GAT_EDR_PIPE(1024, 2);
+      else if (waves == 4) GAT_EDR_PIPE(1024, 4);
+      else GAT_EDR_PIPE(1024, 8);
+    } else {
+      if (waves == 2) GAT_EDR_PIPE(GAT_MAX_TRAJ_LEN, 2);
+      else if (waves == 4) GAT_EDR_PIPE(GAT_MAX_TRAJ_LEN, 4);
+      else GAT_EDR_PIPE(GAT_MAX_TRAJ_LEN, 8);
+    }
+#undef GAT_EDR_PIPE
+    GAT_LAUNCH_CHECK("edr_pipe_kernel");
+    return GAT_OK;
+  }
 #define GAT_EDR_LAUNCH(M, SEG)                                                                                  \
   edr_wave_kernel<M, SEG><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff, \
                                                  qlen, Cs, coff, clen, u, out)

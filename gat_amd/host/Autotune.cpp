@@ -108,6 +108,38 @@ int tuneEdrBatched(Grid& grid, const TrajectoryView& queries, int reps, TuneEdrR
     }
   }
   grid.edrPerCuLong = pcl;
+  // Waves per long pair, on a batch that fits one round of the device's
+  // waves: the longest tuning query against the 512 longest trajectories.
+  {
+    int qbest = 0;
+    for (int q = 1; q < queries.trajNum; ++q)
+      if (queries.offsets[q + 1] - queries.offsets[q] > queries.offsets[qbest + 1] - queries.offsets[qbest]) qbest = q;
+    std::vector<int32_t> pairs;
+    for (int32_t t : grid.longestTrajectories(512)) {
+      pairs.push_back(qbest);
+      pairs.push_back(t);
+    }
+    int wbest = grid.edrWavesPerPair;
+    double sbest = -1;
+    for (int w : {1, 2, 4, 8}) {
+      grid.edrWavesPerPair = w;
+      std::vector<double> t;
+      for (int r = 0; r < reps + 1; ++r) {
+        EdrBatchStats st;
+        if (int rc = grid.edrBatchedPairs(queries, pairs, dist, &st)) return rc;
+        if (r > 0) t.push_back(double(st.kernelMs));
+      }
+      const double ms = median(t);
+      ++out->evaluations;
+      if (sbest < 0 || ms < sbest) {
+        sbest = ms;
+        wbest = w;
+      }
+    }
+    grid.edrWavesPerPair = wbest;
+    out->wavesPerPair = wbest;
+    out->smallMs = sbest;
+  }
   out->split = split;
   out->perCuShort = pcs;
   out->perCuLong = pcl;

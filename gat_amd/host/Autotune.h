@@ -29,6 +29,8 @@ struct TuneRangeResult {
 
 struct TuneEdrResult {
   int split = 0, perCuShort = 0, perCuLong = 0;
+  int wavesPerPair = 1;           // long pairs of one-round batches
+  double smallMs = 0;             // that batch's kernel time at the chosen value
   double ms = 0;                  // best kernel time (median of reps)
   int evaluations = 0;
 };

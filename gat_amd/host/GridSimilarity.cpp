@@ -230,6 +230,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   const int64_t cus = gat_device_props(props) == GAT_OK && props[0] > 0 ? props[0] : 256;
   const bool latencyBound = (np + 3) / 4 < cus * 4 * kLatencyJobsPerSimd;
   const int mid = latencyBound ? split : std::max(split, kMid);
+  const int waves = (edrWavesPerPair == 2 || edrWavesPerPair == 4 || edrWavesPerPair == 8) ? edrWavesPerPair : 1;
   std::vector<int32_t> key(static_cast<size_t>(np));
   int64_t cells = 0;
   int maxlen[kBuckets] = {1, 1, 1};
@@ -293,7 +294,8 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
                              bucketJobs[bucket], counterDev_.as<int32_t>() + bucket, queryXY_.as<float>(),
                              queryOff64_.as<int64_t>(), queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(),
                              trajOff_.as<int64_t>(), trajLen_.as<int32_t>(), trajNum, maxlen[bucket], epsilon,
-                             bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0, bucket == 2 ? 64 : 16,
+                             bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
+                             bucket == 2 ? (latencyBound ? 64 * waves : 64) : 16,
                              outDev_.as<int32_t>(), stream),
              "batched EDR");
     return GAT_OK;
@@ -324,6 +326,24 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     st->hostMs += (h1 - h0);
   }
   return GAT_OK;
+}
+
+int Grid::edrBatchedPairs(const TrajectoryView& queries, std::vector<int32_t>& pairs, std::vector<int32_t>& dist,
+                          EdrBatchStats* st) {
+  if (int rc = ensureQueryStore(queries)) return rc;
+  return runBatched(pairs, queries.trajNum, dist, st);
+}
+
+std::vector<int32_t> Grid::longestTrajectories(int k) const {
+  std::vector<int32_t> ids(trajLenHost_.size());
+  std::iota(ids.begin(), ids.end(), 0);
+  const size_t kk = std::min(ids.size(), size_t(std::max(k, 0)));
+  std::partial_sort(ids.begin(), ids.begin() + long(kk), ids.end(), [&](int32_t a, int32_t b) {
+    return trajLenHost_[size_t(a)] != trajLenHost_[size_t(b)] ? trajLenHost_[size_t(a)] > trajLenHost_[size_t(b)]
+                                                              : a < b;
+  });
+  ids.resize(kk);
+  return ids;
 }
 
 int Grid::edrBatchedAll(const TrajectoryView& queries, std::vector<int32_t>& dist, EdrBatchStats* st) {

@@ -516,6 +516,22 @@ int gath_grid_edr_batched_all(void* p, const int64_t* q_offsets, int nq, const f
   return GAT_OK;
 }
 
+int gath_grid_edr_batched_pairs(void* p, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
+                                const int32_t* pairs, int64_t npairs, double eps, int32_t* dist, int64_t* stats,
+                                double* ms) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || nq < 0 || !q_offsets || npairs < 0 || (npairs > 0 && (!pairs || !dist)))
+    return bad("gath_grid_edr_batched_pairs: bad arguments");
+  ix->grid->epsilon = eps;
+  TrajectoryView q{q_offsets, nq, qlon, qlat};
+  std::vector<int32_t> pv(pairs, pairs + 2 * npairs), d;
+  EdrBatchStats st;
+  if (int rc = ix->grid->edrBatchedPairs(q, pv, d, &st)) return rc;
+  std::copy(d.begin(), d.end(), dist);
+  put_stats(st, stats, ms);
+  return GAT_OK;
+}
+
 int gath_grid_similarity_batched(void* p, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
                                  int k, double eps, int32_t* ids, int32_t* dists, int64_t* stats, double* ms) {
   Index* ix = as_index(p);
@@ -555,6 +571,14 @@ int gath_grid_set_edr_params(void* p, int split, int per_cu_short, int per_cu_lo
   return GAT_OK;
 }
 
+int gath_grid_set_edr_waves(void* p, int waves_per_pair) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || (waves_per_pair != 1 && waves_per_pair != 2 && waves_per_pair != 4 && waves_per_pair != 8))
+    return bad("gath_grid_set_edr_waves: waves per pair must be 1, 2, 4 or 8");
+  ix->grid->edrWavesPerPair = waves_per_pair;
+  return GAT_OK;
+}
+
 int gath_tune_range(void* p, const float* boxes, int nq, int reps, double* out) {
   Index* ix = as_index(p);
   if (!ix || !out) return bad("gath_tune_range: bad arguments");
@@ -583,6 +607,8 @@ int gath_tune_edr(void* p, const int64_t* q_offsets, int nq, const float* qlon, 
   out[2] = r.perCuLong;
   out[3] = r.ms;
   out[4] = r.evaluations;
+  out[5] = r.wavesPerPair;
+  out[6] = r.smallMs;
   return GAT_OK;
 }
 

@@ -115,6 +115,9 @@ int gath_index_last_unique_candidates(void* index, int64_t* points);
  * (0..64, 0 = the kernel's default).  Pairs longer than max(split, 1024)
  * points run in a third launch at the kernel's default. */
 int gath_grid_set_edr_params(void* grid, int split, int per_cu_short, int per_cu_long);
+/* Waves per pair (1, 2, 4 or 8) for the long pairs of batches that fit one
+ * round of the device's waves (top-k survivors, serving batches). */
+int gath_grid_set_edr_waves(void* grid, int waves_per_pair);
 /* Tunables of the batched form: rectangles per leaf-major work item (1..64),
  * and the largest batch that takes the direct form (one refine task per
  * selected leaf and rectangle, nothing sized by the tree touched). */
@@ -158,6 +161,12 @@ int gath_grid_similarity(void* grid, const int64_t* q_offsets, int nq, const flo
  * the all-mismatch table, 0; ms[0..2] = DP kernel, filter kernel, host work. */
 int gath_grid_edr_batched_all(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
                               double eps, int32_t* dist, int64_t* stats, double* ms);
+/* The batched EDR of given (query index, trajectory index) pairs, both
+ * 0-based: dist[i] for pairs[2i], pairs[2i + 1]; stats and ms as
+ * gath_grid_edr_batched_all. */
+int gath_grid_edr_batched_pairs(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
+                                const int32_t* pairs, int64_t npairs, double eps, int32_t* dist, int64_t* stats,
+                                double* ms);
 int gath_grid_similarity_batched(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
                                  int k, double eps, int32_t* ids, int32_t* dists, int64_t* stats, double* ms);
 /* The all-mismatch table, (GAT_MAX_TRAJ_LEN + 1)^2 int32. */
@@ -172,8 +181,10 @@ int gath_grid_mismatch_table(void* grid, int32_t* out);
  * choice, out[1..5] = median device pipeline ms of each, out[6] = the direct
  * form's ms, out[7] = the resulting direct-form threshold (rectangles).
  * gath_tune_edr (Grid): bucket split and workgroups per CU of each bucket, by
- * coordinate descent; out[0..4] = split, per-CU short, per-CU long, best kernel
- * ms, evaluations. */
+ * coordinate descent, then waves per long pair on a one-round batch (the
+ * longest query against the 512 longest trajectories); out[0..6] = split,
+ * per-CU short, per-CU long, best kernel ms, evaluations, waves per pair,
+ * that batch's kernel ms. */
 int gath_tune_range(void* index, const float* boxes, int nq, int reps, double* out);
 int gath_tune_edr(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, double eps,
                   int reps, double* out);

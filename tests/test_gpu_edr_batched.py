@@ -127,10 +127,42 @@ def test_autotuned_edr_gives_the_same_distances(gpu):
     q = api.synth_queries(78, np.array([33, 600, 1500], np.int32), db, copy_mode=2)
     want = _brute(db, q)
     tuned = g.tune_edr(q, EPS, reps=1)
-    assert tuned["split"] in (256, 512, 1024, 2048) and tuned["evaluations"] == 17
+    assert tuned["split"] in (256, 512, 1024, 2048) and tuned["evaluations"] == 21
+    assert tuned["waves_per_long_pair"] in (1, 2, 4, 8)
     dist, _ = g.edr_batched_all(q, EPS)
     assert (dist == want).all()
     for split in (256, 2048):
         g.set_edr_params(split, 6, 24)
         dist, _ = g.edr_batched_all(q, EPS)
         assert (dist == want).all(), split
+
+
+@pytest.mark.parametrize("split", [512, 1])
+def test_waves_per_long_pair(gpu, split):
+    """Long pairs of a one-round batch over 1, 2, 4 or 8 waves in lockstep
+    (edr_pipe_kernel): bit-exact against the oracle on rows fewer than one
+    strip per wave, rows not a multiple of 64, the longest rows and columns,
+    equal lengths (orientation rule); split 1 sends every pair to the
+    long-pair launch."""
+    lengths = [1, 2, 63, 64, 65, 100, 127, 128, 129, 255, 256, 257, 511, 700, 1000, 1023, 1024, 1025, 1500, 2047,
+               2048]
+    db = _varied_db(91, lengths + [600, 900, 1300, 1800])
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    q = api.synth_queries(92, np.array([1, 64, 65, 300, 1024, 1500, 2048], np.int32), db, copy_mode=2)
+    want = _brute(db, q)
+    pairs = np.array([(a, b) for a in range(q.ntraj) for b in range(db.ntraj)], np.int32)
+    g.set_edr_params(split, 0, 0)
+    for waves in (1, 2, 4, 8):
+        g.set_edr_waves(waves)
+        dist, st = g.edr_batched_pairs(q, pairs, EPS)
+        got = dist.reshape(q.ntraj, db.ntraj)
+        bad = np.argwhere(got != want)
+        assert len(bad) == 0, (waves, [(int(a), int(b), int(q.lengths[a]), int(db.lengths[b]), int(got[a, b]),
+                                        int(want[a, b])) for a, b in bad[:8]])
+        assert st["pairs"] == len(pairs)
+    # A shuffled subset: pair order and duplicates do not matter.
+    rng = np.random.default_rng(5)
+    sub = pairs[rng.integers(0, len(pairs), 97)]
+    g.set_edr_waves(4)
+    dist, _ = g.edr_batched_pairs(q, sub, EPS)
+    assert (dist == want[sub[:, 0], sub[:, 1]]).all()
