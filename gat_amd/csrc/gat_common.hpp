@@ -60,6 +60,14 @@ __device__ __forceinline__ void store_pair(int2* dst, int2 v) {
   __builtin_nontemporal_store(w, reinterpret_cast<v2i*>(dst));
 }
 
+// Two consecutive result pairs (16 bytes) in one streaming store; dst is
+// 16-byte aligned.
+__device__ __forceinline__ void store_pair2(int2* dst, const int4& v) {
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  const v4i w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<v4i*>(dst));
+}
+
 // Order-independent digest of a (tid, pid) pair: a 64-bit mix (the splitmix64
 // finaliser) of tid in the high and pid in the low half.  A query's digest is
 // the wrapping sum over its pairs.
