@@ -477,8 +477,12 @@ class GridIndex(_Index):
                 "kernel_ms": round(float(out[3]), 4), "evaluations": int(out[4]),
                 "waves_per_long_pair": int(out[5]), "one_round_ms": round(float(out[6]), 4)}
 
-    def set_edr_waves(self, waves_per_pair: int) -> "GridIndex":
-        """Waves per long pair (1, 2, 4, 8) for batches that fit one round."""
+    def set_edr_waves(self, waves_per_pair: int, every_long_pair: bool = False) -> "GridIndex":
+        """Waves per long pair (1, 2, 4, 8) for batches that fit one round:
+        the pairs one wave would keep busy past the batch's longest pair on
+        that many waves take them, the rest one wave each; every_long_pair
+        gives them to every long pair."""
+        waves_per_pair = -int(waves_per_pair) if every_long_pair and waves_per_pair > 1 else waves_per_pair
         N.check_host(N.host().gath_grid_set_edr_waves(self.handle, int(waves_per_pair)))
         return self
 

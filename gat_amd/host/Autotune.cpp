@@ -109,15 +109,18 @@ int tuneEdrBatched(Grid& grid, const TrajectoryView& queries, int reps, TuneEdrR
   }
   grid.edrPerCuLong = pcl;
   // Waves per long pair, on a batch that fits one round of the device's
-  // waves: the longest tuning query against the 512 longest trajectories.
+  // waves and has a top-k batch's mix of pair shapes: the longest tuning
+  // query against 8192 trajectories spread evenly over the database (most
+  // pairs of a few strips, one wave each; the longest over several).
   {
     int qbest = 0;
     for (int q = 1; q < queries.trajNum; ++q)
       if (queries.offsets[q + 1] - queries.offsets[q] > queries.offsets[qbest + 1] - queries.offsets[qbest]) qbest = q;
     std::vector<int32_t> pairs;
-    for (int32_t t : grid.longestTrajectories(512)) {
+    const int64_t spread = std::min<int64_t>(8192, grid.trajNum);
+    for (int64_t i = 0; i < spread; ++i) {
       pairs.push_back(qbest);
-      pairs.push_back(t);
+      pairs.push_back(int32_t(spread > 1 ? i * (grid.trajNum - 1) / (spread - 1) : 0));
     }
     int wbest = grid.edrWavesPerPair;
     double sbest = -1;

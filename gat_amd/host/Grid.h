@@ -115,15 +115,16 @@ class Grid {
   // Waves per pair (1, 2, 4 or 8) for the long pairs of a batch small enough
   // to fit one round of the device's waves (a top-k batch's survivors, a
   // serving batch), which ends when its longest pair does: above 1 the pair's
-  // strips run on that many waves in lockstep (edr_pipe_kernel).
+  // strips run on that many waves in lockstep (edr_pipe_kernel).  Only pairs
+  // that one wave would keep busy past the longest pair's pipelined schedule
+  // take them, unless edrWavesEveryLongPair is set.
   int edrWavesPerPair = 4;
+  bool edrWavesEveryLongPair = false;
 
   // Batched EDR of given (query index, trajectory index) pairs, both 0-based:
   // dist[i] for pairs[2i], pairs[2i + 1].
   int edrBatchedPairs(const TrajectoryView& queries, std::vector<int32_t>& pairs, std::vector<int32_t>& dist,
                       EdrBatchStats* stats);
-  // 0-based indexes of the k longest trajectories, longest first.
-  std::vector<int32_t> longestTrajectories(int k) const;
 
   // Run-time EDR threshold (the reference's compile-time EPSILON, ConstDefine.h:28).
   double epsilon = 0.005;

@@ -216,7 +216,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   // and jobs of up to four pairs with the same strip count.
   constexpr int kM = GAT_MAX_TRAJ_LEN + 1;
   constexpr int kS = GAT_MAX_TRAJ_LEN / 16 + 1;
-  constexpr int kBuckets = 3;
+  constexpr int kBuckets = 4;  // 3: long pairs of a one-round batch over several waves
   constexpr int kMid = 1024;
   constexpr int64_t kLatencyJobsPerSimd = 4;  // the batched kernel holds 4 waves per SIMD: one round
   const int split = std::max(1, std::min(edrSplit, GAT_MAX_TRAJ_LEN));
@@ -231,19 +231,44 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   const bool latencyBound = (np + 3) / 4 < cus * 4 * kLatencyJobsPerSimd;
   const int mid = latencyBound ? split : std::max(split, kMid);
   const int waves = (edrWavesPerPair == 2 || edrWavesPerPair == 4 || edrWavesPerPair == 8) ? edrWavesPerPair : 1;
-  std::vector<int32_t> key(static_cast<size_t>(np));
-  int64_t cells = 0;
-  int maxlen[kBuckets] = {1, 1, 1};
   for (int64_t i = 0; i < np; ++i) {
     const int q = pairs[2 * i], c = pairs[2 * i + 1];
     if (q < 0 || q >= queryNum || c < 0 || c >= trajNum) {
       gath_set_error("batched EDR: pair %lld out of range", (long long)i);
       return GAT_ERR_ARG;
     }
-    const int a = queryLenHost_[q], b = trajLenHost_[c];
+  }
+  // In a one-round batch with several waves per long pair, only the pairs
+  // that would outlast the batch's longest pair on one wave take several:
+  // a pair of s 64-row strips over a b-block row takes s * b blocks on one
+  // wave and pipe_blocks(s, b) over `waves` waves (the workgroup's schedule,
+  // with the lag of 6 blocks between strips), and the batch cannot end before
+  // its longest pair's pipelined schedule does.  The rest run one pair per
+  // wave, beside it; several waves on a pair of fewer strips than waves leave
+  // the surplus waves idle at every barrier.
+  auto pipe_blocks = [&](int64_t s, int64_t b) {
+    const int64_t P = std::max<int64_t>(b, int64_t(waves) * 6);
+    return ((s - 1) / waves) * P + ((s - 1) % waves) * 6 + b;
+  };
+  const bool multiWave = latencyBound && waves > 1;
+  int64_t critical = 0;
+  if (multiWave)
+    for (int64_t i = 0; i < np; ++i) {
+      const int a = queryLenHost_[pairs[2 * i]], b = trajLenHost_[pairs[2 * i + 1]];
+      const int n = std::min(a, b), m = std::max(a, b);
+      if (m > mid) critical = std::max(critical, pipe_blocks((n + 63) / 64, (m + 63 + 15) / 16));
+    }
+  std::vector<int32_t> key(static_cast<size_t>(np));
+  int64_t cells = 0;
+  int maxlen[kBuckets] = {1, 1, 1, 1};
+  for (int64_t i = 0; i < np; ++i) {
+    const int a = queryLenHost_[pairs[2 * i]], b = trajLenHost_[pairs[2 * i + 1]];
     const int n = std::min(a, b), m = std::max(a, b);
     cells += int64_t(n) * m;
-    const int bucket = m <= split ? 0 : m <= mid ? 1 : 2;
+    int bucket = m <= split ? 0 : m <= mid ? 1 : 2;
+    if (bucket == 2 && multiWave &&
+        (edrWavesEveryLongPair || int64_t((n + 63) / 64) * ((m + 63 + 15) / 16) > critical))
+      bucket = 3;
     maxlen[bucket] = std::max(maxlen[bucket], std::min(m, GAT_MAX_TRAJ_LEN));
     key[i] = (bucket * kS + (kS - 1 - strips_of(n))) * kM + (kM - 1 - m);
   }
@@ -262,12 +287,12 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   std::vector<int32_t> sorted(size_t(np) * 2);
   std::vector<gat_edr_job> jobs;
   jobs.reserve(size_t(np / 4 + 2));
-  int64_t bucketJobs[kBuckets] = {0, 0, 0};
+  int64_t bucketJobs[kBuckets] = {0, 0, 0, 0};
   int64_t i = 0;
   while (i < np) {
     const int32_t k0 = key[perm[i]] / kM;  // bucket and strip count
-    // Jobs of the longest bucket run one pair over a whole wave.
-    const int32_t per_job = k0 / kS == 2 ? 1 : 4;
+    // Jobs of the long buckets run one pair over a whole wave (or workgroup).
+    const int32_t per_job = k0 / kS >= 2 ? 1 : 4;
     int32_t c = 0;
     while (c < per_job && i + c < np && key[perm[i + c]] / kM == k0) ++c;
     jobs.push_back(gat_edr_job{int32_t(i), c});
@@ -288,25 +313,28 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   // Each bucket has its own job counter.  The longest bucket holds few jobs
   // whose strips run for milliseconds each: launched first on a side stream, it
   // overlaps the other buckets instead of trailing them on an idle device.
-  int64_t j0[kBuckets] = {0, bucketJobs[0], bucketJobs[0] + bucketJobs[1]};
+  int64_t j0[kBuckets] = {0, bucketJobs[0], bucketJobs[0] + bucketJobs[1],
+                          bucketJobs[0] + bucketJobs[1] + bucketJobs[2]};
   auto launch = [&](int bucket, void* stream) -> int {
     GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>() + j0[bucket],
                              bucketJobs[bucket], counterDev_.as<int32_t>() + bucket, queryXY_.as<float>(),
                              queryOff64_.as<int64_t>(), queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(),
                              trajOff_.as<int64_t>(), trajLen_.as<int32_t>(), trajNum, maxlen[bucket], epsilon,
                              bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
-                             bucket == 2 ? (latencyBound ? 64 * waves : 64) : 16,
-                             outDev_.as<int32_t>(), stream),
+                             bucket == 3 ? 64 * waves : bucket == 2 ? 64 : 16, outDev_.as<int32_t>(), stream),
              "batched EDR");
     return GAT_OK;
   };
-  const bool side = bucketJobs[2] > 0 && bucketJobs[0] + bucketJobs[1] > 0;
+  const int longest = bucketJobs[3] > 0 ? 3 : 2;
+  int64_t others = 0;
+  for (int bucket = 0; bucket < kBuckets; ++bucket) others += bucket == longest ? 0 : bucketJobs[bucket];
+  const bool side = bucketJobs[longest] > 0 && others > 0;
   if (side) {
     if (int rc = edrSide_.fork(s)) return rc;
-    if (int rc = launch(2, edrSide_.stream())) return rc;
+    if (int rc = launch(longest, edrSide_.stream())) return rc;
   }
   for (int bucket = 0; bucket < kBuckets; ++bucket) {
-    if (bucketJobs[bucket] == 0 || (side && bucket == 2)) continue;
+    if (bucketJobs[bucket] == 0 || (side && bucket == longest)) continue;
     if (int rc = launch(bucket, s)) return rc;
   }
   if (side)
@@ -332,18 +360,6 @@ int Grid::edrBatchedPairs(const TrajectoryView& queries, std::vector<int32_t>& p
                           EdrBatchStats* st) {
   if (int rc = ensureQueryStore(queries)) return rc;
   return runBatched(pairs, queries.trajNum, dist, st);
-}
-
-std::vector<int32_t> Grid::longestTrajectories(int k) const {
-  std::vector<int32_t> ids(trajLenHost_.size());
-  std::iota(ids.begin(), ids.end(), 0);
-  const size_t kk = std::min(ids.size(), size_t(std::max(k, 0)));
-  std::partial_sort(ids.begin(), ids.begin() + long(kk), ids.end(), [&](int32_t a, int32_t b) {
-    return trajLenHost_[size_t(a)] != trajLenHost_[size_t(b)] ? trajLenHost_[size_t(a)] > trajLenHost_[size_t(b)]
-                                                              : a < b;
-  });
-  ids.resize(kk);
-  return ids;
 }
 
 int Grid::edrBatchedAll(const TrajectoryView& queries, std::vector<int32_t>& dist, EdrBatchStats* st) {

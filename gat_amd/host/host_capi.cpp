@@ -573,9 +573,11 @@ int gath_grid_set_edr_params(void* p, int split, int per_cu_short, int per_cu_lo
 
 int gath_grid_set_edr_waves(void* p, int waves_per_pair) {
   Index* ix = as_index(p);
-  if (!ix || !ix->grid || (waves_per_pair != 1 && waves_per_pair != 2 && waves_per_pair != 4 && waves_per_pair != 8))
-    return bad("gath_grid_set_edr_waves: waves per pair must be 1, 2, 4 or 8");
-  ix->grid->edrWavesPerPair = waves_per_pair;
+  const int w = waves_per_pair < 0 ? -waves_per_pair : waves_per_pair;
+  if (!ix || !ix->grid || (w != 1 && w != 2 && w != 4 && w != 8))
+    return bad("gath_grid_set_edr_waves: waves per pair must be 1, 2, 4 or 8 (or -2, -4, -8)");
+  ix->grid->edrWavesPerPair = w;
+  ix->grid->edrWavesEveryLongPair = waves_per_pair < 0;
   return GAT_OK;
 }
 

@@ -116,7 +116,10 @@ int gath_index_last_unique_candidates(void* index, int64_t* points);
  * points run in a third launch at the kernel's default. */
 int gath_grid_set_edr_params(void* grid, int split, int per_cu_short, int per_cu_long);
 /* Waves per pair (1, 2, 4 or 8) for the long pairs of batches that fit one
- * round of the device's waves (top-k survivors, serving batches). */
+ * round of the device's waves (top-k survivors, serving batches).  Only the
+ * pairs that one wave would keep busy longer than the batch's longest pair
+ * takes over that many waves get them; the others run one pair per wave.
+ * -2, -4 or -8 give every long pair of such a batch 2, 4 or 8 waves. */
 int gath_grid_set_edr_waves(void* grid, int waves_per_pair);
 /* Tunables of the batched form: rectangles per leaf-major work item (1..64),
  * and the largest batch that takes the direct form (one refine task per
@@ -182,7 +185,8 @@ int gath_grid_mismatch_table(void* grid, int32_t* out);
  * form's ms, out[7] = the resulting direct-form threshold (rectangles).
  * gath_tune_edr (Grid): bucket split and workgroups per CU of each bucket, by
  * coordinate descent, then waves per long pair on a one-round batch (the
- * longest query against the 512 longest trajectories); out[0..6] = split,
+ * longest query against 8192 trajectories spread evenly over the database);
+ * out[0..6] = split,
  * per-CU short, per-CU long, best kernel ms, evaluations, waves per pair,
  * that batch's kernel ms. */
 int gath_tune_range(void* index, const float* boxes, int nq, int reps, double* out);

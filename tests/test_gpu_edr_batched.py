@@ -143,7 +143,8 @@ def test_waves_per_long_pair(gpu, split):
     (edr_pipe_kernel): bit-exact against the oracle on rows fewer than one
     strip per wave, rows not a multiple of 64, the longest rows and columns,
     equal lengths (orientation rule); split 1 sends every pair to the
-    long-pair launch."""
+    long-pair launches.  Every long pair over several waves, and the default
+    split between pairs over several waves and pairs on one."""
     lengths = [1, 2, 63, 64, 65, 100, 127, 128, 129, 255, 256, 257, 511, 700, 1000, 1023, 1024, 1025, 1500, 2047,
                2048]
     db = _varied_db(91, lengths + [600, 900, 1300, 1800])
@@ -152,13 +153,13 @@ def test_waves_per_long_pair(gpu, split):
     want = _brute(db, q)
     pairs = np.array([(a, b) for a in range(q.ntraj) for b in range(db.ntraj)], np.int32)
     g.set_edr_params(split, 0, 0)
-    for waves in (1, 2, 4, 8):
-        g.set_edr_waves(waves)
+    for waves, every in ((1, False), (2, True), (4, True), (8, True), (2, False), (4, False), (8, False)):
+        g.set_edr_waves(waves, every_long_pair=every)
         dist, st = g.edr_batched_pairs(q, pairs, EPS)
         got = dist.reshape(q.ntraj, db.ntraj)
         bad = np.argwhere(got != want)
-        assert len(bad) == 0, (waves, [(int(a), int(b), int(q.lengths[a]), int(db.lengths[b]), int(got[a, b]),
-                                        int(want[a, b])) for a, b in bad[:8]])
+        assert len(bad) == 0, (waves, every, [(int(a), int(b), int(q.lengths[a]), int(db.lengths[b]),
+                                               int(got[a, b]), int(want[a, b])) for a, b in bad[:8]])
         assert st["pairs"] == len(pairs)
     # A shuffled subset: pair order and duplicates do not matter.
     rng = np.random.default_rng(5)
