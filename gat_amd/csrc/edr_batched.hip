@@ -462,6 +462,229 @@ __global__ __launch_bounds__(1024) void edr_mismatch_table_kernel(int32_t maxlen
   }
 }
 
+// Exact top-k merge of one query per workgroup (no reference counterpart: the
+// reference keeps a k-entry priority queue per query on the host,
+// Grid.cpp:1600-1729).  A candidate's distance is its DP distance when it
+// survived the filter, else the all-mismatch value of its length pair.  (The
+// reference's cell rule is not a minimum, so a DP distance can exceed that
+// value: survivors are never assumed to lie below it.)  The distances are
+// histogrammed per trajectory length (at most GAT_MAX_TRAJ_LEN lengths,
+// whatever the database size), each survivor moved from its length's value to
+// its DP distance, and the cut T is the smallest distance whose running count
+// reaches kk.  Every candidate below T is taken (fewer than kk: the
+// non-survivors of the lengths whose value is below T, the survivors whose DP
+// distance is), then the r smallest ids at distance T, found by a binary
+// search for the id bound X over the ascending id lists of the lengths whose
+// value is T (their survivors left out) and the survivors at T.  The kk
+// results are ranked by (distance, id).
+constexpr int kTopkMax = 1024;
+constexpr int kLenBins = GAT_MAX_TRAJ_LEN + 1;     // lengths 0..GAT_MAX_TRAJ_LEN
+constexpr int kDistBins = 2 * GAT_MAX_TRAJ_LEN + 1;  // distances 0..n + m
+constexpr int kTopkBlock = 256;
+
+// Index of c in the ascending sid[s0, s1), or -1.
+__device__ __forceinline__ int32_t survivor_index(const int32_t* __restrict__ sid, int32_t s0, int32_t s1, int32_t c) {
+  int32_t lo = s0, hi = s1;
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (sid[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < s1 && sid[lo] == c) ? lo : -1;
+}
+
+// Number of entries <= x in the ascending ids[0, n).
+__device__ __forceinline__ int32_t count_le(const int32_t* __restrict__ ids, int32_t n, int32_t x) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (ids[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Sum over the workgroup, returned to every thread.
+__device__ __forceinline__ int32_t topk_block_sum(int32_t v, int32_t* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();  // every thread has read red from the previous call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
+    const int32_t* __restrict__ ftable, int32_t w, const int32_t* __restrict__ qlen,
+    const int32_t* __restrict__ clen, int32_t nc, const int32_t* __restrict__ len_ids,
+    const int32_t* __restrict__ len_off, const int32_t* __restrict__ sstart, const int32_t* __restrict__ sid,
+    const int32_t* __restrict__ sdist, int32_t k, int32_t kk, int32_t* __restrict__ out_ids,
+    int32_t* __restrict__ out_dist, int32_t* __restrict__ status) {
+  __shared__ int32_t val[kLenBins];    // the query's all-mismatch value per candidate length
+  __shared__ int32_t hist[kDistBins];  // candidates per distance
+  __shared__ int32_t out_d[kTopkMax], out_i[kTopkMax];
+  __shared__ int32_t red[kTopkBlock / 64];
+  __shared__ int32_t n_out, sh_T, sh_below;
+  const int q = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int a = qlen[q];
+  if (a < 1 || a > GAT_MAX_TRAJ_LEN) {
+    if (tid == 0) atomicOr(status, 4);
+    return;
+  }
+  for (int L = tid; L < kLenBins; L += kTopkBlock) val[L] = L >= 1 ? ftable[int64_t(min(a, L)) * w + max(a, L)] : 0;
+  for (int b = tid; b < kDistBins; b += kTopkBlock) hist[b] = 0;
+  if (tid == 0) {
+    n_out = 0;
+    sh_T = -1;
+    sh_below = 0;
+  }
+  __syncthreads();
+  for (int L = 1 + tid; L < kLenBins; L += kTopkBlock) {
+    const int32_t cnt = len_off[L + 1] - len_off[L];
+    const int32_t v = val[L];
+    if (cnt > 0) {
+      if (v < 0 || v >= kDistBins) atomicOr(status, 1);
+      else atomicAdd(&hist[v], cnt);
+    }
+  }
+  const int32_t s0 = sstart[q], s1 = sstart[q + 1];
+  for (int32_t i = s0 + tid; i < s1; i += kTopkBlock) {
+    const int32_t c = sid[i];
+    const int32_t L = (c >= 0 && c < nc) ? clen[c] : 0;
+    const int32_t d = sdist[i];
+    if (L < 1 || L > GAT_MAX_TRAJ_LEN || d < 0 || d >= kDistBins || val[L] >= kDistBins) {
+      atomicOr(status, 1);
+      continue;
+    }
+    atomicSub(&hist[val[L]], 1);
+    atomicAdd(&hist[d], 1);
+  }
+  __syncthreads();
+  // The cut: thread t scans distances [kPer t, kPer (t + 1)).
+  constexpr int kPer = (kDistBins + kTopkBlock - 1) / kTopkBlock;
+  const int lane = tid & 63, wv = tid >> 6;
+  int32_t local = 0;
+  for (int j = 0; j < kPer; ++j) {
+    const int b = tid * kPer + j;
+    if (b < kDistBins) local += hist[b];
+  }
+  int32_t incl = local;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) red[wv] = incl;
+  __syncthreads();
+  int32_t before = incl - local;
+  for (int j = 0; j < wv; ++j) before += red[j];
+  if (before < kk && before + local >= kk) {
+    int32_t acc = before;
+    for (int j = 0; j < kPer; ++j) {
+      const int b = tid * kPer + j;
+      if (b >= kDistBins) break;
+      if (acc + hist[b] >= kk) {
+        sh_T = b;
+        sh_below = acc;
+        break;
+      }
+      acc += hist[b];
+    }
+  }
+  __syncthreads();
+  const int32_t T = sh_T;
+  const int32_t r = kk - sh_below;
+  if (T < 0) {
+    if (tid == 0) atomicOr(status, 2);
+    return;
+  }
+  // Every candidate below T: non-survivors by length, survivors by DP distance.
+  for (int L = 1 + tid; L < kLenBins; L += kTopkBlock) {
+    if (val[L] >= T) continue;
+    for (int32_t i = len_off[L], e = len_off[L + 1]; i < e; ++i) {
+      const int32_t c = len_ids[i];
+      if (s1 > s0 && survivor_index(sid, s0, s1, c) >= 0) continue;
+      const int32_t pos = atomicAdd(&n_out, 1);
+      if (pos < kTopkMax) {
+        out_d[pos] = val[L];
+        out_i[pos] = c;
+      }
+    }
+  }
+  for (int32_t i = s0 + tid; i < s1; i += kTopkBlock) {
+    const int32_t c = sid[i];
+    const int32_t L = (c >= 0 && c < nc) ? clen[c] : 0;
+    if (L < 1 || L > GAT_MAX_TRAJ_LEN) continue;
+    if (sdist[i] < T) {
+      const int32_t pos = atomicAdd(&n_out, 1);
+      if (pos < kTopkMax) {
+        out_d[pos] = sdist[i];
+        out_i[pos] = c;
+      }
+    }
+  }
+  // The r smallest ids at T: the least X with r of them at or below X.
+  auto ties_le = [&](int32_t X) -> int32_t {
+    int32_t part = 0;
+    for (int L = 1 + tid; L < kLenBins; L += kTopkBlock)
+      if (val[L] == T) part += count_le(len_ids + len_off[L], len_off[L + 1] - len_off[L], X);
+    for (int32_t i = s0 + tid; i < s1; i += kTopkBlock) {
+      const int32_t c = sid[i];
+      if (c > X || c < 0 || c >= nc) continue;
+      const int32_t L = clen[c];
+      if (L < 1 || L > GAT_MAX_TRAJ_LEN) continue;
+      part += (sdist[i] == T ? 1 : 0) - (val[L] == T ? 1 : 0);
+    }
+    return topk_block_sum(part, red);
+  };
+  int32_t lo = 0, hi = nc > 0 ? nc - 1 : 0;
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (ties_le(mid) >= r) hi = mid;
+    else lo = mid + 1;
+  }
+  const int32_t X = lo;
+  for (int L = 1 + tid; L < kLenBins; L += kTopkBlock) {
+    if (val[L] != T) continue;
+    const int32_t* ids = len_ids + len_off[L];
+    const int32_t u = count_le(ids, len_off[L + 1] - len_off[L], X);
+    for (int32_t i = 0; i < u; ++i) {
+      const int32_t c = ids[i];
+      if (s1 > s0 && survivor_index(sid, s0, s1, c) >= 0) continue;  // counted at its DP distance
+      const int32_t pos = atomicAdd(&n_out, 1);
+      if (pos < kTopkMax) {
+        out_d[pos] = T;
+        out_i[pos] = c;
+      }
+    }
+  }
+  for (int32_t i = s0 + tid; i < s1; i += kTopkBlock) {
+    const int32_t c = sid[i];
+    if (c > X || c < 0 || c >= nc) continue;
+    const int32_t L = clen[c];
+    if (L < 1 || L > GAT_MAX_TRAJ_LEN) continue;
+    if (sdist[i] == T) {
+      const int32_t pos = atomicAdd(&n_out, 1);
+      if (pos < kTopkMax) {
+        out_d[pos] = T;
+        out_i[pos] = c;
+      }
+    }
+  }
+  __syncthreads();
+  const int32_t n = n_out;
+  if (n != kk) {
+    if (tid == 0) atomicOr(status, 2);
+    return;
+  }
+  for (int i = tid; i < n; i += kTopkBlock) {
+    const int32_t d = out_d[i], c = out_i[i];
+    int32_t rank = 0;
+    for (int j = 0; j < n; ++j) rank += (out_d[j] < d || (out_d[j] == d && out_i[j] < c)) ? 1 : 0;
+    out_dist[int64_t(q) * k + rank] = d;
+    out_ids[int64_t(q) * k + rank] = c + 1;
+  }
+}
+
 }  // namespace
 
 extern "C" int gat_edr_filter(const float* qbox, int32_t nq, const float* cbox, int32_t nc, double eps,
@@ -479,6 +702,27 @@ extern "C" int gat_edr_filter(const float* qbox, int32_t nq, const float* cbox, 
       reinterpret_cast<const float4*>(qbox), nq, reinterpret_cast<const float4*>(cbox), nc, gat_eps_threshold(eps),
       reinterpret_cast<int2*>(pairs_out), uint64_t(capacity), reinterpret_cast<unsigned long long*>(count));
   GAT_LAUNCH_CHECK("edr_filter_kernel");
+  return GAT_OK;
+}
+
+extern "C" int gat_edr_topk(const int32_t* ftable, int32_t w, const int32_t* qlen, int32_t nq, const int32_t* clen,
+                            int32_t nc, const int32_t* len_ids, const int32_t* len_off, const int32_t* surv_start,
+                            const int32_t* surv_id, const int32_t* surv_dist, int32_t k, int32_t kk, int32_t* out_ids,
+                            int32_t* out_dist, int32_t* status, void* stream) {
+  GAT_REQUIRE(w == GAT_MAX_TRAJ_LEN + 1, "EDR top-k: the mismatch table must be %d wide", GAT_MAX_TRAJ_LEN + 1);
+  GAT_REQUIRE(nq >= 0 && nc >= 0, "EDR top-k: bad sizes");
+  GAT_REQUIRE(k >= 1 && k <= kTopkMax && kk >= 0 && kk <= k && kk <= nc,
+              "EDR top-k: k outside 1..%d or kk outside 0..min(k, candidates)", kTopkMax);
+  GAT_REQUIRE(status, "EDR top-k: null status");
+  hipStream_t s = gat::as_stream(stream);
+  GAT_HIP_CHECK(hipMemsetAsync(status, 0, sizeof(int32_t), s));
+  if (nq == 0 || kk == 0) return GAT_OK;
+  GAT_REQUIRE(ftable && qlen && clen && len_ids && len_off && surv_start && out_ids && out_dist,
+              "EDR top-k: null pointer");
+  edr_topk_kernel<<<dim3(uint32_t(nq)), dim3(kTopkBlock), 0, s>>>(ftable, w, qlen, clen, nc, len_ids, len_off,
+                                                                  surv_start, surv_id, surv_dist, k, kk, out_ids,
+                                                                  out_dist, status);
+  GAT_LAUNCH_CHECK("edr_topk_kernel");
   return GAT_OK;
 }
 

@@ -168,6 +168,10 @@ class Grid {
   int buildNode(int level, int id);
   int ensureQueryStore(const TrajectoryView& queries);
   int runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* stats);
+  // Top-k of up to kDeviceTopkMax per query merged on the device; larger k on the host.
+  static constexpr int kDeviceTopkMax = 1024;
+  int deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_t>& dist, int nq, int kValue,
+                 std::vector<TopK>& out);
   // Trajectory-ordered copy of the points for EDR (the reference reassembles
   // trajectories from the cell-ordered store per query, Grid.cpp:1500-1508).
   // Stores are NaN-padded (GAT_EDR_PAD points before every trajectory,
@@ -175,6 +179,10 @@ class Grid {
   DeviceBuffer trajXY_, trajOff_, trajLen_, trajBox_, allTasks_, dist_;
   DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
   DeviceBuffer pairsDev_, jobsDev_, outDev_, counterDev_, ftableDev_;
+  // Top-k merge on the device: trajectories grouped by length (0-based ids,
+  // ascending) with per-length offsets; per batch, each query's survivors
+  // ascending by id with their distances, and the results.
+  DeviceBuffer lenIdsDev_, lenOffDev_, survStartDev_, survIdDev_, survDistDev_, topIdsDev_, topDistDev_;
   EventTimer edrTimer_;
   SideStream edrSide_;  // the longest EDR bucket runs here, beside the others
   std::vector<int32_t> trajLenHost_, queryLenHost_;

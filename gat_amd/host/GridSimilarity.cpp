@@ -86,6 +86,15 @@ int Grid::uploadTrajectories(const TrajectoryView& db) {
   lengthOrder_.assign(GAT_MAX_TRAJ_LEN + 1, {});
   for (int t = 0; t < T; ++t)
     if (trajLenHost_[t] >= 1 && trajLenHost_[t] <= GAT_MAX_TRAJ_LEN) lengthIds_[trajLenHost_[t]].push_back(t + 1);
+  // The same groups for the device merge: 0-based ids, offsets per length.
+  std::vector<int32_t> lenOff(GAT_MAX_TRAJ_LEN + 2, 0), lenIds;
+  for (int L = 0; L <= GAT_MAX_TRAJ_LEN; ++L) lenOff[L + 1] = lenOff[L] + int32_t(lengthIds_[L].size());
+  lenIds.reserve(size_t(lenOff.back()));
+  for (int L = 0; L <= GAT_MAX_TRAJ_LEN; ++L)
+    for (int32_t id : lengthIds_[L]) lenIds.push_back(id - 1);
+  if (int rc = upload_vec(lenIdsDev_, lenIds, s, "length groups upload")) return rc;
+  if (int rc = upload_vec(lenOffDev_, lenOff, s, "length groups upload")) return rc;
+  GATH_TRY(gat_stream_sync(s), "length groups upload");
   return GAT_OK;
 }
 
@@ -374,6 +383,62 @@ int Grid::edrBatchedAll(const TrajectoryView& queries, std::vector<int32_t>& dis
   return runBatched(pairs, nq, dist, st);
 }
 
+// The top-k merge on the device (gat_edr_topk): each query's survivors,
+// ascending by id with their DP distances, go up as one CSR; every other
+// candidate takes the all-mismatch value of its length there.
+int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_t>& dist, int nq, int kValue,
+                     std::vector<TopK>& out) {
+  const size_t count = pairs.size() / 2;
+  std::vector<int32_t> sstart(size_t(nq) + 1, 0);
+  for (size_t i = 0; i < count; ++i) ++sstart[size_t(pairs[2 * i]) + 1];
+  for (int q = 0; q < nq; ++q) sstart[size_t(q) + 1] += sstart[size_t(q)];
+  std::vector<std::pair<int32_t, int32_t>> surv(count);  // (id, distance), grouped by query
+  {
+    std::vector<int32_t> fill(sstart.begin(), sstart.end() - 1);
+    for (size_t i = 0; i < count; ++i) surv[size_t(fill[size_t(pairs[2 * i])]++)] = {pairs[2 * i + 1], dist[i]};
+  }
+  std::vector<int32_t> sid(std::max<size_t>(count, 1)), sdist(std::max<size_t>(count, 1));
+  for (int q = 0; q < nq; ++q) {
+    std::sort(surv.begin() + sstart[size_t(q)], surv.begin() + sstart[size_t(q) + 1]);
+    for (int32_t i = sstart[size_t(q)]; i < sstart[size_t(q) + 1]; ++i) {
+      sid[size_t(i)] = surv[size_t(i)].first;
+      sdist[size_t(i)] = surv[size_t(i)].second;
+    }
+  }
+  int64_t eligible = 0;
+  for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L) eligible += int64_t(lengthIds_[L].size());
+  const int kk = int(std::min<int64_t>(kValue, eligible));
+  void* s = engine_.stream();
+  if (int rc = upload_vec(survStartDev_, sstart, s, "survivor upload")) return rc;
+  if (int rc = upload_vec(survIdDev_, sid, s, "survivor upload")) return rc;
+  if (int rc = upload_vec(survDistDev_, sdist, s, "survivor upload")) return rc;
+  const size_t cells = std::max<size_t>(size_t(nq) * size_t(kValue), 1);
+  if (int rc = topIdsDev_.reserve(cells * 4)) return rc;
+  if (int rc = topDistDev_.reserve(cells * 4)) return rc;
+  if (int rc = counterDev_.reserve(16)) return rc;
+  GATH_TRY(gat_edr_topk(ftableDev_.as<int32_t>(), GAT_MAX_TRAJ_LEN + 1, queryLen_.as<int32_t>(), nq,
+                        trajLen_.as<int32_t>(), trajNum, lenIdsDev_.as<int32_t>(), lenOffDev_.as<int32_t>(),
+                        survStartDev_.as<int32_t>(), survIdDev_.as<int32_t>(), survDistDev_.as<int32_t>(), kValue, kk,
+                        topIdsDev_.as<int32_t>(), topDistDev_.as<int32_t>(), counterDev_.as<int32_t>(), s),
+           "top-k merge");
+  std::vector<int32_t> ids(cells), dists(cells);
+  int32_t status = 0;
+  GATH_TRY(gat_memcpy_d2h(ids.data(), topIdsDev_.get(), cells * 4, s), "top-k readback");
+  GATH_TRY(gat_memcpy_d2h(dists.data(), topDistDev_.get(), cells * 4, s), "top-k readback");
+  GATH_TRY(gat_memcpy_d2h(&status, counterDev_.get(), 4, s), "top-k readback");
+  GATH_TRY(gat_stream_sync(s), "top-k readback");
+  if (status != 0) {
+    gath_set_error("top-k merge: inconsistent inputs (status %d)", status);
+    return GAT_ERR_HIP;
+  }
+  out.assign(size_t(nq), TopK{});
+  for (int q = 0; q < nq; ++q) {
+    out[size_t(q)].ids.assign(ids.begin() + int64_t(q) * kValue, ids.begin() + int64_t(q) * kValue + kk);
+    out[size_t(q)].dist.assign(dists.begin() + int64_t(q) * kValue, dists.begin() + int64_t(q) * kValue + kk);
+  }
+  return GAT_OK;
+}
+
 int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vector<TopK>& out, EdrBatchStats* st) {
   if (int rc = ensureQueryStore(queries)) return rc;
   const int nq = queries.trajNum;
@@ -409,6 +474,15 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
   std::vector<int32_t> dist;
   if (int rc = runBatched(pairs, nq, dist, st)) return rc;
   const double h0 = now_ms();
+  if (kValue >= 1 && kValue <= kDeviceTopkMax) {
+    if (int rc = deviceTopk(pairs, dist, nq, kValue, out)) return rc;
+    if (st) {
+      st->filterMs += fms;
+      st->pruned += int64_t(nq) * trajNum - int64_t(count);
+      st->hostMs += now_ms() - h0;
+    }
+    return GAT_OK;
+  }
   // Per query: the survivors with their exact distances, and every other
   // trajectory at the all-mismatch value of its length, merged by (distance, id).
   std::vector<int64_t> start(size_t(nq) + 1, 0);

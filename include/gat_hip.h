@@ -282,6 +282,27 @@ int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job*
 int gat_edr_filter(const float* query_box_dev, int32_t nq, const float* cand_box_dev, int32_t nc, double eps,
                    int32_t* pairs_out_dev, int64_t capacity, uint64_t* count_dev, void* stream);
 
+/* Exact top-k merge on the device (no reference counterpart: the reference
+ * keeps a k-entry priority queue per query on the host, Grid.cpp:1600-1729,
+ * fed in its frequency-distance order).  For
+ * query q (length qlen_dev[q]), every candidate c in 0..nc-1 with a length
+ * clen_dev[c] in 1..GAT_MAX_TRAJ_LEN has distance surv_dist_dev[j] when c =
+ * surv_id_dev[j] for some j in [surv_start_dev[q], surv_start_dev[q + 1])
+ * (each query's survivors ascending by candidate, at most once), else
+ * ftable_dev[min * w + max] of the two lengths (gat_edr_mismatch_table,
+ * w = GAT_MAX_TRAJ_LEN + 1).  len_ids_dev holds the candidates grouped by
+ * length, ascending inside each length, length L at [len_off_dev[L],
+ * len_off_dev[L + 1]) for L in 0..GAT_MAX_TRAJ_LEN.  Writes the kk smallest
+ * by (distance, candidate) to out_*_dev[q * k + i], i < kk, ids 1-based;
+ * kk <= min(k, candidates), k <= 1024.  *status_dev (reset by the call) ends
+ * non-zero when the inputs break these rules (a survivor distance outside
+ * 0..2 * GAT_MAX_TRAJ_LEN, a survivor listed twice).  One workgroup per query;
+ * asynchronous on `stream`. */
+int gat_edr_topk(const int32_t* ftable_dev, int32_t w, const int32_t* qlen_dev, int32_t nq, const int32_t* clen_dev,
+                 int32_t nc, const int32_t* len_ids_dev, const int32_t* len_off_dev, const int32_t* surv_start_dev,
+                 const int32_t* surv_id_dev, const int32_t* surv_dist_dev, int32_t k, int32_t kk, int32_t* out_ids_dev,
+                 int32_t* out_dist_dev, int32_t* status_dev, void* stream);
+
 /* out_dev[r * (maxlen + 1) + c] = the recurrence of Grid.cpp:1264-1306 with
  * every substitution costing 1, for 0 <= r, c <= maxlen: the distance of a pair
  * of lengths r <= c with no matching point.  Asynchronous on `stream`. */

@@ -88,6 +88,32 @@ def test_similarity_batched_topk(gpu):
     assert (pi == ids).all() and (pd == dists).all()
 
 
+def test_device_topk_ties_and_sizes(gpu):
+    """The device top-k merge (k up to 1024) against the oracle where the cut
+    falls inside large groups of equal distance (600 trajectories of five
+    lengths, a third of them on one spot, so many survivors), for k = 1, a
+    few, 50 and above the database size; k = 1025 takes the host merge and
+    gives the same answers."""
+    rng = np.random.default_rng(17)
+    db = _varied_db(17, rng.choice([20, 40, 60, 64, 100], 600).astype(np.int32))
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    q = api.synth_queries(18, np.array([20, 64, 100, 300], np.int32), db, copy_mode=2)
+    full = _brute(db, q)
+    tid = np.arange(1, db.ntraj + 1, dtype=np.int32)
+    answers = {}
+    for k in (1, 7, 50, 1024, 1025):
+        ids, dists, st = g.similarity_batched(q, k, EPS)
+        assert st["pairs"] > 0
+        for i in range(q.ntraj):
+            oi, od = O.topk(full[i], tid, k)
+            n = len(oi)
+            assert n == min(k, db.ntraj)
+            assert (ids[i, :n] == oi).all() and (dists[i, :n] == od).all(), (k, i)
+            assert (ids[i, n:] == 0).all() and (dists[i, n:] == -1).all(), (k, i)
+        answers[k] = (ids[:, :db.ntraj], dists[:, :db.ntraj])
+    assert (answers[1024][0] == answers[1025][0]).all() and (answers[1024][1] == answers[1025][1]).all()
+
+
 def test_similarity_batched_cfg1(gpu):
     db = W.CFG1.db()
     g = api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).build(db)
