@@ -120,6 +120,22 @@ def test_edr_all_mismatch_table_values():
         assert F[n, m] == v, (n, m)
 
 
+def test_edr_with_matches_can_exceed_the_all_mismatch_value():
+    """The cell rule ((M == L) ? U : min(M, L, U), Grid.cpp:1297-1300) is not a
+    minimum, so a pair with matching points can end above the distance of
+    the same two lengths with no match at all: two pairs of the 20k-trajectory
+    workload against a 511-point query at eps 0.02.  The device top-k merge
+    keeps survivors at their own distance for this reason."""
+    db = W.CFG1.db()
+    q = api.synth_queries(5, np.array([511], np.int32), db, copy_mode=2)
+    F = O.ftable(600)
+    for cand, want in ((4580, 512), (8919, 529)):
+        c = db.xy(cand + 1)
+        assert O.edr(q.xy(1), c, 0.02) == want
+        n, m = sorted((511, len(c)))
+        assert F[n, m] == 511 < want
+
+
 def test_edr_eps_boundary():
     below = np.float32(4.9999997e-4)
     at = np.float32(5e-4)  # 5.0000002e-4 > 5e-4 (double)
