@@ -182,7 +182,7 @@ class Grid {
   // Top-k merge on the device: trajectories grouped by length (0-based ids,
   // ascending) with per-length offsets; per batch, each query's survivors
   // ascending by id with their distances, and the results.
-  DeviceBuffer lenIdsDev_, lenOffDev_, survStartDev_, survIdDev_, survDistDev_, topIdsDev_, topDistDev_;
+  DeviceBuffer lenIdsDev_, lenOffDev_, survDev_, topDev_;
   EventTimer edrTimer_;
   SideStream edrSide_;  // the longest EDR bucket runs here, beside the others
   std::vector<int32_t> trajLenHost_, queryLenHost_;

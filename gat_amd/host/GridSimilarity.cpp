@@ -397,44 +397,45 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
     std::vector<int32_t> fill(sstart.begin(), sstart.end() - 1);
     for (size_t i = 0; i < count; ++i) surv[size_t(fill[size_t(pairs[2 * i])]++)] = {pairs[2 * i + 1], dist[i]};
   }
-  std::vector<int32_t> sid(std::max<size_t>(count, 1)), sdist(std::max<size_t>(count, 1));
+  // One upload: [start (nq + 1) | ids (count) | distances (count)]; one
+  // readback: [ids (nq k) | distances (nq k) | status].
+  const size_t ns = size_t(nq) + 1;
+  std::vector<int32_t> up(ns + 2 * count);
+  std::copy(sstart.begin(), sstart.end(), up.begin());
   for (int q = 0; q < nq; ++q) {
     std::sort(surv.begin() + sstart[size_t(q)], surv.begin() + sstart[size_t(q) + 1]);
     for (int32_t i = sstart[size_t(q)]; i < sstart[size_t(q) + 1]; ++i) {
-      sid[size_t(i)] = surv[size_t(i)].first;
-      sdist[size_t(i)] = surv[size_t(i)].second;
+      up[ns + size_t(i)] = surv[size_t(i)].first;
+      up[ns + count + size_t(i)] = surv[size_t(i)].second;
     }
   }
   int64_t eligible = 0;
   for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L) eligible += int64_t(lengthIds_[L].size());
   const int kk = int(std::min<int64_t>(kValue, eligible));
   void* s = engine_.stream();
-  if (int rc = upload_vec(survStartDev_, sstart, s, "survivor upload")) return rc;
-  if (int rc = upload_vec(survIdDev_, sid, s, "survivor upload")) return rc;
-  if (int rc = upload_vec(survDistDev_, sdist, s, "survivor upload")) return rc;
-  const size_t cells = std::max<size_t>(size_t(nq) * size_t(kValue), 1);
-  if (int rc = topIdsDev_.reserve(cells * 4)) return rc;
-  if (int rc = topDistDev_.reserve(cells * 4)) return rc;
-  if (int rc = counterDev_.reserve(16)) return rc;
+  if (int rc = upload_vec(survDev_, up, s, "survivor upload")) return rc;
+  const size_t cells = size_t(nq) * size_t(kValue);
+  if (int rc = topDev_.reserve((2 * cells + 1) * 4)) return rc;
+  const int32_t* sv = survDev_.as<int32_t>();
+  int32_t* top = topDev_.as<int32_t>();
   GATH_TRY(gat_edr_topk(ftableDev_.as<int32_t>(), GAT_MAX_TRAJ_LEN + 1, queryLen_.as<int32_t>(), nq,
-                        trajLen_.as<int32_t>(), trajNum, lenIdsDev_.as<int32_t>(), lenOffDev_.as<int32_t>(),
-                        survStartDev_.as<int32_t>(), survIdDev_.as<int32_t>(), survDistDev_.as<int32_t>(), kValue, kk,
-                        topIdsDev_.as<int32_t>(), topDistDev_.as<int32_t>(), counterDev_.as<int32_t>(), s),
+                        trajLen_.as<int32_t>(), trajNum, lenIdsDev_.as<int32_t>(), lenOffDev_.as<int32_t>(), sv,
+                        sv + ns, sv + ns + count, kValue, kk, top, top + cells, top + 2 * cells, s),
            "top-k merge");
-  std::vector<int32_t> ids(cells), dists(cells);
-  int32_t status = 0;
-  GATH_TRY(gat_memcpy_d2h(ids.data(), topIdsDev_.get(), cells * 4, s), "top-k readback");
-  GATH_TRY(gat_memcpy_d2h(dists.data(), topDistDev_.get(), cells * 4, s), "top-k readback");
-  GATH_TRY(gat_memcpy_d2h(&status, counterDev_.get(), 4, s), "top-k readback");
+  std::vector<int32_t> down(2 * cells + 1);
+  GATH_TRY(gat_memcpy_d2h(down.data(), topDev_.get(), down.size() * 4, s), "top-k readback");
   GATH_TRY(gat_stream_sync(s), "top-k readback");
+  const int32_t status = down[2 * cells];
   if (status != 0) {
     gath_set_error("top-k merge: inconsistent inputs (status %d)", status);
     return GAT_ERR_HIP;
   }
+  const auto ids = down.begin();
+  const auto dists = down.begin() + int64_t(cells);
   out.assign(size_t(nq), TopK{});
   for (int q = 0; q < nq; ++q) {
-    out[size_t(q)].ids.assign(ids.begin() + int64_t(q) * kValue, ids.begin() + int64_t(q) * kValue + kk);
-    out[size_t(q)].dist.assign(dists.begin() + int64_t(q) * kValue, dists.begin() + int64_t(q) * kValue + kk);
+    out[size_t(q)].ids.assign(ids + int64_t(q) * kValue, ids + int64_t(q) * kValue + kk);
+    out[size_t(q)].dist.assign(dists + int64_t(q) * kValue, dists + int64_t(q) * kValue + kk);
   }
   return GAT_OK;
 }
