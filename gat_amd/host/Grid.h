@@ -185,6 +185,7 @@ class Grid {
   DeviceBuffer lenIdsDev_, lenOffDev_, survDev_, topDev_;
   EventTimer edrTimer_;
   SideStream edrSide_;  // the longest EDR bucket runs here, beside the others
+  SideStream edrSide2_;  // one-wave long pairs of a batch whose longest run over several waves
   std::vector<int32_t> trajLenHost_, queryLenHost_;
   std::vector<int32_t> ftable_;
   std::vector<std::vector<int32_t>> lengthIds_;  // per length: trajectory ids, ascending

@@ -281,17 +281,24 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     maxlen[bucket] = std::max(maxlen[bucket], std::min(m, GAT_MAX_TRAJ_LEN));
     key[i] = (bucket * kS + (kS - 1 - strips_of(n))) * kM + (kM - 1 - m);
   }
-  std::vector<int64_t> perm(static_cast<size_t>(np));
-  std::iota(perm.begin(), perm.end(), int64_t(0));
-  if (np < (int64_t(1) << 16)) {
-    // Few pairs (a serving batch): a comparison sort beats clearing and
-    // scanning the 2 x 129 x 2049-entry count table.
-    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
-  } else {
-    std::vector<int64_t> cnt(size_t(kBuckets * kS) * kM + 1, 0);
-    for (int64_t i = 0; i < np; ++i) ++cnt[key[i] + 1];
-    for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
-    for (int64_t i = 0; i < np; ++i) perm[cnt[key[i]]++] = i;
+  // Stable order by key: two counting passes over 11-bit digits (keys are
+  // below kBuckets * kS * kM < 2^22).
+  static_assert(int64_t(kBuckets) * kS * kM <= (int64_t(1) << 22), "pair keys must fit two 11-bit digits");
+  std::vector<int64_t> perm(static_cast<size_t>(np)), tmp(static_cast<size_t>(np));
+  {
+    constexpr int kDigit = 11, kRadix = 1 << kDigit;
+    std::vector<int64_t> cnt(kRadix + 1);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t i = 0; i < np; ++i) ++cnt[size_t(key[i] & (kRadix - 1)) + 1];
+    for (int d = 1; d <= kRadix; ++d) cnt[size_t(d)] += cnt[size_t(d) - 1];
+    for (int64_t i = 0; i < np; ++i) tmp[size_t(cnt[size_t(key[i] & (kRadix - 1))]++)] = i;
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t i = 0; i < np; ++i) ++cnt[size_t(key[i] >> kDigit) + 1];
+    for (int d = 1; d <= kRadix; ++d) cnt[size_t(d)] += cnt[size_t(d) - 1];
+    for (int64_t j = 0; j < np; ++j) {
+      const int64_t i = tmp[size_t(j)];
+      perm[size_t(cnt[size_t(key[i] >> kDigit)]++)] = i;
+    }
   }
   std::vector<int32_t> sorted(size_t(np) * 2);
   std::vector<gat_edr_job> jobs;
@@ -334,20 +341,30 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
              "batched EDR");
     return GAT_OK;
   };
+  // Several-wave pairs (bucket 3) and one-wave long pairs (bucket 2) each get
+  // a side stream when other buckets have jobs, so the three run together
+  // instead of the long ones queueing behind the short ones.
   const int longest = bucketJobs[3] > 0 ? 3 : 2;
   int64_t others = 0;
   for (int bucket = 0; bucket < kBuckets; ++bucket) others += bucket == longest ? 0 : bucketJobs[bucket];
   const bool side = bucketJobs[longest] > 0 && others > 0;
+  const bool side2 = side && longest == 3 && bucketJobs[2] > 0 && others > bucketJobs[2];
   if (side) {
     if (int rc = edrSide_.fork(s)) return rc;
     if (int rc = launch(longest, edrSide_.stream())) return rc;
   }
+  if (side2) {
+    if (int rc = edrSide2_.fork(s)) return rc;
+    if (int rc = launch(2, edrSide2_.stream())) return rc;
+  }
   for (int bucket = 0; bucket < kBuckets; ++bucket) {
-    if (bucketJobs[bucket] == 0 || (side && bucket == longest)) continue;
+    if (bucketJobs[bucket] == 0 || (side && bucket == longest) || (side2 && bucket == 2)) continue;
     if (int rc = launch(bucket, s)) return rc;
   }
   if (side)
     if (int rc = edrSide_.join(s)) return rc;
+  if (side2)
+    if (int rc = edrSide2_.join(s)) return rc;
   if (int rc = edrTimer_.stop(s)) return rc;
   std::vector<int32_t> out(static_cast<size_t>(np));
   GATH_TRY(gat_memcpy_d2h(out.data(), outDev_.get(), size_t(np) * 4, s), "batched EDR readback");
