@@ -408,23 +408,35 @@ __device__ __forceinline__ bool may_match(const float4& q, const float4& c, floa
   return !((c.x - q.z) >= u || (q.x - c.z) >= u || (c.y - q.w) >= u || (q.y - c.w) >= u);
 }
 
+// A workgroup tests kFilterQ queries against its candidates: each lane keeps
+// its candidate's box in registers across the queries, so the candidate boxes
+// are read once per kFilterQ queries instead of once per query.  The order of
+// the output pairs is not defined.
+constexpr int kFilterQ = 16;
+
 __global__ __launch_bounds__(256) void edr_filter_kernel(const float4* __restrict__ qbox, int32_t nq,
                                                          const float4* __restrict__ cbox, int32_t nc, float u,
                                                          int2* __restrict__ out, uint64_t capacity,
                                                          unsigned long long* __restrict__ count) {
-  const int q = blockIdx.y;
-  if (q >= nq) return;
-  const float4 qb = qbox[q];
+  __shared__ float4 qs[kFilterQ];
+  const int q0 = int(blockIdx.y) * kFilterQ;
+  if (threadIdx.x < kFilterQ && q0 + int(threadIdx.x) < nq) qs[threadIdx.x] = qbox[q0 + int(threadIdx.x)];
+  __syncthreads();
+  const int nqb = min(kFilterQ, nq - q0);
   const int lane = threadIdx.x & 63;
   for (int c = blockIdx.x * 256 + threadIdx.x; c - lane < nc; c += gridDim.x * 256) {
-    const bool keep = c < nc && may_match(qb, cbox[c], u);
-    const uint64_t mask = __ballot(keep);
-    if (mask == 0) continue;
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(mask));
-    base = __builtin_amdgcn_readfirstlane(uint32_t(base)) | (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base >> 32))) << 32);
-    const uint64_t at = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-    if (keep && at < capacity) out[at] = make_int2(q, c);
+    const float4 cb = c < nc ? cbox[c] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int j = 0; j < nqb; ++j) {
+      const bool keep = c < nc && may_match(qs[j], cb, u);
+      const uint64_t mask = __ballot(keep);
+      if (mask == 0) continue;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+      base = __builtin_amdgcn_readfirstlane(uint32_t(base)) |
+             (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base >> 32))) << 32);
+      const uint64_t at = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+      if (keep && at < capacity) out[at] = make_int2(q0 + j, c);
+    }
   }
 }
 
@@ -698,7 +710,7 @@ extern "C" int gat_edr_filter(const float* qbox, int32_t nq, const float* cbox, 
   GAT_REQUIRE(eps > 0, "EDR filter: eps must be positive");
   int64_t bx = (int64_t(nc) + 255) / 256;
   if (bx > 64) bx = 64;
-  edr_filter_kernel<<<dim3(uint32_t(bx), uint32_t(nq)), dim3(256), 0, s>>>(
+  edr_filter_kernel<<<dim3(uint32_t(bx), uint32_t((nq + kFilterQ - 1) / kFilterQ)), dim3(256), 0, s>>>(
       reinterpret_cast<const float4*>(qbox), nq, reinterpret_cast<const float4*>(cbox), nc, gat_eps_threshold(eps),
       reinterpret_cast<int2*>(pairs_out), uint64_t(capacity), reinterpret_cast<unsigned long long*>(count));
   GAT_LAUNCH_CHECK("edr_filter_kernel");

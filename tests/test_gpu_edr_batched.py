@@ -147,6 +147,34 @@ def test_filter_boundary_is_exact(gpu):
     assert O.edr(np.array([[0.0, 0.35]], np.float32), np.array([[below, 0.35]], np.float32), EPS) == 0
 
 
+def test_filter_many_queries_matches_the_box_rule(gpu):
+    """The filter over 37 queries (blocks of 16 queries and a partial one)
+    against 1,500 candidate boxes keeps exactly the pairs the float32 box rule
+    keeps, each once (output order is not defined)."""
+    h = N.hip()
+    rng = np.random.default_rng(11)
+    u = np.float32(api.eps_threshold(EPS))
+
+    def boxes(n, side):
+        lo = rng.random((n, 2), dtype=np.float32) * np.float32(0.05)
+        hi = lo + rng.random((n, 2), dtype=np.float32) * np.float32(side)
+        return np.ascontiguousarray(np.concatenate([lo, hi], 1))
+
+    qb, cb = boxes(37, 0.01), boxes(1500, 0.002)
+    keep = ~(((cb[None, :, 0] - qb[:, None, 2]) >= u) | ((qb[:, None, 0] - cb[None, :, 2]) >= u) |
+             ((cb[None, :, 1] - qb[:, None, 3]) >= u) | ((qb[:, None, 1] - cb[None, :, 3]) >= u))
+    want = {(int(a), int(b)) for a, b in np.argwhere(keep)}
+    assert 0 < len(want) < keep.size
+    dq, dc = DevBuf.of(qb), DevBuf.of(cb)
+    cap = keep.size
+    out = DevBuf(8 * cap)
+    cnt = DevBuf(8)
+    N.check_hip(h.gat_edr_filter(dq.ptr, 37, dc.ptr, 1500, EPS, out.ptr, cap, cnt.ptr, None))
+    n = int(cnt.get(np.uint64, 1)[0])
+    got = [tuple(p) for p in out.get(np.int32, 2 * n).reshape(-1, 2).tolist()]
+    assert n == len(want) and len(set(got)) == n and set(got) == want
+
+
 def test_autotuned_edr_gives_the_same_distances(gpu):
     db = _varied_db(77, [1, 17, 64, 300, 511, 513, 700, 1024, 1025, 2048] + list(range(20, 1200, 37)))
     g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
