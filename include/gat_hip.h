@@ -273,9 +273,9 @@ int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job*
 /* Candidate filter for exact top-k pruning (no reference counterpart; the
  * reference orders candidates by an unproven frequency-distance bound,
  * FVTable.cpp:77-207): writes every (query, candidate) index pair whose
- * eps-dilated bounding boxes can hold a matching point pair to pairs_out_dev
- * (up to `capacity` pairs; *count_dev, reset by the call, ends as the full
- * number).  Boxes are (xmin, ymin, xmax, ymax) floats per trajectory.  The test
+ * eps-dilated bounding boxes can hold a matching point pair to pairs_out_dev,
+ * each once, in no defined order (up to `capacity` pairs; *count_dev, reset by
+ * the call, ends as the full number).  Boxes are (xmin, ymin, xmax, ymax) floats per trajectory.  The test
  * is exactly conservative under float rounding, so every dropped pair has no
  * matching point pair and its distance is gat_edr_mismatch_table's value.
  * nq <= 65535.  Asynchronous on `stream`. */
