@@ -140,7 +140,7 @@ HOST_FUNCS = {
     "gath_index_set_batch_params": (INT, [P, INT, INT]),
     "gath_index_last_unique_candidates": (INT, [P, P]),
     "gath_index_last_counts": (INT, [P, P]),
-    "gath_serve": (INT, [P, P, INT, P, P, P, P, P, P, P]),
+    "gath_serve": (INT, [P, P, INT, P, P, I64, P, P, P, P, P]),
     "gath_mixed_new": (P, [P, P, P]),
     "gath_mixed_free": (None, [P]),
     "gath_mixed_run": (INT, [P, P, INT, P, P, P, P, P]),

@@ -566,13 +566,18 @@ def serve(grid: "GridIndex", requests: np.ndarray, qlon: np.ndarray, qlat: np.nd
     n = len(r)
     lon = _req(qlon, np.float32)
     lat = _req(qlat, np.float32)
+    if len(lon) != len(lat):
+        raise ValueError("qlon and qlat must have the same length")
+    topk = r["kind"] == 1
+    if topk.any() and ((r["qoff"][topk] < 0) | (r["qoff"][topk] + r["qlen"][topk] > len(lon))).any():
+        raise ValueError("a top-k request reads past the query point arrays")
     out = np.zeros(max(n, 1), OUTCOME)
     ids = np.zeros((max(n, 1), k), np.int32)
     dists = np.zeros((max(n, 1), k), np.int32)
     cfg = np.array([range_slots, max_range_batch, max_topk_batch, range_budget, topk_budget, k, eps, int(batched)],
                    np.float64)
     st = np.zeros(9, np.float64)
-    N.check_host(N.host().gath_serve(grid.handle, r.ctypes.data, n, _ptr(lon), _ptr(lat), _ptr(cfg),
+    N.check_host(N.host().gath_serve(grid.handle, r.ctypes.data, n, _ptr(lon), _ptr(lat), len(lon), _ptr(cfg),
                                      out.ctypes.data, _ptr(ids), _ptr(dists), _ptr(st)))
     keys = ["range_batches", "topk_batches", "range_done", "topk_done", "wall_ms", "peak_range_bytes",
             "peak_topk_bytes", "range_budget", "topk_budget"]
@@ -604,11 +609,11 @@ class MixedRange:
         counts = np.zeros(max(n, 1), np.uint32)
         digests = np.zeros(max(n, 1), np.uint64)
         route = np.zeros(max(n, 1), np.int8)
-        st = np.zeros(6, np.float64)
+        st = np.zeros(7, np.float64)
         cfg = np.array([fsg_max_area, grid_max_area, group_size, int(graph)], np.float64)
         N.check_host(N.host().gath_mixed_run(self.handle, _ptr(b), n, _ptr(cfg), _ptr(counts), _ptr(digests),
                                              _ptr(route), _ptr(st)))
-        keys = ["pairs", "candidates", "groups", "replays", "ms", "digest_ms"]
+        keys = ["pairs", "candidates", "groups", "replays", "ms", "digest_ms", "captures"]
         return counts[:n], digests[:n], route[:n], dict(zip(keys, st.tolist()))
 
 

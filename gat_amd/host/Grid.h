@@ -167,6 +167,7 @@ class Grid {
  private:
   int buildNode(int level, int id);
   int ensureQueryStore(const TrajectoryView& queries);
+  int checkTrajLengths(const char* what) const;
   int runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* stats);
   // Top-k of up to kDeviceTopkMax per query merged on the device; larger k on the host.
   static constexpr int kDeviceTopkMax = 1024;
@@ -187,6 +188,7 @@ class Grid {
   SideStream edrSide_;  // the longest EDR bucket runs here, beside the others
   SideStream edrSide2_;  // one-wave long pairs of a batch whose longest run over several waves
   std::vector<int32_t> trajLenHost_, queryLenHost_;
+  int badTrajLen_ = 0;  // first trajectory (1-based) whose length the EDR kernels reject, or 0
   std::vector<int32_t> ftable_;
   std::vector<std::vector<int32_t>> lengthIds_;  // per length: trajectory ids, ascending
   // Per query length a: the index's trajectory lengths L ordered by their

@@ -72,6 +72,9 @@ int Grid::uploadTrajectories(const TrajectoryView& db) {
   const int T = db.trajNum;
   Padded p = pad_store(db);
   trajLenHost_ = p.lengths;
+  badTrajLen_ = 0;
+  for (int t = 0; t < T && badTrajLen_ == 0; ++t)
+    if (trajLenHost_[t] < 1 || trajLenHost_[t] > GAT_MAX_TRAJ_LEN) badTrajLen_ = t + 1;
   std::vector<gat_edr_task> tasks(static_cast<size_t>(T));
   for (int t = 0; t < T; ++t) tasks[t] = gat_edr_task{0, t + 1};
   void* s = engine_.stream();
@@ -137,13 +140,18 @@ void topk_of(const std::vector<int32_t>& dist, int k, TopK& out) {
 }
 }  // namespace
 
+// Every indexed trajectory must have a length the EDR kernels take: one
+// outside 1..GAT_MAX_TRAJ_LEN would come back as distance -1 (the kernels'
+// marker for a malformed pair), which a top-k would rank first.
+int Grid::checkTrajLengths(const char* what) const {
+  if (badTrajLen_ == 0) return GAT_OK;
+  gath_set_error("%s: trajectory %d has %d points (supported: 1..%d)", what, badTrajLen_,
+                 trajLenHost_[size_t(badTrajLen_) - 1], GAT_MAX_TRAJ_LEN);
+  return GAT_ERR_ARG;
+}
+
 int Grid::similarityPlain(const TrajectoryView& queries, int kValue, std::vector<TopK>& out) {
-  for (int t = 0; t < trajNum; ++t)
-    if (trajLenHost_[t] < 1 || trajLenHost_[t] > GAT_MAX_TRAJ_LEN) {
-      gath_set_error("similarity: trajectory %d has %d points (supported: 1..%d)", t + 1, trajLenHost_[t],
-                     GAT_MAX_TRAJ_LEN);
-      return GAT_ERR_ARG;
-    }
+  if (int rc = checkTrajLengths("similarity")) return rc;
   if (int rc = ensureQueryStore(queries)) return rc;
   out.assign(size_t(queries.trajNum), TopK{});
   std::vector<int32_t> dist(static_cast<size_t>(trajNum));
@@ -458,6 +466,7 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
 }
 
 int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vector<TopK>& out, EdrBatchStats* st) {
+  if (int rc = checkTrajLengths("similarityBatched")) return rc;
   if (int rc = ensureQueryStore(queries)) return rc;
   const int nq = queries.trajNum;
   if (nq > 65535) {
@@ -482,7 +491,14 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
     GATH_TRY(gat_memcpy_d2h(&count, counterDev_.get(), 8, s), "filter readback");
     GATH_TRY(gat_stream_sync(s), "filter readback");
     if (int64_t(count) <= cap) break;
-    cap = int64_t(count) + int64_t(count) / 4 + 1024;
+    // Pair indexes, job offsets and the survivor lists are 32-bit.
+    if (count > uint64_t(INT32_MAX)) {
+      gath_set_error("similarityBatched: %llu candidate pairs pass the filter (at most %d per call; split the "
+                     "queries into smaller batches)",
+                     (unsigned long long)count, INT32_MAX);
+      return GAT_ERR_CAPACITY;
+    }
+    cap = std::min<int64_t>(int64_t(count) + int64_t(count) / 4 + 1024, INT32_MAX);
   }
   float fms = 0;
   if (int rc = ft.elapsed(&fms)) return rc;

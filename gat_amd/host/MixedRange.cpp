@@ -45,7 +45,15 @@ int MixedRange::capture(const int nq[3]) {
   GATH_TRY(rc_end, "mixed capture");
   graph_ = g;
   std::copy(nq, nq + 3, shape_);
+  for (int i = 0; i < 3; ++i) keys_[i] = engine(i).layoutKey(tree(i), nq[i], batch_[i]);
+  ++captures;
   return GAT_OK;
+}
+
+bool MixedRange::keysMatch(const int nq[3]) {
+  for (int i = 0; i < 3; ++i)
+    if (nq[i] > 0 && engine(i).layoutKey(tree(i), nq[i], batch_[i]) != keys_[i]) return false;
+  return true;
 }
 
 int MixedRange::run(const MBB* bounds, int n, const MixedConfig& cfg, MixedResult* out) {
@@ -105,10 +113,13 @@ int MixedRange::run(const MBB* bounds, int n, const MixedConfig& cfg, MixedResul
         if (nq[i] > 0) rc = engine(i).stageBatch(boxes[i].data(), nq[i]);
       for (int i = 0; i < 3 && rc == GAT_OK; ++i)
         if (nq[i] > 0) rc = gat_stream_sync(engine(i).stream());
-      if (rc == GAT_OK) rc = gat_graph_launch(graph_, coord_.get());
+      // A workspace that moved since the capture: the graph's kernels would
+      // read freed memory, so this group runs step by step and is recaptured.
+      const bool current = rc == GAT_OK && keysMatch(nq);
+      if (rc == GAT_OK && current) rc = gat_graph_launch(graph_, coord_.get());
       if (rc == GAT_OK) rc = coord_.sync();
-      bool fits = true;
-      for (int i = 0; i < 3 && rc == GAT_OK; ++i) {
+      bool fits = current;
+      for (int i = 0; i < 3 && rc == GAT_OK && current; ++i) {
         bool f = true;
         if (nq[i] > 0) rc = engine(i).finishBatch(nq[i], batch_[i], &f);
         fits = fits && f;

@@ -11,7 +11,8 @@
 // group.  The first group of a shape is run step by step on every index
 // (sizing the workspaces), then the three device pipelines are captured into
 // one graph — forked from a coordinating stream onto each index's stream and
-// joined back — and every later group of that shape is one graph launch.
+// joined back — and every later group of that shape is one graph launch, as
+// long as no engine's workspace has moved since the capture.
 #pragma once
 
 #include <cstdint>
@@ -61,4 +62,11 @@ class MixedRange {
   void* join_[3] = {nullptr, nullptr, nullptr};
   void* graph_ = nullptr;
   int shape_[3] = {-1, -1, -1};
+  // Each engine's layout key at capture: the graph replays only while all
+  // three still hold (an engine's own range_batched, tune or reserve call can
+  // grow and so reallocate its workspace between two runs).
+  RangeEngine::GraphKey keys_[3];
+  bool keysMatch(const int nq[3]);
+ public:
+  int captures = 0;  // graphs captured over this object's life
 };

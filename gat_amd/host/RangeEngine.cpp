@@ -617,6 +617,25 @@ int RangeEngine::runDirect(const DeviceTree& tree, int nq, RangeBatch& out) {
   return GAT_OK;
 }
 
+RangeEngine::GraphKey RangeEngine::layoutKey(const DeviceTree& tree, int nq, const RangeBatch& out) const {
+  GraphKey key;
+  const void* p[GraphKey::kPtrs] = {tree.nodes.get(), tree.cellWords.get(), devBoxes_.get(), out.pairs.get(),
+                                    out.qbase.get(),  out.qcount.get(),     out.runs.get(),  lq_.get(),
+                                    nodeCount_.get(), nodeFill_.get(),      nodeOff_.get(),  tiles_.get(),
+                                    cand_.get(),      qlist_.get(),         items_.get(),    itemBase_.get(),
+                                    itemTiles_.get(), counters_.get(),      store_.xy_dev,   store_.id_dev};
+  std::copy(p, p + GraphKey::kPtrs, key.ptr);
+  key.runs = runIndexed;
+  key.nq = nq;
+  key.lqCap = lqCap_;
+  key.qlistCap = qlistCap_;
+  key.itemsCap = itemsCap_;
+  key.pairsCap = pairsCap_;
+  key.runsCap = int64_t(out.runs.size() / sizeof(gat_range_run));
+  key.qb = queriesPerItem;
+  return key;
+}
+
 void RangeEngine::dropGraph() {
   if (graph_) gat_graph_destroy(graph_);
   graph_ = nullptr;
@@ -869,23 +888,7 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   auto fits = [&] { return countersFit(); };
   auto finish = [&](bool replay) -> int { return finishCounters(out, replay); };
   GraphKey key;
-  auto make_key = [&] {
-    key.ptr[0] = tree.nodes.get();
-    key.ptr[1] = devBoxes_.get();
-    key.ptr[2] = out.pairs.get();
-    key.ptr[3] = out.qbase.get();
-    key.ptr[4] = out.qcount.get();
-    key.ptr[5] = lq_.get();
-    key.ptr[6] = out.runs.get();
-    key.ptr[7] = tree.cellWords.get();
-    key.runs = runIndexed;
-    key.nq = nq;
-    key.lqCap = lqCap_;
-    key.qlistCap = qlistCap_;
-    key.itemsCap = itemsCap_;
-    key.pairsCap = pairsCap_;
-    key.qb = queriesPerItem;
-  };
+  auto make_key = [&] { key = layoutKey(tree, nq, out); };
   if (graph && graph_) {
     make_key();
     if (key == graphKey_) {

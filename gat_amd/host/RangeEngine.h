@@ -184,6 +184,27 @@ class RangeEngine {
   int enqueueBatch(const DeviceTree& tree, int nq, RangeBatch& out);
   int finishBatch(int nq, RangeBatch& out, bool* fits);
 
+  // Everything a captured batch pipeline has baked into its kernel arguments:
+  // every device buffer it reads or writes, the capacities it was sized with,
+  // the batch size and the layout switches.  A captured graph may be replayed
+  // only while the key it was captured under still equals layoutKey(): any
+  // workspace growth reallocates a buffer and changes the key.
+  struct GraphKey {
+    static constexpr int kPtrs = 20;
+    const void* ptr[kPtrs] = {};
+    int64_t nq = -1, lqCap = 0, qlistCap = 0, itemsCap = 0, pairsCap = 0, runsCap = 0;
+    int qb = 0;
+    bool runs = false;
+    bool operator==(const GraphKey& o) const {
+      for (int i = 0; i < kPtrs; ++i)
+        if (ptr[i] != o.ptr[i]) return false;
+      return nq == o.nq && lqCap == o.lqCap && qlistCap == o.qlistCap && itemsCap == o.itemsCap &&
+             pairsCap == o.pairsCap && runsCap == o.runsCap && qb == o.qb && runs == o.runs;
+    }
+    bool operator!=(const GraphKey& o) const { return !(*this == o); }
+  };
+  GraphKey layoutKey(const DeviceTree& tree, int nq, const RangeBatch& out) const;
+
   // Shares `src`'s resident store (which must outlive this engine): a second
   // engine over the same points with its own stream and batch workspace.
   int attachStore(const RangeEngine& src);
@@ -228,16 +249,5 @@ class RangeEngine {
   EventTimer selTimer_, refineTimer_;
   bool timeRefine_ = false;
   void* graph_ = nullptr;
-  struct GraphKey {
-    const void* ptr[8] = {};  // tree nodes, boxes, pairs, qbase, qcount, lq, runs, cell words
-    int64_t nq = -1, lqCap = 0, qlistCap = 0, itemsCap = 0, pairsCap = 0;
-    int qb = 0;
-    bool runs = false;
-    bool operator==(const GraphKey& o) const {
-      for (int i = 0; i < 8; ++i)
-        if (ptr[i] != o.ptr[i]) return false;
-      return nq == o.nq && lqCap == o.lqCap && qlistCap == o.qlistCap && itemsCap == o.itemsCap &&
-             pairsCap == o.pairsCap && qb == o.qb && runs == o.runs;
-    }
-  } graphKey_;
+  GraphKey graphKey_;
 };

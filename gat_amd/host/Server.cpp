@@ -51,8 +51,8 @@ double Server::rangeBytes(const float* b) const {
 
 double Server::topkBytes(int qlen) const { return 16.0 * qlen + 64.0 * 1024.0; }
 
-int Server::run(const gath_request* reqs, int n, const float* qlon, const float* qlat, gath_outcome* out,
-                int32_t* topkIds, int32_t* topkDists, ServeStats* stats) {
+int Server::run(const gath_request* reqs, int n, const float* qlon, const float* qlat, int64_t nqpoints,
+                gath_outcome* out, int32_t* topkIds, int32_t* topkDists, ServeStats* stats) {
   if (n < 0 || (n > 0 && (!reqs || !out)) || cfg_.k < 1 || cfg_.rangeSlots < 1 || cfg_.maxRangeBatch < 1 ||
       cfg_.maxTopkBatch < 1 || cfg_.maxTopkBatch > 65535) {
     gath_set_error("serve: bad arguments");
@@ -73,6 +73,11 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
       anyTopk = true;
       if (r.qlen < 1 || r.qlen > GAT_MAX_TRAJ_LEN || r.qoff < 0 || !qlon || !qlat || !topkIds || !topkDists) {
         gath_set_error("serve: bad top-k request %d", i);
+        return GAT_ERR_ARG;
+      }
+      if (r.qoff > nqpoints - r.qlen) {
+        gath_set_error("serve: top-k request %d reads query points [%lld, %lld) of %lld", i, (long long)r.qoff,
+                       (long long)r.qoff + r.qlen, (long long)nqpoints);
         return GAT_ERR_ARG;
       }
     }

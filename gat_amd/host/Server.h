@@ -45,9 +45,10 @@ class Server {
   // Replays `reqs` (arrival times in ms from the start of the run, ascending)
   // and fills one outcome per request; top-k requests also fill k ids and
   // distances at topkIds / topkDists + i * k (ascending (distance, id)).
-  // Top-k query points are (qlon, qlat)[qoff, qoff + qlen).
-  int run(const gath_request* reqs, int n, const float* qlon, const float* qlat, gath_outcome* out, int32_t* topkIds,
-          int32_t* topkDists, ServeStats* stats);
+  // Top-k query points are (qlon, qlat)[qoff, qoff + qlen), inside the
+  // arrays' nqpoints points.
+  int run(const gath_request* reqs, int n, const float* qlon, const float* qlat, int64_t nqpoints, gath_outcome* out,
+          int32_t* topkIds, int32_t* topkDists, ServeStats* stats);
 
   // Device bytes the candidate and result buffers of one request need (the
   // admission estimate).

@@ -643,14 +643,15 @@ int gath_mixed_run(void* m, const float* boxes, int nq, const double* cfg, uint3
   std::copy(r.digest.begin(), r.digest.end(), digests);
   std::copy(r.route.begin(), r.route.end(), route);
   if (stats) {
-    const double v[6] = {double(r.pairs), double(r.candidates), double(r.groups), double(r.replays), r.ms, r.digestMs};
-    for (int i = 0; i < 6; ++i) stats[i] = v[i];
+    const double v[7] = {double(r.pairs), double(r.candidates), double(r.groups), double(r.replays),
+                         r.ms,           r.digestMs,           double(mr->captures)};
+    for (int i = 0; i < 7; ++i) stats[i] = v[i];
   }
   return GAT_OK;
 }
 
-int gath_serve(void* p, const gath_request* reqs, int n, const float* qlon, const float* qlat, const double* cfg,
-               gath_outcome* out, int32_t* topk_ids, int32_t* topk_dists, double* stats) {
+int gath_serve(void* p, const gath_request* reqs, int n, const float* qlon, const float* qlat, int64_t nqpoints,
+               const double* cfg, gath_outcome* out, int32_t* topk_ids, int32_t* topk_dists, double* stats) {
   Index* ix = as_index(p);
   if (!ix || !ix->grid || !cfg) return bad("gath_serve: bad arguments (a Grid index and a config are required)");
   ServeConfig c;
@@ -664,7 +665,7 @@ int gath_serve(void* p, const gath_request* reqs, int n, const float* qlon, cons
   c.batched = cfg[7] != 0;
   ServeStats st;
   Server server(*ix->grid, c);
-  if (int rc = server.run(reqs, n, qlon, qlat, out, topk_ids, topk_dists, &st)) return rc;
+  if (int rc = server.run(reqs, n, qlon, qlat, nqpoints, out, topk_ids, topk_dists, &st)) return rc;
   if (stats) {
     const double v[9] = {double(st.rangeBatches), double(st.topkBatches), double(st.rangeDone), double(st.topkDone),
                          st.wallMs, st.peakRangeBytes, st.peakTopkBytes, st.rangeBudgetBytes, st.topkBudgetBytes};

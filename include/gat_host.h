@@ -198,8 +198,11 @@ int gath_tune_edr(void* grid, const int64_t* q_offsets, int nq, const float* qlo
  * FSG (area < cfg[0]), the GT-quadtree (area < cfg[1]) or STIG, in groups of
  * cfg[2] rectangles per index, the three device pipelines of a group captured
  * as one graph and replayed (cfg[3] != 0).  Per rectangle: counts, digests
- * (as gat_pairs_digest), route (0 FSG, 1 GT-quadtree, 2 STIG); stats[0..5] =
- * pairs, candidates, groups, graph replays, wall ms, of which digest ms. */
+ * (as gat_pairs_digest), route (0 FSG, 1 GT-quadtree, 2 STIG); stats[0..6] =
+ * pairs, candidates, groups, graph replays, wall ms, of which digest ms, and
+ * the graphs this object has captured so far (a group whose engines' buffers
+ * moved since the capture, e.g. grown by an index's own batched call, is run
+ * step by step and recaptured instead of replayed). */
 void* gath_mixed_new(void* grid, void* stig, void* fsg);
 void gath_mixed_free(void* mixed);
 int gath_mixed_run(void* mixed, const float* boxes, int nq, const double* cfg, uint32_t* counts, uint64_t* digests,
@@ -235,8 +238,8 @@ typedef struct gath_outcome {
  * short answer).  stats[0..8] = range batches, top-k batches, range done,
  * top-k done, wall ms, largest range batch estimate, largest top-k batch
  * estimate, range budget and top-k budget after the free-memory check. */
-int gath_serve(void* grid, const gath_request* reqs, int n, const float* qlon, const float* qlat, const double* cfg,
-               gath_outcome* out, int32_t* topk_ids, int32_t* topk_dists, double* stats);
+int gath_serve(void* grid, const gath_request* reqs, int n, const float* qlon, const float* qlat, int64_t nqpoints,
+               const double* cfg, gath_outcome* out, int32_t* topk_ids, int32_t* topk_dists, double* stats);
 
 #pragma GCC visibility pop
 

@@ -61,6 +61,13 @@ _FUNCS = {
     "orc_edr_batch": (INT, [P, INT, P, INT, P, F64, P, INT]),
     "orc_ftable": (INT, [INT, P]),
     "orc_topk": (INT, [P, P, INT, INT, P, P]),
+    "orc_morton_x86": (INT, [INT, INT]),
+    "orc_fv_neighbors": (INT, [INT, INT, P]),
+    "orc_fd_value": (INT, [P, INT, P, INT, INT]),
+    "orc_fv_new": (P, [P, F32, P, INT, P, P]),
+    "orc_fv_free": (None, [P]),
+    "orc_fv_fd": (INT, [P, P, INT, P]),
+    "orc_fv_topk": (INT, [P, P, INT, P, INT, F64, P, P, P]),
 }
 
 _lib = None
@@ -325,3 +332,57 @@ def topk(dist, ids, k):
     oi, od = np.zeros(k, np.int32), np.zeros(k, np.int32)
     n = lib().orc_topk(_p(dist), _p(ids), len(dist), k, _p(oi), _p(od))
     return oi[:n], od[:n]
+
+
+# ---- the reference's own CPU top-k: frequency-distance queue + 80-candidate
+# rounds (FVTable.cpp:18-207, Grid.cpp:1021-1220).  Not exact; a timed baseline.
+
+def morton_x86(x: int, y: int) -> int:
+    """getIdxFromXY as the reference's x86-64 build evaluates it, negative
+    coordinates included (FVTable.cpp:67-73 passes them at the grid's edge)."""
+    return lib().orc_morton_x86(x, y)
+
+
+def fv_neighbors(cell: int, cell_num: int) -> np.ndarray:
+    out = np.zeros(8, np.int32)
+    lib().orc_fv_neighbors(cell, cell_num, _p(out))
+    return out
+
+
+def fd_value(t: dict, q: dict, cell_num: int) -> int:
+    """Frequency distance of two {virtual cell: points} vectors (FVTable.cpp:86-198)."""
+    a = np.array(sorted(t.items()), np.int32).reshape(-1, 2)
+    b = np.array(sorted(q.items()), np.int32).reshape(-1, 2)
+    return lib().orc_fd_value(_p(a), len(a), _p(b), len(b), cell_num)
+
+
+class FdTopk:
+    """The reference's SimilarityQueryBatchCPUParallel (Grid.cpp:1133-1220) over
+    a database, with the Grid geometry of (range_box, cell_size)."""
+
+    def __init__(self, range_box, cell_size, db):
+        self._db = db
+        self.handle = lib().orc_fv_new(_p(_f32(range_box)), cell_size, _p(db.offsets), db.ntraj, _p(db.lon),
+                                       _p(db.lat))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().orc_fv_free(self.handle)
+            self.handle = None
+
+    def fd(self, q) -> np.ndarray:
+        q = _xy(q)
+        out = np.zeros(self._db.ntraj, np.int32)
+        lib().orc_fv_fd(self.handle, _p(q), len(q), _p(out))
+        return out
+
+    def topk(self, queries, k: int, eps: float):
+        """(ids (nq, k) worst first as the reference pops them, distances,
+        stats (nq, 2): rounds, candidates evaluated)."""
+        xy = queries.interleaved()
+        nq = queries.ntraj
+        ids = np.zeros((max(nq, 1), k), np.int32)
+        dist = np.zeros((max(nq, 1), k), np.int32)
+        st = np.zeros((max(nq, 1), 2), np.int64)
+        lib().orc_fv_topk(self.handle, _p(queries.offsets), nq, _p(xy), k, eps, _p(ids), _p(dist), _p(st))
+        return ids[:nq], dist[:nq], st[:nq]

@@ -21,11 +21,20 @@
 //  * FSG clamps the rectangle's cell range to the grid (FSG.cpp:183-190 does not).
 //  * eps is a run-time double instead of the EPSILON macro (ConstDefine.h:28).
 //  * top-k is exact brute force ordered by (distance, trajectory id).
+//  * The reference's own pruned CPU top-k (FVTable + SimilarityMultiThreadHandler
+//    + SimilarityQueryBatchCPUParallel) is restated too, as the timed baseline
+//    the reference runs (SystemTest.cpp:207); it is not exact and is never used
+//    as the parity answer.  Two points of it are undefined behaviour in C++ and
+//    are restated as the reference's x86-64 build evaluates them or as its
+//    evident intent (see fv_neighbors and fd_value).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <queue>
 #include <thread>
 #include <vector>
 
@@ -683,6 +692,237 @@ int edr_textbook(const float* q, int nq, const float* cand, int ncand, double ep
   return prev[nc];
 }
 
+// ------------------------------------------------------------------------
+// The reference's CPU top-k: frequency vectors over virtual cells, the
+// frequency-distance (FD) queue and the rounds of 80 candidates.
+
+// getIdxFromXY (Grid.cpp:25-54) as the reference's x86-64 g++ -O2 build
+// evaluates it for the negative coordinates findNeighbor passes at the grid's
+// edge (FVTable.cpp:67-73): int(log2(negative)) converts NaN, which x86-64
+// turns into INT_MIN (bit length INT_MIN + 1), and 2 * max(lenx, leny) wraps
+// modulo 2^32.  Equal to morton() for non-negative coordinates.
+int morton_x86(int x, int y) {
+  auto bitlen = [](int v) { return v == 0 ? 1 : v < 0 ? INT_MIN + 1 : int(std::log2(v)) + 1; };
+  const int total = int(2u * uint32_t(std::max(bitlen(x), bitlen(y))));
+  int result = 0, xb = 0, yb = 0;
+  for (int i = 1; i <= total; ++i) {
+    if (i & 1) {
+      result += ((x >> xb) & 1) * (1 << (i - 1));
+      ++xb;
+    } else {
+      result += ((y >> yb) & 1) * (1 << (i - 1));
+      ++yb;
+    }
+  }
+  return result;
+}
+
+// FVTable::findNeighbor — FVTable.cpp:53-75: decode the cell's int(log2(cellNum))
+// bits (even bits x, odd bits y), then the eight cells around it, x-major.
+void fv_neighbors(int cell, int cellNum, int out[8]) {
+  int x = 0, y = 0;
+  const int bits = int(std::log2(cellNum));
+  for (int bit = 0; bit < bits; ++bit) {
+    if (bit % 2 == 0)
+      x += ((cell >> bit) & 1) * (1 << (bit / 2));
+    else
+      y += ((cell >> bit) & 1) * (1 << (bit / 2));
+  }
+  int n = 0;
+  for (int xx = x - 1; xx <= x + 1; ++xx)
+    for (int yy = y - 1; yy <= y + 1; ++yy)
+      if (xx != x || yy != y) out[n++] = morton_x86(xx, yy);
+}
+
+using FreqVector = std::map<int, int>;  // virtual cell -> points
+
+// One trajectory's frequency distance to the query — the body of
+// FVTable::formPriorityQueue, FVTable.cpp:86-198.  The difference t - q is
+// split into positive and negative parts (89-123); each positive cell then
+// cancels against negative neighbours (129-156) and each negative cell against
+// positive neighbours (160-187); FD = max(sum of positives, sum of negatives).
+// As in the reference, a cell's value is read once before its neighbour loop
+// (132, 163): after a partial cancellation the next neighbour is compared
+// with the cell's original value.  Where the reference erases the element its
+// loop iterator points to and then advances that iterator (145-146, 150-152,
+// 176-177, 181-183: undefined behaviour), the loop continues at the element
+// that followed it; nothing but the current element is ever erased from the
+// map being walked, so that is the order the loop would have seen.
+int fd_value(const FreqVector& t, const FreqVector& q, int cellNum) {
+  FreqVector pos, neg;
+  for (const auto& e : t) {
+    auto it = q.find(e.first);
+    if (it == q.end()) {
+      pos.insert(e);
+    } else {
+      const int d = e.second - it->second;
+      if (d > 0)
+        pos.insert({e.first, d});
+      else if (d < 0)
+        neg.insert({e.first, -d});
+    }
+  }
+  for (const auto& e : q) {
+    if (t.count(e.first)) continue;
+    const int d = -e.second;
+    if (d > 0)
+      pos.insert({e.first, d});
+    else if (d < 0)
+      neg.insert({e.first, -d});
+  }
+  int nb[8];
+  for (auto it = pos.begin(); it != pos.end();) {
+    const auto next = std::next(it);
+    const int cid = it->first, posi = it->second;
+    fv_neighbors(cid, cellNum, nb);
+    for (int i = 0; i < 8; ++i) {
+      auto jn = neg.find(nb[i]);
+      if (jn == neg.end()) continue;
+      const int nega = jn->second;
+      if (posi > nega) {
+        it->second = posi - nega;
+        neg.erase(jn);
+      } else if (posi == nega) {
+        pos.erase(it);
+        neg.erase(jn);
+        break;
+      } else {
+        jn->second = nega - posi;
+        pos.erase(it);
+        break;
+      }
+    }
+    it = next;
+  }
+  for (auto it = neg.begin(); it != neg.end();) {
+    const auto next = std::next(it);
+    const int cid = it->first, nega = it->second;
+    fv_neighbors(cid, cellNum, nb);
+    for (int i = 0; i < 8; ++i) {
+      auto jp = pos.find(nb[i]);
+      if (jp == pos.end()) continue;
+      const int posi = jp->second;
+      if (nega > posi) {
+        it->second = nega - posi;
+        pos.erase(jp);
+      } else if (posi == nega) {
+        neg.erase(it);
+        pos.erase(jp);
+        break;
+      } else {
+        jp->second = posi - nega;
+        neg.erase(it);
+        break;
+      }
+    }
+    it = next;
+  }
+  int sp = 0, sn = 0;
+  for (const auto& e : pos) sp += e.second;
+  for (const auto& e : neg) sn += e.second;
+  return std::max(sp, sn);
+}
+
+// FDwithID and the two heap orders of FVTable.h:8-23.
+struct FdWithId {
+  int traID;
+  int FD;
+};
+struct FdMinFirst {  // cmp: the smallest FD on top
+  bool operator()(const FdWithId& a, const FdWithId& b) const { return a.FD > b.FD; }
+};
+struct FdMaxFirst {  // cmpBig: the largest on top
+  bool operator()(const FdWithId& a, const FdWithId& b) const { return a.FD < b.FD; }
+};
+using FdQueue = std::priority_queue<FdWithId, std::vector<FdWithId>, FdMinFirst>;
+using ResultHeap = std::priority_queue<FdWithId, std::vector<FdWithId>, FdMaxFirst>;
+
+// The reference's Grid as its CPU top-k sees it: the cell geometry (for the
+// query's cells), each trajectory's frequency vector over virtual cells
+// (Morton cell >> VITURAL_CELL_PARAM, Grid.cpp:152-182, VITURAL_CELL_PARAM = 2
+// as main.cpp builds the grid) and the trajectories' points (the reference
+// reassembles them from the cell-ordered store, Grid.cpp:1056-1071, which
+// gives each trajectory's points in order).
+struct FvIndex {
+  GridIndex grid;
+  int shift = 2;
+  int cellNum = 0;  // FVTable::cellNum = cellnum >> 2 (Grid.cpp:214)
+  std::vector<FreqVector> fv;  // by 1-based trajectory id
+  std::vector<int64_t> offs;
+  std::vector<float> xy;  // interleaved (lon, lat)
+
+  void build(const Box& r, float cs, const int64_t* o, int ntraj, const float* lon, const float* lat) {
+    grid.init(r, cs, 256);
+    cellNum = grid.cellnum >> 2;
+    fv.assign(size_t(ntraj) + 1, FreqVector{});
+    offs.assign(o, o + ntraj + 1);
+    xy.resize(size_t(o[ntraj] - o[0]) * 2);
+    for (int t = 0; t < ntraj; ++t)
+      for (int64_t i = o[t]; i < o[t + 1]; ++i) {
+        // addTrajectoryIntoCell adds each same-cell run's point count to its
+        // virtual cell (Grid.cpp:128-199): every point once.
+        ++fv[size_t(t) + 1][grid.cell_of(lon[i], lat[i]) >> shift];
+        xy[size_t(i - o[0]) * 2] = lon[i];
+        xy[size_t(i - o[0]) * 2 + 1] = lat[i];
+      }
+  }
+  // The query's frequency vector — Grid.cpp:1141-1157.
+  FreqVector query_fv(const float* q, int n) const {
+    FreqVector f;
+    for (int i = 0; i < n; ++i) ++f[grid.cell_of(q[2 * i], q[2 * i + 1]) >> shift];
+    return f;
+  }
+  // FVTable::formPriorityQueue — FVTable.cpp:77-207: trajectories 1..trajNum
+  // pushed in id order.
+  void form_queue(const FreqVector& qf, FdQueue* queue) const {
+    for (size_t i = 1; i < fv.size(); ++i) queue->push(FdWithId{int(i), fd_value(fv[i], qf, cellNum)});
+  }
+  const float* traj(int id, int* len) const {
+    *len = int(offs[size_t(id)] - offs[size_t(id) - 1]);
+    return xy.data() + size_t(offs[size_t(id) - 1] - offs[0]) * 2;
+  }
+};
+
+struct FdTopkStats {
+  int64_t rounds = 0, evaluated = 0;
+};
+
+// Grid::SimilarityMultiThreadHandler for one query — Grid.cpp:1021-1125:
+// rounds of KSIMILARITY = 80 candidates taken from the FD queue (1046-1051),
+// their EDR by SimilarityExecuter (1078), a max-heap of the kValue best
+// (1082-1106), until the worst kept distance no longer exceeds the next FD
+// (1040).  Where the reference would take top() of an exhausted queue
+// (undefined), a round takes what is left and the loop ends with the queue.
+void fd_topk_query(const FvIndex& ix, FdQueue& queue, const float* q, int qlen, int kValue, double eps,
+                   ResultHeap& calc, FdTopkStats& st) {
+  constexpr int kBatch = 80;  // KSIMILARITY, ConstDefine.h:38
+  int worstNow = 9999999;
+  int held = 0;
+  while (!queue.empty() && worstNow > queue.top().FD) {
+    int cand[kBatch];
+    int m = 0;
+    while (m < kBatch && !queue.empty()) {
+      cand[m++] = queue.top().traID;
+      queue.pop();
+    }
+    for (int i = 0; i < m; ++i) {
+      int len = 0;
+      const float* c = ix.traj(cand[i], &len);
+      const int d = edr_rowmajor(q, qlen, c, len, eps);
+      if (held < kValue) {
+        calc.push(FdWithId{cand[i], d});
+        ++held;
+      } else if (d < calc.top().FD) {
+        calc.pop();
+        calc.push(FdWithId{cand[i], d});
+      }
+    }
+    worstNow = calc.top().FD;
+    ++st.rounds;
+    st.evaluated += m;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -918,6 +1158,85 @@ int orc_topk(const int32_t* dist, const int32_t* ids, int n, int k, int32_t* out
     out_dist[i] = dist[idx[i]];
   }
   return kk;
+}
+
+// ---- the reference's CPU top-k (FVTable + 80-candidate rounds), a timed baseline
+
+int orc_morton_x86(int x, int y) { return morton_x86(x, y); }
+
+int orc_fv_neighbors(int cell, int cellNum, int32_t* out) {
+  int nb[8];
+  fv_neighbors(cell, cellNum, nb);
+  for (int i = 0; i < 8; ++i) out[i] = nb[i];
+  return 0;
+}
+
+// FD of two frequency vectors given as (cell, count) pairs.
+int orc_fd_value(const int32_t* t, int nt, const int32_t* q, int nq, int cellNum) {
+  FreqVector a, b;
+  for (int i = 0; i < nt; ++i) a[t[2 * i]] += t[2 * i + 1];
+  for (int i = 0; i < nq; ++i) b[q[2 * i]] += q[2 * i + 1];
+  return fd_value(a, b, cellNum);
+}
+
+void* orc_fv_new(const float* range, float cell_size, const int64_t* offs, int ntraj, const float* lon,
+                 const float* lat) {
+  auto* f = new FvIndex();
+  f->build(box_from(range), cell_size, offs, ntraj, lon, lat);
+  return f;
+}
+
+void orc_fv_free(void* f) { delete static_cast<FvIndex*>(f); }
+
+// FD of query (qlen interleaved points) to every trajectory: out[t - 1].
+int orc_fv_fd(void* fp, const float* q, int qlen, int32_t* out) {
+  const FvIndex& f = *static_cast<FvIndex*>(fp);
+  const FreqVector qf = f.query_fv(q, qlen);
+  for (size_t i = 1; i < f.fv.size(); ++i) out[i - 1] = fd_value(f.fv[i], qf, f.cellNum);
+  return 0;
+}
+
+// Grid::SimilarityQueryBatchCPUParallel — Grid.cpp:1133-1220: the query
+// frequency vectors (1141-1157), one FD thread per query (1162-1168), then
+// batches of THREAD_CPU = 20 query threads, each running the 80-candidate
+// rounds for its query (1183-1197); out_ids[q * kValue + i] in the order the
+// reference pops its max-heap, worst first (1202-1209), 0 past the heap's
+// size.  stats[2q], stats[2q + 1]: rounds and candidates evaluated.
+int orc_fv_topk(void* fp, const int64_t* qoffs, int nq, const float* qxy, int kValue, double eps, int32_t* out_ids,
+                int32_t* out_dist, int64_t* stats) {
+  const FvIndex& f = *static_cast<FvIndex*>(fp);
+  std::vector<FdQueue> queues(static_cast<size_t>(nq));
+  std::vector<FreqVector> qfv(static_cast<size_t>(nq));
+  for (int q = 0; q < nq; ++q) qfv[q] = f.query_fv(qxy + 2 * qoffs[q], int(qoffs[q + 1] - qoffs[q]));
+  {
+    std::vector<std::thread> th;
+    for (int q = 0; q < nq; ++q) th.emplace_back([&, q] { f.form_queue(qfv[q], &queues[q]); });
+    for (auto& t : th) t.join();
+  }
+  std::vector<ResultHeap> calc(static_cast<size_t>(nq));
+  std::vector<FdTopkStats> st(static_cast<size_t>(nq));
+  constexpr int kThreads = 20;  // THREAD_CPU, Grid.cpp:1183
+  for (int q0 = 0; q0 < nq; q0 += kThreads) {
+    std::vector<std::thread> th;
+    for (int q = q0; q < std::min(nq, q0 + kThreads); ++q)
+      th.emplace_back([&, q] {
+        fd_topk_query(f, queues[q], qxy + 2 * qoffs[q], int(qoffs[q + 1] - qoffs[q]), kValue, eps, calc[q], st[q]);
+      });
+    for (auto& t : th) t.join();
+  }
+  for (int q = 0; q < nq; ++q) {
+    for (int i = 0; i < kValue; ++i) {
+      const bool has = !calc[q].empty();
+      out_ids[int64_t(q) * kValue + i] = has ? calc[q].top().traID : 0;
+      if (out_dist) out_dist[int64_t(q) * kValue + i] = has ? calc[q].top().FD : -1;
+      if (has) calc[q].pop();
+    }
+    if (stats) {
+      stats[2 * q] = st[q].rounds;
+      stats[2 * q + 1] = st[q].evaluated;
+    }
+  }
+  return 0;
 }
 
 }  // extern "C"

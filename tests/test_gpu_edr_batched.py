@@ -221,3 +221,19 @@ def test_waves_per_long_pair(gpu, split):
     g.set_edr_waves(4)
     dist, _ = g.edr_batched_pairs(q, sub, EPS)
     assert (dist == want[sub[:, 0], sub[:, 1]]).all()
+
+
+@pytest.mark.parametrize("k", [10, 1100])
+def test_topk_rejects_overlong_database_trajectory(gpu, k):
+    """A database trajectory longer than the EDR kernels take (2049 points)
+    is rejected before any work, by both top-k paths and for both merges
+    (device merge for k <= 1024, host merge above): never a distance of -1
+    ranked first, never a batch failing with an unclear status."""
+    lengths = np.array([64, 2049, 100, 300], np.int32)
+    db = api.synth_walks(7, lengths)
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    q = api.synth_queries(8, np.array([64, 128], np.int32), db, copy_mode=2)
+    with pytest.raises(RuntimeError, match="2049 points"):
+        g.similarity_batched(q, k, EPS)
+    with pytest.raises(RuntimeError, match="2049 points"):
+        g.similarity(q, k, EPS)

@@ -54,3 +54,31 @@ def test_mixed_routing_thresholds(setup):
         assert (route == want).all()
         c, d = _expected(route, (gref[0][:500], gref[1][:500]), (bref[0][:500], bref[1][:500]))
         assert (counts == c).all() and (digests == d).all(), want
+
+
+def test_mixed_recaptures_after_engine_growth(setup):
+    """A mixed run captures its groups' pipelines over the three engines'
+    workspaces.  When an index's own batched call grows its engine's
+    workspace between two mixed runs, the buffers the graph holds are freed:
+    the next group of the same shape must be run step by step and recaptured,
+    not replayed, and the answers must still equal the oracle's."""
+    db, rects, g, s, f, gref, bref = setup
+    m = api.MixedRange(g, s, f)
+    sub = rects[:1200]
+    # Everything to the GT-quadtree, twelve groups of one shape.
+    c1, d1, route, st1 = m.run(sub, 0.0, 1.0, group_size=100)
+    assert (route == 1).all()
+    assert st1["captures"] == 1 and st1["replays"] == st1["groups"] - 1
+    c, d = gref[0][:1200], gref[1][:1200]
+    assert (c1 == c).all() and (d1 == d).all()
+    # Grow the quadtree engine's workspace (3,000 rectangles > 100).
+    big = g.range_batched(rects)
+    assert (big.counts() == gref[0]).all()
+    c2, d2, _, st2 = m.run(sub, 0.0, 1.0, group_size=100)
+    assert st2["captures"] == 2, "the first group after the growth must be recaptured"
+    assert st2["replays"] == st2["groups"] - 1
+    assert (c2 == c).all() and (d2 == d).all()
+    # Nothing moved since: the third run replays every group.
+    c3, d3, _, st3 = m.run(sub, 0.0, 1.0, group_size=100)
+    assert st3["captures"] == 2 and st3["replays"] == st3["groups"]
+    assert (c3 == c).all() and (d3 == d).all()

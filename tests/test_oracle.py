@@ -354,3 +354,136 @@ def test_mgrid_matches_grid_restatement(cell, leaf):
     mc, md = m.range(rects, "digest")
     assert (gc == mc).all() and (gd == md).all()
     assert _sets(m.range(rects[:30])) == _sets(g.range(rects[:30]))
+
+
+# ---- the reference's own CPU top-k (FVTable.cpp:18-207, Grid.cpp:1021-1220):
+# a timed baseline, pinned here by hand-derived answers and by the cases the
+# loop's rules fix.
+
+def test_morton_x86_known_answers():
+    """getIdxFromXY for the negative coordinates findNeighbor passes at the
+    grid's edge (FVTable.cpp:67-73), as the reference's x86-64 build computes
+    it: a negative coordinate's bit length is INT_MIN + 1 (int(log2(-1)) is
+    the conversion of NaN), so only the other coordinate's length counts, and
+    2 * (INT_MIN + 1) wraps to 2 when both are negative.  By hand:
+    (-1, -1): 2 bits, x0 = 1, y0 = 1 -> 3; (-1, 5): 6 bits, x bits all 1 at
+    positions 0, 2, 4 (21) plus y = 101b at 1, 3, 5 (2 + 32) -> 55;
+    (0, -1): 2 bits, y0 = 1 -> 2; (2, -1): 4 bits, x = 10b at 0, 2 (4) plus
+    y at 1, 3 (2 + 8) -> 14."""
+    assert O.morton_x86(-1, -1) == 3
+    assert O.morton_x86(-1, 5) == 55
+    assert O.morton_x86(0, -1) == 2
+    assert O.morton_x86(2, -1) == 14
+    assert O.morton_x86(3, -1) == 15
+    for x, y in itertools.product(range(0, 70, 3), range(0, 70, 5)):
+        assert O.morton_x86(x, y) == O.morton(x, y)
+
+
+def test_fv_neighbors_known_answers():
+    """FVTable::findNeighbor (FVTable.cpp:53-75): decode the cell (even bits x,
+    odd bits y), then the eight cells around it with x outermost.  Inside a
+    64 x 64 virtual grid the cell (5, 5) gives the eight real neighbours; on a
+    4 x 4 grid the corner cell 0 gives its three real neighbours 1, 2, 3 and
+    five edge ids that happen to name them again, and cell 4 = (2, 0) names
+    cells 3, 14 and 15 for its three off-grid neighbours (2, -1) -> 14 lies
+    on the opposite edge."""
+    m = O.morton
+    want = [m(4, 4), m(4, 5), m(4, 6), m(5, 4), m(5, 6), m(6, 4), m(6, 5), m(6, 6)]
+    assert O.fv_neighbors(m(5, 5), 4096).tolist() == want
+    assert O.fv_neighbors(0, 16).tolist() == [3, 1, 3, 2, 2, 3, 1, 3]
+    assert O.fv_neighbors(4, 16).tolist() == [3, 1, 3, 14, m(2, 1), 15, m(3, 0), m(3, 1)]
+
+
+def test_fd_value_known_answers():
+    """The frequency distance of FVTable::formPriorityQueue (FVTable.cpp:86-198)
+    on hand-built vectors over the 64 x 64 virtual grid, a = cell (5, 5):"""
+    m = O.morton
+    a = m(5, 5)
+    C = 4096
+    # No overlap: all of t is positive, all of q negative.
+    assert O.fd_value({a: 3}, {}, C) == 3
+    assert O.fd_value({}, {a: 2}, C) == 2
+    assert O.fd_value({a: 3, m(40, 40): 4}, {m(20, 20): 5}, C) == 7
+    # Same cell: the difference.
+    assert O.fd_value({a: 3}, {a: 1}, C) == 2
+    # A neighbour cancels part of a positive cell.
+    assert O.fd_value({a: 3}, {m(6, 5): 1}, C) == 2
+    # Equal values cancel both; a larger negative neighbour keeps the rest.
+    assert O.fd_value({a: 2}, {m(5, 6): 2}, C) == 0
+    assert O.fd_value({a: 1}, {m(4, 5): 3}, C) == 2
+    # The positive cell's value is read once before its neighbour loop
+    # (FVTable.cpp:132): two neighbours of 2 each both cancel against the
+    # original 5, leaving 5 - 2 = 3, not 1.
+    assert O.fd_value({a: 5}, {m(4, 4): 2, m(6, 6): 2}, C) == 3
+    # The negative pass (FVTable.cpp:160-187) acts where the positive pass did
+    # not see the pair: on the 4 x 4 grid cell 14 does not list cell 4, but
+    # cell 4 lists 14 as its off-grid neighbour (2, -1), so the negative cell 4
+    # (1 point) cancels against the positive cell 14 (2 points): max(1, 0).
+    assert O.fd_value({14: 2}, {4: 1}, 16) == 1
+    # The other way round the positive pass sees it: 4 cancels into 14.
+    assert O.fd_value({4: 1}, {14: 2}, 16) == 1
+
+
+def _fd_db(rng, centres, length, spread=2e-5):
+    lengths = np.full(len(centres), length, np.int32)
+    db = api.TrajSet.from_lengths(lengths)
+    for t, (x, y) in enumerate(centres):
+        b, e = db.offsets[t], db.offsets[t + 1]
+        db.lon[b:e] = np.float32(x) + rng.normal(0, spread, e - b).astype(np.float32)
+        db.lat[b:e] = np.float32(y) + rng.normal(0, spread, e - b).astype(np.float32)
+    return db
+
+
+def _query(db, t):
+    b, e = db.offsets[t], db.offsets[t + 1]
+    return api.TrajSet(np.array([0, e - b], np.int64), db.lon[b:e].copy(), db.lat[b:e].copy())
+
+
+def test_fd_topk_one_round_evaluates_everything():
+    """A database of at most 80 trajectories is one round of KSIMILARITY = 80
+    candidates (Grid.cpp:1046-1051): every distance is computed, so the heap
+    of the k best (Grid.cpp:1082-1106) holds the exact top-k distances, popped
+    worst first (Grid.cpp:1202-1209), and each id carries its own distance."""
+    rng = np.random.default_rng(11)
+    centres = rng.uniform(0.2, 0.8, size=(60, 2))
+    centres[:20] = centres[0] + rng.normal(0, 3e-4, size=(20, 2))
+    db = _fd_db(rng, centres, 24)
+    q = _query(db, 0)
+    fdk = O.FdTopk(W.UNIT_BOX, 0.01, db)
+    ids, dist, st = fdk.topk(q, 10, 5e-4)
+    assert st[0].tolist() == [1, 60]
+    exact = O.edr_batch(q.xy(1), db, 5e-4)
+    assert (dist[0] == exact[ids[0] - 1]).all()
+    assert dist[0].tolist() == sorted(dist[0].tolist(), reverse=True)
+    assert sorted(dist[0].tolist()) == sorted(exact.tolist())[:10]
+
+
+def test_fd_topk_stop_rule():
+    """The loop stops when the worst kept distance no longer exceeds the next
+    frequency distance (Grid.cpp:1040).  80 near copies of the query (FD 0)
+    come out first and fill the heap with small distances; the 80 far
+    trajectories have FD = max(24, 24) = 24 > every kept distance: one round.
+    With far trajectories twice the query's length instead, every FD is
+    max(48, 24) = 48 and every distance F(24, 48) > 48 (the all-mismatch
+    value), so the rule never stops the loop and the queue runs dry:
+    ceil(200 / 80) = 3 rounds over all 200."""
+    rng = np.random.default_rng(12)
+    near = np.tile([[0.505, 0.505]], (80, 1))  # mid-cell: every copy in the query's one cell
+    far = rng.uniform(0.05, 0.3, size=(80, 2))
+    db = _fd_db(rng, np.concatenate([near, far]), 24, spread=1e-5)
+    q = _query(db, 0)
+    fdk = O.FdTopk(W.UNIT_BOX, 0.01, db)
+    fd = fdk.fd(q.xy(1))
+    assert fd[:80].max() == 0 and fd[80:].min() == 24
+    ids, dist, st = fdk.topk(q, 10, 5e-4)
+    assert st[0].tolist() == [1, 80]
+    assert dist[0].max() < 24 and (ids[0] <= 80).all()
+    F = O.ftable(64)
+    assert F[24, 48] > 48
+    far_db = _fd_db(rng, rng.uniform(0.05, 0.3, size=(200, 2)), 48, spread=1e-5)
+    q2 = _fd_db(rng, np.array([[0.705, 0.705]]), 24, spread=1e-5)
+    fdk2 = O.FdTopk(W.UNIT_BOX, 0.01, far_db)
+    assert (fdk2.fd(q2.xy(1)) == 48).all()
+    ids2, dist2, st2 = fdk2.topk(q2, 10, 5e-4)
+    assert st2[0].tolist() == [3, 200]
+    assert (dist2[0] == F[24, 48]).all()
