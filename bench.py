@@ -12,10 +12,13 @@ resident in HBM before timing starts; the result pairs stay there.
 Beside it on the same line: the plain drop-in on the same batch (host leaf
 selection as Grid::rangeQueryBatchGPU, task upload, the per-task refine
 kernel) and the ratio; STIG and FSG; the plain and batched EDR cell-update
-rates over one pair set and the batched top-k rate; the leaf-major refine
+rates over one pair set, the EDR kernel's VALU roofline and the batched top-k
+rate; the serving loop on a 2B-point index; the mixed batch of 100k rectangles
+over 10M trajectories routed across the three indexes; the leaf-major refine
 kernel's roofline (algorithmic bytes 12 per point of a selected leaf + 8 per
 result pair, over its HIP-event time on its own stream); and the oracle's CPU
-restatement of Grid::rangeQueryBatch timed on this host.
+restatements timed on this host: Grid::rangeQueryBatch, the EDR brute force
+and the reference's frequency-distance top-k loop.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -34,6 +37,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# Integer VALU issue of one MI355X: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per
+# wave64 instruction x 64 lanes = 3.93e13 lane-ops/s.
+VALU_PEAK_TOPS = 39.3
+# VALU instructions per DP cell of edr_wave_kernel<N, 16> (AMD ISA of the built
+# kernel: 2 subtractions + 2 compares for the match, the row shift, M + s,
+# min3, compare, select, the +1 add).
+EDR_VALU_PER_CELL = 10
 METRIC = "range queries/s + EDR cell-updates/s, batched vs plain; p99 latency"
 
 
@@ -86,6 +96,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 10M-trajectory mixed batch")
     ap.add_argument("--serve-ntraj", type=int, default=None,
                     help="trajectories of the serving index (default: the full 22.7M, about 2B points)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of the served request stream")
@@ -219,6 +230,7 @@ def main() -> int:
     topk_cpu = edr.pop("topk_cpu")
     del grid
     serve = bench_serve(G, W, args.serve_ntraj, args.serve_seconds)
+    mixed = {} if args.no_mixed else bench_mixed(G, W, args.mixed_ntraj, args.mixed_nq)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -319,6 +331,7 @@ def main() -> int:
             "device_build_equals_host": same_store,
             **edr,
             **serve,
+            **mixed,
         },
         "device": dev,
     }
@@ -332,13 +345,14 @@ def bench_mixed(G, W, ntraj, nq) -> dict:
     area to FSG, the GT-quadtree or STIG, run in groups whose three pipelines
     are one captured graph; per-rectangle counts and digests, 1000 of them
     checked against the oracle's restatement of the index they were routed to.
-    (Run with --only mixed: the full batch writes ~0.5 TB of pairs.)"""
+    The full batch writes ~0.5 TB of pairs, group by group into one reused
+    buffer."""
     w = W.CFG4
     t = time.perf_counter()
     db = w.db(ntraj)
     rects = w.rects(nq)
     g = G.GridIndex(W.UNIT_BOX, w.grid_cell, w.leaf).build(db, device=True)
-    s = G.StigIndex(w.leaf).build(db)
+    s = G.StigIndex(w.leaf).build(db, device=True)
     f = G.FsgIndex(W.UNIT_BOX, w.fsg_cell).build(db, device=True)
     log(f"mixed workload {w.name}: {db.ntraj:,} trajectories, {db.npoints:,} points, {len(rects):,} rectangles, "
         f"three indexes resident ({time.perf_counter() - t:.1f} s)")
@@ -441,6 +455,68 @@ def bench_hbm(G) -> dict:
     return out
 
 
+def _subset(G, ts, idx):
+    """Trajectories idx (0-based) of a TrajSet, as a new TrajSet."""
+    lens = np.diff(ts.offsets)[idx]
+    offs = np.zeros(len(idx) + 1, np.int64)
+    offs[1:] = np.cumsum(lens)
+    lon = np.concatenate([ts.lon[ts.offsets[i]:ts.offsets[i + 1]] for i in idx])
+    lat = np.concatenate([ts.lat[ts.offsets[i]:ts.offsets[i + 1]] for i in idx])
+    return G.TrajSet(offs, lon, lat)
+
+
+def _loop_vs_exact(fids, fdist, ids, dists):
+    """The frequency-distance loop's answers against the exact top-k: ids of the
+    exact top-k it missed, and ranks whose distance differs (both sorted)."""
+    missed = sum(len(set(ids[i].tolist()) - set(fids[i].tolist())) for i in range(len(ids)))
+    worse = int((np.sort(fdist, axis=1) != dists).sum())
+    return missed, worse
+
+
+def cpu_reference_topk(G, W, O, queries, qlen, ids, dists, db) -> dict:
+    """The reference's own CPU top-k, Grid::SimilarityQueryBatchCPUParallel
+    (Grid.cpp:1133-1220): a frequency-distance queue per query (FVTable.cpp:77-207),
+    rounds of 80 candidates through the DP until the worst kept distance no
+    longer exceeds the next frequency distance, 20 query threads per batch;
+    restated in the oracle.  Timed on 20 queries of the 1024-query batch (k =
+    50, 200k candidates) and on the 20k-trajectory workload's one 128-point
+    query (k = 10), and compared with the exact top-k: the loop is not exact
+    (its bound is unproven and its stop rule drops ties)."""
+    w3 = W.CFG3
+    sample = np.arange(0, queries.ntraj, queries.ntraj // 20)[:20]
+    sub = _subset(G, queries, sample)
+    fdk = O.FdTopk(W.UNIT_BOX, w3.grid_cell, db)
+    t = time.perf_counter()
+    fids, fdist, fst = fdk.topk(sub, w3.k, w3.eps)
+    s3 = time.perf_counter() - t
+    missed, worse = _loop_vs_exact(fids, fdist, ids[sample], dists[sample])
+    del fdk
+    w1 = W.CFG1
+    db1 = w1.db()
+    q1 = G.synth_queries(w1.seed, np.array([128, 128], np.int32), db1, copy_mode=2)
+    q1 = _subset(G, q1, np.array([1]))  # the noisy copy of a database trajectory
+    fdk1 = O.FdTopk(W.UNIT_BOX, w1.grid_cell, db1)
+    t = time.perf_counter()
+    f1ids, f1dist, f1st = fdk1.topk(q1, w1.k, w1.eps)
+    s1 = time.perf_counter() - t
+    e1 = O.edr_batch(q1.xy(1), db1, w1.eps, nthreads=16)
+    oi, od = O.topk(e1, np.arange(1, db1.ntraj + 1, dtype=np.int32), w1.k)
+    missed1, worse1 = _loop_vs_exact(f1ids, f1dist, oi[None, :], od[None, :])
+    out = {"value": round(len(sample) / s3, 4), "unit": "queries/s", "cores": 20, "kind": "port",
+           "sample": f"queries {sample.tolist()} of the 1024-query batch (lengths {qlen[sample].min()}-"
+                     f"{qlen[sample].max()}), top-{w3.k} over {db.ntraj:,} candidates, one batch of 20 threads",
+           "rounds_mean": round(float(fst[:, 0].mean()), 2), "candidates_evaluated": int(fst[:, 1].sum()),
+           "exact_ids_missed": int(missed), "ranks_with_other_distance": int(worse),
+           "of_ids": int(len(sample) * w3.k),
+           "workload_20k": {"ms": round(1e3 * s1, 3), "k": w1.k, "query_len": 128,
+                            "rounds": int(f1st[0, 0]), "candidates_evaluated": int(f1st[0, 1]),
+                            "exact_ids_missed": int(missed1), "ranks_with_other_distance": int(worse1)}}
+    log(f"CPU top-k, the reference's frequency-distance loop: {out['value']} queries/s on 20 threads "
+        f"({missed} of {out['of_ids']} exact ids missed, {worse} ranks at another distance); 20k-trajectory "
+        f"query {1e3 * s1:.1f} ms ({missed1} of {w1.k} exact ids missed, {worse1} ranks at another distance)")
+    return out
+
+
 def bench_edr(G, W, cpu_topk: bool = False) -> dict:
     """EDR cell-update rates over one pair set (4 queries against the 200k
     candidates of the top-k workload, both query kinds): plain = one launch per
@@ -466,11 +542,11 @@ def bench_edr(G, W, cpu_topk: bool = False) -> dict:
     plain_s = time.perf_counter() - t
     g.edr_batched_all(q, w.eps)
     t = time.perf_counter()
-    batched, st = g.edr_batched_all(q, w.eps)
+    batched, st_all = g.edr_batched_all(q, w.eps)
     batched_s = time.perf_counter() - t
     same = bool((batched.reshape(plain.shape) == plain).all())
     plain_rate, batched_rate = cells / plain_s, cells / batched_s
-    kernel_rate = cells / (st["kernel_ms"] * 1e-3)
+    kernel_rate = cells / (st_all["kernel_ms"] * 1e-3)
     log(f"EDR over {q.ntraj * db.ntraj:,} pairs ({cells:.3e} cells): plain {plain_rate:.3e} cells/s, batched "
         f"{batched_rate:.3e} cells/s wall, {kernel_rate:.3e} in-kernel; equal: {same}")
     del plain, batched
@@ -487,6 +563,7 @@ def bench_edr(G, W, cpu_topk: bool = False) -> dict:
     cpu = None
     if cpu_topk:
         import oracle as O
+        cpu = {"reference_loop": cpu_reference_topk(G, W, O, queries, qlen, ids, dists, db)}
         order = np.argsort(qlen, kind="stable")
         sample = [int(order[len(order) // 4]), int(order[len(order) // 2])]  # lengths near the 25th and 50th percentile
         all_ids = np.arange(1, db.ntraj + 1, dtype=np.int32)
@@ -497,15 +574,26 @@ def bench_edr(G, W, cpu_topk: bool = False) -> dict:
             oi, od = O.topk(d, all_ids, w.k)
             ok = ok and bool((oi == ids[qi][:len(oi)]).all() and (od == dists[qi][:len(od)]).all())
         cpu_s = time.perf_counter() - t
-        cells = int(db.lengths.astype(np.int64).sum()) * int(qlen[sample].sum())
-        cpu = {"value": round(len(sample) / cpu_s, 4), "unit": "queries/s", "cores": 16,
-               "sample": f"queries {sample} (lengths {qlen[sample].tolist()}) of the 1024-query batch, exact "
-                         f"top-{w.k} by brute force over all {db.ntraj:,} candidates ({cells:.3e} cells; the "
-                         f"reference's SimilarityQueryBatchCPUParallel prunes with its frequency queue first)",
-               "equals_gpu": ok}
-        log(f"CPU top-{w.k}: {cpu['value']} queries/s on 16 threads ({cells / cpu_s:.3e} cells/s), GPU answers "
+        cpu_cells = int(db.lengths.astype(np.int64).sum()) * int(qlen[sample].sum())
+        cpu.update({"value": round(len(sample) / cpu_s, 4), "unit": "queries/s", "cores": 16,
+                    "sample": f"queries {sample} (lengths {qlen[sample].tolist()}) of the 1024-query batch, exact "
+                              f"top-{w.k} by brute force over all {db.ntraj:,} candidates ({cpu_cells:.3e} cells)",
+                    "equals_gpu": ok})
+        log(f"CPU top-{w.k}: {cpu['value']} queries/s on 16 threads ({cpu_cells / cpu_s:.3e} cells/s), GPU answers "
             f"identical: {ok}")
-    return {"edr_plain_cells_per_s": float(f"{plain_rate:.4e}"),
+    # The EDR roofline: integer VALU issue.  Every DP cell of the batched
+    # kernel is EDR_VALU_PER_CELL VALU instructions (counted in the AMD ISA of
+    # the built kernel), so the algorithmic lane-ops are cells x that count.
+    lane_ops = kernel_rate * EDR_VALU_PER_CELL
+    roof = {"bound": "valu", "achieved": float(f"{lane_ops / 1e12:.4g}"), "peak": VALU_PEAK_TOPS,
+            "unit": "Tlane-ops/s", "frac": round(lane_ops / 1e12 / VALU_PEAK_TOPS, 4),
+            "cells_per_s": float(f"{kernel_rate:.4e}"), "valu_per_cell": EDR_VALU_PER_CELL,
+            "kernel": "edr_wave_kernel (all length buckets of one batched launch set)",
+            "kernel_ms": round(st_all["kernel_ms"], 4), "cells": cells}
+    log(f"EDR roofline: {kernel_rate:.3e} cells/s x {EDR_VALU_PER_CELL} VALU = {lane_ops:.3e} lane-ops/s, "
+        f"{roof['frac']:.1%} of {VALU_PEAK_TOPS}e12")
+    return {"edr_roofline": roof,
+            "edr_plain_cells_per_s": float(f"{plain_rate:.4e}"),
             "edr_batched_cells_per_s": float(f"{batched_rate:.4e}"),
             "edr_batched_kernel_cells_per_s": float(f"{kernel_rate:.4e}"),
             "edr_batched_over_plain": round(batched_rate / plain_rate, 3),

@@ -115,6 +115,7 @@ HOST_FUNCS = {
     "gath_morton_grid_new": (P, [P, F32, INT]),
     "gath_grid_set_edr_waves": (INT, [P, INT]),
     "gath_system_range_test": (INT, [P, INT, P, INT, P]),
+    "gath_system_similarity_test": (INT, [P, P, P, INT, INT, INT, F64, P, P, P, P, P]),
     "gath_fsg_new": (P, [P, F32]),
     "gath_stig_new": (P, [INT]),
     "gath_index_free": (None, [P]),

@@ -502,6 +502,42 @@ class GridIndex(_Index):
                                                    _ptr(dists)))
         return ids, dists
 
+    def system_similarity_test(self, query_xy: np.ndarray, scale: int, k: int, eps: float, cpu=None) -> dict:
+        """The reference's SystemTest::similarityQueryTest (SystemTest.cpp:164-249):
+        the query replicated `scale` times, the CPU leg timed (`cpu(queries:
+        TrajSet, k) -> ids (nq, k)`, skipped when None: this library has no
+        CPU query path), then the GPU leg, Grid::SimilarityQueryBatchOnGPU.
+        Returns {"gpu_ids", "cpu_ids", "cpu_ms", "gpu_ms"}."""
+        q = _req(query_xy, np.float32).reshape(-1, 2)
+        qlon, qlat = np.ascontiguousarray(q[:, 0]), np.ascontiguousarray(q[:, 1])
+        gpu_ids = np.zeros((scale, k), np.int32)
+        cpu_ids = np.zeros((scale, k), np.int32)
+        ms = np.zeros(2, np.float64)
+        fn_t = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_float),
+                           C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int32))
+        failure = []
+
+        def leg(_ctx, offs, nq, lon, lat, kk, ids):
+            try:
+                o = np.ctypeslib.as_array(offs, (nq + 1,)).copy()
+                m = int(o[-1])
+                qs = TrajSet(o, np.ctypeslib.as_array(lon, (m,)).copy(), np.ctypeslib.as_array(lat, (m,)).copy())
+                got = np.asarray(cpu(qs, kk), np.int32).reshape(nq, kk)
+                np.ctypeslib.as_array(ids, (nq * kk,))[:] = got.reshape(-1)
+                return 0
+            except Exception as e:  # noqa: BLE001 - reported after the C call returns
+                failure.append(e)
+                return 1
+
+        cb = fn_t(leg) if cpu is not None else None
+        rc = N.host().gath_system_similarity_test(self.handle, _ptr(qlon), _ptr(qlat), len(q), int(scale), int(k),
+                                                  eps, C.cast(cb, C.c_void_p) if cb else None, None, _ptr(gpu_ids),
+                                                  _ptr(cpu_ids) if cpu is not None else None, _ptr(ms))
+        if failure:
+            raise failure[0]
+        N.check_host(rc)
+        return {"gpu_ids": gpu_ids, "cpu_ids": cpu_ids if cpu is not None else None, "cpu_ms": float(ms[0]),
+                "gpu_ms": float(ms[1])}
 
     def edr_batched_all(self, queries: TrajSet, eps: float) -> tuple[np.ndarray, dict]:
         """Distances of every (query, trajectory) pair, (nq, ntraj) int32, from
@@ -576,11 +612,11 @@ def serve(grid: "GridIndex", requests: np.ndarray, qlon: np.ndarray, qlat: np.nd
     dists = np.zeros((max(n, 1), k), np.int32)
     cfg = np.array([range_slots, max_range_batch, max_topk_batch, range_budget, topk_budget, k, eps, int(batched)],
                    np.float64)
-    st = np.zeros(9, np.float64)
+    st = np.zeros(11, np.float64)
     N.check_host(N.host().gath_serve(grid.handle, r.ctypes.data, n, _ptr(lon), _ptr(lat), len(lon), _ptr(cfg),
                                      out.ctypes.data, _ptr(ids), _ptr(dists), _ptr(st)))
     keys = ["range_batches", "topk_batches", "range_done", "topk_done", "wall_ms", "peak_range_bytes",
-            "peak_topk_bytes", "range_budget", "topk_budget"]
+            "peak_topk_bytes", "range_budget", "topk_budget", "peak_topk_survivors", "device_allocs"]
     return out[:n], ids[:n], dists[:n], dict(zip(keys, st.tolist()))
 
 

@@ -1,5 +1,6 @@
 #include "DeviceArena.h"
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 
@@ -21,10 +22,15 @@ int gath_fail_hip(int rc, const char* what) {
 
 extern "C" __attribute__((visibility("default"))) const char* gath_last_error(void) { return g_host_error.c_str(); }
 
+static std::atomic<int64_t> g_device_allocs{0};
+
+int64_t gath_device_alloc_count() { return g_device_allocs.load(std::memory_order_relaxed); }
+
 int DeviceBuffer::reserve(size_t bytes) {
   if (bytes <= n_) return GAT_OK;
   release();
   void* p = nullptr;
+  g_device_allocs.fetch_add(1, std::memory_order_relaxed);
   GATH_TRY(gat_malloc(&p, bytes), "device allocation");
   p_ = p;
   n_ = bytes;

@@ -16,6 +16,9 @@
 void gath_set_error(const char* fmt, ...);
 // Copies the device library's last error into the host error, with context.
 int gath_fail_hip(int rc, const char* what);
+// Device allocations DeviceBuffer::reserve has made in this process (each
+// growth is one); the serving loop reports how many happened while it served.
+int64_t gath_device_alloc_count();
 
 #define GATH_TRY(expr, what)                       \
   do {                                             \

@@ -285,6 +285,14 @@ void Grid::addLeafRuns(const MBB& bound, uint32_t q, TaskList& list) const {
   for (int li : leaves) list.addRun(uint64_t(nodes[li].pointStart), uint64_t(nodes[li].numPoints), q);
 }
 
+void Grid::selectionCost(const MBB& bound, int64_t* points, int64_t* leaves) const {
+  TaskList list;
+  list.candidates.assign(1, 0);
+  addLeafRuns(bound, 0, list);
+  *points = int64_t(list.pointsAdded);
+  *leaves = int64_t(list.runsAdded);
+}
+
 void Grid::selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
                             std::vector<uint64_t>& candidates) const {
   RangeEngine::select(

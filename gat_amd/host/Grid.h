@@ -82,6 +82,10 @@ class Grid {
   virtual int64_t leafNum() const;
   // Grid.cpp:428-446: the leaves one rectangle selects, in traversal order.
   void findMatchNodeInQuadTree(const MBB& bound, std::vector<int>* leaves) const;
+  // The leaves one rectangle selects (the batched form's device selection
+  // applies the same rule) and their points: what its candidate and result
+  // buffers need, at most `points` result pairs over `leaves` list entries.
+  void selectionCost(const MBB& bound, int64_t* points, int64_t* leaves) const;
   // Host candidate selection for a batch (tasks + per-query candidate counts).
   void selectCandidates(const MBB* bounds, int rangeNum, std::vector<gat_range_task>& tasks,
                         std::vector<uint64_t>& candidates) const;

@@ -15,6 +15,8 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 void TaskList::addRun(uint64_t start, uint64_t count, uint32_t qid) {
   if (count == 0) return;
+  ++runsAdded;
+  pointsAdded += count;
   if (open_ && qid == runQid_ && runStart_ + runCount_ == start) {
     runCount_ += count;
     return;

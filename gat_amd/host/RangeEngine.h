@@ -33,6 +33,8 @@ class TaskList {
   std::vector<gat_range_task> tasks;
   std::vector<uint64_t> candidates;  // per query handled by this list (indexed by qid - firstQuery)
   uint32_t firstQuery = 0;
+  uint64_t runsAdded = 0;    // addRun calls with points (one per selected leaf), before merging
+  uint64_t pointsAdded = 0;  // their points
 
  private:
   bool open_ = false;

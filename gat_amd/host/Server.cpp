@@ -37,19 +37,22 @@ struct Shared {
 }  // namespace
 
 double Server::rangeBytes(const float* b) const {
-  const MBB& r = grid_.range;
-  const double w = std::max(0.0, double(std::min(b[2], r.xmax)) - double(std::max(b[0], r.xmin)));
-  const double h = std::max(0.0, double(std::min(b[3], r.ymax)) - double(std::max(b[1], r.ymin)));
-  const double area = double(r.xmax - r.xmin) * double(r.ymax - r.ymin);
-  const double frac = area > 0 ? w * h / area : 1.0;
-  // Selected leaves reach past the rectangle's edges: a quarter more plus two
-  // leaves' worth of candidates; 8 bytes per result pair plus the (leaf,
-  // query) records (8 + 32 bytes) at one per 16 candidates.
-  const double cand = frac * double(grid_.totalPointNum) * 1.25 + 2.0 * grid_.maxPointsInNode;
-  return cand * 8.0 + cand / 16.0 * 40.0;
+  int64_t points = 0, leaves = 0;
+  grid_.selectionCost(MBB(b[0], b[1], b[2], b[3]), &points, &leaves);
+  return kRangeBytesPerPair * double(rangePairBound(points, leaves));
 }
 
-double Server::topkBytes(int qlen) const { return 16.0 * qlen + 64.0 * 1024.0; }
+int64_t Server::rangePairBound(int64_t points, int64_t leaves) {
+  // Result pairs are at most the candidate points; each selected leaf's work
+  // item rounds its pair region up to whole lines of 16 pairs.
+  return points + 16 * leaves;
+}
+
+double Server::topkBytes(int qlen, double survivors) const {
+  // Query points (8 B each) and box, k result ids and distances, and the
+  // filter survivors: a (query, trajectory) pair and its distance (12 B each).
+  return 8.0 * qlen + 16.0 + 8.0 * cfg_.k + 12.0 * survivors;
+}
 
 int Server::run(const gath_request* reqs, int n, const float* qlon, const float* qlat, int64_t nqpoints,
                 gath_outcome* out, int32_t* topkIds, int32_t* topkDists, ServeStats* stats) {
@@ -83,6 +86,9 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
     }
   }
   ServeStats st;
+  // Each range request's buffer need, from the leaves it selects (the device
+  // selection applies the same rule): computed as it arrives.
+  std::vector<int64_t> rangeNeed(size_t(n), 0);
   // Admission is bounded by the device memory that is actually free: the
   // budgets of every slot together stay within 80% of it.
   size_t freeB = 0, totalB = 0;
@@ -169,7 +175,9 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
     std::vector<MBB> boxes;
   };
   std::vector<std::unique_ptr<RangeSlot>> slots;
-  const int64_t maxCand = int64_t(st.rangeBudgetBytes / (8.0 + 40.0 / 16.0));
+  // A slot's workspace holds maxCand result pairs and maxCand / 16 list
+  // entries; a batch is admitted while its requests' pair bounds fit.
+  const int64_t maxCand = std::max<int64_t>(int64_t(st.rangeBudgetBytes / kRangeBytesPerPair), 1);
   for (int s = 0; s < cfg_.rangeSlots; ++s) {
     auto slot = std::make_unique<RangeSlot>();
     if (int rc = slot->engine.attachStore(grid_.engine())) return rc;
@@ -190,7 +198,7 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
         if (sh.failed || (sh.range.empty() && sh.arrivalsDone)) return;
         // Every waiting rectangle that fits the budget (at least one).
         while (!sh.range.empty() && int(take.size()) < cfg_.maxRangeBatch) {
-          const double e = rangeBytes(reqs[sh.range.front()].box);
+          const double e = kRangeBytesPerPair * double(rangeNeed[size_t(sh.range.front())]);
           if (!take.empty() && est + e > st.rangeBudgetBytes) break;
           est += e;
           take.push_back(sh.range.front());
@@ -225,6 +233,9 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
     }
   };
 
+  // Filter survivors per top-k query, as observed: seeded by the warm-up
+  // batch below, then updated after every batch.
+  double survivorsPerQuery = 0;
   auto topk_worker = [&]() {
     TrajectoryDB qdb;
     std::vector<TopK> res;
@@ -237,7 +248,7 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
         sh.cv.wait(lk, [&] { return sh.failed || !sh.topk.empty() || sh.arrivalsDone; });
         if (sh.failed || (sh.topk.empty() && sh.arrivalsDone)) return;
         while (!sh.topk.empty() && int(take.size()) < cfg_.maxTopkBatch) {
-          const double e = topkBytes(reqs[sh.topk.front()].qlen);
+          const double e = topkBytes(reqs[sh.topk.front()].qlen, survivorsPerQuery);
           if (!take.empty() && est + e > st.topkBudgetBytes) break;
           est += e;
           take.push_back(sh.topk.front());
@@ -248,12 +259,20 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
       }
       const double admit = ms_since(t0);
       gather_queries(take, qdb);
-      const int rc = grid_.similarityBatched(TrajectoryView::of(qdb), cfg_.k, res, nullptr);
+      EdrBatchStats es;
+      const int rc = grid_.similarityBatched(TrajectoryView::of(qdb), cfg_.k, res, &es);
       if (rc != GAT_OK) {
         sh.fail(rc);
         return;
       }
       finish_topk(take, res);
+      {
+        // The survivors this batch really had, folded into the estimate the
+        // next admissions charge.
+        std::lock_guard<std::mutex> lk(sh.m);
+        survivorsPerQuery = 0.75 * survivorsPerQuery + 0.25 * double(es.pairs) / double(take.size());
+        st.peakTopkSurvivors = std::max<int64_t>(st.peakTopkSurvivors, es.pairs);
+      }
       const double done = ms_since(t0);
       for (int i : take) {
         out[i].admit_ms = admit;
@@ -276,9 +295,12 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
     TrajectoryDB qdb;
     std::vector<TopK> res;
     gather_queries(first, qdb);
-    if (int rc = grid_.similarityBatched(TrajectoryView::of(qdb), cfg_.k, res, nullptr)) return rc;
+    EdrBatchStats es;
+    if (int rc = grid_.similarityBatched(TrajectoryView::of(qdb), cfg_.k, res, &es)) return rc;
+    survivorsPerQuery = double(es.pairs) / double(std::max<size_t>(first.size(), 1));
   }
   std::vector<std::thread> workers;
+  const int64_t allocs0 = gath_device_alloc_count();
   t0 = Clock::now();
   for (auto& s : slots) workers.emplace_back(range_worker, s.get());
   if (anyTopk) workers.emplace_back(topk_worker);
@@ -286,6 +308,12 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
   for (int i = 0; i < n; ++i) {
     const double wait = reqs[i].arrival_ms - ms_since(t0);
     if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
+    if (reqs[i].kind == GATH_REQ_RANGE) {
+      int64_t points = 0, leaves = 0;
+      const float* b = reqs[i].box;
+      grid_.selectionCost(MBB(b[0], b[1], b[2], b[3]), &points, &leaves);
+      rangeNeed[size_t(i)] = rangePairBound(points, leaves);
+    }
     std::lock_guard<std::mutex> lk(sh.m);
     if (sh.failed) break;
     (reqs[i].kind == GATH_REQ_RANGE ? sh.range : sh.topk).push_back(i);
@@ -298,6 +326,7 @@ int Server::run(const gath_request* reqs, int n, const float* qlon, const float*
   }
   for (auto& w : workers) w.join();
   st.wallMs = ms_since(t0);
+  st.deviceAllocs = gath_device_alloc_count() - allocs0;
   if (sh.failed) {
     gath_set_error("%s", sh.error.c_str());
     return sh.rc;

@@ -29,7 +29,8 @@ int SystemTest::rangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum) {
 
 int SystemTest::rangeQueryTest(const MBB* bounds, int rangeQueryNum) {
   if (!g) return GAT_ERR_STATE;
-  RangeBatch b;
+  lastBatch = RangeBatch{};
+  RangeBatch& b = lastBatch;
   const double t0 = now_ms();
   if (int rc = g->rangeQueryBatchGPUPairs(bounds, rangeQueryNum, b)) return rc;
   return report("GT-quadtree", rangeQueryNum, b, now_ms() - t0);
@@ -42,7 +43,8 @@ int SystemTest::STIGrangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum) {
 
 int SystemTest::STIGrangeQueryTest(const MBB* bounds, int rangeQueryNum) {
   if (!stig) return GAT_ERR_STATE;
-  RangeBatch b;
+  lastBatch = RangeBatch{};
+  RangeBatch& b = lastBatch;
   const double t0 = now_ms();
   if (int rc = stig->rangeQueryGPUPairs(bounds, rangeQueryNum, b)) return rc;
   return report("STIG", rangeQueryNum, b, now_ms() - t0);
@@ -55,7 +57,8 @@ int SystemTest::FSGrangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum) {
 
 int SystemTest::FSGrangeQueryTest(const MBB* bounds, int rangeQueryNum) {
   if (!fsg) return GAT_ERR_STATE;
-  RangeBatch b;
+  lastBatch = RangeBatch{};
+  RangeBatch& b = lastBatch;
   const double t0 = now_ms();
   if (int rc = fsg->rangeQueryBatchGPUPairs(bounds, rangeQueryNum, b)) return rc;
   return report("FSG", rangeQueryNum, b, now_ms() - t0);
@@ -68,7 +71,8 @@ int SystemTest::MortonGridRangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum) {
 
 int SystemTest::MortonGridRangeQueryTest(const MBB* bounds, int rangeQueryNum) {
   if (!mgrid) return GAT_ERR_STATE;
-  RangeBatch b;
+  lastBatch = RangeBatch{};
+  RangeBatch& b = lastBatch;
   const double t0 = now_ms();
   if (int rc = mgrid->rangeQueryBatchGPUPairs(bounds, rangeQueryNum, b)) return rc;
   return report("MortonGrid", rangeQueryNum, b, now_ms() - t0);
@@ -76,11 +80,23 @@ int SystemTest::MortonGridRangeQueryTest(const MBB* bounds, int rangeQueryNum) {
 
 int SystemTest::similarityQueryTest(Trajectory t, int similarityScale, int similarityKValue) {
   if (!g) return GAT_ERR_STATE;
-  std::vector<Trajectory> q(size_t(std::max(similarityScale, 0)), t);
-  std::vector<int> ids(size_t(std::max(similarityScale, 0)) * size_t(std::max(similarityKValue, 0)));
-  const double t0 = now_ms();
-  if (int rc = g->SimilarityQueryBatchOnGPU(q.data(), similarityScale, ids.data(), similarityKValue)) return rc;
+  if (similarityScale < 0 || similarityKValue < 0) return GAT_ERR_ARG;
+  std::vector<Trajectory> q(size_t(similarityScale), t);
+  const size_t n = size_t(similarityScale) * size_t(similarityKValue);
   last = Result{};
+  cpuMs = 0;
+  cpuIds.assign(cpuSimilarity ? n : 0, 0);
+  if (cpuSimilarity) {
+    const double t0 = now_ms();
+    if (int rc = cpuSimilarity(q.data(), similarityScale, similarityKValue, cpuIds.data())) return rc;
+    cpuMs = now_ms() - t0;
+    if (verbose)
+      std::printf("multi-core CPU similarity @ k=%d and #query=%d: %.3f ms\n", similarityKValue, similarityScale,
+                  cpuMs);
+  }
+  gpuIds.assign(n, 0);
+  const double t0 = now_ms();
+  if (int rc = g->SimilarityQueryBatchOnGPU(q.data(), similarityScale, gpuIds.data(), similarityKValue)) return rc;
   last.wallMs = now_ms() - t0;
   if (verbose)
     std::printf("one GPU similarity @ k=%d and #query=%d: %.3f ms\n", similarityKValue, similarityScale, last.wallMs);

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "FSG.h"
@@ -33,7 +34,19 @@ class SystemTest {
   // SystemTest.cpp:386-445 (the reference issues 50 rectangles per call).
   int MortonGridRangeQueryTest(MBB rangeQueryMBB, int rangeQueryNum);
   int MortonGridRangeQueryTest(const MBB* bounds, int rangeQueryNum);
+  // SystemTest.cpp:164-249: the query replicated similarityScale times, the
+  // CPU top-k timed first (SimilarityQueryBatchCPUParallel, SystemTest.cpp:207),
+  // then the GPU top-k (SimilarityQueryBatchOnGPU, SystemTest.cpp:226).  This
+  // library keeps no CPU query path: the CPU leg runs the caller's cpuSimilarity
+  // when one is set and is skipped otherwise.  The ids of both legs stay in
+  // cpuIds / gpuIds (query-major, k per query).
   int similarityQueryTest(Trajectory t, int similarityScale, int similarityKValue);
+
+  // The CPU top-k of the similarity test: (queries, query count, k, ids out).
+  using CpuSimilarity = std::function<int(Trajectory*, int, int, int*)>;
+  CpuSimilarity cpuSimilarity;
+  std::vector<int> cpuIds, gpuIds;
+  double cpuMs = 0;
 
   struct Result {
     double wallMs = 0;      // whole call: host candidate selection, upload, kernel, count readback
@@ -43,6 +56,9 @@ class SystemTest {
     int64_t pairs = 0;
   };
   Result last;
+  // The last range test's result pairs (per-query counts, digests and pairs
+  // stay readable on the device).
+  RangeBatch lastBatch;
   bool verbose = true;
 
   Trajectory* tradb = nullptr;

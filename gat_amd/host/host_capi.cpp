@@ -292,6 +292,44 @@ int gath_system_range_test(void* p, int which, const float* boxes, int nq, int64
   if (rc) return rc;
   out[0] = t.last.candidates;
   out[1] = t.last.pairs;
+  ix->last = std::move(t.lastBatch);  // readable through gath_index_last_*
+  return GAT_OK;
+}
+
+int gath_system_similarity_test(void* p, const float* qlon, const float* qlat, int qlen, int scale, int k,
+                                double eps, gath_cpu_similarity_fn cpu_fn, void* cpu_ctx, int32_t* gpu_ids,
+                                int32_t* cpu_ids, double* ms) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid || qlen <= 0 || !qlon || !qlat || scale < 0 || k < 0 || !gpu_ids || !ms)
+    return bad("gath_system_similarity_test: bad arguments");
+  if (cpu_fn && !cpu_ids) return bad("gath_system_similarity_test: a CPU leg needs cpu_ids");
+  Trajectory t(1);
+  for (int i = 0; i < qlen; ++i) t.addSamplePoint(qlon[i], qlat[i]);
+  SystemTest st;
+  st.verbose = false;
+  st.g = ix->grid.get();
+  st.g->epsilon = eps;
+  if (cpu_fn)
+    st.cpuSimilarity = [&](Trajectory* q, int nq, int kk, int* ids) {
+      TrajectoryDB db;
+      for (int i = 0; i < nq; ++i) {
+        for (const SamplePoint& sp : q[i].points) {
+          db.lon.push_back(sp.lon);
+          db.lat.push_back(sp.lat);
+        }
+        db.offsets.push_back(int64_t(db.lon.size()));
+      }
+      if (cpu_fn(cpu_ctx, db.offsets.data(), nq, db.lon.data(), db.lat.data(), kk, ids) != 0) {
+        gath_set_error("gath_system_similarity_test: the CPU leg failed");
+        return int(GAT_ERR_STATE);
+      }
+      return int(GAT_OK);
+    };
+  if (int rc = st.similarityQueryTest(t, scale, k)) return rc;
+  std::copy(st.gpuIds.begin(), st.gpuIds.end(), gpu_ids);
+  if (cpu_fn) std::copy(st.cpuIds.begin(), st.cpuIds.end(), cpu_ids);
+  ms[0] = st.cpuMs;
+  ms[1] = st.last.wallMs;
   return GAT_OK;
 }
 
@@ -667,9 +705,11 @@ int gath_serve(void* p, const gath_request* reqs, int n, const float* qlon, cons
   Server server(*ix->grid, c);
   if (int rc = server.run(reqs, n, qlon, qlat, nqpoints, out, topk_ids, topk_dists, &st)) return rc;
   if (stats) {
-    const double v[9] = {double(st.rangeBatches), double(st.topkBatches), double(st.rangeDone), double(st.topkDone),
-                         st.wallMs, st.peakRangeBytes, st.peakTopkBytes, st.rangeBudgetBytes, st.topkBudgetBytes};
-    for (int i = 0; i < 9; ++i) stats[i] = v[i];
+    const double v[11] = {double(st.rangeBatches), double(st.topkBatches), double(st.rangeDone),
+                          double(st.topkDone),     st.wallMs,           st.peakRangeBytes,
+                          st.peakTopkBytes,        st.rangeBudgetBytes, st.topkBudgetBytes,
+                          double(st.peakTopkSurvivors), double(st.deviceAllocs)};
+    for (int i = 0; i < 11; ++i) stats[i] = v[i];
   }
   return GAT_OK;
 }

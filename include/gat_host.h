@@ -129,8 +129,22 @@ int gath_index_set_batch_params(void* index, int queries_per_item, int direct_ma
  * 251-384, 386-445) over the rectangles `boxes`: which = 0 rangeQueryTest
  * (a Grid or MortonGrid index), 1 STIGrangeQueryTest, 2 FSGrangeQueryTest,
  * 3 MortonGridRangeQueryTest.  out[0] = candidate points scanned, out[1] =
- * result pairs. */
+ * result pairs; the pairs are then the index's last result (gath_index_last_*). */
 int gath_system_range_test(void* index, int which, const float* boxes, int nq, int64_t* out);
+/* The CPU top-k a caller hands to gath_system_similarity_test: nq queries
+ * (q_offsets, nq + 1 entries, into qlon / qlat), k; writes nq * k trajectory
+ * ids to `ids` and returns 0, or non-zero on failure. */
+typedef int (*gath_cpu_similarity_fn)(void* ctx, const int64_t* q_offsets, int nq, const float* qlon,
+                                      const float* qlat, int k, int32_t* ids);
+/* The reference's SystemTest::similarityQueryTest (SystemTest.cpp:164-249) on
+ * a Grid: the query (qlen points) replicated `scale` times; the CPU leg
+ * (SystemTest.cpp:207: cpu_fn, skipped when NULL — this library has no CPU
+ * query path) timed, then the GPU leg, Grid::SimilarityQueryBatchOnGPU
+ * (SystemTest.cpp:226) with match threshold eps.  gpu_ids and cpu_ids (may be
+ * NULL) receive scale * k ids each; ms[0] = CPU leg, ms[1] = GPU leg. */
+int gath_system_similarity_test(void* grid, const float* qlon, const float* qlat, int qlen, int scale, int k,
+                                double eps, gath_cpu_similarity_fn cpu_fn, void* cpu_ctx, int32_t* gpu_ids,
+                                int32_t* cpu_ids, double* ms);
 /* The index's tree as the batched form uploads it (breadth-first
  * gat_tree_node; a MortonGrid's implicit tree gives its leaves in Morton
  * order); returns the node count, copies up to cap nodes. */
@@ -235,9 +249,11 @@ typedef struct gath_outcome {
  * top-k batch, range batch budget (bytes), top-k batch budget (bytes), k, eps,
  * batched (0: the plain per-request path).  topk_ids / topk_dists receive k
  * entries per request (used for top-k requests; ids 0 / distance -1 pad a
- * short answer).  stats[0..8] = range batches, top-k batches, range done,
- * top-k done, wall ms, largest range batch estimate, largest top-k batch
- * estimate, range budget and top-k budget after the free-memory check. */
+ * short answer).  stats[0..10] = range batches, top-k batches, range done,
+ * top-k done, wall ms, largest range batch charge, largest top-k batch
+ * charge (bytes), range budget and top-k budget after the free-memory check,
+ * most filter survivors of one top-k batch, device allocations made while
+ * serving (buffers that had to grow). */
 int gath_serve(void* grid, const gath_request* reqs, int n, const float* qlon, const float* qlat, int64_t nqpoints,
                const double* cfg, gath_outcome* out, int32_t* topk_ids, int32_t* topk_dists, double* stats);
 

@@ -37,13 +37,24 @@ def cfg2(gpu):
     return db, rects, sample
 
 
-def _sets(per_query):
-    return [set(map(tuple, np.asarray(r).tolist())) for r in per_query]
+def _keys(per_query):
+    """Each query's (trajectory id, point id) pairs as a sorted int64 array:
+    equal arrays are equal sets (no query holds a pair twice)."""
+    out = []
+    for r in per_query:
+        a = np.asarray(r, np.int64).reshape(-1, 2)
+        out.append(np.sort((a[:, 0] << 32) | a[:, 1]))
+    return out
+
+
+def _same_sets(got, want):
+    return len(got) == len(want) and all(np.array_equal(g, w) for g, w in zip(got, want))
 
 
 def _check_full(ix, oix, db, rects, sample, name):
     want_c, want_d = oix.range(rects, "digest", nthreads=THREADS)
     assert want_c.sum() > 600_000_000, name  # the batch really is the full workload
+    print(f"[cfg2] {name}: oracle counts and digests", flush=True)
     plain = ix.range_pairs(rects)
     assert (plain.counts() == want_c).all(), f"{name}: plain counts"
     assert (plain.digests() == want_d).all(), f"{name}: plain digests"
@@ -56,16 +67,16 @@ def _check_full(ix, oix, db, rects, sample, name):
     print(f"[cfg2] {name}: batched form and its graph replay equal the oracle", flush=True)
     # Full pair sets on the sample: plain, direct form, leaf-major form.
     sub = rects[sample]
-    want = _sets(oix.range(sub, nthreads=THREADS))
-    got_plain = _sets(ix.range_pairs(sub).per_query())
-    assert got_plain == want, f"{name}: plain sample sets"
+    want = _keys(oix.range(sub, nthreads=THREADS))
+    print(f"[cfg2] {name}: oracle pair sets of {SAMPLE} rectangles ({sum(map(len, want)):,} pairs)", flush=True)
+    assert _same_sets(_keys(ix.range_pairs(sub).per_query()), want), f"{name}: plain sample sets"
     ix.set_batch_params(16, 1 << 30)
     got_direct = ix.range_batched(sub)
-    assert _sets(got_direct.per_query()) == want, f"{name}: direct-form sample sets"
+    assert _same_sets(_keys(got_direct.per_query()), want), f"{name}: direct-form sample sets"
     ix.set_batch_params(16, 0)
     got_leaf = ix.range_batched(sub)
     assert got_leaf.work_items > 0
-    assert _sets(got_leaf.per_query()) == want, f"{name}: leaf-major sample sets"
+    assert _same_sets(_keys(got_leaf.per_query()), want), f"{name}: leaf-major sample sets"
     ix.set_batch_params(16, 256)
     print(f"[cfg2] {name}: {SAMPLE} sampled pair sets equal the oracle's", flush=True)
 

@@ -46,6 +46,40 @@ def test_cfg1_topk(cfg1_grid):
         assert (ids[qi] == oi).all() and (dists[qi] == od).all()
 
 
+def test_system_similarity_query_test(cfg1_grid):
+    """SystemTest::similarityQueryTest (SystemTest.cpp:164-249) through the C
+    ABI: the query replicated 3 times, the CPU leg (the oracle's restatement of
+    SimilarityQueryBatchCPUParallel, Grid.cpp:1133-1220, handed in as the
+    callback) timed first, then Grid::SimilarityQueryBatchOnGPU(Trajectory*, ...)
+    (Grid.cpp:1321-1755).  Every GPU row must be the exact top-k; the CPU leg's
+    frequency-distance loop is not exact (its bound is unproven and its stop
+    rule drops ties), so its k distances are only checked to be no better than
+    the exact ones, rank by rank."""
+    db, g, queries = cfg1_grid
+    k = W.CFG1.k
+    fdk = O.FdTopk(W.UNIT_BOX, W.CFG1.grid_cell, db)
+    calls = []
+
+    def cpu(qs, kk):
+        calls.append(qs.ntraj)
+        return fdk.topk(qs, kk, EPS)[0]
+
+    all_ids = np.arange(1, db.ntraj + 1, dtype=np.int32)
+    for qi in range(queries.ntraj):
+        q = queries.xy(qi + 1)
+        out = g.system_similarity_test(q, 3, k, EPS, cpu=cpu)
+        d = O.edr_batch(q, db, EPS, nthreads=8)
+        oi, od = O.topk(d, all_ids, k)
+        assert (out["gpu_ids"] == oi[None, :]).all(), f"query {qi}: GPU leg"
+        assert out["cpu_ms"] > 0 and out["gpu_ms"] > 0
+        cpu_d = np.sort(d[out["cpu_ids"] - 1], axis=1)
+        assert (out["cpu_ids"] > 0).all() and (cpu_d >= od[None, :]).all(), f"query {qi}: CPU leg"
+    assert calls == [3] * queries.ntraj
+    # Without a CPU leg only the GPU leg runs.
+    out = g.system_similarity_test(queries.xy(1), 2, k, EPS)
+    assert out["cpu_ids"] is None and out["cpu_ms"] == 0 and (out["gpu_ids"][1] == out["gpu_ids"][0]).all()
+
+
 def _pair_case(rng, n, m, kind):
     base = rng.random(2) * 0.8 + 0.1
     q = base + np.cumsum(rng.normal(0, 1e-4, (n, 2)), 0)

@@ -39,10 +39,38 @@ def test_each_result_same_alone_and_in_the_mix(served):
         assert (ids[i] == a_ids[0]).all() and (dists[i] == a_d[0]).all(), i
 
 
+def test_range_in_the_mix_matches_oracle(served):
+    """Every range result of the mix against the oracle's restatement of
+    Grid::rangeQueryBatch (Grid.cpp:341-405): count and pair digest."""
+    db, g, req, qlon, qlat, out, ids, dists, st = served
+    rng = np.nonzero(req["kind"] == api.REQ_RANGE)[0]
+    want_c, want_d = O.Grid(W.UNIT_BOX, 0.004, 256).add(db).range(req["box"][rng], "digest", nthreads=16)
+    assert want_c.sum() > 0
+    assert (out["count"][rng] == want_c).all() and (out["digest"][rng] == want_d).all()
+
+
+def test_range_admission_never_grows_buffers(served):
+    """Range requests are charged an exact bound of their result and list
+    buffers (from the leaves they select), so a range-only stream whose
+    requests fit the budget is served with the workspaces reserved up front:
+    no device allocation while serving.  A request above the budget on its own
+    is still admitted, alone, and its buffers grow."""
+    db, g, req, qlon, qlat, out, ids, dists, st = served
+    rng = req[req["kind"] == api.REQ_RANGE].copy()
+    want = out["count"][req["kind"] == api.REQ_RANGE]
+    r_out, _, _, r_st = api.serve(g, rng, qlon, qlat, K, EPS, range_budget=4e9)
+    assert r_st["device_allocs"] == 0
+    assert r_st["range_done"] == len(rng) and (r_out["count"] == want).all()
+    assert 0 < r_st["peak_range_bytes"] <= 4e9
+    t_out, _, _, t_st = api.serve(g, rng, qlon, qlat, K, EPS, range_budget=1.0)
+    assert t_st["range_batches"] == len(rng) and (t_out["count"] == want).all()
+
+
 def test_topk_in_the_mix_matches_oracle(served):
     db, g, req, qlon, qlat, out, ids, dists, st = served
     tid = np.arange(1, db.ntraj + 1, dtype=np.int32)
-    for i in np.nonzero(req["kind"] == api.REQ_TOPK)[0][:3]:
+    assert st["peak_topk_survivors"] > 0
+    for i in np.nonzero(req["kind"] == api.REQ_TOPK)[0][:12]:
         full = O.edr_batch(_query(req, i, qlon, qlat).xy(1), db, EPS, nthreads=16)
         oi, od = O.topk(full, tid, K)
         assert (ids[i] == oi).all() and (dists[i] == od).all(), i
