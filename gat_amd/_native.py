@@ -71,6 +71,7 @@ HIP_FUNCS = {
     "gat_cell_store_build": (INT, [P, P, P, P, I32, I64, I32, P, P, P, P, P, P]),
     "gatRangeQueryTestHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
     "gatRangeQuerySTIGHandler": (INT, [P, P, INT, P, P, INT, INT, P]),
+    "gatRangeQueryTestHandlerNoAlloc": (INT, [P, P, INT, P, P, INT, INT, P]),
     "gat_range_refine_bitmap": (INT, [P, P, I64, P, I64, P, I64, P]),
     "gat_range_refine_pairs": (INT, [P, P, I64, P, I64, P, P, P, I64, P]),
     "gat_pairs_digest": (INT, [P, P, P, I64, I64, P, P]),

@@ -25,6 +25,10 @@ inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 // Number of compute units of the current device (cached per device).
 int cu_count();
 
+// Resolves the current device's address of the no-alloc handler's bitmap
+// scratch (range.hip); gat_init calls it once per device.
+int scratch_setup();
+
 }  // namespace gat
 
 #define GAT_HIP_CHECK(call)                                              \

@@ -15,6 +15,9 @@
 // Ids are stored compactly: (tid << pid_bits) | pid in 32 bits when the
 // trajectory and point ids fit, otherwise a 32-bit tid array and a 16-bit pid
 // array — 4 or 6 bytes per hit instead of the 8 of an (int, int) pair.
+#include <atomic>
+#include <mutex>
+
 #include "gat_common.hpp"
 
 namespace {
@@ -220,6 +223,53 @@ __global__ __launch_bounds__(kBlock) void pairs_digest_kernel(const int2* __rest
 
 }  // namespace
 
+// Device bitmap of gatRangeQueryTestHandlerNoAlloc, for callers that cannot
+// allocate: one copy per GPU, held until the process ends, shared by every call
+// on that GPU (calls that overlap on one GPU race on it).  In host code the
+// name `scratch` only stands for the symbol; kernels and copies get the address
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchBytes = size_t(GAT_NOALLOC_BITMAP_MAX);
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+std::atomic<uint8_t*> g_scratch_ptr[kMaxDevices];
+
+// The current device's copy of `scratch`, resolved once per device (by
+// gat_init, or by the first no-alloc call on a device gat_init never saw).
+int scratch_address(uint8_t** out) {
+  int dev = 0;
+  GAT_HIP_CHECK(hipGetDevice(&dev));
+  GAT_REQUIRE(dev >= 0 && dev < kMaxDevices, "scratch: device %d beyond the %d supported", dev, kMaxDevices);
+  uint8_t* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void* addr = nullptr;
+      GAT_HIP_CHECK(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+      GAT_REQUIRE(addr, "scratch: hipGetSymbolAddress returned no address");
+      p = static_cast<uint8_t*>(addr);
+      g_scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return GAT_OK;
+}
+
+}  // namespace
+
+namespace gat {
+
+int scratch_setup() {
+  uint8_t* p = nullptr;
+  return scratch_address(&p);
+}
+
+}  // namespace gat
+
 extern "C" {
 
 int gat_range_refine_bitmap(const gat_point_store* store, const gat_range_task* tasks, int64_t ntasks,
@@ -245,6 +295,35 @@ int gatRangeQuerySTIGHandler(const gat_point_store* store, const gat_range_task*
                              const float* boxes, uint8_t* result_host, int maxTrajNum, int maxQueryNum,
                              void* stream) {
   return bitmap_handler(store, tasks, stateTableLength, boxes, result_host, maxTrajNum, maxQueryNum, stream);
+}
+
+int gatRangeQueryTestHandlerNoAlloc(const gat_point_store* store, const gat_range_task* tasks, int stateTableLength,
+                                    const float* boxes, uint8_t* result_host, int maxTrajNum, int maxQueryNum,
+                                    void* stream) {
+  GAT_REQUIRE(result_host, "range handler (no-alloc): null result buffer");
+  GAT_REQUIRE(maxTrajNum > 0 && maxQueryNum >= 0 && stateTableLength >= 0, "range handler (no-alloc): bad sizes");
+  const size_t bytes = size_t(maxTrajNum) * size_t(maxQueryNum);
+  if (bytes > kScratchBytes) {
+    gat::set_error("range handler (no-alloc): a %d x %d bitmap is %zu bytes, above the %zu-byte device scratch",
+                   maxQueryNum, maxTrajNum, bytes, kScratchBytes);
+    return GAT_ERR_CAPACITY;
+  }
+  if (bytes == 0) return GAT_OK;
+  uint8_t* bitmap = nullptr;
+  if (int rc = scratch_address(&bitmap)) return rc;
+  hipStream_t s = gat::as_stream(stream);
+  // The previous call's bitmap is still there: clear what this call reads.
+  int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(bitmap), 0u, int64_t((bytes + 3) / 4), stream);
+  if (rc == GAT_OK)
+    rc = launch_refine(false, store, tasks, stateTableLength, boxes, maxQueryNum, nullptr, nullptr, nullptr, 0, bitmap,
+                       maxTrajNum, stream);
+  if (rc == GAT_OK) {
+    hipError_t e = hipMemcpyAsync(result_host, bitmap, bytes, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) rc = gat::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+  }
+  hipError_t es = hipStreamSynchronize(s);
+  if (rc == GAT_OK && es != hipSuccess) rc = gat::hip_fail(es, "hipStreamSynchronize", __FILE__, __LINE__);
+  return rc;
 }
 
 int gat_pairs_digest(const int32_t* pairs, const uint64_t* qbase, const uint32_t* qcount, int64_t nq,

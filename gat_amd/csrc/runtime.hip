@@ -49,7 +49,7 @@ int gat_init(int device) {
   }
   GAT_HIP_CHECK(hipSetDevice(device));
   GAT_HIP_CHECK(hipFree(nullptr));  // force context creation
-  return GAT_OK;
+  return gat::scratch_setup();
 }
 
 int gat_device_name(char* buf, int len) {

@@ -184,6 +184,25 @@ int gatRangeQuerySTIGHandler(const gat_point_store* store, const gat_range_task*
                              const float* boxes_dev, uint8_t* result_host, int maxTrajNum, int maxQueryNum,
                              void* stream);
 
+/* Largest bitmap (maxQueryNum * maxTrajNum bytes) gatRangeQueryTestHandlerNoAlloc
+ * takes: 64 MiB, e.g. 100 rectangles x 671,088 trajectory slots. */
+#define GAT_NOALLOC_BITMAP_MAX (64u << 20)
+
+/* No counterpart in the reference: gatRangeQueryTestHandler for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and
+ * result as gatRangeQueryTestHandler, but the device bitmap is a fixed
+ * GAT_NOALLOC_BITMAP_MAX-byte array the library holds on every GPU it runs on
+ * until the process ends, so the call makes no hipMalloc, hipFree or other
+ * allocating call.  A bitmap larger than that returns GAT_ERR_CAPACITY before
+ * any device work.  Every call on one GPU shares its array (each call clears the
+ * part it uses first): calls that overlap on one GPU, from several threads or on
+ * several streams, race and must be serialised by the caller.  Synchronous on
+ * `stream`. */
+int gatRangeQueryTestHandlerNoAlloc(const gat_point_store* store, const gat_range_task* tasks_dev,
+                                    int stateTableLength, const float* boxes_dev, uint8_t* result_host,
+                                    int maxTrajNum, int maxQueryNum, void* stream);
+
 /* The same refine, with the trajectory bitmap left in device memory
  * (bitmap_dev: rows of `traj_stride` bytes, one row per query, zeroed by the
  * caller).  Asynchronous on `stream`.

@@ -24,5 +24,11 @@ def gpu():
 
 
 def pytest_collection_modifyitems(config, items):
-    # Run GPU tests in file order but keep the slow full-size ones last.
-    items.sort(key=lambda it: 1 if it.get_closest_marker("slow") else 0)
+    # Run GPU tests in file order but keep the slow full-size ones late and
+    # the allocation-free entry point's test (tests/test_zz_no_alloc.py) last.
+    def rank(it):
+        if "test_zz_" in it.nodeid:
+            return 2
+        return 1 if it.get_closest_marker("slow") else 0
+
+    items.sort(key=rank)
