@@ -559,7 +559,8 @@ def bench_edr(G, W, cpu_topk: bool = False) -> dict:
     topk_s = time.perf_counter() - t
     log(f"top-{w.k} batched: {queries.ntraj / topk_s:,.1f} queries/s ({1e3 * topk_s:.1f} ms for {queries.ntraj}; "
         f"{st['pairs']:,} pairs evaluated, {st['pruned']:,} resolved by the mismatch table; filter kernel "
-        f"{st['filter_ms']:.3f} ms, DP kernel {st['kernel_ms']:.3f} ms, host merge {st['host_ms']:.3f} ms)")
+        f"{st['filter_ms']:.3f} ms, DP kernel {st['kernel_ms']:.3f} ms, host merge {st['host_ms']:.3f} ms, query "
+        f"upload {st['upload_ms']:.3f} ms)")
     cpu = None
     if cpu_topk:
         import oracle as O

@@ -487,8 +487,9 @@ __global__ __launch_bounds__(1024) void edr_mismatch_table_kernel(int32_t maxlen
 // non-survivors of the lengths whose value is below T, the survivors whose DP
 // distance is), then the r smallest ids at distance T, found by a binary
 // search for the id bound X over the ascending id lists of the lengths whose
-// value is T (their survivors left out) and the survivors at T.  The kk
-// results are ranked by (distance, id).
+// value is T (their survivors left out) and the survivors at T; the search
+// range and each list's part of it are cut to the lists' first r + (number of
+// survivors) ids.  The kk results are ranked by (distance, id).
 constexpr int kTopkMax = 1024;
 constexpr int kLenBins = GAT_MAX_TRAJ_LEN + 1;     // lengths 0..GAT_MAX_TRAJ_LEN
 constexpr int kDistBins = 2 * GAT_MAX_TRAJ_LEN + 1;  // distances 0..n + m
@@ -535,7 +536,7 @@ __global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
   __shared__ int32_t hist[kDistBins];  // candidates per distance
   __shared__ int32_t out_d[kTopkMax], out_i[kTopkMax];
   __shared__ int32_t red[kTopkBlock / 64];
-  __shared__ int32_t n_out, sh_T, sh_below;
+  __shared__ int32_t n_out, sh_T, sh_below, sh_nt;
   const int q = blockIdx.x;
   const int tid = threadIdx.x;
   const int a = qlen[q];
@@ -609,12 +610,42 @@ __global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
     if (tid == 0) atomicOr(status, 2);
     return;
   }
+  // The distance histogram is done with: its space now holds the lengths
+  // valued T (when they fit) and this query's survivor ids, ascending (when
+  // they fit), so survivor lookups stay in LDS.
+  constexpr int32_t kTlMax = 2048, kStMax = kDistBins - kTlMax;
+  int32_t* tl = hist;             // lengths whose value is T
+  int32_t* st = hist + kTlMax;    // staged survivor ids
+  const int32_t ns = s1 - s0;
+  const bool staged = ns <= kStMax;
+  if (tid == 0) sh_nt = 0;
+  __syncthreads();
+  if (staged)
+    for (int32_t i = tid; i < ns; i += kTopkBlock) st[i] = sid[s0 + i];
+  for (int L = 1 + tid; L < kLenBins; L += kTopkBlock)
+    if (val[L] == T && len_off[L + 1] > len_off[L]) {
+      const int32_t at = atomicAdd(&sh_nt, 1);
+      if (at < kTlMax) tl[at] = L;
+    }
+  __syncthreads();
+  const int32_t nt = sh_nt;
+  auto is_survivor = [&](int32_t c) -> bool {
+    if (ns == 0) return false;
+    if (!staged) return survivor_index(sid, s0, s1, c) >= 0;
+    int32_t lo = 0, hi = ns;
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if (hist[kTlMax + mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo < ns && hist[kTlMax + lo] == c;
+  };
   // Every candidate below T: non-survivors by length, survivors by DP distance.
   for (int L = 1 + tid; L < kLenBins; L += kTopkBlock) {
     if (val[L] >= T) continue;
     for (int32_t i = len_off[L], e = len_off[L + 1]; i < e; ++i) {
       const int32_t c = len_ids[i];
-      if (s1 > s0 && survivor_index(sid, s0, s1, c) >= 0) continue;
+      if (is_survivor(c)) continue;
       const int32_t pos = atomicAdd(&n_out, 1);
       if (pos < kTopkMax) {
         out_d[pos] = val[L];
@@ -634,11 +665,40 @@ __global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
       }
     }
   }
-  // The r smallest ids at T: the least X with r of them at or below X.
+  // The r smallest ids at T: the least id X with r candidates at T at or
+  // below it, by a binary search over ids.  Each length valued T holds its ids
+  // ascending and at most ns of them are survivors (counted at their DP
+  // distance), so a list of at least r + ns ids has r non-survivors at or
+  // below its (r + ns)-th id: the least such id over those lists bounds X
+  // from above, and below that bound every list's count of ids <= X lies in
+  // its first r + ns ids.  The search then runs over that short range with
+  // short binary searches per list.
+  const int32_t pre = r + ns;
+  int32_t bound = nc > 0 ? nc - 1 : 0;
+  for (int32_t t = tid; t < nt && nt <= kTlMax; t += kTopkBlock) {
+    const int L = tl[t];
+    if (len_off[L + 1] - len_off[L] >= pre) bound = min(bound, len_ids[len_off[L] + pre - 1]);
+  }
+  if (nt > kTlMax)
+    for (int L = 1 + tid; L < kLenBins; L += kTopkBlock)
+      if (val[L] == T && len_off[L + 1] - len_off[L] >= pre) bound = min(bound, len_ids[len_off[L] + pre - 1]);
+  for (int o = 32; o > 0; o >>= 1) bound = min(bound, __shfl_xor(bound, o));
+  if ((tid & 63) == 0) red[tid >> 6] = bound;
+  __syncthreads();
+  bound = min(min(red[0], red[1]), min(red[2], red[3]));
+  __syncthreads();  // every thread has read red before ties_le reuses it
+  auto list_le = [&](int L, int32_t X) -> int32_t {
+    const int32_t size = len_off[L + 1] - len_off[L];
+    return count_le(len_ids + len_off[L], size < pre ? size : pre, X);
+  };
   auto ties_le = [&](int32_t X) -> int32_t {
     int32_t part = 0;
-    for (int L = 1 + tid; L < kLenBins; L += kTopkBlock)
-      if (val[L] == T) part += count_le(len_ids + len_off[L], len_off[L + 1] - len_off[L], X);
+    if (nt <= kTlMax) {
+      for (int32_t t = tid; t < nt; t += kTopkBlock) part += list_le(tl[t], X);
+    } else {
+      for (int L = 1 + tid; L < kLenBins; L += kTopkBlock)
+        if (val[L] == T) part += list_le(L, X);
+    }
     for (int32_t i = s0 + tid; i < s1; i += kTopkBlock) {
       const int32_t c = sid[i];
       if (c > X || c < 0 || c >= nc) continue;
@@ -648,26 +708,31 @@ __global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
     }
     return topk_block_sum(part, red);
   };
-  int32_t lo = 0, hi = nc > 0 ? nc - 1 : 0;
+  int32_t lo = 0, hi = bound;
   while (lo < hi) {
     const int32_t mid = lo + ((hi - lo) >> 1);
     if (ties_le(mid) >= r) hi = mid;
     else lo = mid + 1;
   }
   const int32_t X = lo;
-  for (int L = 1 + tid; L < kLenBins; L += kTopkBlock) {
-    if (val[L] != T) continue;
+  auto collect = [&](int L) {
     const int32_t* ids = len_ids + len_off[L];
-    const int32_t u = count_le(ids, len_off[L + 1] - len_off[L], X);
+    const int32_t u = list_le(L, X);
     for (int32_t i = 0; i < u; ++i) {
       const int32_t c = ids[i];
-      if (s1 > s0 && survivor_index(sid, s0, s1, c) >= 0) continue;  // counted at its DP distance
+      if (is_survivor(c)) continue;  // counted at its DP distance
       const int32_t pos = atomicAdd(&n_out, 1);
       if (pos < kTopkMax) {
         out_d[pos] = T;
         out_i[pos] = c;
       }
     }
+  };
+  if (nt <= kTlMax) {
+    for (int32_t t = tid; t < nt; t += kTopkBlock) collect(tl[t]);
+  } else {
+    for (int L = 1 + tid; L < kLenBins; L += kTopkBlock)
+      if (val[L] == T) collect(L);
   }
   for (int32_t i = s0 + tid; i < s1; i += kTopkBlock) {
     const int32_t c = sid[i];

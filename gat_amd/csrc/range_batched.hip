@@ -99,12 +99,13 @@ struct LeafEmitter {
   uint64_t cand;  // this lane's share of the rectangle's candidate points
 
   // Records the lanes with emit set (their leaf idx holds cnt points); flushes
-  // when the buffer could not take another full wave.
+  // when the buffer could not take another full wave.  The per-leaf counts are
+  // added at the flush: an atomic here would hold the walk's next node loads
+  // behind it (gfx9 waits for stores and atomics with the loads in vmcnt).
   __device__ void push(bool emit, uint32_t idx, uint32_t cnt) {
     const uint64_t em = __ballot(emit);
     if (emit) {
       buf[n + lanes_below(em)] = idx;
-      if (node_qcount) atomicAdd(node_qcount + idx, 1u);
       cand += cnt;
     }
     n += uint32_t(__popcll(em));
@@ -120,7 +121,9 @@ struct LeafEmitter {
     base = uniform64(base);
     for (uint32_t e = lane; e < n; e += kWave) {
       const uint64_t at = base + e;
-      if (at < cap) lq[at] = make_uint2(buf[e], q);
+      const uint32_t leaf = buf[e];
+      if (at < cap) lq[at] = make_uint2(leaf, q);
+      if (node_qcount) atomicAdd(node_qcount + leaf, 1u);
     }
     n = 0;
     __syncthreads();  // the buffer is reused only after every lane has read it
@@ -901,17 +904,21 @@ __device__ __forceinline__ void write_pass_staged(const Store& store, const uint
     // adds at most 256, so appends never reach a slot still waiting.
     while (fill - flushed >= kFlush) flush_block();
   }
-  // The tail: fewer than 128 pairs, two per lane while both exist.
+  // The tail: fewer than 128 pairs, two per lane, padded with (-1, -1) to
+  // the end of its 128-byte line.  The item's region starts on a line and
+  // holds its bound rounded up to whole lines, so the padding stays inside
+  // it; every line of the region is then written whole, and no partial line
+  // reaches HBM (a partial line costs a read of the line there).
   lds_order();
   const uint32_t rem = fill - flushed;
+  const uint32_t padded = (rem + 15u) & ~15u;
   const uint32_t i = 2 * lane;
   const uint64_t at = base + flushed + i;
-  if (i + 1 < rem) {
-    const int4 v = reinterpret_cast<const int4*>(ring)[((flushed & (kRing - 1)) >> 1) + lane];
+  if (i < padded) {
+    int4 v = reinterpret_cast<const int4*>(ring)[((flushed & (kRing - 1)) >> 1) + lane];
+    if (i >= rem) v.x = v.y = -1;
+    if (i + 1 >= rem) v.z = v.w = -1;
     if (at + 1 < capacity) gat::store_pair2(pairs + at, v);
-  } else if (i < rem) {
-    const int2 v = ring[(flushed + i) & (kRing - 1)];
-    if (at < capacity) gat::store_pair(pairs + at, v);
   }
   lds_order();
 }

@@ -40,6 +40,7 @@ struct EdrBatchStats {
   float kernelMs = 0;      // batched DP kernel (device events)
   float filterMs = 0;      // candidate filter kernel (device events)
   double hostMs = 0;       // host job building and top-k assembly
+  double uploadMs = 0;     // query store packing and upload
 };
 
 class Grid {
@@ -183,6 +184,7 @@ class Grid {
   // GAT_EDR_TAIL_PAD after the last) for the batched kernel.
   DeviceBuffer trajXY_, trajOff_, trajLen_, trajBox_, allTasks_, dist_;
   DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
+  PinnedBuffer queryStage_;  // the padded query store, packed here and uploaded from here
   DeviceBuffer pairsDev_, jobsDev_, outDev_, counterDev_, ftableDev_;
   // Top-k merge on the device: trajectories grouped by length (0-based ids,
   // ascending) with per-length offsets; per batch, each query's survivors

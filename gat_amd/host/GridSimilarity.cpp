@@ -109,15 +109,67 @@ int Grid::ensureQueryStore(const TrajectoryView& q) {
       return GAT_ERR_ARG;
     }
   }
-  Padded p = pad_store(q);
-  queryLenHost_ = p.lengths;
-  std::vector<int32_t> off32(p.offsets.begin(), p.offsets.end());
+  // The padded store is packed straight into pinned memory (point pairs,
+  // 64- and 32-bit offsets, lengths, boxes), so each array goes up as one
+  // DMA instead of a staged pageable copy.
+  const int T = q.trajNum;
+  const int64_t total = q.pointNum() + int64_t(GAT_EDR_PAD) * T + GAT_EDR_TAIL_PAD + GAT_EDR_PAD;
+  auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
+  const size_t bxy = up16(size_t(total) * 8), b64 = up16(size_t(T) * 8), b32 = up16(size_t(T) * 4);
+  const size_t bbox = up16(size_t(T) * 16);
+  if (int rc = queryStage_.reserve(bxy + b64 + 2 * b32 + bbox)) return rc;
+  char* base = queryStage_.as<char>();
+  float* xy = reinterpret_cast<float*>(base);
+  int64_t* off64 = reinterpret_cast<int64_t*>(base + bxy);
+  int32_t* off32 = reinterpret_cast<int32_t*>(base + bxy + b64);
+  int32_t* lens = reinterpret_cast<int32_t*>(base + bxy + b64 + b32);
+  float* boxes = reinterpret_cast<float*>(base + bxy + b64 + 2 * b32);
+  const float nan = std::numeric_limits<float>::quiet_NaN();
+  queryLenHost_.resize(size_t(T));
+  int64_t at = GAT_EDR_PAD;
+  for (int t = 0; t < T; ++t) {
+    const int64_t len = q.offsets[t + 1] - q.offsets[t];
+    off64[t] = at;
+    off32[t] = int32_t(at);
+    lens[t] = int32_t(len);
+    queryLenHost_[size_t(t)] = int32_t(len);
+    at += len + GAT_EDR_PAD;
+  }
+  std::fill(xy, xy + 2 * GAT_EDR_PAD, nan);
+  std::fill(xy + 2 * at, xy + 2 * total, nan);
+  // Each trajectory's points, box and trailing padding; a large batch packs
+  // on several threads.
+  auto pack = [&](int t) {
+    const int64_t b = q.offsets[t], e = q.offsets[t + 1];
+    float* out = xy + 2 * off64[t];
+    float xl = INFINITY, yl = INFINITY, xh = -INFINITY, yh = -INFINITY;
+    for (int64_t i = b; i < e; ++i, out += 2) {
+      out[0] = q.lon[i];
+      out[1] = q.lat[i];
+      xl = std::min(xl, q.lon[i]);
+      xh = std::max(xh, q.lon[i]);
+      yl = std::min(yl, q.lat[i]);
+      yh = std::max(yh, q.lat[i]);
+    }
+    std::fill(out, out + 2 * GAT_EDR_PAD, nan);
+    boxes[4 * t] = xl;
+    boxes[4 * t + 1] = yl;
+    boxes[4 * t + 2] = xh;
+    boxes[4 * t + 3] = yh;
+  };
+  const int packThreads = q.pointNum() >= (int64_t(1) << 17) ? std::min(threads(), 8) : 1;
+  parallel_for(T, packThreads, pack);
   void* s = engine_.stream();
-  if (int rc = upload_vec(queryXY_, p.xy, s, "query upload")) return rc;
-  if (int rc = upload_vec(queryOff_, off32, s, "query upload")) return rc;
-  if (int rc = upload_vec(queryOff64_, p.offsets, s, "query upload")) return rc;
-  if (int rc = upload_vec(queryLen_, p.lengths, s, "query upload")) return rc;
-  if (int rc = upload_vec(queryBox_, p.boxes, s, "query upload")) return rc;
+  auto put = [&](DeviceBuffer& d, const void* src, size_t bytes) -> int {
+    if (int rc = d.reserve(std::max<size_t>(bytes, 16))) return rc;
+    if (bytes) GATH_TRY(gat_memcpy_h2d(d.get(), src, bytes, s), "query upload");
+    return GAT_OK;
+  };
+  if (int rc = put(queryXY_, xy, size_t(total) * 8)) return rc;
+  if (int rc = put(queryOff64_, off64, size_t(T) * 8)) return rc;
+  if (int rc = put(queryOff_, off32, size_t(T) * 4)) return rc;
+  if (int rc = put(queryLen_, lens, size_t(T) * 4)) return rc;
+  if (int rc = put(queryBox_, boxes, size_t(T) * 16)) return rc;
   GATH_TRY(gat_stream_sync(s), "query upload");
   return GAT_OK;
 }
@@ -467,7 +519,9 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
 
 int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vector<TopK>& out, EdrBatchStats* st) {
   if (int rc = checkTrajLengths("similarityBatched")) return rc;
+  const double u0 = now_ms();
   if (int rc = ensureQueryStore(queries)) return rc;
+  if (st) st->uploadMs += now_ms() - u0;
   const int nq = queries.trajNum;
   if (nq > 65535) {
     gath_set_error("similarityBatched: at most 65535 queries per call");

@@ -536,6 +536,7 @@ void put_stats(const EdrBatchStats& st, int64_t* stats, double* ms) {
     ms[0] = st.kernelMs;
     ms[1] = st.filterMs;
     ms[2] = st.hostMs;
+    ms[3] = st.uploadMs;
   }
 }
 }  // namespace
