@@ -79,6 +79,7 @@ HIP_FUNCS = {
     "gat_edr_batched": (INT, [P, I64, P, I64, P, P, P, P, I32, P, P, P, I32, I32, F64, I32, I32, P, P]),
     "gat_edr_filter": (INT, [P, I32, P, I32, F64, P, I64, P, P]),
     "gat_edr_mismatch_table": (INT, [I32, P, P]),
+    "gat_edr_pack_trajectories": (INT, [P, P, P, P, I32, I64, P, P, P]),
     "gat_edr_topk": (INT, [P, I32, P, I32, P, I32, P, P, P, P, P, I32, I32, P, P, P, P]),
     "gat_fill_u32": (INT, [P, U32, I64, P]),
     "gat_scan_tiles": (I64, [I64]),

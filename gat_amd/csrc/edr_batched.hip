@@ -803,6 +803,57 @@ extern "C" int gat_edr_topk(const int32_t* ftable, int32_t w, const int32_t* qle
   return GAT_OK;
 }
 
+namespace {
+
+// One wave per trajectory: its points, given as separate lon / lat arrays, go
+// to the padded interleaved store at dst_off[t], and its box is reduced over
+// the wave.  The store is NaN-filled first by the entry point.
+__global__ __launch_bounds__(64) void edr_pack_kernel(const float* __restrict__ lon, const float* __restrict__ lat,
+                                                      const int64_t* __restrict__ src_off,
+                                                      const int64_t* __restrict__ dst_off, int32_t ntraj,
+                                                      int64_t total, float2* __restrict__ xy,
+                                                      float4* __restrict__ box) {
+  for (int32_t t = int32_t(blockIdx.x); t < ntraj; t += int32_t(gridDim.x)) {
+    const int64_t b = src_off[t], e = src_off[t + 1], d = dst_off[t];
+    float xl = INFINITY, yl = INFINITY, xh = -INFINITY, yh = -INFINITY;
+    for (int64_t i = b + int64_t(threadIdx.x); i < e; i += 64) {
+      const float x = lon[i], y = lat[i];
+      const int64_t at = d + (i - b);
+      if (at >= 0 && at < total) xy[at] = make_float2(x, y);
+      xl = fminf(xl, x);
+      xh = fmaxf(xh, x);
+      yl = fminf(yl, y);
+      yh = fmaxf(yh, y);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      xl = fminf(xl, __shfl_xor(xl, o));
+      xh = fmaxf(xh, __shfl_xor(xh, o));
+      yl = fminf(yl, __shfl_xor(yl, o));
+      yh = fmaxf(yh, __shfl_xor(yh, o));
+    }
+    if (threadIdx.x == 0) box[t] = make_float4(xl, yl, xh, yh);
+  }
+}
+
+}  // namespace
+
+extern "C" int gat_edr_pack_trajectories(const float* lon, const float* lat, const int64_t* src_off,
+                                         const int64_t* dst_off, int32_t ntraj, int64_t total, float* xy, float* box,
+                                         void* stream) {
+  GAT_REQUIRE(ntraj >= 0 && total >= 0, "gat_edr_pack_trajectories: bad sizes");
+  GAT_REQUIRE(total == 0 || xy, "gat_edr_pack_trajectories: null store");
+  if (total > 0)
+    if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(xy), 0x7FC00000u, 2 * total, stream)) return rc;
+  if (ntraj == 0) return GAT_OK;
+  GAT_REQUIRE(lon && lat && src_off && dst_off && box, "gat_edr_pack_trajectories: null pointer");
+  const int64_t cap = int64_t(gat::cu_count()) * 16;
+  const uint32_t blocks = uint32_t(ntraj < cap ? ntraj : cap);
+  edr_pack_kernel<<<dim3(blocks), dim3(64), 0, gat::as_stream(stream)>>>(
+      lon, lat, src_off, dst_off, ntraj, total, reinterpret_cast<float2*>(xy), reinterpret_cast<float4*>(box));
+  GAT_LAUNCH_CHECK("edr_pack_kernel");
+  return GAT_OK;
+}
+
 extern "C" int gat_edr_mismatch_table(int32_t maxlen, int32_t* out, void* stream) {
   GAT_REQUIRE(maxlen >= 0 && maxlen <= GAT_MAX_TRAJ_LEN, "mismatch table: maxlen outside 0..%d", GAT_MAX_TRAJ_LEN);
   GAT_REQUIRE(out, "mismatch table: null output");

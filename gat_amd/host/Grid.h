@@ -184,7 +184,8 @@ class Grid {
   // GAT_EDR_TAIL_PAD after the last) for the batched kernel.
   DeviceBuffer trajXY_, trajOff_, trajLen_, trajBox_, allTasks_, dist_;
   DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
-  PinnedBuffer queryStage_;  // the padded query store, packed here and uploaded from here
+  DeviceBuffer queryRaw_, querySrcOff_;  // raw query coordinates and offsets, packed on the device
+  PinnedBuffer queryStage_;  // query coordinates and offsets, uploaded from here
   DeviceBuffer pairsDev_, jobsDev_, outDev_, counterDev_, ftableDev_;
   // Top-k merge on the device: trajectories grouped by length (0-based ids,
   // ascending) with per-length offsets; per batch, each query's survivors

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <numeric>
 
@@ -109,67 +110,60 @@ int Grid::ensureQueryStore(const TrajectoryView& q) {
       return GAT_ERR_ARG;
     }
   }
-  // The padded store is packed straight into pinned memory (point pairs,
-  // 64- and 32-bit offsets, lengths, boxes), so each array goes up as one
-  // DMA instead of a staged pageable copy.
+  // The raw coordinates and the offsets go up from pinned memory (two plain
+  // copies on the host, one DMA each), and the device packs the padded store
+  // and the boxes (gat_edr_pack_trajectories).
   const int T = q.trajNum;
-  const int64_t total = q.pointNum() + int64_t(GAT_EDR_PAD) * T + GAT_EDR_TAIL_PAD + GAT_EDR_PAD;
+  const int64_t o0 = T > 0 ? q.offsets[0] : 0;
+  const int64_t n = T > 0 ? q.offsets[T] - o0 : 0;
+  const int64_t total = n + int64_t(GAT_EDR_PAD) * T + GAT_EDR_TAIL_PAD + GAT_EDR_PAD;
   auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
-  const size_t bxy = up16(size_t(total) * 8), b64 = up16(size_t(T) * 8), b32 = up16(size_t(T) * 4);
-  const size_t bbox = up16(size_t(T) * 16);
-  if (int rc = queryStage_.reserve(bxy + b64 + 2 * b32 + bbox)) return rc;
+  const size_t braw = up16(size_t(n) * 4), bsrc = up16((size_t(T) + 1) * 8), b64 = up16(size_t(T) * 8);
+  const size_t b32 = up16(size_t(T) * 4);
+  if (int rc = queryStage_.reserve(2 * braw + bsrc + b64 + 2 * b32)) return rc;
   char* base = queryStage_.as<char>();
-  float* xy = reinterpret_cast<float*>(base);
-  int64_t* off64 = reinterpret_cast<int64_t*>(base + bxy);
-  int32_t* off32 = reinterpret_cast<int32_t*>(base + bxy + b64);
-  int32_t* lens = reinterpret_cast<int32_t*>(base + bxy + b64 + b32);
-  float* boxes = reinterpret_cast<float*>(base + bxy + b64 + 2 * b32);
-  const float nan = std::numeric_limits<float>::quiet_NaN();
+  float* lon = reinterpret_cast<float*>(base);
+  float* lat = reinterpret_cast<float*>(base + braw);
+  int64_t* src = reinterpret_cast<int64_t*>(base + 2 * braw);
+  int64_t* off64 = reinterpret_cast<int64_t*>(base + 2 * braw + bsrc);
+  int32_t* off32 = reinterpret_cast<int32_t*>(base + 2 * braw + bsrc + b64);
+  int32_t* lens = reinterpret_cast<int32_t*>(base + 2 * braw + bsrc + b64 + b32);
+  if (n > 0) {
+    std::memcpy(lon, q.lon + o0, size_t(n) * 4);
+    std::memcpy(lat, q.lat + o0, size_t(n) * 4);
+  }
   queryLenHost_.resize(size_t(T));
   int64_t at = GAT_EDR_PAD;
   for (int t = 0; t < T; ++t) {
     const int64_t len = q.offsets[t + 1] - q.offsets[t];
+    src[t] = q.offsets[t] - o0;
     off64[t] = at;
     off32[t] = int32_t(at);
     lens[t] = int32_t(len);
     queryLenHost_[size_t(t)] = int32_t(len);
     at += len + GAT_EDR_PAD;
   }
-  std::fill(xy, xy + 2 * GAT_EDR_PAD, nan);
-  std::fill(xy + 2 * at, xy + 2 * total, nan);
-  // Each trajectory's points, box and trailing padding; a large batch packs
-  // on several threads.
-  auto pack = [&](int t) {
-    const int64_t b = q.offsets[t], e = q.offsets[t + 1];
-    float* out = xy + 2 * off64[t];
-    float xl = INFINITY, yl = INFINITY, xh = -INFINITY, yh = -INFINITY;
-    for (int64_t i = b; i < e; ++i, out += 2) {
-      out[0] = q.lon[i];
-      out[1] = q.lat[i];
-      xl = std::min(xl, q.lon[i]);
-      xh = std::max(xh, q.lon[i]);
-      yl = std::min(yl, q.lat[i]);
-      yh = std::max(yh, q.lat[i]);
-    }
-    std::fill(out, out + 2 * GAT_EDR_PAD, nan);
-    boxes[4 * t] = xl;
-    boxes[4 * t + 1] = yl;
-    boxes[4 * t + 2] = xh;
-    boxes[4 * t + 3] = yh;
-  };
-  const int packThreads = q.pointNum() >= (int64_t(1) << 17) ? std::min(threads(), 8) : 1;
-  parallel_for(T, packThreads, pack);
+  src[T] = n;
   void* s = engine_.stream();
-  auto put = [&](DeviceBuffer& d, const void* src, size_t bytes) -> int {
+  auto put = [&](DeviceBuffer& d, const void* from, size_t bytes) -> int {
     if (int rc = d.reserve(std::max<size_t>(bytes, 16))) return rc;
-    if (bytes) GATH_TRY(gat_memcpy_h2d(d.get(), src, bytes, s), "query upload");
+    if (bytes) GATH_TRY(gat_memcpy_h2d(d.get(), from, bytes, s), "query upload");
     return GAT_OK;
   };
-  if (int rc = put(queryXY_, xy, size_t(total) * 8)) return rc;
+  // Raw lon then lat, adjacent in pinned memory, go up as one copy.
+  if (int rc = queryRaw_.reserve(std::max<size_t>(2 * braw, 16))) return rc;
+  if (n > 0) GATH_TRY(gat_memcpy_h2d(queryRaw_.get(), lon, 2 * braw, s), "query upload");
+  if (int rc = put(querySrcOff_, src, (size_t(T) + 1) * 8)) return rc;
   if (int rc = put(queryOff64_, off64, size_t(T) * 8)) return rc;
   if (int rc = put(queryOff_, off32, size_t(T) * 4)) return rc;
   if (int rc = put(queryLen_, lens, size_t(T) * 4)) return rc;
-  if (int rc = put(queryBox_, boxes, size_t(T) * 16)) return rc;
+  if (int rc = queryXY_.reserve(size_t(total) * 8)) return rc;
+  if (int rc = queryBox_.reserve(std::max<size_t>(size_t(T) * 16, 16))) return rc;
+  const float* rlon = queryRaw_.as<float>();
+  const float* rlat = reinterpret_cast<const float*>(queryRaw_.as<char>() + braw);
+  GATH_TRY(gat_edr_pack_trajectories(rlon, rlat, querySrcOff_.as<int64_t>(), queryOff64_.as<int64_t>(), T, total,
+                                     queryXY_.as<float>(), queryBox_.as<float>(), s),
+           "query packing");
   GATH_TRY(gat_stream_sync(s), "query upload");
   return GAT_OK;
 }

@@ -322,6 +322,18 @@ int gat_edr_topk(const int32_t* ftable_dev, int32_t w, const int32_t* qlen_dev, 
                  const int32_t* surv_id_dev, const int32_t* surv_dist_dev, int32_t k, int32_t kk, int32_t* out_ids_dev,
                  int32_t* out_dist_dev, int32_t* status_dev, void* stream);
 
+/* Builds the padded store the batched EDR kernels read (no reference
+ * counterpart: the reference packs its queries into SPoint arrays on the host and
+ * copies them one by one, Grid.cpp:1388-1412): fills
+ * xy_dev (total_points (lon, lat) float pairs) with NaN, then writes trajectory
+ * t's points lon_dev / lat_dev[src_offsets_dev[t] .. src_offsets_dev[t + 1]) as
+ * pairs from point dst_offsets_dev[t], and its (xmin, ymin, xmax, ymax) to
+ * box_dev[4t..4t+3].  Points that would land outside the store are dropped.
+ * Asynchronous on `stream`. */
+int gat_edr_pack_trajectories(const float* lon_dev, const float* lat_dev, const int64_t* src_offsets_dev,
+                              const int64_t* dst_offsets_dev, int32_t ntraj, int64_t total_points, float* xy_dev,
+                              float* box_dev, void* stream);
+
 /* out_dev[r * (maxlen + 1) + c] = the recurrence of Grid.cpp:1264-1306 with
  * every substitution costing 1, for 0 <= r, c <= maxlen: the distance of a pair
  * of lengths r <= c with no matching point.  Asynchronous on `stream`. */
