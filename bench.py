@@ -565,8 +565,8 @@ def bench_edr(G, W, cpu_topk: bool = False) -> dict:
     log(f"top-{w.k} batched: {queries.ntraj / topk_s:,.1f} queries/s (median of {len(runs)} batches: "
         f"{1e3 * topk_s:.2f} ms for {queries.ntraj}, fastest {1e3 * runs[0][0]:.2f} ms; {st['pairs']:,} pairs "
         f"evaluated, {st['pruned']:,} resolved by the mismatch table; filter kernel {st['filter_ms']:.3f} ms, DP "
-        f"kernel {st['kernel_ms']:.3f} ms, host work and merge {st['host_ms']:.3f} ms, query upload "
-        f"{st['upload_ms']:.3f} ms)")
+        f"kernel {st['kernel_ms']:.3f} ms, host job building {st['host_ms']:.3f} ms, merge stage "
+        f"{st['merge_ms']:.3f} ms, query upload {st['upload_ms']:.3f} ms)")
     cpu = None
     if cpu_topk:
         import oracle as O
@@ -608,7 +608,8 @@ def bench_edr(G, W, cpu_topk: bool = False) -> dict:
             "topk_batched_qps": round(queries.ntraj / topk_s, 2),
             "topk_batch_ms": round(1e3 * topk_s, 3),
             "topk_stage_ms": {"query_upload": round(st["upload_ms"], 3), "filter_kernel": round(st["filter_ms"], 3),
-                              "dp_kernels": round(st["kernel_ms"], 3), "host_and_merge": round(st["host_ms"], 3)},
+                              "dp_kernels": round(st["kernel_ms"], 3), "job_building": round(st["host_ms"], 3),
+                              "merge_stage": round(st["merge_ms"], 3)},
             "topk_pairs_evaluated": int(st["pairs"]),
             "edr_autotuned": tuned,
             "topk_cpu": cpu}

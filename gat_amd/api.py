@@ -544,7 +544,7 @@ class GridIndex(_Index):
         one persistent batched launch."""
         dist = np.empty((queries.ntraj, self.ntraj), np.int32)
         stats = np.zeros(5, np.int64)
-        ms = np.zeros(4, np.float64)
+        ms = np.zeros(5, np.float64)
         N.check_host(N.host().gath_grid_edr_batched_all(
             self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj, _ptr(_req(queries.lon, np.float32)),
             _ptr(_req(queries.lat, np.float32)), eps, _ptr(dist), _ptr(stats), _ptr(ms)))
@@ -556,7 +556,7 @@ class GridIndex(_Index):
         p = _req(pairs, np.int32).reshape(-1, 2)
         dist = np.empty(max(len(p), 1), np.int32)
         stats = np.zeros(5, np.int64)
-        ms = np.zeros(4, np.float64)
+        ms = np.zeros(5, np.float64)
         N.check_host(N.host().gath_grid_edr_batched_pairs(
             self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj, _ptr(_req(queries.lon, np.float32)),
             _ptr(_req(queries.lat, np.float32)), _ptr(p), len(p), eps, _ptr(dist), _ptr(stats), _ptr(ms)))
@@ -569,7 +569,7 @@ class GridIndex(_Index):
         ids = np.empty((queries.ntraj, k), np.int32)
         dists = np.empty((queries.ntraj, k), np.int32)
         stats = np.zeros(5, np.int64)
-        ms = np.zeros(4, np.float64)
+        ms = np.zeros(5, np.float64)
         N.check_host(N.host().gath_grid_similarity_batched(
             self.handle, _ptr(_req(queries.offsets, np.int64)), queries.ntraj, _ptr(_req(queries.lon, np.float32)),
             _ptr(_req(queries.lat, np.float32)), k, eps, _ptr(ids), _ptr(dists), _ptr(stats), _ptr(ms)))
@@ -588,7 +588,7 @@ MAX_TRAJ_LEN = 2048
 def _edr_stats(stats, ms) -> dict:
     return {"pairs": int(stats[0]), "cells": int(stats[1]), "jobs": int(stats[2]), "pruned": int(stats[3]),
             "kernel_ms": float(ms[0]), "filter_ms": float(ms[1]), "host_ms": float(ms[2]),
-            "upload_ms": float(ms[3])}
+            "upload_ms": float(ms[3]), "merge_ms": float(ms[4])}
 
 
 def serve(grid: "GridIndex", requests: np.ndarray, qlon: np.ndarray, qlat: np.ndarray, k: int, eps: float,

@@ -1,5 +1,6 @@
 #include "DeviceArena.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
@@ -57,6 +58,19 @@ void PinnedBuffer::release() {
   if (p_) gat_host_free(p_);
   p_ = nullptr;
   n_ = 0;
+}
+
+int PinnedArena::begin(size_t bytes, void* stream) {
+  if (stream) GATH_TRY(gat_stream_sync(stream), "pinned arena");
+  used_ = 0;
+  return buf_.reserve(std::max<size_t>(bytes, 256));
+}
+
+void* PinnedArena::take(size_t bytes) {
+  const size_t at = (used_ + 255) & ~size_t(255);
+  if (at + bytes > buf_.size()) return nullptr;
+  used_ = at + bytes;
+  return buf_.as<char>() + at;
 }
 
 int Stream::create() {

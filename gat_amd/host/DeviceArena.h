@@ -72,6 +72,25 @@ class PinnedBuffer {
   size_t n_ = 0;
 };
 
+// Pinned host memory for one batch's transfers, handed out in 256-byte
+// aligned regions, so each copy is a DMA from or to pinned memory.  begin()
+// waits for `stream` (no copy may still use the arena), sizes it for `bytes`
+// and starts over; take() returns the next region, or null when the batch asks
+// for more than begin() sized.
+class PinnedArena {
+ public:
+  int begin(size_t bytes, void* stream);
+  void* take(size_t bytes);
+  template <class T>
+  T* take_as(size_t count) {
+    return static_cast<T*>(take(count * sizeof(T)));
+  }
+
+ private:
+  PinnedBuffer buf_;
+  size_t used_ = 0;
+};
+
 class Stream {
  public:
   Stream() = default;

@@ -39,8 +39,9 @@ struct EdrBatchStats {
   int64_t pruned = 0;      // pairs resolved by the all-mismatch table
   float kernelMs = 0;      // batched DP kernel (device events)
   float filterMs = 0;      // candidate filter kernel (device events)
-  double hostMs = 0;       // host job building and top-k assembly
+  double hostMs = 0;       // host job building (and the host top-k merge, k > 1024)
   double uploadMs = 0;     // query store packing and upload
+  double mergeMs = 0;      // device top-k merge stage: survivor lists, upload, merge kernel, readback
 };
 
 class Grid {
@@ -174,6 +175,8 @@ class Grid {
   int ensureQueryStore(const TrajectoryView& queries);
   int checkTrajLengths(const char* what) const;
   int runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* stats);
+  // Pinned arena bytes runBatched takes for np pairs (its callers begin() it).
+  static size_t runBatchedStaging(int64_t np);
   // Top-k of up to kDeviceTopkMax per query merged on the device; larger k on the host.
   static constexpr int kDeviceTopkMax = 1024;
   int deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_t>& dist, int nq, int kValue,
@@ -186,6 +189,7 @@ class Grid {
   DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
   DeviceBuffer queryRaw_, querySrcOff_;  // raw query coordinates and offsets, packed on the device
   PinnedBuffer queryStage_;  // query coordinates and offsets, uploaded from here
+  PinnedArena stage_;        // the transfers of one batched EDR / top-k call
   DeviceBuffer pairsDev_, jobsDev_, outDev_, counterDev_, ftableDev_;
   // Top-k merge on the device: trajectories grouped by length (0-based ids,
   // ascending) with per-length offsets; per batch, each query's survivors

@@ -266,6 +266,12 @@ const std::vector<int32_t>& Grid::mismatchTable() {
 // Orders the pairs so that consecutive groups of four share a strip count
 // (ceil(min length / 16)) and similar column counts, longest first, and runs
 // the batched kernel; dist receives each pair's distance in the original order.
+size_t Grid::runBatchedStaging(int64_t np) {
+  // Sorted pairs, jobs (at most one per pair), distances, and the regions'
+  // alignment.
+  return size_t(np) * 8 + (size_t(np) + 2) * sizeof(gat_edr_job) + size_t(np) * 4 + 3 * 256;
+}
+
 int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int32_t>& dist, EdrBatchStats* st) {
   const double h0 = now_ms();
   const int64_t np = int64_t(pairs.size() / 2);
@@ -354,9 +360,16 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
       perm[size_t(cnt[size_t(key[i] >> kDigit)]++)] = i;
     }
   }
-  std::vector<int32_t> sorted(size_t(np) * 2);
-  std::vector<gat_edr_job> jobs;
-  jobs.reserve(size_t(np / 4 + 2));
+  // Sorted pairs, jobs and distances pass through the pinned arena (its
+  // callers size it for runBatchedStaging(np)).
+  int32_t* sorted = stage_.take_as<int32_t>(size_t(np) * 2);
+  gat_edr_job* jobs = stage_.take_as<gat_edr_job>(size_t(np) + 2);
+  int32_t* outH = stage_.take_as<int32_t>(size_t(np));
+  if (!sorted || !jobs || !outH) {
+    gath_set_error("batched EDR: transfer arena too small for %lld pairs", (long long)np);
+    return GAT_ERR_STATE;
+  }
+  int64_t njobs = 0;
   int64_t bucketJobs[kBuckets] = {0, 0, 0, 0};
   int64_t i = 0;
   while (i < np) {
@@ -365,7 +378,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     const int32_t per_job = k0 / kS >= 2 ? 1 : 4;
     int32_t c = 0;
     while (c < per_job && i + c < np && key[perm[i + c]] / kM == k0) ++c;
-    jobs.push_back(gat_edr_job{int32_t(i), c});
+    jobs[njobs++] = gat_edr_job{int32_t(i), c};
     ++bucketJobs[k0 / kS];
     i += c;
   }
@@ -374,8 +387,10 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
     sorted[2 * j + 1] = pairs[2 * perm[j] + 1];
   }
   void* s = engine_.stream();
-  if (int rc = upload_vec(pairsDev_, sorted, s, "pair upload")) return rc;
-  if (int rc = upload_vec(jobsDev_, jobs, s, "job upload")) return rc;
+  if (int rc = pairsDev_.reserve(size_t(np) * 8)) return rc;
+  if (int rc = jobsDev_.reserve(std::max<size_t>(size_t(njobs) * sizeof(gat_edr_job), 16))) return rc;
+  GATH_TRY(gat_memcpy_h2d(pairsDev_.get(), sorted, size_t(np) * 8, s), "pair upload");
+  GATH_TRY(gat_memcpy_h2d(jobsDev_.get(), jobs, size_t(njobs) * sizeof(gat_edr_job), s), "job upload");
   if (int rc = outDev_.reserve(size_t(np) * 4)) return rc;
   if (int rc = counterDev_.reserve(16)) return rc;
   const double h1 = now_ms();
@@ -420,16 +435,15 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   if (side2)
     if (int rc = edrSide2_.join(s)) return rc;
   if (int rc = edrTimer_.stop(s)) return rc;
-  std::vector<int32_t> out(static_cast<size_t>(np));
-  GATH_TRY(gat_memcpy_d2h(out.data(), outDev_.get(), size_t(np) * 4, s), "batched EDR readback");
+  GATH_TRY(gat_memcpy_d2h(outH, outDev_.get(), size_t(np) * 4, s), "batched EDR readback");
   GATH_TRY(gat_stream_sync(s), "batched EDR readback");
   float ms = 0;
   if (int rc = edrTimer_.elapsed(&ms)) return rc;
-  for (int64_t j = 0; j < np; ++j) dist[perm[j]] = out[j];
+  for (int64_t j = 0; j < np; ++j) dist[perm[j]] = outH[j];
   if (st) {
     st->pairs += np;
     st->cells += cells;
-    st->jobs += int64_t(jobs.size());
+    st->jobs += njobs;
     st->kernelMs += ms;
     st->hostMs += (h1 - h0);
   }
@@ -439,6 +453,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
 int Grid::edrBatchedPairs(const TrajectoryView& queries, std::vector<int32_t>& pairs, std::vector<int32_t>& dist,
                           EdrBatchStats* st) {
   if (int rc = ensureQueryStore(queries)) return rc;
+  if (int rc = stage_.begin(runBatchedStaging(int64_t(pairs.size() / 2)), engine_.stream())) return rc;
   return runBatched(pairs, queries.trajNum, dist, st);
 }
 
@@ -451,6 +466,7 @@ int Grid::edrBatchedAll(const TrajectoryView& queries, std::vector<int32_t>& dis
       pairs[2 * (size_t(q) * trajNum + t)] = q;
       pairs[2 * (size_t(q) * trajNum + t) + 1] = t;
     }
+  if (int rc = stage_.begin(runBatchedStaging(int64_t(pairs.size() / 2)), engine_.stream())) return rc;
   return runBatched(pairs, nq, dist, st);
 }
 
@@ -471,8 +487,14 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
   // One upload: [start (nq + 1) | ids (count) | distances (count)]; one
   // readback: [ids (nq k) | distances (nq k) | status].
   const size_t ns = size_t(nq) + 1;
-  std::vector<int32_t> up(ns + 2 * count);
-  std::copy(sstart.begin(), sstart.end(), up.begin());
+  const size_t cells = size_t(nq) * size_t(kValue);
+  int32_t* up = stage_.take_as<int32_t>(ns + 2 * count);
+  int32_t* down = stage_.take_as<int32_t>(2 * cells + 1);
+  if (!up || !down) {
+    gath_set_error("top-k merge: transfer arena too small");
+    return GAT_ERR_STATE;
+  }
+  std::copy(sstart.begin(), sstart.end(), up);
   for (int q = 0; q < nq; ++q) {
     std::sort(surv.begin() + sstart[size_t(q)], surv.begin() + sstart[size_t(q) + 1]);
     for (int32_t i = sstart[size_t(q)]; i < sstart[size_t(q) + 1]; ++i) {
@@ -484,8 +506,8 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
   for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L) eligible += int64_t(lengthIds_[L].size());
   const int kk = int(std::min<int64_t>(kValue, eligible));
   void* s = engine_.stream();
-  if (int rc = upload_vec(survDev_, up, s, "survivor upload")) return rc;
-  const size_t cells = size_t(nq) * size_t(kValue);
+  if (int rc = survDev_.reserve((ns + 2 * count) * 4)) return rc;
+  GATH_TRY(gat_memcpy_h2d(survDev_.get(), up, (ns + 2 * count) * 4, s), "survivor upload");
   if (int rc = topDev_.reserve((2 * cells + 1) * 4)) return rc;
   const int32_t* sv = survDev_.as<int32_t>();
   int32_t* top = topDev_.as<int32_t>();
@@ -493,16 +515,15 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
                         trajLen_.as<int32_t>(), trajNum, lenIdsDev_.as<int32_t>(), lenOffDev_.as<int32_t>(), sv,
                         sv + ns, sv + ns + count, kValue, kk, top, top + cells, top + 2 * cells, s),
            "top-k merge");
-  std::vector<int32_t> down(2 * cells + 1);
-  GATH_TRY(gat_memcpy_d2h(down.data(), topDev_.get(), down.size() * 4, s), "top-k readback");
+  GATH_TRY(gat_memcpy_d2h(down, topDev_.get(), (2 * cells + 1) * 4, s), "top-k readback");
   GATH_TRY(gat_stream_sync(s), "top-k readback");
   const int32_t status = down[2 * cells];
   if (status != 0) {
     gath_set_error("top-k merge: inconsistent inputs (status %d)", status);
     return GAT_ERR_HIP;
   }
-  const auto ids = down.begin();
-  const auto dists = down.begin() + int64_t(cells);
+  const int32_t* ids = down;
+  const int32_t* dists = down + cells;
   out.assign(size_t(nq), TopK{});
   for (int q = 0; q < nq; ++q) {
     out[size_t(q)].ids.assign(ids + int64_t(q) * kValue, ids + int64_t(q) * kValue + kk);
@@ -550,9 +571,19 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
   }
   float fms = 0;
   if (int rc = ft.elapsed(&fms)) return rc;
-  std::vector<int32_t> pairs(size_t(count) * 2);
-  GATH_TRY(gat_memcpy_d2h(pairs.data(), pairsDev_.get(), size_t(count) * 8, s), "survivor readback");
+  // Every transfer of the batch from here passes through the pinned arena.
+  const size_t need = size_t(count) * 8 + runBatchedStaging(int64_t(count)) +
+                      (size_t(nq) + 1 + 2 * size_t(count)) * 4 + (2 * size_t(nq) * size_t(std::max(kValue, 0)) + 1) * 4 +
+                      4 * 256;
+  if (int rc = stage_.begin(need, s)) return rc;
+  int32_t* pairsH = stage_.take_as<int32_t>(size_t(count) * 2);
+  if (!pairsH) {
+    gath_set_error("similarityBatched: transfer arena too small");
+    return GAT_ERR_STATE;
+  }
+  GATH_TRY(gat_memcpy_d2h(pairsH, pairsDev_.get(), size_t(count) * 8, s), "survivor readback");
   GATH_TRY(gat_stream_sync(s), "survivor readback");
+  std::vector<int32_t> pairs(pairsH, pairsH + size_t(count) * 2);
   std::vector<int32_t> dist;
   if (int rc = runBatched(pairs, nq, dist, st)) return rc;
   const double h0 = now_ms();
@@ -561,7 +592,7 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
     if (st) {
       st->filterMs += fms;
       st->pruned += int64_t(nq) * trajNum - int64_t(count);
-      st->hostMs += now_ms() - h0;
+      st->mergeMs += now_ms() - h0;
     }
     return GAT_OK;
   }

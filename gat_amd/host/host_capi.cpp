@@ -537,6 +537,7 @@ void put_stats(const EdrBatchStats& st, int64_t* stats, double* ms) {
     ms[1] = st.filterMs;
     ms[2] = st.hostMs;
     ms[3] = st.uploadMs;
+    ms[4] = st.mergeMs;
   }
 }
 }  // namespace

@@ -175,8 +175,9 @@ int gath_grid_edr_all(void* grid, const float* qlon, const float* qlat, int qlen
 int gath_grid_similarity(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat, int k,
                          double eps, int32_t* ids, int32_t* dists);
 /* Batched forms.  stats[0..4] = pairs run, DP cells, jobs, pairs resolved by
- * the all-mismatch table, 0; ms[0..3] = DP kernel, filter kernel, host work,
- * query store packing and upload. */
+ * the all-mismatch table, 0; ms[0..4] = DP kernel, filter kernel, host work
+ * (job building; the host merge for k > 1024), query store packing and upload,
+ * the device merge stage (survivor lists, upload, merge kernel, readback). */
 int gath_grid_edr_batched_all(void* grid, const int64_t* q_offsets, int nq, const float* qlon, const float* qlat,
                               double eps, int32_t* dist, int64_t* stats, double* ms);
 /* The batched EDR of given (query index, trajectory index) pairs, both
