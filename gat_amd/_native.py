@@ -85,10 +85,10 @@ HIP_FUNCS = {
     "gat_scan_tiles": (I64, [I64]),
     "gat_scan_u64": (INT, [P, I64, P, P, P, P]),
     "gat_scan_leaf_lists": (INT, [P, I64, I32, P, P, P, P]),
-    "gat_tree_select": (INT, [P, I64, I32, P, P, I64, I64, P, P, P, I64, P, P, P]),
+    "gat_tree_select": (INT, [P, I64, I32, P, P, I64, I64, P, P, P, P, I64, P, P, P]),
     "gat_leaf_tasks": (INT, [P, P, I64, P, I64, P, P]),
-    "gat_grid_select": (INT, [P, I32, I32, F32, F32, F32, P, P, I64, I64, P, P, P, I64, P, P]),
-    "gat_morton_select": (INT, [P, I32, I64, F32, F32, F32, P, P, I64, I64, P, P, P, I64, P, P, P]),
+    "gat_grid_select": (INT, [P, I32, I32, F32, F32, F32, P, P, I64, I64, P, P, P, P, I64, P, P]),
+    "gat_morton_select": (INT, [P, I32, I64, F32, F32, F32, P, P, I64, I64, P, P, P, P, I64, P, P, P]),
     "gat_kd_store_build": (INT, [P, P, P, I32, I64, I32, P, P, P, I32, P, P, I64, P, P, P, P, P, P]),
     "gat_leaf_scatter": (INT, [P, P, I64, P, I64, P, P, P, I32, I64, P, I64, P, I64, P]),
     "gat_leaf_refine": (INT, [P, P, P, I64, P, I64, P, P, I64, P, P, I64, P, P]),

@@ -86,11 +86,15 @@ extern "C" int gat_hbm_pattern(int kind, void* buf, size_t bytes, uint32_t run, 
   GAT_REQUIRE(bytes % 16 == 0 && bytes >= 16, "HBM pattern: size must be a positive multiple of 16 bytes");
   GAT_REQUIRE(buf, "HBM pattern: null buffer");
   hipStream_t s = gat::as_stream(stream);
-  const uint64_t cap = uint64_t(gat::cu_count()) * 16;
+  // Writes run fastest from a wide grid (64 workgroups per CU: 5.5 TB/s
+  // against 4.1-5.0 from 16), reads from a narrower one (16 per CU: 6.3-7.1
+  // TB/s against 5.7 from 64).
+  const uint64_t cap = uint64_t(gat::cu_count()) * 64;
   if (kind <= 1) {
     const uint64_t n = bytes / 16;
     uint64_t blocks = (n + 255) / 256;
-    if (blocks > cap) blocks = cap;
+    const uint64_t stream_cap = kind == 0 ? cap : uint64_t(gat::cu_count()) * 16;
+    if (blocks > stream_cap) blocks = stream_cap;
     if (kind == 0)
       stream_write_kernel<<<dim3(uint32_t(blocks)), dim3(256), 0, s>>>(static_cast<v4f*>(buf), n);
     else
@@ -117,7 +121,8 @@ extern "C" int gat_stream_copy(void* dst, const void* src, size_t bytes, void* s
   GAT_REQUIRE(dst && src, "stream copy: null pointer");
   const uint64_t n = bytes / 16;
   uint64_t blocks = (n + 255) / 256;
-  const uint64_t cap = uint64_t(gat::cu_count()) * 16;
+  // 64 workgroups per CU: 5.22 TB/s, against 4.72-4.81 from 16 per CU.
+  const uint64_t cap = uint64_t(gat::cu_count()) * 64;
   if (blocks > cap) blocks = cap;
   stream_copy_kernel<<<dim3(uint32_t(blocks)), dim3(256), 0, gat::as_stream(stream)>>>(
       static_cast<const v4f*>(src), static_cast<v4f*>(dst), n);

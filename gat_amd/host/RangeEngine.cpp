@@ -528,7 +528,6 @@ int RangeEngine::reserveBatched(const DeviceTree& tree, int maxQueries, int64_t 
   if (int rc = devBoxes_.reserve(align_up(size_t(nq) * 16, 256) + size_t(nq) * 4)) return rc;
   if (int rc = hostStage_.reserve(align_up(size_t(nq) * 16, 256) + size_t(nq) * 4)) return rc;
   if (int rc = nodeCount_.reserve(size_t(N) * 4)) return rc;
-  if (int rc = nodeFill_.reserve(size_t(N) * 4)) return rc;
   if (int rc = nodeOff_.reserve(size_t(N) * 8)) return rc;
   if (int rc = tiles_.reserve(size_t(std::max(2 * gat_scan_tiles(N) * 4, gat_scan_tiles(nq) * 8)))) return rc;
   if (int rc = cand_.reserve(size_t(nq) * 8)) return rc;
@@ -543,6 +542,7 @@ int RangeEngine::reserveBatched(const DeviceTree& tree, int maxQueries, int64_t 
   itemsCap_ = std::max(itemsCap_, entries);
   pairsCap_ = std::max(pairsCap_, std::max<int64_t>(maxCandidates, 1024));
   if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+  if (int rc = lqRank_.reserve(size_t(lqCap_) * 4)) return rc;
   if (int rc = qlist_.reserve(size_t(qlistCap_) * 4)) return rc;
   if (int rc = items_.reserve(size_t(itemsCap_) * 16)) return rc;
   if (int rc = directTasks_.reserve(size_t(lqCap_) * sizeof(gat_range_task))) return rc;
@@ -623,7 +623,7 @@ RangeEngine::GraphKey RangeEngine::layoutKey(const DeviceTree& tree, int nq, con
   GraphKey key;
   const void* p[GraphKey::kPtrs] = {tree.nodes.get(), tree.cellWords.get(), devBoxes_.get(), out.pairs.get(),
                                     out.qbase.get(),  out.qcount.get(),     out.runs.get(),  lq_.get(),
-                                    nodeCount_.get(), nodeFill_.get(),      nodeOff_.get(),  tiles_.get(),
+                                    nodeCount_.get(), lqRank_.get(),        nodeOff_.get(),  tiles_.get(),
                                     cand_.get(),      qlist_.get(),         items_.get(),    itemBase_.get(),
                                     itemTiles_.get(), counters_.get(),      store_.xy_dev,   store_.id_dev};
   std::copy(p, p + GraphKey::kPtrs, key.ptr);
@@ -683,16 +683,18 @@ int RangeEngine::enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nod
   if (tree.flat()) {
     GATH_TRY(gat_grid_select(tree.nodes.as<gat_tree_node>(), tree.nx, tree.ny, tree.xmin, tree.ymax, tree.cellSize,
                              devBoxes_.as<float>(), order, 0, nq, nodeCount, cand_.as<uint64_t>(),
-                             lq_.as<uint32_t>(), lqCap_, total, s),
+                             lq_.as<uint32_t>(), nodeCount ? lqRank_.as<uint32_t>() : nullptr, lqCap_, total, s),
              "grid select");
   } else if (tree.implicit()) {
     GATH_TRY(gat_morton_select(tree.cellWords.as<uint32_t>(), tree.implicitLevel, tree.nnodes, tree.xmin, tree.ymax,
                                tree.cellSize, devBoxes_.as<float>(), order, 0, nq, nodeCount, cand_.as<uint64_t>(),
-                               lq_.as<uint32_t>(), lqCap_, total, ctr + 4, s),
+                               lq_.as<uint32_t>(), nodeCount ? lqRank_.as<uint32_t>() : nullptr, lqCap_, total,
+                               ctr + 4, s),
              "implicit tree select");
   } else {
     GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), tree.nnodes, tree.nchild, devBoxes_.as<float>(), order,
-                             0, nq, nodeCount, cand_.as<uint64_t>(), lq_.as<uint32_t>(), lqCap_, total, ctr + 4, s),
+                             0, nq, nodeCount, cand_.as<uint64_t>(), lq_.as<uint32_t>(),
+                             nodeCount ? lqRank_.as<uint32_t>() : nullptr, lqCap_, total, ctr + 4, s),
              "tree select");
   }
   return GAT_OK;
@@ -703,7 +705,6 @@ int RangeEngine::enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out) 
   const int64_t N = tree.nnodes;
   uint32_t* ctr = counters_.as<uint32_t>();
   GATH_TRY(gat_fill_u32(nodeCount_.as<uint32_t>(), 0, N, s), "batch reset");
-  GATH_TRY(gat_fill_u32(nodeFill_.as<uint32_t>(), 0, N, s), "batch reset");
   GATH_TRY(gat_fill_u32(out.qcount.as<uint32_t>(), 0, nq, s), "batch reset");
   GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
   if (int rc = enqueueTreeSelect(tree, nq, nodeCount_.as<uint32_t>())) return rc;
@@ -722,7 +723,7 @@ int RangeEngine::enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out) {
   uint32_t* ctr = counters_.as<uint32_t>();
   GATH_TRY(gat_leaf_scatter(lq_.as<uint32_t>(), reinterpret_cast<const uint64_t*>(ctr), lqCap_,
                             tree.nodes.as<gat_tree_node>(), N, nodeOff_.as<uint32_t>(), nodeCount_.as<uint32_t>(),
-                            nodeFill_.as<uint32_t>(), queriesPerItem, nq, qlist_.as<uint32_t>(), qlistCap_,
+                            lqRank_.as<uint32_t>(), queriesPerItem, nq, qlist_.as<uint32_t>(), qlistCap_,
                             items_.as<uint32_t>(), itemsCap_, s),
            "leaf scatter");
   if (runIndexed) {
@@ -877,7 +878,6 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
       out.qbase.size() < size_t(nq) * 8 || out.qcount.size() < size_t(nq) * 4)
     dropGraph();
   if (int rc = nodeCount_.reserve(size_t(N) * 4)) return rc;
-  if (int rc = nodeFill_.reserve(size_t(N) * 4)) return rc;
   if (int rc = nodeOff_.reserve(size_t(N) * 8)) return rc;
   if (int rc = tiles_.reserve(tileBytes)) return rc;
   if (int rc = cand_.reserve(size_t(nq) * 8)) return rc;
@@ -907,10 +907,9 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
     }
   }
   // Selection, then capacities from its totals, then the join.
-  if (lqCap_ < int64_t(nq) * 16) {
-    lqCap_ = std::max<int64_t>(int64_t(nq) * 16, 1 << 16);
-    if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
-  }
+  if (lqCap_ < int64_t(nq) * 16) lqCap_ = std::max<int64_t>(int64_t(nq) * 16, 1 << 16);
+  if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+  if (int rc = lqRank_.reserve(size_t(lqCap_) * 4)) return rc;
   for (int attempt = 0;; ++attempt) {
     if (int rc = selTimer_.start(stream_.get())) return rc;
     if (int rc = enqueueSelect(tree, nq, out)) return rc;
@@ -924,6 +923,7 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
     }
     lqCap_ = int64_t(u64(0) + u64(0) / 4 + 1024);
     if (int rc = lq_.reserve(size_t(lqCap_) * 8)) return rc;
+    if (int rc = lqRank_.reserve(size_t(lqCap_) * 4)) return rc;
   }
   float selMs = 0;
   if (int rc = selTimer_.elapsed(&selMs)) return rc;

@@ -244,7 +244,7 @@ class RangeEngine {
   // Batched-form workspace.  counters_ (uint32 words): [0,1] selected entries,
   // [2,3] candidate total, [4] status, [5] list total, [6] work items,
   // [8,9] result pairs, [10,11] run-indexed region total.
-  DeviceBuffer nodeCount_, nodeFill_, nodeOff_, tiles_, cand_, lq_, qlist_, items_, counters_, directTasks_;
+  DeviceBuffer nodeCount_, lqRank_, nodeOff_, tiles_, cand_, lq_, qlist_, items_, counters_, directTasks_;
   DeviceBuffer itemBase_, itemTiles_;  // run-indexed refine: item regions and their scan scratch
   PinnedBuffer countersHost_;
   int64_t lqCap_ = 0, qlistCap_ = 0, itemsCap_ = 0, pairsCap_ = 0;

@@ -404,11 +404,15 @@ int gat_scan_leaf_lists(const uint32_t* count_dev, int64_t nnodes, int32_t qb, u
  * uint32 per entry, order unspecified) while *lq_total < lq_capacity, always
  * advancing *lq_total; adds 1 to node_qcount[leaf] per entry (unless
  * node_qcount_dev is null); writes cand[q] = the points of q's selected leaves.
- * *lq_total and node_qcount must start at zero; leaves with no points are
- * skipped. */
+ * When node_qcount_dev and lq_rank_dev (lq_capacity entries) are both given,
+ * lq_rank[i] = entry i's rank among its leaf's entries (the value
+ * node_qcount[leaf] had before the entry's increment), which gat_leaf_scatter
+ * uses as the entry's place in the leaf's list.  *lq_total and node_qcount
+ * must start at zero; leaves with no points are skipped. */
 int gat_tree_select(const gat_tree_node* nodes_dev, int64_t nnodes, int32_t nchild, const float* boxes_dev,
-                    const uint32_t* order_dev, int64_t q0, int64_t nq, uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev,
-                    int64_t lq_capacity, uint64_t* lq_total_dev, uint32_t* status_dev, void* stream);
+                    const uint32_t* order_dev, int64_t q0, int64_t nq, uint32_t* node_qcount_dev, uint64_t* cand_dev,
+                    uint32_t* leaf_query_dev, uint32_t* lq_rank_dev, int64_t lq_capacity, uint64_t* lq_total_dev,
+                    uint32_t* status_dev, void* stream);
 
 /* The same selection for the flat grid (FSG::rangeQueryBatchGPU, FSG.cpp:183-312):
  * rectangle q visits every cell of its column range g1..g2 and row range
@@ -418,8 +422,8 @@ int gat_tree_select(const gat_tree_node* nodes_dev, int64_t nnodes, int32_t nchi
  * Outputs as gat_tree_select. */
 int gat_grid_select(const gat_tree_node* cells_dev, int32_t nx, int32_t ny, float xmin, float ymax, float cell_size,
                     const float* boxes_dev, const uint32_t* order_dev, int64_t q0, int64_t nq,
-                    uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, int64_t lq_capacity,
-                    uint64_t* lq_total_dev, void* stream);
+                    uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, uint32_t* lq_rank_dev,
+                    int64_t lq_capacity, uint64_t* lq_total_dev, void* stream);
 
 /* The same selection over an implicit quadtree (MortonGrid, MortonGrid.cpp:8-13,
  * 238-268, 355-393, which replaces the reference's findMatchNodeInQuadTree
@@ -438,8 +442,8 @@ int gat_grid_select(const gat_tree_node* cells_dev, int32_t nx, int32_t ny, floa
  * leaf-major kernels then read leaf records in rank order. */
 int gat_morton_select(const uint32_t* cell_words_dev, int32_t total_level, int64_t nleaves, float xmin, float ymax,
                       float cell_size, const float* boxes_dev, const uint32_t* order_dev, int64_t q0, int64_t nq,
-                      uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, int64_t lq_capacity,
-                      uint64_t* lq_total_dev, uint32_t* status_dev, void* stream);
+                      uint32_t* node_qcount_dev, uint64_t* cand_dev, uint32_t* leaf_query_dev, uint32_t* lq_rank_dev,
+                      int64_t lq_capacity, uint64_t* lq_total_dev, uint32_t* status_dev, void* stream);
 
 /* The direct form for small batches: entry i of the (leaf, query) list becomes
  * tasks_dev[i] = {leaf start, leaf count, query} for gat_range_refine_pairs, so
@@ -447,15 +451,15 @@ int gat_morton_select(const uint32_t* cell_words_dev, int32_t total_level, int64
 int gat_leaf_tasks(const uint32_t* leaf_query_dev, const uint64_t* lq_total_dev, int64_t lq_capacity,
                    const gat_tree_node* nodes_dev, int64_t nnodes, gat_range_task* tasks_dev, void* stream);
 
-/* Groups the selected (leaf, query) entries by leaf.  List entry r of leaf L
- * (the rectangle's id, one uint32) goes to qlist[node_off[2*L] + r]; every
- * qb-th entry starts a work item (4 uint32: the leaf's store start and count,
- * the item's first list entry, its number of entries <= qb) at
- * items[4 * (node_off[2*L+1] + r/qb)].  node_fill (nnodes) must start at zero;
- * entries beyond lq_capacity and ids >= nqueries are ignored; qb is 1..64. */
+/* Groups the selected (leaf, query) entries by leaf.  Entry i, of leaf L and
+ * rank r = lq_rank[i] (from the selection), puts the rectangle's id (one
+ * uint32) at qlist[node_off[2*L] + r]; every qb-th rank starts a work item
+ * (4 uint32: the leaf's store start and count, the item's first list entry,
+ * its number of entries <= qb) at items[4 * (node_off[2*L+1] + r/qb)].
+ * Entries beyond lq_capacity and ids >= nqueries are ignored; qb is 1..64. */
 int gat_leaf_scatter(const uint32_t* leaf_query_dev, const uint64_t* lq_total_dev, int64_t lq_capacity,
                      const gat_tree_node* nodes_dev, int64_t nnodes, const uint32_t* node_off_dev,
-                     const uint32_t* node_qcount_dev, uint32_t* node_fill_dev, int32_t qb, int64_t nqueries,
+                     const uint32_t* node_qcount_dev, const uint32_t* lq_rank_dev, int32_t qb, int64_t nqueries,
                      uint32_t* qlist_dev, int64_t qlist_capacity, uint32_t* items_dev, int64_t items_capacity,
                      void* stream);
 
