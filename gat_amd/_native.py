@@ -85,7 +85,7 @@ HIP_FUNCS = {
     "gat_scan_tiles": (I64, [I64]),
     "gat_scan_u64": (INT, [P, I64, P, P, P, P]),
     "gat_scan_leaf_lists": (INT, [P, I64, I32, P, P, P, P]),
-    "gat_tree_select": (INT, [P, I64, I32, P, P, I64, I64, P, P, P, P, I64, P, P, P]),
+    "gat_tree_select": (INT, [P, I64, I32, P, P, P, I64, I64, P, P, P, P, I64, P, P, P]),
     "gat_leaf_tasks": (INT, [P, P, I64, P, I64, P, P]),
     "gat_grid_select": (INT, [P, I32, I32, F32, F32, F32, P, P, I64, I64, P, P, P, P, I64, P, P]),
     "gat_morton_select": (INT, [P, I32, I64, F32, F32, F32, P, P, I64, I64, P, P, P, P, I64, P, P, P]),
@@ -138,6 +138,7 @@ HOST_FUNCS = {
     "gath_index_range_pairs": (INT, [P, P, INT, P, P]),
     "gath_index_range_batched": (INT, [P, P, INT, INT, P, P]),
     "gath_index_tree_nodes": (I64, [P, P, I64, P]),
+    "gath_index_start_table": (I64, [P, P, I64, P, P]),
     "gath_index_fetch_store": (INT, [P, P, P, P, P]),
     "gath_index_cell_starts": (I64, [P, P, I64]),
     "gath_index_set_batch_params": (INT, [P, INT, INT]),

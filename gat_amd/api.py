@@ -451,6 +451,20 @@ class _Index:
         N.host().gath_index_tree_nodes(self.handle, out.ctypes.data, n, _ptr(depth))
         return out[:n], int(depth[0])
 
+    def start_table(self) -> tuple[np.ndarray, int, np.ndarray]:
+        """The start table a GT-quadtree gives its device walk: per block of
+        the table's level (Morton order) the owning node as (level << 27) |
+        breadth-first index, the level, and (xmin, ymax, block side); an empty
+        table (level 0) when the index has none."""
+        level = np.zeros(1, np.int32)
+        geom = np.zeros(3, np.float32)
+        n = int(N.host().gath_index_start_table(self.handle, None, 0, _ptr(level), _ptr(geom)))
+        if n < 0:
+            N.check_host(n)
+        out = np.zeros(max(n, 1), np.uint32)
+        N.host().gath_index_start_table(self.handle, out.ctypes.data, n, _ptr(level), _ptr(geom))
+        return out[:n], int(level[0]), geom
+
 
 class GridIndex(_Index):
     """GT-quadtree (Grid.cpp): Morton-ordered cells, quadtree leaves < max_points_in_node."""

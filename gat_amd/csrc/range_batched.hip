@@ -16,6 +16,8 @@
 //                        with ballots and written as (tid, pid) pairs.
 // A point of a leaf selected by r rectangles is read from HBM once instead of
 // r times, which is what takes the batch below the per-query scan's traffic.
+#include <cmath>
+
 #include "gat_common.hpp"
 
 namespace {
@@ -119,13 +121,10 @@ struct LeafEmitter {
   __device__ void flush() {
     if (n == 0) return;
     __syncthreads();  // the buffer writes of every lane are visible
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
-    base = uniform64(base);
     if (node_qcount && rank) {
       // The count atomics return each entry's rank among its leaf's entries,
-      // which places it in the leaf's list without another atomic.  All of
-      // the buffer's atomics are issued before the first result is used, so
+      // which places it in the leaf's list without another atomic.  They and
+      // the list atomic are all issued before the first result is used, so
       // the flush waits on one round trip.
       constexpr uint32_t kPer = kEmitCap / kWave;
       uint32_t r[kPer];
@@ -134,6 +133,9 @@ struct LeafEmitter {
         const uint32_t e = lane + k * kWave;
         r[k] = e < n ? atomicAdd(node_qcount + buf[e], 1u) : 0u;
       }
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
+      base = uniform64(base);
 #pragma unroll
       for (uint32_t k = 0; k < kPer; ++k) {
         const uint32_t e = lane + k * kWave;
@@ -144,6 +146,9 @@ struct LeafEmitter {
         }
       }
     } else {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
+      base = uniform64(base);
       for (uint32_t e = lane; e < n; e += kWave) {
         const uint64_t at = base + e;
         const uint32_t leaf = buf[e];
@@ -170,9 +175,44 @@ struct LeafEmitter {
 // bottom of the stack to its top, so each depth keeps at most nchild * 64
 // entries and a tree of depth D needs (D + 1) * nchild * 64 slots; the step
 // size is also cut so a push can never pass the end.
+//
+// With a start table (quadtrees, gat_tree_start) the walk skips the levels
+// above the table: the candidate blocks (those within one block of the
+// rectangle's extent) are taken 64 at a time, each lane loading its block's
+// owner and testing it once (an owner covering several blocks is taken at the
+// first of them inside the candidate range); selected leaf owners are emitted
+// and selected inner owners pushed, at most 64 entries of one depth, and the
+// walk runs until the stack is empty before the next 64 blocks.  On 1M
+// trajectories / 10k rectangles a level-7 table takes the walk from 125k to
+// 67k steps.
+struct StartTable {
+  const uint32_t* owner;
+  uint32_t level;
+  float xmin, ymax, block;
+  const uint2* tasks;  // (query, chunk | last << 31) per walk task, or null
+  const uint32_t* ntasks;  // tasks, and the order entries they stand for
+  uint32_t tasks_cap;
+};
+
+// Column (or row) bits of a Morton index: v's bits go to the even positions.
+__device__ __forceinline__ uint32_t morton_spread(uint32_t v) {
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// Block of a coordinate offset v (in blocks), moved by d and clamped to 0..nb-1.
+__device__ __forceinline__ uint32_t block_of(float v, int d, uint32_t nb) {
+  const float t = floorf(v) + float(d);
+  return uint32_t(fminf(fmaxf(t, 0.0f), float(nb - 1)));
+}
+
 template <int NCH>
 __global__ __launch_bounds__(kWave) void tree_select_kernel(
-    const Node* __restrict__ nodes, uint32_t nnodes, const float4* __restrict__ boxes,
+    const Node* __restrict__ nodes, uint32_t nnodes, StartTable st, const float4* __restrict__ boxes,
     const uint32_t* __restrict__ order, uint32_t q0, uint32_t nq, uint32_t* __restrict__ node_qcount,
     uint64_t* __restrict__ cand, uint2* __restrict__ lq, uint32_t* __restrict__ lq_rank, uint64_t lq_cap,
     unsigned long long* __restrict__ lq_total,
@@ -180,13 +220,44 @@ __global__ __launch_bounds__(kWave) void tree_select_kernel(
   __shared__ uint32_t stack[GAT_TREE_STACK];
   __shared__ uint32_t emitted[kEmitCap];
   const uint32_t lane = threadIdx.x;
-  for (uint32_t i = blockIdx.x; i < nq; i += gridDim.x) {
-    const uint32_t q = order ? uniform(order[i]) : q0 + i;
-    if (q - q0 >= nq) continue;  // an order entry outside the batch (uniform)
+  const uint32_t ntask = st.tasks ? min(st.ntasks[0], st.tasks_cap) : 0u;
+  const uint32_t nskip = st.tasks ? min(st.ntasks[1], nq) : 0u;
+  const uint32_t nwork = ntask + (nq - nskip);
+  for (uint32_t i = blockIdx.x; i < nwork; i += gridDim.x) {
+    uint32_t q, chunk = 0;
+    bool last = true;
+    if (i < ntask) {
+      const uint2 t = st.tasks[i];
+      q = uniform(t.x);
+      chunk = uniform(t.y & 0x7FFFFFFFu);
+      last = (uniform(t.y) >> 31) != 0u;
+    } else {
+      const uint32_t j = nskip + (i - ntask);
+      q = order ? uniform(order[j]) : q0 + j;
+    }
+    if (q - q0 >= nq) continue;  // an entry outside the batch (uniform)
     const float4 box = boxes[q];
     LeafEmitter out{emitted, 0, q, lane, lq, lq_cap, lq_total, node_qcount, lq_rank, 0};
-    uint32_t top = 0;
-    {
+    // NaN and inverted rectangles take the walk from the root, in their
+    // chunk-0 task.
+    const bool tabled = st.owner != nullptr && box.x <= box.z && box.y <= box.w;
+    if (!tabled && chunk != 0) continue;
+    uint32_t top = 0, c0 = 0, r0 = 0, bw = 1, ncand = 0, tfirst = 0, tend = 0;
+    if (tabled) {
+      const uint32_t nb = 1u << st.level;
+      const float inv = 1.0f / st.block;
+      c0 = block_of((box.x - st.xmin) * inv, -1, nb);
+      const uint32_t c1 = block_of((box.z - st.xmin) * inv, 1, nb);
+      r0 = block_of((st.ymax - box.w) * inv, -1, nb);
+      const uint32_t r1 = block_of((st.ymax - box.y) * inv, 1, nb);
+      bw = c1 - c0 + 1;
+      ncand = bw * (r1 - r0 + 1);
+      // This task's blocks: its chunk of 64, or every block from it on.
+      const uint32_t first = chunk < ncand / kWave + 1u ? chunk * kWave : ncand;
+      const uint32_t end = last ? ncand : min(ncand, first + kWave);
+      tfirst = first;
+      tend = end;
+    } else {
       const Node root = nodes[0];
       const bool sel = node_selected(root, box);
       if (root.first_child < 0) {
@@ -197,57 +268,110 @@ __global__ __launch_bounds__(kWave) void tree_select_kernel(
       }
     }
     __syncthreads();
-    while (top > 0) {
-      uint32_t m = top < kWave ? top : kWave;
-      const uint32_t room = (GAT_TREE_STACK - top) / (NCH - 1);
-      if (m > room) m = room;
-      if (m == 0) {
-        if (lane == 0) atomicOr(status, GAT_STATUS_STACK);
-        break;
-      }
-      top -= m;
-      const bool live = lane < m;
-      const uint32_t fc = live ? stack[top + lane] : 0u;
-      __syncthreads();  // every lane has read its entry before pushes reuse the slots
-      uint32_t grand[NCH];
-      bool inner[NCH];
-      uint32_t below = 0, pushed = 0;
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const uint32_t idx = fc + uint32_t(c);
-        bool emit = false;
-        uint32_t cnt = 0;
-        inner[c] = false;
-        grand[c] = 0;
-        if (live && idx < nnodes) {
-          const Node n = nodes[idx];
-          if (node_selected(n, box)) {
-            if (n.first_child < 0) {
-              emit = n.count > 0;
-              cnt = n.count;
-            } else if (uint32_t(n.first_child) + NCH <= nnodes) {
-              inner[c] = true;
-              grand[c] = uint32_t(n.first_child);
+    bool overflow = false;
+    for (uint32_t tb = tfirst;; tb += kWave) {
+      if (tabled) {
+        if (tb >= tend || overflow) break;
+        const uint32_t b = tb + lane;
+        bool emit = false, push = false;
+        uint32_t o = 0, cnt = 0, first = 0;
+        if (b < tend) {
+          const uint32_t r = r0 + b / bw, c = c0 + b % bw;
+          const uint32_t w = st.owner[morton_spread(c) | (morton_spread(r) << 1)];
+          o = w & ((1u << 27) - 1u);
+          const uint32_t lv = w >> 27;
+          const uint32_t span = lv <= st.level ? 1u << (st.level - lv) : 1u;
+          const uint32_t fr = max(r & ~(span - 1u), r0), fcol = max(c & ~(span - 1u), c0);
+          if (fr == r && fcol == c && o < nnodes) {
+            const Node n = nodes[o];
+            if (node_selected(n, box)) {
+              if (n.first_child < 0) {
+                emit = n.count > 0;
+                cnt = n.count;
+              } else if (uint32_t(n.first_child) + NCH <= nnodes) {
+                push = true;
+                first = uint32_t(n.first_child);
+              }
             }
           }
         }
-        out.push(emit, idx, cnt);
+        out.push(emit, o, cnt);
         out.maybe_flush();
-        const uint64_t dm = __ballot(inner[c]);
-        below += lanes_below(dm);
-        pushed += uint32_t(__popcll(dm));
+        const uint64_t pm = __ballot(push);
+        if (push) stack[lanes_below(pm)] = first;  // the stack is empty here
+        top = uint32_t(__popcll(pm));
+        __syncthreads();
       }
-      // Lane-major order: a lane's selected inner children follow every lower lane's.
-      uint32_t at = top + below;
+      while (top > 0) {
+        uint32_t m = top < kWave ? top : kWave;
+        const uint32_t room = (GAT_TREE_STACK - top) / (NCH - 1);
+        if (m > room) m = room;
+        if (m == 0) {
+          if (lane == 0) atomicOr(status, GAT_STATUS_STACK);
+          overflow = true;
+          break;
+        }
+        top -= m;
+        const bool live = lane < m;
+        const uint32_t fc = live ? stack[top + lane] : 0u;
+        __syncthreads();  // every lane has read its entry before pushes reuse the slots
+        // Every child's record is loaded before the first push: a push may
+        // flush (barrier, atomics, stores), which the later loads would
+        // otherwise wait behind, one memory round trip per child.
+        Node kid[NCH];
 #pragma unroll
-      for (int c = 0; c < NCH; ++c)
-        if (inner[c]) stack[at++] = grand[c];
-      top += pushed;
-      __syncthreads();
+        for (int c = 0; c < NCH; ++c) {
+          const uint32_t idx = fc + uint32_t(c);
+          kid[c].first_child = -1;
+          kid[c].flags = 0;
+          if (live && idx < nnodes) kid[c] = nodes[idx];
+        }
+        uint32_t grand[NCH];
+        bool inner[NCH];
+        uint32_t below = 0, pushed = 0;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          const uint32_t idx = fc + uint32_t(c);
+          bool emit = false;
+          uint32_t cnt = 0;
+          inner[c] = false;
+          grand[c] = 0;
+          if (live && idx < nnodes) {
+            const Node& n = kid[c];
+            if (node_selected(n, box)) {
+              if (n.first_child < 0) {
+                emit = n.count > 0;
+                cnt = n.count;
+              } else if (uint32_t(n.first_child) + NCH <= nnodes) {
+                inner[c] = true;
+                grand[c] = uint32_t(n.first_child);
+              }
+            }
+          }
+          out.push(emit, idx, cnt);
+          out.maybe_flush();
+          const uint64_t dm = __ballot(inner[c]);
+          below += lanes_below(dm);
+          pushed += uint32_t(__popcll(dm));
+        }
+        // Lane-major order: a lane's selected inner children follow every lower lane's.
+        uint32_t at = top + below;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+          if (inner[c]) stack[at++] = grand[c];
+        top += pushed;
+        __syncthreads();
+      }
+      if (!tabled) break;
     }
     out.flush();
     const uint64_t total = out.wave_cand();
-    if (lane == 0) cand[q] = total;
+    if (lane == 0) {
+      if (st.tasks)
+        atomicAdd(reinterpret_cast<unsigned long long*>(cand + q), (unsigned long long)total);
+      else
+        cand[q] = total;
+    }
   }
 }
 
@@ -280,17 +404,19 @@ __global__ __launch_bounds__(kWave) void grid_select_kernel(
     if (g1 <= g2 && g3 <= g4) {
       const uint32_t w = uint32_t(g2 - g1 + 1);
       const uint32_t ncell = w * uint32_t(g4 - g3 + 1);  // <= nx * ny <= 2^30
+      // The next step's counts load before this step's push (which may
+      // flush: barrier, atomics, stores, that a later load would wait behind).
+      auto cell_of = [&](uint32_t c) { return uint32_t(g3 + int(c / w)) * uint32_t(nx) + uint32_t(g1 + int(c % w)); };
+      uint32_t idx = lane < ncell ? cell_of(lane) : 0u;
+      uint32_t cnt = lane < ncell ? cells[idx].count : 0u;
       for (uint32_t base = 0; base < ncell; base += kWave) {
-        const uint32_t c = base + lane;
-        bool emit = false;
-        uint32_t idx = 0, cnt = 0;
-        if (c < ncell) {
-          idx = uint32_t(g3 + int(c / w)) * uint32_t(nx) + uint32_t(g1 + int(c % w));
-          cnt = cells[idx].count;
-          emit = cnt > 0;
-        }
-        out.push(emit, idx, cnt);
+        const uint32_t c = base + lane, cn = c + kWave;
+        const uint32_t nidx = cn < ncell ? cell_of(cn) : 0u;
+        const uint32_t ncnt = cn < ncell ? cells[nidx].count : 0u;
+        out.push(c < ncell && cnt > 0, idx, cnt);
         out.maybe_flush();
+        idx = nidx;
+        cnt = ncnt;
       }
     }
     out.flush();
@@ -387,6 +513,23 @@ __global__ __launch_bounds__(kWave) void morton_select_kernel(
       const uint32_t first = e >> 4, cl = (e & 15u) + 1u;  // children's level
       const bool deeper = live && cl <= L;
       const uint32_t shift = deeper ? 2u * (L - cl) : 0u;
+      // Every selected child's words are loaded before the first push (a
+      // push may flush: barrier, atomics, stores, which later loads would
+      // wait behind).
+      bool hit[NCH];
+      uint2 cw[NCH];
+      uint32_t cend[NCH];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const uint32_t fk = first + (uint32_t(c) << shift);
+        hit[c] = deeper && box_intersect(box, geo.block_box(fk, 1u << shift));
+        cw[c] = make_uint2(0xFFFFFFFFu, 0u);
+        cend[c] = 0;
+        if (hit[c]) {
+          cw[c] = cells[fk];
+          cend[c] = cells[fk + (1u << shift)].y;
+        }
+      }
       uint32_t grand[NCH];
       bool inner[NCH];
       uint32_t below = 0, pushed = 0;
@@ -397,9 +540,9 @@ __global__ __launch_bounds__(kWave) void morton_select_kernel(
         uint32_t cnt = 0, rank = 0;
         inner[c] = false;
         grand[c] = 0;
-        if (deeper && box_intersect(box, geo.block_box(fk, 1u << shift))) {
-          const uint2 w = cells[fk];
-          const uint32_t end = cells[fk + (1u << shift)].y;
+        if (hit[c]) {
+          const uint2 w = cw[c];
+          const uint32_t end = cend[c];
           if ((w.x & 15u) == cl) {
             rank = w.x >> 4;
             if (rank < nleaves) {
@@ -1231,17 +1374,37 @@ int gat_scan_leaf_lists(const uint32_t* count, int64_t nnodes, int32_t qb, uint3
                      reinterpret_cast<V2*>(node_off), reinterpret_cast<V2*>(totals), gat::as_stream(stream));
 }
 
-int gat_tree_select(const gat_tree_node* nodes, int64_t nnodes, int32_t nchild, const float* boxes,
-                    const uint32_t* order, int64_t q0, int64_t nq, uint32_t* node_qcount, uint64_t* cand,
-                    uint32_t* leaf_query, uint32_t* lq_rank, int64_t lq_capacity, uint64_t* lq_total,
-                    uint32_t* status, void* stream) {
+int gat_tree_select(const gat_tree_node* nodes, int64_t nnodes, int32_t nchild, const gat_tree_start* start,
+                    const float* boxes, const uint32_t* order, int64_t q0, int64_t nq, uint32_t* node_qcount,
+                    uint64_t* cand, uint32_t* leaf_query, uint32_t* lq_rank, int64_t lq_capacity,
+                    uint64_t* lq_total, uint32_t* status, void* stream) {
   GAT_REQUIRE(nnodes >= 1 && nnodes < int64_t(INT32_MAX) && nodes, "gat_tree_select: bad tree");
   GAT_REQUIRE(nchild == 2 || nchild == 4, "gat_tree_select: nchild must be 2 or 4");
+  StartTable st{nullptr, 0, 0.0f, 0.0f, 1.0f, nullptr, nullptr, 0};
+  int64_t nwork = nq;
+  if (start) {
+    GAT_REQUIRE(nchild == 4 && start->owner_dev && start->level >= 1 && start->level <= 15 &&
+                    start->block > 0.0f && std::isfinite(start->block) && std::isfinite(start->xmin) &&
+                    std::isfinite(start->ymax),
+                "gat_tree_select: bad start table");
+    GAT_REQUIRE(!start->tasks_dev || (start->ntasks_dev && start->tasks_capacity >= 0 &&
+                                      start->tasks_capacity <= int64_t(UINT32_MAX) - nq),
+                "gat_tree_select: bad walk tasks");
+    st = StartTable{start->owner_dev,
+                    uint32_t(start->level),
+                    start->xmin,
+                    start->ymax,
+                    start->block,
+                    reinterpret_cast<const uint2*>(start->tasks_dev),
+                    start->ntasks_dev,
+                    uint32_t(start->tasks_capacity)};
+    if (start->tasks_dev) nwork = nq + start->tasks_capacity;
+  }
   GAT_REQUIRE(q0 >= 0 && nq >= 0 && q0 + nq <= int64_t(UINT32_MAX), "gat_tree_select: bad query range");
   GAT_REQUIRE(lq_capacity >= 0 && lq_total && status, "gat_tree_select: bad outputs");
-  if (nq == 0) return GAT_OK;
+  if (nq == 0 || nwork == 0) return GAT_OK;
   GAT_REQUIRE(boxes && cand && (lq_capacity == 0 || leaf_query), "gat_tree_select: null pointer");
-  const int blocks = grid_for(uint64_t(nq), 1, 40);
+  const int blocks = grid_for(uint64_t(nwork), 1, 40);
   const dim3 grid{uint32_t(blocks)}, block{kWave};
   hipStream_t s = gat::as_stream(stream);
   const auto* N = reinterpret_cast<const Node*>(nodes);
@@ -1249,10 +1412,10 @@ int gat_tree_select(const gat_tree_node* nodes, int64_t nnodes, int32_t nchild, 
   auto* L = reinterpret_cast<uint2*>(leaf_query);
   auto* T = reinterpret_cast<unsigned long long*>(lq_total);
   if (nchild == 4)
-    tree_select_kernel<4><<<grid, block, 0, s>>>(N, uint32_t(nnodes), B, order, uint32_t(q0), uint32_t(nq),
+    tree_select_kernel<4><<<grid, block, 0, s>>>(N, uint32_t(nnodes), st, B, order, uint32_t(q0), uint32_t(nq),
                                                  node_qcount, cand, L, lq_rank, uint64_t(lq_capacity), T, status);
   else
-    tree_select_kernel<2><<<grid, block, 0, s>>>(N, uint32_t(nnodes), B, order, uint32_t(q0), uint32_t(nq),
+    tree_select_kernel<2><<<grid, block, 0, s>>>(N, uint32_t(nnodes), st, B, order, uint32_t(q0), uint32_t(nq),
                                                  node_qcount, cand, L, lq_rank, uint64_t(lq_capacity), T, status);
   GAT_LAUNCH_CHECK("tree_select_kernel");
   return GAT_OK;

@@ -80,6 +80,13 @@ class Grid {
   // The quadtree as breadth-first gat_tree_nodes (the root untested, every
   // other node tested with BoxIntersect != 0); *depth = deepest level.
   virtual std::vector<gat_tree_node> treeNodes(int* depth) const;
+  // The owner of every level-k block for the device walk's start table
+  // (gat_tree_start): (level << 27) | breadth-first index; empty when the
+  // tree does not qualify.
+  std::vector<uint32_t> startOwners(int k) const;
+  // The start table the device tree gets for a tree of this depth: its
+  // owners, *level and *block (the blocks' side); empty (level 0) when none.
+  std::vector<uint32_t> startTable(int depth, int* level, float* block) const;
   // Number of quadtree leaves.
   virtual int64_t leafNum() const;
   // Grid.cpp:428-446: the leaves one rectangle selects, in traversal order.

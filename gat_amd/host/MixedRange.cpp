@@ -110,7 +110,7 @@ int MixedRange::run(const MBB* bounds, int n, const MixedConfig& cfg, MixedResul
       // Replay: stage every index's rectangles, launch the one graph, check capacities.
       int rc = GAT_OK;
       for (int i = 0; i < 3 && rc == GAT_OK; ++i)
-        if (nq[i] > 0) rc = engine(i).stageBatch(boxes[i].data(), nq[i]);
+        if (nq[i] > 0) rc = engine(i).stageBatch(tree(i), boxes[i].data(), nq[i]);
       for (int i = 0; i < 3 && rc == GAT_OK; ++i)
         if (nq[i] > 0) rc = gat_stream_sync(engine(i).stream());
       // A workspace that moved since the capture: the graph's kernels would

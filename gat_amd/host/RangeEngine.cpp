@@ -1,6 +1,7 @@
 #include "RangeEngine.h"
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <thread>
@@ -459,6 +460,31 @@ int DeviceTree::upload(const std::vector<gat_tree_node>& host, int nc, int d, vo
   nnodes = int64_t(host.size());
   nchild = nc;
   depth = d;
+  startLevel = 0;  // a start table belongs to one tree
+  ++generation;
+  return GAT_OK;
+}
+
+int DeviceTree::uploadStart(const std::vector<uint32_t>& owner, int level, float x0, float y1, float block,
+                            void* stream) {
+  if (nnodes == 0 || nchild != 4 || level < 1 || level > 15 || owner.size() != (size_t(1) << (2 * level)) ||
+      !(block > 0.0f)) {
+    gath_set_error("device tree: bad start table (level %d, %zu owners)", level, owner.size());
+    return GAT_ERR_ARG;
+  }
+  for (uint32_t o : owner)
+    if ((o & ((1u << 27) - 1)) >= uint64_t(nnodes) || int(o >> 27) > level) {
+      gath_set_error("device tree: start table names a node outside the tree");
+      return GAT_ERR_ARG;
+    }
+  if (int rc = startOwner.reserve(owner.size() * 4)) return rc;
+  GATH_TRY(gat_memcpy_h2d(startOwner.get(), owner.data(), owner.size() * 4, stream), "start table upload");
+  GATH_TRY(gat_stream_sync(stream), "start table upload");
+  startLevel = level;
+  xmin = x0;
+  ymax = y1;
+  startBlock = block;
+  ++generation;
   return GAT_OK;
 }
 
@@ -479,6 +505,7 @@ int DeviceTree::uploadGrid(const std::vector<gat_tree_node>& cells, int gx, int 
   xmin = x0;
   ymax = y1;
   cellSize = cs;
+  ++generation;
   return GAT_OK;
 }
 
@@ -508,6 +535,7 @@ int DeviceTree::uploadImplicit(const std::vector<gat_tree_node>& leaves, const s
   xmin = x0;
   ymax = y1;
   cellSize = cs;
+  ++generation;
   return GAT_OK;
 }
 
@@ -525,8 +553,8 @@ int RangeEngine::reserveBatched(const DeviceTree& tree, int maxQueries, int64_t 
   const int64_t N = tree.nnodes;
   const int64_t nq = std::max(maxQueries, 1);
   dropGraph();
-  if (int rc = devBoxes_.reserve(align_up(size_t(nq) * 16, 256) + size_t(nq) * 4)) return rc;
-  if (int rc = hostStage_.reserve(align_up(size_t(nq) * 16, 256) + size_t(nq) * 4)) return rc;
+  if (int rc = devBoxes_.reserve(boxLayout(int(nq)).bytes)) return rc;
+  if (int rc = hostStage_.reserve(boxLayout(int(nq)).bytes)) return rc;
   if (int rc = nodeCount_.reserve(size_t(N) * 4)) return rc;
   if (int rc = nodeOff_.reserve(size_t(N) * 8)) return rc;
   if (int rc = tiles_.reserve(size_t(std::max(2 * gat_scan_tiles(N) * 4, gat_scan_tiles(nq) * 8)))) return rc;
@@ -628,6 +656,7 @@ RangeEngine::GraphKey RangeEngine::layoutKey(const DeviceTree& tree, int nq, con
                                     itemTiles_.get(), counters_.get(),      store_.xy_dev,   store_.id_dev};
   std::copy(p, p + GraphKey::kPtrs, key.ptr);
   key.runs = runIndexed;
+  key.treeGeneration = tree.generation;
   key.nq = nq;
   key.lqCap = lqCap_;
   key.qlistCap = qlistCap_;
@@ -644,12 +673,41 @@ void RangeEngine::dropGraph() {
   graphKey_ = GraphKey{};
 }
 
+// Layout of the batch's rectangle buffer: the boxes, the order the selection
+// takes them in, and the walk tasks of the largest rectangles of a tree with a
+// start table (their count and the order entries they cover, then the tasks).
+RangeEngine::BoxLayout RangeEngine::boxLayout(int nq) {
+  BoxLayout l;
+  l.tasksCap = 1024 + size_t(nq) / 2;
+  l.order = align_up(size_t(nq) * 16, 256);
+  l.ntasks = align_up(l.order + size_t(nq) * 4, 256);
+  l.tasks = l.ntasks + 8;
+  l.bytes = l.tasks + l.tasksCap * 8;
+  return l;
+}
+
+// Block of a coordinate offset in blocks, moved by d and clamped: the float
+// steps the device's tree walk takes.
+static uint32_t start_block(float v, int d, uint32_t nb) {
+  const float t = std::floor(v) + float(d);
+  return uint32_t(std::min(std::max(t, 0.0f), float(nb - 1)));
+}
+
+static float clamped_area(const MBB& b) {
+  return std::max(0.0f, b.xmax - b.xmin) * std::max(0.0f, b.ymax - b.ymin);
+}
+
 // Uploads the rectangles and the order the selection takes them in: largest
 // area first, so the longest traversals start first and the tail is short.
-int RangeEngine::uploadBoxes(const MBB* bounds, int nq) {
+// With a start table, each of the largest rectangles whose candidate blocks
+// span several 64-block chunks gets one walk task per chunk (while the task
+// list has room), so its walk runs on several waves; its last task also takes
+// any blocks past its chunks.  Rectangles under four blocks of area are not
+// looked at (a thin one among them keeps one task: slower, never wrong).
+int RangeEngine::uploadBoxes(const DeviceTree* tree, const MBB* bounds, int nq) {
   static_assert(sizeof(MBB) == 16, "MBB must be four packed floats");
-  const size_t box_bytes = align_up(size_t(nq) * 16, 256);
-  const size_t bytes = box_bytes + size_t(nq) * 4;
+  const BoxLayout lay = boxLayout(nq);
+  const size_t bytes = lay.bytes;
   if (int rc = hostStage_.reserve(std::max<size_t>(bytes, 256))) return rc;
   if (devBoxes_.size() < bytes) dropGraph();  // the captured pipeline reads the old buffer
   if (int rc = devBoxes_.reserve(std::max<size_t>(bytes, 256))) return rc;
@@ -657,20 +715,51 @@ int RangeEngine::uploadBoxes(const MBB* bounds, int nq) {
   std::memcpy(h, bounds, size_t(nq) * 16);
   // Counting sort on the binary exponent of the area, largest first: an order
   // within a factor of two is all the scheduling needs, in O(n).
-  auto* order = reinterpret_cast<uint32_t*>(h + box_bytes);
+  auto* order = reinterpret_cast<uint32_t*>(h + lay.order);
   std::vector<uint8_t> key(static_cast<size_t>(nq));
   uint32_t start[257] = {};
   for (int q = 0; q < nq; ++q) {
-    const float area =
-        std::max(0.0f, bounds[q].xmax - bounds[q].xmin) * std::max(0.0f, bounds[q].ymax - bounds[q].ymin);
+    const float area = clamped_area(bounds[q]);
     uint32_t bits;
     std::memcpy(&bits, &area, 4);
-    key[q] = uint8_t(255 - std::min<uint32_t>(bits >> 23, 255));  // NaN areas sort last
+    key[q] = uint8_t(255 - std::min<uint32_t>(bits >> 23, 255));  // NaN areas are 0: last
     ++start[key[q] + 1];
   }
   for (int b = 0; b < 256; ++b) start[b + 1] += start[b];
   for (int q = 0; q < nq; ++q) order[start[key[q]]++] = uint32_t(q);
-  GATH_TRY(gat_memcpy_h2d(devBoxes_.get(), h, bytes, stream_.get()), "box upload");
+  size_t upload = lay.order + size_t(nq) * 4;
+  if (tree && tree->startLevel > 0) {
+    auto* counts = reinterpret_cast<uint32_t*>(h + lay.ntasks);
+    auto* tasks = reinterpret_cast<uint32_t*>(h + lay.tasks);
+    const uint32_t nb = 1u << tree->startLevel;
+    const float inv = 1.0f / tree->startBlock;
+    const float small = 4.0f * tree->startBlock * tree->startBlock;
+    size_t n = 0;
+    int i = 0;
+    for (; i < nq; ++i) {
+      const uint32_t q = order[i];
+      const MBB& b = bounds[q];
+      if (clamped_area(b) < small) break;  // order is by area: every later one is under 8 blocks
+      size_t nch = 1;
+      if (b.xmin <= b.xmax && b.ymin <= b.ymax) {
+        const uint32_t c0 = start_block((b.xmin - tree->xmin) * inv, -1, nb);
+        const uint32_t c1 = start_block((b.xmax - tree->xmin) * inv, 1, nb);
+        const uint32_t r0 = start_block((tree->ymax - b.ymax) * inv, -1, nb);
+        const uint32_t r1 = start_block((tree->ymax - b.ymin) * inv, 1, nb);
+        const size_t ncand = size_t(c1 - c0 + 1) * size_t(r1 - r0 + 1);
+        nch = (ncand + 63) / 64;
+      }
+      if (n + nch > lay.tasksCap) break;
+      for (size_t c = 0; c < nch; ++c, ++n) {
+        tasks[2 * n] = q;
+        tasks[2 * n + 1] = uint32_t(c) | (c + 1 == nch ? 0x80000000u : 0u);
+      }
+    }
+    counts[0] = uint32_t(n);  // tasks
+    counts[1] = uint32_t(i);  // the order entries they cover
+    upload = lay.tasks + n * 8;
+  }
+  GATH_TRY(gat_memcpy_h2d(devBoxes_.get(), h, upload, stream_.get()), "box upload");
   return GAT_OK;
 }
 
@@ -692,7 +781,20 @@ int RangeEngine::enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nod
                                ctr + 4, s),
              "implicit tree select");
   } else {
-    GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), tree.nnodes, tree.nchild, devBoxes_.as<float>(), order,
+    const BoxLayout lay = boxLayout(nq);
+    char* d = devBoxes_.as<char>();
+    const gat_tree_start start{tree.startOwner.as<uint32_t>(),
+                               tree.startLevel,
+                               tree.xmin,
+                               tree.ymax,
+                               tree.startBlock,
+                               reinterpret_cast<const uint32_t*>(d + lay.tasks),
+                               reinterpret_cast<const uint32_t*>(d + lay.ntasks),
+                               int64_t(lay.tasksCap)};
+    // Several tasks add to one rectangle's candidate count.
+    if (tree.startLevel > 0) GATH_TRY(gat_fill_u32(cand_.as<uint32_t>(), 0, int64_t(nq) * 2, s), "batch reset");
+    GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), tree.nnodes, tree.nchild,
+                             tree.startLevel > 0 ? &start : nullptr, devBoxes_.as<float>(), order,
                              0, nq, nodeCount, cand_.as<uint64_t>(), lq_.as<uint32_t>(),
                              nodeCount ? lqRank_.as<uint32_t>() : nullptr, lqCap_, total, ctr + 4, s),
              "tree select");
@@ -808,13 +910,12 @@ int RangeEngine::finishCounters(RangeBatch& out, bool replay) {
   return GAT_OK;
 }
 
-int RangeEngine::stageBatch(const MBB* bounds, int nq) {
-  const size_t need = align_up(size_t(nq) * 16, 256) + size_t(nq) * 4;
-  if (nq < 1 || !bounds || devBoxes_.size() < need) {
+int RangeEngine::stageBatch(const DeviceTree& tree, const MBB* bounds, int nq) {
+  if (nq < 1 || !bounds || devBoxes_.size() < boxLayout(nq).bytes) {
     gath_set_error("stageBatch: the workspace was not sized for %d rectangles", nq);
     return GAT_ERR_STATE;
   }
-  return uploadBoxes(bounds, nq);
+  return uploadBoxes(&tree, bounds, nq);
 }
 
 int RangeEngine::enqueueBatch(const DeviceTree& tree, int nq, RangeBatch& out) {
@@ -854,7 +955,7 @@ int RangeEngine::runTree(const DeviceTree& tree, const MBB* bounds, int nq, Rang
   out.kernelMs = out.refineMs = 0;
   if (nq == 0) return GAT_OK;
   const double t0 = now_ms();
-  if (int rc = uploadBoxes(bounds, nq)) return rc;
+  if (int rc = uploadBoxes(&tree, bounds, nq)) return rc;
   const int64_t N = tree.nnodes;
   if (nq <= directMaxQueries) {
     // Direct form: workspace sized by the batch only.

@@ -103,10 +103,20 @@ struct DeviceTree {
   float xmin = 0, ymax = 0, cellSize = 0;
   DeviceBuffer cellWords;  // implicit quadtree only
   int implicitLevel = 0;
+  // Start table of a quadtree walk (gat_tree_start): the owner of each block
+  // of level startLevel, blocks of side startBlock from (xmin, ymax).
+  DeviceBuffer startOwner;
+  int startLevel = 0;
+  float startBlock = 0;
+  // Bumped by every upload: a captured pipeline bakes the tree's sizes and
+  // geometry into its launches and is replayed only for the same generation.
+  uint64_t generation = 0;
   bool flat() const { return nx > 0; }
   bool implicit() const { return implicitLevel > 0; }
   // Checks the traversal stack bound and uploads `host`.
   int upload(const std::vector<gat_tree_node>& host, int nchild, int depth, void* stream);
+  // Uploads the start table (4^level owners) of the tree upload() holds.
+  int uploadStart(const std::vector<uint32_t>& owner, int level, float xmin, float ymax, float block, void* stream);
   // Uploads one leaf record per cell of an nx x ny grid.
   int uploadGrid(const std::vector<gat_tree_node>& cells, int nx, int ny, float xmin, float ymax, float cellSize,
                  void* stream);
@@ -182,7 +192,7 @@ class RangeEngine {
   // whole device pipeline on this engine's stream (no host sync, capturable),
   // finishBatch reads the counters back and reports in *fits whether the batch
   // stayed inside the workspace (if not, rerun it with runTree).
-  int stageBatch(const MBB* bounds, int nq);
+  int stageBatch(const DeviceTree& tree, const MBB* bounds, int nq);
   int enqueueBatch(const DeviceTree& tree, int nq, RangeBatch& out);
   int finishBatch(int nq, RangeBatch& out, bool* fits);
 
@@ -195,13 +205,15 @@ class RangeEngine {
     static constexpr int kPtrs = 20;
     const void* ptr[kPtrs] = {};
     int64_t nq = -1, lqCap = 0, qlistCap = 0, itemsCap = 0, pairsCap = 0, runsCap = 0;
+    uint64_t treeGeneration = 0;
     int qb = 0;
     bool runs = false;
     bool operator==(const GraphKey& o) const {
       for (int i = 0; i < kPtrs; ++i)
         if (ptr[i] != o.ptr[i]) return false;
       return nq == o.nq && lqCap == o.lqCap && qlistCap == o.qlistCap && itemsCap == o.itemsCap &&
-             pairsCap == o.pairsCap && runsCap == o.runsCap && qb == o.qb && runs == o.runs;
+             pairsCap == o.pairsCap && runsCap == o.runsCap && qb == o.qb && runs == o.runs &&
+             treeGeneration == o.treeGeneration;
     }
     bool operator!=(const GraphKey& o) const { return !(*this == o); }
   };
@@ -222,7 +234,11 @@ class RangeEngine {
  private:
   int stage(const MBB* bounds, int nq, const std::vector<gat_range_task>& tasks, const uint64_t* qbase);
   int stageBuild(const TrajectoryView& db, int pidBits, DeviceBuffer& lon, DeviceBuffer& lat, DeviceBuffer& off);
-  int uploadBoxes(const MBB* bounds, int nq);
+  struct BoxLayout {
+    size_t order = 0, ntasks = 0, tasks = 0, bytes = 0, tasksCap = 0;
+  };
+  static BoxLayout boxLayout(int nq);
+  int uploadBoxes(const DeviceTree* tree, const MBB* bounds, int nq);
   int enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out);
   // The selection kernel for the tree's kind (nodeCount may be null).
   int enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nodeCount);

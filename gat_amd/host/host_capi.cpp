@@ -346,6 +346,26 @@ int64_t gath_index_tree_nodes(void* p, gat_tree_node* out, int64_t cap, int32_t*
   return int64_t(t.size());
 }
 
+int64_t gath_index_start_table(void* p, uint32_t* out, int64_t cap, int32_t* level, float* geometry) {
+  Index* ix = as_index(p);
+  if (!ix || !level || !geometry || (cap > 0 && !out)) return bad("gath_index_start_table: bad arguments");
+  *level = 0;
+  geometry[0] = geometry[1] = geometry[2] = 0.0f;
+  if (!ix->grid || dynamic_cast<MortonGrid*>(ix->grid.get())) return 0;  // explicit quadtree walks only
+  int depth = 0;
+  (void)ix->grid->treeNodes(&depth);
+  int k = 0;
+  float block = 0;
+  const std::vector<uint32_t> t = ix->grid->startTable(depth, &k, &block);
+  if (t.empty()) return 0;
+  *level = k;
+  geometry[0] = ix->grid->range.xmin;
+  geometry[1] = ix->grid->range.ymax;
+  geometry[2] = block;
+  for (int64_t i = 0; i < std::min<int64_t>(cap, int64_t(t.size())); ++i) out[i] = t[size_t(i)];
+  return int64_t(t.size());
+}
+
 // ------------------------------------------------------------ file formats
 
 namespace {

@@ -398,6 +398,36 @@ int gat_scan_u64(const uint64_t* in_dev, int64_t n, uint64_t* out_dev, uint64_t*
 int gat_scan_leaf_lists(const uint32_t* count_dev, int64_t nnodes, int32_t qb, uint32_t* node_off_dev,
                         uint32_t* totals_dev, uint32_t* tiles_dev, void* stream);
 
+/* Start table of a quadtree walk (no reference counterpart): owner_dev holds,
+ * for each of the 4^level blocks of one level of a quadtree over a 2^L x 2^L
+ * Morton grid (numbered in Morton order: column bits even, row bits odd, rows
+ * counted down from ymax), the node that owns it as (node level << 27) |
+ * node index: the block's own node at that level, or the leaf above it that
+ * covers it.  A rectangle's walk then starts at the owners of the blocks
+ * within one block of its extent (blocks of side `block` from (xmin, ymax)),
+ * each owner taken once and tested, instead of at the root.  Only for trees
+ * where every box holds its children's boxes and is wider than zero in x
+ * (BoxIntersect is then monotone along every root path, so a node is reached
+ * exactly when its own box intersects the rectangle); rectangles with NaN or
+ * inverted extents start at the root.
+ * Optional walk tasks for the first rectangles of order_dev (tasks_dev null:
+ * none): ntasks_dev[0] tasks (at most tasks_capacity) stand for the first
+ * ntasks_dev[1] entries of order_dev; task i = (query tasks_dev[2i], chunk
+ * tasks_dev[2i+1] & 0x7fffffff, bit 31 set on the query's last task) walks
+ * the query's candidate blocks [64 chunk, 64 chunk + 64) in row-major order,
+ * its last task every block from 64 chunk on; every other entry of order_dev
+ * is one task that walks all its blocks.  cand[q] is then accumulated over a
+ * query's tasks (it must start at zero).  A NaN or inverted rectangle's walk
+ * runs in its chunk-0 task. */
+typedef struct gat_tree_start {
+  const uint32_t* owner_dev;
+  int32_t level; /* 1..15 */
+  float xmin, ymax, block;
+  const uint32_t* tasks_dev;
+  const uint32_t* ntasks_dev; /* 2 entries */
+  int64_t tasks_capacity;
+} gat_tree_start;
+
 /* Selects, for rectangles q0 .. q0+nq-1 (or, when order_dev is not null, for
  * rectangles order_dev[0 .. nq-1], taken in that order — largest first
  * shortens the tail), the leaves they visit (one wave per rectangle).  Appends each (leaf, query) to leaf_query (2
@@ -408,8 +438,10 @@ int gat_scan_leaf_lists(const uint32_t* count_dev, int64_t nnodes, int32_t qb, u
  * lq_rank[i] = entry i's rank among its leaf's entries (the value
  * node_qcount[leaf] had before the entry's increment), which gat_leaf_scatter
  * uses as the entry's place in the leaf's list.  *lq_total and node_qcount
- * must start at zero; leaves with no points are skipped. */
-int gat_tree_select(const gat_tree_node* nodes_dev, int64_t nnodes, int32_t nchild, const float* boxes_dev,
+ * must start at zero; leaves with no points are skipped.  start (a host
+ * pointer, may be null; nchild 4 only) gives the walk a start table. */
+int gat_tree_select(const gat_tree_node* nodes_dev, int64_t nnodes, int32_t nchild, const gat_tree_start* start,
+                    const float* boxes_dev,
                     const uint32_t* order_dev, int64_t q0, int64_t nq, uint32_t* node_qcount_dev, uint64_t* cand_dev,
                     uint32_t* leaf_query_dev, uint32_t* lq_rank_dev, int64_t lq_capacity, uint64_t* lq_total_dev,
                     uint32_t* status_dev, void* stream);

@@ -149,6 +149,10 @@ int gath_system_similarity_test(void* grid, const float* qlon, const float* qlat
  * gat_tree_node; a MortonGrid's implicit tree gives its leaves in Morton
  * order); returns the node count, copies up to cap nodes. */
 int64_t gath_index_tree_nodes(void* index, gat_tree_node* out, int64_t cap, int32_t* depth);
+/* The start table a GT-quadtree index gives its device walk (gat_tree_start):
+ * writes up to cap owners, *level and geometry[0..2] = xmin, ymax, block side;
+ * returns the number of owners (4^level), 0 when the index has none. */
+int64_t gath_index_start_table(void* index, uint32_t* out, int64_t cap, int32_t* level, float* geometry);
 /* The resident point store: meta[0] = points, meta[1] = pid_bits (0: point ids
  * in `pid`); with xy non-null, copies xy (2 floats per point), id and, when
  * pid_bits is 0, pid to the host. */

@@ -233,6 +233,12 @@ def _wave_select(nodes, box, nchild, stack_cap=4096, wave=64):
             sel.append((int(root["start"]), int(root["count"])))
     elif _node_selected(root, box):
         stack.append(int(root["first_child"]))
+    return sel, _walk(nodes, box, nchild, stack, sel, stack_cap, wave)
+
+
+def _walk(nodes, box, nchild, stack, sel, stack_cap=4096, wave=64):
+    """The walk of tree_select_kernel from the given stack until it is empty;
+    appends selected leaf runs to sel and returns the largest stack size."""
     top_max = len(stack)
     while stack:
         m = min(len(stack), wave, (stack_cap - len(stack)) // (nchild - 1))
@@ -252,6 +258,58 @@ def _wave_select(nodes, box, nchild, stack_cap=4096, wave=64):
                     pushes.append(int(n["first_child"]))
         stack.extend(pushes)
         top_max = max(top_max, len(stack))
+    return top_max
+
+
+def _spread(v):
+    """Morton interleave: v's bits to the even positions."""
+    out = 0
+    for b in range(16):
+        out |= ((v >> b) & 1) << (2 * b)
+    return out
+
+
+def _table_select(nodes, table, level, geom, box, stack_cap=4096, wave=64):
+    """tree_select_kernel with a start table restated: the candidate blocks
+    within one block of the rectangle (float32 arithmetic as on the device),
+    64 at a time; each owner taken at its first block inside the range and
+    tested once; then the walk from the selected inner owners.  NaN and
+    inverted rectangles walk from the root."""
+    f32 = np.float32
+    box = np.asarray(box, np.float32)
+    if not (box[0] <= box[2] and box[1] <= box[3]):
+        return _wave_select(nodes, box, 4, stack_cap, wave)
+    x0, y1, block = (f32(v) for v in geom)
+    nb = 1 << level
+    inv = f32(1) / block
+
+    def block_of(v, d):
+        t = np.floor(f32(v)) + f32(d)
+        return int(min(max(t, f32(0)), f32(nb - 1)))
+
+    c0, c1 = block_of((box[0] - x0) * inv, -1), block_of((box[2] - x0) * inv, 1)
+    r0, r1 = block_of((y1 - box[3]) * inv, -1), block_of((y1 - box[1]) * inv, 1)
+    bw = c1 - c0 + 1
+    ncand = bw * (r1 - r0 + 1)
+    sel, top_max = [], 0
+    for tb in range(0, ncand, wave):
+        stack = []
+        for b in range(tb, min(tb + wave, ncand)):
+            r, c = r0 + b // bw, c0 + b % bw
+            w = int(table[_spread(c) | (_spread(r) << 1)])
+            o, lv = w & ((1 << 27) - 1), w >> 27
+            span = 1 << (level - lv)
+            if (max(r & ~(span - 1), r0), max(c & ~(span - 1), c0)) != (r, c):
+                continue
+            n = nodes[o]
+            if not _node_selected(n, box):
+                continue
+            if n["first_child"] < 0:
+                if n["count"] > 0:
+                    sel.append((int(n["start"]), int(n["count"])))
+            else:
+                stack.append(int(n["first_child"]))
+        top_max = max(top_max, _walk(nodes, box, 4, stack, sel, stack_cap, wave))
     return sel, top_max
 
 
@@ -286,6 +344,43 @@ def test_grid_device_tree_selects_oracle_leaves(cell, leaf):
         sel, top = _wave_select(nodes, r, 4)
         want = [[int(s), int(c)] for s, c in og.leaves(r) if c > 0]
         assert _runs(sel) == _runs(want)
+        assert top <= (depth + 1) * 4 * 64
+
+
+@pytest.mark.parametrize("cell,leaf", [(1 / 1024, 8), (0.001, 16), (0.001, 64)])
+def test_grid_start_table_selects_oracle_leaves(cell, leaf):
+    """The walk from the start table (the owners of one level's blocks) selects
+    exactly the leaves the walk from the root selects, which are the oracle's
+    (Grid.cpp:428-446), on rectangles from points to the whole domain, edge-
+    and block-aligned ones, and NaN and inverted ones (which start at the
+    root); every block has one owner at or above the table's level."""
+    db = W.CFG1.db(3000)
+    og = O.Grid(W.UNIT_BOX, cell, leaf).add(db)
+    g = api.GridIndex(W.UNIT_BOX, cell, leaf).build(db, upload=False)
+    nodes, depth = g.tree_nodes()
+    table, level, geom = g.start_table()
+    assert level == min(7, depth - 2) >= 2 and len(table) == 4 ** level
+    own, lv = table & ((1 << 27) - 1), table >> 27
+    assert (lv <= level).all() and (own < len(nodes)).all()
+    assert ((lv == level) | (nodes["first_child"][own] < 0)).all()
+    rng = np.random.default_rng(29)
+    nb = 1 << level
+    k = rng.integers(0, nb, size=(30, 2)).astype(np.float32)
+    bs, x0, y1 = geom[2], geom[0], geom[1]
+    aligned = np.stack([x0 + k[:, 0] * bs, y1 - (k[:, 1] + 3) * bs, x0 + (k[:, 0] + 3) * bs, y1 - k[:, 1] * bs], 1)
+    pts = rng.random((20, 2)).astype(np.float32)
+    rects = np.concatenate([
+        W.CFG1.rects()[:40], api.synth_rects(7, 60, 1e-4, 0.5, True), aligned,
+        np.concatenate([pts, pts], 1),
+        np.array([[0, 0, 1, 1], [-1, -1, 2, 2], [0.3, 0.3, 0.2, 0.4], [np.nan, 0, 1, 1],
+                  [0.5, 0.5, 0.5, 0.5], [-np.inf, -np.inf, np.inf, np.inf]], np.float32)]).astype(np.float32)
+    for r in rects:
+        sel, top = _table_select(nodes, table, level, geom, r)
+        root_sel, _ = _wave_select(nodes, r, 4)
+        assert _runs(sel) == _runs(root_sel), r
+        if np.isfinite(r).all():
+            want = [[int(s), int(c)] for s, c in og.leaves(r) if c > 0]
+            assert _runs(sel) == _runs(want), r
         assert top <= (depth + 1) * 4 * 64
 
 
