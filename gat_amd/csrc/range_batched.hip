@@ -1213,8 +1213,12 @@ __global__ __launch_bounds__(kRefBlock, 8) void leaf_refine_runs_kernel(
     const uint32_t list = uniform(next.list), nql = uniform(next.n);
     if (t + nwaves < nitems) next = items[t + nwaves];
     if (uint64_t(start) + count > store.npoints || count == 0 || nql == 0 || nql > kMaxQb ||
-        uint64_t(list) + nql > qlist_cap)
+        uint64_t(list) + nql > qlist_cap) {
+      // Its records read as empty to the per-query counts.
+      const uint32_t nrec = nql < uint32_t(kMaxQb) ? nql : uint32_t(kMaxQb);
+      if (lane < nrec && uint64_t(list) + lane < o.runs_cap) o.runs[uint64_t(list) + lane] = Run{0xFFFFFFFFu, 0u, 0u};
       continue;
+    }
     uint32_t qj = 0xFFFFFFFFu;
     float4 bj = make_float4(nan, nan, nan, nan);
     if (lane < nql) {
@@ -1258,45 +1262,53 @@ __global__ __launch_bounds__(kRefBlock, 8) void leaf_refine_runs_kernel(
   }
 }
 
-// qcount[q] += the hits of every run of query q.  Each workgroup takes a
-// contiguous range of work items (neighbouring leaves, so few distinct
-// rectangles), one thread per item walking its list entries' run records, and
-// with kLds sums into LDS counters for all nqueries rectangles before adding
-// the non-zero ones to qcount: adding every record to qcount directly put
-// 5.77M atomics on 10k counters (0.6 ms on 10k rectangles).  The records of
-// items the refine skipped are never read: the same checks skip them here.
+// qcount[q] += the hits of every run record of query q.  The records cover
+// the list entries [0, L) in order (L = the last work item's list end; the
+// refine writes empty records for items it skips), so each workgroup sums a
+// contiguous range of them, one 16-byte record per thread, into LDS counters
+// for all nqueries rectangles (neighbouring leaves share few rectangles) and
+// adds the non-zero ones to qcount; each wave adds its records' sum to
+// *pair_total.  Walking the records item by item, a thread per item, took 76
+// us on 1M trajectories / 10k rectangles; adding every record to qcount
+// directly put 5.77M atomics on 10k counters (0.6 ms).
 constexpr uint32_t kCountLdsQueries = 16384;  // 64 KB of LDS counters
 
 template <bool kLds>
 __global__ __launch_bounds__(256) void run_counts_kernel(const Item* __restrict__ items,
                                                          const uint32_t* __restrict__ item_total, uint64_t items_cap,
-                                                         uint64_t npoints, uint64_t list_cap,
-                                                         const Run* __restrict__ runs, uint32_t nqueries,
-                                                         uint32_t* __restrict__ qcount) {
+                                                         uint64_t list_cap, const Run* __restrict__ runs,
+                                                         uint32_t nqueries, uint32_t* __restrict__ qcount,
+                                                         unsigned long long* __restrict__ pair_total) {
   extern __shared__ uint32_t acc[];  // nqueries counters when kLds
   const uint64_t total = *item_total;
   const uint64_t nitems = total < items_cap ? total : items_cap;
+  uint64_t nrec = 0;
+  if (nitems > 0) {
+    const Item last = items[nitems - 1];
+    nrec = uint64_t(last.list) + last.n;
+    if (nrec > list_cap) nrec = list_cap;
+  }
   if constexpr (kLds) {
     for (uint32_t q = threadIdx.x; q < nqueries; q += blockDim.x) acc[q] = 0;
     __syncthreads();
   }
-  const uint64_t per = (nitems + gridDim.x - 1) / gridDim.x;
+  const uint64_t per = (nrec + gridDim.x - 1) / gridDim.x;
   const uint64_t lo = uint64_t(blockIdx.x) * per;
-  const uint64_t hi = lo + per < nitems ? lo + per : nitems;
-  for (uint64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-    const Item it = items[t];
-    if (uint64_t(it.start) + it.count > npoints || it.count == 0 || it.n == 0 || it.n > uint32_t(kMaxQb) ||
-        uint64_t(it.list) + it.n > list_cap)
-      continue;
-    for (uint32_t j = 0; j < it.n; ++j) {
-      const Run r = runs[uint64_t(it.list) + j];
-      if (r.count == 0 || r.q >= nqueries) continue;
-      if constexpr (kLds) {
-        atomicAdd(acc + r.q, r.count);
-      } else {
-        atomicAdd(qcount + r.q, r.count);
-      }
+  const uint64_t hi = lo + per < nrec ? lo + per : nrec;
+  uint64_t mine = 0;
+  for (uint64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+    const Run r = runs[e];
+    if (r.count == 0 || r.q >= nqueries) continue;
+    mine += r.count;
+    if constexpr (kLds) {
+      atomicAdd(acc + r.q, r.count);
+    } else {
+      atomicAdd(qcount + r.q, r.count);
     }
+  }
+  if (pair_total) {
+    for (int off = kWave / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+    if ((threadIdx.x & (kWave - 1)) == 0 && mine) atomicAdd(pair_total, (unsigned long long)mine);
   }
   if constexpr (kLds) {
     __syncthreads();
@@ -1600,20 +1612,16 @@ int gat_run_counts(const gat_point_store* sp, const uint32_t* items, const uint3
     const uint64_t list_cap = uint64_t(qlist_capacity < runs_capacity ? qlist_capacity : runs_capacity);
     auto* I = reinterpret_cast<const Item*>(items);
     const auto* R = reinterpret_cast<const Run*>(runs);
-    const dim3 cgrid{uint32_t(grid_for(uint64_t(items_capacity), 256, 2))};
+    auto* T = reinterpret_cast<unsigned long long*>(pair_total);
+    // Up to 2 workgroups per CU: the LDS counters take 40 KB on 10k rectangles.
+    const dim3 cgrid{uint32_t(grid_for(uint64_t(items_capacity) * 8, 256, 2))};
     if (uint64_t(nqueries) <= kCountLdsQueries)
       run_counts_kernel<true><<<cgrid, dim3{256}, size_t(nqueries) * 4, s>>>(
-          I, item_total, uint64_t(items_capacity), uint64_t(sp->npoints), list_cap, R, uint32_t(nqueries), qcount);
+          I, item_total, uint64_t(items_capacity), list_cap, R, uint32_t(nqueries), qcount, T);
     else
-      run_counts_kernel<false><<<cgrid, dim3{256}, 0, s>>>(I, item_total, uint64_t(items_capacity),
-                                                         uint64_t(sp->npoints), list_cap, R, uint32_t(nqueries),
-                                                         qcount);
+      run_counts_kernel<false><<<cgrid, dim3{256}, 0, s>>>(I, item_total, uint64_t(items_capacity), list_cap, R,
+                                                         uint32_t(nqueries), qcount, T);
     GAT_LAUNCH_CHECK("run_counts_kernel");
-  }
-  if (pair_total) {
-    sum_u32_kernel<<<dim3{uint32_t(grid_for(uint64_t(nqueries), 256, 1))}, dim3{256}, 0, s>>>(
-        qcount, uint64_t(nqueries), reinterpret_cast<unsigned long long*>(pair_total));
-    GAT_LAUNCH_CHECK("sum_u32_kernel");
   }
   return GAT_OK;
 }

@@ -516,13 +516,14 @@ typedef struct gat_range_run {
  * gat_leaf_refine_runs: an item's hits fill its region from the start,
  * rectangle after rectangle; its list entry e records the rectangle's run
  * (query, hits, first pair) in runs_dev[e] (count 0 when none; runs_capacity
- * must cover the list total).  boxes_dev: the batch's rectangles (4 floats
- * each) that the list's ids index.  No atomics.
+ * must cover the list total); an item it skips as malformed gets empty
+ * records.  boxes_dev: the batch's rectangles (4 floats each) that the list's
+ * ids index.  No atomics.
  *
  * gat_run_counts: qcount[q] = the pairs of query q summed from the run
- * records (qcount must start at zero); when pair_total_dev is not null, the
- * sum of qcount is added to it.  Items the refine skipped as malformed are
- * skipped here by the same checks. */
+ * records of list entries [0, L), L = the last work item's list end (items
+ * cover the list in order); qcount must start at zero; when pair_total_dev is
+ * not null, the sum of qcount is added to it. */
 int gat_leaf_item_regions(const uint32_t* items_dev, const uint32_t* item_total_dev, int64_t items_capacity,
                           uint64_t* item_base_dev, uint64_t* tiles_dev, uint64_t* bound_total_dev, void* stream);
 int gat_leaf_refine_runs(const gat_point_store* store, const uint32_t* items_dev, const uint32_t* item_total_dev,

@@ -306,11 +306,13 @@ def test_plain_after_batched_keeps_its_layout(cfg1):
 
 @pytest.mark.parametrize("nqueries", [3, 20000])
 def test_run_indexed_abi_skips_malformed_items(gpu, nqueries):
-    """The run-indexed join through the C ABI on hand-built work items, three
-    of them malformed (a run past the store, no rectangles, a list slice past
-    the list), which the refine must skip and the count pass must skip too:
-    their run records hold garbage that must not reach the counts.  Both count
-    paths: LDS-aggregated (3 rectangles) and direct (20,000)."""
+    """The run-indexed join through the C ABI on hand-built work items that
+    cover the list in order, three of them malformed (a run past the store, no
+    rectangles, a list slice past the list): the refine skips them and writes
+    empty run records for their entries inside the record array, where garbage
+    lay before, so the count pass over the list's records counts only the good
+    items.  Both count paths: LDS-aggregated (3 rectangles) and direct
+    (20,000)."""
     import ctypes
     from gat_amd import _native as N
     from tests.devbuf import DevBuf
@@ -326,8 +328,8 @@ def test_run_indexed_abi_skips_malformed_items(gpu, nqueries):
     qlist = np.array([0, 1, 2, 0, 1, 0], np.uint32)
     items = np.array([[0, 4, 0, 2],    # rectangles 0 and 1 over points 0..3
                       [4, 2, 2, 1],    # rectangle 2 over points 4..5
-                      [5, 4, 3, 1],    # run past the store
-                      [0, 2, 4, 0],    # no rectangles
+                      [5, 4, 3, 2],    # run past the store
+                      [0, 2, 5, 0],    # no rectangles
                       [0, 2, 5, 3]],   # list slice past the list
                      np.uint32)
     dboxes, dq, ditems = DevBuf.of(boxes), DevBuf.of(qlist), DevBuf.of(items)
@@ -344,7 +346,7 @@ def test_run_indexed_abi_skips_malformed_items(gpu, nqueries):
                                        runs.ptr, 6, base.ptr, None))
     got = runs.get(api.RUN, 6)
     assert [(int(r["q"]), int(r["count"]), int(r["first"])) for r in got[:3]] == [(0, 2, 0), (1, 2, 2), (2, 1, 16)]
-    assert (got["count"][3:] == 77).all()  # entries of skipped items are left alone
+    assert (got["count"][3:] == 0).all()  # entries of skipped items read as empty
     out = pairs.get(np.int32, 128).reshape(64, 2)
     assert set(map(tuple, out[0:2].tolist())) == {(1, 0), (1, 1)}
     assert set(map(tuple, out[2:4].tolist())) == {(2, 0), (3, 0)}
