@@ -1348,6 +1348,23 @@ __global__ void sum_u32_kernel(const uint32_t* __restrict__ in, uint64_t n, unsi
   if ((threadIdx.x & (kWave - 1)) == 0 && s) atomicAdd(out, (unsigned long long)s);
 }
 
+// *total = in[0] + ... + in[n-1], one workgroup.
+constexpr int kReduceBlock = 1024;
+__global__ __launch_bounds__(kReduceBlock) void reduce_u64_kernel(const uint64_t* __restrict__ in, uint64_t n,
+                                                                  uint64_t* __restrict__ total) {
+  __shared__ uint64_t part[kReduceBlock / kWave];
+  uint64_t s = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += kReduceBlock) s += in[i];
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < kReduceBlock / kWave; ++w) t += part[w];
+    *total = t;
+  }
+}
+
 int grid_for(uint64_t work, int per_block, int cap_per_cu) {
   uint64_t blocks = (work + per_block - 1) / per_block;
   const uint64_t cap = uint64_t(gat::cu_count()) * cap_per_cu;
@@ -1372,8 +1389,15 @@ int gat_fill_u32(uint32_t* dst, uint32_t value, int64_t n, void* stream) {
 int64_t gat_scan_tiles(int64_t n) { return n <= 0 ? 1 : (n + kScanTile - 1) / kScanTile; }
 
 int gat_scan_u64(const uint64_t* in, int64_t n, uint64_t* out, uint64_t* total, uint64_t* tiles, void* stream) {
-  GAT_REQUIRE(n >= 0 && total && tiles && (n == 0 || (in && out)), "gat_scan_u64: bad arguments");
+  GAT_REQUIRE(n >= 0 && total && (n == 0 || in), "gat_scan_u64: bad arguments");
   GAT_REQUIRE(uint64_t(n) / kScanTile < UINT32_MAX, "gat_scan_u64: too many elements");
+  if (!out) {
+    // The total alone: one workgroup sums the input, one launch instead of three.
+    reduce_u64_kernel<<<dim3{1}, dim3{kReduceBlock}, 0, gat::as_stream(stream)>>>(in, uint64_t(n), total);
+    GAT_LAUNCH_CHECK("reduce_u64_kernel");
+    return GAT_OK;
+  }
+  GAT_REQUIRE(tiles, "gat_scan_u64: bad arguments");
   return launch_scan(LoadU64{in}, uint64_t(n), tiles, out, total, gat::as_stream(stream));
 }
 

@@ -321,7 +321,11 @@ int RangeBatch::syncHost() {
   count.resize(size_t(queryNum));
   base.resize(size_t(queryNum));
   GATH_TRY(gat_memcpy_d2h(count.data(), qcount.get(), size_t(queryNum) * 4, stream), "count readback");
-  GATH_TRY(gat_memcpy_d2h(base.data(), qbase.get(), size_t(queryNum) * 8, stream), "region readback");
+  // A run-indexed batch has no per-query regions (its runs place the pairs).
+  if (runIndexed)
+    std::fill(base.begin(), base.end(), uint64_t(0));
+  else
+    GATH_TRY(gat_memcpy_d2h(base.data(), qbase.get(), size_t(queryNum) * 8, stream), "region readback");
   GATH_TRY(gat_stream_sync(stream), "count readback");
   hostStale = false;
   return GAT_OK;
@@ -810,8 +814,11 @@ int RangeEngine::enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out) 
   GATH_TRY(gat_fill_u32(out.qcount.as<uint32_t>(), 0, nq, s), "batch reset");
   GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
   if (int rc = enqueueTreeSelect(tree, nq, nodeCount_.as<uint32_t>())) return rc;
-  GATH_TRY(gat_scan_u64(cand_.as<uint64_t>(), nq, out.qbase.as<uint64_t>(), reinterpret_cast<uint64_t*>(ctr + 2),
-                        tiles_.as<uint64_t>(), s),
+  // The run-indexed join places its output by work item: it needs the
+  // candidate total but no per-query regions.  (The walk adding the total
+  // with one atomic per wave cost it 20 us: every wave on one address.)
+  GATH_TRY(gat_scan_u64(cand_.as<uint64_t>(), nq, runIndexed ? nullptr : out.qbase.as<uint64_t>(),
+                        reinterpret_cast<uint64_t*>(ctr + 2), tiles_.as<uint64_t>(), s),
            "region scan");
   GATH_TRY(gat_scan_leaf_lists(nodeCount_.as<uint32_t>(), N, queriesPerItem, nodeOff_.as<uint32_t>(), ctr + 5,
                                tiles_.as<uint32_t>(), s),

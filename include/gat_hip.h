@@ -387,7 +387,8 @@ int gat_fill_u32(uint32_t* dst_dev, uint32_t value, int64_t n, void* stream);
 int64_t gat_scan_tiles(int64_t n);
 
 /* Exclusive prefix sum: out[i] = in[0] + ... + in[i-1]; *total = sum of all.
- * tiles_dev holds gat_scan_tiles(n) entries of scratch.  out may alias in. */
+ * tiles_dev holds gat_scan_tiles(n) entries of scratch.  out may alias in.
+ * With out_dev null only *total is written (tiles_dev unused). */
 int gat_scan_u64(const uint64_t* in_dev, int64_t n, uint64_t* out_dev, uint64_t* total_dev, uint64_t* tiles_dev,
                  void* stream);
 
