@@ -76,7 +76,9 @@ HIP_FUNCS = {
     "gat_range_refine_pairs": (INT, [P, P, I64, P, I64, P, P, P, I64, P]),
     "gat_pairs_digest": (INT, [P, P, P, I64, I64, P, P]),
     "gatEDRDistanceBatchHandler": (INT, [INT, P, P, P, P, P, P, P, F64, P, P]),
-    "gat_edr_batched": (INT, [P, I64, P, I64, P, P, P, P, I32, P, P, P, I32, I32, F64, I32, I32, P, P]),
+    "gat_edr_batched": (INT, [P, I64, P, I64, P, P, P, P, P, I32, P, P, P, I32, I32, F64, I32, I32, P, P]),
+    "gat_edr_jobs": (INT, [P, P, I64, P, I32, P, I32, I32, I32, I32, P, P, P, P, P, P, P]),
+    "gat_edr_survivor_lists": (INT, [P, P, P, I64, I32, P, P, P, P, P]),
     "gat_edr_filter": (INT, [P, I32, P, I32, F64, P, I64, P, P]),
     "gat_edr_mismatch_table": (INT, [I32, P, P]),
     "gat_edr_pack_trajectories": (INT, [P, P, P, P, I32, I64, P, P, P]),
@@ -116,6 +118,7 @@ HOST_FUNCS = {
     "gath_grid_new": (P, [P, F32, INT]),
     "gath_morton_grid_new": (P, [P, F32, INT]),
     "gath_grid_set_edr_waves": (INT, [P, INT]),
+    "gath_grid_set_topk_device_chain": (INT, [P, INT]),
     "gath_system_range_test": (INT, [P, INT, P, INT, P]),
     "gath_system_similarity_test": (INT, [P, P, P, INT, INT, INT, F64, P, P, P, P, P]),
     "gath_fsg_new": (P, [P, F32]),

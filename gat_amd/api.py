@@ -500,6 +500,12 @@ class GridIndex(_Index):
         N.check_host(N.host().gath_grid_set_edr_waves(self.handle, int(waves_per_pair)))
         return self
 
+    def set_topk_device_chain(self, on: bool) -> "GridIndex":
+        """Top-k batches chained on the device (filter, jobs, DP, survivor
+        lists, merge; the default) or with jobs and lists built on the host."""
+        N.check_host(N.host().gath_grid_set_topk_device_chain(self.handle, int(bool(on))))
+        return self
+
     def edr_all(self, q_xy: np.ndarray, eps: float) -> np.ndarray:
         q = _req(q_xy, np.float32).reshape(-1, 2)
         lon, lat = np.ascontiguousarray(q[:, 0]), np.ascontiguousarray(q[:, 1])

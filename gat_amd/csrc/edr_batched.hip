@@ -66,6 +66,16 @@ __device__ __forceinline__ int shift_in(int old, int src) {
   return __builtin_amdgcn_update_dpp(old, src, SEG == 16 ? 0x111 : 0x138, 0xf, 0xf, false);
 }
 
+// A launch over a window of the job array: jobs [window[0], window[0] +
+// window[1]) of the njobs the array holds, the window written on the device
+// (by edr_jobs_kernel) after the launch was queued.
+__device__ __forceinline__ int32_t take_window(const int32_t* window, int32_t& njobs) {
+  if (!window) return 0;
+  const int32_t first = min(max(window[0], 0), njobs);
+  njobs = min(njobs - first, max(window[1], 0));
+  return first;
+}
+
 // SEG lanes per pair: 16 (four pairs per wave) or 64 (one pair per wave, a
 // quarter of the strips, for the longest pairs, whose strips would otherwise
 // run one after another for milliseconds).  Waves per SIMD each class can hold:
@@ -74,6 +84,7 @@ __device__ __forceinline__ int shift_in(int old, int src) {
 template <int MAXM, int SEG>
 __global__ __launch_bounds__(64, (MAXM >= 2048 && SEG == 16) ? 2 : 4) void edr_wave_kernel(const int2* __restrict__ pairs, int64_t npairs,
                                                       const Job* __restrict__ jobs, int32_t njobs,
+                                                      const int32_t* __restrict__ window,
                                                       int32_t* __restrict__ counter, int32_t nqueries, int32_t ncands,
                                                       const float2* __restrict__ qxy, const int64_t* __restrict__ qoff,
                                                       const int32_t* __restrict__ qlen, const float2* __restrict__ cxy,
@@ -94,6 +105,7 @@ __global__ __launch_bounds__(64, (MAXM >= 2048 && SEG == 16) ? 2 : 4) void edr_w
   const int seg = lane / kSeg;
   const int l = lane & (kSeg - 1);
   const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+  jobs += take_window(window, njobs);
   for (;;) {
     int ji = 0;
     if (lane == 0) ji = atomicAdd(counter, 1);
@@ -264,6 +276,7 @@ constexpr int kLag = 6;
 template <int MAXM, int W>
 __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict__ pairs, int64_t npairs,
                                                           const Job* __restrict__ jobs, int32_t njobs,
+                                                          const int32_t* __restrict__ window,
                                                           int32_t* __restrict__ counter, int32_t nqueries,
                                                           int32_t ncands, const float2* __restrict__ qxy,
                                                           const int64_t* __restrict__ qoff,
@@ -281,6 +294,7 @@ __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict
   const int wv = threadIdx.x / 64;
   const int l = threadIdx.x & 63;
   const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+  jobs += take_window(window, njobs);
   for (;;) {
     if (threadIdx.x == 0) job_slot = atomicAdd(counter, 1);
     __syncthreads();
@@ -762,6 +776,260 @@ __global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
   }
 }
 
+// ---------------------------------------------------------------- device jobs
+// The batch's job list built on the device from the filter's survivors, so a
+// top-k batch runs filter -> jobs -> DP -> survivor lists -> merge with no host
+// round trip.  One workgroup: per pair the shorter and longer length, the
+// length bucket (runBatched's rules: short up to `split`, long up to `mid`,
+// the rest one pair per wave, and in a batch of one round of waves the pairs
+// that would outlast the longest pair's several-wave schedule on one wave
+// take several), then a counting sort in LDS by (bucket, strips descending,
+// longer length descending in 128-point steps) and jobs of up to four pairs of
+// one strip count (one pair in the long buckets).  Pairs of equal sort key
+// keep no particular order: it only changes which pairs share a job.
+constexpr int kJobThreads = 1024;
+constexpr int kStripsMax = GAT_MAX_TRAJ_LEN / 16 + 1;  // strips_of(n) is 0..128
+constexpr int kJobGroups = 4 * kStripsMax;             // (bucket, strips)
+constexpr int kLenSteps = 16;                          // longer length in 128-point steps
+constexpr int kSortBins = kJobGroups * kLenSteps;
+constexpr int kSortPer = (kSortBins + kJobThreads - 1) / kJobThreads;
+
+// Exclusive scan of a[0..n) in LDS by the whole workgroup (n <= per *
+// kJobThreads); returns the total.  red holds kJobThreads / 64 + 1 words.
+template <int PER>
+__device__ uint32_t block_exclusive_scan(uint32_t* a, int n, uint32_t* red) {
+  const int t = threadIdx.x;
+  uint32_t v[PER], sum = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = t * PER + k;
+    v[k] = i < n ? a[i] : 0u;
+    sum += v[k];
+  }
+  uint32_t incl = sum;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = uint32_t(__shfl_up(int(incl), d, 64));
+    if ((t & 63) >= d) incl += y;
+  }
+  if ((t & 63) == 63) red[t / 64] = incl;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < kJobThreads / 64; ++w) {
+      const uint32_t x = red[w];
+      red[w] = run;
+      run += x;
+    }
+    red[kJobThreads / 64] = run;
+  }
+  __syncthreads();
+  uint32_t base = red[t / 64] + incl - sum;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = t * PER + k;
+    if (i < n) a[i] = base;
+    base += v[k];
+  }
+  const uint32_t total = red[kJobThreads / 64];
+  __syncthreads();
+  return total;
+}
+
+// Several-wave schedule of a pair of s 64-row strips over a row of b blocks
+// (runBatched's pipe_blocks: a lag of kLag blocks between strips).
+__device__ __forceinline__ int64_t pipe_blocks(int64_t s, int64_t b, int waves) {
+  const int64_t P = b > int64_t(waves) * kLag ? b : int64_t(waves) * kLag;
+  return ((s - 1) / waves) * P + ((s - 1) % waves) * kLag + b;
+}
+
+struct JobRules {
+  int32_t split, waves, every_long, cus;
+};
+
+__global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __restrict__ pairs,
+                                                               const unsigned long long* __restrict__ count,
+                                                               uint32_t cap, const int32_t* __restrict__ qlen,
+                                                               int32_t nq, const int32_t* __restrict__ clen,
+                                                               int32_t nc, JobRules rules, int2* __restrict__ sorted,
+                                                               int2* __restrict__ nm, Job* __restrict__ jobs,
+                                                               int32_t* __restrict__ windows,
+                                                               int32_t* __restrict__ status,
+                                                               unsigned long long* __restrict__ cells) {
+  __shared__ uint32_t bins[kSortBins];
+  __shared__ uint32_t gjobs[kJobGroups + 1];
+  __shared__ uint32_t red[kJobThreads / 64 + 1];
+  __shared__ unsigned long long red64[kJobThreads / 64];
+  const int t = threadIdx.x;
+  const unsigned long long total = *count;
+  if (total > cap) {  // uniform: nothing to run; the caller grows the buffers
+    if (t < 8) windows[t] = 0;
+    if (t == 0) atomicOr(status, 1);
+    return;
+  }
+  const uint32_t np = uint32_t(total);
+  // Lengths, the cell count and the several-wave threshold.
+  const bool latency_bound = (int64_t(np) + 3) / 4 < int64_t(rules.cus) * 16;
+  const int mid = latency_bound ? rules.split : max(rules.split, 1024);
+  const bool multi = latency_bound && rules.waves > 1;
+  unsigned long long my_cells = 0;
+  int64_t my_crit = 0;
+  for (uint32_t i = t; i < np; i += kJobThreads) {
+    const int2 p = pairs[i];
+    const bool ok = p.x >= 0 && p.x < nq && p.y >= 0 && p.y < nc;
+    const int a = ok ? qlen[p.x] : 0, b = ok ? clen[p.y] : 0;
+    const int n = min(a, b), m = max(a, b);
+    nm[i] = make_int2(n, m);
+    my_cells += (unsigned long long)(n) * (unsigned long long)(m);
+    if (multi && m > mid) {
+      const int64_t c = pipe_blocks((n + 63) / 64, (m + 63 + 15) / 16, rules.waves);
+      if (c > my_crit) my_crit = c;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    my_cells += __shfl_xor(my_cells, off);
+    const int64_t o = __shfl_xor(my_crit, off);
+    if (o > my_crit) my_crit = o;
+  }
+  __shared__ int64_t crit_w[kJobThreads / 64];
+  if ((t & 63) == 0) {
+    red64[t / 64] = my_cells;
+    crit_w[t / 64] = my_crit;
+  }
+  for (int i = t; i < kSortBins; i += kJobThreads) bins[i] = 0;
+  __syncthreads();
+  int64_t critical = 0;
+  for (int w = 0; w < kJobThreads / 64; ++w) critical = crit_w[w] > critical ? crit_w[w] : critical;
+  if (t == 0) {
+    unsigned long long c = 0;
+    for (int w = 0; w < kJobThreads / 64; ++w) c += red64[w];
+    *cells = c;
+  }
+  auto key_of = [&](int2 v) {
+    const int n = v.x, m = v.y;
+    int bucket = m <= rules.split ? 0 : m <= mid ? 1 : 2;
+    if (bucket == 2 && multi &&
+        (rules.every_long || int64_t((n + 63) / 64) * ((m + 63 + 15) / 16) > critical))
+      bucket = 3;
+    const int strips = (n + 15) / 16;
+    const int len = m >= 1 ? min((m - 1) / 128, kLenSteps - 1) : 0;
+    return uint32_t((bucket * kStripsMax + (kStripsMax - 1 - strips)) * kLenSteps + (kLenSteps - 1 - len));
+  };
+  for (uint32_t i = t; i < np; i += kJobThreads) atomicAdd(&bins[key_of(nm[i])], 1u);
+  __syncthreads();
+  block_exclusive_scan<kSortPer>(bins, kSortBins, red);
+  for (uint32_t i = t; i < np; i += kJobThreads) {
+    const uint32_t pos = atomicAdd(&bins[key_of(nm[i])], 1u);
+    sorted[pos] = pairs[i];
+  }
+  __syncthreads();
+  // bins[b] now ends bin b: group g spans [end of bin g*16 - 1, end of bin g*16 + 15).
+  auto group_start = [&](int g) { return g == 0 ? 0u : bins[g * kLenSteps - 1]; };
+  auto per_job = [](int g) { return g / kStripsMax >= 2 ? 1u : 4u; };
+  for (int g = t; g < kJobGroups; g += kJobThreads) {
+    const uint32_t len = bins[g * kLenSteps + kLenSteps - 1] - group_start(g);
+    gjobs[g] = (len + per_job(g) - 1) / per_job(g);
+  }
+  __syncthreads();
+  const uint32_t njobs = block_exclusive_scan<1>(gjobs, kJobGroups, red);
+  if (t == 0) gjobs[kJobGroups] = njobs;
+  __syncthreads();
+  for (uint32_t j = t; j < njobs; j += kJobThreads) {
+    int lo = 0, hi = kJobGroups - 1;  // the last group whose first job is at or below j
+    while (lo < hi) {
+      const int mid_g = (lo + hi + 1) / 2;
+      if (gjobs[mid_g] <= j) lo = mid_g; else hi = mid_g - 1;
+    }
+    const uint32_t g = uint32_t(lo), k = j - gjobs[g], per = per_job(int(g));
+    const uint32_t first = group_start(int(g)) + k * per;
+    const uint32_t end = bins[g * kLenSteps + kLenSteps - 1];
+    jobs[j] = Job{int32_t(first), int32_t(min(per, end - first))};
+  }
+  if (t < 4) {
+    windows[2 * t] = int32_t(gjobs[t * kStripsMax]);
+    windows[2 * t + 1] = int32_t(gjobs[(t + 1) * kStripsMax] - gjobs[t * kStripsMax]);
+  }
+}
+
+// Per-query survivor lists for edr_topk_kernel from the DP's sorted pairs and
+// distances: counts in LDS, their scan as the start offsets, then each pair
+// to its query's next slot (in no order; edr_lists_sort_kernel sorts by id).
+constexpr int kListQueries = 8192;
+
+__global__ __launch_bounds__(kJobThreads) void edr_lists_kernel(const int2* __restrict__ sorted,
+                                                                const int32_t* __restrict__ dist,
+                                                                const unsigned long long* __restrict__ count,
+                                                                uint32_t cap, int32_t nq, int32_t* __restrict__ start,
+                                                                int32_t* __restrict__ ids,
+                                                                int32_t* __restrict__ dists) {
+  __shared__ uint32_t cnt[kListQueries];
+  __shared__ uint32_t red[kJobThreads / 64 + 1];
+  const int t = threadIdx.x;
+  const unsigned long long total = *count;
+  const uint32_t np = total > cap ? 0u : uint32_t(total);
+  for (int q = t; q < nq; q += kJobThreads) cnt[q] = 0;
+  __syncthreads();
+  for (uint32_t i = t; i < np; i += kJobThreads) {
+    const int q = sorted[i].x;
+    if (q >= 0 && q < nq) atomicAdd(&cnt[q], 1u);
+  }
+  __syncthreads();
+  const uint32_t n = block_exclusive_scan<kListQueries / kJobThreads>(cnt, nq, red);
+  for (int q = t; q < nq; q += kJobThreads) start[q] = int32_t(cnt[q]);
+  if (t == 0) start[nq] = int32_t(n);
+  __syncthreads();
+  for (uint32_t i = t; i < np; i += kJobThreads) {
+    const int2 p = sorted[i];
+    if (p.x < 0 || p.x >= nq) continue;
+    const uint32_t pos = atomicAdd(&cnt[p.x], 1u);
+    ids[pos] = p.y;
+    dists[pos] = dist[i];
+  }
+}
+
+// Each query's survivors sorted by id (ids are distinct within a query): a
+// bitonic sort in LDS of up to kSortMax (id, distance) pairs per workgroup;
+// a query with more sets status bit 2 and keeps its list unsorted.
+constexpr int kSortMax = 2048;
+constexpr int kSortThreads = 256;
+
+__global__ __launch_bounds__(kSortThreads) void edr_lists_sort_kernel(const int32_t* __restrict__ start,
+                                                                      int32_t* __restrict__ ids,
+                                                                      int32_t* __restrict__ dists,
+                                                                      int32_t* __restrict__ status) {
+  __shared__ unsigned long long v[kSortMax];
+  const int q = blockIdx.x, t = threadIdx.x;
+  const int s0 = start[q], m = start[q + 1] - s0;
+  if (m <= 1) return;
+  if (m > kSortMax) {
+    if (t == 0) atomicOr(status, 2);
+    return;
+  }
+  int P = 2;
+  while (P < m) P <<= 1;
+  for (int i = t; i < P; i += kSortThreads)
+    v[i] = i < m ? ((unsigned long long)(uint32_t(ids[s0 + i])) << 32) | uint32_t(dists[s0 + i]) : ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P; i += kSortThreads) {
+        const int o = i ^ j;
+        if (o > i) {
+          const bool up = (i & k) == 0;
+          const unsigned long long x = v[i], y = v[o];
+          if ((x > y) == up) {
+            v[i] = y;
+            v[o] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = t; i < m; i += kSortThreads) {
+    ids[s0 + i] = int32_t(v[i] >> 32);
+    dists[s0 + i] = int32_t(uint32_t(v[i]));
+  }
+}
+
 }  // namespace
 
 extern "C" int gat_edr_filter(const float* qbox, int32_t nq, const float* cbox, int32_t nc, double eps,
@@ -863,7 +1131,7 @@ extern "C" int gat_edr_mismatch_table(int32_t maxlen, int32_t* out, void* stream
 }
 
 extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_edr_job* jobs, int64_t njobs,
-                               int32_t* counter, const float* qxy, const int64_t* qoff, const int32_t* qlen,
+                               const int32_t* window, int32_t* counter, const float* qxy, const int64_t* qoff, const int32_t* qlen,
                                int32_t nqueries, const float* cxy, const int64_t* coff, const int32_t* clen,
                                int32_t ncands, int32_t maxlen, double eps, int32_t per_cu, int32_t lanes_per_pair,
                                int32_t* out, void* stream) {
@@ -900,8 +1168,8 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
   if (waves > 1) {
     const dim3 wblock{uint32_t(64 * waves)};
 #define GAT_EDR_PIPE(M, W)                                                                                       \
-  edr_pipe_kernel<M, W><<<grid, wblock, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff, \
-                                                qlen, Cs, coff, clen, u, out)
+  edr_pipe_kernel<M, W><<<grid, wblock, 0, s>>>(P, npairs, J, int32_t(njobs), window, counter, nqueries, ncands, Q, \
+                                                qoff, qlen, Cs, coff, clen, u, out)
     if (cls == 1024) {
       if (waves == 2) GAT_EDR_PIPE(1024, 2);
       else if (waves == 4) GAT_EDR_PIPE(1024, 4);
@@ -916,8 +1184,8 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
     return GAT_OK;
   }
 #define GAT_EDR_LAUNCH(M, SEG)                                                                                  \
-  edr_wave_kernel<M, SEG><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), counter, nqueries, ncands, Q, qoff, \
-                                                 qlen, Cs, coff, clen, u, out)
+  edr_wave_kernel<M, SEG><<<grid, block, 0, s>>>(P, npairs, J, int32_t(njobs), window, counter, nqueries, ncands, Q, \
+                                                 qoff, qlen, Cs, coff, clen, u, out)
   if (wide) {
     if (cls == 1024)
       GAT_EDR_LAUNCH(1024, 64);
@@ -933,5 +1201,41 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
   }
 #undef GAT_EDR_LAUNCH
   GAT_LAUNCH_CHECK("edr_wave_kernel");
+  return GAT_OK;
+}
+
+extern "C" int gat_edr_jobs(const int32_t* pairs, const uint64_t* count, int64_t capacity, const int32_t* qlen,
+                            int32_t nq, const int32_t* clen, int32_t nc, int32_t split, int32_t waves,
+                            int32_t every_long, int32_t* sorted, int32_t* scratch, gat_edr_job* jobs,
+                            int32_t* windows, int32_t* status, uint64_t* cells, void* stream) {
+  GAT_REQUIRE(capacity >= 0 && capacity <= int64_t(INT32_MAX) - 8 && nq >= 0 && nc >= 0,
+              "EDR jobs: bad sizes");
+  GAT_REQUIRE(split >= 1 && split <= GAT_MAX_TRAJ_LEN && waves >= 1 && waves <= 8, "EDR jobs: bad rules");
+  GAT_REQUIRE(pairs && count && qlen && clen && sorted && scratch && jobs && windows && status && cells,
+              "EDR jobs: null pointer");
+  if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 1, stream)) return rc;
+  const JobRules rules{split, waves, every_long ? 1 : 0, int32_t(gat::cu_count())};
+  edr_jobs_kernel<<<dim3(1), dim3(kJobThreads), 0, gat::as_stream(stream)>>>(
+      reinterpret_cast<const int2*>(pairs), reinterpret_cast<const unsigned long long*>(count), uint32_t(capacity),
+      qlen, nq, clen, nc, rules, reinterpret_cast<int2*>(sorted), reinterpret_cast<int2*>(scratch),
+      reinterpret_cast<Job*>(jobs), windows, status, reinterpret_cast<unsigned long long*>(cells));
+  GAT_LAUNCH_CHECK("edr_jobs_kernel");
+  return GAT_OK;
+}
+
+extern "C" int gat_edr_survivor_lists(const int32_t* sorted, const int32_t* dist, const uint64_t* count,
+                                      int64_t capacity, int32_t nq, int32_t* start, int32_t* ids, int32_t* dists,
+                                      int32_t* status, void* stream) {
+  GAT_REQUIRE(capacity >= 0 && capacity <= int64_t(INT32_MAX) && nq >= 0 && nq <= kListQueries,
+              "EDR survivor lists: at most %d queries", kListQueries);
+  GAT_REQUIRE(sorted && dist && count && start && ids && dists && status, "EDR survivor lists: null pointer");
+  if (nq == 0) return GAT_OK;
+  hipStream_t s = gat::as_stream(stream);
+  edr_lists_kernel<<<dim3(1), dim3(kJobThreads), 0, s>>>(reinterpret_cast<const int2*>(sorted), dist,
+                                                          reinterpret_cast<const unsigned long long*>(count),
+                                                          uint32_t(capacity), nq, start, ids, dists);
+  GAT_LAUNCH_CHECK("edr_lists_kernel");
+  edr_lists_sort_kernel<<<dim3(uint32_t(nq)), dim3(kSortThreads), 0, s>>>(start, ids, dists, status);
+  GAT_LAUNCH_CHECK("edr_lists_sort_kernel");
   return GAT_OK;
 }

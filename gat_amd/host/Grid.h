@@ -133,6 +133,10 @@ class Grid {
   // take them, unless edrWavesEveryLongPair is set.
   int edrWavesPerPair = 4;
   bool edrWavesEveryLongPair = false;
+  // Top-k batches of k <= 1024 and at most 8192 queries run filter, jobs, DP,
+  // survivor lists and merge on the device with no host step between them;
+  // off: the jobs and lists are built on the host from the survivors.
+  bool deviceChain = true;
 
   // Batched EDR of given (query index, trajectory index) pairs, both 0-based:
   // dist[i] for pairs[2i], pairs[2i + 1].
@@ -188,6 +192,12 @@ class Grid {
   static constexpr int kDeviceTopkMax = 1024;
   int deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_t>& dist, int nq, int kValue,
                  std::vector<TopK>& out);
+  // The whole top-k batch on the device (filter, jobs, DP, survivor lists,
+  // merge) with one readback at the end; *done is false when the batch must
+  // take the host-built path instead (a query with more survivors than the
+  // device's per-query sort holds).
+  int similarityOnDevice(int nq, int kValue, std::vector<TopK>& out, EdrBatchStats* st, bool* done);
+  static constexpr int kDeviceListQueries = 8192;
   // Trajectory-ordered copy of the points for EDR (the reference reassembles
   // trajectories from the cell-ordered store per query, Grid.cpp:1500-1508).
   // Stores are NaN-padded (GAT_EDR_PAD points before every trajectory,
@@ -202,7 +212,11 @@ class Grid {
   // ascending) with per-length offsets; per batch, each query's survivors
   // ascending by id with their distances, and the results.
   DeviceBuffer lenIdsDev_, lenOffDev_, survDev_, topDev_;
-  EventTimer edrTimer_;
+  // The device-built job list: pairs grouped by bucket, scratch, bucket
+  // windows + status + cell count, and the DP's job counters.
+  DeviceBuffer sortedDev_, nmDev_, windowsDev_, dpCounterDev_;
+  int64_t deviceTopkCap_ = 0;  // survivor capacity of the device path, grown on overflow
+  EventTimer edrTimer_, filterTimer_, mergeTimer_;
   SideStream edrSide_;  // the longest EDR bucket runs here, beside the others
   SideStream edrSide2_;  // one-wave long pairs of a batch whose longest run over several waves
   std::vector<int32_t> trajLenHost_, queryLenHost_;

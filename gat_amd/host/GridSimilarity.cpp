@@ -402,7 +402,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
                           bucketJobs[0] + bucketJobs[1] + bucketJobs[2]};
   auto launch = [&](int bucket, void* stream) -> int {
     GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>() + j0[bucket],
-                             bucketJobs[bucket], counterDev_.as<int32_t>() + bucket, queryXY_.as<float>(),
+                             bucketJobs[bucket], nullptr, counterDev_.as<int32_t>() + bucket, queryXY_.as<float>(),
                              queryOff64_.as<int64_t>(), queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(),
                              trajOff_.as<int64_t>(), trajLen_.as<int32_t>(), trajNum, maxlen[bucket], epsilon,
                              bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
@@ -532,6 +532,138 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
   return GAT_OK;
 }
 
+// The batch chained on the device: the filter writes the survivors,
+// gat_edr_jobs groups them into the DP's length buckets and jobs, the DP
+// launches read their job windows on the device, gat_edr_survivor_lists builds
+// each query's list and gat_edr_topk merges; one readback at the end carries
+// the results and the statuses.  A survivor count above the capacity empties
+// every window: the capacity grows and the batch runs again.
+int Grid::similarityOnDevice(int nq, int kValue, std::vector<TopK>& out, EdrBatchStats* st, bool* done) {
+  *done = false;
+  void* s = engine_.stream();
+  int64_t eligible = 0;
+  for (int L = 1; L <= GAT_MAX_TRAJ_LEN; ++L) eligible += int64_t(lengthIds_[L].size());
+  const int kk = int(std::min<int64_t>(kValue, eligible));
+  const int split = std::max(1, std::min(edrSplit, GAT_MAX_TRAJ_LEN));
+  const int waves = (edrWavesPerPair == 2 || edrWavesPerPair == 4 || edrWavesPerPair == 8) ? edrWavesPerPair : 1;
+  const size_t cells = size_t(nq) * size_t(kValue);
+  int64_t cap = std::max<int64_t>({deviceTopkCap_, int64_t(nq) * 64, 1024});
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const size_t ns = size_t(nq) + 1;
+    if (int rc = pairsDev_.reserve(size_t(cap) * 8)) return rc;
+    if (int rc = sortedDev_.reserve(size_t(cap) * 8)) return rc;
+    if (int rc = nmDev_.reserve(size_t(cap) * 8)) return rc;
+    if (int rc = jobsDev_.reserve((size_t(cap) + 4) * sizeof(gat_edr_job))) return rc;
+    if (int rc = outDev_.reserve(size_t(cap) * 4)) return rc;
+    if (int rc = survDev_.reserve((ns + 2 * size_t(cap)) * 4)) return rc;
+    if (int rc = topDev_.reserve((2 * cells + 1) * 4)) return rc;
+    if (int rc = counterDev_.reserve(16)) return rc;
+    if (int rc = windowsDev_.reserve(64)) return rc;
+    if (int rc = dpCounterDev_.reserve(16)) return rc;
+    // windows: [0..7] bucket windows, [8] status, [10..11] cell count.
+    int32_t* win = windowsDev_.as<int32_t>();
+    auto* cellCount = reinterpret_cast<uint64_t*>(win + 10);
+    if (int rc = filterTimer_.start(s)) return rc;
+    GATH_TRY(gat_edr_filter(queryBox_.as<float>(), nq, trajBox_.as<float>(), trajNum, epsilon, pairsDev_.as<int32_t>(),
+                            cap, counterDev_.as<uint64_t>(), s),
+             "candidate filter");
+    if (int rc = filterTimer_.stop(s)) return rc;
+    GATH_TRY(gat_edr_jobs(pairsDev_.as<int32_t>(), counterDev_.as<uint64_t>(), cap, queryLen_.as<int32_t>(), nq,
+                          trajLen_.as<int32_t>(), trajNum, split, waves, edrWavesEveryLongPair ? 1 : 0,
+                          sortedDev_.as<int32_t>(), nmDev_.as<int32_t>(), jobsDev_.as<gat_edr_job>(), win, win + 8,
+                          cellCount, s),
+             "device jobs");
+    if (int rc = edrTimer_.start(s)) return rc;
+    // Buckets 3 (several waves per pair) and 2 (one pair per wave) beside the
+    // short buckets, as runBatched launches them.
+    auto launch = [&](int bucket, void* stream) -> int {
+      const int maxlen = bucket == 0 ? split : bucket == 1 ? std::max(split, 1024) : GAT_MAX_TRAJ_LEN;
+      GATH_TRY(gat_edr_batched(sortedDev_.as<int32_t>(), cap, jobsDev_.as<gat_edr_job>(), cap + 4, win + 2 * bucket,
+                               dpCounterDev_.as<int32_t>() + bucket, queryXY_.as<float>(), queryOff64_.as<int64_t>(),
+                               queryLen_.as<int32_t>(), nq, trajXY_.as<float>(), trajOff_.as<int64_t>(),
+                               trajLen_.as<int32_t>(), trajNum, maxlen, epsilon,
+                               bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
+                               bucket == 3 ? 64 * waves : bucket == 2 ? 64 : 16, outDev_.as<int32_t>(), stream),
+               "batched EDR");
+      return GAT_OK;
+    };
+    if (int rc = edrSide_.fork(s)) return rc;
+    if (waves > 1)
+      if (int rc = launch(3, edrSide_.stream())) return rc;
+    if (int rc = edrSide2_.fork(s)) return rc;
+    if (int rc = launch(2, edrSide2_.stream())) return rc;
+    if (int rc = launch(0, s)) return rc;
+    if (int rc = launch(1, s)) return rc;
+    if (int rc = edrSide_.join(s)) return rc;
+    if (int rc = edrSide2_.join(s)) return rc;
+    if (int rc = edrTimer_.stop(s)) return rc;
+    if (int rc = mergeTimer_.start(s)) return rc;
+    int32_t* sv = survDev_.as<int32_t>();
+    GATH_TRY(gat_edr_survivor_lists(sortedDev_.as<int32_t>(), outDev_.as<int32_t>(), counterDev_.as<uint64_t>(), cap,
+                                    nq, sv, sv + ns, sv + ns + cap, win + 8, s),
+             "survivor lists");
+    int32_t* top = topDev_.as<int32_t>();
+    GATH_TRY(gat_edr_topk(ftableDev_.as<int32_t>(), GAT_MAX_TRAJ_LEN + 1, queryLen_.as<int32_t>(), nq,
+                          trajLen_.as<int32_t>(), trajNum, lenIdsDev_.as<int32_t>(), lenOffDev_.as<int32_t>(), sv,
+                          sv + ns, sv + ns + cap, kValue, kk, top, top + cells, top + 2 * cells, s),
+             "top-k merge");
+    if (int rc = mergeTimer_.stop(s)) return rc;
+    if (int rc = stage_.begin((2 * cells + 1) * 4 + 2 * 256, s)) return rc;
+    int32_t* down = stage_.take_as<int32_t>(2 * cells + 1);
+    int32_t* ctl = stage_.take_as<int32_t>(16);
+    if (!down || !ctl) {
+      gath_set_error("device top-k: transfer arena too small");
+      return GAT_ERR_STATE;
+    }
+    GATH_TRY(gat_memcpy_d2h(down, topDev_.get(), (2 * cells + 1) * 4, s), "top-k readback");
+    GATH_TRY(gat_memcpy_d2h(ctl, windowsDev_.get(), 48, s), "top-k readback");
+    GATH_TRY(gat_memcpy_d2h(ctl + 12, counterDev_.get(), 8, s), "top-k readback");
+    GATH_TRY(gat_stream_sync(s), "top-k readback");
+    uint64_t count = 0;
+    std::memcpy(&count, ctl + 12, 8);
+    const int32_t status = ctl[8];
+    if (status & 1) {  // more survivors than the capacity: grow and run again
+      if (count > uint64_t(INT32_MAX) - 8) {
+        gath_set_error("similarityBatched: %llu candidate pairs pass the filter (at most %d per call; split the "
+                       "queries into smaller batches)",
+                       (unsigned long long)count, INT32_MAX - 8);
+        return GAT_ERR_CAPACITY;
+      }
+      cap = std::min<int64_t>(int64_t(count) + int64_t(count) / 4 + 1024, INT32_MAX - 8);
+      deviceTopkCap_ = cap;
+      continue;
+    }
+    if (status & 2) return GAT_OK;  // a query's list exceeds the device sort: the host-built path
+    if (down[2 * cells] != 0) {
+      gath_set_error("top-k merge: inconsistent inputs (status %d)", down[2 * cells]);
+      return GAT_ERR_HIP;
+    }
+    out.assign(size_t(nq), TopK{});
+    for (int q = 0; q < nq; ++q) {
+      out[size_t(q)].ids.assign(down + int64_t(q) * kValue, down + int64_t(q) * kValue + kk);
+      out[size_t(q)].dist.assign(down + cells + int64_t(q) * kValue, down + cells + int64_t(q) * kValue + kk);
+    }
+    if (st) {
+      float fms = 0, dms = 0, mms = 0;
+      if (int rc = filterTimer_.elapsed(&fms)) return rc;
+      if (int rc = edrTimer_.elapsed(&dms)) return rc;
+      if (int rc = mergeTimer_.elapsed(&mms)) return rc;
+      uint64_t c = 0;
+      std::memcpy(&c, ctl + 10, 8);
+      st->pairs += int64_t(count);
+      st->cells += int64_t(c);
+      st->jobs += int64_t(ctl[1]) + ctl[3] + ctl[5] + ctl[7];
+      st->pruned += int64_t(nq) * trajNum - int64_t(count);
+      st->filterMs += fms;
+      st->kernelMs += dms;
+      st->mergeMs += mms;
+    }
+    *done = true;
+    return GAT_OK;
+  }
+  return GAT_OK;  // still over capacity after growing once: the host-built path
+}
+
 int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vector<TopK>& out, EdrBatchStats* st) {
   if (int rc = checkTrajLengths("similarityBatched")) return rc;
   const double u0 = now_ms();
@@ -544,6 +676,11 @@ int Grid::similarityBatched(const TrajectoryView& queries, int kValue, std::vect
   }
   const std::vector<int32_t>& F = mismatchTable();
   if (F.empty()) return GAT_ERR_HIP;
+  if (deviceChain && kValue >= 1 && kValue <= kDeviceTopkMax && nq <= kDeviceListQueries) {
+    bool done = false;
+    if (int rc = similarityOnDevice(nq, kValue, out, st, &done)) return rc;
+    if (done) return GAT_OK;
+  }
   void* s = engine_.stream();
   // Candidate filter on the device; grow the survivor buffer until it fits.
   int64_t cap = std::max<int64_t>(int64_t(nq) * 64, 1024);

@@ -641,6 +641,13 @@ int gath_grid_set_edr_waves(void* p, int waves_per_pair) {
   return GAT_OK;
 }
 
+int gath_grid_set_topk_device_chain(void* p, int on) {
+  Index* ix = as_index(p);
+  if (!ix || !ix->grid) return bad("gath_grid_set_topk_device_chain: not a grid index");
+  ix->grid->deviceChain = on != 0;
+  return GAT_OK;
+}
+
 int gath_tune_range(void* p, const float* boxes, int nq, int reps, double* out) {
   Index* ix = as_index(p);
   if (!ix || !out) return bad("gath_tune_range: bad arguments");

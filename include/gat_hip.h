@@ -282,12 +282,44 @@ typedef struct gat_edr_job {
  * pairs; a job's entries past its first are not computed, so such jobs hold one
  * pair).  Callers split pairs into length buckets and launch each bucket with
  * its own maxlen, so short pairs are not run at the occupancy of the longest.
- * Asynchronous on `stream`. */
+ * window_dev (may be null): two int32 on the device, read when the launch
+ * runs, that select jobs [window[0], window[0] + window[1]) of the njobs the
+ * array holds (gat_edr_jobs writes them).  Asynchronous on `stream`. */
 int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job* jobs_dev, int64_t njobs,
-                    int32_t* counter_dev, const float* query_xy_dev, const int64_t* query_offset_dev,
+                    const int32_t* window_dev, int32_t* counter_dev, const float* query_xy_dev, const int64_t* query_offset_dev,
                     const int32_t* query_length_dev, int32_t nqueries, const float* cand_xy_dev,
                     const int64_t* cand_offset_dev, const int32_t* cand_length_dev, int32_t ncands, int32_t maxlen,
                     double eps, int32_t per_cu, int32_t lanes_per_pair, int32_t* out_dev, void* stream);
+
+/* The job list of a batch of survivor pairs built on the device (no reference
+ * counterpart), so that filter, DP and merge run without a host round trip.
+ * Reads the *count_dev pairs (query, candidate) of pairs_dev (the filter's
+ * output; at most capacity); writes them to sorted_dev grouped by length
+ * bucket (longer length up to `split`; up to max(split, 1024), or `split` in a
+ * batch of fewer than 64 jobs per CU; longer; and, in such a batch with waves
+ * > 1, the long pairs that would outlast the batch's longest several-wave
+ * pair, or every long pair when every_long) and within a bucket by strip count
+ * and longer length, descending; jobs_dev (capacity + 4 entries) gets jobs of
+ * up to four pairs of one strip count (one pair in the two long buckets) and
+ * windows_dev[2b], [2b+1] the first job and job count of bucket b (four
+ * buckets), for gat_edr_batched's windows.  scratch_dev: capacity * 2 int32;
+ * *cells_dev = the sum of the pairs' DP cells.  When *count_dev exceeds the
+ * capacity nothing is sorted, every window is empty and status bit 1 is set
+ * (*status_dev is cleared first).  Asynchronous on `stream`. */
+int gat_edr_jobs(const int32_t* pairs_dev, const uint64_t* count_dev, int64_t capacity, const int32_t* query_length_dev,
+                 int32_t nqueries, const int32_t* cand_length_dev, int32_t ncands, int32_t split, int32_t waves,
+                 int32_t every_long, int32_t* sorted_dev, int32_t* scratch_dev, gat_edr_job* jobs_dev,
+                 int32_t* windows_dev, int32_t* status_dev, uint64_t* cells_dev, void* stream);
+
+/* Per-query survivor lists for gat_edr_topk from gat_edr_jobs' sorted pairs
+ * and their distances (dist_dev in the same order): start_dev (nq + 1) = the
+ * lists' offsets, ids_dev / dists_dev = each query's candidates ascending by
+ * id with their distances.  nq <= 8192.  A query with more than 2048
+ * survivors keeps its list unsorted and sets status bit 2.  Nothing is
+ * listed when *count_dev exceeds the capacity.  Asynchronous on `stream`. */
+int gat_edr_survivor_lists(const int32_t* sorted_dev, const int32_t* dist_dev, const uint64_t* count_dev,
+                           int64_t capacity, int32_t nq, int32_t* start_dev, int32_t* ids_dev, int32_t* dists_dev,
+                           int32_t* status_dev, void* stream);
 
 /* Candidate filter for exact top-k pruning (no reference counterpart; the
  * reference orders candidates by an unproven frequency-distance bound,

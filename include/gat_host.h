@@ -121,6 +121,10 @@ int gath_grid_set_edr_params(void* grid, int split, int per_cu_short, int per_cu
  * takes over that many waves get them; the others run one pair per wave.
  * -2, -4 or -8 give every long pair of such a batch 2, 4 or 8 waves. */
 int gath_grid_set_edr_waves(void* grid, int waves_per_pair);
+/* Top-k batches (k <= 1024, at most 8192 queries) chained on the device —
+ * filter, jobs, DP, survivor lists and merge with one readback (on, the
+ * default) — or with the jobs and survivor lists built on the host (off). */
+int gath_grid_set_topk_device_chain(void* grid, int on);
 /* Tunables of the batched form: rectangles per leaf-major work item (1..64),
  * and the largest batch that takes the direct form (one refine task per
  * selected leaf and rectangle, nothing sized by the tree touched). */

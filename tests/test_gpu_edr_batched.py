@@ -237,3 +237,53 @@ def test_topk_rejects_overlong_database_trajectory(gpu, k):
         g.similarity_batched(q, k, EPS)
     with pytest.raises(RuntimeError, match="2049 points"):
         g.similarity(q, k, EPS)
+
+
+def test_topk_device_chain_matches_host_built_path(gpu):
+    """The top-k batch chained on the device (gat_edr_jobs groups the filter's
+    survivors into buckets and jobs, the DP launches read their job windows
+    on the device, gat_edr_survivor_lists builds the per-query lists) gives
+    the oracle's answers, as the host-built path does, under every wave
+    setting of the long pairs (the device job builder applies the same
+    one-round rules), on lengths that fill all four buckets."""
+    rng = np.random.default_rng(23)
+    lengths = list(rng.choice([16, 40, 100, 300, 700, 1100, 1600, 2048], 500))
+    db = _varied_db(23, np.array(lengths, np.int32))
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    tid = np.arange(1, db.ntraj + 1, dtype=np.int32)
+    for qlen in ([64, 900], [16, 100, 700, 1024, 1500, 2048, 300, 40]):
+        q = api.synth_queries(24 + len(qlen), np.array(qlen, np.int32), db, copy_mode=2)
+        full = _brute(db, q)
+        for waves, every in ((1, False), (4, False), (4, True), (8, False)):
+            g.set_edr_waves(waves, every_long_pair=every)
+            got = {}
+            for chain in (True, False):
+                g.set_topk_device_chain(chain)
+                ids, dists, st = g.similarity_batched(q, 25, EPS)
+                assert st["pairs"] + st["pruned"] == q.ntraj * db.ntraj
+                for i in range(q.ntraj):
+                    oi, od = O.topk(full[i], tid, 25)
+                    assert (ids[i] == oi).all() and (dists[i] == od).all(), (qlen, waves, every, chain, i)
+                got[chain] = st["pairs"]
+            assert got[True] == got[False]
+    g.set_topk_device_chain(True)
+    g.set_edr_waves(4)
+
+
+def test_topk_device_chain_falls_back_for_long_lists(gpu):
+    """Survivors past the chain's first capacity (1,024) grow it and run the
+    batch again; then a query with more survivors than the device's per-query
+    sort holds (2,600 trajectories on one spot, all within eps of the query)
+    sends the batch to the host-built lists, with the oracle's answers."""
+    db = api.synth_walks(31, np.full(2600, 20, np.int32), step_sigma=1e-5)
+    db.lon[:] = np.float32(0.5) + (db.lon - db.lon.mean()) * np.float32(1e-3)
+    db.lat[:] = np.float32(0.5) + (db.lat - db.lat.mean()) * np.float32(1e-3)
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    q = api.synth_queries(32, np.array([20, 12], np.int32), db, copy_mode=2)
+    ids, dists, st = g.similarity_batched(q, 10, 0.01)
+    assert st["pairs"] > 2048  # above the chain's first capacity and one query's sortable list
+    full = _brute(db, q, 0.01)
+    tid = np.arange(1, db.ntraj + 1, dtype=np.int32)
+    for i in range(q.ntraj):
+        oi, od = O.topk(full[i], tid, 10)
+        assert (ids[i] == oi).all() and (dists[i] == od).all(), i
