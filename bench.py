@@ -608,8 +608,9 @@ def bench_edr(G, W, cpu_topk: bool = False) -> dict:
             "topk_batched_qps": round(queries.ntraj / topk_s, 2),
             "topk_batch_ms": round(1e3 * topk_s, 3),
             "topk_stage_ms": {"query_upload": round(st["upload_ms"], 3), "filter_kernel": round(st["filter_ms"], 3),
-                              "dp_kernels": round(st["kernel_ms"], 3), "job_building": round(st["host_ms"], 3),
+                              "dp_kernels": round(st["kernel_ms"], 3), "host_job_building": round(st["host_ms"], 3),
                               "merge_stage": round(st["merge_ms"], 3)},
+            "topk_chain": "device: filter, jobs, DP, survivor lists and merge queued with no host step",
             "topk_pairs_evaluated": int(st["pairs"]),
             "edr_autotuned": tuned,
             "topk_cpu": cpu}
