@@ -396,7 +396,14 @@ class _Index:
         b = _req(boxes, np.float32).reshape(-1, 4)
         out = np.zeros(2, np.int64)
         N.check_host(N.host().gath_system_range_test(self.handle, kinds[which], _ptr(b), len(b), _ptr(out)))
+        self._system_last = (len(b), int(out[0]), int(out[1]))
         return int(out[0]), int(out[1])
+
+    def system_range_result(self) -> RangeResult:
+        """The result of the last system_range_test call (its pairs stay in
+        HBM until the index's next query): counts, pairs and digests."""
+        nq, cand, pairs = self._system_last
+        return RangeResult(self, nq, np.array([pairs, cand, 0], np.int64), np.zeros(3))
 
     def set_batch_params(self, queries_per_item: int = 32, direct_max_queries: int = 256) -> "_Index":
         """Batched form: rectangles per leaf-major work item (1..64), and the

@@ -170,22 +170,29 @@ def test_refine_abi_skips_malformed_tasks(gpu, packed):
 
 def test_system_test_entry_points(cfg1):
     """The reference's SystemTest range entry points (SystemTest.cpp:24-102,
-    251-384, 386-445) run the plain GPU path of each index: as many result
-    pairs as the oracle finds, over the same candidates as the index's own
-    plain call; a MortonGrid answers rangeQueryTest too (it is a Grid)."""
+    251-384, 386-445) run the plain GPU path of each index: every rectangle's
+    result pairs equal the oracle's as sets (GT-quadtree and MortonGrid: the
+    Grid restatement; STIG and FSG: brute force), with the oracle's per-query
+    counts and digests, over the same candidates as the index's own plain
+    call; a MortonGrid answers rangeQueryTest too (it is a Grid)."""
     db, rects = cfg1
-    want = sum(len(r) for r in O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db).range(rects))
-    brute = sum(len(r) for r in O.brute_range(db, rects))
-    cases = [("grid", api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf), want),
-             ("mgrid", api.MortonGridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf), want),
-             ("stig", api.StigIndex(W.CFG1.leaf), brute),
-             ("fsg", api.FsgIndex(W.UNIT_BOX, W.CFG1.grid_cell), brute)]
-    for which, ix, pairs in cases:
+    og = O.Grid(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf).add(db)
+    want_sets, want_dig = _sets(og.range(rects)), og.range(rects, "digest")
+    brute_sets, brute_dig = _sets(O.brute_range(db, rects)), O.brute_range(db, rects, "digest")
+    cases = [("grid", api.GridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf), want_sets, want_dig),
+             ("mgrid", api.MortonGridIndex(W.UNIT_BOX, W.CFG1.grid_cell, W.CFG1.leaf), want_sets, want_dig),
+             ("stig", api.StigIndex(W.CFG1.leaf), brute_sets, brute_dig),
+             ("fsg", api.FsgIndex(W.UNIT_BOX, W.CFG1.grid_cell), brute_sets, brute_dig)]
+    for which, ix, sets, (counts, dig) in cases:
         ix.build(db)
         cand, got = ix.system_range_test(which, rects)
-        assert got == pairs, which
+        assert got == sum(len(w) for w in sets), which
+        res = ix.system_range_result()
+        assert (res.counts() == counts).all() and (res.digests() == dig).all(), which
+        _check_pairs(res, sets, rects)
         assert cand == ix.range_pairs(rects).candidates, which
     mg = cases[1][1]
-    assert mg.system_range_test("grid", rects)[1] == want
+    assert mg.system_range_test("grid", rects)[1] == sum(len(w) for w in want_sets)
+    assert (mg.system_range_result().digests() == want_dig[1]).all()
     with pytest.raises(GatError):
         cases[0][1].system_range_test("mgrid", rects)
