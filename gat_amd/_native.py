@@ -77,7 +77,9 @@ HIP_FUNCS = {
     "gat_pairs_digest": (INT, [P, P, P, I64, I64, P, P]),
     "gatEDRDistanceBatchHandler": (INT, [INT, P, P, P, P, P, P, P, F64, P, P]),
     "gat_edr_batched": (INT, [P, I64, P, I64, P, P, P, P, P, I32, P, P, P, I32, I32, F64, I32, I32, P, P]),
-    "gat_edr_jobs": (INT, [P, P, I64, P, I32, P, I32, I32, I32, I32, P, P, P, P, P, P, P]),
+    "gat_edr_jobs": (INT, [P, P, I64, P, I32, P, I32, I32, I32, I32, P, P, P, P, P, P, P, P]),
+    "gat_edr_all_pairs": (INT, [I32, I32, P, P, P]),
+    "gat_edr_unsort": (INT, [P, P, P, I64, P, P]),
     "gat_edr_survivor_lists": (INT, [P, P, P, I64, I32, P, P, P, P, P]),
     "gat_edr_filter": (INT, [P, I32, P, I32, F64, P, I64, P, P]),
     "gat_edr_mismatch_table": (INT, [I32, P, P]),

@@ -784,8 +784,8 @@ __global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
 // the rest one pair per wave, and in a batch of one round of waves the pairs
 // that would outlast the longest pair's several-wave schedule on one wave
 // take several), then a counting sort in LDS by (bucket, strips descending,
-// longer length descending in 128-point steps) and jobs of up to four pairs of
-// one strip count (one pair in the long buckets).  Pairs of equal sort key
+// longer length descending in sixteen steps of the bucket's length range) and
+// jobs of up to four pairs of one strip count (one pair in the long buckets).  Pairs of equal sort key
 // keep no particular order: it only changes which pairs share a job.
 constexpr int kJobThreads = 1024;
 constexpr int kStripsMax = GAT_MAX_TRAJ_LEN / 16 + 1;  // strips_of(n) is 0..128
@@ -851,6 +851,7 @@ __global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __res
                                                                uint32_t cap, const int32_t* __restrict__ qlen,
                                                                int32_t nq, const int32_t* __restrict__ clen,
                                                                int32_t nc, JobRules rules, int2* __restrict__ sorted,
+                                                               int32_t* __restrict__ origin,
                                                                int2* __restrict__ nm, Job* __restrict__ jobs,
                                                                int32_t* __restrict__ windows,
                                                                int32_t* __restrict__ status,
@@ -911,7 +912,10 @@ __global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __res
         (rules.every_long || int64_t((n + 63) / 64) * ((m + 63 + 15) / 16) > critical))
       bucket = 3;
     const int strips = (n + 15) / 16;
-    const int len = m >= 1 ? min((m - 1) / 128, kLenSteps - 1) : 0;
+    // The bucket's length range in kLenSteps steps (short pairs need finer ones).
+    const int top = bucket == 0 ? rules.split : bucket == 1 ? mid : GAT_MAX_TRAJ_LEN;
+    const int step = max(1, (top + kLenSteps - 1) / kLenSteps);
+    const int len = m >= 1 ? min((m - 1) / step, kLenSteps - 1) : 0;
     return uint32_t((bucket * kStripsMax + (kStripsMax - 1 - strips)) * kLenSteps + (kLenSteps - 1 - len));
   };
   for (uint32_t i = t; i < np; i += kJobThreads) atomicAdd(&bins[key_of(nm[i])], 1u);
@@ -920,6 +924,7 @@ __global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __res
   for (uint32_t i = t; i < np; i += kJobThreads) {
     const uint32_t pos = atomicAdd(&bins[key_of(nm[i])], 1u);
     sorted[pos] = pairs[i];
+    if (origin) origin[pos] = int32_t(i);
   }
   __syncthreads();
   // bins[b] now ends bin b: group g spans [end of bin g*16 - 1, end of bin g*16 + 15).
@@ -1027,6 +1032,28 @@ __global__ __launch_bounds__(kSortThreads) void edr_lists_sort_kernel(const int3
   for (int i = t; i < m; i += kSortThreads) {
     ids[s0 + i] = int32_t(v[i] >> 32);
     dists[s0 + i] = int32_t(uint32_t(v[i]));
+  }
+}
+
+// Every (query, candidate) pair of nq x nc, pair i = (i / nc, i % nc).
+__global__ __launch_bounds__(256) void edr_all_pairs_kernel(int32_t nq, int32_t nc, int2* __restrict__ pairs,
+                                                            unsigned long long* __restrict__ count) {
+  const uint64_t n = uint64_t(nq) * uint64_t(nc);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    pairs[i] = make_int2(int32_t(i / uint64_t(nc)), int32_t(i % uint64_t(nc)));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = n;
+}
+
+// out[origin[p]] = sorted_dist[p]: distances back in the pairs' first order.
+__global__ __launch_bounds__(256) void edr_unsort_kernel(const int32_t* __restrict__ origin,
+                                                         const int32_t* __restrict__ sorted_dist,
+                                                         const unsigned long long* __restrict__ count, uint32_t cap,
+                                                         int32_t* __restrict__ out) {
+  const unsigned long long total = *count;
+  const uint32_t n = total > cap ? 0u : uint32_t(total);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const int32_t o = origin[p];
+    if (o >= 0 && uint32_t(o) < n) out[o] = sorted_dist[p];
   }
 }
 
@@ -1206,8 +1233,8 @@ extern "C" int gat_edr_batched(const int32_t* pairs, int64_t npairs, const gat_e
 
 extern "C" int gat_edr_jobs(const int32_t* pairs, const uint64_t* count, int64_t capacity, const int32_t* qlen,
                             int32_t nq, const int32_t* clen, int32_t nc, int32_t split, int32_t waves,
-                            int32_t every_long, int32_t* sorted, int32_t* scratch, gat_edr_job* jobs,
-                            int32_t* windows, int32_t* status, uint64_t* cells, void* stream) {
+                            int32_t every_long, int32_t* sorted, int32_t* origin, int32_t* scratch,
+                            gat_edr_job* jobs, int32_t* windows, int32_t* status, uint64_t* cells, void* stream) {
   GAT_REQUIRE(capacity >= 0 && capacity <= int64_t(INT32_MAX) - 8 && nq >= 0 && nc >= 0,
               "EDR jobs: bad sizes");
   GAT_REQUIRE(split >= 1 && split <= GAT_MAX_TRAJ_LEN && waves >= 1 && waves <= 8, "EDR jobs: bad rules");
@@ -1217,7 +1244,7 @@ extern "C" int gat_edr_jobs(const int32_t* pairs, const uint64_t* count, int64_t
   const JobRules rules{split, waves, every_long ? 1 : 0, int32_t(gat::cu_count())};
   edr_jobs_kernel<<<dim3(1), dim3(kJobThreads), 0, gat::as_stream(stream)>>>(
       reinterpret_cast<const int2*>(pairs), reinterpret_cast<const unsigned long long*>(count), uint32_t(capacity),
-      qlen, nq, clen, nc, rules, reinterpret_cast<int2*>(sorted), reinterpret_cast<int2*>(scratch),
+      qlen, nq, clen, nc, rules, reinterpret_cast<int2*>(sorted), origin, reinterpret_cast<int2*>(scratch),
       reinterpret_cast<Job*>(jobs), windows, status, reinterpret_cast<unsigned long long*>(cells));
   GAT_LAUNCH_CHECK("edr_jobs_kernel");
   return GAT_OK;
@@ -1239,3 +1266,32 @@ extern "C" int gat_edr_survivor_lists(const int32_t* sorted, const int32_t* dist
   GAT_LAUNCH_CHECK("edr_lists_sort_kernel");
   return GAT_OK;
 }
+
+extern "C" int gat_edr_all_pairs(int32_t nq, int32_t nc, int32_t* pairs, uint64_t* count, void* stream) {
+  GAT_REQUIRE(nq >= 0 && nc >= 0 && int64_t(nq) * nc <= int64_t(INT32_MAX), "EDR all pairs: bad sizes");
+  GAT_REQUIRE(pairs && count, "EDR all pairs: null pointer");
+  const int64_t n = int64_t(nq) * nc;
+  int64_t blocks = (n + 255) / 256;
+  const int64_t cap = int64_t(gat::cu_count()) * 16;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  edr_all_pairs_kernel<<<dim3(uint32_t(blocks)), dim3(256), 0, gat::as_stream(stream)>>>(
+      nq, nc, reinterpret_cast<int2*>(pairs), reinterpret_cast<unsigned long long*>(count));
+  GAT_LAUNCH_CHECK("edr_all_pairs_kernel");
+  return GAT_OK;
+}
+
+extern "C" int gat_edr_unsort(const int32_t* origin, const int32_t* sorted_dist, const uint64_t* count,
+                              int64_t capacity, int32_t* out, void* stream) {
+  GAT_REQUIRE(capacity >= 0 && capacity <= int64_t(INT32_MAX), "EDR unsort: bad capacity");
+  GAT_REQUIRE(origin && sorted_dist && count && out, "EDR unsort: null pointer");
+  int64_t blocks = (capacity + 255) / 256;
+  const int64_t cap = int64_t(gat::cu_count()) * 16;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  edr_unsort_kernel<<<dim3(uint32_t(blocks)), dim3(256), 0, gat::as_stream(stream)>>>(
+      origin, sorted_dist, reinterpret_cast<const unsigned long long*>(count), uint32_t(capacity), out);
+  GAT_LAUNCH_CHECK("edr_unsort_kernel");
+  return GAT_OK;
+}
+

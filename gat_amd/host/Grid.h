@@ -134,8 +134,9 @@ class Grid {
   int edrWavesPerPair = 4;
   bool edrWavesEveryLongPair = false;
   // Top-k batches of k <= 1024 and at most 8192 queries run filter, jobs, DP,
-  // survivor lists and merge on the device with no host step between them;
-  // off: the jobs and lists are built on the host from the survivors.
+  // survivor lists and merge on the device with no host step between them,
+  // and the batched EDR forms build their job lists on the device; off: the
+  // jobs (and lists) are built on the host.
   bool deviceChain = true;
 
   // Batched EDR of given (query index, trajectory index) pairs, both 0-based:
@@ -197,6 +198,12 @@ class Grid {
   // take the host-built path instead (a query with more survivors than the
   // device's per-query sort holds).
   int similarityOnDevice(int nq, int kValue, std::vector<TopK>& out, EdrBatchStats* st, bool* done);
+  // The DP launches over a device-built job list (windows in win).
+  int launchDeviceBuckets(int64_t cap, int nq, const int32_t* win);
+  // Batched EDR with the job list built on the device (pairsHost null: every
+  // query x trajectory pair); dist in the pairs' order.
+  int runBatchedOnDevice(const int32_t* pairsHost, int64_t np, int queryNum, std::vector<int32_t>& dist,
+                         EdrBatchStats* st);
   static constexpr int kDeviceListQueries = 8192;
   // Trajectory-ordered copy of the points for EDR (the reference reassembles
   // trajectories from the cell-ordered store per query, Grid.cpp:1500-1508).
@@ -214,7 +221,7 @@ class Grid {
   DeviceBuffer lenIdsDev_, lenOffDev_, survDev_, topDev_;
   // The device-built job list: pairs grouped by bucket, scratch, bucket
   // windows + status + cell count, and the DP's job counters.
-  DeviceBuffer sortedDev_, nmDev_, windowsDev_, dpCounterDev_;
+  DeviceBuffer sortedDev_, nmDev_, windowsDev_, dpCounterDev_, originDev_, unsortDev_;
   int64_t deviceTopkCap_ = 0;  // survivor capacity of the device path, grown on overflow
   EventTimer edrTimer_, filterTimer_, mergeTimer_;
   SideStream edrSide_;  // the longest EDR bucket runs here, beside the others

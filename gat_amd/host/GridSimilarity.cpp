@@ -453,6 +453,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
 int Grid::edrBatchedPairs(const TrajectoryView& queries, std::vector<int32_t>& pairs, std::vector<int32_t>& dist,
                           EdrBatchStats* st) {
   if (int rc = ensureQueryStore(queries)) return rc;
+  if (deviceChain) return runBatchedOnDevice(pairs.data(), int64_t(pairs.size() / 2), queries.trajNum, dist, st);
   if (int rc = stage_.begin(runBatchedStaging(int64_t(pairs.size() / 2)), engine_.stream())) return rc;
   return runBatched(pairs, queries.trajNum, dist, st);
 }
@@ -460,6 +461,7 @@ int Grid::edrBatchedPairs(const TrajectoryView& queries, std::vector<int32_t>& p
 int Grid::edrBatchedAll(const TrajectoryView& queries, std::vector<int32_t>& dist, EdrBatchStats* st) {
   if (int rc = ensureQueryStore(queries)) return rc;
   const int nq = queries.trajNum;
+  if (deviceChain) return runBatchedOnDevice(nullptr, int64_t(nq) * trajNum, nq, dist, st);
   std::vector<int32_t> pairs(size_t(nq) * size_t(trajNum) * 2);
   for (int q = 0; q < nq; ++q)
     for (int t = 0; t < trajNum; ++t) {
@@ -532,6 +534,126 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
   return GAT_OK;
 }
 
+// The DP over a device-built job list (gat_edr_jobs, sortedDev_ / jobsDev_ /
+// the windows in win): one launch per bucket reading its window on the
+// device, buckets 3 (several waves per pair) and 2 (one pair per wave) on side
+// streams beside the short buckets, as runBatched launches them.
+int Grid::launchDeviceBuckets(int64_t cap, int nq, const int32_t* win) {
+  void* s = engine_.stream();
+  const int split = std::max(1, std::min(edrSplit, GAT_MAX_TRAJ_LEN));
+  const int waves = (edrWavesPerPair == 2 || edrWavesPerPair == 4 || edrWavesPerPair == 8) ? edrWavesPerPair : 1;
+  auto launch = [&](int bucket, void* stream) -> int {
+    const int maxlen = bucket == 0 ? split : bucket == 1 ? std::max(split, 1024) : GAT_MAX_TRAJ_LEN;
+    GATH_TRY(gat_edr_batched(sortedDev_.as<int32_t>(), cap, jobsDev_.as<gat_edr_job>(), cap + 4, win + 2 * bucket,
+                             dpCounterDev_.as<int32_t>() + bucket, queryXY_.as<float>(), queryOff64_.as<int64_t>(),
+                             queryLen_.as<int32_t>(), nq, trajXY_.as<float>(), trajOff_.as<int64_t>(),
+                             trajLen_.as<int32_t>(), trajNum, maxlen, epsilon,
+                             bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
+                             bucket == 3 ? 64 * waves : bucket == 2 ? 64 : 16, outDev_.as<int32_t>(), stream),
+             "batched EDR");
+    return GAT_OK;
+  };
+  if (int rc = edrSide_.fork(s)) return rc;
+  if (waves > 1)
+    if (int rc = launch(3, edrSide_.stream())) return rc;
+  if (int rc = edrSide2_.fork(s)) return rc;
+  if (int rc = launch(2, edrSide2_.stream())) return rc;
+  if (int rc = launch(0, s)) return rc;
+  if (int rc = launch(1, s)) return rc;
+  if (int rc = edrSide_.join(s)) return rc;
+  return edrSide2_.join(s);
+}
+
+// Batched EDR over np pairs with the job list built on the device: the pairs
+// (pairsHost, or every query x trajectory pair when it is null, generated on
+// the device) go through gat_edr_jobs, the bucket launches and gat_edr_unsort,
+// and dist comes back in the pairs' order.
+int Grid::runBatchedOnDevice(const int32_t* pairsHost, int64_t np, int queryNum, std::vector<int32_t>& dist,
+                             EdrBatchStats* st) {
+  dist.assign(size_t(np), -1);
+  if (np == 0) return GAT_OK;
+  if (np > int64_t(INT32_MAX) - 8) {
+    gath_set_error("batched EDR: %lld pairs (at most %d per call)", (long long)np, INT32_MAX - 8);
+    return GAT_ERR_CAPACITY;
+  }
+  const double h0 = now_ms();
+  if (pairsHost)
+    for (int64_t i = 0; i < np; ++i) {
+      const int q = pairsHost[2 * i], c = pairsHost[2 * i + 1];
+      if (q < 0 || q >= queryNum || c < 0 || c >= trajNum) {
+        gath_set_error("batched EDR: pair %lld out of range", (long long)i);
+        return GAT_ERR_ARG;
+      }
+    }
+  void* s = engine_.stream();
+  const int split = std::max(1, std::min(edrSplit, GAT_MAX_TRAJ_LEN));
+  const int waves = (edrWavesPerPair == 2 || edrWavesPerPair == 4 || edrWavesPerPair == 8) ? edrWavesPerPair : 1;
+  if (int rc = pairsDev_.reserve(size_t(np) * 8)) return rc;
+  if (int rc = sortedDev_.reserve(size_t(np) * 8)) return rc;
+  if (int rc = nmDev_.reserve(size_t(np) * 8)) return rc;
+  if (int rc = originDev_.reserve(size_t(np) * 4)) return rc;
+  if (int rc = jobsDev_.reserve((size_t(np) + 4) * sizeof(gat_edr_job))) return rc;
+  if (int rc = outDev_.reserve(size_t(np) * 4)) return rc;
+  if (int rc = unsortDev_.reserve(size_t(np) * 4)) return rc;
+  if (int rc = counterDev_.reserve(16)) return rc;
+  if (int rc = windowsDev_.reserve(64)) return rc;
+  if (int rc = dpCounterDev_.reserve(16)) return rc;
+  if (int rc = stage_.begin(size_t(np) * 4 + (pairsHost ? size_t(np) * 8 + 8 : 0) + 3 * 256 + 64, s)) return rc;
+  int32_t* win = windowsDev_.as<int32_t>();
+  auto* count = counterDev_.as<uint64_t>();
+  if (pairsHost) {
+    int32_t* up = stage_.take_as<int32_t>(size_t(np) * 2);
+    auto* cnt = stage_.take_as<uint64_t>(1);
+    if (!up || !cnt) {
+      gath_set_error("batched EDR: transfer arena too small");
+      return GAT_ERR_STATE;
+    }
+    std::memcpy(up, pairsHost, size_t(np) * 8);
+    *cnt = uint64_t(np);
+    GATH_TRY(gat_memcpy_h2d(pairsDev_.get(), up, size_t(np) * 8, s), "pair upload");
+    GATH_TRY(gat_memcpy_h2d(count, cnt, 8, s), "pair upload");
+  } else {
+    GATH_TRY(gat_edr_all_pairs(queryNum, trajNum, pairsDev_.as<int32_t>(), count, s), "all pairs");
+  }
+  GATH_TRY(gat_edr_jobs(pairsDev_.as<int32_t>(), count, np, queryLen_.as<int32_t>(), queryNum, trajLen_.as<int32_t>(),
+                        trajNum, split, waves, edrWavesEveryLongPair ? 1 : 0, sortedDev_.as<int32_t>(),
+                        originDev_.as<int32_t>(), nmDev_.as<int32_t>(), jobsDev_.as<gat_edr_job>(), win, win + 8,
+                        reinterpret_cast<uint64_t*>(win + 10), s),
+           "device jobs");
+  const double h1 = now_ms();
+  if (int rc = edrTimer_.start(s)) return rc;
+  if (int rc = launchDeviceBuckets(np, queryNum, win)) return rc;
+  if (int rc = edrTimer_.stop(s)) return rc;
+  GATH_TRY(gat_edr_unsort(originDev_.as<int32_t>(), outDev_.as<int32_t>(), count, np, unsortDev_.as<int32_t>(), s),
+           "distance order");
+  int32_t* down = stage_.take_as<int32_t>(size_t(np));
+  int32_t* ctl = stage_.take_as<int32_t>(16);
+  if (!down || !ctl) {
+    gath_set_error("batched EDR: transfer arena too small");
+    return GAT_ERR_STATE;
+  }
+  GATH_TRY(gat_memcpy_d2h(down, unsortDev_.get(), size_t(np) * 4, s), "batched EDR readback");
+  GATH_TRY(gat_memcpy_d2h(ctl, windowsDev_.get(), 48, s), "batched EDR readback");
+  GATH_TRY(gat_stream_sync(s), "batched EDR readback");
+  if (ctl[8] != 0) {
+    gath_set_error("batched EDR: device job list status %d", ctl[8]);
+    return GAT_ERR_HIP;
+  }
+  std::memcpy(dist.data(), down, size_t(np) * 4);
+  if (st) {
+    float ms = 0;
+    if (int rc = edrTimer_.elapsed(&ms)) return rc;
+    uint64_t cells = 0;
+    std::memcpy(&cells, ctl + 10, 8);
+    st->pairs += np;
+    st->cells += int64_t(cells);
+    st->jobs += int64_t(ctl[1]) + ctl[3] + ctl[5] + ctl[7];
+    st->kernelMs += ms;
+    st->hostMs += (h1 - h0);
+  }
+  return GAT_OK;
+}
+
 // The batch chained on the device: the filter writes the survivors,
 // gat_edr_jobs groups them into the DP's length buckets and jobs, the DP
 // launches read their job windows on the device, gat_edr_survivor_lists builds
@@ -570,32 +692,11 @@ int Grid::similarityOnDevice(int nq, int kValue, std::vector<TopK>& out, EdrBatc
     if (int rc = filterTimer_.stop(s)) return rc;
     GATH_TRY(gat_edr_jobs(pairsDev_.as<int32_t>(), counterDev_.as<uint64_t>(), cap, queryLen_.as<int32_t>(), nq,
                           trajLen_.as<int32_t>(), trajNum, split, waves, edrWavesEveryLongPair ? 1 : 0,
-                          sortedDev_.as<int32_t>(), nmDev_.as<int32_t>(), jobsDev_.as<gat_edr_job>(), win, win + 8,
-                          cellCount, s),
+                          sortedDev_.as<int32_t>(), nullptr, nmDev_.as<int32_t>(), jobsDev_.as<gat_edr_job>(), win,
+                          win + 8, cellCount, s),
              "device jobs");
     if (int rc = edrTimer_.start(s)) return rc;
-    // Buckets 3 (several waves per pair) and 2 (one pair per wave) beside the
-    // short buckets, as runBatched launches them.
-    auto launch = [&](int bucket, void* stream) -> int {
-      const int maxlen = bucket == 0 ? split : bucket == 1 ? std::max(split, 1024) : GAT_MAX_TRAJ_LEN;
-      GATH_TRY(gat_edr_batched(sortedDev_.as<int32_t>(), cap, jobsDev_.as<gat_edr_job>(), cap + 4, win + 2 * bucket,
-                               dpCounterDev_.as<int32_t>() + bucket, queryXY_.as<float>(), queryOff64_.as<int64_t>(),
-                               queryLen_.as<int32_t>(), nq, trajXY_.as<float>(), trajOff_.as<int64_t>(),
-                               trajLen_.as<int32_t>(), trajNum, maxlen, epsilon,
-                               bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
-                               bucket == 3 ? 64 * waves : bucket == 2 ? 64 : 16, outDev_.as<int32_t>(), stream),
-               "batched EDR");
-      return GAT_OK;
-    };
-    if (int rc = edrSide_.fork(s)) return rc;
-    if (waves > 1)
-      if (int rc = launch(3, edrSide_.stream())) return rc;
-    if (int rc = edrSide2_.fork(s)) return rc;
-    if (int rc = launch(2, edrSide2_.stream())) return rc;
-    if (int rc = launch(0, s)) return rc;
-    if (int rc = launch(1, s)) return rc;
-    if (int rc = edrSide_.join(s)) return rc;
-    if (int rc = edrSide2_.join(s)) return rc;
+    if (int rc = launchDeviceBuckets(cap, nq, win)) return rc;
     if (int rc = edrTimer_.stop(s)) return rc;
     if (int rc = mergeTimer_.start(s)) return rc;
     int32_t* sv = survDev_.as<int32_t>();

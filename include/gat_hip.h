@@ -299,17 +299,31 @@ int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job*
  * batch of fewer than 64 jobs per CU; longer; and, in such a batch with waves
  * > 1, the long pairs that would outlast the batch's longest several-wave
  * pair, or every long pair when every_long) and within a bucket by strip count
- * and longer length, descending; jobs_dev (capacity + 4 entries) gets jobs of
+ * and longer length (in sixteen steps of the bucket's range), descending; jobs_dev (capacity + 4 entries) gets jobs of
  * up to four pairs of one strip count (one pair in the two long buckets) and
  * windows_dev[2b], [2b+1] the first job and job count of bucket b (four
- * buckets), for gat_edr_batched's windows.  scratch_dev: capacity * 2 int32;
+ * buckets), for gat_edr_batched's windows; origin_dev (may be null; capacity
+ * entries) gets each sorted pair's index in pairs_dev.  scratch_dev: capacity * 2 int32;
  * *cells_dev = the sum of the pairs' DP cells.  When *count_dev exceeds the
  * capacity nothing is sorted, every window is empty and status bit 1 is set
  * (*status_dev is cleared first).  Asynchronous on `stream`. */
 int gat_edr_jobs(const int32_t* pairs_dev, const uint64_t* count_dev, int64_t capacity, const int32_t* query_length_dev,
                  int32_t nqueries, const int32_t* cand_length_dev, int32_t ncands, int32_t split, int32_t waves,
-                 int32_t every_long, int32_t* sorted_dev, int32_t* scratch_dev, gat_edr_job* jobs_dev,
-                 int32_t* windows_dev, int32_t* status_dev, uint64_t* cells_dev, void* stream);
+                 int32_t every_long, int32_t* sorted_dev, int32_t* origin_dev, int32_t* scratch_dev,
+                 gat_edr_job* jobs_dev, int32_t* windows_dev, int32_t* status_dev, uint64_t* cells_dev,
+                 void* stream);
+
+/* Every (query, candidate) pair of nq x nc: pairs_dev[2i], [2i+1] = (i / nc,
+ * i % nc); *count_dev = nq * nc (at most INT32_MAX).  For gat_edr_jobs over a
+ * whole query batch.  Asynchronous on `stream`. */
+int gat_edr_all_pairs(int32_t nq, int32_t nc, int32_t* pairs_dev, uint64_t* count_dev, void* stream);
+
+/* Distances back in the pairs' first order: out_dev[origin_dev[p]] =
+ * sorted_dist_dev[p] for p below *count_dev (nothing when it exceeds the
+ * capacity); origin_dev and the order are gat_edr_jobs'.  Asynchronous on
+ * `stream`. */
+int gat_edr_unsort(const int32_t* origin_dev, const int32_t* sorted_dist_dev, const uint64_t* count_dev,
+                   int64_t capacity, int32_t* out_dev, void* stream);
 
 /* Per-query survivor lists for gat_edr_topk from gat_edr_jobs' sorted pairs
  * and their distances (dist_dev in the same order): start_dev (nq + 1) = the
