@@ -788,6 +788,8 @@ __global__ __launch_bounds__(kTopkBlock) void edr_topk_kernel(
 // jobs of up to four pairs of one strip count (one pair in the long buckets).  Pairs of equal sort key
 // keep no particular order: it only changes which pairs share a job.
 constexpr int kJobThreads = 1024;
+constexpr int64_t kJobsOneBlock = 131072;  // larger batches take the whole device
+constexpr int kJobsScratchExtra = GAT_EDR_JOBS_SCRATCH_EXTRA;
 constexpr int kStripsMax = GAT_MAX_TRAJ_LEN / 16 + 1;  // strips_of(n) is 0..128
 constexpr int kJobGroups = 4 * kStripsMax;             // (bucket, strips)
 constexpr int kLenSteps = 16;                          // longer length in 128-point steps
@@ -846,6 +848,35 @@ struct JobRules {
   int32_t split, waves, every_long, cus;
 };
 
+// The bucket rules of a batch of np pairs: in a batch of fewer than 64 jobs
+// per CU (one round of waves) long means longer than `split`, otherwise
+// longer than max(split, 1024); with several waves per pair such a batch
+// sends some long pairs to the several-wave bucket.
+struct JobShape {
+  int mid;
+  bool multi;
+  __device__ JobShape(const JobRules& r, uint32_t np) {
+    const bool latency_bound = (int64_t(np) + 3) / 4 < int64_t(r.cus) * 16;
+    mid = latency_bound ? r.split : max(r.split, 1024);
+    multi = latency_bound && r.waves > 1;
+  }
+};
+
+// Sort key of a pair of shorter length n and longer m: (bucket, strips
+// descending, longer length descending in sixteen steps of the bucket's
+// range); critical is the batch's longest several-wave schedule.
+__device__ __forceinline__ uint32_t job_key(int n, int m, const JobRules& rules, const JobShape& sh,
+                                            int64_t critical) {
+  int bucket = m <= rules.split ? 0 : m <= sh.mid ? 1 : 2;
+  if (bucket == 2 && sh.multi && (rules.every_long || int64_t((n + 63) / 64) * ((m + 63 + 15) / 16) > critical))
+    bucket = 3;
+  const int strips = (n + 15) / 16;
+  const int top = bucket == 0 ? rules.split : bucket == 1 ? sh.mid : GAT_MAX_TRAJ_LEN;
+  const int step = max(1, (top + kLenSteps - 1) / kLenSteps);
+  const int len = m >= 1 ? min((m - 1) / step, kLenSteps - 1) : 0;
+  return uint32_t((bucket * kStripsMax + (kStripsMax - 1 - strips)) * kLenSteps + (kLenSteps - 1 - len));
+}
+
 __global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __restrict__ pairs,
                                                                const unsigned long long* __restrict__ count,
                                                                uint32_t cap, const int32_t* __restrict__ qlen,
@@ -869,9 +900,9 @@ __global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __res
   }
   const uint32_t np = uint32_t(total);
   // Lengths, the cell count and the several-wave threshold.
-  const bool latency_bound = (int64_t(np) + 3) / 4 < int64_t(rules.cus) * 16;
-  const int mid = latency_bound ? rules.split : max(rules.split, 1024);
-  const bool multi = latency_bound && rules.waves > 1;
+  const JobShape sh0(rules, np);
+  const int mid = sh0.mid;
+  const bool multi = sh0.multi;
   unsigned long long my_cells = 0;
   int64_t my_crit = 0;
   for (uint32_t i = t; i < np; i += kJobThreads) {
@@ -905,19 +936,8 @@ __global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __res
     for (int w = 0; w < kJobThreads / 64; ++w) c += red64[w];
     *cells = c;
   }
-  auto key_of = [&](int2 v) {
-    const int n = v.x, m = v.y;
-    int bucket = m <= rules.split ? 0 : m <= mid ? 1 : 2;
-    if (bucket == 2 && multi &&
-        (rules.every_long || int64_t((n + 63) / 64) * ((m + 63 + 15) / 16) > critical))
-      bucket = 3;
-    const int strips = (n + 15) / 16;
-    // The bucket's length range in kLenSteps steps (short pairs need finer ones).
-    const int top = bucket == 0 ? rules.split : bucket == 1 ? mid : GAT_MAX_TRAJ_LEN;
-    const int step = max(1, (top + kLenSteps - 1) / kLenSteps);
-    const int len = m >= 1 ? min((m - 1) / step, kLenSteps - 1) : 0;
-    return uint32_t((bucket * kStripsMax + (kStripsMax - 1 - strips)) * kLenSteps + (kLenSteps - 1 - len));
-  };
+  const JobShape shape(rules, np);
+  auto key_of = [&](int2 v) { return job_key(v.x, v.y, rules, shape, critical); };
   for (uint32_t i = t; i < np; i += kJobThreads) atomicAdd(&bins[key_of(nm[i])], 1u);
   __syncthreads();
   block_exclusive_scan<kSortPer>(bins, kSortBins, red);
@@ -952,6 +972,183 @@ __global__ __launch_bounds__(kJobThreads) void edr_jobs_kernel(const int2* __res
   if (t < 4) {
     windows[2 * t] = int32_t(gjobs[t * kStripsMax]);
     windows[2 * t + 1] = int32_t(gjobs[(t + 1) * kStripsMax] - gjobs[t * kStripsMax]);
+  }
+}
+
+// The same job list for large batches, over the whole device: lengths and
+// the batch's cell count and several-wave threshold; global bin totals;
+// their scan, the groups' job counts and the windows (one workgroup); the
+// pairs to their bins (each workgroup counts its contiguous share in LDS,
+// reserves its range of every bin with one atomic per bin, then places its
+// pairs); the jobs.  Scratch after the capacity's (n, m): bin totals, bin
+// cursors, group starts and group job offsets.
+constexpr int kJobsBlock = 256;
+
+__device__ __forceinline__ bool jobs_overflow(const unsigned long long* count, uint32_t cap, uint32_t* np) {
+  const unsigned long long total = *count;
+  *np = total > cap ? 0u : uint32_t(total);
+  return total > cap;
+}
+
+__global__ __launch_bounds__(kJobsBlock) void edr_jobs_lengths_kernel(
+    const int2* __restrict__ pairs, const unsigned long long* __restrict__ count, uint32_t cap,
+    const int32_t* __restrict__ qlen, int32_t nq, const int32_t* __restrict__ clen, int32_t nc, JobRules rules,
+    int2* __restrict__ nm, unsigned long long* __restrict__ cells, unsigned long long* __restrict__ crit) {
+  uint32_t np;
+  if (jobs_overflow(count, cap, &np)) return;
+  const JobShape sh(rules, np);
+  unsigned long long my_cells = 0, my_crit = 0;
+  for (uint32_t i = blockIdx.x * kJobsBlock + threadIdx.x; i < np; i += gridDim.x * kJobsBlock) {
+    const int2 p = pairs[i];
+    const bool ok = p.x >= 0 && p.x < nq && p.y >= 0 && p.y < nc;
+    const int a = ok ? qlen[p.x] : 0, b = ok ? clen[p.y] : 0;
+    const int n = min(a, b), m = max(a, b);
+    nm[i] = make_int2(n, m);
+    my_cells += (unsigned long long)(n) * (unsigned long long)(m);
+    if (sh.multi && m > sh.mid) {
+      const unsigned long long c =
+          (unsigned long long)pipe_blocks((n + 63) / 64, (m + 63 + 15) / 16, rules.waves);
+      my_crit = c > my_crit ? c : my_crit;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    my_cells += __shfl_xor(my_cells, off);
+    const unsigned long long o = __shfl_xor(my_crit, off);
+    my_crit = o > my_crit ? o : my_crit;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (my_cells) atomicAdd(cells, my_cells);
+    if (my_crit) atomicMax(crit, my_crit);
+  }
+}
+
+__device__ __forceinline__ void jobs_share(uint32_t np, uint32_t* lo, uint32_t* hi) {
+  const uint32_t per = (np + gridDim.x - 1) / gridDim.x;
+  *lo = min(np, blockIdx.x * per);
+  *hi = min(np, *lo + per);
+}
+
+__global__ __launch_bounds__(kJobsBlock) void edr_jobs_hist_kernel(const unsigned long long* __restrict__ count,
+                                                                   uint32_t cap, JobRules rules,
+                                                                   const int2* __restrict__ nm,
+                                                                   const unsigned long long* __restrict__ crit,
+                                                                   uint32_t* __restrict__ gbins) {
+  __shared__ uint32_t h[kSortBins];
+  uint32_t np;
+  if (jobs_overflow(count, cap, &np)) return;
+  const JobShape sh(rules, np);
+  const int64_t critical = int64_t(*crit);
+  for (int b = threadIdx.x; b < kSortBins; b += kJobsBlock) h[b] = 0;
+  __syncthreads();
+  uint32_t lo, hi;
+  jobs_share(np, &lo, &hi);
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kJobsBlock) {
+    const int2 v = nm[i];
+    atomicAdd(&h[job_key(v.x, v.y, rules, sh, critical)], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSortBins; b += kJobsBlock)
+    if (h[b]) atomicAdd(gbins + b, h[b]);
+}
+
+__global__ __launch_bounds__(kJobThreads) void edr_jobs_scan_kernel(const unsigned long long* __restrict__ count,
+                                                                   uint32_t cap, const uint32_t* __restrict__ gbins,
+                                                                   uint32_t* __restrict__ gcur,
+                                                                   uint32_t* __restrict__ gstart,
+                                                                   uint32_t* __restrict__ gjobs_out,
+                                                                   int32_t* __restrict__ windows,
+                                                                   int32_t* __restrict__ status) {
+  __shared__ uint32_t bins[kSortBins];
+  __shared__ uint32_t gjobs[kJobGroups + 1];
+  __shared__ uint32_t red[kJobThreads / 64 + 1];
+  const int t = threadIdx.x;
+  uint32_t np;
+  if (jobs_overflow(count, cap, &np)) {
+    if (t < 8) windows[t] = 0;
+    if (t == 0) {
+      atomicOr(status, 1);
+      gjobs_out[kJobGroups] = 0;
+    }
+    return;
+  }
+  for (int b = t; b < kSortBins; b += kJobThreads) bins[b] = gbins[b];
+  __syncthreads();
+  block_exclusive_scan<kSortPer>(bins, kSortBins, red);
+  for (int b = t; b < kSortBins; b += kJobThreads) gcur[b] = bins[b];
+  auto per_job = [](int g) { return g / kStripsMax >= 2 ? 1u : 4u; };
+  auto group_end = [&](int g) { return g + 1 < kJobGroups ? bins[(g + 1) * kLenSteps] : np; };
+  for (int g = t; g < kJobGroups; g += kJobThreads) {
+    const uint32_t first = bins[g * kLenSteps];
+    gstart[g] = first;
+    gjobs[g] = (group_end(g) - first + per_job(g) - 1) / per_job(g);
+  }
+  if (t == 0) gstart[kJobGroups] = np;
+  __syncthreads();
+  const uint32_t njobs = block_exclusive_scan<1>(gjobs, kJobGroups, red);
+  for (int g = t; g < kJobGroups; g += kJobThreads) gjobs_out[g] = gjobs[g];
+  if (t == 0) gjobs_out[kJobGroups] = njobs;
+  if (t < 4) {
+    const uint32_t a = gjobs[t * kStripsMax];
+    const uint32_t b = t == 3 ? njobs : gjobs[(t + 1) * kStripsMax];
+    windows[2 * t] = int32_t(a);
+    windows[2 * t + 1] = int32_t(b - a);
+  }
+}
+
+__global__ __launch_bounds__(kJobsBlock) void edr_jobs_scatter_kernel(
+    const int2* __restrict__ pairs, const unsigned long long* __restrict__ count, uint32_t cap, JobRules rules,
+    const int2* __restrict__ nm, const unsigned long long* __restrict__ crit, uint32_t* __restrict__ gcur,
+    int2* __restrict__ sorted, int32_t* __restrict__ origin) {
+  __shared__ uint32_t h[kSortBins];
+  uint32_t np;
+  if (jobs_overflow(count, cap, &np)) return;
+  const JobShape sh(rules, np);
+  const int64_t critical = int64_t(*crit);
+  for (int b = threadIdx.x; b < kSortBins; b += kJobsBlock) h[b] = 0;
+  __syncthreads();
+  uint32_t lo, hi;
+  jobs_share(np, &lo, &hi);
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kJobsBlock) {
+    const int2 v = nm[i];
+    atomicAdd(&h[job_key(v.x, v.y, rules, sh, critical)], 1u);
+  }
+  __syncthreads();
+  // This workgroup's range of every bin it uses; h becomes its cursors.
+  for (int b = threadIdx.x; b < kSortBins; b += kJobsBlock)
+    if (h[b]) h[b] = atomicAdd(gcur + b, h[b]);
+  __syncthreads();
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kJobsBlock) {
+    const int2 v = nm[i];
+    const uint32_t pos = atomicAdd(&h[job_key(v.x, v.y, rules, sh, critical)], 1u);
+    if (pos < np) {
+      sorted[pos] = pairs[i];
+      if (origin) origin[pos] = int32_t(i);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kJobsBlock) void edr_jobs_write_kernel(const unsigned long long* __restrict__ count,
+                                                                    uint32_t cap, const uint32_t* __restrict__ gstart,
+                                                                    const uint32_t* __restrict__ gjobs_in,
+                                                                    Job* __restrict__ jobs) {
+  __shared__ uint32_t gs[kJobGroups + 1], gj[kJobGroups + 1];
+  uint32_t np;
+  if (jobs_overflow(count, cap, &np)) return;
+  for (int g = threadIdx.x; g <= kJobGroups; g += kJobsBlock) {
+    gs[g] = gstart[g];
+    gj[g] = gjobs_in[g];
+  }
+  __syncthreads();
+  const uint32_t njobs = gj[kJobGroups];
+  for (uint32_t j = blockIdx.x * kJobsBlock + threadIdx.x; j < njobs; j += gridDim.x * kJobsBlock) {
+    int lo = 0, hi = kJobGroups - 1;  // the last group whose first job is at or below j
+    while (lo < hi) {
+      const int mid_g = (lo + hi + 1) / 2;
+      if (gj[mid_g] <= j) lo = mid_g; else hi = mid_g - 1;
+    }
+    const uint32_t per = lo / kStripsMax >= 2 ? 1u : 4u;
+    const uint32_t first = gs[lo] + (j - gj[lo]) * per;
+    jobs[j] = Job{int32_t(first), int32_t(min(per, gs[lo + 1] - first))};
   }
 }
 
@@ -1242,11 +1439,47 @@ extern "C" int gat_edr_jobs(const int32_t* pairs, const uint64_t* count, int64_t
               "EDR jobs: null pointer");
   if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 1, stream)) return rc;
   const JobRules rules{split, waves, every_long ? 1 : 0, int32_t(gat::cu_count())};
-  edr_jobs_kernel<<<dim3(1), dim3(kJobThreads), 0, gat::as_stream(stream)>>>(
-      reinterpret_cast<const int2*>(pairs), reinterpret_cast<const unsigned long long*>(count), uint32_t(capacity),
-      qlen, nq, clen, nc, rules, reinterpret_cast<int2*>(sorted), origin, reinterpret_cast<int2*>(scratch),
-      reinterpret_cast<Job*>(jobs), windows, status, reinterpret_cast<unsigned long long*>(cells));
-  GAT_LAUNCH_CHECK("edr_jobs_kernel");
+  hipStream_t s = gat::as_stream(stream);
+  const auto* P = reinterpret_cast<const int2*>(pairs);
+  const auto* C = reinterpret_cast<const unsigned long long*>(count);
+  auto* NM = reinterpret_cast<int2*>(scratch);
+  auto* cells_u = reinterpret_cast<unsigned long long*>(cells);
+  if (capacity <= kJobsOneBlock) {
+    edr_jobs_kernel<<<dim3(1), dim3(kJobThreads), 0, s>>>(P, C, uint32_t(capacity), qlen, nq, clen, nc, rules,
+                                                          reinterpret_cast<int2*>(sorted), origin, NM,
+                                                          reinterpret_cast<Job*>(jobs), windows, status, cells_u);
+    GAT_LAUNCH_CHECK("edr_jobs_kernel");
+    return GAT_OK;
+  }
+  // Scratch after the (n, m) pairs (8-byte aligned: 2 * capacity words): the
+  // several-wave threshold, bin totals, bin cursors, group starts, group job
+  // offsets.
+  static_assert(2 + 2 * kSortBins + 2 * (kJobGroups + 1) <= kJobsScratchExtra, "job scratch");
+  auto* crit = reinterpret_cast<unsigned long long*>(scratch + 2 * capacity);
+  uint32_t* gbins = reinterpret_cast<uint32_t*>(scratch + 2 * capacity + 2);
+  uint32_t* gcur = gbins + kSortBins;
+  uint32_t* gstart = gcur + kSortBins;
+  uint32_t* gjobs = gstart + kJobGroups + 1;
+  if (int rc = gat_fill_u32(gbins, 0u, kSortBins, stream)) return rc;
+  if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(cells), 0u, 2, stream)) return rc;
+  if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(crit), 0u, 2, stream)) return rc;
+  int64_t blocks = (capacity + 4095) / 4096;
+  const int64_t most = int64_t(gat::cu_count()) * 4;
+  if (blocks > most) blocks = most;
+  if (blocks < 1) blocks = 1;
+  const dim3 grid{uint32_t(blocks)}, block{kJobsBlock};
+  const uint32_t cap = uint32_t(capacity);
+  edr_jobs_lengths_kernel<<<grid, block, 0, s>>>(P, C, cap, qlen, nq, clen, nc, rules, NM, cells_u, crit);
+  GAT_LAUNCH_CHECK("edr_jobs_lengths_kernel");
+  edr_jobs_hist_kernel<<<grid, block, 0, s>>>(C, cap, rules, NM, crit, gbins);
+  GAT_LAUNCH_CHECK("edr_jobs_hist_kernel");
+  edr_jobs_scan_kernel<<<dim3(1), dim3(kJobThreads), 0, s>>>(C, cap, gbins, gcur, gstart, gjobs, windows, status);
+  GAT_LAUNCH_CHECK("edr_jobs_scan_kernel");
+  edr_jobs_scatter_kernel<<<grid, block, 0, s>>>(P, C, cap, rules, NM, crit, gcur, reinterpret_cast<int2*>(sorted),
+                                                 origin);
+  GAT_LAUNCH_CHECK("edr_jobs_scatter_kernel");
+  edr_jobs_write_kernel<<<grid, block, 0, s>>>(C, cap, gstart, gjobs, reinterpret_cast<Job*>(jobs));
+  GAT_LAUNCH_CHECK("edr_jobs_write_kernel");
   return GAT_OK;
 }
 

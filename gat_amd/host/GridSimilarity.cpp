@@ -590,7 +590,7 @@ int Grid::runBatchedOnDevice(const int32_t* pairsHost, int64_t np, int queryNum,
   const int waves = (edrWavesPerPair == 2 || edrWavesPerPair == 4 || edrWavesPerPair == 8) ? edrWavesPerPair : 1;
   if (int rc = pairsDev_.reserve(size_t(np) * 8)) return rc;
   if (int rc = sortedDev_.reserve(size_t(np) * 8)) return rc;
-  if (int rc = nmDev_.reserve(size_t(np) * 8)) return rc;
+  if (int rc = nmDev_.reserve((size_t(np) * 2 + GAT_EDR_JOBS_SCRATCH_EXTRA) * 4)) return rc;
   if (int rc = originDev_.reserve(size_t(np) * 4)) return rc;
   if (int rc = jobsDev_.reserve((size_t(np) + 4) * sizeof(gat_edr_job))) return rc;
   if (int rc = outDev_.reserve(size_t(np) * 4)) return rc;
@@ -674,7 +674,7 @@ int Grid::similarityOnDevice(int nq, int kValue, std::vector<TopK>& out, EdrBatc
     const size_t ns = size_t(nq) + 1;
     if (int rc = pairsDev_.reserve(size_t(cap) * 8)) return rc;
     if (int rc = sortedDev_.reserve(size_t(cap) * 8)) return rc;
-    if (int rc = nmDev_.reserve(size_t(cap) * 8)) return rc;
+    if (int rc = nmDev_.reserve((size_t(cap) * 2 + GAT_EDR_JOBS_SCRATCH_EXTRA) * 4)) return rc;
     if (int rc = jobsDev_.reserve((size_t(cap) + 4) * sizeof(gat_edr_job))) return rc;
     if (int rc = outDev_.reserve(size_t(cap) * 4)) return rc;
     if (int rc = survDev_.reserve((ns + 2 * size_t(cap)) * 4)) return rc;

@@ -303,10 +303,12 @@ int gat_edr_batched(const int32_t* pairs_dev, int64_t npairs, const gat_edr_job*
  * up to four pairs of one strip count (one pair in the two long buckets) and
  * windows_dev[2b], [2b+1] the first job and job count of bucket b (four
  * buckets), for gat_edr_batched's windows; origin_dev (may be null; capacity
- * entries) gets each sorted pair's index in pairs_dev.  scratch_dev: capacity * 2 int32;
+ * entries) gets each sorted pair's index in pairs_dev.  scratch_dev: capacity * 2
+ * + GAT_EDR_JOBS_SCRATCH_EXTRA int32;
  * *cells_dev = the sum of the pairs' DP cells.  When *count_dev exceeds the
  * capacity nothing is sorted, every window is empty and status bit 1 is set
  * (*status_dev is cleared first).  Asynchronous on `stream`. */
+#define GAT_EDR_JOBS_SCRATCH_EXTRA 32768
 int gat_edr_jobs(const int32_t* pairs_dev, const uint64_t* count_dev, int64_t capacity, const int32_t* query_length_dev,
                  int32_t nqueries, const int32_t* cand_length_dev, int32_t ncands, int32_t split, int32_t waves,
                  int32_t every_long, int32_t* sorted_dev, int32_t* origin_dev, int32_t* scratch_dev,

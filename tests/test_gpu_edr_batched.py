@@ -287,3 +287,40 @@ def test_topk_device_chain_falls_back_for_long_lists(gpu):
     for i in range(q.ntraj):
         oi, od = O.topk(full[i], tid, 10)
         assert (ids[i] == oi).all() and (dists[i] == od).all(), i
+
+
+def test_device_jobs_large_batches_match_host_built_path(gpu):
+    """Batches above 131,072 pairs build their job list over the whole device
+    (lengths, bin totals, one-workgroup scan, per-workgroup placement, jobs)
+    instead of in one workgroup: the all-pairs, given-pairs and top-k forms
+    give the host-built path's distances, and sampled rows the oracle's."""
+    rng = np.random.default_rng(41)
+    lengths = list(rng.integers(1, 400, 2400)) + [513, 1024, 1500, 2048]
+    db = _varied_db(41, np.array(lengths, np.int32))
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    q = api.synth_queries(42, np.array(list(rng.integers(1, 600, 58)) + [1024, 2048], np.int32), db, copy_mode=2)
+    assert q.ntraj * db.ntraj > 131072
+    dist, pairs_dist = {}, {}
+    pairs = np.stack([rng.integers(0, q.ntraj, 140000), rng.integers(0, db.ntraj, 140000)], 1).astype(np.int32)
+    for chain in (True, False):
+        g.set_topk_device_chain(chain)
+        dist[chain], st = g.edr_batched_all(q, EPS)
+        assert st["pairs"] == q.ntraj * db.ntraj
+        pairs_dist[chain], _ = g.edr_batched_pairs(q, pairs, EPS)
+    assert (dist[True] == dist[False]).all()
+    assert (pairs_dist[True] == pairs_dist[False]).all()
+    assert (pairs_dist[True] == dist[True][pairs[:, 0], pairs[:, 1]]).all()
+    for i in (0, 31, q.ntraj - 1):
+        assert (dist[True][i] == O.edr_batch(q.xy(i + 1), db, EPS, nthreads=THREADS)).all(), i
+    # Top-k with a chain capacity above the bound (64 per query, 2,100 queries).
+    small = api.synth_walks(43, rng.integers(8, 80, 300).astype(np.int32), step_sigma=2e-4)
+    gs = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(small)
+    qs = api.synth_queries(44, rng.integers(8, 64, 2100).astype(np.int32), small, copy_mode=2)
+    got = {}
+    for chain in (True, False):
+        gs.set_topk_device_chain(chain)
+        got[chain] = gs.similarity_batched(qs, 10, EPS)
+    assert (got[True][0] == got[False][0]).all() and (got[True][1] == got[False][1]).all()
+    assert got[True][2]["pairs"] == got[False][2]["pairs"]
+    g.set_topk_device_chain(True)
+    gs.set_topk_device_chain(True)
