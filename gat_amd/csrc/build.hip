@@ -8,7 +8,7 @@
 //                      Morton interleave (Grid.cpp:25-54) or row * columns +
 //                      column (FSG.cpp:90-92);
 //   tid_kernel         trajectory id of every point (one wave per trajectory);
-//   radix sort         stable, by cell, of the point indices (rocPRIM): points
+//   radix sort         stable, by cell, of the point indices: points
 //                      of one cell keep input order, i.e. trajectory id, then
 //                      position in the trajectory — the reference's order of
 //                      sub-trajectories within a cell (Grid.cpp:242-261);
@@ -40,8 +40,6 @@
 // gather_kernel the store in the final order.
 #include <algorithm>
 #include <utility>
-
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "gat_common.hpp"
 
@@ -218,6 +216,251 @@ int blocks_for(uint64_t n, int per_cu) {
   return int(b == 0 ? 1 : b);
 }
 
+
+// Stable LSD radix sort of (key, value) pairs, eight bits of the key per pass
+// from bit 0 up to end_bit.  A pass is three steps over tiles of 4,096 pairs:
+//   radix_count_kernel    each tile's digit counts (LDS histogram), stored
+//                         digit-major: counts[d * ntiles + tile];
+//   scan                  their exclusive scan in that order, so offsets[d *
+//                         ntiles + tile] is where the tile's pairs of digit d
+//                         start (every smaller digit first, then the earlier
+//                         tiles' pairs of digit d);
+//   radix_scatter_kernel  the tile again, in 16 rounds of 256 pairs: a pair's
+//                         rank among the round's pairs of its digit comes from
+//                         eight ballots (lanes of the same digit) and the
+//                         earlier waves' and rounds' counts, which orders the
+//                         pairs of one digit by input position — the sort is
+//                         stable; the tile is staged in LDS grouped by digit
+//                         and written out in that order, so each digit's pairs
+//                         go out as one contiguous run.
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+constexpr int kRadix = 256;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 8;
+constexpr int kScanSpan = kScanThreads * kScanPer;
+constexpr int kScanTopPer = 32;  // the block sums of up to 2^28 counts
+
+template <typename K>
+__device__ __forceinline__ uint32_t digit_of(K key, int shift, int bits) {
+  return uint32_t(key >> shift) & ((1u << bits) - 1u);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void radix_count_kernel(const K* __restrict__ keys, uint64_t n, int shift,
+                                                                   int bits, uint32_t ntiles,
+                                                                   uint32_t* __restrict__ counts) {
+  __shared__ uint32_t h[kRadix];
+  const uint32_t t = threadIdx.x, tile = blockIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  const uint64_t base = uint64_t(tile) * kSortTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortRounds; ++r) {
+    const uint64_t i = base + uint64_t(r) * kSortThreads + t;
+    if (i < n) atomicAdd(&h[digit_of(keys[i], shift, bits)], 1u);
+  }
+  __syncthreads();
+  counts[uint64_t(t) * ntiles + tile] = h[t];
+}
+
+// Exclusive scan of the workgroup's values (PER per thread, in thread order)
+// in place in a; returns the total.  red: one word per wave plus one.
+template <int THREADS, int PER>
+__device__ uint32_t block_scan(uint32_t (&v)[PER], uint32_t* red) {
+  const uint32_t t = threadIdx.x, lane = t & 63;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) sum += v[k];
+  uint32_t incl = sum;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = uint32_t(__shfl_up(int(incl), d, 64));
+    if (lane >= uint32_t(d)) incl += y;
+  }
+  if (lane == 63) red[t / 64] = incl;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < THREADS / 64; ++w) {
+      const uint32_t x = red[w];
+      red[w] = run;
+      run += x;
+    }
+    red[THREADS / 64] = run;
+  }
+  __syncthreads();
+  uint32_t at = red[t / 64] + incl - sum;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t x = v[k];
+    v[k] = at;
+    at += x;
+  }
+  const uint32_t total = red[THREADS / 64];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void radix_scan_sums_kernel(const uint32_t* __restrict__ in, uint64_t m,
+                                                                       uint32_t* __restrict__ sums) {
+  __shared__ uint32_t red[kScanThreads / 64 + 1];
+  const uint64_t first = uint64_t(blockIdx.x) * kScanSpan + uint64_t(threadIdx.x) * kScanPer;
+  uint32_t v[kScanPer];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) v[k] = first + k < m ? in[first + k] : 0u;
+  const uint32_t total = block_scan<kScanThreads, kScanPer>(v, red);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void radix_scan_top_kernel(uint32_t* __restrict__ sums, uint32_t nblocks) {
+  __shared__ uint32_t red[kScanThreads / 64 + 1];
+  const uint32_t first = threadIdx.x * kScanTopPer;
+  uint32_t v[kScanTopPer];
+#pragma unroll
+  for (int k = 0; k < kScanTopPer; ++k) v[k] = first + k < nblocks ? sums[first + k] : 0u;
+  block_scan<kScanThreads, kScanTopPer>(v, red);
+#pragma unroll
+  for (int k = 0; k < kScanTopPer; ++k)
+    if (first + k < nblocks) sums[first + k] = v[k];
+}
+
+__global__ __launch_bounds__(kScanThreads) void radix_scan_apply_kernel(const uint32_t* __restrict__ in, uint64_t m,
+                                                                        const uint32_t* __restrict__ sums,
+                                                                        uint32_t* __restrict__ out) {
+  __shared__ uint32_t red[kScanThreads / 64 + 1];
+  const uint64_t first = uint64_t(blockIdx.x) * kScanSpan + uint64_t(threadIdx.x) * kScanPer;
+  uint32_t v[kScanPer];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) v[k] = first + k < m ? in[first + k] : 0u;
+  block_scan<kScanThreads, kScanPer>(v, red);
+  const uint32_t carry = sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k)
+    if (first + k < m) out[first + k] = carry + v[k];
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, uint64_t n, int shift, int bits, uint32_t ntiles,
+    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets) {
+  __shared__ K skey[kSortTile];
+  __shared__ uint32_t sval[kSortTile];
+  __shared__ uint32_t start[kRadix], gofs[kRadix], run[kRadix];
+  __shared__ uint32_t wcnt[kSortWaves][kRadix], wpre[kSortWaves][kRadix];
+  __shared__ uint32_t red[kSortThreads / 64 + 1];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6, tile = blockIdx.x;
+  const uint64_t base = uint64_t(tile) * kSortTile;
+  const uint32_t tile_n = uint32_t(n - base < uint64_t(kSortTile) ? n - base : uint64_t(kSortTile));
+  uint32_t c[1] = {counts[uint64_t(t) * ntiles + tile]};
+  gofs[t] = offsets[uint64_t(t) * ntiles + tile];
+  block_scan<kSortThreads, 1>(c, red);
+  start[t] = c[0];
+  run[t] = 0;
+  for (int k = 0; k < kSortWaves; ++k) wcnt[k][t] = 0;
+  __syncthreads();
+  const uint64_t lt = (uint64_t(1) << lane) - 1u;
+  // The whole tile is loaded before the first round: a round's loads would
+  // otherwise wait behind the previous round's barriers, one memory round
+  // trip per round.
+  K kr[kSortRounds];
+  uint32_t vr[kSortRounds];
+#pragma unroll
+  for (int r = 0; r < kSortRounds; ++r) {
+    const uint32_t j = uint32_t(r) * kSortThreads + t;
+    kr[r] = j < tile_n ? keys_in[base + j] : K(0);
+    vr[r] = j < tile_n ? vals_in[base + j] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortRounds; ++r) {
+    const uint32_t j = uint32_t(r) * kSortThreads + t;
+    const bool valid = j < tile_n;
+    const K key = kr[r];
+    const uint32_t val = vr[r];
+    const uint32_t d = digit_of(key, shift, bits);
+    uint64_t m = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool set = (d >> b) & 1u;
+      const uint64_t bb = __ballot(set);
+      m &= set ? bb : ~bb;
+    }
+    const uint32_t below = uint32_t(__popcll(m & lt));
+    if (valid && below == 0) wcnt[w][d] = uint32_t(__popcll(m));
+    __syncthreads();
+    // Thread t = digit t: where each wave's pairs of digit t start this round.
+    uint32_t acc = run[t];
+#pragma unroll
+    for (int k = 0; k < kSortWaves; ++k) {
+      const uint32_t x = wcnt[k][t];
+      wpre[k][t] = acc;
+      wcnt[k][t] = 0;
+      acc += x;
+    }
+    run[t] = acc;
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = start[d] + wpre[w][d] + below;
+      skey[pos] = key;
+      sval[pos] = val;
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = t; j < tile_n; j += kSortThreads) {
+    const K key = skey[j];
+    const uint32_t d = digit_of(key, shift, bits);
+    const uint64_t at = uint64_t(gofs[d]) + (j - start[d]);
+    keys_out[at] = key;
+    vals_out[at] = sval[j];
+  }
+}
+
+// Device bytes radix_sort_pairs needs beside the two key and value buffers.
+size_t radix_scratch_bytes(uint64_t n) {
+  const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  const uint64_t m = ntiles * kRadix;
+  const uint64_t nblocks = (m + kScanSpan - 1) / kScanSpan;
+  return size_t((2 * m + nblocks) * 4 + 256);
+}
+
+// Sorts n pairs by bits [0, end_bit) of the key, stably, ping-ponging between
+// (*keys, *vals) and (*keys_alt, *vals_alt); on return *keys / *vals point to
+// the sorted pairs (the other pair of buffers holds whatever the last pass
+// left).  Asynchronous on s.
+template <typename K>
+int radix_sort_pairs(K** keys, K** keys_alt, uint32_t** vals, uint32_t** vals_alt, uint64_t n, int end_bit,
+                     void* scratch, hipStream_t s) {
+  GAT_REQUIRE(n < (uint64_t(1) << 32), "radix sort: 2^32 or more pairs");
+  GAT_REQUIRE(end_bit >= 0 && end_bit <= int(8 * sizeof(K)), "radix sort: bad bit count");
+  if (n == 0 || end_bit == 0) return GAT_OK;
+  const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  const uint64_t m = ntiles * kRadix;
+  const uint64_t nblocks = (m + kScanSpan - 1) / kScanSpan;
+  GAT_REQUIRE(nblocks <= uint64_t(kScanThreads) * kScanTopPer, "radix sort: too many pairs for the scan");
+  auto* counts = static_cast<uint32_t*>(scratch);
+  uint32_t* offsets = counts + m;
+  uint32_t* sums = offsets + m;
+  for (int shift = 0; shift < end_bit; shift += 8) {
+    const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
+    radix_count_kernel<K><<<dim3(uint32_t(ntiles)), dim3(kSortThreads), 0, s>>>(*keys, n, shift, bits,
+                                                                                uint32_t(ntiles), counts);
+    GAT_LAUNCH_CHECK("radix_count_kernel");
+    radix_scan_sums_kernel<<<dim3(uint32_t(nblocks)), dim3(kScanThreads), 0, s>>>(counts, m, sums);
+    GAT_LAUNCH_CHECK("radix_scan_sums_kernel");
+    radix_scan_top_kernel<<<dim3(1), dim3(kScanThreads), 0, s>>>(sums, uint32_t(nblocks));
+    GAT_LAUNCH_CHECK("radix_scan_top_kernel");
+    radix_scan_apply_kernel<<<dim3(uint32_t(nblocks)), dim3(kScanThreads), 0, s>>>(counts, m, sums, offsets);
+    GAT_LAUNCH_CHECK("radix_scan_apply_kernel");
+    radix_scatter_kernel<K><<<dim3(uint32_t(ntiles)), dim3(kSortThreads), 0, s>>>(
+        *keys, *vals, *keys_alt, *vals_alt, n, shift, bits, uint32_t(ntiles), counts, offsets);
+    GAT_LAUNCH_CHECK("radix_scatter_kernel");
+    std::swap(*keys, *keys_alt);
+    std::swap(*vals, *vals_alt);
+  }
+  return GAT_OK;
+}
+
 }  // namespace
 
 extern "C" int gat_cell_store_build(const gat_cell_grid* grid, const float* lon, const float* lat,
@@ -248,25 +491,22 @@ extern "C" int gat_cell_store_build(const gat_cell_grid* grid, const float* lon,
   // trajectory id of every point, the off-grid flag, then the sort's storage.
   int end_bit = 1;
   while (end_bit < 32 && (uint64_t(1) << end_bit) < cellnum) ++end_bit;
-  size_t sort_bytes = 0;
-  GAT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, sort_bytes, static_cast<const uint32_t*>(nullptr),
-                                          static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
-                                          static_cast<uint32_t*>(nullptr), size_t(n), 0, end_bit, s));
+  const size_t sort_bytes = radix_scratch_bytes(n);
   const size_t arr = (n * 4 + 255) & ~size_t(255);
   const size_t total = 5 * arr + 256 + sort_bytes;
   Scratch scratch;
   GAT_HIP_CHECK(hipMalloc(&scratch.p, total));
   char* base = static_cast<char*>(scratch.p);
-  auto* keys_in = reinterpret_cast<uint32_t*>(base);
-  auto* keys_out = reinterpret_cast<uint32_t*>(base + arr);
-  auto* vals_in = reinterpret_cast<uint32_t*>(base + 2 * arr);
-  auto* vals_out = reinterpret_cast<uint32_t*>(base + 3 * arr);
+  auto* keys = reinterpret_cast<uint32_t*>(base);
+  auto* keys_alt = reinterpret_cast<uint32_t*>(base + arr);
+  auto* vals = reinterpret_cast<uint32_t*>(base + 2 * arr);
+  auto* vals_alt = reinterpret_cast<uint32_t*>(base + 3 * arr);
   auto* tid = reinterpret_cast<uint32_t*>(base + 4 * arr);
   auto* bad = reinterpret_cast<unsigned long long*>(base + 5 * arr);
   void* sort_tmp = base + 5 * arr + 256;
 
   GAT_HIP_CHECK(hipMemsetAsync(bad, 0xFF, sizeof(unsigned long long), s));
-  cell_kernel<<<dim3(uint32_t(blocks_for(n, 16))), dim3(kBlock), 0, s>>>(g, lon, lat, n, keys_in, vals_in, bad);
+  cell_kernel<<<dim3(uint32_t(blocks_for(n, 16))), dim3(kBlock), 0, s>>>(g, lon, lat, n, keys, vals, bad);
   GAT_LAUNCH_CHECK("cell_kernel");
   GAT_HIP_CHECK(hipMemsetAsync(tid, 0, n * 4, s));
   tid_kernel<<<dim3(uint32_t(blocks_for(uint64_t(ntraj) * 64, 16))), dim3(kBlock), 0, s>>>(offsets, uint32_t(ntraj),
@@ -279,13 +519,12 @@ extern "C" int gat_cell_store_build(const gat_cell_grid* grid, const float* lon,
     *bad_point = int64_t(first_bad);
     return GAT_OK;
   }
-  GAT_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, vals_in, vals_out, size_t(n), 0,
-                                          end_bit, s));
+  if (int rc = radix_sort_pairs<uint32_t>(&keys, &keys_alt, &vals, &vals_alt, n, end_bit, sort_tmp, s)) return rc;
   gather_kernel<<<dim3(uint32_t(blocks_for(n, 16))), dim3(kBlock), 0, s>>>(
-      lon, lat, offsets, tid, vals_out, n, uint32_t(ntraj), uint32_t(pid_bits), reinterpret_cast<float2*>(xy_out),
+      lon, lat, offsets, tid, vals, n, uint32_t(ntraj), uint32_t(pid_bits), reinterpret_cast<float2*>(xy_out),
       id_out, pid_out);
   GAT_LAUNCH_CHECK("gather_kernel");
-  cell_start_kernel<<<dim3(uint32_t(blocks_for(n, 16))), dim3(kBlock), 0, s>>>(keys_out, n, cellnum,
+  cell_start_kernel<<<dim3(uint32_t(blocks_for(n, 16))), dim3(kBlock), 0, s>>>(keys, n, cellnum,
                                                                                cell_start_out);
   GAT_LAUNCH_CHECK("cell_start_kernel");
   // The scratch block is freed on return, after the stream has drained.
@@ -321,15 +560,7 @@ extern "C" int gat_kd_store_build(const float* lon, const float* lat, const int6
     GAT_REQUIRE(leaf_lo[k] >= 0 && leaf_lo[k] <= leaf_hi[k] && leaf_hi[k] < npoints,
                 "kd store build: leaf outside the points");
   hipStream_t s = gat::as_stream(stream);
-  int max_rank_bits = 1;
-  for (int32_t d = 0; d < depths; ++d) {
-    const uint64_t top = 2 * uint64_t(depth_first[d + 1] - depth_first[d]) + 1;
-    while (max_rank_bits < 31 && (uint64_t(1) << max_rank_bits) <= top) ++max_rank_bits;
-  }
-  size_t sort_bytes = 0;
-  GAT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, sort_bytes, static_cast<const uint64_t*>(nullptr),
-                                          static_cast<uint64_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
-                                          static_cast<uint32_t*>(nullptr), size_t(n), 0, 32 + max_rank_bits, s));
+  const size_t sort_bytes = radix_scratch_bytes(n);
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t k8 = al(n * 8), k4 = al(n * 4);
   const size_t sp = al(size_t(std::max<int64_t>(nsplit, 1)) * 8), lf = al(size_t(nleaves) * 8);
@@ -372,9 +603,8 @@ extern "C" int gat_kd_store_build(const float* lon, const float* lat, const int6
     GAT_LAUNCH_CHECK("kd_key_kernel");
     int bits = 1;
     while (bits < 31 && (uint64_t(1) << bits) <= uint64_t(2 * cnt + 1)) ++bits;
-    GAT_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_a, keys_b, perm_a, perm_b, size_t(n), 0,
-                                            32 + bits, s));
-    std::swap(perm_a, perm_b);
+    if (int rc = radix_sort_pairs<uint64_t>(&keys_a, &keys_b, &perm_a, &perm_b, n, 32 + bits, sort_tmp, s))
+      return rc;
     kd_median_kernel<<<dim3(uint32_t((cnt + kBlock - 1) / kBlock)), block, 0, s>>>(
         lon, lat, perm_a, dlo + first, dhi + first, uint32_t(cnt), dim, dmed + first);
     GAT_LAUNCH_CHECK("kd_median_kernel");

@@ -1194,16 +1194,18 @@ __device__ __forceinline__ void single_chunk_item(const Store& store, uint32_t s
 // load and store latency, and the eighth wave shortened the cfg2 refine from
 // 1.81 to 1.74 ms; loading the next item's inputs ahead of the current item's
 // stores instead needs about 100 VGPRs and ran slower (1.93 ms).
+constexpr int kRunsBlock = 256;
+
 template <bool kPacked>
-__global__ __launch_bounds__(kRefBlock, 8) void leaf_refine_runs_kernel(
+__global__ __launch_bounds__(kRunsBlock, 8) void leaf_refine_runs_kernel(
     Store store, const Item* __restrict__ items, const uint32_t* __restrict__ item_total, uint64_t items_cap,
     const uint32_t* __restrict__ qlist, uint64_t qlist_cap, const float4* __restrict__ boxes, uint32_t nqueries,
     int2* __restrict__ pairs, uint64_t capacity, RunOut o) {
-  __shared__ int4 rings[kRefBlock / kWave][kRing / 2];
+  __shared__ int4 rings[kRunsBlock / kWave][kRing / 2];
   const uint32_t lane = threadIdx.x & (kWave - 1);
   int2* ring = reinterpret_cast<int2*>(rings[threadIdx.x / kWave]);
-  const uint64_t wave = uniform((blockIdx.x * kRefBlock + threadIdx.x) / kWave);
-  const uint64_t nwaves = uint64_t(gridDim.x) * (kRefBlock / kWave);
+  const uint64_t wave = uniform((blockIdx.x * kRunsBlock + threadIdx.x) / kWave);
+  const uint64_t nwaves = uint64_t(gridDim.x) * (kRunsBlock / kWave);
   const uint64_t total = *item_total;
   const uint64_t nitems = total < items_cap ? total : items_cap;
   const float nan = __builtin_nanf("");
@@ -1604,8 +1606,8 @@ int gat_leaf_refine_runs(const gat_point_store* sp, const uint32_t* items, const
   // CU left each wave a fixed share of items and the launch waited on the
   // slowest shares (1.40 ms per refine against 1.26 ms; 64 per CU: 1.32,
   // 128: 1.28, 4096: 1.26).
-  const int blocks = grid_for(uint64_t(items_capacity), kRefBlock / kWave, 256);
-  const dim3 grid{uint32_t(blocks)}, block{kRefBlock};
+  const int blocks = grid_for(uint64_t(items_capacity), kRunsBlock / kWave, 256 * 256 / kRunsBlock);
+  const dim3 grid{uint32_t(blocks)}, block{kRunsBlock};
   auto* I = reinterpret_cast<const Item*>(items);
   auto* B = reinterpret_cast<const float4*>(boxes);
   auto* P = reinterpret_cast<int2*>(pairs);
