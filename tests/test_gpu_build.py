@@ -172,38 +172,43 @@ def test_stig_device_build(gpu, block):
     inclusive tests rely on, STIG.cpp:179-252), deterministic, and answers
     equal to inclusive brute force on both query forms."""
     for name, db in _dbs():
-        host = api.StigIndex(block).build(db)
-        dev = api.StigIndex(block).build(db, device=True)
-        hn, hd = host.tree_nodes()
-        dn, dd = dev.tree_nodes()
-        assert hd == dd and len(hn) == len(dn), name
-        for f in ("first_child", "start", "count", "flags"):
-            assert (hn[f] == dn[f]).all(), (name, f)
-        xy, ids, pid, bits = dev.fetch_store()
-        t, p = _ids(ids, pid, bits)
-        tids = np.repeat(np.arange(1, db.ntraj + 1), np.diff(db.offsets))
-        pids = np.arange(db.npoints) - np.repeat(db.offsets[:-1], np.diff(db.offsets))
-        order = np.lexsort((p, t))
-        assert (t[order] == tids).all() and (p[order] == pids).all(), name
-        # The coordinates travel with their ids.
-        src = db.offsets[t - 1] + p
-        assert (xy[:, 0] == db.lon[src]).all() and (xy[:, 1] == db.lat[src]).all(), name
-        lo, hi = _subtree_ranges(dn)
-        for i, nd in enumerate(dn):
-            x, y = xy[lo[i]:hi[i], 0], xy[lo[i]:hi[i], 1]
-            b = nd["box"]
-            if nd["first_child"] < 0:
-                assert list(b) == [x.min(), y.min(), x.max(), y.max()], (name, i)
-            else:
-                assert (x >= b[0]).all() and (x <= b[2]).all() and (y >= b[1]).all() and (y <= b[3]).all(), (name, i)
-        again = api.StigIndex(block).build(db, device=True).fetch_store()
-        assert (again[0].view(np.uint32) == xy.view(np.uint32)).all() and (again[1] == ids).all(), name
+        _check_stig(db, block, name)
+
+
+def _check_stig(db, block, name, rects=None):
+    host = api.StigIndex(block).build(db)
+    dev = api.StigIndex(block).build(db, device=True)
+    hn, hd = host.tree_nodes()
+    dn, dd = dev.tree_nodes()
+    assert hd == dd and len(hn) == len(dn), name
+    for f in ("first_child", "start", "count", "flags"):
+        assert (hn[f] == dn[f]).all(), (name, f)
+    xy, ids, pid, bits = dev.fetch_store()
+    t, p = _ids(ids, pid, bits)
+    tids = np.repeat(np.arange(1, db.ntraj + 1), np.diff(db.offsets))
+    pids = np.arange(db.npoints) - np.repeat(db.offsets[:-1], np.diff(db.offsets))
+    order = np.lexsort((p, t))
+    assert (t[order] == tids).all() and (p[order] == pids).all(), name
+    # The coordinates travel with their ids.
+    src = db.offsets[t - 1] + p
+    assert (xy[:, 0] == db.lon[src]).all() and (xy[:, 1] == db.lat[src]).all(), name
+    lo, hi = _subtree_ranges(dn)
+    for i, nd in enumerate(dn):
+        x, y = xy[lo[i]:hi[i], 0], xy[lo[i]:hi[i], 1]
+        b = nd["box"]
+        if nd["first_child"] < 0:
+            assert list(b) == [x.min(), y.min(), x.max(), y.max()], (name, i)
+        else:
+            assert (x >= b[0]).all() and (x <= b[2]).all() and (y >= b[1]).all() and (y <= b[3]).all(), (name, i)
+    again = api.StigIndex(block).build(db, device=True).fetch_store()
+    assert (again[0].view(np.uint32) == xy.view(np.uint32)).all() and (again[1] == ids).all(), name
+    if rects is None:
         rects = np.concatenate([api.synth_rects(5, 60, 1e-3, 0.3, True), _edge_rects(db, 1 / 16, 30, 6)])
-        counts, dig = O.brute_range(db, rects, "digest")
-        # A result reads its index's last batch: check each before the next call.
-        for run in (lambda: dev.range_pairs(rects), lambda: dev.set_batch_params(32, 0).range_batched(rects)):
-            res = run()
-            assert (res.counts() == counts).all() and (res.digests() == dig).all(), name
+    counts, dig = O.brute_range(db, rects, "digest")
+    # A result reads its index's last batch: check each before the next call.
+    for run in (lambda: dev.range_pairs(rects), lambda: dev.set_batch_params(32, 0).range_batched(rects)):
+        res = run()
+        assert (res.counts() == counts).all() and (res.digests() == dig).all(), name
 
 
 def test_stig_device_build_root_leaf_and_cfg1(gpu):
@@ -221,3 +226,42 @@ def test_stig_device_build_root_leaf_and_cfg1(gpu):
     b = dev.range_batched(rects)
     assert (b.counts() == ca).all() and (b.digests() == da).all()
     _check_pairs(dev.range_pairs(rects), _sets(O.brute_range(db, rects)), rects)
+
+
+def _sized(n, seed, spot):
+    """n points in trajectories of 1 to 12 points: spread over the unit
+    square, or all inside one 1/1024 cell (every sort key equal)."""
+    rng = np.random.default_rng(seed)
+    lens = []
+    while sum(lens) < n:
+        lens.append(int(min(rng.integers(1, 13), n - sum(lens))))
+    offsets = np.zeros(len(lens) + 1, np.int64)
+    offsets[1:] = np.cumsum(lens)
+    if spot:
+        lon = (np.float32(0.3001) + rng.random(n).astype(np.float32) * np.float32(5e-4)).astype(np.float32)
+        lat = (np.float32(0.6001) + rng.random(n).astype(np.float32) * np.float32(5e-4)).astype(np.float32)
+    else:
+        lon, lat = rng.random(n).astype(np.float32), rng.random(n).astype(np.float32)
+    return api.TrajSet(offsets, lon, lat)
+
+
+@pytest.mark.parametrize("n", [1, 4095, 4096, 4097, 12289])
+def test_device_sort_tile_edges(gpu, n):
+    """The builds' radix sort works in tiles of 4,096 pairs: sizes just below,
+    at and above a tile, a single pair, and a dataset whose points all share
+    one cell (one digit everywhere, so every pass's stability across tiles
+    decides the store order) — the cell store equals the oracle's (20 key bits,
+    three passes; 16 cells, one pass) and the kd store passes the kd checks
+    (64-bit keys)."""
+    for spot in (False, True):
+        db = _sized(n, n, spot)
+        name = f"n={n} spot={spot}"
+        for cell, leaf in ((1 / 1024, 256), (1 / 16, 8)):
+            og = O.Grid(W.UNIT_BOX, cell, leaf).add(db)
+            x, y, t, p = og.store()
+            xy, ids, pid, bits = api.GridIndex(W.UNIT_BOX, cell, leaf).build(db, device=True).fetch_store()
+            assert (xy[:, 0].view(np.uint32) == x.view(np.uint32)).all(), name
+            assert (xy[:, 1].view(np.uint32) == y.view(np.uint32)).all(), name
+            tt, pp = _ids(ids, pid, bits)
+            assert (tt == t).all() and (pp == p).all(), name
+        _check_stig(db, 8, name, api.synth_rects(n + 1, 40, 1e-4, 0.3, True))
