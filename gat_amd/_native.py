@@ -86,6 +86,7 @@ HIP_FUNCS = {
     "gat_edr_pack_trajectories": (INT, [P, P, P, P, I32, I64, P, P, P]),
     "gat_edr_topk": (INT, [P, I32, P, I32, P, I32, P, P, P, P, P, I32, I32, P, P, P, P]),
     "gat_fill_u32": (INT, [P, U32, I64, P]),
+    "gat_fill_u32_spans": (INT, [I32, P, P, U32, P]),
     "gat_scan_tiles": (I64, [I64]),
     "gat_scan_u64": (INT, [P, I64, P, P, P, P]),
     "gat_scan_leaf_lists": (INT, [P, I64, I32, P, P, P, P]),

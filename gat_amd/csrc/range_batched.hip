@@ -76,6 +76,24 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, uint64_t n
     p[i] = v;
 }
 
+// Several spans cleared by one launch: a batch's resets are a few small
+// buffers beside the per-node counts, and each launch of its own costs a
+// graph node.
+constexpr int kFillSpans = 4;
+struct FillSpans {
+  uint32_t* p[kFillSpans];
+  uint64_t end[kFillSpans];  // running totals of the spans' lengths
+};
+
+__global__ void fill_spans_kernel(FillSpans f, uint32_t v) {
+  const uint64_t n = f.end[kFillSpans - 1];
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    int k = 0;
+    while (i >= f.end[k]) ++k;  // i < end[kFillSpans - 1], so k stays in range
+    f.p[k][i - (k ? f.end[k - 1] : 0)] = v;
+  }
+}
+
 // ---------------------------------------------------------------- selection
 // The stack holds node indices still to be tested.  Each step pops up to 64
 // entries from the top (one per lane), tests them, records selected leaves and
@@ -1298,16 +1316,26 @@ __global__ __launch_bounds__(256) void run_counts_kernel(const Item* __restrict_
   const uint64_t lo = uint64_t(blockIdx.x) * per;
   const uint64_t hi = lo + per < nrec ? lo + per : nrec;
   uint64_t mine = 0;
-  for (uint64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-    const Run r = runs[e];
-    if (r.count == 0 || r.q >= nqueries) continue;
+  auto add = [&](const Run& r) {
+    if (r.count == 0 || r.q >= nqueries) return;
     mine += r.count;
     if constexpr (kLds) {
       atomicAdd(acc + r.q, r.count);
     } else {
       atomicAdd(qcount + r.q, r.count);
     }
+  };
+  // Four records in flight per thread: the loop otherwise waits out one
+  // load's latency per record.
+  uint64_t e = lo + threadIdx.x;
+  for (; e + 3 * blockDim.x < hi; e += 4 * blockDim.x) {
+    const Run r0 = runs[e], r1 = runs[e + blockDim.x], r2 = runs[e + 2 * blockDim.x], r3 = runs[e + 3 * blockDim.x];
+    add(r0);
+    add(r1);
+    add(r2);
+    add(r3);
   }
+  for (; e < hi; e += blockDim.x) add(runs[e]);
   if (pair_total) {
     for (int off = kWave / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
     if ((threadIdx.x & (kWave - 1)) == 0 && mine) atomicAdd(pair_total, (unsigned long long)mine);
@@ -1385,6 +1413,24 @@ int gat_fill_u32(uint32_t* dst, uint32_t value, int64_t n, void* stream) {
   fill_u32_kernel<<<dim3{uint32_t(grid_for(uint64_t(n), 256, 8))}, dim3{256}, 0, gat::as_stream(stream)>>>(
       dst, value, uint64_t(n));
   GAT_LAUNCH_CHECK("fill_u32_kernel");
+  return GAT_OK;
+}
+
+int gat_fill_u32_spans(int32_t nspans, uint32_t* const* dst, const int64_t* n, uint32_t value, void* stream) {
+  GAT_REQUIRE(nspans >= 0 && nspans <= kFillSpans && (nspans == 0 || (dst && n)), "gat_fill_u32_spans: bad arguments");
+  FillSpans f{};
+  uint64_t total = 0;
+  for (int k = 0; k < kFillSpans; ++k) {
+    if (k < nspans) {
+      GAT_REQUIRE(n[k] >= 0 && (n[k] == 0 || dst[k]), "gat_fill_u32_spans: bad span");
+      f.p[k] = dst[k];
+      total += uint64_t(n[k]);
+    }
+    f.end[k] = total;
+  }
+  if (total == 0) return GAT_OK;
+  fill_spans_kernel<<<dim3{uint32_t(grid_for(total, 256, 8))}, dim3{256}, 0, gat::as_stream(stream)>>>(f, value);
+  GAT_LAUNCH_CHECK("fill_spans_kernel");
   return GAT_OK;
 }
 
@@ -1640,13 +1686,17 @@ int gat_run_counts(const gat_point_store* sp, const uint32_t* items, const uint3
     const auto* R = reinterpret_cast<const Run*>(runs);
     auto* T = reinterpret_cast<unsigned long long*>(pair_total);
     // Up to 2 workgroups per CU: the LDS counters take 40 KB on 10k rectangles.
-    const dim3 cgrid{uint32_t(grid_for(uint64_t(items_capacity) * 8, 256, 2))};
+    // With LDS counters one workgroup per CU: each zeroes and flushes a full
+    // set of counters, so fewer, longer ranges cost less (cfg2: 37 us, against
+    // 54 us with two per CU and one record in flight per thread, 46 us with
+    // four).
     if (uint64_t(nqueries) <= kCountLdsQueries)
-      run_counts_kernel<true><<<cgrid, dim3{256}, size_t(nqueries) * 4, s>>>(
-          I, item_total, uint64_t(items_capacity), list_cap, R, uint32_t(nqueries), qcount, T);
+      run_counts_kernel<true><<<dim3{uint32_t(grid_for(uint64_t(items_capacity) * 8, 256, 1))}, dim3{256},
+                                size_t(nqueries) * 4, s>>>(I, item_total, uint64_t(items_capacity), list_cap, R,
+                                                           uint32_t(nqueries), qcount, T);
     else
-      run_counts_kernel<false><<<cgrid, dim3{256}, 0, s>>>(I, item_total, uint64_t(items_capacity), list_cap, R,
-                                                         uint32_t(nqueries), qcount, T);
+      run_counts_kernel<false><<<dim3{uint32_t(grid_for(uint64_t(items_capacity) * 8, 256, 2))}, dim3{256}, 0, s>>>(
+          I, item_total, uint64_t(items_capacity), list_cap, R, uint32_t(nqueries), qcount, T);
     GAT_LAUNCH_CHECK("run_counts_kernel");
   }
   return GAT_OK;

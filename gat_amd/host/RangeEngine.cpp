@@ -767,7 +767,7 @@ int RangeEngine::uploadBoxes(const DeviceTree* tree, const MBB* bounds, int nq) 
   return GAT_OK;
 }
 
-int RangeEngine::enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nodeCount) {
+int RangeEngine::enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nodeCount, bool candCleared) {
   void* s = stream_.get();
   uint32_t* ctr = counters_.as<uint32_t>();
   auto* total = reinterpret_cast<uint64_t*>(ctr);
@@ -796,7 +796,8 @@ int RangeEngine::enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nod
                                reinterpret_cast<const uint32_t*>(d + lay.ntasks),
                                int64_t(lay.tasksCap)};
     // Several tasks add to one rectangle's candidate count.
-    if (tree.startLevel > 0) GATH_TRY(gat_fill_u32(cand_.as<uint32_t>(), 0, int64_t(nq) * 2, s), "batch reset");
+    if (tree.startLevel > 0 && !candCleared)
+      GATH_TRY(gat_fill_u32(cand_.as<uint32_t>(), 0, int64_t(nq) * 2, s), "batch reset");
     GATH_TRY(gat_tree_select(tree.nodes.as<gat_tree_node>(), tree.nnodes, tree.nchild,
                              tree.startLevel > 0 ? &start : nullptr, devBoxes_.as<float>(), order,
                              0, nq, nodeCount, cand_.as<uint64_t>(), lq_.as<uint32_t>(),
@@ -810,10 +811,13 @@ int RangeEngine::enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out) 
   void* s = stream_.get();
   const int64_t N = tree.nnodes;
   uint32_t* ctr = counters_.as<uint32_t>();
-  GATH_TRY(gat_fill_u32(nodeCount_.as<uint32_t>(), 0, N, s), "batch reset");
-  GATH_TRY(gat_fill_u32(out.qcount.as<uint32_t>(), 0, nq, s), "batch reset");
-  GATH_TRY(gat_fill_u32(ctr, 0, 16, s), "batch reset");
-  if (int rc = enqueueTreeSelect(tree, nq, nodeCount_.as<uint32_t>())) return rc;
+  // The batch's resets in one launch: per-node counts, per-query counts, the
+  // counters and, for the start-table walk, the candidate counts.
+  const bool clearCand = selectAddsCandidates(tree);
+  uint32_t* const spans[4] = {nodeCount_.as<uint32_t>(), out.qcount.as<uint32_t>(), ctr, cand_.as<uint32_t>()};
+  const int64_t lens[4] = {N, nq, 16, clearCand ? int64_t(nq) * 2 : 0};
+  GATH_TRY(gat_fill_u32_spans(4, spans, lens, 0, s), "batch reset");
+  if (int rc = enqueueTreeSelect(tree, nq, nodeCount_.as<uint32_t>(), clearCand)) return rc;
   // The run-indexed join places its output by work item: it needs the
   // candidate total but no per-query regions.  (The walk adding the total
   // with one atomic per wave cost it 20 us: every wave on one address.)

@@ -240,8 +240,13 @@ class RangeEngine {
   static BoxLayout boxLayout(int nq);
   int uploadBoxes(const DeviceTree* tree, const MBB* bounds, int nq);
   int enqueueSelect(const DeviceTree& tree, int nq, RangeBatch& out);
-  // The selection kernel for the tree's kind (nodeCount may be null).
-  int enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nodeCount);
+  // The selection kernel for the tree's kind (nodeCount may be null).  The
+  // start-table walk adds to the candidate counts, which it clears first
+  // unless the caller already has (candCleared).
+  static bool selectAddsCandidates(const DeviceTree& tree) {
+    return !tree.flat() && !tree.implicit() && tree.startLevel > 0;
+  }
+  int enqueueTreeSelect(const DeviceTree& tree, int nq, uint32_t* nodeCount, bool candCleared = false);
   int enqueueJoin(const DeviceTree& tree, int nq, RangeBatch& out);
   int readCounters(int nq, RangeBatch& out);
   bool countersFit() const;

@@ -431,6 +431,11 @@ typedef struct gat_tree_node {
 /* dst[0 .. n) = value, by a kernel (safe inside a captured graph). */
 int gat_fill_u32(uint32_t* dst_dev, uint32_t value, int64_t n, void* stream);
 
+/* The same for up to four spans in one launch: dst_dev[k][0 .. n[k]) = value
+ * for k < nspans (dst_dev and n are host arrays; a span of length 0 may have
+ * a null pointer). */
+int gat_fill_u32_spans(int32_t nspans, uint32_t* const* dst_dev, const int64_t* n, uint32_t value, void* stream);
+
 /* Number of tile partials the scans below need for n elements. */
 int64_t gat_scan_tiles(int64_t n);
 
