@@ -592,8 +592,8 @@ __global__ __launch_bounds__(kWave) void morton_select_kernel(
 }
 
 // ---------------------------------------------------------------- scans
-// Reduce-then-scan over tiles of 4096 elements: per-tile sums, a carried
-// exclusive scan of the tile sums in one workgroup, then a per-tile scan.
+// Reduce-then-scan over tiles of 4096 elements: per-tile sums, then a
+// per-tile scan that takes its carry from the sums of the tiles before it.
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanBlock * kScanItems;
@@ -660,36 +660,26 @@ __global__ __launch_bounds__(kScanBlock) void scan_tiles_kernel(L load, uint64_t
   if (threadIdx.x == 0) tiles[blockIdx.x] = all;
 }
 
-template <class T>
-__global__ __launch_bounds__(kScanBlock) void scan_carry_kernel(T* __restrict__ tiles, uint64_t ntiles,
-                                                                T* __restrict__ total) {
-  T carry = vzero(T{});
-  for (uint64_t base = 0; base < ntiles; base += kScanTile) {
-    const uint64_t b = base + uint64_t(threadIdx.x) * kScanItems;
-    T v[kScanItems];
-    T s = vzero(T{});
-    for (int k = 0; k < kScanItems; ++k) {
-      v[k] = b + k < ntiles ? tiles[b + k] : vzero(T{});
-      s = vadd(s, v[k]);
-    }
-    T all;
-    T run = vadd(carry, block_exclusive(s, &all));
-    for (int k = 0; k < kScanItems; ++k)
-      if (b + k < ntiles) {
-        tiles[b + k] = run;
-        run = vadd(run, v[k]);
-      }
-    carry = vadd(carry, all);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
-
+// Each workgroup sums the tile totals before its own for its carry (a few
+// hundred L2-resident values), and the first also all of them for the
+// total, so no separate launch scans the tile totals.
 template <class L>
 __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(L load, uint64_t n,
                                                                 const typename L::T* __restrict__ tiles,
-                                                                typename L::T* __restrict__ out) {
+                                                                uint64_t ntiles, typename L::T* __restrict__ out,
+                                                                typename L::T* __restrict__ total) {
   using T = typename L::T;
+  T c = vzero(T{});
+  for (uint64_t t = threadIdx.x; t < blockIdx.x; t += kScanBlock) c = vadd(c, tiles[t]);
+  T carry;
+  block_exclusive(c, &carry);
+  if (blockIdx.x == 0) {
+    T g = vzero(T{});
+    for (uint64_t t = threadIdx.x; t < ntiles; t += kScanBlock) g = vadd(g, tiles[t]);
+    T all;
+    block_exclusive(g, &all);
+    if (threadIdx.x == 0) *total = all;
+  }
   const uint64_t b = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
   T v[kScanItems];
   T s = vzero(T{});
@@ -698,7 +688,7 @@ __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(L load, uint64_t
     s = vadd(s, v[k]);
   }
   T all;
-  T run = vadd(tiles[blockIdx.x], block_exclusive(s, &all));
+  T run = vadd(carry, block_exclusive(s, &all));
   for (int k = 0; k < kScanItems; ++k)
     if (b + k < n) {
       out[b + k] = run;
@@ -709,17 +699,13 @@ __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(L load, uint64_t
 template <class L>
 int launch_scan(L load, uint64_t n, typename L::T* tiles, typename L::T* out, typename L::T* total, hipStream_t s) {
   using T = typename L::T;
+  static_assert(sizeof(T) == 8, "scan totals are two words");
   const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
-  if (ntiles > 0) {
-    scan_tiles_kernel<L><<<dim3{uint32_t(ntiles)}, dim3{kScanBlock}, 0, s>>>(load, n, tiles);
-    GAT_LAUNCH_CHECK("scan_tiles_kernel");
-  }
-  scan_carry_kernel<T><<<dim3{1}, dim3{kScanBlock}, 0, s>>>(tiles, ntiles, total);
-  GAT_LAUNCH_CHECK("scan_carry_kernel");
-  if (ntiles > 0) {
-    scan_apply_kernel<L><<<dim3{uint32_t(ntiles)}, dim3{kScanBlock}, 0, s>>>(load, n, tiles, out);
-    GAT_LAUNCH_CHECK("scan_apply_kernel");
-  }
+  if (ntiles == 0) return gat_fill_u32(reinterpret_cast<uint32_t*>(total), 0u, 2, s);
+  scan_tiles_kernel<L><<<dim3{uint32_t(ntiles)}, dim3{kScanBlock}, 0, s>>>(load, n, tiles);
+  GAT_LAUNCH_CHECK("scan_tiles_kernel");
+  scan_apply_kernel<L><<<dim3{uint32_t(ntiles)}, dim3{kScanBlock}, 0, s>>>(load, n, tiles, ntiles, out, total);
+  GAT_LAUNCH_CHECK("scan_apply_kernel");
   return GAT_OK;
 }
 
