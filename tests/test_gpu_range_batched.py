@@ -4,12 +4,15 @@ Grid::rangeQueryBatch (Grid.cpp:341-405) and inclusive brute force for STIG.
 
 Bar: bit-exact result sets per query, and per-query digests equal to the
 plain path's and the oracle's."""
+import ctypes
+
 import numpy as np
 import pytest
 
 import oracle as O
-from gat_amd import api, workloads as W
+from gat_amd import _native as N, api, workloads as W
 
+from tests.devbuf import DevBuf
 from tests.test_gpu_range import _check_pairs, _edge_rects, _sets
 
 pytestmark = pytest.mark.gpu
@@ -313,9 +316,6 @@ def test_run_indexed_abi_skips_malformed_items(gpu, nqueries):
     lay before, so the count pass over the list's records counts only the good
     items.  Both count paths: LDS-aggregated (3 rectangles) and direct
     (20,000)."""
-    import ctypes
-    from gat_amd import _native as N
-    from tests.devbuf import DevBuf
     h = N.hip()
     xy = np.array([[0.1, 0.1], [0.2, 0.2], [0.3, 0.3], [0.9, 0.9], [0.5, 0.5], [0.6, 0.6]], np.float32)
     tid = np.array([1, 1, 2, 3, 4, 4], np.uint32)
@@ -360,3 +360,36 @@ def test_run_indexed_abi_skips_malformed_items(gpu, nqueries):
     assert h.gat_run_counts(sp, ditems.ptr, total.ptr, 5, 6, runs.ptr, 6, nqueries, None, None, None) == 2
     assert h.gat_leaf_refine_runs(sp, None, total.ptr, 5, dq.ptr, 6, dboxes.ptr, nqueries, pairs.ptr, 64, runs.ptr,
                                   6, base.ptr, None) == 2
+
+
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 4097, 3 * 4096 + 5, 200_000])
+def test_scan_u64_over_tiles(gpu, n):
+    """The exclusive scan the pipeline's regions and lists come from: tiles
+    of 4,096 values, each tile's carry summed from the tile totals before it
+    in the apply pass; the total from the first workgroup (zero for n = 0)."""
+    h = N.hip()
+    rng = np.random.default_rng(n)
+    vals = rng.integers(0, 1 << 40, max(n, 1), dtype=np.uint64)[:n]
+    din = DevBuf.of(vals) if n else DevBuf(8)
+    out, total = DevBuf(8 * max(n, 1)).zero(), DevBuf.of(np.array([12345], np.uint64))
+    tiles = DevBuf(8 * int(h.gat_scan_tiles(n))).zero()
+    N.check_hip(h.gat_scan_u64(din.ptr, n, out.ptr, total.ptr, tiles.ptr, None))
+    want = np.concatenate([[0], np.cumsum(vals, dtype=np.uint64)]).astype(np.uint64)
+    assert (out.get(np.uint64, n) == want[:n]).all()
+    assert total.get(np.uint64, 1)[0] == want[n]
+
+
+def test_fill_spans(gpu):
+    """gat_fill_u32_spans clears up to four spans in one launch (empty spans,
+    with or without a pointer, included) and nothing beside them."""
+    h = N.hip()
+    lens = [5000, 0, 1, 100_003]
+    bufs = [DevBuf.of(np.full(m + 2, 7, np.uint32)) for m in lens]
+    # Each span starts one word into its buffer, so the words on both sides stay 7.
+    ptrs = (ctypes.c_void_p * 4)(*[b.ptr.value + 4 if m else None for b, m in zip(bufs, lens)])
+    n = (ctypes.c_int64 * 4)(*lens)
+    N.check_hip(h.gat_fill_u32_spans(4, ptrs, n, 0xABCD, None))
+    for b, m in zip(bufs, lens):
+        got = b.get(np.uint32, m + 2)
+        assert got[0] == 7 and got[-1] == 7 and (got[1:-1] == 0xABCD).all()
+    assert h.gat_fill_u32_spans(5, ptrs, n, 0, None) == 2
