@@ -290,6 +290,13 @@ __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict
   constexpr int kOff = 64;
   constexpr int kLen = kOff + MAXM + 96;
   __shared__ int16_t rows[W][kLen];
+  // The column trajectory in LDS, columns -63 .. 16 * nblocks - 1 (<= m + 77)
+  // at [kColOff + column]: a pair over W waves keeps one wave per SIMD, too
+  // few to hide a global load's latency behind a block's 16 steps, so each
+  // block's points come from LDS instead.
+  constexpr int kColOff = 64;
+  constexpr int kColLen = kColOff + MAXM + 80;
+  __shared__ float2 cols[kColLen];
   __shared__ int32_t job_slot;
   const int wv = threadIdx.x / 64;
   const int l = threadIdx.x & 63;
@@ -333,25 +340,22 @@ __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict
     // previous strip.
     const int P = max(nblocks, W * kLag);
     const int kend = ((strips - 1) / W) * P + ((strips - 1) % W) * kLag + nblocks;
+    // Stage the columns the blocks read (lane l of block tb reads columns
+    // tb - l .. tb - l + 15); those more than kPad before the trajectory are
+    // NaN, never matching.
+    for (int i = int(threadIdx.x) - 63; i < nblocks * kBlk; i += 64 * W) cols[kColOff + i] = i < -kPad ? nan2 : C[i];
+    __syncthreads();
     // Per-lane state of the wave's current strip.
     int cur = -1;       // strip
     int c1 = 0, u1p = 0, tcap = -1, res = 0;
     float rx = 0.0f, ry = 0.0f;
-    const float2* cp = C;
+    // The row point of the wave's next strip, loaded a strip ahead.
+    float2 next_row = 64 * wv + l + 1 <= n ? R[64 * wv + l] : nan2;
     float2 colA[kBlk], colB[kBlk];
     auto load_block = [&](float2(&dst)[kBlk], int tb) __attribute__((always_inline)) {
-      if (tb - 63 >= -kPad) {
+      const float2* cl = &cols[kColOff + tb - l];
 #pragma unroll
-        for (int j = 0; j < kBlk; ++j) dst[j] = cp[tb + j];
-      } else {
-        const bool before = tb - l < -kPad;
-        const float2* bp = before ? C - kPad : cp + tb;
-#pragma unroll
-        for (int j = 0; j < kBlk; ++j) {
-          const float2 v = bp[j];
-          dst[j] = before ? nan2 : v;
-        }
-      }
+      for (int j = 0; j < kBlk; ++j) dst[j] = cl[j];
     };
     // One iteration: the wave's block of its current strip, if any, with the
     // block's column points in src and the next block's loading into dst.
@@ -366,13 +370,13 @@ __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict
       if (b == 0) {  // a new strip: row point, D[r][0] + 1, D[r-1][0] + 1
         cur = s;
         const int r = 64 * s + l + 1;
-        const float2 rp = r <= n ? R[r - 1] : nan2;
-        rx = rp.x;
-        ry = rp.y;
+        rx = next_row.x;
+        ry = next_row.y;
+        const int rn = r + 64 * W;  // the wave's next strip
+        next_row = rn <= n ? R[rn - 1] : nan2;
         c1 = r + 1;
         u1p = r;
         tcap = (s == strips - 1) ? m + l - 1 : -1;
-        cp = C - l;
         load_block(src, 0);
       }
       // Feed: D[r0][t0 + j + 1] + 1 from the previous strip's wave (row 0: c + 1).
