@@ -76,6 +76,16 @@ __device__ __forceinline__ int32_t take_window(const int32_t* window, int32_t& n
   return first;
 }
 
+
+// Issue priority of a wave by the length of its job's schedule in steps:
+// the longest pairs of a batch set its end, so their waves go first.
+__device__ __forceinline__ void job_priority(int64_t steps) {
+  if (steps >= 16384) __builtin_amdgcn_s_setprio(3);
+  else if (steps >= 8192) __builtin_amdgcn_s_setprio(2);
+  else if (steps >= 4096) __builtin_amdgcn_s_setprio(1);
+  else __builtin_amdgcn_s_setprio(0);
+}
+
 // SEG lanes per pair: 16 (four pairs per wave) or 64 (one pair per wave, a
 // quarter of the strips, for the longest pairs, whose strips would otherwise
 // run one after another for milliseconds).  Waves per SIMD each class can hold:
@@ -156,6 +166,7 @@ __global__ __launch_bounds__(64, (MAXM >= 2048 && SEG == 16) ? 2 : 4) void edr_w
     const int strips = (nmax + kSeg - 1) / kSeg;
     const int my_last = (n + kSeg - 1) / kSeg - 1;
     const int steps = mmax + kSeg - 1;
+    job_priority(int64_t(strips) * steps);
     // Row 0: D[0][c] + 1 = c + 1.  (Kept rolled: unrolled, its length grows with MAXM
     // and so does the register count.)
 #pragma unroll 2
@@ -340,6 +351,7 @@ __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict
     // previous strip.
     const int P = max(nblocks, W * kLag);
     const int kend = ((strips - 1) / W) * P + ((strips - 1) % W) * kLag + nblocks;
+    job_priority(int64_t(kend) * kBlk);
     // Stage the columns the blocks read (lane l of block tb reads columns
     // tb - l .. tb - l + 15); those more than kPad before the trajectory are
     // NaN, never matching.
