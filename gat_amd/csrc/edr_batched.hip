@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64, (MAXM >= 2048 && SEG == 16) ? 2 : 4) void edr_w
     int ji = 0;
     if (lane == 0) ji = atomicAdd(counter, 1);
     ji = __builtin_amdgcn_readfirstlane(ji);
-    if (ji >= njobs) break;
+    if (ji < 0 || ji >= njobs) break;  // a counter that was not cleared hands out nothing
     const Job job = jobs[ji];
     // Jobs and pairs that point outside their arrays are skipped.
     const bool live = seg < job.count && job.first >= 0 && int64_t(job.first) + seg < npairs;
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict
     __syncthreads();
     const int ji = job_slot;
     __syncthreads();  // every wave has read the slot before the next job overwrites it
-    if (ji >= njobs) break;
+    if (ji < 0 || ji >= njobs) break;  // a counter that was not cleared hands out nothing
     const Job job = jobs[ji];
     // One pair per job; everything below is uniform across the workgroup.
     const bool live = job.count >= 1 && job.first >= 0 && int64_t(job.first) < npairs;
