@@ -374,7 +374,8 @@ def test_scan_u64_over_tiles(gpu, n):
     out, total = DevBuf(8 * max(n, 1)).zero(), DevBuf.of(np.array([12345], np.uint64))
     tiles = DevBuf(8 * int(h.gat_scan_tiles(n))).zero()
     N.check_hip(h.gat_scan_u64(din.ptr, n, out.ptr, total.ptr, tiles.ptr, None))
-    want = np.concatenate([[0], np.cumsum(vals, dtype=np.uint64)]).astype(np.uint64)
+    want = np.zeros(n + 1, np.uint64)  # integer throughout: sums pass 2^53
+    want[1:] = np.cumsum(vals, dtype=np.uint64)
     assert (out.get(np.uint64, n) == want[:n]).all()
     assert total.get(np.uint64, 1)[0] == want[n]
 
