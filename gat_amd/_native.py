@@ -82,6 +82,7 @@ HIP_FUNCS = {
     "gat_edr_unsort": (INT, [P, P, P, I64, P, P]),
     "gat_edr_survivor_lists": (INT, [P, P, P, I64, I32, P, P, P, P, P]),
     "gat_edr_filter": (INT, [P, I32, P, I32, F64, P, I64, P, P]),
+    "gat_edr_filter_grid": (INT, [P, I32, P, I32, P, F64, P, I64, P, P]),
     "gat_edr_mismatch_table": (INT, [I32, P, P]),
     "gat_edr_pack_trajectories": (INT, [P, P, P, P, I32, I64, P, P, P]),
     "gat_edr_topk": (INT, [P, I32, P, I32, P, I32, P, P, P, P, P, I32, I32, P, P, P, P]),

@@ -34,6 +34,8 @@
 // strip count, ordered longest first, handed out by one atomic counter per
 // launch, so the batch's tail is short and no host round trip separates length
 // buckets.
+#include <cmath>
+
 #include "gat_common.hpp"
 
 namespace {
@@ -466,6 +468,117 @@ __global__ __launch_bounds__(256) void edr_filter_kernel(const float4* __restric
              (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base >> 32))) << 32);
       const uint64_t at = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
       if (keep && at < capacity) out[at] = make_int2(q0 + j, c);
+    }
+  }
+}
+
+// The same filter through a grid over the candidates' boxes (no reference
+// counterpart): every candidate whose box spans at most kGridMaxCells cells
+// is listed in each cell it touches (gat_edr_cand_grid, built by the host);
+// a query tests the candidates of the cells its box, widened by u and one
+// cell on every side, touches, so it sees every candidate that could pass
+// may_match (keep implies c.xmin < q.xmax + u and c.xmax > q.xmin - u in
+// exact arithmetic, and the widening covers the rounding of the cell of
+// q.xmax + u).  A pair is reported by one cell only: the one at (max(cell of
+// c.xmin, first column), max(cell of c.ymin, first row)), which both the
+// candidate's cells and the query's hold.  Candidates the grid does not list
+// (boxes not finite, inverted or too wide) are tested by every query, and a
+// query whose box is not finite or inverted tests every candidate.  One wave
+// per query; the output set equals edr_filter_kernel's.
+struct CandGrid {
+  int32_t g;
+  float x0, y0, inv_x, inv_y;
+  const uint32_t* start;  // g * g + 1 entry offsets, cells row by row
+  const int32_t* id;
+  const float4* box;
+  const int32_t* always;
+  int32_t nalways;
+};
+
+// Cell column (or row) of v: floor((v - o) * inv) clamped to 0 .. g - 1 (the
+// host lists candidates with the same float operations).
+__device__ __forceinline__ int grid_cell(float v, float o, float inv, int g) {
+  const float t = floorf((v - o) * inv);
+  return int(fminf(fmaxf(t, 0.0f), float(g - 1)));
+}
+
+constexpr int kGridFilterWaves = 4;
+
+__global__ __launch_bounds__(64 * kGridFilterWaves) void edr_filter_grid_kernel(
+    const float4* __restrict__ qbox, int32_t nq, const float4* __restrict__ cbox, int32_t nc, CandGrid gr, float u,
+    int2* __restrict__ out, uint64_t capacity, unsigned long long* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  const int q = int(blockIdx.x) * kGridFilterWaves + int(threadIdx.x >> 6);
+  if (q >= nq) return;  // the whole wave
+  const float4 qb = qbox[q];
+  auto emit = [&](bool keep, int c) __attribute__((always_inline)) {
+    const uint64_t mask = __ballot(keep);
+    if (mask == 0) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+    base = __builtin_amdgcn_readfirstlane(uint32_t(base)) |
+           (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base >> 32))) << 32);
+    const uint64_t at = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+    if (keep && at < capacity) out[at] = make_int2(q, c);
+  };
+  const bool boxed = isfinite(qb.x) && isfinite(qb.y) && isfinite(qb.z) && isfinite(qb.w) && qb.x <= qb.z &&
+                     qb.y <= qb.w;
+  if (!boxed) {
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+      const int c = c0 + lane;
+      emit(c < nc && may_match(qb, cbox[c], u), c);
+    }
+    return;
+  }
+  for (int a0 = 0; a0 < gr.nalways; a0 += 64) {
+    const int a = a0 + lane;
+    const int c = a < gr.nalways ? gr.always[a] : 0;
+    emit(a < gr.nalways && c >= 0 && c < nc && may_match(qb, cbox[c], u), c);
+  }
+  const int cx0 = max(grid_cell(qb.x - u, gr.x0, gr.inv_x, gr.g) - 1, 0);
+  const int cx1 = min(grid_cell(qb.z + u, gr.x0, gr.inv_x, gr.g) + 1, gr.g - 1);
+  const int cy0 = max(grid_cell(qb.y - u, gr.y0, gr.inv_y, gr.g) - 1, 0);
+  const int cy1 = min(grid_cell(qb.w + u, gr.y0, gr.inv_y, gr.g) + 1, gr.g - 1);
+  const int wx = cx1 - cx0 + 1, ncells = wx * (cy1 - cy0 + 1);
+  for (int k0 = 0; k0 < ncells; k0 += 64) {
+    // Lane k takes cell k0 + k; its entries are numbered after the lower lanes'.
+    const int k = k0 + lane;
+    int cellx = 0, celly = 0;
+    uint32_t s = 0, cnt = 0;
+    if (k < ncells) {
+      cellx = cx0 + k % wx;
+      celly = cy0 + k / wx;
+      const int cell = celly * gr.g + cellx;
+      s = gr.start[cell];
+      cnt = gr.start[cell + 1] - s;
+    }
+    uint32_t incl = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = uint32_t(__shfl_up(int(incl), d, 64));
+      if (lane >= d) incl += y;
+    }
+    const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+    for (uint32_t i0 = 0; i0 < total; i0 += 64) {
+      const uint32_t i = i0 + uint32_t(lane);
+      // The lane whose cell holds entry i: the first with incl > i.
+      int j = 0;
+      for (int step = 32; step >= 1; step >>= 1)
+        if (uint32_t(__shfl(int(incl), j + step - 1, 64)) <= i) j += step;
+      j = min(j, 63);
+      const uint32_t sj = uint32_t(__shfl(int(s), j, 64));
+      const uint32_t before = uint32_t(__shfl(int(incl), j, 64)) - uint32_t(__shfl(int(cnt), j, 64));
+      const int jx = __shfl(cellx, j, 64), jy = __shfl(celly, j, 64);
+      bool keep = false;
+      int c = 0;
+      if (i < total) {
+        const uint32_t e = sj + (i - before);
+        const float4 b = gr.box[e];
+        c = gr.id[e];
+        const int rx = max(grid_cell(b.x, gr.x0, gr.inv_x, gr.g), cx0);
+        const int ry = max(grid_cell(b.y, gr.y0, gr.inv_y, gr.g), cy0);
+        keep = rx == jx && ry == jy && c >= 0 && c < nc && may_match(qb, b, u);
+      }
+      emit(keep, c);
     }
   }
 }
@@ -1287,6 +1400,40 @@ extern "C" int gat_edr_filter(const float* qbox, int32_t nq, const float* cbox, 
       reinterpret_cast<const float4*>(qbox), nq, reinterpret_cast<const float4*>(cbox), nc, gat_eps_threshold(eps),
       reinterpret_cast<int2*>(pairs_out), uint64_t(capacity), reinterpret_cast<unsigned long long*>(count));
   GAT_LAUNCH_CHECK("edr_filter_kernel");
+  return GAT_OK;
+}
+
+extern "C" int gat_edr_filter_grid(const float* qbox, int32_t nq, const float* cbox, int32_t nc,
+                                   const gat_edr_cand_grid* grid, double eps, int32_t* pairs_out, int64_t capacity,
+                                   uint64_t* count, void* stream) {
+  GAT_REQUIRE(nq >= 0 && nc >= 0 && capacity >= 0, "EDR grid filter: bad sizes");
+  GAT_REQUIRE(count && grid, "EDR grid filter: null count or grid");
+  GAT_REQUIRE(grid->cells_per_side >= 1 && grid->cells_per_side <= 4096 && grid->nalways >= 0 &&
+                  grid->cell_start_dev && (grid->nalways == 0 || grid->always_dev) && std::isfinite(grid->x0) &&
+                  std::isfinite(grid->y0) && std::isfinite(grid->inv_x) && std::isfinite(grid->inv_y) &&
+                  grid->inv_x >= 0.0f && grid->inv_y >= 0.0f,
+              "EDR grid filter: bad grid");
+  hipStream_t s = gat::as_stream(stream);
+  if (int rc = gat_fill_u32(reinterpret_cast<uint32_t*>(count), 0u, 2, stream)) return rc;
+  if (nq == 0 || nc == 0) return GAT_OK;
+  GAT_REQUIRE(qbox && cbox && (pairs_out || capacity == 0), "EDR grid filter: null pointer");
+  GAT_REQUIRE(eps > 0, "EDR grid filter: eps must be positive");
+  const CandGrid gr{grid->cells_per_side,
+                    grid->x0,
+                    grid->y0,
+                    grid->inv_x,
+                    grid->inv_y,
+                    grid->cell_start_dev,
+                    grid->entry_id_dev,
+                    reinterpret_cast<const float4*>(grid->entry_box_dev),
+                    grid->always_dev,
+                    grid->nalways};
+  edr_filter_grid_kernel<<<dim3(uint32_t((nq + kGridFilterWaves - 1) / kGridFilterWaves)),
+                           dim3(64 * kGridFilterWaves), 0, s>>>(
+      reinterpret_cast<const float4*>(qbox), nq, reinterpret_cast<const float4*>(cbox), nc, gr,
+      gat_eps_threshold(eps), reinterpret_cast<int2*>(pairs_out), uint64_t(capacity),
+      reinterpret_cast<unsigned long long*>(count));
+  GAT_LAUNCH_CHECK("edr_filter_grid_kernel");
   return GAT_OK;
 }
 

@@ -210,6 +210,14 @@ class Grid {
   // Stores are NaN-padded (GAT_EDR_PAD points before every trajectory,
   // GAT_EDR_TAIL_PAD after the last) for the batched kernel.
   DeviceBuffer trajXY_, trajOff_, trajLen_, trajBox_, allTasks_, dist_;
+  // The candidate grid of the top-k filter (gat_edr_filter_grid), built with
+  // the trajectory store.
+  DeviceBuffer candStart_, candId_, candBox_, candAlways_;
+  gat_edr_cand_grid candGrid_{};
+  int buildCandidateGrid(const std::vector<float>& boxes, int T);
+  // The top-k filter over the uploaded query boxes: every (query, trajectory)
+  // pair whose boxes could hold a matching point pair.
+  int filterPairs(int nq, int64_t cap, void* stream);
   DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
   DeviceBuffer queryRaw_, querySrcOff_;  // raw query coordinates and offsets, packed on the device
   PinnedBuffer queryStage_;  // query coordinates and offsets, uploaded from here

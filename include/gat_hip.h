@@ -349,6 +349,36 @@ int gat_edr_survivor_lists(const int32_t* sorted_dev, const int32_t* dist_dev, c
 int gat_edr_filter(const float* query_box_dev, int32_t nq, const float* cand_box_dev, int32_t nc, double eps,
                    int32_t* pairs_out_dev, int64_t capacity, uint64_t* count_dev, void* stream);
 
+/* A grid over the candidates' boxes for gat_edr_filter_grid (no reference
+ * counterpart; built by the host index, Grid::uploadTrajectories).  The cell
+ * column of a coordinate v is floor((v - x0) * inv_x) clamped to
+ * 0 .. cells_per_side - 1, in float (rows likewise with y0 and inv_y).
+ * Cells are numbered row by row; entries cell_start_dev[cell] ..
+ * cell_start_dev[cell + 1] - 1 list the candidates (entry_id_dev, with their
+ * boxes in entry_box_dev) whose box is finite, not inverted, and touches the
+ * cell (every cell from the cell of its min corner to the cell of its max
+ * corner).  Every other candidate, and any the builder chose not to list, is
+ * in always_dev. */
+typedef struct gat_edr_cand_grid {
+  int32_t cells_per_side;
+  float x0, y0, inv_x, inv_y;
+  const uint32_t* cell_start_dev; /* cells_per_side^2 + 1 */
+  const int32_t* entry_id_dev;
+  const float* entry_box_dev; /* 4 floats per entry */
+  const int32_t* always_dev;
+  int32_t nalways;
+} gat_edr_cand_grid;
+
+/* gat_edr_filter's output through the candidate grid: the same set of pairs
+ * (each once, in no defined order), found by testing only the candidates of
+ * the cells each query's box (widened by eps and one cell) touches, plus the
+ * always-tested ones; a query whose box is not finite or inverted tests every
+ * candidate.  cand_box_dev holds every candidate's box as for gat_edr_filter.
+ * *count_dev is reset by the call.  Asynchronous on `stream`. */
+int gat_edr_filter_grid(const float* query_box_dev, int32_t nq, const float* cand_box_dev, int32_t nc,
+                        const gat_edr_cand_grid* grid, double eps, int32_t* pairs_out_dev, int64_t capacity,
+                        uint64_t* count_dev, void* stream);
+
 /* Exact top-k merge on the device (no reference counterpart: the reference
  * keeps a k-entry priority queue per query on the host, Grid.cpp:1600-1729,
  * fed in its frequency-distance order).  For

@@ -175,6 +175,75 @@ def test_filter_many_queries_matches_the_box_rule(gpu):
     assert n == len(want) and len(set(got)) == n and set(got) == want
 
 
+class _CandGrid(ctypes.Structure):
+    _fields_ = [("cells_per_side", ctypes.c_int32), ("x0", ctypes.c_float), ("y0", ctypes.c_float),
+                ("inv_x", ctypes.c_float), ("inv_y", ctypes.c_float), ("cell_start", ctypes.c_void_p),
+                ("entry_id", ctypes.c_void_p), ("entry_box", ctypes.c_void_p), ("always", ctypes.c_void_p),
+                ("nalways", ctypes.c_int32)]
+
+
+def _grid_filter(qb, cb, grid, eps=EPS):
+    h = N.hip()
+    bufs = [DevBuf.of(grid[k]) for k in ("start", "id", "box", "always")]
+    s = _CandGrid(grid["g"], grid["x0"], grid["y0"], grid["invx"], grid["invy"], *[b.ptr.value for b in bufs],
+                  grid["nalways"])
+    dq, dc = DevBuf.of(np.ascontiguousarray(qb)), DevBuf.of(np.ascontiguousarray(cb))
+    cap = len(qb) * len(cb)
+    out, cnt = DevBuf(8 * cap), DevBuf.of(np.array([12345], np.uint64))
+    N.check_hip(h.gat_edr_filter_grid(dq.ptr, len(qb), dc.ptr, len(cb), ctypes.byref(s), eps, out.ptr, cap, cnt.ptr,
+                                      None))
+    n = int(cnt.get(np.uint64, 1)[0])
+    return [tuple(p) for p in out.get(np.int32, 2 * n).reshape(-1, 2).tolist()]
+
+
+@pytest.mark.parametrize("g", [1, 7, 64, 313])
+def test_grid_filter_matches_the_box_rule(gpu, g):
+    """gat_edr_filter_grid over a candidate grid of g x g cells keeps exactly
+    the pairs the float32 box rule keeps, each once — with NaN, inverted and
+    wide candidate boxes (tested by every query) and queries whose boxes are
+    NaN or inverted (tested against every candidate), and a candidate a
+    float-difference of exactly u from a query (dropped) and one float closer
+    (kept) — and the same set as the every-pair filter."""
+    from tests import cand_grid as CG
+    u = np.float32(api.eps_threshold(EPS))
+    qb, cb = CG.cases(g + 100, 70, 3000)
+    qb[0] = np.array([0.3, 0.3, 0.31, 0.31], np.float32)
+    edge = np.float32(qb[0, 2] + u)
+    while edge - qb[0, 2] < u:
+        edge = np.nextafter(edge, np.float32(1))
+    cb[0] = np.array([edge, 0.3, edge + np.float32(0.001), 0.31], np.float32)
+    cb[1] = np.array([np.nextafter(edge, np.float32(0)), 0.3, edge + np.float32(0.001), 0.31], np.float32)
+    want = {(i, int(j)) for i in range(len(qb)) for j in np.nonzero(CG.keep(qb[i], cb, u))[0]}
+    got = _grid_filter(qb, cb, CG.build(cb, g))
+    assert len(got) == len(set(got)) and set(got) == want
+    assert (0, 1) in want and (0, 0) not in want
+    h = N.hip()
+    dq, dc = DevBuf.of(qb), DevBuf.of(cb)
+    out, cnt = DevBuf(8 * len(qb) * len(cb)), DevBuf(8)
+    N.check_hip(h.gat_edr_filter(dq.ptr, len(qb), dc.ptr, len(cb), EPS, out.ptr, len(qb) * len(cb), cnt.ptr, None))
+    n = int(cnt.get(np.uint64, 1)[0])
+    assert {tuple(p) for p in out.get(np.int32, 2 * n).reshape(-1, 2).tolist()} == want
+
+
+def test_topk_survivors_equal_the_box_rule(gpu):
+    """The top-k batch's filter (the index's candidate grid) passes exactly as
+    many pairs as the every-pair box rule on the trajectories' boxes."""
+    db = _varied_db(91, np.array(list(np.random.default_rng(91).integers(2, 300, 800)), np.int32))
+    g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)
+    q = api.synth_queries(92, np.array([5, 40, 128, 300, 64, 17], np.int32), db, copy_mode=2)
+    _, _, st = g.similarity_batched(q, 10, EPS)
+
+    def boxes(ts):
+        return np.array([[ts.xy(i + 1)[:, 0].min(), ts.xy(i + 1)[:, 1].min(), ts.xy(i + 1)[:, 0].max(),
+                          ts.xy(i + 1)[:, 1].max()] for i in range(ts.ntraj)], np.float32)
+
+    from tests import cand_grid as CG
+    u = np.float32(api.eps_threshold(EPS))
+    qb, cb = boxes(q), boxes(db)
+    want = sum(int(CG.keep(qb[i], cb, u).sum()) for i in range(len(qb)))
+    assert st["pairs"] == want and st["pairs"] + st["pruned"] == q.ntraj * db.ntraj
+
+
 def test_autotuned_edr_gives_the_same_distances(gpu):
     db = _varied_db(77, [1, 17, 64, 300, 511, 513, 700, 1024, 1025, 2048] + list(range(20, 1200, 37)))
     g = api.GridIndex(W.UNIT_BOX, 0.01, 256).build(db)

@@ -575,3 +575,26 @@ def test_mgrid_device_tree_selects_oracle_leaves(cell, leaf):
         want = [(int(s), int(c)) for _, _, s, c in om.leaves(r).tolist() if c > 0]
         assert _runs(sel) == _runs(want)
         assert top <= (L + 1) * 4 * 64
+
+
+@pytest.mark.parametrize("g", [1, 7, 64])
+def test_candidate_grid_scan_reports_every_kept_pair_once(g):
+    """The top-k filter's grid scan (restated in tests/cand_grid.py with the
+    kernel's float32 cell arithmetic): every pair the box rule keeps is found
+    exactly once, on random, NaN, inverted and wide boxes and on boxes placed
+    a float-difference of exactly u apart."""
+    from tests import cand_grid as CG
+    u = np.float32(api.eps_threshold(5e-4))
+    q, c = CG.cases(g, 60, 400)
+    # Candidates right at the rule's edge of query 0: c.xmin - q.xmax == u and one float below.
+    q[0] = np.array([0.3, 0.3, 0.31, 0.31], np.float32)
+    edge = np.float32(q[0, 2] + u)
+    while edge - q[0, 2] < u:
+        edge = np.nextafter(edge, np.float32(1))
+    c[0] = np.array([edge, 0.3, edge + np.float32(0.001), 0.31], np.float32)
+    c[1] = np.array([np.nextafter(edge, np.float32(0)), 0.3, edge + np.float32(0.001), 0.31], np.float32)
+    want = {(i, j) for i in range(len(q)) for j in np.nonzero(CG.keep(q[i], c, u))[0]}
+    got = CG.scan(CG.build(c, g), q, c, u)
+    assert len(got) == len(set(got))
+    assert set(got) == want
+    assert (0, 1) in want and (0, 0) not in want
