@@ -218,8 +218,14 @@ class Grid {
   // The top-k filter over the uploaded query boxes: every (query, trajectory)
   // pair whose boxes could hold a matching point pair.
   int filterPairs(int nq, int64_t cap, void* stream);
-  DeviceBuffer queryXY_, queryOff_, queryOff64_, queryLen_, queryBox_;
-  DeviceBuffer queryRaw_, querySrcOff_;  // raw query coordinates and offsets, packed on the device
+  DeviceBuffer queryXY_, queryBox_;
+  // The uploaded queries: offsets and lengths, then the raw coordinates
+  // (packed into queryXY_ on the device); the pointers below view it.
+  DeviceBuffer queryUp_;
+  const int64_t* querySrcOffDev_ = nullptr;
+  const int64_t* queryOff64Dev_ = nullptr;
+  const int32_t* queryOffDev_ = nullptr;
+  const int32_t* queryLenDev_ = nullptr;
   PinnedBuffer queryStage_;  // query coordinates and offsets, uploaded from here
   PinnedArena stage_;        // the transfers of one batched EDR / top-k call
   DeviceBuffer pairsDev_, jobsDev_, outDev_, counterDev_, ftableDev_;

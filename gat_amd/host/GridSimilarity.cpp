@@ -221,9 +221,11 @@ int Grid::ensureQueryStore(const TrajectoryView& q) {
       return GAT_ERR_ARG;
     }
   }
-  // The raw coordinates and the offsets go up from pinned memory (two plain
-  // copies on the host, one DMA each), and the device packs the padded store
-  // and the boxes (gat_edr_pack_trajectories).
+  // The offsets, then the raw coordinates, go up from pinned memory in two
+  // DMAs (the second, of the latitudes, copied on the host while the first
+  // runs), and the device packs the padded store and the boxes
+  // (gat_edr_pack_trajectories).  The offsets and lengths the kernels read
+  // are views into the same device buffer.
   const int T = q.trajNum;
   const int64_t o0 = T > 0 ? q.offsets[0] : 0;
   const int64_t n = T > 0 ? q.offsets[T] - o0 : 0;
@@ -231,18 +233,15 @@ int Grid::ensureQueryStore(const TrajectoryView& q) {
   auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
   const size_t braw = up16(size_t(n) * 4), bsrc = up16((size_t(T) + 1) * 8), b64 = up16(size_t(T) * 8);
   const size_t b32 = up16(size_t(T) * 4);
-  if (int rc = queryStage_.reserve(2 * braw + bsrc + b64 + 2 * b32)) return rc;
+  const size_t meta = bsrc + b64 + 2 * b32;
+  if (int rc = queryStage_.reserve(meta + 2 * braw)) return rc;
   char* base = queryStage_.as<char>();
-  float* lon = reinterpret_cast<float*>(base);
-  float* lat = reinterpret_cast<float*>(base + braw);
-  int64_t* src = reinterpret_cast<int64_t*>(base + 2 * braw);
-  int64_t* off64 = reinterpret_cast<int64_t*>(base + 2 * braw + bsrc);
-  int32_t* off32 = reinterpret_cast<int32_t*>(base + 2 * braw + bsrc + b64);
-  int32_t* lens = reinterpret_cast<int32_t*>(base + 2 * braw + bsrc + b64 + b32);
-  if (n > 0) {
-    std::memcpy(lon, q.lon + o0, size_t(n) * 4);
-    std::memcpy(lat, q.lat + o0, size_t(n) * 4);
-  }
+  int64_t* src = reinterpret_cast<int64_t*>(base);
+  int64_t* off64 = reinterpret_cast<int64_t*>(base + bsrc);
+  int32_t* off32 = reinterpret_cast<int32_t*>(base + bsrc + b64);
+  int32_t* lens = reinterpret_cast<int32_t*>(base + bsrc + b64 + b32);
+  float* lon = reinterpret_cast<float*>(base + meta);
+  float* lat = reinterpret_cast<float*>(base + meta + braw);
   queryLenHost_.resize(size_t(T));
   int64_t at = GAT_EDR_PAD;
   for (int t = 0; t < T; ++t) {
@@ -256,24 +255,24 @@ int Grid::ensureQueryStore(const TrajectoryView& q) {
   }
   src[T] = n;
   void* s = engine_.stream();
-  auto put = [&](DeviceBuffer& d, const void* from, size_t bytes) -> int {
-    if (int rc = d.reserve(std::max<size_t>(bytes, 16))) return rc;
-    if (bytes) GATH_TRY(gat_memcpy_h2d(d.get(), from, bytes, s), "query upload");
-    return GAT_OK;
-  };
-  // Raw lon then lat, adjacent in pinned memory, go up as one copy.
-  if (int rc = queryRaw_.reserve(std::max<size_t>(2 * braw, 16))) return rc;
-  if (n > 0) GATH_TRY(gat_memcpy_h2d(queryRaw_.get(), lon, 2 * braw, s), "query upload");
-  if (int rc = put(querySrcOff_, src, (size_t(T) + 1) * 8)) return rc;
-  if (int rc = put(queryOff64_, off64, size_t(T) * 8)) return rc;
-  if (int rc = put(queryOff_, off32, size_t(T) * 4)) return rc;
-  if (int rc = put(queryLen_, lens, size_t(T) * 4)) return rc;
+  if (int rc = queryUp_.reserve(std::max<size_t>(meta + 2 * braw, 16))) return rc;
+  char* dev = queryUp_.as<char>();
+  if (n > 0) std::memcpy(lon, q.lon + o0, size_t(n) * 4);
+  GATH_TRY(gat_memcpy_h2d(dev, base, meta + braw, s), "query upload");
+  if (n > 0) {
+    std::memcpy(lat, q.lat + o0, size_t(n) * 4);
+    GATH_TRY(gat_memcpy_h2d(dev + meta + braw, lat, braw, s), "query upload");
+  }
+  querySrcOffDev_ = reinterpret_cast<const int64_t*>(dev);
+  queryOff64Dev_ = reinterpret_cast<const int64_t*>(dev + bsrc);
+  queryOffDev_ = reinterpret_cast<const int32_t*>(dev + bsrc + b64);
+  queryLenDev_ = reinterpret_cast<const int32_t*>(dev + bsrc + b64 + b32);
   if (int rc = queryXY_.reserve(size_t(total) * 8)) return rc;
   if (int rc = queryBox_.reserve(std::max<size_t>(size_t(T) * 16, 16))) return rc;
-  const float* rlon = queryRaw_.as<float>();
-  const float* rlat = reinterpret_cast<const float*>(queryRaw_.as<char>() + braw);
-  GATH_TRY(gat_edr_pack_trajectories(rlon, rlat, querySrcOff_.as<int64_t>(), queryOff64_.as<int64_t>(), T, total,
-                                     queryXY_.as<float>(), queryBox_.as<float>(), s),
+  const float* rlon = reinterpret_cast<const float*>(dev + meta);
+  const float* rlat = reinterpret_cast<const float*>(dev + meta + braw);
+  GATH_TRY(gat_edr_pack_trajectories(rlon, rlat, querySrcOffDev_, queryOff64Dev_, T, total, queryXY_.as<float>(),
+                                     queryBox_.as<float>(), s),
            "query packing");
   GATH_TRY(gat_stream_sync(s), "query upload");
   return GAT_OK;
@@ -317,7 +316,7 @@ int Grid::similarityPlain(const TrajectoryView& queries, int kValue, std::vector
     // The candidate list is every trajectory; task i pairs "query 0" of the
     // shifted query arrays (i.e. query q) with trajectory i + 1.
     GATH_TRY(gatEDRDistanceBatchHandler(trajNum, allTasks_.as<gat_edr_task>(), queryXY_.as<float>(),
-                                        queryOff_.as<int32_t>() + q, queryLen_.as<int32_t>() + q, trajXY_.as<float>(),
+                                        queryOffDev_ + q, queryLenDev_ + q, trajXY_.as<float>(),
                                         trajOff_.as<int64_t>(), trajLen_.as<int32_t>(), epsilon, dist_.as<int32_t>(),
                                         s),
              "EDR batch");
@@ -351,7 +350,7 @@ int Grid::edrAll(const float* qlon, const float* qlat, int qlen, int32_t* dist) 
   if (int rc = ensureQueryStore(q)) return rc;
   void* s = engine_.stream();
   GATH_TRY(gatEDRDistanceBatchHandler(trajNum, allTasks_.as<gat_edr_task>(), queryXY_.as<float>(),
-                                      queryOff_.as<int32_t>(), queryLen_.as<int32_t>(), trajXY_.as<float>(),
+                                      queryOffDev_, queryLenDev_, trajXY_.as<float>(),
                                       trajOff_.as<int64_t>(), trajLen_.as<int32_t>(), epsilon, dist_.as<int32_t>(), s),
            "EDR batch");
   GATH_TRY(gat_memcpy_d2h(dist, dist_.get(), size_t(trajNum) * 4, s), "EDR readback");
@@ -514,7 +513,7 @@ int Grid::runBatched(std::vector<int32_t>& pairs, int queryNum, std::vector<int3
   auto launch = [&](int bucket, void* stream) -> int {
     GATH_TRY(gat_edr_batched(pairsDev_.as<int32_t>(), np, jobsDev_.as<gat_edr_job>() + j0[bucket],
                              bucketJobs[bucket], nullptr, counterDev_.as<int32_t>() + bucket, queryXY_.as<float>(),
-                             queryOff64_.as<int64_t>(), queryLen_.as<int32_t>(), queryNum, trajXY_.as<float>(),
+                             queryOff64Dev_, queryLenDev_, queryNum, trajXY_.as<float>(),
                              trajOff_.as<int64_t>(), trajLen_.as<int32_t>(), trajNum, maxlen[bucket], epsilon,
                              bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
                              bucket == 3 ? 64 * waves : bucket == 2 ? 64 : 16, outDev_.as<int32_t>(), stream),
@@ -624,7 +623,7 @@ int Grid::deviceTopk(const std::vector<int32_t>& pairs, const std::vector<int32_
   if (int rc = topDev_.reserve((2 * cells + 1) * 4)) return rc;
   const int32_t* sv = survDev_.as<int32_t>();
   int32_t* top = topDev_.as<int32_t>();
-  GATH_TRY(gat_edr_topk(ftableDev_.as<int32_t>(), GAT_MAX_TRAJ_LEN + 1, queryLen_.as<int32_t>(), nq,
+  GATH_TRY(gat_edr_topk(ftableDev_.as<int32_t>(), GAT_MAX_TRAJ_LEN + 1, queryLenDev_, nq,
                         trajLen_.as<int32_t>(), trajNum, lenIdsDev_.as<int32_t>(), lenOffDev_.as<int32_t>(), sv,
                         sv + ns, sv + ns + count, kValue, kk, top, top + cells, top + 2 * cells, s),
            "top-k merge");
@@ -656,8 +655,8 @@ int Grid::launchDeviceBuckets(int64_t cap, int nq, const int32_t* win) {
   auto launch = [&](int bucket, void* stream) -> int {
     const int maxlen = bucket == 0 ? split : bucket == 1 ? std::max(split, 1024) : GAT_MAX_TRAJ_LEN;
     GATH_TRY(gat_edr_batched(sortedDev_.as<int32_t>(), cap, jobsDev_.as<gat_edr_job>(), cap + 4, win + 2 * bucket,
-                             dpCounterDev_.as<int32_t>() + bucket, queryXY_.as<float>(), queryOff64_.as<int64_t>(),
-                             queryLen_.as<int32_t>(), nq, trajXY_.as<float>(), trajOff_.as<int64_t>(),
+                             dpCounterDev_.as<int32_t>() + bucket, queryXY_.as<float>(), queryOff64Dev_,
+                             queryLenDev_, nq, trajXY_.as<float>(), trajOff_.as<int64_t>(),
                              trajLen_.as<int32_t>(), trajNum, maxlen, epsilon,
                              bucket == 0 ? edrPerCuShort : bucket == 1 ? edrPerCuLong : 0,
                              bucket == 3 ? 64 * waves : bucket == 2 ? 64 : 16, outDev_.as<int32_t>(), stream),
@@ -726,7 +725,7 @@ int Grid::runBatchedOnDevice(const int32_t* pairsHost, int64_t np, int queryNum,
   } else {
     GATH_TRY(gat_edr_all_pairs(queryNum, trajNum, pairsDev_.as<int32_t>(), count, s), "all pairs");
   }
-  GATH_TRY(gat_edr_jobs(pairsDev_.as<int32_t>(), count, np, queryLen_.as<int32_t>(), queryNum, trajLen_.as<int32_t>(),
+  GATH_TRY(gat_edr_jobs(pairsDev_.as<int32_t>(), count, np, queryLenDev_, queryNum, trajLen_.as<int32_t>(),
                         trajNum, split, waves, edrWavesEveryLongPair ? 1 : 0, sortedDev_.as<int32_t>(),
                         originDev_.as<int32_t>(), nmDev_.as<int32_t>(), jobsDev_.as<gat_edr_job>(), win, win + 8,
                         reinterpret_cast<uint64_t*>(win + 10), s),
@@ -799,7 +798,7 @@ int Grid::similarityOnDevice(int nq, int kValue, std::vector<TopK>& out, EdrBatc
     if (int rc = filterTimer_.start(s)) return rc;
     if (int rc = filterPairs(nq, cap, s)) return rc;
     if (int rc = filterTimer_.stop(s)) return rc;
-    GATH_TRY(gat_edr_jobs(pairsDev_.as<int32_t>(), counterDev_.as<uint64_t>(), cap, queryLen_.as<int32_t>(), nq,
+    GATH_TRY(gat_edr_jobs(pairsDev_.as<int32_t>(), counterDev_.as<uint64_t>(), cap, queryLenDev_, nq,
                           trajLen_.as<int32_t>(), trajNum, split, waves, edrWavesEveryLongPair ? 1 : 0,
                           sortedDev_.as<int32_t>(), nullptr, nmDev_.as<int32_t>(), jobsDev_.as<gat_edr_job>(), win,
                           win + 8, cellCount, s),
@@ -813,7 +812,7 @@ int Grid::similarityOnDevice(int nq, int kValue, std::vector<TopK>& out, EdrBatc
                                     nq, sv, sv + ns, sv + ns + cap, win + 8, s),
              "survivor lists");
     int32_t* top = topDev_.as<int32_t>();
-    GATH_TRY(gat_edr_topk(ftableDev_.as<int32_t>(), GAT_MAX_TRAJ_LEN + 1, queryLen_.as<int32_t>(), nq,
+    GATH_TRY(gat_edr_topk(ftableDev_.as<int32_t>(), GAT_MAX_TRAJ_LEN + 1, queryLenDev_, nq,
                           trajLen_.as<int32_t>(), trajNum, lenIdsDev_.as<int32_t>(), lenOffDev_.as<int32_t>(), sv,
                           sv + ns, sv + ns + cap, kValue, kk, top, top + cells, top + 2 * cells, s),
              "top-k merge");
