@@ -393,3 +393,35 @@ def test_device_jobs_large_batches_match_host_built_path(gpu):
     assert got[True][2]["pairs"] == got[False][2]["pairs"]
     g.set_topk_device_chain(True)
     gs.set_topk_device_chain(True)
+
+
+@pytest.mark.parametrize("capacity", [1000, 140_000])
+def test_edr_jobs_overflow_empties_the_windows(gpu, capacity):
+    """gat_edr_jobs with more pairs than its capacity, in both forms (one
+    workgroup up to 131,072 pairs of capacity, the device-wide launches
+    above): nothing is placed, every bucket window is empty and status bit 1
+    is set; with the count at the capacity every pair lands in one window."""
+    h = N.hip()
+    nq, nc = 50, 4000
+    rng = np.random.default_rng(capacity)
+    qlen = rng.integers(1, 2049, nq).astype(np.int32)
+    clen = rng.integers(1, 2049, nc).astype(np.int32)
+    pairs = np.stack([rng.integers(0, nq, capacity), rng.integers(0, nc, capacity)], 1).astype(np.int32)
+    dp, dq, dcl = DevBuf.of(pairs), DevBuf.of(qlen), DevBuf.of(clen)
+    sorted_, origin = DevBuf(8 * capacity), DevBuf(4 * capacity)
+    scratch = DevBuf(4 * (2 * capacity + 32768))
+    jobs, windows, status, cells = DevBuf(8 * (capacity + 4)), DevBuf(64).zero(), DevBuf(4).zero(), DevBuf(8)
+    for count, overflow in ((capacity + 1, True), (capacity, False)):
+        cnt = DevBuf.of(np.array([count], np.uint64))
+        N.check_hip(h.gat_edr_jobs(dp.ptr, cnt.ptr, capacity, dq.ptr, nq, dcl.ptr, nc, 256, 4, 0, sorted_.ptr,
+                                   origin.ptr, scratch.ptr, jobs.ptr, windows.ptr, status.ptr, cells.ptr, None))
+        win = windows.get(np.int32, 8)
+        st = int(status.get(np.int32, 1)[0])
+        if overflow:
+            assert st & 1 and not win[1::2].any(), (count, win.tolist())
+        else:
+            assert st == 0
+            got = np.sort(origin.get(np.int32, capacity))
+            assert (got == np.arange(capacity)).all()  # every pair placed once
+            n, m = np.minimum(qlen[pairs[:, 0]], clen[pairs[:, 1]]), np.maximum(qlen[pairs[:, 0]], clen[pairs[:, 1]])
+            assert int(cells.get(np.uint64, 1)[0]) == int((n.astype(np.int64) * m).sum())
