@@ -405,17 +405,30 @@ __global__ __launch_bounds__(64 * W) void edr_pipe_kernel(const int2* __restrict
       }
       if (t0 + kBlk < steps) load_block(dst, t0 + kBlk);
       int16_t* own = &rows[wv][kOff + t0 - 63 + 1];
+      // A wave here is often alone on its SIMD, so each step waits on its
+      // dependent chain (shift, min, select, +1).  The block's matches are
+      // tested before the chain and the last lane's cells are stored after
+      // it, so nothing else (no per-step branch for that lane's store) sits
+      // in the chain's way; the stored columns are still written within the
+      // producer block the consumers' lag allows for.
+      int mt[kBlk];  // 1 where the step's points match (values, not lane masks: those would crowd the SGPRs)
+#pragma unroll
+      for (int j = 0; j < kBlk; ++j) mt[j] = (fabsf(rx - src[j].x) < u && fabsf(ry - src[j].y) < u) ? 1 : 0;
+      int out_c[kBlk];
 #pragma unroll
       for (int j = 0; j < kBlk; ++j) {
-        const bool match = fabsf(rx - src[j].x) < u && fabsf(ry - src[j].y) < u;
         const int u1 = shift_in<64>(fv[j], c1);
-        const int mm = u1p - (match ? 1 : 0);
+        const int mm = u1p - mt[j];
         const int mn = min(min(u1, c1), mm);
         const int d = (mm == c1) ? u1 : mn;
         res = (t0 + j == tcap) ? d : res;
         c1 = d + 1;
         u1p = u1;
-        if (l == 63) own[j] = int16_t(c1);
+        out_c[j] = c1;
+      }
+      if (l == 63) {
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j) own[j] = int16_t(out_c[j]);
       }
     };
     for (int k = 0; k < kend; k += 2) {
