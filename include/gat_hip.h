@@ -358,7 +358,9 @@ int gat_edr_filter(const float* query_box_dev, int32_t nq, const float* cand_box
  * boxes in entry_box_dev) whose box is finite, not inverted, and touches the
  * cell (every cell from the cell of its min corner to the cell of its max
  * corner).  Every other candidate, and any the builder chose not to list, is
- * in always_dev. */
+ * in always_dev.  The kernel trusts the layout: cell_start_dev must be
+ * non-decreasing from 0, and entry_id_dev / entry_box_dev must hold
+ * cell_start_dev[cells_per_side^2] entries (ids outside [0, nc) are skipped). */
 typedef struct gat_edr_cand_grid {
   int32_t cells_per_side;
   float x0, y0, inv_x, inv_y;
