@@ -389,7 +389,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t below = uint32_t(__popcll(m & lt));
     if (valid && below == 0) wcnt[w][d] = uint32_t(__popcll(m));
     __syncthreads();
-    // Thread t = digit t: where each wave's pairs of digit t start this round.
+    // Thread t = digit t: where each wave's pairs of digit t start among this group of 256 pairs.
     uint32_t acc = run[t];
 #pragma unroll
     for (int k = 0; k < kSortWaves; ++k) {
